@@ -1,0 +1,286 @@
+// common.hpp — shared device helpers and the context layout of libibmdem.
+//
+// Fields live in HBM as interior-only arrays (nx*ny*nz, x fastest). The
+// reference keeps two ghost layers per side and refills them after every
+// operator (FillGhostRegionForFluid + Boundary, boundary.F90:122-247,
+// fluid.F90:240-250); here every kernel evaluates the ghost value it needs
+// from the interior with the same rules:
+//   periodic side          ghost = wrapped interior value;
+//   wall side, tangential  ghost = mirrored interior value, negated
+//                          (ghost 0 <- 1, -1 <- 2; n+1 <- n, n+2 <- n-1);
+//   wall side, normal      ghost = 0, and the wall face itself (index 1 on
+//                          the low side) is kept 0 by every writer;
+//   pressure               mirrored without negation (Neumann).
+// Corner ghosts compose the per-dimension rules, which is what the
+// reference's x-then-y-then-z halo passes produce.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/ibmdem.h"
+
+#define IBMDEM_MAX_LEVELS 16
+
+struct GridDev {
+    int nx, ny, nz;
+    int bt[6];
+    double dx, dy, dz;
+    double L[3];
+};
+
+// ---------------------------------------------------------------------------
+// boundary rules
+// ---------------------------------------------------------------------------
+__host__ __device__ inline long cidx(const GridDev &g, int i, int j, int k) {
+    // 1-based interior (i,j,k) -> linear index
+    return ((long)(k - 1) * g.ny + (j - 1)) * (long)g.nx + (i - 1);
+}
+
+// Map one 1-based index of dimension d (size n) onto the interior.
+// Returns 0 if the value is a wall-normal ghost (forced zero), flips *sgn on a
+// wall mirror.
+template <bool NORMAL>
+__device__ __forceinline__ bool map_index(int &i, int n, int lo, int hi, double &sgn) {
+    if (i < 1) {
+        if (lo == IBMDEM_BC_PERIODIC) { i += n; return true; }
+        if (NORMAL) return false;
+        i = 1 - i; sgn = -sgn; return true;
+    }
+    if (i > n) {
+        if (hi == IBMDEM_BC_PERIODIC) { i -= n; return true; }
+        if (NORMAL) return false;
+        i = 2 * n + 1 - i; sgn = -sgn; return true;
+    }
+    return true;
+}
+
+// velocity component C (0=u,1=v,2=w) at logical 1-based (i,j,k), each index
+// in [-1, n+2], with the reference's post-Boundary ghost semantics.
+template <int C, typename T>
+__device__ __forceinline__ T vel_at(const T *__restrict__ f, const GridDev &g, int i, int j, int k) {
+    double s = 1.0;
+    if (!map_index<C == 0>(i, g.nx, g.bt[0], g.bt[1], s)) return T(0);
+    if (!map_index<C == 1>(j, g.ny, g.bt[2], g.bt[3], s)) return T(0);
+    if (!map_index<C == 2>(k, g.nz, g.bt[4], g.bt[5], s)) return T(0);
+    T val = f[cidx(g, i, j, k)];
+    return s < 0 ? -val : val;
+}
+
+// Same, for a field with a pending uniform shift sh (ModifyBulkVelocity's
+// u + dt*modifyP, fluid.F90:188-190): the shift is applied to the interior
+// value the ghost is copied from; forced zeros (wall-normal ghosts and the
+// low wall face) stay zero, as Boundary re-zeroes them after the shift.
+template <int C>
+__device__ __forceinline__ double vel_at_shift(const double *__restrict__ f, const GridDev &g,
+                                               int i, int j, int k, double sh) {
+    double s = 1.0;
+    if (!map_index<C == 0>(i, g.nx, g.bt[0], g.bt[1], s)) return 0.0;
+    if (!map_index<C == 1>(j, g.ny, g.bt[2], g.bt[3], s)) return 0.0;
+    if (!map_index<C == 2>(k, g.nz, g.bt[4], g.bt[5], s)) return 0.0;
+    int idx = C == 0 ? i : (C == 1 ? j : k);
+    if (idx == 1 && g.bt[2 * C] == IBMDEM_BC_WALL) return 0.0;
+    double val = f[cidx(g, i, j, k)] + sh;
+    return s < 0 ? -val : val;
+}
+
+// cell-centred scalar (P, P'): mirror without negation on walls
+template <typename T>
+__device__ __forceinline__ T scal_at(const T *__restrict__ f, const GridDev &g, int i, int j, int k) {
+    double s = 1.0;
+    map_index<false>(i, g.nx, g.bt[0], g.bt[1], s);
+    map_index<false>(j, g.ny, g.bt[2], g.bt[3], s);
+    map_index<false>(k, g.nz, g.bt[4], g.bt[5], s);
+    return f[cidx(g, i, j, k)];
+}
+
+// is (i,j,k) the low wall face of velocity component C (kept zero)?
+template <int C>
+__device__ __forceinline__ bool wall_face(const GridDev &g, int i, int j, int k) {
+    int idx = C == 0 ? i : (C == 1 ? j : k);
+    return idx == 1 && g.bt[2 * C] == IBMDEM_BC_WALL;
+}
+
+// ---------------------------------------------------------------------------
+// Roma-Peskin-Berger 3-point kernel, DeltaFunction3D (interpolation.F90:7-30)
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ double phi1(double r) {
+    if (r <= 0.5) return (1.0 + sqrt(1.0 - 3.0 * r * r)) / 3.0;
+    if (r <= 1.5) return (5.0 - 3.0 * r - sqrt(1.0 - 3.0 * ((1.0 - r) * (1.0 - r)))) / 6.0;
+    return 0.0;
+}
+
+// cell of a position, PutParcelsIntoCellsDEM (ZJS_UPDATE_LIST_DEM.F90:41-43)
+__device__ __forceinline__ int cell_of(double x, double d) { return (int)floor((x - 0.0) / d) + 1; }
+
+// ---------------------------------------------------------------------------
+// multigrid level (one precision)
+// ---------------------------------------------------------------------------
+struct LevelDev {
+    int nx, ny, nz;
+    int bt[6];
+    double ihx2, ihy2, ihz2;     // 1/h^2 per dimension
+    double idiag[27];            // 1/diag indexed by neighbour counts (cx,cy,cz)
+    double diag[27];
+    int rb_valid;                // red-black colouring consistent (no odd periodic dim)
+};
+
+template <typename T>
+struct MGLevel {
+    LevelDev d;
+    long n;
+    T *x, *b, *r;                // level solution, RHS, residual
+};
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct Profile {
+    int kernel = 0;
+    std::vector<hipEvent_t> ev;  // pairs
+    double total_ms = 0.0;
+    int64_t launches = 0;
+    double bytes_per_launch = 0.0;
+};
+
+struct ibmdem_ctx {
+    ibmdem_case cs;
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    GridDev g;
+    long n = 0;                  // interior cells
+    double dV = 0.0;             // dx*dy*dz as the reference forms it
+    double eta = 0.0;
+    // fluid (FP64, interior-only)
+    double *u[2] = {}, *v[2] = {}, *w[2] = {};
+    int vcur = 0;
+    double *P[2] = {};
+    int pcur = 0;
+    double *pcorr = nullptr, *rhs = nullptr;
+    double *rhsl[3] = {};        // RHSul etc. (allocated on demand)
+    // PCG (FP64)
+    double *cg_x = nullptr, *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_Ap = nullptr;
+    int nlev = 0;
+    MGLevel<double> lev64[IBMDEM_MAX_LEVELS];
+    MGLevel<float> lev32[IBMDEM_MAX_LEVELS];
+    // reductions
+    double *partial = nullptr;   // per-block partial sums
+    int max_partial = 0;
+    double *scal = nullptr;      // device scalars (see SC_* in kernels)
+    double *scal_host = nullptr; // pinned mirror
+    int *flags = nullptr;        // device error flags
+    int *flags_host = nullptr;
+    bool bulk_pending = false;   // bulk shift computed but not yet applied
+    // markers
+    int M = 0, B = 0;
+    double *mx = nullptr, *mv = nullptr, *fdrag = nullptr, *fcol = nullptr;
+    double *mvol = nullptr, *mmass = nullptr, *mrad = nullptr;
+    int *mbody = nullptr;
+    int *mcell = nullptr;        // 0-based linear interior cell
+    double *wts = nullptr;       // 24 one-dimensional delta weights per marker
+    // bins over interior cells
+    int *bin_count = nullptr, *bin_start = nullptr, *bin_items = nullptr, *bin_tmp = nullptr;
+    int *scan_blocks = nullptr;
+    int scan_nblocks = 0;
+    // IBM spread footprint (per step)
+    unsigned char *act_flag = nullptr;
+    int *act_list = nullptr;
+    int nact = 0;
+    long act_cap = 0;
+    double *fact = nullptr;      // f of the active cells, 3*act_cap
+    bool first_mdf = true;
+    // bodies
+    int *body_start = nullptr;   // B+1
+    double *bx = nullptr, *bv = nullptr, *bvold = nullptr, *bom = nullptr, *bL = nullptr, *bLold = nullptr;
+    double *brot = nullptr, *brotg = nullptr, *bI = nullptr, *bm = nullptr, *brho = nullptr;
+    int *bfixed = nullptr;
+    double *bfdrag = nullptr, *bfmom = nullptr, *bfcol = nullptr, *bfmomcol = nullptr;
+    double *bfhydro = nullptr, *bfmomhydro = nullptr;
+    // pair list for parity (allocated on demand)
+    int *pair_count = nullptr, *pair_partner = nullptr, *pair_shift = nullptr;
+    int pair_cap = 0;
+    Profile prof;
+    int64_t bytes = 0;
+    std::vector<void *> allocs;
+};
+
+// device scalar slots
+enum {
+    SC_MODP = 0,      // 3: modifyPx,y,z * dt (the bulk shift)
+    SC_BULKSUM = 3,   // 3: sums of u,v,w
+    SC_RZ = 6, SC_PAP = 7, SC_RR = 8, SC_ALPHA = 9, SC_BETA = 10, SC_BNORM2 = 11,
+    SC_RZ_NEW = 12, SC_PIN = 13, SC_X0 = 14,
+    SC_COUNT = 32
+};
+enum { FL_CAPACITY = 0, FL_OUTSIDE = 1, FL_NAN = 2, FL_COUNT = 8 };
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+int set_error(ibmdem_ctx *c, int code, const std::string &msg);
+#define HIPX(c, call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return set_error((c), IBMDEM_ERR_HIP,                                  \
+                             std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+#define CHECK_LAUNCH(c) HIPX(c, hipGetLastError())
+
+template <typename T>
+int dev_alloc(ibmdem_ctx *c, T **p, size_t count) {
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 256;
+    void *q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) {
+        *p = nullptr;
+        return set_error(c, IBMDEM_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    if (hipMemsetAsync(q, 0, bytes, c->stream) != hipSuccess)
+        return set_error(c, IBMDEM_ERR_HIP, "hipMemsetAsync failed");
+    c->allocs.push_back(q);
+    c->bytes += (int64_t)bytes;
+    *p = (T *)q;
+    return IBMDEM_OK;
+}
+
+// launch-time profiling of one kernel id
+void prof_begin(ibmdem_ctx *c, int kid);
+void prof_end(ibmdem_ctx *c, int kid, double bytes);
+
+// ---------------------------------------------------------------------------
+// launchers (one per translation unit)
+// ---------------------------------------------------------------------------
+// fluid.hip
+int fluid_predictor(ibmdem_ctx *c, double a, double b, double cc);
+int fluid_bulk(ibmdem_ctx *c, bool apply_now);
+int fluid_divergence(ibmdem_ctx *c, double cc);
+int fluid_project(ibmdem_ctx *c, double cc);
+int fluid_upload_bc(ibmdem_ctx *c);
+// poisson.hip
+int poisson_setup(ibmdem_ctx *c);
+int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info);
+// ibm.hip
+int ibm_begin_step(ibmdem_ctx *c);
+int ibm_interpolate(ibmdem_ctx *c);
+int ibm_spread(ibmdem_ctx *c);
+int ibm_download_force(ibmdem_ctx *c, double *fx, double *fy, double *fz);
+// dem.hip
+int dem_setup(ibmdem_ctx *c);
+int dem_rebuild(ibmdem_ctx *c);
+int dem_contact(ibmdem_ctx *c);
+int dem_body_assemble(ibmdem_ctx *c, bool all_terms);
+int dem_body_integrate(ibmdem_ctx *c);
+int dem_markers_follow(ibmdem_ctx *c);
+int dem_body_hydro(ibmdem_ctx *c);
+int dem_pairs(ibmdem_ctx *c, int cap);
+int dem_check_flags(ibmdem_ctx *c);
+
+// deterministic reductions (fluid.hip)
+// sum over n values of f(i) into scal[slot]; fixed tree, independent of timing
+int reduce_sum(ibmdem_ctx *c, const double *a, const double *b, long n, int slot, long skip_first);
+
+static inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }

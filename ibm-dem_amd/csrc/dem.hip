@@ -1,0 +1,611 @@
+// dem.hip — DEM cell lists, marker contact and rigid-body dynamics.
+//
+//   binning          CompressDEM + PutParcelsIntoCellsDEM
+//                    (ZJS_UPDATE_LIST_DEM.F90:2-97): a marker that left
+//                    through a periodic face is wrapped with x + coordShift,
+//                    cell = floor((x - 0)/dx) + 1; markers are counting-
+//                    sorted by cell with ascending id inside a cell.
+//   contact          RebuildParcelCollisionListDEM + Collision +
+//                    CalculateCollisionForce + CalculateSolidBoundary...
+//                    (ZJS_UPDATE_LIST_DEM.F90:248-286, ZJS_COLLISION_DEM.F90:
+//                    2-148). The candidate scan and the force use the same
+//                    positions (nothing moves between the list rebuild and
+//                    the next contact call), so one marker-parallel kernel
+//                    scans the 125 cells in (kk,jj,ii) order, tests each
+//                    candidate (other body, d^2 < (1.65h)^2, periodic images
+//                    included, wall mirrors excluded) and adds its force in
+//                    list order.
+//   body assembly    AssembleLocalForceMS (ZJS_MS_DEM.F90:494-519): one
+//                    work-group per body over its contiguous markers, fixed
+//                    reduction tree.
+//   Newton-Euler     EvolutionMS (couplesolverCylinder.F90:2-42), fused into
+//                    the body reduction.
+//   marker update    ResetVelocityAndPositionLagrangianPoints +
+//                    AdjustParcelPosition (couplesolverCylinder.F90:44-74,
+//                    ZJS_EVOLUTION_DEM.F90:121-150).
+#include "common.hpp"
+
+#include <cmath>
+
+namespace {
+
+constexpr int T = 256;
+constexpr int CHUNK = 4096;
+constexpr int BT = 256;   // threads per body work-group
+
+// ---------------------------------------------------------------------------
+// binning
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(T) void k_wrap_cells_count(int M, GridDev g, double *__restrict__ mx,
+                                                        int *__restrict__ mcell, int *__restrict__ bin_count,
+                                                        int *__restrict__ flags) {
+    const int m = blockIdx.x * T + threadIdx.x;
+    if (m >= M) return;
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const double dd[3] = {g.dx, g.dy, g.dz};
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        double x = mx[(long)d * M + m];
+        int ci = cell_of(x, dd[d]);
+        if (ci > n[d] && g.bt[2 * d + 1] == IBMDEM_BC_PERIODIC) {
+            x = x + (-g.L[d]);
+            mx[(long)d * M + m] = x;
+            ci = cell_of(x, dd[d]);
+        } else if (ci < 1 && g.bt[2 * d] == IBMDEM_BC_PERIODIC) {
+            x = x + g.L[d];
+            mx[(long)d * M + m] = x;
+            ci = cell_of(x, dd[d]);
+        }
+        // a marker through a wall (or a non-finite position) is an error;
+        // it is still binned in the nearest boundary cell so that every
+        // list stays consistent until the caller sees the status
+        if (!(ci >= 1 && ci <= n[d])) {
+            flags[FL_OUTSIDE] = 1;
+            ci = ci < 1 ? 1 : n[d];
+            if (!(x == x)) ci = 1;
+        }
+        c[d] = ci;
+    }
+    const int cl = (int)cidx(g, c[0], c[1], c[2]);
+    mcell[m] = cl;
+    atomicAdd(&bin_count[cl], 1);
+}
+
+__global__ __launch_bounds__(T) void k_chunk_sum(long n, const int *__restrict__ a, int *__restrict__ out) {
+    __shared__ int red[T];
+    const long lo = (long)blockIdx.x * CHUNK;
+    int s = 0;
+    for (long q = lo + threadIdx.x; q < min(n, lo + CHUNK); q += T) s += a[q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = T / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(int nb, int *__restrict__ a) {
+    __shared__ int buf[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        const int q = base + threadIdx.x;
+        int v = q < nb ? a[q] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int st = 1; st < 1024; st <<= 1) {
+            int t = threadIdx.x >= st ? buf[threadIdx.x - st] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (q < nb) a[q] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a[nb] = carry;
+}
+
+// exclusive scan of counts into start (CHUNK cells per block, 16 per thread)
+__global__ __launch_bounds__(T) void k_chunk_scan(long n, const int *__restrict__ cnt, const int *__restrict__ off,
+                                                  int *__restrict__ start, int nb) {
+    __shared__ int buf[T];
+    constexpr int PER = CHUNK / T;
+    const long lo = (long)blockIdx.x * CHUNK + (long)threadIdx.x * PER;
+    int loc[PER];
+    int s = 0;
+    for (int q = 0; q < PER; ++q) {
+        loc[q] = (lo + q < n) ? cnt[lo + q] : 0;
+        s += loc[q];
+    }
+    buf[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 1; st < T; st <<= 1) {
+        int t = threadIdx.x >= st ? buf[threadIdx.x - st] : 0;
+        __syncthreads();
+        buf[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int pos = off[blockIdx.x] + buf[threadIdx.x] - s;
+    for (int q = 0; q < PER; ++q) {
+        if (lo + q < n) start[lo + q] = pos;
+        pos += loc[q];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[n] = off[nb];
+}
+
+// unordered placement; the counts return to zero
+__global__ __launch_bounds__(T) void k_bin_scatter(int M, const int *__restrict__ mcell, const int *__restrict__ start,
+                                                   int *__restrict__ cnt, int *__restrict__ tmp) {
+    const int m = blockIdx.x * T + threadIdx.x;
+    if (m >= M) return;
+    const int cl = mcell[m];
+    const int slot = start[cl] + atomicSub(&cnt[cl], 1) - 1;
+    tmp[slot] = m;
+}
+
+// ascending id inside each cell: position = start + #{smaller ids in cell}
+__global__ __launch_bounds__(T) void k_bin_rank(int M, const int *__restrict__ mcell, const int *__restrict__ start,
+                                                const int *__restrict__ tmp, int *__restrict__ items) {
+    const int m = blockIdx.x * T + threadIdx.x;
+    if (m >= M) return;
+    const int cl = mcell[m];
+    const int q0 = start[cl], q1 = start[cl + 1];
+    int r = 0;
+    for (int q = q0; q < q1; ++q) r += tmp[q] < m ? 1 : 0;
+    items[q0 + r] = m;
+}
+
+// ---------------------------------------------------------------------------
+// contact
+// ---------------------------------------------------------------------------
+struct ContactParams {
+    double lim2, kn_over, eta;   // (cd(1+buf))^2, knCoeff/(0.1 di), eta
+};
+
+// neighbour source cell of the candidate scan: interior index and the
+// periodic shift of its images; false for cells beyond a wall (mirror
+// copies carry pType 1 and are never partners).
+__device__ __forceinline__ bool nb_cell(int &i, int n, int lo, int hi, double L, double &sh, int &code) {
+    sh = 0.0; code = 0;
+    if (i < 1) { if (lo != IBMDEM_BC_PERIODIC) return false; i += n; sh = -L; code = -1; }
+    else if (i > n) { if (hi != IBMDEM_BC_PERIODIC) return false; i -= n; sh = L; code = 1; }
+    return true;
+}
+
+// fcol of marker i (ZJS_COLLISION_DEM.F90:31-72, 93-146)
+template <bool PAIRS>
+__global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *__restrict__ mx,
+                                               const double *__restrict__ mv, const double *__restrict__ mmass,
+                                               const double *__restrict__ mrad, const int *__restrict__ mbody,
+                                               const int *__restrict__ mcell, const int *__restrict__ bin_start,
+                                               const int *__restrict__ bin_items, double *__restrict__ fcol,
+                                               ContactParams cp, int max_pairs, int *__restrict__ flags,
+                                               int cap, int *__restrict__ pcount, int *__restrict__ ppartner,
+                                               int *__restrict__ pshift) {
+    const int i = blockIdx.x * T + threadIdx.x;
+    if (i >= M) return;
+    const int cl = mcell[i];
+    const int ci = cl % g.nx + 1, cj = (cl / g.nx) % g.ny + 1, ck = cl / (g.nx * g.ny) + 1;
+    const double xi0 = mx[i], xi1 = mx[(long)M + i], xi2 = mx[2L * M + i];
+    const double vi0 = mv[i], vi1 = mv[(long)M + i], vi2 = mv[2L * M + i];
+    const double mi = mmass[i], ri = mrad[i];
+    const int bi = mbody[i];
+    double f0 = 0, f1 = 0, f2 = 0;
+    int cnt = 0;
+    for (int kk = -2; kk <= 2; ++kk) {
+        int kc = ck + kk, kcode; double ksh;
+        if (!nb_cell(kc, g.nz, g.bt[4], g.bt[5], g.L[2], ksh, kcode)) continue;
+        for (int jj = -2; jj <= 2; ++jj) {
+            int jc = cj + jj, jcode; double jsh;
+            if (!nb_cell(jc, g.ny, g.bt[2], g.bt[3], g.L[1], jsh, jcode)) continue;
+            for (int ii = -2; ii <= 2; ++ii) {
+                int ic = ci + ii, icode; double ish;
+                if (!nb_cell(ic, g.nx, g.bt[0], g.bt[1], g.L[0], ish, icode)) continue;
+                const long b = cidx(g, ic, jc, kc);
+                const int q0 = bin_start[b], q1 = bin_start[b + 1];
+                for (int q = q0; q < q1; ++q) {
+                    const int j = bin_items[q];
+                    if (mbody[j] == bi) continue;
+                    double xj0 = mx[j], xj1 = mx[(long)M + j], xj2 = mx[2L * M + j];
+                    if (icode) xj0 = xj0 + ish;
+                    if (jcode) xj1 = xj1 + jsh;
+                    if (kcode) xj2 = xj2 + ksh;
+                    const double r1 = xi0 - xj0, r2 = xi1 - xj1, r3 = xi2 - xj2;
+                    const double d2 = r1 * r1 + r2 * r2 + r3 * r3;
+                    if (!(d2 < cp.lim2)) continue;
+                    if (PAIRS) {
+                        if (cnt < cap) {
+                            ppartner[(long)i * cap + cnt] = j;
+                            pshift[(long)i * cap + cnt] = (icode + 1) + 3 * (jcode + 1) + 9 * (kcode + 1);
+                        }
+                        ++cnt;
+                        continue;
+                    }
+                    ++cnt;
+                    const double dist = sqrt(d2);
+                    const double n1 = r1 / dist, n2 = r2 / dist, n3 = r3 / dist;
+                    const double mj = mmass[j];
+                    const double meff = 2 * mi * mj / (mi + mj);
+                    const double rij = ri + mrad[j];
+                    const double deltan = rij - dist;
+                    const double kn = cp.kn_over * meff;
+                    const double etan = cp.eta * sqrt(meff * kn);
+                    const double vnij = n1 * (vi0 - mv[j]) + n2 * (vi1 - mv[(long)M + j]) + n3 * (vi2 - mv[2L * M + j]);
+                    double g1 = 0, g2 = 0, g3 = 0;
+                    if (deltan > 0) {
+                        g1 = n1 * (kn * deltan - etan * vnij);
+                        g2 = n2 * (kn * deltan - etan * vnij);
+                        g3 = n3 * (kn * deltan - etan * vnij);
+                    }
+                    f0 = f0 + g1;
+                    f1 = f1 + g2;
+                    f2 = f2 + g3;
+                }
+            }
+        }
+    }
+    if (cnt > max_pairs) flags[FL_CAPACITY] = 1;
+    if (PAIRS) {
+        pcount[i] = cnt;
+        return;
+    }
+    // wall contact for markers in the first/last interior layer
+    if (ci == 1 || ci == g.nx || cj == 1 || cj == g.ny || ck == 1 || ck == g.nz) {
+        for (int s = 0; s < 6; ++s) {
+            if (g.bt[s] != IBMDEM_BC_WALL) continue;
+            const double p1 = (s % 2 == 0) ? 0.0 : g.L[0];
+            const double p2 = (s % 2 == 0) ? 0.0 : g.L[1];
+            const double p3 = (s % 2 == 0) ? 0.0 : g.L[2];
+            const double sg = (s % 2 == 0) ? 1.0 : -1.0;
+            const double n1 = (s / 2 == 0) ? sg : 0.0, n2 = (s / 2 == 1) ? sg : 0.0, n3 = (s / 2 == 2) ? sg : 0.0;
+            const double r1 = xi0 - p1, r2 = xi1 - p2, r3 = xi2 - p3;
+            const double dist = r1 * n1 + r2 * n2 + r3 * n3;
+            const double meff = mi;
+            const double deltan = ri - dist;
+            const double kn = cp.kn_over * meff;
+            const double etan = cp.eta * sqrt(meff * kn);
+            const double vnij = n1 * vi0 + n2 * vi1 + n3 * vi2;
+            if (deltan > 0) {
+                f0 = f0 + n1 * (kn * deltan - etan * vnij);
+                f1 = f1 + n2 * (kn * deltan - etan * vnij);
+                f2 = f2 + n3 * (kn * deltan - etan * vnij);
+            }
+        }
+    }
+    fcol[i] = f0;
+    fcol[(long)M + i] = f1;
+    fcol[2L * M + i] = f2;
+}
+
+// ---------------------------------------------------------------------------
+// rigid bodies
+// ---------------------------------------------------------------------------
+struct BodyState {
+    double *x, *v, *vold, *om, *L, *Lold, *rot, *rotg, *I, *m, *rho;
+    int *fixed;
+    double *fdrag, *fmom, *fcol, *fmomcol, *fhydro, *fmomhydro;
+};
+
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double d[3]) {
+    const double c1 = a[1] * b[2] - a[2] * b[1];
+    const double c2 = -a[0] * b[2] + a[2] * b[0];
+    const double c3 = a[0] * b[1] - a[1] * b[0];
+    d[0] = c1; d[1] = c2; d[2] = c3;
+}
+__device__ __forceinline__ void tmul(const double *A, const double *B, double *D) {
+    double C[9];
+    C[0] = A[0] * B[0] + A[1] * B[3] + A[2] * B[6];
+    C[1] = A[0] * B[1] + A[1] * B[4] + A[2] * B[7];
+    C[2] = A[0] * B[2] + A[1] * B[5] + A[2] * B[8];
+    C[3] = A[3] * B[0] + A[4] * B[3] + A[5] * B[6];
+    C[4] = A[3] * B[1] + A[4] * B[4] + A[5] * B[7];
+    C[5] = A[3] * B[2] + A[4] * B[5] + A[5] * B[8];
+    C[6] = A[6] * B[0] + A[7] * B[3] + A[8] * B[6];
+    C[7] = A[6] * B[1] + A[7] * B[4] + A[8] * B[7];
+    C[8] = A[6] * B[2] + A[7] * B[5] + A[8] * B[8];
+    for (int q = 0; q < 9; ++q) D[q] = C[q];
+}
+
+// nearest periodic image of body b's centre seen from position xm
+__device__ __forceinline__ void body_image(const GridDev &g, const double *bx, int B, int b, const double xm[3],
+                                           double xb[3]) {
+    for (int d = 0; d < 3; ++d) {
+        xb[d] = bx[(long)d * B + b];
+        if (g.bt[2 * d] == IBMDEM_BC_PERIODIC) {
+            const double dd = xm[d] - xb[d];
+            if (dd > 0.5 * g.L[d]) xb[d] = xb[d] + g.L[d];
+            else if (dd < -0.5 * g.L[d]) xb[d] = xb[d] + (-g.L[d]);
+        }
+    }
+}
+
+// EvolutionMS for one body (couplesolverCylinder.F90:14-40)
+__device__ void evolve_body(const BodyState &s, int B, int b, double dtD, double rhof, const double grav[3]) {
+    if (s.fixed[b]) return;
+    const double ratio = (s.rho[b] - rhof) / s.rho[b];
+    double *rot = s.rot + 9L * b, *rg = s.rotg + 9L * b;
+    for (int d = 0; d < 3; ++d)
+        s.v[(long)d * B + b] = s.v[(long)d * B + b] + s.fhydro[(long)d * B + b] * dtD +
+                               s.fcol[(long)d * B + b] / s.m[b] * dtD + ratio * grav[d] * dtD;
+    // CalculateRotateMatrix(omega*dtDEM) (couplesolverCylinder.F90:76-97)
+    const double th0 = s.om[b] * dtD, th1 = s.om[(long)B + b] * dtD, th2 = s.om[2L * B + b] * dtD;
+    double mag = sqrt(th0 * th0 + th1 * th1 + th2 * th2);
+    mag = mag > 1e-10 ? mag : 1e-10;
+    const double ux = th0 / mag, uy = th1 / mag, uz = th2 / mag;
+    const double cm = cos(mag), sm = sin(mag);
+    rot[0] = (1 - cm) * ux * ux + cm;
+    rot[1] = (1 - cm) * ux * uy - sm * uz;
+    rot[2] = (1 - cm) * ux * uz + sm * uy;
+    rot[3] = (1 - cm) * uy * ux + sm * uz;
+    rot[4] = (1 - cm) * uy * uy + cm;
+    rot[5] = (1 - cm) * uy * uz - sm * ux;
+    rot[6] = (1 - cm) * uz * ux - sm * uy;
+    rot[7] = (1 - cm) * uz * uy + sm * ux;
+    rot[8] = (1 - cm) * uz * uz + cm;
+    double ITInv[9];
+    for (int q = 0; q < 9; ++q) ITInv[q] = s.I[9L * b + q];
+    ITInv[0] = 1.0 / ITInv[0]; ITInv[4] = 1.0 / ITInv[4]; ITInv[8] = 1.0 / ITInv[8];
+    double rgl[9];
+    for (int q = 0; q < 9; ++q) rgl[q] = rg[q];
+    tmul(rgl, rot, rgl);
+    for (int q = 0; q < 9; ++q) rg[q] = rgl[q];
+    const double RMInv[9] = {rgl[0], rgl[3], rgl[6], rgl[1], rgl[4], rgl[7], rgl[2], rgl[5], rgl[8]};
+    double T1[9], TM[9];
+    tmul(rgl, ITInv, T1);
+    tmul(T1, RMInv, TM);
+    double Lv[3];
+    for (int d = 0; d < 3; ++d) {
+        s.L[(long)d * B + b] = s.L[(long)d * B + b] + (s.fmomhydro[(long)d * B + b] + s.fmomcol[(long)d * B + b]) * dtD;
+        Lv[d] = s.L[(long)d * B + b];
+    }
+    const double o0 = TM[0] * Lv[0] + TM[1] * Lv[1] + TM[2] * Lv[2];
+    const double o1 = TM[3] * Lv[0] + TM[4] * Lv[1] + TM[5] * Lv[2];
+    const double o2 = TM[6] * Lv[0] + TM[7] * Lv[1] + TM[8] * Lv[2];
+    s.om[b] = o0; s.om[(long)B + b] = o1; s.om[2L * B + b] = o2;
+    for (int d = 0; d < 3; ++d) s.x[(long)d * B + b] = s.x[(long)d * B + b] + s.v[(long)d * B + b] * dtD;
+}
+
+// mode 0: fcol, fmomcol only; 1: all four sums (AssembleLocalForceMS);
+// 2: all four + CloudEvolutionMS hydro terms (couplesolverCylinder.F90:194-211)
+// integrate: EvolutionMS after the sums.
+__global__ __launch_bounds__(BT) void k_body_reduce(int M, int B, GridDev g, const int *__restrict__ bstart,
+                                                    const double *__restrict__ mx, const double *__restrict__ fdrag,
+                                                    const double *__restrict__ fcol, const double *__restrict__ mvol,
+                                                    BodyState s, int mode, int integrate, double rhof, double dt,
+                                                    double dtD, double g0, double g1, double g2) {
+    __shared__ double red[12][BT];
+    const int b = blockIdx.x;
+    const int m0 = bstart[b], m1 = bstart[b + 1];
+    double acc[12];
+    for (int q = 0; q < 12; ++q) acc[q] = 0.0;
+    for (int m = m0 + threadIdx.x; m < m1; m += BT) {
+        const double xm[3] = {mx[m], mx[(long)M + m], mx[2L * M + m]};
+        double xb[3], rpc[3], fv[3], fc[3], mt[3], mc[3];
+        body_image(g, s.x, B, b, xm, xb);
+        for (int d = 0; d < 3; ++d) {
+            rpc[d] = xm[d] - xb[d];
+            fc[d] = fcol[(long)d * M + m];
+        }
+        cross3(rpc, fc, mc);
+        for (int d = 0; d < 3; ++d) {
+            acc[3 + d] += fc[d];
+            acc[9 + d] += mc[d];
+        }
+        if (mode >= 1) {
+            for (int d = 0; d < 3; ++d) fv[d] = fdrag[(long)d * M + m] * mvol[m];
+            cross3(rpc, fv, mt);
+            for (int d = 0; d < 3; ++d) {
+                acc[d] += fdrag[(long)d * M + m] * mvol[m] * rhof;
+                acc[6 + d] += mt[d];
+            }
+        }
+    }
+    for (int q = 0; q < 12; ++q) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int st = BT / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int q = 0; q < 12; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    for (int d = 0; d < 3; ++d) {
+        s.fcol[(long)d * B + b] = red[3 + d][0];
+        s.fmomcol[(long)d * B + b] = red[9 + d][0];
+        if (mode >= 1) {
+            s.fdrag[(long)d * B + b] = red[d][0];
+            s.fmom[(long)d * B + b] = red[6 + d][0];
+        }
+    }
+    if (mode == 2) {
+        const double ratio = (s.rho[b] - rhof) / s.rho[b];
+        for (int d = 0; d < 3; ++d) {
+            const long o = (long)d * B + b;
+            double fh = (s.v[o] - s.vold[o]) / dt * (1.0 - ratio);
+            double fmh = (s.L[o] - s.Lold[o]) / dt * (1.0 - ratio);
+            s.vold[o] = s.v[o];
+            s.Lold[o] = s.L[o];
+            s.fhydro[o] = fh + s.fdrag[o] / s.m[b];
+            s.fmomhydro[o] = fmh + s.fmom[o];
+        }
+    }
+    if (integrate) {
+        const double grav[3] = {g0, g1, g2};
+        evolve_body(s, B, b, dtD, rhof, grav);
+    }
+}
+
+__global__ void k_body_integrate(int B, BodyState s, double dtD, double rhof, double g0, double g1, double g2) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const double grav[3] = {g0, g1, g2};
+    evolve_body(s, B, b, dtD, rhof, grav);
+}
+
+// ResetVelocityAndPositionLagrangianPoints + AdjustParcelPosition
+__global__ __launch_bounds__(T) void k_follow(int M, int B, GridDev g, double *__restrict__ mx,
+                                              double *__restrict__ mv, const int *__restrict__ mbody, BodyState s,
+                                              double dtD) {
+    const int m = blockIdx.x * T + threadIdx.x;
+    if (m >= M) return;
+    const int b = mbody[m];
+    double xm[3] = {mx[m], mx[(long)M + m], mx[2L * M + m]};
+    if (!s.fixed[b]) {
+        double xb[3];
+        body_image(g, s.x, B, b, xm, xb);
+        for (int d = 0; d < 3; ++d) xm[d] = xm[d] + s.v[(long)d * B + b] * dtD;
+        const double *mat = s.rot + 9L * b;
+        const double t0 = xm[0] - xb[0], t1 = xm[1] - xb[1], t2 = xm[2] - xb[2];
+        xm[0] = mat[0] * t0 + mat[1] * t1 + mat[2] * t2;
+        xm[1] = mat[3] * t0 + mat[4] * t1 + mat[5] * t2;
+        xm[2] = mat[6] * t0 + mat[7] * t1 + mat[8] * t2;
+        for (int d = 0; d < 3; ++d) xm[d] = xm[d] + xb[d];
+        const double om[3] = {s.om[b], s.om[(long)B + b], s.om[2L * B + b]};
+        const double rr[3] = {xm[0] - xb[0], xm[1] - xb[1], xm[2] - xb[2]};
+        double vv[3];
+        cross3(om, rr, vv);
+        for (int d = 0; d < 3; ++d) mv[(long)d * M + m] = vv[d] + s.v[(long)d * B + b];
+    }
+    const double e[3] = {g.dx * 1e-20, g.dy * 1e-20, g.dz * 1e-20};
+    for (int d = 0; d < 3; ++d) {
+        double x = xm[d];
+        if (fabs(x - 0.0) < e[d]) x = 0.0 + e[d];
+        if (fabs(x - g.L[d]) < e[d]) x = g.L[d] + e[d];
+        mx[(long)d * M + m] = x;
+    }
+}
+
+BodyState body_state(ibmdem_ctx *c) {
+    return BodyState{c->bx, c->bv, c->bvold, c->bom, c->bL, c->bLold, c->brot, c->brotg, c->bI, c->bm, c->brho,
+                     c->bfixed, c->bfdrag, c->bfmom, c->bfcol, c->bfmomcol, c->bfhydro, c->bfmomhydro};
+}
+
+ContactParams contact_params(ibmdem_ctx *c) {
+    const double lim = c->cs.collision_dist * (1.0 + c->cs.collision_buffer);
+    return ContactParams{lim * lim, c->cs.kn_coeff / (0.1 * c->g.dx), c->eta};
+}
+
+}  // namespace
+
+int dem_rebuild(ibmdem_ctx *c) {
+    const int M = c->M;
+    if (M == 0) return IBMDEM_OK;
+    hipStream_t s = c->stream;
+    const long n = c->n;
+    hipLaunchKernelGGL(k_wrap_cells_count, dim3(div_up(M, T)), dim3(T), 0, s, M, c->g, c->mx, c->mcell,
+                       c->bin_count, c->flags);
+    CHECK_LAUNCH(c);
+    const int nb = div_up(n, CHUNK);
+    hipLaunchKernelGGL(k_chunk_sum, dim3(nb), dim3(T), 0, s, n, c->bin_count, c->scan_blocks);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, nb, c->scan_blocks);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(nb), dim3(T), 0, s, n, c->bin_count, c->scan_blocks, c->bin_start, nb);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(div_up(M, T)), dim3(T), 0, s, M, c->mcell, c->bin_start, c->bin_count,
+                       c->bin_tmp);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_bin_rank, dim3(div_up(M, T)), dim3(T), 0, s, M, c->mcell, c->bin_start, c->bin_tmp,
+                       c->bin_items);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+int dem_contact(ibmdem_ctx *c) {
+    const int M = c->M;
+    if (M == 0) return IBMDEM_OK;
+    prof_begin(c, IBMDEM_K_CONTACT);
+    hipLaunchKernelGGL((k_contact<false>), dim3(div_up(M, T)), dim3(T), 0, c->stream, M, c->g, c->mx, c->mv,
+                       c->mmass, c->mrad, c->mbody, c->mcell, c->bin_start, c->bin_items, c->fcol,
+                       contact_params(c), c->cs.max_pairs, c->flags, 0, nullptr, nullptr, nullptr);
+    prof_end(c, IBMDEM_K_CONTACT, 0.0);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+int dem_pairs(ibmdem_ctx *c, int cap) {
+    const int M = c->M;
+    if (M == 0) return IBMDEM_OK;
+    if (cap > c->pair_cap) {
+        int rc;
+        if ((rc = dev_alloc(c, &c->pair_count, M))) return rc;
+        if ((rc = dev_alloc(c, &c->pair_partner, (size_t)M * cap))) return rc;
+        if ((rc = dev_alloc(c, &c->pair_shift, (size_t)M * cap))) return rc;
+        c->pair_cap = cap;
+    }
+    hipLaunchKernelGGL((k_contact<true>), dim3(div_up(M, T)), dim3(T), 0, c->stream, M, c->g, c->mx, c->mv,
+                       c->mmass, c->mrad, c->mbody, c->mcell, c->bin_start, c->bin_items, c->fcol,
+                       contact_params(c), c->cs.max_pairs, c->flags, c->pair_cap, c->pair_count, c->pair_partner,
+                       c->pair_shift);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+int dem_body_assemble(ibmdem_ctx *c, bool all_terms) {
+    if (c->B == 0) return IBMDEM_OK;
+    hipLaunchKernelGGL(k_body_reduce, dim3(c->B), dim3(BT), 0, c->stream, c->M, c->B, c->g, c->body_start, c->mx,
+                       c->fdrag, c->fcol, c->mvol, body_state(c), all_terms ? 1 : 0, 0, c->cs.rhof, c->cs.dt,
+                       c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1], c->cs.gravity[2]);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+int dem_body_hydro(ibmdem_ctx *c) {
+    if (c->B == 0) return IBMDEM_OK;
+    hipLaunchKernelGGL(k_body_reduce, dim3(c->B), dim3(BT), 0, c->stream, c->M, c->B, c->g, c->body_start, c->mx,
+                       c->fdrag, c->fcol, c->mvol, body_state(c), 2, 0, c->cs.rhof, c->cs.dt,
+                       c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1], c->cs.gravity[2]);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+int dem_body_integrate(ibmdem_ctx *c) {
+    if (c->B == 0) return IBMDEM_OK;
+    hipLaunchKernelGGL(k_body_integrate, dim3(div_up(c->B, 64)), dim3(64), 0, c->stream, c->B, body_state(c),
+                       c->cs.dt / c->cs.couple_interval, c->cs.rhof, c->cs.gravity[0], c->cs.gravity[1],
+                       c->cs.gravity[2]);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+int dem_markers_follow(ibmdem_ctx *c) {
+    if (c->M == 0) return IBMDEM_OK;
+    hipLaunchKernelGGL(k_follow, dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->B, c->g, c->mx, c->mv,
+                       c->mbody, body_state(c), c->cs.dt / c->cs.couple_interval);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+// one CloudEvolutionStepMS with the body reduction and Newton-Euler fused
+int dem_substep_fused(ibmdem_ctx *c, bool last) {
+    int rc;
+    if ((rc = dem_contact(c))) return rc;
+    if (c->B > 0) {
+        hipLaunchKernelGGL(k_body_reduce, dim3(c->B), dim3(BT), 0, c->stream, c->M, c->B, c->g, c->body_start,
+                           c->mx, c->fdrag, c->fcol, c->mvol, body_state(c), last ? 1 : 0, 1, c->cs.rhof,
+                           c->cs.dt, c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1],
+                           c->cs.gravity[2]);
+        CHECK_LAUNCH(c);
+    }
+    if ((rc = dem_markers_follow(c))) return rc;
+    return dem_rebuild(c);
+}
+
+int dem_check_flags(ibmdem_ctx *c) {
+    HIPX(c, hipMemcpyAsync(c->flags_host, c->flags, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    if (c->flags_host[FL_OUTSIDE]) {
+        HIPX(c, hipMemsetAsync(c->flags, 0, FL_COUNT * sizeof(int), c->stream));
+        return set_error(c, IBMDEM_ERR_OUTSIDE, "a marker left the domain through a wall");
+    }
+    if (c->flags_host[FL_CAPACITY]) {
+        HIPX(c, hipMemsetAsync(c->flags, 0, FL_COUNT * sizeof(int), c->stream));
+        return set_error(c, IBMDEM_ERR_CAPACITY, "contact candidates exceed maxCollisionNo");
+    }
+    return IBMDEM_OK;
+}
+
+int dem_setup(ibmdem_ctx *c) { (void)c; return IBMDEM_OK; }

@@ -1,0 +1,463 @@
+// fluid.hip — staggered-grid predictor, bulk forcing, divergence and
+// projection (fluid.F90:2-250) as HBM-bound stencil kernels.
+//
+// Layout: interior-only FP64 arrays, x fastest. A thread owns one cell and
+// all three velocity components of it. Blocks are 64 x 4 cells in x-y and
+// one z plane; the block index is remapped so that each of the 8 XCDs walks
+// a contiguous slab of z planes (neighbouring planes share its L2).
+#include "common.hpp"
+
+#include <cmath>
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+constexpr int RED_BLOCKS = 1024;   // fixed: reductions never depend on timing
+constexpr int RED_THREADS = 256;
+
+// XCD-aware block order: hardware dispatches block b to XCD b % 8; give XCD
+// x the planes [x*nz/8, (x+1)*nz/8) so its L2 sees neighbouring planes.
+__device__ __forceinline__ void xcd_block(int &bx, int &by, int &bz) {
+    const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const long nb = (long)gx * gy * gz;
+    long b = (long)blockIdx.x + (long)gx * (blockIdx.y + (long)gy * blockIdx.z);
+    long per = nb / 8;
+    if (per > 0 && nb % 8 == 0) b = (b % 8) * per + b / 8;
+    bx = (int)(b % gx);
+    by = (int)((b / gx) % gy);
+    bz = (int)(b / ((long)gx * gy));
+}
+
+struct Direct {
+    const double *__restrict__ p;
+    long o, sy, sz;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        return p[o + di + dj * sy + dk * sz];
+    }
+};
+template <int C>
+struct Ghost {
+    const double *__restrict__ p;
+    GridDev g;
+    int i, j, k;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        return vel_at<C>(p, g, i + di, j + dj, k + dk);
+    }
+};
+struct GhostP {
+    const double *__restrict__ p;
+    GridDev g;
+    int i, j, k;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        return scal_at(p, g, i + di, j + dj, k + dk);
+    }
+};
+
+struct PredParams {
+    double dt, visc, rhof, a, c;
+    double cp[3];
+};
+
+// ConvectionTensor + ViscousTensor + PressureGradient + VelocityGuess for one
+// cell (fluid.F90:15-44, 58-66, 80-82, 118-126), the reference's operand
+// order, in compute type CT. Returns the three increments
+// a*RHS_noF - c*gradP (without the IBM force, already inside u~).
+template <typename CT, class LU, class LV, class LW, class LP>
+__device__ __forceinline__ void predictor_cell(const LU &u, const LV &v, const LW &w, const LP &P,
+                                               const GridDev &g, const PredParams &pp, CT out[3]) {
+    const CT h = CT(0.5);
+    const CT dx = CT(g.dx), dy = CT(g.dy), dz = CT(g.dz), visc = CT(pp.visc);
+    const CT u0 = CT(u(0, 0, 0)), uE = CT(u(1, 0, 0)), uW = CT(u(-1, 0, 0)), uN = CT(u(0, 1, 0)),
+             uS = CT(u(0, -1, 0)), uT = CT(u(0, 0, 1)), uB = CT(u(0, 0, -1));
+    const CT v0 = CT(v(0, 0, 0)), vE = CT(v(1, 0, 0)), vW = CT(v(-1, 0, 0)), vN = CT(v(0, 1, 0)),
+             vS = CT(v(0, -1, 0)), vT = CT(v(0, 0, 1)), vB = CT(v(0, 0, -1));
+    const CT w0 = CT(w(0, 0, 0)), wE = CT(w(1, 0, 0)), wW = CT(w(-1, 0, 0)), wN = CT(w(0, 1, 0)),
+             wS = CT(w(0, -1, 0)), wT = CT(w(0, 0, 1)), wB = CT(w(0, 0, -1));
+    // the diagonal taps of the flux averages
+    const CT v_wn = CT(v(-1, 1, 0)), w_wt = CT(w(-1, 0, 1));   // for u
+    const CT u_es = CT(u(1, -1, 0)), w_st = CT(w(0, -1, 1));   // for v
+    const CT u_eb = CT(u(1, 0, -1)), v_nb = CT(v(0, 1, -1));   // for w
+    const CT p0 = CT(P(0, 0, 0)), pW = CT(P(-1, 0, 0)), pS = CT(P(0, -1, 0)), pB = CT(P(0, 0, -1));
+
+    CT uuE = h * (uE + u0) * h * (uE + u0);
+    CT uuW = h * (uW + u0) * h * (uW + u0);
+    CT uvB = h * (uN + u0) * h * (v_wn + vN);
+    CT uvF = h * (uS + u0) * h * (vW + v0);
+    CT uwN = h * (uT + u0) * h * (w_wt + wT);
+    CT uwS = h * (uB + u0) * h * (wW + w0);
+
+    CT vuE = h * (vE + v0) * h * (u_es + uE);
+    CT vuW = h * (vW + v0) * h * (uS + u0);
+    CT vvB = h * (vN + v0) * h * (vN + v0);
+    CT vvF = h * (vS + v0) * h * (vS + v0);
+    CT vwN = h * (vT + v0) * h * (w_st + wT);
+    CT vwS = h * (vB + v0) * h * (wS + w0);
+
+    CT wuE = h * (wE + w0) * h * (u_eb + uE);
+    CT wuW = h * (wW + w0) * h * (uB + u0);
+    CT wvB = h * (wN + w0) * h * (v_nb + vN);
+    CT wvF = h * (wS + w0) * h * (vB + v0);
+    CT wwN = h * (wT + w0) * h * (wT + w0);
+    CT wwS = h * (wB + w0) * h * (wB + w0);
+
+    CT TIux = (uuE - uuW) / dx, TIuy = (uvB - uvF) / dy, TIuz = (uwN - uwS) / dz;
+    CT TIvx = (vuE - vuW) / dx, TIvy = (vvB - vvF) / dy, TIvz = (vwN - vwS) / dz;
+    CT TIwx = (wuE - wuW) / dx, TIwy = (wvB - wvF) / dy, TIwz = (wwN - wwS) / dz;
+
+    CT TVux = (uE + uW - CT(2) * u0) / (dx * dx) * visc;
+    CT TVuy = (uN + uS - CT(2) * u0) / (dy * dy) * visc;
+    CT TVuz = (uT + uB - CT(2) * u0) / (dz * dz) * visc;
+    CT TVvx = (vE + vW - CT(2) * v0) / (dx * dx) * visc;
+    CT TVvy = (vN + vS - CT(2) * v0) / (dy * dy) * visc;
+    CT TVvz = (vT + vB - CT(2) * v0) / (dz * dz) * visc;
+    CT TVwx = (wE + wW - CT(2) * w0) / (dx * dx) * visc;
+    CT TVwy = (wN + wS - CT(2) * w0) / (dy * dy) * visc;
+    CT TVwz = (wT + wB - CT(2) * w0) / (dz * dz) * visc;
+
+    const CT rhof = CT(pp.rhof);
+    CT Px = (p0 - pW) / dx / rhof;
+    CT Py = (p0 - pS) / dy / rhof;
+    CT Pz = (p0 - pB) / dz / rhof;
+
+    CT Ru = -(TIux + TIuy + TIuz) + (TVux + TVuy + TVuz) + CT(pp.cp[0]);
+    CT Rv = -(TIvx + TIvy + TIvz) + (TVvx + TVvy + TVvz) + CT(pp.cp[1]);
+    CT Rw = -(TIwx + TIwy + TIwz) + (TVwx + TVwy + TVwz) + CT(pp.cp[2]);
+    const CT a = CT(pp.a), c = CT(pp.c);
+    out[0] = a * Ru - c * Px;
+    out[1] = a * Rv - c * Py;
+    out[2] = a * Rw - c * Pz;
+}
+
+__device__ __forceinline__ void block_sum3(double s[3], double *part) {
+    __shared__ double red[3][BX * BY];
+    const int t = threadIdx.x + BX * threadIdx.y;
+    for (int q = 0; q < 3; ++q) red[q][t] = s[q];
+    __syncthreads();
+    for (int st = BX * BY / 2; st > 0; st >>= 1) {
+        if (t < st)
+            for (int q = 0; q < 3; ++q) red[q][t] += red[q][t + st];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < 3; ++q) part[q] = red[q][0];
+}
+
+// VelocityGuess fused form: u* = u~ + dt*(a*RHS_noF(u~) - c*gradP_old).
+// Because u~ = uold + dt*f after the multi-direct-forcing loop, this equals
+// the reference's uold + dt*(a*RHS(u~) + 0*RHSl - c*gradP) for a = 1.
+template <typename CT, bool BULK>
+__global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *__restrict__ u,
+                                                      const double *__restrict__ v,
+                                                      const double *__restrict__ w,
+                                                      const double *__restrict__ P, double *__restrict__ un,
+                                                      double *__restrict__ vn, double *__restrict__ wn,
+                                                      PredParams pp, double *__restrict__ partial) {
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (i <= g.nx && j <= g.ny) {
+        const long o = cidx(g, i, j, k);
+        CT inc[3];
+        const bool inner = i > 1 && i < g.nx && j > 1 && j < g.ny && k > 1 && k < g.nz;
+        if (inner) {
+            const long sy = g.nx, sz = (long)g.nx * g.ny;
+            predictor_cell<CT>(Direct{u, o, sy, sz}, Direct{v, o, sy, sz}, Direct{w, o, sy, sz},
+                               Direct{P, o, sy, sz}, g, pp, inc);
+        } else {
+            predictor_cell<CT>(Ghost<0>{u, g, i, j, k}, Ghost<1>{v, g, i, j, k}, Ghost<2>{w, g, i, j, k},
+                               GhostP{P, g, i, j, k}, g, pp, inc);
+        }
+        const double dt = pp.dt;
+        double nu_ = wall_face<0>(g, i, j, k) ? 0.0 : u[o] + dt * (double)inc[0];
+        double nv_ = wall_face<1>(g, i, j, k) ? 0.0 : v[o] + dt * (double)inc[1];
+        double nw_ = wall_face<2>(g, i, j, k) ? 0.0 : w[o] + dt * (double)inc[2];
+        un[o] = nu_;
+        vn[o] = nv_;
+        wn[o] = nw_;
+        s[0] = nu_; s[1] = nv_; s[2] = nw_;
+    }
+    if (BULK) {
+        const long blk = (long)bx + (long)gridDim.x * (by + (long)gridDim.y * bz);
+        block_sum3(s, partial + 3 * blk);
+    }
+}
+
+// ModifyBulkVelocity finalize (fluid.F90:177-183): one block sums the block
+// partials in a fixed order and stores the shifts dt*modifyP.
+__global__ __launch_bounds__(256) void k_bulk_finalize(const double *__restrict__ partial, long nparts,
+                                                       double *__restrict__ scal, int sw0, int sw1, int sw2,
+                                                       double t0, double t1, double t2, double ntot,
+                                                       double dt) {
+    __shared__ double red[3][256];
+    const int t = threadIdx.x;
+    double s[3] = {0, 0, 0};
+    for (long q = t; q < nparts; q += 256)
+        for (int c = 0; c < 3; ++c) s[c] += partial[3 * q + c];
+    for (int c = 0; c < 3; ++c) red[c][t] = s[c];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (t < st)
+            for (int c = 0; c < 3; ++c) red[c][t] += red[c][t + st];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const int sw[3] = {sw0, sw1, sw2};
+        const double tg[3] = {t0, t1, t2};
+        for (int c = 0; c < 3; ++c) {
+            double sum = red[c][0];
+            scal[SC_BULKSUM + c] = sum;
+            double mod = 0.0;
+            if (sw[c] == 1) mod = -(sum / ntot - tg[c]) / dt;
+            scal[SC_MODP + c] = dt * mod;
+        }
+    }
+}
+
+__global__ void k_bulk_from_sums(double *scal, int sw0, int sw1, int sw2, double t0, double t1, double t2,
+                                 double ntot, double dt) {
+    const int sw[3] = {sw0, sw1, sw2};
+    const double tg[3] = {t0, t1, t2};
+    for (int c = 0; c < 3; ++c) {
+        double mod = 0.0;
+        if (sw[c] == 1) mod = -(scal[SC_BULKSUM + c] / ntot - tg[c]) / dt;
+        scal[SC_MODP + c] = dt * mod;
+    }
+}
+
+// apply the bulk shift in place (standalone ModifyBulkVelocity)
+__global__ __launch_bounds__(BX *BY) void k_bulk_apply(GridDev g, double *u, double *v, double *w,
+                                                       const double *__restrict__ scal) {
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
+    if (i > g.nx || j > g.ny) return;
+    const long o = cidx(g, i, j, k);
+    if (!wall_face<0>(g, i, j, k)) u[o] = u[o] + scal[SC_MODP + 0];
+    if (!wall_face<1>(g, i, j, k)) v[o] = v[o] + scal[SC_MODP + 1];
+    if (!wall_face<2>(g, i, j, k)) w[o] = w[o] + scal[SC_MODP + 2];
+}
+
+// VelocityDivergence(c) (fluid.F90:97-100) on the (pending-shifted) velocity
+__global__ __launch_bounds__(BX *BY) void k_divergence(GridDev g, const double *__restrict__ u,
+                                                       const double *__restrict__ v,
+                                                       const double *__restrict__ w, double *__restrict__ rhs,
+                                                       const double *__restrict__ scal, int shifted, double dt,
+                                                       double cRK, double rhof) {
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
+    if (i > g.nx || j > g.ny) return;
+    double su = 0.0, sv = 0.0, sw = 0.0;
+    if (shifted) { su = scal[SC_MODP]; sv = scal[SC_MODP + 1]; sw = scal[SC_MODP + 2]; }
+    double u1, u0, v1, v0, w1, w0;
+    if (shifted) {
+        u1 = vel_at_shift<0>(u, g, i + 1, j, k, su); u0 = vel_at_shift<0>(u, g, i, j, k, su);
+        v1 = vel_at_shift<1>(v, g, i, j + 1, k, sv); v0 = vel_at_shift<1>(v, g, i, j, k, sv);
+        w1 = vel_at_shift<2>(w, g, i, j, k + 1, sw); w0 = vel_at_shift<2>(w, g, i, j, k, sw);
+    } else {
+        u1 = vel_at<0>(u, g, i + 1, j, k); u0 = vel_at<0>(u, g, i, j, k);
+        v1 = vel_at<1>(v, g, i, j + 1, k); v0 = vel_at<1>(v, g, i, j, k);
+        w1 = vel_at<2>(w, g, i, j, k + 1); w0 = vel_at<2>(w, g, i, j, k);
+    }
+    double d = (u1 - u0) / g.dx + (v1 - v0) / g.dy + (w1 - w0) / g.dz;
+    rhs[cidx(g, i, j, k)] = -d / dt / cRK * rhof;
+}
+
+// PressureCorrection + VelocityCorrection(c) (fluid.F90:206, 221-223):
+// P_new = P + P' and u = u* + c*Px_old*dt - c*(P_new(i)-P_new(i-1))/dx*dt/rhof,
+// with Px_old = (P(i)-P(i-1))/dx/rhof from the pressure before correction.
+__global__ __launch_bounds__(BX *BY) void k_project(GridDev g, double *u, double *v, double *w,
+                                                    const double *__restrict__ P, const double *__restrict__ Pc,
+                                                    double *__restrict__ Pn, const double *__restrict__ scal,
+                                                    int shifted, double dt, double cRK, double rhof) {
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
+    if (i > g.nx || j > g.ny) return;
+    const long o = cidx(g, i, j, k);
+    const double p0 = P[o], q0 = Pc[o];
+    const double pn0 = p0 + q0;
+    Pn[o] = pn0;
+    const double d[3] = {g.dx, g.dy, g.dz};
+    double *vel[3] = {u, v, w};
+    for (int c = 0; c < 3; ++c) {
+        const int im = c == 0 ? i - 1 : i, jm = c == 1 ? j - 1 : j, km = c == 2 ? k - 1 : k;
+        const double pm = scal_at(P, g, im, jm, km), qm = scal_at(Pc, g, im, jm, km);
+        const double Pold_grad = (p0 - pm) / d[c] / rhof;
+        const double pnm = pm + qm;
+        const bool face = (c == 0 ? i : c == 1 ? j : k) == 1 && g.bt[2 * c] == IBMDEM_BC_WALL;
+        if (face) { vel[c][o] = 0.0; continue; }
+        double us = vel[c][o];
+        if (shifted) us = us + scal[SC_MODP + c];
+        vel[c][o] = us + cRK * Pold_grad * dt - cRK * (pn0 - pnm) / (d[c]) * dt / rhof;
+    }
+}
+
+// force the wall-face normal components to zero (upload-time Boundary)
+__global__ __launch_bounds__(BX *BY) void k_wall_faces(GridDev g, double *u, double *v, double *w) {
+    const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1,
+              k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const long o = cidx(g, i, j, k);
+    if (wall_face<0>(g, i, j, k)) u[o] = 0.0;
+    if (wall_face<1>(g, i, j, k)) v[o] = 0.0;
+    if (wall_face<2>(g, i, j, k)) w[o] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// deterministic reduction: RED_BLOCKS blocks each sum a contiguous chunk,
+// thread-strided, then an LDS tree; the finalize block sums the partials the
+// same way. The association is fixed by (n, RED_BLOCKS, RED_THREADS).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const double *__restrict__ a,
+                                                                const double *__restrict__ b, long n,
+                                                                long skip, double *__restrict__ partial) {
+    __shared__ double red[RED_THREADS];
+    const long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+    double s = 0.0;
+    for (long q = lo + threadIdx.x; q < hi; q += RED_THREADS) {
+        if (q < skip) continue;
+        s += b ? a[q] * b[q] : a[q];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__restrict__ partial, int np,
+                                                              double *__restrict__ out) {
+    __shared__ double red[RED_THREADS];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < np; q += RED_THREADS) s += partial[q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+dim3 field_grid(const GridDev &g) { return dim3(div_up(g.nx, BX), div_up(g.ny, BY), g.nz); }
+
+}  // namespace
+
+int reduce_sum(ibmdem_ctx *c, const double *a, const double *b, long n, int slot, long skip_first) {
+    int nb = (int)std::min<long>(RED_BLOCKS, std::max<long>(1, n));
+    hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(RED_THREADS), 0, c->stream, a, b, n, skip_first,
+                       c->partial);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->partial, nb,
+                       c->scal + slot);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+int fluid_predictor(ibmdem_ctx *c, double a, double b, double cc) {
+    if (b != 0.0 || a != 1.0)
+        return set_error(c, IBMDEM_ERR_UNSUPPORTED,
+                         "predictor supports the forward-Euler coefficients (a=1, b=0)");
+    const GridDev &g = c->g;
+    dim3 grid = field_grid(g), blk(BX, BY);
+    PredParams pp{c->cs.dt, c->cs.visc, c->cs.rhof, a, cc, {c->cs.cp[0], c->cs.cp[1], c->cs.cp[2]}};
+    const int cur = c->vcur, nxt = 1 - c->vcur;
+    const bool bulk = c->cs.bulk_switch[0] == 1 || c->cs.bulk_switch[1] == 1 || c->cs.bulk_switch[2] == 1;
+    const double bytes = 7.0 * 8.0 * (double)c->n;
+    prof_begin(c, IBMDEM_K_PREDICTOR);
+    if (c->cs.precision == IBMDEM_MIXED) {
+        if (bulk)
+            hipLaunchKernelGGL((k_predictor<float, true>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],
+                               c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial);
+        else
+            hipLaunchKernelGGL((k_predictor<float, false>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],
+                               c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial);
+    } else {
+        if (bulk)
+            hipLaunchKernelGGL((k_predictor<double, true>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],
+                               c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial);
+        else
+            hipLaunchKernelGGL((k_predictor<double, false>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],
+                               c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial);
+    }
+    CHECK_LAUNCH(c);
+    prof_end(c, IBMDEM_K_PREDICTOR, bytes);
+    c->vcur = nxt;
+    c->bulk_pending = false;
+    if (bulk) {
+        // partials of this launch feed the bulk finalize
+        const long nparts = (long)grid.x * grid.y * grid.z;
+        hipLaunchKernelGGL(k_bulk_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nparts, c->scal,
+                           c->cs.bulk_switch[0], c->cs.bulk_switch[1], c->cs.bulk_switch[2],
+                           c->cs.bulk_target[0], c->cs.bulk_target[1], c->cs.bulk_target[2],
+                           (double)((long)g.nx * g.ny * g.nz), c->cs.dt);
+        CHECK_LAUNCH(c);
+    }
+    return IBMDEM_OK;
+}
+
+int fluid_bulk(ibmdem_ctx *c, bool apply_now) {
+    const bool bulk = c->cs.bulk_switch[0] == 1 || c->cs.bulk_switch[1] == 1 || c->cs.bulk_switch[2] == 1;
+    if (!bulk) return IBMDEM_OK;
+    const GridDev &g = c->g;
+    if (apply_now) {
+        // standalone ModifyBulkVelocity: sum u,v,w now (fluid.F90:163-165),
+        // then shift in place
+        for (int q = 0; q < 3; ++q) {
+            const double *f = q == 0 ? c->u[c->vcur] : q == 1 ? c->v[c->vcur] : c->w[c->vcur];
+            int rc = reduce_sum(c, f, nullptr, c->n, SC_BULKSUM + q, 0);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_bulk_from_sums, dim3(1), dim3(1), 0, c->stream, c->scal, c->cs.bulk_switch[0],
+                           c->cs.bulk_switch[1], c->cs.bulk_switch[2], c->cs.bulk_target[0],
+                           c->cs.bulk_target[1], c->cs.bulk_target[2], (double)((long)g.nx * g.ny * g.nz),
+                           c->cs.dt);
+        CHECK_LAUNCH(c);
+        hipLaunchKernelGGL(k_bulk_apply, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
+                           c->v[c->vcur], c->w[c->vcur], c->scal);
+        CHECK_LAUNCH(c);
+        c->bulk_pending = false;
+    } else {
+        // the sums were reduced inside the predictor; divergence and
+        // projection apply the shift on the fly
+        c->bulk_pending = true;
+    }
+    return IBMDEM_OK;
+}
+
+int fluid_divergence(ibmdem_ctx *c, double cc) {
+    const GridDev &g = c->g;
+    prof_begin(c, IBMDEM_K_DIVERGENCE);
+    hipLaunchKernelGGL(k_divergence, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur], c->v[c->vcur],
+                       c->w[c->vcur], c->rhs, c->scal, c->bulk_pending ? 1 : 0, c->cs.dt, cc, c->cs.rhof);
+    CHECK_LAUNCH(c);
+    prof_end(c, IBMDEM_K_DIVERGENCE, 4.0 * 8.0 * (double)c->n);
+    return IBMDEM_OK;
+}
+
+int fluid_project(ibmdem_ctx *c, double cc) {
+    const GridDev &g = c->g;
+    const int pn = 1 - c->pcur;
+    prof_begin(c, IBMDEM_K_PROJECT);
+    hipLaunchKernelGGL(k_project, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur], c->v[c->vcur],
+                       c->w[c->vcur], c->P[c->pcur], c->pcorr, c->P[pn], c->scal, c->bulk_pending ? 1 : 0,
+                       c->cs.dt, cc, c->cs.rhof);
+    CHECK_LAUNCH(c);
+    prof_end(c, IBMDEM_K_PROJECT, 9.0 * 8.0 * (double)c->n);
+    c->pcur = pn;
+    c->bulk_pending = false;
+    return IBMDEM_OK;
+}
+
+int fluid_upload_bc(ibmdem_ctx *c) {
+    const GridDev &g = c->g;
+    hipLaunchKernelGGL(k_wall_faces, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
+                       c->v[c->vcur], c->w[c->vcur]);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}

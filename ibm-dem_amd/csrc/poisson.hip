@@ -1,0 +1,522 @@
+// poisson.hip — pressure Poisson solve: multigrid-preconditioned CG.
+//
+// Replaces SetPressureVector + PoissonSolver (PETSc KSPSolve, GMRES(30) +
+// BoomerAMG) + GetLocalArray (poisson.F90:107-186).
+//
+// Operator (poisson.F90:74-94, initialization.F90:105-173): the 7-point
+// -lap_h with wall neighbours omitted (Neumann) and periodic neighbours
+// wrapped; (Ax)_i = diag_i x_i - nbsum_i, nbsum = (xw+xe)/hx^2 +
+// (xs+xn)/hy^2 + (xb+xt)/hz^2, wall neighbours contributing 0.
+// The reference pins row P'(1,1,1) to the identity with b[0] = 0
+// (poisson.F90:78,97-100,126); that system's solution is the solution of
+// the singular system with b[0] := -sum_{i>0} b[i], shifted so that
+// P'(1,1,1) = 0, which is what is solved here.
+//
+// Preconditioner: one symmetric V-cycle. Smoother red-black Gauss-Seidel
+// (pre: red then black, post: black then red); coarsest level: symmetric
+// red-black sweeps in one work-group (damped Jacobi if a periodic dimension
+// is odd); transfers: cell-centred trilinear prolongation (3/4, 1/4 per
+// dimension) and restriction = prolongation^T / 8, with wall mirror and
+// periodic wrap index rules. In the mixed form the whole V-cycle runs in
+// FP32 on FP32 level arrays; the CG vectors, matrix-vector product, inner
+// products and the stopping test stay FP64.
+//
+// Reductions have a fixed association (fixed block counts, LDS trees), so
+// results are run-to-run identical.
+#include "common.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+constexpr int RB = 256;           // threads per block for level kernels
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_T = 256;
+constexpr int COARSE_T = 1024;
+
+struct LaunchShape {
+    dim3 grid, block;
+};
+int pow2ceil(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+LaunchShape shape_for(int nx, int ny, int nz) {
+    int bx = std::min(64, pow2ceil(nx));
+    int by = std::min(RB / bx, pow2ceil(ny));
+    return {dim3(div_up(nx, bx), div_up(ny, by), nz), dim3(bx, by, 1)};
+}
+
+template <typename T>
+__device__ __forceinline__ T nb_val(const T *__restrict__ x, const LevelDev &L, int i, int j, int k, int d,
+                                    int dir) {
+    int a[3] = {i, j, k};
+    const int n[3] = {L.nx, L.ny, L.nz};
+    int q = a[d] + dir;
+    if (q < 0) {
+        if (L.bt[2 * d] == IBMDEM_BC_WALL) return T(0);
+        q += n[d];
+    } else if (q >= n[d]) {
+        if (L.bt[2 * d + 1] == IBMDEM_BC_WALL) return T(0);
+        q -= n[d];
+    }
+    a[d] = q;
+    return x[((long)a[2] * L.ny + a[1]) * L.nx + a[0]];
+}
+
+// nbsum in the oracle's association: ((xw+xe)*ihx2 + (xs+xn)*ihy2) + (xb+xt)*ihz2
+template <typename T>
+__device__ __forceinline__ T nbsum(const T *__restrict__ x, const LevelDev &L, int i, int j, int k) {
+    const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+    if (i > 0 && i < L.nx - 1 && j > 0 && j < L.ny - 1 && k > 0 && k < L.nz - 1) {
+        const long o = ((long)k * L.ny + j) * L.nx + i, sy = L.nx, sz = (long)L.nx * L.ny;
+        T sx = (x[o - 1] + x[o + 1]) * ihx2;
+        T syy = (x[o - sy] + x[o + sy]) * ihy2;
+        T szz = (x[o - sz] + x[o + sz]) * ihz2;
+        return sx + syy + szz;
+    }
+    T sx = (nb_val(x, L, i, j, k, 0, -1) + nb_val(x, L, i, j, k, 0, 1)) * ihx2;
+    T syy = (nb_val(x, L, i, j, k, 1, -1) + nb_val(x, L, i, j, k, 1, 1)) * ihy2;
+    T szz = (nb_val(x, L, i, j, k, 2, -1) + nb_val(x, L, i, j, k, 2, 1)) * ihz2;
+    return sx + syy + szz;
+}
+
+__device__ __forceinline__ int diag_index(const LevelDev &L, int i, int j, int k) {
+    int cx = 2 - ((i == 0 && L.bt[0] == IBMDEM_BC_WALL) ? 1 : 0) - ((i == L.nx - 1 && L.bt[1] == IBMDEM_BC_WALL) ? 1 : 0);
+    int cy = 2 - ((j == 0 && L.bt[2] == IBMDEM_BC_WALL) ? 1 : 0) - ((j == L.ny - 1 && L.bt[3] == IBMDEM_BC_WALL) ? 1 : 0);
+    int cz = 2 - ((k == 0 && L.bt[4] == IBMDEM_BC_WALL) ? 1 : 0) - ((k == L.nz - 1 && L.bt[5] == IBMDEM_BC_WALL) ? 1 : 0);
+    return cx * 9 + cy * 3 + cz;
+}
+
+// one red-black half sweep; each thread owns the cell of the colour in a
+// pair of x cells. zero_init: the level's x is implicitly 0 (first sweep of
+// a V-cycle visit); the other colour is written 0.
+template <typename T, typename TB>
+__global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
+                                                int color, int zero_init) {
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (j >= L.ny) return;
+    const int i = 2 * ip + ((j + k + color) & 1);
+    const long o = ((long)k * L.ny + j) * L.nx + i;
+    if (zero_init) {
+        const int io = 2 * ip + ((j + k + color + 1) & 1);
+        if (io < L.nx) x[o - i + io] = T(0);
+        if (i >= L.nx) return;
+        const T id = T(L.idiag[diag_index(L, i, j, k)]);
+        x[o] = (T(b[o]) + T(0)) * id;
+        return;
+    }
+    if (i >= L.nx) return;
+    const T id = T(L.idiag[diag_index(L, i, j, k)]);
+    x[o] = (T(b[o]) + nbsum(x, L, i, j, k)) * id;
+}
+
+// r = b - A x
+template <typename T, typename TB>
+__global__ __launch_bounds__(RB) void k_residual(LevelDev L, const T *__restrict__ x, const TB *__restrict__ b,
+                                                 T *__restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    const long o = ((long)k * L.ny + j) * L.nx + i;
+    const T dg = T(L.diag[diag_index(L, i, j, k)]);
+    r[o] = T(b[o]) - (dg * x[o] - nbsum(x, L, i, j, k));
+}
+
+__device__ __forceinline__ int fine_index(int f, int n, int lo, int hi) {
+    if (f < 0) return lo == IBMDEM_BC_WALL ? 0 : f + n;
+    if (f >= n) return hi == IBMDEM_BC_WALL ? n - 1 : f - n;
+    return f;
+}
+
+// b_c = (1/8) sum w r_f over the 4x4x4 fine cells, w = wz*wy*wx from (1/4,3/4,3/4,1/4)
+template <typename T>
+__global__ __launch_bounds__(RB) void k_restrict(LevelDev F, const T *__restrict__ rf, LevelDev Cl,
+                                                 T *__restrict__ bc) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
+    if (I >= Cl.nx || J >= Cl.ny) return;
+    const T wt[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
+    int fi[4], fj[4], fk[4];
+    for (int a = 0; a < 4; ++a) {
+        fi[a] = fine_index(2 * I - 1 + a, F.nx, F.bt[0], F.bt[1]);
+        fj[a] = fine_index(2 * J - 1 + a, F.ny, F.bt[2], F.bt[3]);
+        fk[a] = fine_index(2 * K - 1 + a, F.nz, F.bt[4], F.bt[5]);
+    }
+    T s = T(0);
+    for (int c3 = 0; c3 < 4; ++c3)
+        for (int c2 = 0; c2 < 4; ++c2) {
+            const long row = ((long)fk[c3] * F.ny + fj[c2]) * F.nx;
+            for (int c1 = 0; c1 < 4; ++c1) {
+                T w = wt[c3] * wt[c2] * wt[c1];
+                s = s + w * rf[row + fi[c1]];
+            }
+        }
+    bc[((long)K * Cl.ny + J) * Cl.nx + I] = T(0.125) * s;
+}
+
+__device__ __forceinline__ void parents(int f, int nc, int lo, int hi, int &p0, int &p1) {
+    int cc = f / 2;
+    int nb = (f & 1) ? cc + 1 : cc - 1;
+    if (nb < 0) nb = lo == IBMDEM_BC_WALL ? cc : nb + nc;
+    else if (nb >= nc) nb = hi == IBMDEM_BC_WALL ? cc : nb - nc;
+    p0 = cc; p1 = nb;
+}
+
+// x_f += sum (wz*wy*wx) x_c over the 8 parents, weights (3/4 own, 1/4 neighbour)
+template <typename T>
+__global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, LevelDev Cl,
+                                                const T *__restrict__ xc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= F.nx || j >= F.ny) return;
+    int pi[2], pj[2], pk[2];
+    parents(i, Cl.nx, Cl.bt[0], Cl.bt[1], pi[0], pi[1]);
+    parents(j, Cl.ny, Cl.bt[2], Cl.bt[3], pj[0], pj[1]);
+    parents(k, Cl.nz, Cl.bt[4], Cl.bt[5], pk[0], pk[1]);
+    const T wt[2] = {T(0.75), T(0.25)};
+    T s = T(0);
+    for (int c3 = 0; c3 < 2; ++c3)
+        for (int c2 = 0; c2 < 2; ++c2)
+            for (int c1 = 0; c1 < 2; ++c1) {
+                T w = wt[c3] * wt[c2] * wt[c1];
+                s = s + w * xc[((long)pk[c3] * Cl.ny + pj[c2]) * Cl.nx + pi[c1]];
+            }
+    const long o = ((long)k * F.ny + j) * F.nx + i;
+    xf[o] = xf[o] + s;
+}
+
+// coarsest level: one work-group, symmetric sweeps in place
+template <typename T>
+__global__ __launch_bounds__(COARSE_T) void k_coarse(LevelDev L, T *__restrict__ x, const T *__restrict__ b,
+                                                     T *__restrict__ tmp, int sweeps) {
+    const long n = (long)L.nx * L.ny * L.nz;
+    for (long q = threadIdx.x; q < n; q += COARSE_T) x[q] = T(0);
+    __syncthreads();
+    auto half = [&](int color) {
+        for (long q = threadIdx.x; q < n; q += COARSE_T) {
+            const int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+            if (((i + j + k) & 1) != color) continue;
+            x[q] = (b[q] + nbsum(x, L, i, j, k)) * T(L.idiag[diag_index(L, i, j, k)]);
+        }
+        __syncthreads();
+    };
+    if (L.rb_valid) {
+        for (int s = 0; s < sweeps; ++s) { half(0); half(1); }
+        for (int s = 0; s < sweeps; ++s) { half(1); half(0); }
+    } else {
+        const T om = T(0.8);
+        for (int s = 0; s < 2 * sweeps; ++s) {
+            for (long q = threadIdx.x; q < n; q += COARSE_T) {
+                const int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+                T xn = (b[q] + nbsum(x, L, i, j, k)) * T(L.idiag[diag_index(L, i, j, k)]);
+                tmp[q] = x[q] + om * (xn - x[q]);
+            }
+            __syncthreads();
+            for (long q = threadIdx.x; q < n; q += COARSE_T) x[q] = tmp[q];
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CG kernels (FP64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_reduce(double s, double *red) {
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = RED_T / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// Ap = A p, partial sums of p.Ap (chunked: block b owns a contiguous range)
+__global__ __launch_bounds__(RED_T) void k_matvec_dot(LevelDev L, const double *__restrict__ p,
+                                                      double *__restrict__ Ap, double *__restrict__ partial) {
+    __shared__ double red[RED_T];
+    const long n = (long)L.nx * L.ny * L.nz;
+    const long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+    double s = 0.0;
+    for (long q = lo + threadIdx.x; q < hi; q += RED_T) {
+        const int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+        const double a = L.diag[diag_index(L, i, j, k)] * p[q] - nbsum(p, L, i, j, k);
+        Ap[q] = a;
+        s += p[q] * a;
+    }
+    double t = block_reduce(s, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// x += alpha p; r -= alpha Ap; partial sums of r.r
+__global__ __launch_bounds__(RED_T) void k_cg_update(long n, double *__restrict__ x, double *__restrict__ r,
+                                                     const double *__restrict__ p, const double *__restrict__ Ap,
+                                                     const double *__restrict__ scal, double *__restrict__ partial) {
+    __shared__ double red[RED_T];
+    const double alpha = scal[SC_ALPHA];
+    const long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+    double s = 0.0;
+    for (long q = lo + threadIdx.x; q < hi; q += RED_T) {
+        x[q] = x[q] + alpha * p[q];
+        double rq = r[q] - alpha * Ap[q];
+        r[q] = rq;
+        s += rq * rq;
+    }
+    double t = block_reduce(s, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// partial sums of a.b
+__global__ __launch_bounds__(RED_T) void k_dot(long n, const double *__restrict__ a, const double *__restrict__ b,
+                                               double *__restrict__ partial) {
+    __shared__ double red[RED_T];
+    const long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+    double s = 0.0;
+    for (long q = lo + threadIdx.x; q < hi; q += RED_T) s += a[q] * b[q];
+    double t = block_reduce(s, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// finalize a partial sum into scal[slot]; op: 0 store, 1 alpha = rz/pAp,
+// 2 beta = rz_new/rz and rz = rz_new
+__global__ __launch_bounds__(RED_T) void k_final(const double *__restrict__ partial, int np, double *__restrict__ scal,
+                                                 int slot, int op) {
+    __shared__ double red[RED_T];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < np; q += RED_T) s += partial[q];
+    double t = block_reduce(s, red);
+    if (threadIdx.x == 0) {
+        scal[slot] = t;
+        if (op == 1) scal[SC_ALPHA] = scal[SC_RZ] / t;
+        if (op == 2) { scal[SC_BETA] = t / scal[SC_RZ]; scal[SC_RZ] = t; }
+    }
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(RED_T) void k_cg_p(long n, double *__restrict__ p, const double *__restrict__ z,
+                                                const double *__restrict__ scal) {
+    const double beta = scal[SC_BETA];
+    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T)
+        p[q] = z[q] + beta * p[q];
+}
+
+// r = rhs with the pinned row, x = 0
+__global__ __launch_bounds__(RED_T) void k_cg_init(long n, const double *__restrict__ rhs, double *__restrict__ r,
+                                                   double *__restrict__ x, const double *__restrict__ scal) {
+    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
+        r[q] = q == 0 ? -scal[SC_PIN] : rhs[q];
+        x[q] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(RED_T) void k_copy(long n, double *__restrict__ d, const double *__restrict__ s) {
+    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) d[q] = s[q];
+}
+template <typename TD, typename TS>
+__global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, const TS *__restrict__ s) {
+    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) d[q] = TD(s[q]);
+}
+
+// gauge P'(1,1,1) = 0: pcorr = x - x[0]
+__global__ __launch_bounds__(RED_T) void k_gauge(long n, const double *__restrict__ x, double *__restrict__ pc) {
+    const double x0 = x[0];
+    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) pc[q] = x[q] - x0;
+}
+
+int stream_blocks(long n) { return (int)std::min<long>(4096, std::max<long>(1, (n + RED_T - 1) / RED_T)); }
+
+LevelDev make_level(int nx, int ny, int nz, const int bt[6], const double h[3]) {
+    LevelDev L;
+    std::memset(&L, 0, sizeof L);
+    L.nx = nx; L.ny = ny; L.nz = nz;
+    for (int s = 0; s < 6; ++s) L.bt[s] = bt[s];
+    L.ihx2 = 1.0 / (h[0] * h[0]);
+    L.ihy2 = 1.0 / (h[1] * h[1]);
+    L.ihz2 = 1.0 / (h[2] * h[2]);
+    for (int cx = 0; cx < 3; ++cx)
+        for (int cy = 0; cy < 3; ++cy)
+            for (int cz = 0; cz < 3; ++cz) {
+                double dsum0 = cx * L.ihx2, dsum1 = cy * L.ihy2, dsum2 = cz * L.ihz2;
+                double d = dsum0 + dsum1 + dsum2;
+                L.diag[cx * 9 + cy * 3 + cz] = d;
+                L.idiag[cx * 9 + cy * 3 + cz] = d > 0 ? 1.0 / d : 0.0;
+            }
+    const int n[3] = {nx, ny, nz};
+    L.rb_valid = 1;
+    for (int d = 0; d < 3; ++d)
+        if (bt[2 * d] == IBMDEM_BC_PERIODIC && (n[d] % 2)) L.rb_valid = 0;
+    return L;
+}
+
+template <typename T>
+int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
+    MGLevel<T> &L = lev[l];
+    const LevelDev &d = L.d;
+    hipStream_t s = c->stream;
+    if (l == c->nlev - 1) {
+        hipLaunchKernelGGL((k_coarse<T>), dim3(1), dim3(COARSE_T), 0, s, d, L.x, L.b, L.r, c->cs.mg_coarse_sweeps);
+        CHECK_LAUNCH(c);
+        return IBMDEM_OK;
+    }
+    LaunchShape rb = shape_for(div_up(d.nx, 2), d.ny, d.nz);
+    LaunchShape full = shape_for(d.nx, d.ny, d.nz);
+    for (int q = 0; q < c->cs.mg_pre; ++q) {
+        prof_begin(c, IBMDEM_K_MG_SMOOTH);
+        hipLaunchKernelGGL((k_rb_half<T, T>), rb.grid, rb.block, 0, s, d, L.x, L.b, 0, q == 0 ? 1 : 0);
+        prof_end(c, IBMDEM_K_MG_SMOOTH, (l == 0) ? (q == 0 ? 2.0 : 2.5) * sizeof(T) * (double)L.n : 0.0);
+        hipLaunchKernelGGL((k_rb_half<T, T>), rb.grid, rb.block, 0, s, d, L.x, L.b, 1, 0);
+        CHECK_LAUNCH(c);
+    }
+    hipLaunchKernelGGL((k_residual<T, T>), full.grid, full.block, 0, s, d, L.x, L.b, L.r);
+    CHECK_LAUNCH(c);
+    MGLevel<T> &C = lev[l + 1];
+    LaunchShape cs = shape_for(C.d.nx, C.d.ny, C.d.nz);
+    hipLaunchKernelGGL((k_restrict<T>), cs.grid, cs.block, 0, s, d, L.r, C.d, C.b);
+    CHECK_LAUNCH(c);
+    int rc = vcycle<T>(c, lev, l + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_prolong<T>), full.grid, full.block, 0, s, d, L.x, C.d, C.x);
+    CHECK_LAUNCH(c);
+    for (int q = 0; q < c->cs.mg_post; ++q) {
+        hipLaunchKernelGGL((k_rb_half<T, T>), rb.grid, rb.block, 0, s, d, L.x, L.b, 1, 0);
+        hipLaunchKernelGGL((k_rb_half<T, T>), rb.grid, rb.block, 0, s, d, L.x, L.b, 0, 0);
+        CHECK_LAUNCH(c);
+    }
+    return IBMDEM_OK;
+}
+
+// z = M^{-1} r
+int precondition(ibmdem_ctx *c) {
+    const long n = c->n;
+    if (c->cs.precision == IBMDEM_MIXED) {
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(stream_blocks(n)), dim3(RED_T), 0, c->stream, n,
+                           c->lev32[0].b, c->cg_r);
+        CHECK_LAUNCH(c);
+        int rc = vcycle<float>(c, c->lev32, 0);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(stream_blocks(n)), dim3(RED_T), 0, c->stream, n,
+                           c->cg_z, c->lev32[0].x);
+        CHECK_LAUNCH(c);
+        return IBMDEM_OK;
+    }
+    c->lev64[0].b = c->cg_r;
+    c->lev64[0].x = c->cg_z;
+    return vcycle<double>(c, c->lev64, 0);
+}
+
+int dot_into(ibmdem_ctx *c, const double *a, const double *b, int slot, int op) {
+    hipLaunchKernelGGL(k_dot, dim3(RED_BLOCKS), dim3(RED_T), 0, c->stream, c->n, a, b, c->partial);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, c->stream, c->partial, RED_BLOCKS, c->scal, slot, op);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+int read_scalars(ibmdem_ctx *c) {
+    HIPX(c, hipMemcpyAsync(c->scal_host, c->scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return IBMDEM_OK;
+}
+
+}  // namespace
+
+int poisson_setup(ibmdem_ctx *c) {
+    int bt[6];
+    for (int s = 0; s < 6; ++s) bt[s] = c->g.bt[s];
+    double h[3] = {c->g.dx, c->g.dy, c->g.dz};
+    int nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
+    const int maxl = c->cs.mg_max_levels > 0 ? std::min(c->cs.mg_max_levels, IBMDEM_MAX_LEVELS) : IBMDEM_MAX_LEVELS;
+    c->nlev = 0;
+    while (true) {
+        LevelDev L = make_level(nx, ny, nz, bt, h);
+        long n = (long)nx * ny * nz;
+        const int l = c->nlev;
+        c->lev64[l].d = L; c->lev64[l].n = n;
+        c->lev32[l].d = L; c->lev32[l].n = n;
+        int rc;
+        if (c->cs.precision == IBMDEM_MIXED) {
+            if ((rc = dev_alloc(c, &c->lev32[l].x, n))) return rc;
+            if ((rc = dev_alloc(c, &c->lev32[l].b, n))) return rc;
+            if ((rc = dev_alloc(c, &c->lev32[l].r, n))) return rc;
+        } else {
+            if (l > 0) {
+                if ((rc = dev_alloc(c, &c->lev64[l].x, n))) return rc;
+                if ((rc = dev_alloc(c, &c->lev64[l].b, n))) return rc;
+            }
+            if ((rc = dev_alloc(c, &c->lev64[l].r, n))) return rc;
+        }
+        c->nlev++;
+        if (c->nlev >= maxl) break;
+        if ((nx % 2) || (ny % 2) || (nz % 2)) break;
+        if (nx < 4 || ny < 4 || nz < 4) break;
+        nx /= 2; ny /= 2; nz /= 2;
+        h[0] *= 2; h[1] *= 2; h[2] *= 2;
+    }
+    return IBMDEM_OK;
+}
+
+int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    const long n = c->n;
+    hipStream_t s = c->stream;
+    int rc;
+    // pinned row: b[0] = -sum_{q>0} b[q]
+    if ((rc = reduce_sum(c, c->rhs, nullptr, n, SC_PIN, 1))) return rc;
+    hipLaunchKernelGGL(k_cg_init, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->rhs, c->cg_r, c->cg_x, c->scal);
+    CHECK_LAUNCH(c);
+    if ((rc = dot_into(c, c->cg_r, c->cg_r, SC_BNORM2, 0))) return rc;
+    if ((rc = read_scalars(c))) return rc;
+    const double bnorm = std::sqrt(c->scal_host[SC_BNORM2]);
+    int it = 0, conv = 0;
+    double rr = bnorm * bnorm;
+    const LevelDev &L0 = c->lev64[0].d;
+    if (bnorm == 0.0) {
+        conv = 1;
+    } else {
+        if ((rc = precondition(c))) return rc;
+        hipLaunchKernelGGL(k_copy, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z);
+        CHECK_LAUNCH(c);
+        if ((rc = dot_into(c, c->cg_r, c->cg_z, SC_RZ, 0))) return rc;
+        const double tol = c->cs.poisson_rtol * bnorm;
+        for (it = 1; it <= c->cs.poisson_max_iter; ++it) {
+            prof_begin(c, IBMDEM_K_MATVEC);
+            hipLaunchKernelGGL(k_matvec_dot, dim3(RED_BLOCKS), dim3(RED_T), 0, s, L0, c->cg_p, c->cg_Ap, c->partial);
+            prof_end(c, IBMDEM_K_MATVEC, 2.0 * 8.0 * (double)n);
+            CHECK_LAUNCH(c);
+            hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_PAP, 1);
+            CHECK_LAUNCH(c);
+            prof_begin(c, IBMDEM_K_CG_UPDATE);
+            hipLaunchKernelGGL(k_cg_update, dim3(RED_BLOCKS), dim3(RED_T), 0, s, n, c->cg_x, c->cg_r, c->cg_p,
+                               c->cg_Ap, c->scal, c->partial);
+            prof_end(c, IBMDEM_K_CG_UPDATE, 6.0 * 8.0 * (double)n);
+            CHECK_LAUNCH(c);
+            hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_RR, 0);
+            CHECK_LAUNCH(c);
+            if ((rc = read_scalars(c))) return rc;
+            rr = c->scal_host[SC_RR];
+            if (!(rr == rr)) return set_error(c, IBMDEM_ERR_NAN, "non-finite residual in the Poisson solve");
+            if (std::sqrt(rr) <= tol) { conv = 1; break; }
+            if ((rc = precondition(c))) return rc;
+            if ((rc = dot_into(c, c->cg_r, c->cg_z, SC_RZ_NEW, 2))) return rc;
+            hipLaunchKernelGGL(k_cg_p, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z, c->scal);
+            CHECK_LAUNCH(c);
+        }
+        if (it > c->cs.poisson_max_iter) it = c->cs.poisson_max_iter;
+    }
+    hipLaunchKernelGGL(k_gauge, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_x, c->pcorr);
+    CHECK_LAUNCH(c);
+    if (info) {
+        info->iterations = it;
+        info->converged = conv;
+        info->rel_residual = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
+        info->rhs_norm = bnorm;
+    }
+    if (!conv) return set_error(c, IBMDEM_ERR_NOT_CONVERGED, "Poisson solve did not converge");
+    return IBMDEM_OK;
+}

@@ -1,0 +1,326 @@
+"""Seeded case generator: grids, spheres, markers and bodies.
+
+Lattice units (h = 1, rho_f = 1 unless stated). Every array is produced on
+the host from a named seed, so the GPU library and the CPU oracle receive
+byte-identical initial states.
+
+Formulas follow the reference's set-up code (ZJS_INIT_DEM.F90):
+  * markers sit on a sphere retracted by 0.3h: rs = R - 0.3h (:250);
+  * marker volume vol = pi_f*h/3/N*(12 rs^2 + h^2) with pi_f the binary32
+    literal 3.1415926536 the reference multiplies by (:312);
+  * marker contact radius radeff = 0.75h and collisionDist = (0.75h)*2
+    (:311, :413); marker mass = rho_p * vol (:314);
+  * body radius R, volume 4/3 pi R^3, mass rho_p V, inertia 0.4 m R^2 on the
+    diagonal of the body-frame tensor (:327-331);
+  * knCoeff = 4e4 and e = 1e-4 are the reference defaults (:73-74).
+The Fibonacci lattice replaces the reference's missing SPHERE_LP600.dat.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .abi import (BC_PERIODIC, BC_WALL, FP64, BodyArrays, Case, MarkerArrays)
+
+PI_F32 = float(np.float32(3.1415926536))
+SEED_BASE = 20260000
+
+# The reference's contact stiffness is in SI units at its own resolution:
+# dt = 5e-5 s (initialization.F90:259), dtDEM = dt/10 (ZJS_INIT_DEM.F90:76),
+# dx = 9e-3 m / 240 (main.F90:20). kn = knCoeff/(0.1 di) * meff
+# (ZJS_COLLISION_DEM.F90:46), so kn dtDEM^2 / meff = knCoeff dtDEM^2/(0.1 di)
+# is the dimensionless stiffness that sets the explicit integrator's
+# stability. lattice_kn keeps it equal at the lattice case's dtDEM and h.
+REF_DT_DEM = 5e-5 / 10
+REF_DX = 9e-3 / 240
+
+
+def lattice_kn(kn_ref: float, dt_dem: float, h: float = 1.0) -> float:
+    """knCoeff in lattice units with the reference's kn dtDEM^2 / meff."""
+    return kn_ref * (REF_DT_DEM / dt_dem) ** 2 * (h / REF_DX)
+
+
+def fibonacci_sphere(n: int) -> np.ndarray:
+    """n points on the unit sphere, (n, 3), golden-angle spiral."""
+    k = np.arange(n, dtype=np.float64)
+    z = 1.0 - (2.0 * k + 1.0) / n
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    phi = k * math.pi * (3.0 - math.sqrt(5.0))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def markers_for_diameter(d: float, h: float = 1.0) -> int:
+    """About pi D^2 / h^2 markers (one per h^2 of surface)."""
+    return int(round(math.pi * d * d / (h * h)))
+
+
+def stable_dt(h: float, nu: float, u_max: float) -> float:
+    """Forward-Euler/central-difference (FTCS) limits: CFL 0.5, diffusion
+    nu dt/h^2 <= 0.9/6, advective dt <= 0.9 * 2 nu / U^2."""
+    lims = [0.9 * h * h / (6.0 * nu)]
+    if u_max > 0:
+        lims += [0.5 * h / u_max, 0.9 * 2.0 * nu / (u_max * u_max)]
+    return min(lims)
+
+
+@dataclass
+class Sphere:
+    centre: tuple
+    diameter: float
+    rho: float
+    n_markers: int = 0
+    fixed: bool = False
+
+
+@dataclass
+class Setup:
+    name: str
+    case: Case
+    markers: MarkerArrays
+    bodies: BodyArrays
+    fields: tuple          # u, v, w, p  (nz, ny, nx)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def cells(self) -> int:
+        return int(self.case.nx) * int(self.case.ny) * int(self.case.nz)
+
+
+def build_case(nx, ny, nz, btype, *, h=1.0, visc=0.04, rhof=1.0, dt=None,
+               gravity=(0.0, 0.0, 0.0), kn_coeff=4e4, restitution=1e-4,
+               bulk_switch=(0, 0, 0), bulk_target=(0.0, 0.0, 0.0),
+               precision=FP64, rtol=1e-10, max_iter=200, n_mdf=5,
+               couple_interval=10, u_max=0.2, n_markers=0, n_bodies=0,
+               max_pairs=200, mg_pre=2, mg_post=2, mg_coarse=24) -> Case:
+    c = Case()
+    c.nx, c.ny, c.nz = nx, ny, nz
+    for s in range(6):
+        c.btype[s] = int(btype[s])
+    c.dx = c.dy = c.dz = float(h)
+    c.visc, c.rhof = float(visc), float(rhof)
+    c.dt = float(dt if dt is not None else stable_dt(h, visc, u_max))
+    for d in range(3):
+        c.cp[d] = 0.0
+        c.bulk_switch[d] = int(bulk_switch[d])
+        c.bulk_target[d] = float(bulk_target[d])
+        c.gravity[d] = float(gravity[d])
+    c.n_mdf = n_mdf
+    c.couple_interval = couple_interval
+    # kn_coeff is given in the reference's units (4e4 is its default)
+    c.kn_coeff = float(lattice_kn(kn_coeff, c.dt / couple_interval, h))
+    c.restitution = float(restitution)
+    c.collision_dist = (h * 0.75) * 2
+    c.collision_buffer = 0.1
+    c.eta_textbook = 0
+    c.n_markers = n_markers
+    c.n_bodies = n_bodies
+    c.max_pairs = max_pairs
+    c.precision = precision
+    c.poisson_rtol = rtol
+    c.poisson_max_iter = max_iter
+    c.mg_pre, c.mg_post, c.mg_coarse_sweeps = mg_pre, mg_post, mg_coarse
+    c.mg_max_levels = 0
+    c.stencil_tile = 0
+    return c
+
+
+def place_spheres(case: Case, spheres: list[Sphere], h: float = 1.0):
+    """Markers (body-contiguous) and bodies for a list of spheres."""
+    nb = len(spheres)
+    counts = [s.n_markers or markers_for_diameter(s.diameter, h) for s in spheres]
+    m = int(sum(counts))
+    mk = MarkerArrays.empty(m)
+    bd = BodyArrays.empty(nb)
+    L = np.array([case.nx * case.dx, case.ny * case.dy, case.nz * case.dz])
+    periodic = [case.btype[2 * d] == BC_PERIODIC for d in range(3)]
+    o = 0
+    for b, (s, n) in enumerate(zip(spheres, counts)):
+        R = 0.5 * s.diameter
+        rs = R - 0.3 * h
+        pts = fibonacci_sphere(n) * rs + np.asarray(s.centre, dtype=np.float64)
+        for d in range(3):
+            if periodic[d]:
+                pts[:, d] = np.mod(pts[:, d], L[d])
+        vol = PI_F32 * h / 3 / n * (12 * rs ** 2 + h ** 2)
+        mk.x[:, o:o + n] = pts.T
+        mk.vol[o:o + n] = vol
+        mk.mass[o:o + n] = s.rho * vol
+        mk.radeff[o:o + n] = h * 0.75
+        mk.body[o:o + n] = b
+        bvol = 4.0 / 3.0 * math.pi * R ** 3
+        mass = bvol * s.rho
+        bd.x[:, b] = s.centre
+        bd.mass[b] = mass
+        bd.rho[b] = s.rho
+        inert = 0.4 * mass * R * R
+        bd.inertia[b] = [inert, 0, 0, 0, inert, 0, 0, 0, inert]
+        bd.rot[b] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+        bd.rot_glo[b] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+        bd.fixed[b] = 1 if s.fixed else 0
+        o += n
+    case.n_markers = m
+    case.n_bodies = nb
+    return mk, bd
+
+
+def rsa_centres(rng, n, diameter, box, min_gap, periodic, margin=0.0,
+                max_tries=200000):
+    """Random sequential addition of n equal spheres (seeded)."""
+    box = np.asarray(box, dtype=np.float64)
+    lo = np.array([0.0 if p else margin for p in periodic])
+    hi = np.array([b if p else b - margin for b, p in zip(box, periodic)])
+    dmin = diameter + min_gap
+    out = np.zeros((n, 3))
+    k = 0
+    tries = 0
+    while k < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError(f"RSA placed only {k} of {n} spheres")
+        c = lo + rng.random(3) * (hi - lo)
+        if k:
+            dd = out[:k] - c
+            for d in range(3):
+                if periodic[d]:
+                    dd[:, d] -= box[d] * np.round(dd[:, d] / box[d])
+            if np.min(np.einsum("ij,ij->i", dd, dd)) < dmin * dmin:
+                continue
+        out[k] = c
+        k += 1
+    return out
+
+
+def zero_fields(case: Case):
+    shp = (case.nz, case.ny, case.nx)
+    return tuple(np.zeros(shp) for _ in range(4))
+
+
+# ---------------------------------------------------------------------------
+# The benchmark configurations
+# ---------------------------------------------------------------------------
+def config_settling(precision=FP64, rtol=1e-10) -> Setup:
+    """Plain FP64 single sphere settling in a closed box: 64x64x256, D=16h,
+    800 markers, rho_p/rho_f = 1.5, Re_g = 40 (nu = 0.04, g = 1.25e-3),
+    six no-slip walls."""
+    nu = 0.04
+    D, rr = 16.0, 1.5
+    g = (0.1 ** 2) / ((rr - 1.0) * D)   # U_ref = sqrt((rr-1) g D) = 0.1
+    case = build_case(64, 64, 256, [BC_WALL] * 6, visc=nu,
+                      gravity=(0.0, 0.0, -g), u_max=0.2, precision=precision,
+                      rtol=rtol)
+    mk, bd = place_spheres(case, [Sphere((32.0, 32.0, 192.0), D, rr, 800)])
+    return Setup("sphere-settling-64x64x256", case, mk, bd, zero_fields(case),
+                 {"steps": 1000, "re_g": 40.0, "g": g})
+
+
+def config_suspension(n_spheres=2000, n=256, precision=FP64, rtol=1e-10,
+                      seed=SEED_BASE + 2) -> Setup:
+    """Plain FP64 triply periodic suspension: n^3 grid, n_spheres spheres of
+    D = 12h (452 markers) placed by seeded RSA (gap >= 2h), rho_p/rho_f = 2,
+    knCoeff = 1e5, e = 0.9, gravity by the Re_g = 40 recipe
+    (U_ref = 0.1, nu = U_ref D / 40), zero-net-flux bulk forcing."""
+    D, rr = 12.0, 2.0
+    u_ref = 0.1
+    nu = u_ref * D / 40.0
+    g = u_ref ** 2 / ((rr - 1.0) * D)
+    case = build_case(n, n, n, [BC_PERIODIC] * 6, visc=nu,
+                      gravity=(0.0, 0.0, -g), kn_coeff=1e5, restitution=0.9,
+                      bulk_switch=(1, 1, 1), u_max=2 * u_ref,
+                      precision=precision, rtol=rtol)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cen = rsa_centres(rng, n_spheres, D, (n, n, n), 2.0, [True] * 3)
+    mk, bd = place_spheres(case, [Sphere(tuple(c), D, rr, 452) for c in cen])
+    phi = n_spheres * math.pi * D ** 3 / 6 / n ** 3
+    return Setup(f"suspension-{n}^3-{n_spheres}x{int(D)}h", case, mk, bd,
+                 zero_fields(case), {"steps": 500, "solid_fraction": phi,
+                                     "seed": seed})
+
+
+# ---------------------------------------------------------------------------
+# Small cases for parity tests
+# ---------------------------------------------------------------------------
+def small_periodic(n=(32, 24, 28), n_spheres=6, D=8.0, rho=2.0, seed=SEED_BASE + 11,
+                   precision=FP64, rtol=1e-12, contact=True, gap=0.9,
+                   velocities=True) -> Setup:
+    """Triply periodic box with a cluster of spheres in near contact (marker
+    shells closer than the 1.65h candidate distance), some straddling the
+    periodic faces, random initial body and fluid velocities."""
+    nx, ny, nz = n
+    rng = np.random.Generator(np.random.PCG64(seed))
+    case = build_case(nx, ny, nz, [BC_PERIODIC] * 6, visc=0.05,
+                      gravity=(0.0, 0.0, -1e-3), kn_coeff=1e5, restitution=0.9,
+                      bulk_switch=(1, 0, 1), bulk_target=(0.01, 0.0, 0.0),
+                      u_max=0.3, precision=precision, rtol=rtol)
+    spheres = []
+    # a 3 x 2 grid at pitch D + gap (marker shells 2*0.3h + gap apart, inside
+    # the 1.65h candidate distance), straddling the x and y periodic faces
+    pitch = D + (gap if contact else 4.0)
+    start = np.array([nx - 0.5 * D + 0.37, 0.5 * ny + 0.21, 0.5 * nz - 0.13])
+    for k in range(n_spheres):
+        off = np.array([(k % 3) * pitch, (k // 3) * pitch, 0.35 * math.sin(1.3 * k)])
+        c = np.mod(start + off, [nx, ny, nz])
+        spheres.append(Sphere(tuple(c), D, rho, markers_for_diameter(D)))
+    mk, bd = place_spheres(case, spheres)
+    if velocities:
+        bd.v[:] = rng.normal(0.0, 0.02, size=bd.v.shape)
+        bd.vold[:] = bd.v
+        bd.omega[:] = rng.normal(0.0, 0.01, size=bd.omega.shape)
+        for b in range(bd.mass.size):
+            inert = bd.inertia[b].reshape(3, 3)
+            bd.angmom[:, b] = inert @ bd.omega[:, b]
+        bd.angmom_old[:] = bd.angmom
+        # marker velocities consistent with the rigid motion
+        for i in range(mk.body.size):
+            b = mk.body[i]
+            r = mk.x[:, i] - bd.x[:, b]
+            L = np.array([nx, ny, nz], dtype=np.float64)
+            r -= L * np.round(r / L)
+            mk.v[:, i] = bd.v[:, b] + np.cross(bd.omega[:, b], r)
+    u, v, w, p = zero_fields(case)
+    if velocities:
+        z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        u[:] = 0.02 * np.sin(2 * np.pi * y / ny) * np.cos(2 * np.pi * z / nz)
+        v[:] = 0.015 * np.cos(2 * np.pi * x / nx) * np.sin(4 * np.pi * z / nz)
+        w[:] = 0.01 * np.sin(2 * np.pi * x / nx + 0.3) * np.cos(2 * np.pi * y / ny)
+        p[:] = 0.001 * rng.normal(size=p.shape)
+    return Setup(f"periodic-{nx}x{ny}x{nz}-{n_spheres}", case, mk, bd, (u, v, w, p),
+                 {"seed": seed})
+
+
+def small_walls(n=(24, 20, 32), D=8.0, rho=1.5, seed=SEED_BASE + 12,
+                precision=FP64, rtol=1e-12, periodic_x=False) -> Setup:
+    """Closed box (or x-periodic channel) with two spheres: one touching the
+    bottom wall (wall contact, wall-mirror markers), one above it in near
+    contact, both off-centre so every wall sees mirror images."""
+    nx, ny, nz = n
+    rng = np.random.Generator(np.random.PCG64(seed))
+    bt = [BC_WALL] * 6
+    if periodic_x:
+        bt[0] = bt[1] = BC_PERIODIC
+    case = build_case(nx, ny, nz, bt, visc=0.04, gravity=(0.0, 0.0, -2e-3),
+                      u_max=0.3, precision=precision, rtol=rtol)
+    R = 0.5 * D
+    c1 = (R + 0.45, R + 0.25, R - 0.3 * 1.0 + 0.05)
+    c2 = (R + 1.2, R + 0.9, c1[2] + D + 0.95)
+    mk, bd = place_spheres(case, [Sphere(c1, D, rho), Sphere(c2, D, rho)])
+    bd.v[:] = rng.normal(0.0, 0.02, size=bd.v.shape)
+    bd.v[2, 0] = -0.03
+    bd.vold[:] = bd.v
+    for i in range(mk.body.size):
+        mk.v[:, i] = bd.v[:, mk.body[i]]
+    u, v, w, p = zero_fields(case)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    u[:] = 0.01 * np.sin(np.pi * (y + 0.5) / ny) * np.sin(np.pi * (z + 0.5) / nz)
+    v[:] = 0.01 * np.sin(np.pi * (x + 0.5) / nx) * np.sin(np.pi * (z + 0.5) / nz)
+    w[:] = 0.01 * np.sin(np.pi * (x + 0.5) / nx) * np.sin(np.pi * (y + 0.5) / ny)
+    p[:] = 0.001 * rng.normal(size=p.shape)
+    return Setup(f"walls-{nx}x{ny}x{nz}", case, mk, bd, (u, v, w, p), {"seed": seed})
+
+
+def load(sim, setup: Setup) -> None:
+    """Upload a setup's initial state into a Simulation."""
+    sim.upload_bodies(setup.bodies)
+    sim.upload_markers(setup.markers)
+    sim.upload_fields(*setup.fields)

@@ -1,0 +1,232 @@
+/*
+ * ibmdem.h — C ABI of the MI355X coupled IBM-DEM time step.
+ *
+ * One context owns one case on one GPU: the staggered (MAC) velocity field,
+ * the pressure, the Lagrangian markers ("parcels") and the rigid bodies
+ * ("multi-spheres"). Every entry point returns an int status (IBMDEM_OK on
+ * success) and never throws, aborts or STOPs across the boundary. Host buffers
+ * passed in are borrowed for the duration of the call only. Calls are ordered
+ * on the context's HIP stream; calls that return host data synchronize.
+ *
+ * Index conventions follow the reference (vars.F90:130-153):
+ *   - cell (i,j,k), 1-based, i in 1..nx fastest; fields cross the boundary
+ *     as interior-only arrays of nx*ny*nz doubles, linear index
+ *     (k-1)*nx*ny + (j-1)*nx + (i-1);
+ *   - u(i,j,k) lives on the x-face at x=(i-1)dx, v and w likewise on their
+ *     faces, P at the cell centre x=(i-1/2)dx (initialization.F90:251,
+ *     fluid.F90:80,97, ZJS_INTERPOLATE_DEM.F90:12-14);
+ *   - boundary sides 0..5 = reference bType(1..6) = x-, x+, y-, y+, z-, z+
+ *     (initialization.F90:341-377).
+ * Marker and body vectors are component-major: x[0..n) then y[0..n) then
+ * z[0..n). 3x3 matrices are row-major per body, as the reference's
+ * mat(1:9) (couplesolverCylinder.F90:87-95): 9 consecutive doubles per body.
+ */
+#ifndef IBMDEM_H
+#define IBMDEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define IBMDEM_OK                 0
+#define IBMDEM_ERR_HIP            1  /* a HIP runtime call failed            */
+#define IBMDEM_ERR_ARG            2  /* invalid argument or case             */
+#define IBMDEM_ERR_CAPACITY       3  /* pair list or bin capacity exceeded   */
+#define IBMDEM_ERR_OUTSIDE        4  /* a marker left the domain             */
+#define IBMDEM_ERR_NOT_CONVERGED  5  /* Poisson solve hit max iterations     */
+#define IBMDEM_ERR_NAN            6  /* non-finite value detected            */
+#define IBMDEM_ERR_ALLOC          7  /* device allocation failed             */
+#define IBMDEM_ERR_UNSUPPORTED    8  /* feature not available in this form   */
+
+/* ---- boundary types (reference bType: 0 periodic, 1 solid wall) -------- */
+#define IBMDEM_BC_PERIODIC 0
+#define IBMDEM_BC_WALL     1
+
+/* ---- precision forms ---------------------------------------------------- */
+#define IBMDEM_FP64  0   /* everything in FP64                                   */
+#define IBMDEM_MIXED 1   /* FP32 stencil predictor and FP32 multigrid inside FP64
+                            CG (residual, inner products, projection, DEM in FP64) */
+
+/* ---- case description --------------------------------------------------- */
+typedef struct ibmdem_case {
+    int32_t nx, ny, nz;          /* interior cells                          */
+    int32_t btype[6];            /* IBMDEM_BC_* per side                    */
+    double  dx, dy, dz;          /* grid spacing                            */
+    double  visc, rhof, dt;      /* nu, rho_f, fluid time step              */
+    double  cp[3];               /* constant pressure gradient CPx,CPy,CPz  */
+    int32_t bulk_switch[3];      /* DefAveBulk{U,V,W}Switch                 */
+    double  bulk_target[3];      /* bulkU, bulkV, bulkW                     */
+    double  gravity[3];          /* body gravity (reference: (0,0,-9.81))   */
+    int32_t n_mdf;               /* multi-direct-forcing iterations Ns (5)  */
+    int32_t couple_interval;     /* DEM sub-steps per fluid step (10)       */
+    double  kn_coeff;            /* knCoeff (4e4)                           */
+    double  restitution;         /* ResistCoeff e (1e-4)                    */
+    double  collision_dist;      /* collisionDist = (0.75 dx)*2             */
+    double  collision_buffer;    /* collisionBuffer = 0.1                   */
+    int32_t eta_textbook;        /* 0: reference eta, 1: -2ln e/sqrt(pi^2+ln^2 e) */
+    int32_t n_markers;           /* M                                       */
+    int32_t n_bodies;            /* B                                       */
+    int32_t max_pairs;           /* maxCollisionNo (200)                    */
+    int32_t precision;           /* IBMDEM_FP64 / IBMDEM_MIXED              */
+    double  poisson_rtol;        /* ||b-Ax||_2 <= rtol ||b||_2              */
+    int32_t poisson_max_iter;    /* CG iteration cap                        */
+    int32_t mg_pre, mg_post;     /* red-black GS sweeps (0 -> 2,2)          */
+    int32_t mg_coarse_sweeps;    /* sweeps on the coarsest level (0 -> 24)  */
+    int32_t mg_max_levels;       /* 0 -> as many as the grid allows         */
+    int32_t stencil_tile;        /* predictor tile choice (0 -> default)    */
+    int32_t reserved[7];
+} ibmdem_case;
+
+/* Markers (reference type Parcel, ZJS_VARS.F90:6-12), SoA. Pointers may be
+ * NULL in a download to skip that array; every array is required in an
+ * upload except fdrag/fcol (NULL -> zero). */
+typedef struct ibmdem_markers {
+    double  *x;       /* 3*M positions                                    */
+    double  *v;       /* 3*M velocities                                   */
+    double  *fdrag;   /* 3*M IBM force density accumulator (fdrag)        */
+    double  *fcol;    /* 3*M contact force                                */
+    double  *vol;     /* M   marker volume                                */
+    double  *mass;    /* M   marker mass m                                */
+    double  *radeff;  /* M   contact radius radeff                        */
+    int32_t *body;    /* M   0-based owning body (MSIdx-1); markers of one
+                               body must be contiguous and bodies ascending */
+} ibmdem_markers;
+
+/* Bodies (reference type MultiSphere, ZJS_VARS.F90:97-102), SoA. */
+typedef struct ibmdem_bodies {
+    double  *x, *v, *vold;           /* 3*B                                */
+    double  *omega, *angmom, *angmom_old; /* 3*B                           */
+    double  *rot;                    /* 9*B incremental rotation rotMat    */
+    double  *rot_glo;                /* 9*B accumulated rotation rotMatGlo */
+    double  *inertia;                /* 9*B body-frame inertia tensor      */
+    double  *mass, *rho;             /* B                                  */
+    int32_t *fixed;                  /* B   1 -> body does not move        */
+    /* diagnostics (download only; may be NULL) */
+    double  *fdrag, *fmom, *fcol, *fmomcol, *fhydro, *fmomhydro; /* 3*B    */
+} ibmdem_bodies;
+
+typedef struct ibmdem_solve_info {
+    int32_t iterations;     /* CG iterations performed                    */
+    int32_t converged;      /* 1 if ||r|| <= rtol ||b||                   */
+    double  rel_residual;   /* final ||r||_2 / ||b||_2 (recurrence)        */
+    double  rhs_norm;       /* ||b||_2                                    */
+} ibmdem_solve_info;
+
+typedef struct ibmdem_ctx ibmdem_ctx;
+
+/* ---- lifetime and state (replace allocate.F90, AllocateDEM, the
+ *      Fluid/InitializationDEM state set-up and the output copies) ------- */
+int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out);
+int ibmdem_destroy(ibmdem_ctx *ctx);
+const char *ibmdem_last_error(const ibmdem_ctx *ctx);
+int ibmdem_sync(ibmdem_ctx *ctx);
+/* u, v, w, p: nx*ny*nz each; NULL skips. Upload applies the boundary
+ * rules (wall-normal face velocities forced to zero, fluid.F90:243-248). */
+int ibmdem_upload_fields(ibmdem_ctx *ctx, const double *u, const double *v,
+                         const double *w, const double *p);
+int ibmdem_download_fields(ibmdem_ctx *ctx, double *u, double *v, double *w,
+                           double *p);
+/* IBM momentum source f = (fx,fy,fz) on the interior (fluid.F90 fx,fy,fz). */
+int ibmdem_download_force(ibmdem_ctx *ctx, double *fx, double *fy, double *fz);
+/* Pressure correction P' of the last solve and the Poisson RHS velDiv. */
+int ibmdem_download_poisson(ibmdem_ctx *ctx, double *pcorr, double *rhs);
+int ibmdem_upload_poisson_rhs(ibmdem_ctx *ctx, const double *rhs);
+int ibmdem_upload_markers(ibmdem_ctx *ctx, const ibmdem_markers *m);
+int ibmdem_download_markers(ibmdem_ctx *ctx, ibmdem_markers *m);
+int ibmdem_upload_bodies(ibmdem_ctx *ctx, const ibmdem_bodies *b);
+int ibmdem_download_bodies(ibmdem_ctx *ctx, ibmdem_bodies *b);
+
+/* ---- the coupled step, operator by operator (fluid.F90:252-327) -------- */
+/* Start of FirstOrderForwardEuler (fluid.F90:258-263): fdrag = 0, f = 0,
+ * and the per-step IBM tables (delta weights, spread footprint). */
+int ibmdem_begin_step(ibmdem_ctx *ctx);
+/* UpdateFluidVelocityOnLagrangianPoints (ZJS_VOLUMEFRACTION_DEM.F90:2-36)
+ * with the weight lists of ZJS_UPDATE_LIST_DEM.F90:323-416 folded in. */
+int ibmdem_ibm_interpolate(ibmdem_ctx *ctx);
+/* UpdateParcelInfoInGhostRegion mirror images (ZJS_MPI_DEM.F90:222-269) +
+ * UpdateMomentumSourceTerm (ZJS_VOLUMEFRACTION_DEM.F90:38-67) +
+ * VelocityPostGuess (fluid.F90:135-153) + FillGhost/Boundary. */
+int ibmdem_ibm_spread(ibmdem_ctx *ctx);
+/* VelocityGuess(a,b,c) (fluid.F90:107-133), fused ConvectionTensor,
+ * ViscousTensor and PressureGradient (fluid.F90:2-87). */
+int ibmdem_fluid_predictor(ibmdem_ctx *ctx, double a, double b, double c);
+/* ModifyBulkVelocity (fluid.F90:155-197). */
+int ibmdem_bulk_forcing(ibmdem_ctx *ctx);
+/* VelocityDivergence(c) (fluid.F90:89-105). */
+int ibmdem_divergence(ibmdem_ctx *ctx, double c);
+/* SetPressureVector + PoissonSolver + GetLocalArray (poisson.F90:107-186):
+ * multigrid-preconditioned CG replacing PETSc KSPSolve. info may be NULL. */
+int ibmdem_poisson_solve(ibmdem_ctx *ctx, ibmdem_solve_info *info);
+/* PressureCorrection + VelocityCorrection(c) (fluid.F90:199-238). */
+int ibmdem_project(ibmdem_ctx *ctx, double c);
+/* FillGhostRegionForFluid + Boundary (boundary.F90:122-247,
+ * fluid.F90:240-250): boundary rules are applied inside every kernel, so
+ * this entry point only validates the context. */
+int ibmdem_apply_bc(ibmdem_ctx *ctx);
+
+/* ---- DEM and rigid bodies (couplesolverCylinder.F90) -------------------- */
+/* CloudEvolutionMS:194-211: fhydro/fmomHydro from (v-vold),(L-Lold),
+ * AssembleForceMS, then fhydro += fdrag/m, fmomHydro += fmom. */
+int ibmdem_body_hydro(ibmdem_ctx *ctx);
+/* CalculateCollisionForce + CalculateSolidBoundaryCollisionForce
+ * (ZJS_COLLISION_DEM.F90:21-148) over the pair list of
+ * RebuildParcelCollisionListDEM (ZJS_UPDATE_LIST_DEM.F90:248-286). */
+int ibmdem_dem_contact(ibmdem_ctx *ctx);
+/* AssembleLocalForceMS (ZJS_MS_DEM.F90:494-519). */
+int ibmdem_body_assemble(ibmdem_ctx *ctx);
+/* EvolutionMS (couplesolverCylinder.F90:2-42). */
+int ibmdem_body_integrate(ibmdem_ctx *ctx);
+/* ResetVelocityAndPositionLagrangianPoints (couplesolverCylinder.F90:44-74)
+ * + AdjustParcelPosition (ZJS_EVOLUTION_DEM.F90:121-150). */
+int ibmdem_markers_follow(ibmdem_ctx *ctx);
+/* RebuildAllListOrNot (ZJS_EVOLUTION_DEM.F90:34-41): periodic migration
+ * (CompressDEM), cell binning (PutParcelsIntoCellsDEM). */
+int ibmdem_dem_rebuild_lists(ibmdem_ctx *ctx);
+/* One CloudEvolutionStepMS (couplesolverCylinder.F90:99-187). */
+int ibmdem_dem_substep(ibmdem_ctx *ctx);
+/* CloudEvolutionMS (couplesolverCylinder.F90:189-217). */
+int ibmdem_cloud_evolution(ibmdem_ctx *ctx);
+
+/* ---- whole step ---------------------------------------------------------- */
+/* FirstOrderForwardEuler (fluid.F90:252-327) with CloudEvolutionMS. */
+int ibmdem_step(ibmdem_ctx *ctx, ibmdem_solve_info *info);
+/* Run n steps; info (may be NULL) receives the last solve. */
+int ibmdem_run(ibmdem_ctx *ctx, int32_t n_steps, ibmdem_solve_info *info);
+
+/* ---- integer lists for parity / diagnostics ------------------------------ */
+/* Per-marker cell as the reference's PutParcelsIntoCellsDEM computes it
+ * (ZJS_UPDATE_LIST_DEM.F90:41-43): 1-based (i,j,k), 3*M int32,
+ * component-major. */
+int ibmdem_get_cells(ibmdem_ctx *ctx, int32_t *cell_ijk);
+/* Canonical contact candidate list: for marker m (id order), count[m]
+ * partners; partner[m*cap + q] = partner id, shift[m*cap + q] = periodic
+ * image code (sx+1) + 3(sy+1) + 9(sz+1), s in {-1,0,1}; order = the
+ * reference scan order (125 cells kk,jj,ii; in-bin ascending id). */
+int ibmdem_get_pairs(ibmdem_ctx *ctx, int32_t cap, int32_t *count,
+                     int32_t *partner, int32_t *shift);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Kernel ids for ibmdem_profile_* (HIP events around every launch of that
+ * kernel on the context stream). */
+#define IBMDEM_K_PREDICTOR   1
+#define IBMDEM_K_MG_SMOOTH   2
+#define IBMDEM_K_MATVEC      3
+#define IBMDEM_K_SPREAD      4
+#define IBMDEM_K_INTERP      5
+#define IBMDEM_K_CONTACT     6
+#define IBMDEM_K_PROJECT     7
+#define IBMDEM_K_DIVERGENCE  8
+#define IBMDEM_K_CG_UPDATE   9
+int ibmdem_profile_kernel(ibmdem_ctx *ctx, int32_t kernel_id);
+int ibmdem_profile_read(ibmdem_ctx *ctx, double *total_ms, int64_t *launches,
+                        double *bytes_per_launch);
+/* Device bytes the context holds. */
+int ibmdem_memory_bytes(const ibmdem_ctx *ctx, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IBMDEM_H */
