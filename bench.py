@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark of the coupled IBM-DEM time step on one MI355X.
+
+Workload (the plain-FP64 GPU-vs-CPU configuration): triply periodic 256^3
+grid, 2000 spheres of D = 12h with 452 markers each (904,000 markers),
+seeded random sequential addition (solid fraction 10.8%), rho_p/rho_f = 2,
+knCoeff 1e5 and e = 0.9 (reference units), Re_g = 40 gravity and zero-net-flux
+bulk forcing. One "step" is one FirstOrderForwardEuler + CloudEvolutionMS:
+5 multi-direct-forcing iterations, predictor, bulk forcing, divergence, MG-PCG
+Poisson solve to rtol 1e-10, projection and 10 DEM sub-steps.
+
+The FP64 form is the headline (value = steps/s); the mixed-precision form
+(FP32 predictor and FP32 multigrid inside FP64 CG) is timed on the same
+state and reported beside it. The dominant kernel (the level-0 red-black
+smoother half-sweep) is timed live with HIP events on the context stream;
+its algorithmic bytes are 1.5 * s * cells per launch (half the cells read
+their right-hand side and the other colour, and are written). The CPU
+baseline is the oracle (one core) timed on a bounded sample.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--precision fp64|mixed|both]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ibm-dem_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--n", type=int, default=256, help="grid edge (256 is the workload)")
+    ap.add_argument("--spheres", type=int, default=2000)
+    return ap.parse_args()
+
+
+def timed_run(setup, precision, steps, warmup, profile_kernel):
+    import ibmdem
+    from ibmdem import cases
+    from ibmdem.abi import K_MG_SMOOTH
+
+    case = setup.case
+    case.precision = precision
+    sim = ibmdem.simulation(case)
+    cases.load(sim, setup)
+    iters = []
+    for _ in range(warmup):
+        iters.append(sim.step().iterations)
+    sim.sync()
+    if profile_kernel:
+        sim.profile_kernel(K_MG_SMOOTH)
+    sim.sync()
+    t0 = time.perf_counter()
+    info = None
+    for _ in range(steps):
+        info = sim.step()
+        iters.append(info.iterations)
+    sim.sync()
+    t1 = time.perf_counter()
+    prof = sim.profile_read() if profile_kernel else None
+    state = (sim.download_fields(), sim.download_markers(), sim.download_bodies())
+    mem = sim.memory_bytes()
+    sim.close()
+    return (t1 - t0), iters, info, prof, state, mem
+
+
+def cpu_baseline():
+    """The oracle (one core) on a bounded sample: one coupled step of a
+    128^3 sub-box holding 250 spheres (the workload's solid fraction), taken
+    after one warm-up step, reported in cell-steps per second and scaled to
+    steps/s of the 256^3 workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from ibmdem import cases
+
+    setup = cases.config_suspension(n_spheres=250, n=128)
+    sim = oracle.simulation(setup.case)
+    cases.load(sim, setup)
+    sim.step()               # warm-up (the flow starts at rest)
+    t0 = time.perf_counter()
+    info = sim.step()
+    dt = time.perf_counter() - t0
+    sim.close()
+    cells = setup.cells
+    gcell = cells / dt / 1e9
+    return {
+        "value": gcell * 1e9 / 256 ** 3,
+        "unit": "steps/s (256^3 workload, scaled by cell count)",
+        "gcell_per_s": gcell,
+        "cores": 1,
+        "kind": "port",
+        "sample": f"1 step of a 128^3 periodic sub-box with 250 spheres (same solid fraction), "
+                  f"{dt:.1f} s, {info.iterations} CG iterations",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design, not a run)"}))
+        sys.exit(2)
+    import numpy as np
+    from ibmdem import cases
+    from ibmdem.abi import FP64, MIXED
+
+    setup = cases.config_suspension(n_spheres=args.spheres, n=args.n)
+    cells = setup.cells
+    out = {}
+    results = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        if args.precision not in (name, "both"):
+            continue
+        secs, iters, info, prof, state, mem = timed_run(setup, prec, args.steps, args.warmup, True)
+        results[name] = dict(secs=secs, iters=iters, prof=prof, state=state, mem=mem)
+        print(f"[bench] {name}: {args.steps} steps in {secs:.3f} s, "
+              f"CG iterations {iters}", file=sys.stderr, flush=True)
+
+    head = "fp64" if "fp64" in results else "mixed"
+    r = results[head]
+    sps = args.steps / r["secs"]
+    total_ms, launches, bpl = r["prof"]
+    avg_ms = total_ms / max(launches, 1)
+    achieved = bpl / (avg_ms * 1e-3) / 1e9 if launches else None
+    line = {
+        "metric": "steps/s",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["secs"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f64+f32",
+        "data": "synthetic: seeded RSA sphere suspension (seed 20260002), fluid at rest at t=0",
+        "config": {
+            "workload": f"periodic suspension {args.n}^3, {args.spheres} spheres D=12h, "
+                        f"{setup.case.n_markers} markers, FP64, MG-PCG rtol 1e-10",
+            "grid": [args.n] * 3,
+            "spheres": args.spheres,
+            "markers": int(setup.case.n_markers),
+            "dt": setup.case.dt,
+        },
+        "gcell_per_s": cells * sps / 1e9,
+        "cg_iterations_mean": float(np.mean(r["iters"][args.warmup:])) if len(r["iters"]) > args.warmup else None,
+        "device_memory_gb": r["mem"] / 1e9,
+        "roofline": {
+            "kernel": "k_rb_half (level-0 red-black Gauss-Seidel half-sweep)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "avg_launch_us": avg_ms * 1e3,
+            "launches": launches,
+            "algorithmic_bytes_per_launch": bpl,
+        },
+    }
+    if "mixed" in results and head == "fp64":
+        m = results["mixed"]
+        msps = args.steps / m["secs"]
+        uf, mf, bf = results["fp64"]["state"]
+        um, mm, bm = m["state"]
+        def rel(a, b):
+            return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+        mt, ml, mb = m["prof"]
+        line["mixed"] = {
+            "steps_per_s": msps,
+            "gcell_per_s": cells * msps / 1e9,
+            "speedup_vs_fp64": msps / sps,
+            "cg_iterations_mean": float(np.mean(m["iters"][args.warmup:])),
+            "smoother_avg_launch_us": 1e3 * mt / max(ml, 1),
+            "error_vs_fp64_after_run": {
+                "u": rel(um[0], uf[0]), "v": rel(um[1], uf[1]), "w": rel(um[2], uf[2]),
+                "p": rel(um[3], uf[3]), "marker_x": rel(mm.x, mf.x), "body_x": rel(bm.x, bf.x),
+            },
+        }
+    if not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the baseline must not hide the GPU line
+            line["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

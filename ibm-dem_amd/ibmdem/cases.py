@@ -303,7 +303,7 @@ def small_walls(n=(24, 20, 32), D=8.0, rho=1.5, seed=SEED_BASE + 12,
                       u_max=0.3, precision=precision, rtol=rtol)
     R = 0.5 * D
     c1 = (R + 0.45, R + 0.25, R - 0.3 * 1.0 + 0.05)
-    c2 = (R + 1.2, R + 0.9, c1[2] + D + 0.95)
+    c2 = (R + 1.2, R + 0.9, c1[2] + D + 0.6)
     mk, bd = place_spheres(case, [Sphere(c1, D, rho), Sphere(c2, D, rho)])
     bd.v[:] = rng.normal(0.0, 0.02, size=bd.v.shape)
     bd.v[2, 0] = -0.03
