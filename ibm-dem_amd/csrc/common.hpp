@@ -171,6 +171,8 @@ struct ibmdem_ctx {
     int max_partial = 0;
     double *scal = nullptr;      // device scalars (see SC_* in kernels)
     double *scal_host = nullptr; // pinned mirror
+    double *conv_host = nullptr; // pinned, one convergence flag per CG iteration
+    hipEvent_t iter_ev[2] = {};   // end of the last two queued CG iterations
     int *flags = nullptr;        // device error flags
     int *flags_host = nullptr;
     bool bulk_pending = false;   // bulk shift computed but not yet applied
@@ -180,17 +182,18 @@ struct ibmdem_ctx {
     double *mvol = nullptr, *mmass = nullptr, *mrad = nullptr;
     int *mbody = nullptr;
     int *mcell = nullptr;        // 0-based linear interior cell
-    double *wts = nullptr;       // 24 one-dimensional delta weights per marker
+    double *wts = nullptr;       // 24 one-dimensional delta weights per marker (marker-major)
+    double *fv = nullptr;        // fdrag*vol per marker, 4 per marker
     // bins over interior cells
     int *bin_count = nullptr, *bin_start = nullptr, *bin_items = nullptr, *bin_tmp = nullptr;
     int *scan_blocks = nullptr;
     int scan_nblocks = 0;
-    // IBM spread footprint (per step)
-    unsigned char *act_flag = nullptr;
+    // IBM spread footprint (per step): active 4x4x4 tiles
+    unsigned char *act_flag = nullptr;   // one flag per tile
     int *act_list = nullptr;
     int nact = 0;
-    long act_cap = 0;
-    double *fact = nullptr;      // f of the active cells, 3*act_cap
+    long act_cap = 0;                    // tiles
+    double *fact = nullptr;              // f of the tiles' cells, 3 * 64 * act_cap
     bool first_mdf = true;
     // bodies
     int *body_start = nullptr;   // B+1
@@ -199,6 +202,9 @@ struct ibmdem_ctx {
     int *bfixed = nullptr;
     double *bfdrag = nullptr, *bfmom = nullptr, *bfcol = nullptr, *bfmomcol = nullptr;
     double *bfhydro = nullptr, *bfmomhydro = nullptr;
+    double *brmax = nullptr;     // max |x_m - X_b| over the body's markers
+    int *bnear = nullptr;        // 1 if another body is within contact reach
+    bool rmax_dirty = true;
     // pair list for parity (allocated on demand)
     int *pair_count = nullptr, *pair_partner = nullptr, *pair_shift = nullptr;
     int pair_cap = 0;
@@ -213,6 +219,9 @@ enum {
     SC_BULKSUM = 3,   // 3: sums of u,v,w
     SC_RZ = 6, SC_PAP = 7, SC_RR = 8, SC_ALPHA = 9, SC_BETA = 10, SC_BNORM2 = 11,
     SC_RZ_NEW = 12, SC_PIN = 13, SC_X0 = 14,
+    SC_CONV = 15,     // 1 once ||r|| <= tol: every later CG kernel returns at once
+    SC_TOL = 16,      // rtol * ||b||
+    SC_ITER = 17,     // iteration at which SC_CONV was set
     SC_COUNT = 32
 };
 enum { FL_CAPACITY = 0, FL_OUTSIDE = 1, FL_NAN = 2, FL_COUNT = 8 };
@@ -268,6 +277,7 @@ int ibm_begin_step(ibmdem_ctx *c);
 int ibm_interpolate(ibmdem_ctx *c);
 int ibm_spread(ibmdem_ctx *c);
 int ibm_download_force(ibmdem_ctx *c, double *fx, double *fy, double *fz);
+long ibm_tile_count(const GridDev &g);
 // dem.hip
 int dem_setup(ibmdem_ctx *c);
 int dem_rebuild(ibmdem_ctx *c);

@@ -73,23 +73,28 @@ int alloc_all(ibmdem_ctx *c) {
     A(c->flags, FL_COUNT + 8);
     HIPX(c, hipHostMalloc((void **)&c->scal_host, SC_COUNT * sizeof(double), hipHostMallocDefault));
     HIPX(c, hipHostMalloc((void **)&c->flags_host, (FL_COUNT + 8) * sizeof(int), hipHostMallocDefault));
+    HIPX(c, hipEventCreateWithFlags(&c->iter_ev[0], hipEventDisableTiming));
+    HIPX(c, hipEventCreateWithFlags(&c->iter_ev[1], hipEventDisableTiming));
+    HIPX(c, hipHostMalloc((void **)&c->conv_host, (c->cs.poisson_max_iter + 2) * sizeof(double), hipHostMallocDefault));
     const int M = c->M, B = c->B;
     A(c->mx, 3L * M); A(c->mv, 3L * M); A(c->fdrag, 3L * M); A(c->fcol, 3L * M);
     A(c->mvol, M); A(c->mmass, M); A(c->mrad, M); A(c->mbody, M); A(c->mcell, M);
-    A(c->wts, 24L * M);
+    A(c->wts, 24L * M); A(c->fv, 4L * M);
     A(c->bin_count, n); A(c->bin_start, n + 1); A(c->bin_items, M); A(c->bin_tmp, M);
     c->scan_nblocks = div_up(n, 4096) + 2;
     A(c->scan_blocks, c->scan_nblocks);
-    A(c->act_flag, n);
-    c->act_cap = std::min<long>(n, 64L * M + 64);
+    const long ntiles = ibm_tile_count(c->g);
+    A(c->act_flag, ntiles);
+    c->act_cap = std::min<long>(ntiles, 8L * M + 8);   // a marker reaches at most 2x2x2 tiles
     A(c->act_list, c->act_cap);
-    A(c->fact, 3 * c->act_cap);
+    A(c->fact, 3 * 64 * c->act_cap);
     A(c->body_start, B + 1);
     A(c->bx, 3L * B); A(c->bv, 3L * B); A(c->bvold, 3L * B); A(c->bom, 3L * B); A(c->bL, 3L * B);
     A(c->bLold, 3L * B); A(c->brot, 9L * B); A(c->brotg, 9L * B); A(c->bI, 9L * B); A(c->bm, B);
     A(c->brho, B); A(c->bfixed, B);
     A(c->bfdrag, 3L * B); A(c->bfmom, 3L * B); A(c->bfcol, 3L * B); A(c->bfmomcol, 3L * B);
     A(c->bfhydro, 3L * B); A(c->bfmomhydro, 3L * B);
+    A(c->brmax, B); A(c->bnear, B);
 #undef A
     return poisson_setup(c);
 }
@@ -162,6 +167,9 @@ int ibmdem_destroy(ibmdem_ctx *c) {
     for (hipEvent_t e : c->prof.ev) (void)hipEventDestroy(e);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
     if (c->flags_host) (void)hipHostFree(c->flags_host);
+    if (c->conv_host) (void)hipHostFree(c->conv_host);
+    for (int q = 0; q < 2; ++q)
+        if (c->iter_ev[q]) (void)hipEventDestroy(c->iter_ev[q]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return IBMDEM_OK;
@@ -241,6 +249,7 @@ int ibmdem_upload_markers(ibmdem_ctx *c, const ibmdem_markers *m) {
     TRY(h2d(c, c->mmass, m->mass, v1));
     TRY(h2d(c, c->mrad, m->radeff, v1));
     TRY(h2d(c, c->mbody, m->body, (size_t)M * sizeof(int)));
+    c->rmax_dirty = true;
     TRY(dem_rebuild(c));
     TRY(dem_check_flags(c));
     return finish(c);
@@ -274,6 +283,7 @@ int ibmdem_upload_bodies(ibmdem_ctx *c, const ibmdem_bodies *b) {
     TRY(h2d(c, c->bm, b->mass, v1)); TRY(h2d(c, c->brho, b->rho, v1));
     TRY(h2d(c, c->bfixed, b->fixed, (size_t)B * sizeof(int)));
     TRY(h2d(c, c->bfhydro, b->fhydro, v3)); TRY(h2d(c, c->bfmomhydro, b->fmomhydro, v3));
+    c->rmax_dirty = true;
     return finish(c);
 }
 

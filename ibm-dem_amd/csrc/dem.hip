@@ -176,6 +176,55 @@ __device__ __forceinline__ bool nb_cell(int &i, int n, int lo, int hi, double L,
     return true;
 }
 
+// max |x_m - X_b| over each body's markers (rigid: constant up to rounding)
+__global__ __launch_bounds__(T) void k_body_rmax(int M, int B, GridDev g, const int *__restrict__ bstart,
+                                                 const double *__restrict__ mx, const double *__restrict__ bx,
+                                                 double *__restrict__ rmax) {
+    __shared__ double red[T];
+    const int b = blockIdx.x;
+    double r = 0.0;
+    for (int m = bstart[b] + threadIdx.x; m < bstart[b + 1]; m += T) {
+        double d2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            double dd = mx[(long)d * M + m] - bx[(long)d * B + b];
+            if (g.bt[2 * d] == IBMDEM_BC_PERIODIC) dd -= g.L[d] * rint(dd / g.L[d]);
+            d2 += dd * dd;
+        }
+        r = fmax(r, sqrt(d2));
+    }
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int st = T / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rmax[b] = red[0];
+}
+
+// near[b] = 1 if some other body's markers can come within the candidate
+// distance of b's: |X_b - X_c| < rmax_b + rmax_c + lim (+ a safety margin)
+__global__ __launch_bounds__(64) void k_body_near(int B, GridDev g, const double *__restrict__ bx,
+                                                  const double *__restrict__ rmax, double lim, int *__restrict__ near) {
+    const int b = blockIdx.x;
+    const double xb0 = bx[b], xb1 = bx[(long)B + b], xb2 = bx[2L * B + b];
+    const double rb = rmax[b] + lim + 1e-6 * (g.dx + lim);
+    int hit = 0;
+    for (int c = threadIdx.x; c < B && !hit; c += 64) {
+        if (c == b) continue;
+        double d[3] = {bx[c] - xb0, bx[(long)B + c] - xb1, bx[2L * B + c] - xb2};
+        double d2 = 0.0;
+        for (int q = 0; q < 3; ++q) {
+            if (g.bt[2 * q] == IBMDEM_BC_PERIODIC) d[q] -= g.L[q] * rint(d[q] / g.L[q]);
+            d2 += d[q] * d[q];
+        }
+        const double lim_bc = rb + rmax[c];
+        if (d2 < lim_bc * lim_bc) hit = 1;
+    }
+    // any-reduction over the wavefront
+    hit = __any(hit) ? 1 : 0;
+    if (threadIdx.x == 0) near[b] = hit;
+}
+
 // fcol of marker i (ZJS_COLLISION_DEM.F90:31-72, 93-146)
 template <bool PAIRS>
 __global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *__restrict__ mx,
@@ -185,7 +234,7 @@ __global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *_
                                                const int *__restrict__ bin_items, double *__restrict__ fcol,
                                                ContactParams cp, int max_pairs, int *__restrict__ flags,
                                                int cap, int *__restrict__ pcount, int *__restrict__ ppartner,
-                                               int *__restrict__ pshift) {
+                                               int *__restrict__ pshift, const int *__restrict__ bnear) {
     const int i = blockIdx.x * T + threadIdx.x;
     if (i >= M) return;
     const int cl = mcell[i];
@@ -196,17 +245,32 @@ __global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *_
     const int bi = mbody[i];
     double f0 = 0, f1 = 0, f2 = 0;
     int cnt = 0;
-    for (int kk = -2; kk <= 2; ++kk) {
+    // a body with no other body in reach has no candidate pairs: skipping
+    // its scan adds nothing, exactly as the scan would
+    const int kstart = bnear[bi] ? -2 : 3;
+    for (int kk = kstart; kk <= 2; ++kk) {
         int kc = ck + kk, kcode; double ksh;
         if (!nb_cell(kc, g.nz, g.bt[4], g.bt[5], g.L[2], ksh, kcode)) continue;
         for (int jj = -2; jj <= 2; ++jj) {
             int jc = cj + jj, jcode; double jsh;
             if (!nb_cell(jc, g.ny, g.bt[2], g.bt[3], g.L[1], jsh, jcode)) continue;
-            for (int ii = -2; ii <= 2; ++ii) {
+            // the five x cells of this row; when none wraps they are
+            // consecutive cells, so their sorted items form one range
+            const bool row = ci - 2 >= 1 && ci + 2 <= g.nx;
+            for (int ii = row ? 0 : -2; ii <= 2; ++ii) {
                 int ic = ci + ii, icode; double ish;
-                if (!nb_cell(ic, g.nx, g.bt[0], g.bt[1], g.L[0], ish, icode)) continue;
-                const long b = cidx(g, ic, jc, kc);
-                const int q0 = bin_start[b], q1 = bin_start[b + 1];
+                int q0, q1;
+                if (row) {
+                    icode = 0; ish = 0.0;
+                    q0 = bin_start[cidx(g, ci - 2, jc, kc)];
+                    q1 = bin_start[cidx(g, ci + 2, jc, kc) + 1];
+                    ii = 2;
+                } else {
+                    if (!nb_cell(ic, g.nx, g.bt[0], g.bt[1], g.L[0], ish, icode)) continue;
+                    const long b = cidx(g, ic, jc, kc);
+                    q0 = bin_start[b];
+                    q1 = bin_start[b + 1];
+                }
                 for (int q = q0; q < q1; ++q) {
                     const int j = bin_items[q];
                     if (mbody[j] == bi) continue;
@@ -514,13 +578,29 @@ int dem_rebuild(ibmdem_ctx *c) {
     return IBMDEM_OK;
 }
 
+int dem_near(ibmdem_ctx *c) {
+    if (c->B == 0) return IBMDEM_OK;
+    if (c->rmax_dirty) {
+        hipLaunchKernelGGL(k_body_rmax, dim3(c->B), dim3(T), 0, c->stream, c->M, c->B, c->g, c->body_start, c->mx,
+                           c->bx, c->brmax);
+        CHECK_LAUNCH(c);
+        c->rmax_dirty = false;
+    }
+    const double lim = c->cs.collision_dist * (1.0 + c->cs.collision_buffer);
+    hipLaunchKernelGGL(k_body_near, dim3(c->B), dim3(64), 0, c->stream, c->B, c->g, c->bx, c->brmax, lim, c->bnear);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
 int dem_contact(ibmdem_ctx *c) {
     const int M = c->M;
     if (M == 0) return IBMDEM_OK;
+    int rc;
+    if ((rc = dem_near(c))) return rc;
     prof_begin(c, IBMDEM_K_CONTACT);
     hipLaunchKernelGGL((k_contact<false>), dim3(div_up(M, T)), dim3(T), 0, c->stream, M, c->g, c->mx, c->mv,
                        c->mmass, c->mrad, c->mbody, c->mcell, c->bin_start, c->bin_items, c->fcol,
-                       contact_params(c), c->cs.max_pairs, c->flags, 0, nullptr, nullptr, nullptr);
+                       contact_params(c), c->cs.max_pairs, c->flags, 0, nullptr, nullptr, nullptr, c->bnear);
     prof_end(c, IBMDEM_K_CONTACT, 0.0);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
@@ -529,17 +609,18 @@ int dem_contact(ibmdem_ctx *c) {
 int dem_pairs(ibmdem_ctx *c, int cap) {
     const int M = c->M;
     if (M == 0) return IBMDEM_OK;
+    int rc;
     if (cap > c->pair_cap) {
-        int rc;
         if ((rc = dev_alloc(c, &c->pair_count, M))) return rc;
         if ((rc = dev_alloc(c, &c->pair_partner, (size_t)M * cap))) return rc;
         if ((rc = dev_alloc(c, &c->pair_shift, (size_t)M * cap))) return rc;
         c->pair_cap = cap;
     }
+    if ((rc = dem_near(c))) return rc;
     hipLaunchKernelGGL((k_contact<true>), dim3(div_up(M, T)), dim3(T), 0, c->stream, M, c->g, c->mx, c->mv,
                        c->mmass, c->mrad, c->mbody, c->mcell, c->bin_start, c->bin_items, c->fcol,
                        contact_params(c), c->cs.max_pairs, c->flags, c->pair_cap, c->pair_count, c->pair_partner,
-                       c->pair_shift);
+                       c->pair_shift, c->bnear);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }

@@ -95,7 +95,8 @@ __device__ __forceinline__ int diag_index(const LevelDev &L, int i, int j, int k
 // a V-cycle visit); the other colour is written 0.
 template <typename T, typename TB>
 __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
-                                                int color, int zero_init) {
+                                                int color, int zero_init, const double *__restrict__ stop) {
+    if (*stop != 0.0) return;
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     if (j >= L.ny) return;
@@ -117,7 +118,8 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
 // r = b - A x
 template <typename T, typename TB>
 __global__ __launch_bounds__(RB) void k_residual(LevelDev L, const T *__restrict__ x, const TB *__restrict__ b,
-                                                 T *__restrict__ r) {
+                                                 T *__restrict__ r, const double *__restrict__ stop) {
+    if (*stop != 0.0) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     if (i >= L.nx || j >= L.ny) return;
@@ -135,7 +137,8 @@ __device__ __forceinline__ int fine_index(int f, int n, int lo, int hi) {
 // b_c = (1/8) sum w r_f over the 4x4x4 fine cells, w = wz*wy*wx from (1/4,3/4,3/4,1/4)
 template <typename T>
 __global__ __launch_bounds__(RB) void k_restrict(LevelDev F, const T *__restrict__ rf, LevelDev Cl,
-                                                 T *__restrict__ bc) {
+                                                 T *__restrict__ bc, const double *__restrict__ stop) {
+    if (*stop != 0.0) return;
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     const int J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
     if (I >= Cl.nx || J >= Cl.ny) return;
@@ -169,7 +172,8 @@ __device__ __forceinline__ void parents(int f, int nc, int lo, int hi, int &p0, 
 // x_f += sum (wz*wy*wx) x_c over the 8 parents, weights (3/4 own, 1/4 neighbour)
 template <typename T>
 __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, LevelDev Cl,
-                                                const T *__restrict__ xc) {
+                                                const T *__restrict__ xc, const double *__restrict__ stop) {
+    if (*stop != 0.0) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     if (i >= F.nx || j >= F.ny) return;
@@ -189,10 +193,72 @@ __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, 
     xf[o] = xf[o] + s;
 }
 
-// coarsest level: one work-group, symmetric sweeps in place
+// coarsest level, one work-group. Small levels (<= COARSE_LDS cells, every
+// level the grids here coarsen to) live in LDS with a precomputed neighbour
+// table whose missing (wall) neighbours point at a zero slot, so the sweeps
+// evaluate exactly nbsum's expression without index arithmetic; larger
+// coarsest levels run the same sweeps in global memory.
+constexpr int COARSE_LDS = 512;
+template <typename T>
+__global__ __launch_bounds__(64) void k_coarse_lds(LevelDev L, T *__restrict__ xg, const T *__restrict__ bg,
+                                                   int sweeps, const double *__restrict__ stop) {
+    if (*stop != 0.0) return;
+    __shared__ T x[COARSE_LDS + 1], b[COARSE_LDS], id[COARSE_LDS], t[COARSE_LDS];
+    __shared__ short nb[6][COARSE_LDS];
+    __shared__ unsigned char col[COARSE_LDS];
+    const int n = L.nx * L.ny * L.nz;
+    const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+    for (int q = threadIdx.x; q < n; q += 64) {
+        const int i = q % L.nx, j = (q / L.nx) % L.ny, k = q / (L.nx * L.ny);
+        const int a[3] = {i, j, k}, m[3] = {L.nx, L.ny, L.nz};
+        for (int d = 0; d < 3; ++d)
+            for (int dir = 0; dir < 2; ++dir) {
+                int p[3] = {a[0], a[1], a[2]};
+                int v = p[d] + (dir ? 1 : -1);
+                int idx;
+                if (v < 0) idx = L.bt[2 * d] == IBMDEM_BC_WALL ? -1 : v + m[d];
+                else if (v >= m[d]) idx = L.bt[2 * d + 1] == IBMDEM_BC_WALL ? -1 : v - m[d];
+                else idx = v;
+                p[d] = idx;
+                nb[2 * d + dir][q] = (short)(idx < 0 ? n : (p[2] * L.ny + p[1]) * L.nx + p[0]);
+            }
+        id[q] = T(L.idiag[diag_index(L, i, j, k)]);
+        col[q] = (unsigned char)((i + j + k) & 1);
+        b[q] = bg[q];
+        x[q] = T(0);
+    }
+    if (threadIdx.x == 0) x[n] = T(0);
+    __syncthreads();
+    auto upd = [&](int q) -> T {
+        const T sx = (x[nb[0][q]] + x[nb[1][q]]) * ihx2;
+        const T sy = (x[nb[2][q]] + x[nb[3][q]]) * ihy2;
+        const T sz = (x[nb[4][q]] + x[nb[5][q]]) * ihz2;
+        return (b[q] + (sx + sy + sz)) * id[q];
+    };
+    auto half = [&](int color) {
+        for (int q = threadIdx.x; q < n; q += 64)
+            if (col[q] == color) x[q] = upd(q);
+        __syncthreads();
+    };
+    if (L.rb_valid) {
+        for (int s = 0; s < sweeps; ++s) { half(0); half(1); }
+        for (int s = 0; s < sweeps; ++s) { half(1); half(0); }
+    } else {
+        const T om = T(0.8);
+        for (int s = 0; s < 2 * sweeps; ++s) {
+            for (int q = threadIdx.x; q < n; q += 64) t[q] = x[q] + om * (upd(q) - x[q]);
+            __syncthreads();
+            for (int q = threadIdx.x; q < n; q += 64) x[q] = t[q];
+            __syncthreads();
+        }
+    }
+    for (int q = threadIdx.x; q < n; q += 64) xg[q] = x[q];
+}
+
 template <typename T>
 __global__ __launch_bounds__(COARSE_T) void k_coarse(LevelDev L, T *__restrict__ x, const T *__restrict__ b,
-                                                     T *__restrict__ tmp, int sweeps) {
+                                                     T *__restrict__ tmp, int sweeps, const double *__restrict__ stop) {
+    if (*stop != 0.0) return;
     const long n = (long)L.nx * L.ny * L.nz;
     for (long q = threadIdx.x; q < n; q += COARSE_T) x[q] = T(0);
     __syncthreads();
@@ -237,7 +303,9 @@ __device__ __forceinline__ double block_reduce(double s, double *red) {
 
 // Ap = A p, partial sums of p.Ap (chunked: block b owns a contiguous range)
 __global__ __launch_bounds__(RED_T) void k_matvec_dot(LevelDev L, const double *__restrict__ p,
-                                                      double *__restrict__ Ap, double *__restrict__ partial) {
+                                                      double *__restrict__ Ap, double *__restrict__ partial,
+                                                      const double *__restrict__ scal) {
+    if (scal[SC_CONV] != 0.0) return;
     __shared__ double red[RED_T];
     const long n = (long)L.nx * L.ny * L.nz;
     const long chunk = (n + gridDim.x - 1) / gridDim.x;
@@ -257,6 +325,7 @@ __global__ __launch_bounds__(RED_T) void k_matvec_dot(LevelDev L, const double *
 __global__ __launch_bounds__(RED_T) void k_cg_update(long n, double *__restrict__ x, double *__restrict__ r,
                                                      const double *__restrict__ p, const double *__restrict__ Ap,
                                                      const double *__restrict__ scal, double *__restrict__ partial) {
+    if (scal[SC_CONV] != 0.0) return;
     __shared__ double red[RED_T];
     const double alpha = scal[SC_ALPHA];
     const long chunk = (n + gridDim.x - 1) / gridDim.x;
@@ -274,7 +343,8 @@ __global__ __launch_bounds__(RED_T) void k_cg_update(long n, double *__restrict_
 
 // partial sums of a.b
 __global__ __launch_bounds__(RED_T) void k_dot(long n, const double *__restrict__ a, const double *__restrict__ b,
-                                               double *__restrict__ partial) {
+                                               double *__restrict__ partial, const double *__restrict__ scal) {
+    if (scal[SC_CONV] != 0.0) return;
     __shared__ double red[RED_T];
     const long chunk = (n + gridDim.x - 1) / gridDim.x;
     const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
@@ -285,9 +355,11 @@ __global__ __launch_bounds__(RED_T) void k_dot(long n, const double *__restrict_
 }
 
 // finalize a partial sum into scal[slot]; op: 0 store, 1 alpha = rz/pAp,
-// 2 beta = rz_new/rz and rz = rz_new
+// 2 beta = rz_new/rz and rz = rz_new, 3 r.r and the convergence test of
+// iteration it, 4 ||b||^2 and tol = rtol ||b||
 __global__ __launch_bounds__(RED_T) void k_final(const double *__restrict__ partial, int np, double *__restrict__ scal,
-                                                 int slot, int op) {
+                                                 int slot, int op, int it, double rtol) {
+    if (scal[SC_CONV] != 0.0) return;
     __shared__ double red[RED_T];
     double s = 0.0;
     for (int q = threadIdx.x; q < np; q += RED_T) s += partial[q];
@@ -296,12 +368,15 @@ __global__ __launch_bounds__(RED_T) void k_final(const double *__restrict__ part
         scal[slot] = t;
         if (op == 1) scal[SC_ALPHA] = scal[SC_RZ] / t;
         if (op == 2) { scal[SC_BETA] = t / scal[SC_RZ]; scal[SC_RZ] = t; }
+        if (op == 3 && (sqrt(t) <= scal[SC_TOL] || !(t == t))) { scal[SC_CONV] = 1.0; scal[SC_ITER] = it; }
+        if (op == 4) scal[SC_TOL] = rtol * sqrt(t);
     }
 }
 
 // p = z + beta p
 __global__ __launch_bounds__(RED_T) void k_cg_p(long n, double *__restrict__ p, const double *__restrict__ z,
                                                 const double *__restrict__ scal) {
+    if (scal[SC_CONV] != 0.0) return;
     const double beta = scal[SC_BETA];
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T)
         p[q] = z[q] + beta * p[q];
@@ -309,7 +384,8 @@ __global__ __launch_bounds__(RED_T) void k_cg_p(long n, double *__restrict__ p, 
 
 // r = rhs with the pinned row, x = 0
 __global__ __launch_bounds__(RED_T) void k_cg_init(long n, const double *__restrict__ rhs, double *__restrict__ r,
-                                                   double *__restrict__ x, const double *__restrict__ scal) {
+                                                   double *__restrict__ x, double *__restrict__ scal) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[SC_CONV] = 0.0; scal[SC_ITER] = 0.0; }
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
         r[q] = q == 0 ? -scal[SC_PIN] : rhs[q];
         x[q] = 0.0;
@@ -320,7 +396,9 @@ __global__ __launch_bounds__(RED_T) void k_copy(long n, double *__restrict__ d, 
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) d[q] = s[q];
 }
 template <typename TD, typename TS>
-__global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, const TS *__restrict__ s) {
+__global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, const TS *__restrict__ s,
+                                                   const double *__restrict__ stop) {
+    if (*stop != 0.0) return;
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) d[q] = TD(s[q]);
 }
 
@@ -361,7 +439,12 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
     const LevelDev &d = L.d;
     hipStream_t s = c->stream;
     if (l == c->nlev - 1) {
-        hipLaunchKernelGGL((k_coarse<T>), dim3(1), dim3(COARSE_T), 0, s, d, L.x, L.b, L.r, c->cs.mg_coarse_sweeps);
+        if (L.n <= COARSE_LDS)
+            hipLaunchKernelGGL((k_coarse_lds<T>), dim3(1), dim3(64), 0, s, d, L.x, L.b, c->cs.mg_coarse_sweeps,
+                               c->scal + SC_CONV);
+        else
+            hipLaunchKernelGGL((k_coarse<T>), dim3(1), dim3(COARSE_T), 0, s, d, L.x, L.b, L.r,
+                               c->cs.mg_coarse_sweeps, c->scal + SC_CONV);
         CHECK_LAUNCH(c);
         return IBMDEM_OK;
     }
@@ -372,7 +455,7 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
     const double hb = l == 0 ? 1.5 * sizeof(T) * (double)L.n : 0.0;
     auto half = [&](int color, int zero) {
         if (l == 0) prof_begin(c, IBMDEM_K_MG_SMOOTH);
-        hipLaunchKernelGGL((k_rb_half<T, T>), rb.grid, rb.block, 0, s, d, L.x, L.b, color, zero);
+        hipLaunchKernelGGL((k_rb_half<T, T>), rb.grid, rb.block, 0, s, d, L.x, L.b, color, zero, c->scal + SC_CONV);
         if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, hb);
     };
     for (int q = 0; q < c->cs.mg_pre; ++q) {
@@ -380,15 +463,15 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
         half(1, 0);
         CHECK_LAUNCH(c);
     }
-    hipLaunchKernelGGL((k_residual<T, T>), full.grid, full.block, 0, s, d, L.x, L.b, L.r);
+    hipLaunchKernelGGL((k_residual<T, T>), full.grid, full.block, 0, s, d, L.x, L.b, L.r, c->scal + SC_CONV);
     CHECK_LAUNCH(c);
     MGLevel<T> &C = lev[l + 1];
     LaunchShape cs = shape_for(C.d.nx, C.d.ny, C.d.nz);
-    hipLaunchKernelGGL((k_restrict<T>), cs.grid, cs.block, 0, s, d, L.r, C.d, C.b);
+    hipLaunchKernelGGL((k_restrict<T>), cs.grid, cs.block, 0, s, d, L.r, C.d, C.b, c->scal + SC_CONV);
     CHECK_LAUNCH(c);
     int rc = vcycle<T>(c, lev, l + 1);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_prolong<T>), full.grid, full.block, 0, s, d, L.x, C.d, C.x);
+    hipLaunchKernelGGL((k_prolong<T>), full.grid, full.block, 0, s, d, L.x, C.d, C.x, c->scal + SC_CONV);
     CHECK_LAUNCH(c);
     for (int q = 0; q < c->cs.mg_post; ++q) {
         half(1, 0);
@@ -403,12 +486,12 @@ int precondition(ibmdem_ctx *c) {
     const long n = c->n;
     if (c->cs.precision == IBMDEM_MIXED) {
         hipLaunchKernelGGL((k_convert<float, double>), dim3(stream_blocks(n)), dim3(RED_T), 0, c->stream, n,
-                           c->lev32[0].b, c->cg_r);
+                           c->lev32[0].b, c->cg_r, c->scal + SC_CONV);
         CHECK_LAUNCH(c);
         int rc = vcycle<float>(c, c->lev32, 0);
         if (rc) return rc;
         hipLaunchKernelGGL((k_convert<double, float>), dim3(stream_blocks(n)), dim3(RED_T), 0, c->stream, n,
-                           c->cg_z, c->lev32[0].x);
+                           c->cg_z, c->lev32[0].x, c->scal + SC_CONV);
         CHECK_LAUNCH(c);
         return IBMDEM_OK;
     }
@@ -418,9 +501,10 @@ int precondition(ibmdem_ctx *c) {
 }
 
 int dot_into(ibmdem_ctx *c, const double *a, const double *b, int slot, int op) {
-    hipLaunchKernelGGL(k_dot, dim3(RED_BLOCKS), dim3(RED_T), 0, c->stream, c->n, a, b, c->partial);
+    hipLaunchKernelGGL(k_dot, dim3(RED_BLOCKS), dim3(RED_T), 0, c->stream, c->n, a, b, c->partial, c->scal);
     CHECK_LAUNCH(c);
-    hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, c->stream, c->partial, RED_BLOCKS, c->scal, slot, op);
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, c->stream, c->partial, RED_BLOCKS, c->scal, slot, op, 0,
+                       c->cs.poisson_rtol);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
@@ -476,12 +560,13 @@ int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
     if ((rc = reduce_sum(c, c->rhs, nullptr, n, SC_PIN, 1))) return rc;
     hipLaunchKernelGGL(k_cg_init, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->rhs, c->cg_r, c->cg_x, c->scal);
     CHECK_LAUNCH(c);
-    if ((rc = dot_into(c, c->cg_r, c->cg_r, SC_BNORM2, 0))) return rc;
+    if ((rc = dot_into(c, c->cg_r, c->cg_r, SC_BNORM2, 4))) return rc;
     if ((rc = read_scalars(c))) return rc;
     const double bnorm = std::sqrt(c->scal_host[SC_BNORM2]);
     int it = 0, conv = 0;
     double rr = bnorm * bnorm;
     const LevelDev &L0 = c->lev64[0].d;
+    const int maxit = c->cs.poisson_max_iter;
     if (bnorm == 0.0) {
         conv = 1;
     } else {
@@ -489,31 +574,50 @@ int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
         hipLaunchKernelGGL(k_copy, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z);
         CHECK_LAUNCH(c);
         if ((rc = dot_into(c, c->cg_r, c->cg_z, SC_RZ, 0))) return rc;
-        const double tol = c->cs.poisson_rtol * bnorm;
-        for (it = 1; it <= c->cs.poisson_max_iter; ++it) {
+        // Iterations are queued without waiting: the host keeps at most two
+        // in flight and reads the convergence flag of the older one. Once the
+        // device sets SC_CONV every later CG kernel returns immediately, so x
+        // is the iterate of the converging iteration however many extra
+        // iterations were queued.
+        for (int q = 0; q <= maxit + 1; ++q) c->conv_host[q] = 0.0;
+        int queued = 0;
+        bool done = false;
+        for (it = 1; it <= maxit && !done; ++it) {
             prof_begin(c, IBMDEM_K_MATVEC);
-            hipLaunchKernelGGL(k_matvec_dot, dim3(RED_BLOCKS), dim3(RED_T), 0, s, L0, c->cg_p, c->cg_Ap, c->partial);
+            hipLaunchKernelGGL(k_matvec_dot, dim3(RED_BLOCKS), dim3(RED_T), 0, s, L0, c->cg_p, c->cg_Ap, c->partial,
+                               c->scal);
             prof_end(c, IBMDEM_K_MATVEC, 2.0 * 8.0 * (double)n);
             CHECK_LAUNCH(c);
-            hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_PAP, 1);
+            hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_PAP, 1, it,
+                               c->cs.poisson_rtol);
             CHECK_LAUNCH(c);
             prof_begin(c, IBMDEM_K_CG_UPDATE);
             hipLaunchKernelGGL(k_cg_update, dim3(RED_BLOCKS), dim3(RED_T), 0, s, n, c->cg_x, c->cg_r, c->cg_p,
                                c->cg_Ap, c->scal, c->partial);
             prof_end(c, IBMDEM_K_CG_UPDATE, 6.0 * 8.0 * (double)n);
             CHECK_LAUNCH(c);
-            hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_RR, 0);
+            hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_RR, 3, it,
+                               c->cs.poisson_rtol);
             CHECK_LAUNCH(c);
-            if ((rc = read_scalars(c))) return rc;
-            rr = c->scal_host[SC_RR];
-            if (!(rr == rr)) return set_error(c, IBMDEM_ERR_NAN, "non-finite residual in the Poisson solve");
-            if (std::sqrt(rr) <= tol) { conv = 1; break; }
+            HIPX(c, hipMemcpyAsync(c->conv_host + it, c->scal + SC_CONV, sizeof(double), hipMemcpyDeviceToHost, s));
             if ((rc = precondition(c))) return rc;
             if ((rc = dot_into(c, c->cg_r, c->cg_z, SC_RZ_NEW, 2))) return rc;
             hipLaunchKernelGGL(k_cg_p, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z, c->scal);
             CHECK_LAUNCH(c);
+            HIPX(c, hipEventRecord(c->iter_ev[it & 1], s));
+            ++queued;
+            if (queued >= 2) {
+                // wait for the previous iteration, then look at its flag
+                HIPX(c, hipEventSynchronize(c->iter_ev[(it - 1) & 1]));
+                if (c->conv_host[it - 1] != 0.0) done = true;
+            }
         }
-        if (it > c->cs.poisson_max_iter) it = c->cs.poisson_max_iter;
+        HIPX(c, hipStreamSynchronize(s));
+        if ((rc = read_scalars(c))) return rc;
+        conv = c->scal_host[SC_CONV] != 0.0;
+        rr = c->scal_host[SC_RR];
+        it = conv ? (int)c->scal_host[SC_ITER] : maxit;
+        if (!(rr == rr)) return set_error(c, IBMDEM_ERR_NAN, "non-finite residual in the Poisson solve");
     }
     hipLaunchKernelGGL(k_gauge, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_x, c->pcorr);
     CHECK_LAUNCH(c);
