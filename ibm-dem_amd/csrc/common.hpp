@@ -16,6 +16,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 #include <string>
 #include <vector>
@@ -101,6 +102,64 @@ template <int C>
 __device__ __forceinline__ bool wall_face(const GridDev &g, int i, int j, int k) {
     int idx = C == 0 ? i : (C == 1 ? j : k);
     return idx == 1 && g.bt[2 * C] == IBMDEM_BC_WALL;
+}
+
+// Per-thread axis map: for offsets O0..O0+NO-1 around a 1-based index, the
+// 0-based interior index the ghost value comes from, whether it is a wall
+// mirror (negated for tangential components) and whether it lies beyond a
+// wall (forced zero for the wall-normal component). Evaluated once per
+// thread so every tap is one load, with no divergent boundary path.
+template <int O0, int NO>
+struct AxisMap {
+    int idx[NO];
+    bool mir[NO];
+    bool wall[NO];
+    __device__ __forceinline__ void init(int i, int n, int lo, int hi) {
+#pragma unroll
+        for (int q = 0; q < NO; ++q) {
+            int j = i + O0 + q;
+            bool m = false;
+            if (j < 1) {
+                if (lo == IBMDEM_BC_PERIODIC) j += n;
+                else { j = 1 - j; m = true; }
+            } else if (j > n) {
+                if (hi == IBMDEM_BC_PERIODIC) j -= n;
+                else { j = 2 * n + 1 - j; m = true; }
+            }
+            idx[q] = j - 1;
+            mir[q] = m;
+            wall[q] = m;
+        }
+    }
+};
+
+// velocity component C at mapped offsets (a,b,c) of the three axis maps
+template <int C, class MX, class MY, class MZ, typename T>
+__device__ __forceinline__ T vel_tap(const T *__restrict__ f, const GridDev &g, const MX &mx, const MY &my,
+                                     const MZ &mz, int a, int b, int c) {
+    const bool zero = (C == 0 && mx.wall[a]) || (C == 1 && my.wall[b]) || (C == 2 && mz.wall[c]);
+    const bool neg = ((C != 0 && mx.mir[a]) ^ (C != 1 && my.mir[b])) ^ (C != 2 && mz.mir[c]);
+    const T v = f[(long)mx.idx[a] + (long)g.nx * ((long)my.idx[b] + (long)g.ny * mz.idx[c])];
+    return zero ? T(0) : (neg ? -v : v);
+}
+// cell-centred scalar at mapped offsets (mirror without negation)
+template <class MX, class MY, class MZ, typename T>
+__device__ __forceinline__ T scal_tap(const T *__restrict__ f, const GridDev &g, const MX &mx, const MY &my,
+                                      const MZ &mz, int a, int b, int c) {
+    return f[(long)mx.idx[a] + (long)g.nx * ((long)my.idx[b] + (long)g.ny * mz.idx[c])];
+}
+
+// x / d. When d is a power of two (P2), 1/d is exact and x * (1/d) rounds to
+// the same double as x / d, so kernels take the multiply without changing a
+// bit; otherwise they divide.
+template <bool P2>
+__device__ __forceinline__ double qdiv(double x, double d, double inv) {
+    return P2 ? x * inv : x / d;
+}
+inline bool is_pow2(double d) {
+    if (!(d > 0.0)) return false;
+    int e;
+    return frexp(d, &e) == 0.5;
 }
 
 // ---------------------------------------------------------------------------

@@ -28,45 +28,45 @@ __device__ __forceinline__ void xcd_block(int &bx, int &by, int &bz) {
     bz = (int)(b / ((long)gx * gy));
 }
 
-struct Direct {
-    const double *__restrict__ p;
-    long o, sy, sz;
-    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
-        return p[o + di + dj * sy + dk * sz];
-    }
-};
+using Map3 = AxisMap<-1, 3>;
 template <int C>
-struct Ghost {
+struct Tap {
     const double *__restrict__ p;
-    GridDev g;
-    int i, j, k;
+    const GridDev *g;
+    const Map3 *mx, *my, *mz;
     __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
-        return vel_at<C>(p, g, i + di, j + dj, k + dk);
+        return vel_tap<C>(p, *g, *mx, *my, *mz, di + 1, dj + 1, dk + 1);
     }
 };
-struct GhostP {
+struct TapP {
     const double *__restrict__ p;
-    GridDev g;
-    int i, j, k;
+    const GridDev *g;
+    const Map3 *mx, *my, *mz;
     __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
-        return scal_at(p, g, i + di, j + dj, k + dk);
+        return scal_tap(p, *g, *mx, *my, *mz, di + 1, dj + 1, dk + 1);
     }
 };
 
 struct PredParams {
     double dt, visc, rhof, a, c;
     double cp[3];
+    double idx, idy, idz, idx2, idy2, idz2, irhof;   // exact reciprocals (used when P2)
 };
 
 // ConvectionTensor + ViscousTensor + PressureGradient + VelocityGuess for one
 // cell (fluid.F90:15-44, 58-66, 80-82, 118-126), the reference's operand
 // order, in compute type CT. Returns the three increments
 // a*RHS_noF - c*gradP (without the IBM force, already inside u~).
-template <typename CT, class LU, class LV, class LW, class LP>
+template <bool P2, typename CT>
+__device__ __forceinline__ CT pdiv(CT x, double d, double inv) {
+    return P2 ? x * CT(inv) : x / CT(d);
+}
+
+template <typename CT, bool P2, class LU, class LV, class LW, class LP>
 __device__ __forceinline__ void predictor_cell(const LU &u, const LV &v, const LW &w, const LP &P,
                                                const GridDev &g, const PredParams &pp, CT out[3]) {
     const CT h = CT(0.5);
-    const CT dx = CT(g.dx), dy = CT(g.dy), dz = CT(g.dz), visc = CT(pp.visc);
+    const CT visc = CT(pp.visc);
     const CT u0 = CT(u(0, 0, 0)), uE = CT(u(1, 0, 0)), uW = CT(u(-1, 0, 0)), uN = CT(u(0, 1, 0)),
              uS = CT(u(0, -1, 0)), uT = CT(u(0, 0, 1)), uB = CT(u(0, 0, -1));
     const CT v0 = CT(v(0, 0, 0)), vE = CT(v(1, 0, 0)), vW = CT(v(-1, 0, 0)), vN = CT(v(0, 1, 0)),
@@ -100,24 +100,25 @@ __device__ __forceinline__ void predictor_cell(const LU &u, const LV &v, const L
     CT wwN = h * (wT + w0) * h * (wT + w0);
     CT wwS = h * (wB + w0) * h * (wB + w0);
 
-    CT TIux = (uuE - uuW) / dx, TIuy = (uvB - uvF) / dy, TIuz = (uwN - uwS) / dz;
-    CT TIvx = (vuE - vuW) / dx, TIvy = (vvB - vvF) / dy, TIvz = (vwN - vwS) / dz;
-    CT TIwx = (wuE - wuW) / dx, TIwy = (wvB - wvF) / dy, TIwz = (wwN - wwS) / dz;
+    const double gdx = g.dx, gdy = g.dy, gdz = g.dz;
+    CT TIux = pdiv<P2>(uuE - uuW, gdx, pp.idx), TIuy = pdiv<P2>(uvB - uvF, gdy, pp.idy), TIuz = pdiv<P2>(uwN - uwS, gdz, pp.idz);
+    CT TIvx = pdiv<P2>(vuE - vuW, gdx, pp.idx), TIvy = pdiv<P2>(vvB - vvF, gdy, pp.idy), TIvz = pdiv<P2>(vwN - vwS, gdz, pp.idz);
+    CT TIwx = pdiv<P2>(wuE - wuW, gdx, pp.idx), TIwy = pdiv<P2>(wvB - wvF, gdy, pp.idy), TIwz = pdiv<P2>(wwN - wwS, gdz, pp.idz);
 
-    CT TVux = (uE + uW - CT(2) * u0) / (dx * dx) * visc;
-    CT TVuy = (uN + uS - CT(2) * u0) / (dy * dy) * visc;
-    CT TVuz = (uT + uB - CT(2) * u0) / (dz * dz) * visc;
-    CT TVvx = (vE + vW - CT(2) * v0) / (dx * dx) * visc;
-    CT TVvy = (vN + vS - CT(2) * v0) / (dy * dy) * visc;
-    CT TVvz = (vT + vB - CT(2) * v0) / (dz * dz) * visc;
-    CT TVwx = (wE + wW - CT(2) * w0) / (dx * dx) * visc;
-    CT TVwy = (wN + wS - CT(2) * w0) / (dy * dy) * visc;
-    CT TVwz = (wT + wB - CT(2) * w0) / (dz * dz) * visc;
+    const double dx2 = gdx * gdx, dy2 = gdy * gdy, dz2 = gdz * gdz;
+    CT TVux = pdiv<P2>(uE + uW - CT(2) * u0, dx2, pp.idx2) * visc;
+    CT TVuy = pdiv<P2>(uN + uS - CT(2) * u0, dy2, pp.idy2) * visc;
+    CT TVuz = pdiv<P2>(uT + uB - CT(2) * u0, dz2, pp.idz2) * visc;
+    CT TVvx = pdiv<P2>(vE + vW - CT(2) * v0, dx2, pp.idx2) * visc;
+    CT TVvy = pdiv<P2>(vN + vS - CT(2) * v0, dy2, pp.idy2) * visc;
+    CT TVvz = pdiv<P2>(vT + vB - CT(2) * v0, dz2, pp.idz2) * visc;
+    CT TVwx = pdiv<P2>(wE + wW - CT(2) * w0, dx2, pp.idx2) * visc;
+    CT TVwy = pdiv<P2>(wN + wS - CT(2) * w0, dy2, pp.idy2) * visc;
+    CT TVwz = pdiv<P2>(wT + wB - CT(2) * w0, dz2, pp.idz2) * visc;
 
-    const CT rhof = CT(pp.rhof);
-    CT Px = (p0 - pW) / dx / rhof;
-    CT Py = (p0 - pS) / dy / rhof;
-    CT Pz = (p0 - pB) / dz / rhof;
+    CT Px = pdiv<P2>(pdiv<P2>(p0 - pW, gdx, pp.idx), pp.rhof, pp.irhof);
+    CT Py = pdiv<P2>(pdiv<P2>(p0 - pS, gdy, pp.idy), pp.rhof, pp.irhof);
+    CT Pz = pdiv<P2>(pdiv<P2>(p0 - pB, gdz, pp.idz), pp.rhof, pp.irhof);
 
     CT Ru = -(TIux + TIuy + TIuz) + (TVux + TVuy + TVuz) + CT(pp.cp[0]);
     CT Rv = -(TIvx + TIvy + TIvz) + (TVvx + TVvy + TVvz) + CT(pp.cp[1]);
@@ -145,7 +146,7 @@ __device__ __forceinline__ void block_sum3(double s[3], double *part) {
 // VelocityGuess fused form: u* = u~ + dt*(a*RHS_noF(u~) - c*gradP_old).
 // Because u~ = uold + dt*f after the multi-direct-forcing loop, this equals
 // the reference's uold + dt*(a*RHS(u~) + 0*RHSl - c*gradP) for a = 1.
-template <typename CT, bool BULK>
+template <typename CT, bool BULK, bool P2>
 __global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *__restrict__ u,
                                                       const double *__restrict__ v,
                                                       const double *__restrict__ w,
@@ -159,15 +160,12 @@ __global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *_
     if (i <= g.nx && j <= g.ny) {
         const long o = cidx(g, i, j, k);
         CT inc[3];
-        const bool inner = i > 1 && i < g.nx && j > 1 && j < g.ny && k > 1 && k < g.nz;
-        if (inner) {
-            const long sy = g.nx, sz = (long)g.nx * g.ny;
-            predictor_cell<CT>(Direct{u, o, sy, sz}, Direct{v, o, sy, sz}, Direct{w, o, sy, sz},
-                               Direct{P, o, sy, sz}, g, pp, inc);
-        } else {
-            predictor_cell<CT>(Ghost<0>{u, g, i, j, k}, Ghost<1>{v, g, i, j, k}, Ghost<2>{w, g, i, j, k},
-                               GhostP{P, g, i, j, k}, g, pp, inc);
-        }
+        Map3 mx, my, mz;
+        mx.init(i, g.nx, g.bt[0], g.bt[1]);
+        my.init(j, g.ny, g.bt[2], g.bt[3]);
+        mz.init(k, g.nz, g.bt[4], g.bt[5]);
+        predictor_cell<CT, P2>(Tap<0>{u, &g, &mx, &my, &mz}, Tap<1>{v, &g, &mx, &my, &mz}, Tap<2>{w, &g, &mx, &my, &mz},
+                           TapP{P, &g, &mx, &my, &mz}, g, pp, inc);
         const double dt = pp.dt;
         double nu_ = wall_face<0>(g, i, j, k) ? 0.0 : u[o] + dt * (double)inc[0];
         double nv_ = wall_face<1>(g, i, j, k) ? 0.0 : v[o] + dt * (double)inc[1];
@@ -369,22 +367,23 @@ int fluid_predictor(ibmdem_ctx *c, double a, double b, double cc) {
     const int cur = c->vcur, nxt = 1 - c->vcur;
     const bool bulk = c->cs.bulk_switch[0] == 1 || c->cs.bulk_switch[1] == 1 || c->cs.bulk_switch[2] == 1;
     const double bytes = 7.0 * 8.0 * (double)c->n;
+    pp.idx = 1.0 / g.dx; pp.idy = 1.0 / g.dy; pp.idz = 1.0 / g.dz;
+    pp.idx2 = 1.0 / (g.dx * g.dx); pp.idy2 = 1.0 / (g.dy * g.dy); pp.idz2 = 1.0 / (g.dz * g.dz);
+    pp.irhof = 1.0 / c->cs.rhof;
+    const bool p2 = is_pow2(g.dx) && is_pow2(g.dy) && is_pow2(g.dz) && is_pow2(c->cs.rhof);
+    const bool mixed = c->cs.precision == IBMDEM_MIXED;
+#define LAUNCH_PRED(CT, BK, PT)                                                                            \
+    hipLaunchKernelGGL((k_predictor<CT, BK, PT>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur], c->w[cur], \
+                       c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial)
     prof_begin(c, IBMDEM_K_PREDICTOR);
-    if (c->cs.precision == IBMDEM_MIXED) {
-        if (bulk)
-            hipLaunchKernelGGL((k_predictor<float, true>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],
-                               c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial);
-        else
-            hipLaunchKernelGGL((k_predictor<float, false>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],
-                               c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial);
+    if (mixed) {
+        if (bulk) { if (p2) LAUNCH_PRED(float, true, true); else LAUNCH_PRED(float, true, false); }
+        else { if (p2) LAUNCH_PRED(float, false, true); else LAUNCH_PRED(float, false, false); }
     } else {
-        if (bulk)
-            hipLaunchKernelGGL((k_predictor<double, true>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],
-                               c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial);
-        else
-            hipLaunchKernelGGL((k_predictor<double, false>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],
-                               c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial);
+        if (bulk) { if (p2) LAUNCH_PRED(double, true, true); else LAUNCH_PRED(double, true, false); }
+        else { if (p2) LAUNCH_PRED(double, false, true); else LAUNCH_PRED(double, false, false); }
     }
+#undef LAUNCH_PRED
     CHECK_LAUNCH(c);
     prof_end(c, IBMDEM_K_PREDICTOR, bytes);
     c->vcur = nxt;

@@ -69,12 +69,13 @@ __global__ __launch_bounds__(T) void k_weights(int M, GridDev g, const double *_
 
 // UpdateFluidVelocityOnLagrangianPoints: U_p = sum w u dx dy dz,
 // fdrag += (U_p - v_p)/dt; also stores fdrag*vol for the spread
+template <bool P2>
 __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const double *__restrict__ u,
                                               const double *__restrict__ v, const double *__restrict__ w,
                                               const int *__restrict__ mcell, const double *__restrict__ wts,
                                               const double *__restrict__ mv, double *__restrict__ fdrag,
                                               const double *__restrict__ mvol, double *__restrict__ fv, double dV,
-                                              double dt) {
+                                              double idV, double dt) {
     const int m = blockIdx.x * T + threadIdx.x;
     if (m >= M) return;
     const int cl = mcell[m];
@@ -92,19 +93,19 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const double *__
     for (int kk = 0; kk < 3; ++kk)
         for (int jj = 0; jj < 3; ++jj)
             for (int ii = 0; ii < 4; ++ii) {
-                const double wt = fx[ii] * cy[jj] * cz[kk] / dV;
+                const double wt = qdiv<P2>(fx[ii] * cy[jj] * cz[kk], dV, idV);
                 ulp = ulp + wt * vel_at<0>(u, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
             }
     for (int kk = 0; kk < 3; ++kk)
         for (int jj = 0; jj < 4; ++jj)
             for (int ii = 0; ii < 3; ++ii) {
-                const double wt = cx[ii] * fy[jj] * cz[kk] / dV;
+                const double wt = qdiv<P2>(cx[ii] * fy[jj] * cz[kk], dV, idV);
                 vlp = vlp + wt * vel_at<1>(v, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
             }
     for (int kk = 0; kk < 4; ++kk)
         for (int jj = 0; jj < 3; ++jj)
             for (int ii = 0; ii < 3; ++ii) {
-                const double wt = cx[ii] * cy[jj] * fz[kk] / dV;
+                const double wt = qdiv<P2>(cx[ii] * cy[jj] * fz[kk], dV, idV);
                 wlp = wlp + wt * vel_at<2>(w, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
             }
     const double f0 = fdrag[m] + 1.0 * (ulp - mv[m]) / dt;
@@ -247,9 +248,10 @@ struct SpreadArgs {
     double *fact;      // 3 x (64 * cap_tiles), tile slot major
     long fstride;      // 64 * cap_tiles
     double *u, *v, *w;
-    double dV, dt;
+    double dV, idV, dt;
 };
 
+template <bool P2>
 __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     __shared__ int s_off[NSRC + 1];
     __shared__ int s_gstart[NSRC];       // first item of the source cell in bin_items
@@ -367,9 +369,9 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
                     const int e = q - e0;
                     const int ox = i - s_cc[e][0] + 1, oy = j - s_cc[e][1] + 1, oz = k - s_cc[e][2] + 1;
                     if (ox < 0 || ox > 3 || oy < 0 || oy > 3 || oz < 0 || oz > 3) continue;
-                    const double wu = s_w[e][ox] * s_w[e][12 + oy] * s_w[e][20 + oz] / a.dV;
-                    const double wv = s_w[e][4 + ox] * s_w[e][8 + oy] * s_w[e][20 + oz] / a.dV;
-                    const double ww = s_w[e][4 + ox] * s_w[e][12 + oy] * s_w[e][16 + oz] / a.dV;
+                    const double wu = qdiv<P2>(s_w[e][ox] * s_w[e][12 + oy] * s_w[e][20 + oz], a.dV, a.idV);
+                    const double wv = qdiv<P2>(s_w[e][4 + ox] * s_w[e][8 + oy] * s_w[e][20 + oz], a.dV, a.idV);
+                    const double ww = qdiv<P2>(s_w[e][4 + ox] * s_w[e][12 + oy] * s_w[e][16 + oz], a.dV, a.idV);
                     f0 = f0 - s_fv[e][0] * wu;
                     f1 = f1 - s_fv[e][1] * wv;
                     f2 = f2 - s_fv[e][2] * ww;
@@ -452,8 +454,14 @@ int ibm_interpolate(ibmdem_ctx *c) {
     if (c->M == 0) return IBMDEM_OK;
     const int cur = c->vcur;
     prof_begin(c, IBMDEM_K_INTERP);
-    hipLaunchKernelGGL(k_interp, dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->u[cur], c->v[cur],
-                       c->w[cur], c->mcell, c->wts, c->mv, c->fdrag, c->mvol, c->fv, c->dV, c->cs.dt);
+    if (is_pow2(c->dV))
+        hipLaunchKernelGGL((k_interp<true>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->u[cur],
+                           c->v[cur], c->w[cur], c->mcell, c->wts, c->mv, c->fdrag, c->mvol, c->fv, c->dV,
+                           1.0 / c->dV, c->cs.dt);
+    else
+        hipLaunchKernelGGL((k_interp<false>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->u[cur],
+                           c->v[cur], c->w[cur], c->mcell, c->wts, c->mv, c->fdrag, c->mvol, c->fv, c->dV,
+                           1.0 / c->dV, c->cs.dt);
     prof_end(c, IBMDEM_K_INTERP, 0.0);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
@@ -476,9 +484,13 @@ int ibm_spread(ibmdem_ctx *c) {
     a.fstride = 64L * c->nact;
     a.u = c->u[cur]; a.v = c->v[cur]; a.w = c->w[cur];
     a.dV = c->dV;
+    a.idV = 1.0 / c->dV;
     a.dt = c->cs.dt;
     prof_begin(c, IBMDEM_K_SPREAD);
-    hipLaunchKernelGGL(k_spread_tiles, dim3(c->nact), dim3(64), 0, c->stream, a);
+    if (is_pow2(c->dV))
+        hipLaunchKernelGGL((k_spread_tiles<true>), dim3(c->nact), dim3(64), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_spread_tiles<false>), dim3(c->nact), dim3(64), 0, c->stream, a);
     prof_end(c, IBMDEM_K_SPREAD, 0.0);
     CHECK_LAUNCH(c);
     c->first_mdf = false;

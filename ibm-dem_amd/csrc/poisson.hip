@@ -49,37 +49,28 @@ LaunchShape shape_for(int nx, int ny, int nz) {
     return {dim3(div_up(nx, bx), div_up(ny, by), nz), dim3(bx, by, 1)};
 }
 
-template <typename T>
-__device__ __forceinline__ T nb_val(const T *__restrict__ x, const LevelDev &L, int i, int j, int k, int d,
-                                    int dir) {
-    int a[3] = {i, j, k};
-    const int n[3] = {L.nx, L.ny, L.nz};
-    int q = a[d] + dir;
-    if (q < 0) {
-        if (L.bt[2 * d] == IBMDEM_BC_WALL) return T(0);
-        q += n[d];
-    } else if (q >= n[d]) {
-        if (L.bt[2 * d + 1] == IBMDEM_BC_WALL) return T(0);
-        q -= n[d];
-    }
-    a[d] = q;
-    return x[((long)a[2] * L.ny + a[1]) * L.nx + a[0]];
-}
-
-// nbsum in the oracle's association: ((xw+xe)*ihx2 + (xs+xn)*ihy2) + (xb+xt)*ihz2
+// nbsum in the oracle's association: ((xw+xe)*ihx2 + (xs+xn)*ihy2) + (xb+xt)*ihz2,
+// wall neighbours 0, periodic neighbours wrapped; neighbour offsets are
+// selected per thread rather than taken on a divergent boundary path
 template <typename T>
 __device__ __forceinline__ T nbsum(const T *__restrict__ x, const LevelDev &L, int i, int j, int k) {
     const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
-    if (i > 0 && i < L.nx - 1 && j > 0 && j < L.ny - 1 && k > 0 && k < L.nz - 1) {
-        const long o = ((long)k * L.ny + j) * L.nx + i, sy = L.nx, sz = (long)L.nx * L.ny;
-        T sx = (x[o - 1] + x[o + 1]) * ihx2;
-        T syy = (x[o - sy] + x[o + sy]) * ihy2;
-        T szz = (x[o - sz] + x[o + sz]) * ihz2;
-        return sx + syy + szz;
-    }
-    T sx = (nb_val(x, L, i, j, k, 0, -1) + nb_val(x, L, i, j, k, 0, 1)) * ihx2;
-    T syy = (nb_val(x, L, i, j, k, 1, -1) + nb_val(x, L, i, j, k, 1, 1)) * ihy2;
-    T szz = (nb_val(x, L, i, j, k, 2, -1) + nb_val(x, L, i, j, k, 2, 1)) * ihz2;
+    const long sy = L.nx, sz = (long)L.nx * L.ny;
+    const long o = k * sz + j * sy + i;
+    const bool pxw = L.bt[0] == IBMDEM_BC_PERIODIC, pyw = L.bt[2] == IBMDEM_BC_PERIODIC,
+               pzw = L.bt[4] == IBMDEM_BC_PERIODIC;
+    const bool hw = i > 0 || pxw, he = i < L.nx - 1 || pxw;
+    const bool hs = j > 0 || pyw, hn = j < L.ny - 1 || pyw;
+    const bool hb = k > 0 || pzw, ht = k < L.nz - 1 || pzw;
+    const long ow = i > 0 ? o - 1 : o + (L.nx - 1), oe = i < L.nx - 1 ? o + 1 : o - (L.nx - 1);
+    const long os = j > 0 ? o - sy : o + (L.ny - 1) * sy, on = j < L.ny - 1 ? o + sy : o - (L.ny - 1) * sy;
+    const long ob = k > 0 ? o - sz : o + (L.nz - 1) * sz, ot = k < L.nz - 1 ? o + sz : o - (L.nz - 1) * sz;
+    const T xw = hw ? x[ow] : T(0), xe = he ? x[oe] : T(0);
+    const T xs = hs ? x[os] : T(0), xn = hn ? x[on] : T(0);
+    const T xb = hb ? x[ob] : T(0), xt = ht ? x[ot] : T(0);
+    const T sx = (xw + xe) * ihx2;
+    const T syy = (xs + xn) * ihy2;
+    const T szz = (xb + xt) * ihz2;
     return sx + syy + szz;
 }
 

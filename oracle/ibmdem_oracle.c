@@ -1475,3 +1475,17 @@ int ibmdem_memory_bytes(const ibmdem_ctx *c, int64_t *bytes) {
     *bytes = 0;
     return IBMDEM_OK;
 }
+
+/* ------------------------------------------------------------------------
+ * Test hooks (oracle only): the scalar building blocks the closed-form tests
+ * pin directly.
+ * ------------------------------------------------------------------------ */
+double ibmdem_oracle_phi(double r) { return phi1(r); }
+double ibmdem_oracle_eta(double restitution, int textbook) {
+    ibmdem_case cs;
+    memset(&cs, 0, sizeof cs);
+    cs.restitution = restitution;
+    cs.eta_textbook = textbook;
+    return eta_of(&cs);
+}
+void ibmdem_oracle_rotation(const double theta[3], double mat[9]) { rotate_matrix(theta, mat); }
