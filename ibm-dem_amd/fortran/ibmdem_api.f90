@@ -27,33 +27,35 @@ module ibmdem_api
     integer(c_int32_t), parameter :: IBMDEM_FP64 = 0
     integer(c_int32_t), parameter :: IBMDEM_MIXED = 1
 
+    ! every component defaults to 0: the library reads 0 as "use the default"
+    ! for the multigrid and tile parameters
     type, bind(C) :: ibmdem_case
-        integer(c_int32_t) :: nx, ny, nz
-        integer(c_int32_t) :: btype(6)
-        real(c_double)     :: dx, dy, dz
-        real(c_double)     :: visc, rhof, dt
-        real(c_double)     :: cp(3)
-        integer(c_int32_t) :: bulk_switch(3)
-        real(c_double)     :: bulk_target(3)
-        real(c_double)     :: gravity(3)
-        integer(c_int32_t) :: n_mdf
-        integer(c_int32_t) :: couple_interval
-        real(c_double)     :: kn_coeff
-        real(c_double)     :: restitution
-        real(c_double)     :: collision_dist
-        real(c_double)     :: collision_buffer
-        integer(c_int32_t) :: eta_textbook
-        integer(c_int32_t) :: n_markers
-        integer(c_int32_t) :: n_bodies
-        integer(c_int32_t) :: max_pairs
-        integer(c_int32_t) :: precision
-        real(c_double)     :: poisson_rtol
-        integer(c_int32_t) :: poisson_max_iter
-        integer(c_int32_t) :: mg_pre, mg_post
-        integer(c_int32_t) :: mg_coarse_sweeps
-        integer(c_int32_t) :: mg_max_levels
-        integer(c_int32_t) :: stencil_tile
-        integer(c_int32_t) :: reserved(7)
+        integer(c_int32_t) :: nx = 0, ny = 0, nz = 0
+        integer(c_int32_t) :: btype(6) = 0
+        real(c_double)     :: dx = 0d0, dy = 0d0, dz = 0d0
+        real(c_double)     :: visc = 0d0, rhof = 0d0, dt = 0d0
+        real(c_double)     :: cp(3) = 0d0
+        integer(c_int32_t) :: bulk_switch(3) = 0
+        real(c_double)     :: bulk_target(3) = 0d0
+        real(c_double)     :: gravity(3) = 0d0
+        integer(c_int32_t) :: n_mdf = 0
+        integer(c_int32_t) :: couple_interval = 0
+        real(c_double)     :: kn_coeff = 0d0
+        real(c_double)     :: restitution = 0d0
+        real(c_double)     :: collision_dist = 0d0
+        real(c_double)     :: collision_buffer = 0d0
+        integer(c_int32_t) :: eta_textbook = 0
+        integer(c_int32_t) :: n_markers = 0
+        integer(c_int32_t) :: n_bodies = 0
+        integer(c_int32_t) :: max_pairs = 0
+        integer(c_int32_t) :: precision = 0
+        real(c_double)     :: poisson_rtol = 0d0
+        integer(c_int32_t) :: poisson_max_iter = 0
+        integer(c_int32_t) :: mg_pre = 0, mg_post = 0
+        integer(c_int32_t) :: mg_coarse_sweeps = 0
+        integer(c_int32_t) :: mg_max_levels = 0
+        integer(c_int32_t) :: stencil_tile = 0
+        integer(c_int32_t) :: reserved(7) = 0
     end type
 
     ! markers (reference type Parcel), component-major SoA
