@@ -11,11 +11,16 @@ Poisson solve to rtol 1e-10, projection and 10 DEM sub-steps.
 
 The FP64 form is the headline (value = steps/s); the mixed-precision form
 (FP32 predictor and FP32 multigrid inside FP64 CG) is timed on the same
-state and reported beside it. The dominant kernel (the level-0 red-black
-smoother half-sweep) is timed live with HIP events on the context stream;
-its algorithmic bytes are 1.5 * s * cells per launch (half the cells read
-their right-hand side and the other colour, and are written). The CPU
-baseline is the oracle (one core) timed on a bounded sample.
+state and reported beside it. The timed steps run without per-launch
+events; afterwards one step per profiled kernel (spread, level-0 smoother,
+interpolation, predictor) is run with HIP events around each launch of that
+kernel on the context stream, and the kernel with the largest time per step
+is the roofline headline. Algorithmic bytes per launch are computed by the
+library: level-0 smoother 1.5 * s * cells (half the cells read their
+right-hand side and the other colour, and are written); spread
+64 * markers + 32 * 3 * footprint cells (24 in the first iteration);
+interpolation 136 * markers + 8 * 3 * footprint cells; predictor 7 * s *
+cells. The CPU baseline is the oracle (one core) timed on a bounded sample.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--precision fp64|mixed|both]
 """
@@ -45,10 +50,15 @@ def parse():
     return ap.parse_args()
 
 
-def timed_run(setup, precision, steps, warmup, profile_kernel):
+PROFILED = ("spread", "mg_smooth", "interp", "predictor")
+
+
+def timed_run(setup, precision, steps, warmup, profile):
+    """Warm-up, `steps` timed steps (no per-launch events: the step runs as
+    captured HIP graphs), the state after them, then one profiled step per
+    kernel in PROFILED with HIP events around each of its launches."""
     import ibmdem
-    from ibmdem import cases
-    from ibmdem.abi import K_MG_SMOOTH
+    from ibmdem import abi, cases
 
     case = setup.case
     case.precision = precision
@@ -58,9 +68,6 @@ def timed_run(setup, precision, steps, warmup, profile_kernel):
     for _ in range(warmup):
         iters.append(sim.step().iterations)
     sim.sync()
-    if profile_kernel:
-        sim.profile_kernel(K_MG_SMOOTH)
-    sim.sync()
     t0 = time.perf_counter()
     info = None
     for _ in range(steps):
@@ -68,11 +75,72 @@ def timed_run(setup, precision, steps, warmup, profile_kernel):
         iters.append(info.iterations)
     sim.sync()
     t1 = time.perf_counter()
-    prof = sim.profile_read() if profile_kernel else None
     state = (sim.download_fields(), sim.download_markers(), sim.download_bodies())
+    prof = {}
+    if profile:
+        ids = {"spread": abi.K_SPREAD, "mg_smooth": abi.K_MG_SMOOTH, "interp": abi.K_INTERP,
+               "predictor": abi.K_PREDICTOR}
+        for name in PROFILED:
+            sim.profile_kernel(ids[name])
+            sim.step()
+            prof[name] = sim.profile_read()     # (total ms, launches, bytes per launch)
+        sim.profile_kernel(0)
     mem = sim.memory_bytes()
     sim.close()
     return (t1 - t0), iters, info, prof, state, mem
+
+
+KERNEL_NAMES = {
+    "spread": "k_spread_tiles (IBM spread + multi-direct-forcing update)",
+    "mg_smooth": "k_rb_half (level-0 red-black Gauss-Seidel half-sweep)",
+    "interp": "k_interp (IBM interpolation)",
+    "predictor": "k_predictor (fused advection-diffusion predictor)",
+}
+
+
+def roofline_of(prof, traffic=None):
+    """Per profiled kernel: average launch time and algorithmic GB/s; the
+    dominant one (largest time per step) is the headline."""
+    rows = {}
+    for name, (total_ms, launches, bpl) in prof.items():
+        if not launches:
+            continue
+        avg_s = total_ms * 1e-3 / launches
+        achieved = bpl / avg_s / 1e9 if bpl else None
+        rows[name] = {
+            "kernel": KERNEL_NAMES[name],
+            "ms_per_step": total_ms,
+            "avg_launch_us": avg_s * 1e6,
+            "launches_per_step": launches,
+            "algorithmic_bytes_per_launch": bpl,
+            "achieved": achieved,
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
+        }
+    if not rows:
+        return None, rows
+    top = max(rows, key=lambda k: rows[k]["ms_per_step"])
+    r = rows[top]
+    head = {"kernel": r["kernel"], "bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": r["frac"], "traffic": (traffic or {}).get(top),
+            "avg_launch_us": r["avg_launch_us"], "launches_per_step": r["launches_per_step"],
+            "algorithmic_bytes_per_launch": r["algorithmic_bytes_per_launch"]}
+    return head, rows
+
+
+def measured_traffic():
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy kernel), keyed like
+    PROFILED; None where the summary has no figure."""
+    path = os.path.join(ROOT, "profiles", "coupled_step_256cube_fp64_hbm_traffic.md")
+    out = {}
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch "):
+                _, name, val = line.split()
+                out[name] = float(val)
+    except OSError:
+        pass
+    return out
 
 
 def cpu_baseline():
@@ -130,9 +198,7 @@ def main():
     head = "fp64" if "fp64" in results else "mixed"
     r = results[head]
     sps = args.steps / r["secs"]
-    total_ms, launches, bpl = r["prof"]
-    avg_ms = total_ms / max(launches, 1)
-    achieved = bpl / (avg_ms * 1e-3) / 1e9 if launches else None
+    roofline, kernels = roofline_of(r["prof"], measured_traffic() if args.n == 256 else None)
     line = {
         "metric": "steps/s",
         "value": sps,
@@ -157,18 +223,8 @@ def main():
         "gcell_per_s": cells * sps / 1e9,
         "cg_iterations_mean": float(np.mean(r["iters"][args.warmup:])) if len(r["iters"]) > args.warmup else None,
         "device_memory_gb": r["mem"] / 1e9,
-        "roofline": {
-            "kernel": "k_rb_half (level-0 red-black Gauss-Seidel half-sweep)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
-            "avg_launch_us": avg_ms * 1e3,
-            "launches": launches,
-            "algorithmic_bytes_per_launch": bpl,
-        },
+        "roofline": roofline,
+        "kernels": kernels,
     }
     if "mixed" in results and head == "fp64":
         m = results["mixed"]
@@ -177,13 +233,13 @@ def main():
         um, mm, bm = m["state"]
         def rel(a, b):
             return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
-        mt, ml, mb = m["prof"]
+        _, mk = roofline_of(m["prof"])
         line["mixed"] = {
             "steps_per_s": msps,
             "gcell_per_s": cells * msps / 1e9,
             "speedup_vs_fp64": msps / sps,
             "cg_iterations_mean": float(np.mean(m["iters"][args.warmup:])),
-            "smoother_avg_launch_us": 1e3 * mt / max(ml, 1),
+            "kernels": mk,
             "error_vs_fp64_after_run": {
                 "u": rel(um[0], uf[0]), "v": rel(um[1], uf[1]), "w": rel(um[2], uf[2]),
                 "p": rel(um[3], uf[3]), "marker_x": rel(mm.x, mf.x), "body_x": rel(bm.x, bf.x),

@@ -165,10 +165,26 @@ inline bool is_pow2(double d) {
 // ---------------------------------------------------------------------------
 // Roma-Peskin-Berger 3-point kernel, DeltaFunction3D (interpolation.F90:7-30)
 // ---------------------------------------------------------------------------
-__host__ __device__ __forceinline__ double phi1(double r) {
-    if (r <= 0.5) return (1.0 + sqrt(1.0 - 3.0 * r * r)) / 3.0;
-    if (r <= 1.5) return (5.0 - 3.0 * r - sqrt(1.0 - 3.0 * ((1.0 - r) * (1.0 - r)))) / 6.0;
+//   r <= 1/2:       (1 + sqrt(1 - 3 r^2)) / 3
+//   1/2 < r <= 3/2: (5 - 3r - sqrt(1 - 3 (1-r)^2)) / 6
+// Both branches are one square root and one division of the same shape, so
+// they are evaluated once with per-lane operands (no divergent second
+// path); each branch keeps its own operation order, so the value is the
+// branch's value bit for bit.
+// Branching form of the same function (lower register use where many
+// factors are live at once; the value is identical).
+__host__ __device__ __forceinline__ double phi1_branch(double r) {
+    if (r <= 0.5) return (1.0 + sqrt(1.0 - (3.0 * r) * r)) / 3.0;
+    if (r <= 1.5) return ((5.0 - 3.0 * r) - sqrt(1.0 - 3.0 * ((1.0 - r) * (1.0 - r)))) / 6.0;
     return 0.0;
+}
+__host__ __device__ __forceinline__ double phi1(double r) {
+    const bool inner = r <= 0.5;
+    const double q = inner ? (3.0 * r) * r : 3.0 * ((1.0 - r) * (1.0 - r));
+    const double s = sqrt(1.0 - q);
+    const double num = inner ? 1.0 + s : (5.0 - 3.0 * r) - s;
+    const double v = num / (inner ? 3.0 : 6.0);
+    return r <= 1.5 ? v : 0.0;
 }
 
 // cell of a position, PutParcelsIntoCellsDEM (ZJS_UPDATE_LIST_DEM.F90:41-43)
@@ -201,7 +217,28 @@ struct Profile {
     std::vector<hipEvent_t> ev;  // pairs
     double total_ms = 0.0;
     int64_t launches = 0;
-    double bytes_per_launch = 0.0;
+    double bytes_total = 0.0;    // algorithmic bytes summed over the launches
+};
+
+// host-side state that the captured launches depend on or change
+struct HostState {
+    int vcur, pcur;
+    bool bulk_pending, first_mdf, spread_valid, rmax_dirty;
+    bool operator==(const HostState &o) const {
+        return vcur == o.vcur && pcur == o.pcur && bulk_pending == o.bulk_pending && first_mdf == o.first_mdf &&
+               spread_valid == o.spread_valid && rmax_dirty == o.rmax_dirty;
+    }
+};
+struct CapturedPart {
+    hipGraphExec_t exec = nullptr;
+    HostState before{}, after{};
+    int cg_last = 0;             // CG iterations captured (first part only)
+};
+struct StepGraphs {
+    bool enabled = true;         // IBMDEM_NO_GRAPH=1 in the environment turns them off
+    int cg_k = -1;               // CG iterations to capture (-1: not yet known)
+    CapturedPart pre[4], post[4];
+    int64_t launches = 0;        // graph launches (for tests)
 };
 
 struct ibmdem_ctx {
@@ -241,19 +278,29 @@ struct ibmdem_ctx {
     double *mvol = nullptr, *mmass = nullptr, *mrad = nullptr;
     int *mbody = nullptr;
     int *mcell = nullptr;        // 0-based linear interior cell
-    double *wts = nullptr;       // 24 one-dimensional delta weights per marker (marker-major)
-    double *fv = nullptr;        // fdrag*vol per marker, 4 per marker
+    // per-marker record, 8 doubles (64 B): x, y, z, bits of the cell index
+    // (set once per step), fdrag*vol (3, set by every interpolation), pad
+    double *mrec = nullptr;
     // bins over interior cells
     int *bin_count = nullptr, *bin_start = nullptr, *bin_items = nullptr, *bin_tmp = nullptr;
     int *scan_blocks = nullptr;
     int scan_nblocks = 0;
-    // IBM spread footprint (per step): active 4x4x4 tiles
+    // IBM spread footprint (per step): active 4x4x4 tiles and, per tile, its
+    // source entries (marker << 9 | source cell) in (kc,jc,ic,id) order
     unsigned char *act_flag = nullptr;   // one flag per tile
     int *act_list = nullptr;
-    int nact = 0;
+    int *act_meta = nullptr;             // [0] active tiles (device-side count)
     long act_cap = 0;                    // tiles
+    long act_grid = 0;                   // launch width: act_cap rounded up to 8
+    int *tile_ecount = nullptr;          // act_cap + 1
+    int *tile_estart = nullptr;          // act_cap + 1 (exclusive scan)
+    uint32_t *tile_ent = nullptr;        // ent_cap
+    long ent_cap = 0;
+    unsigned long long *tile_mask = nullptr;  // lanes with at least one contribution
     double *fact = nullptr;              // f of the tiles' cells, 3 * 64 * act_cap
     bool first_mdf = true;
+    bool spread_valid = false;           // fact/tile_mask hold the last spread
+    long footprint_cells = 0;            // cells the spread writes (measured when profiling)
     // bodies
     int *body_start = nullptr;   // B+1
     double *bx = nullptr, *bv = nullptr, *bvold = nullptr, *bom = nullptr, *bL = nullptr, *bLold = nullptr;
@@ -268,6 +315,10 @@ struct ibmdem_ctx {
     int *pair_count = nullptr, *pair_partner = nullptr, *pair_shift = nullptr;
     int pair_cap = 0;
     Profile prof;
+    // the coupled step as two captured HIP graphs (see ctx.hip), one pair
+    // per (vcur, pcur) parity of the step
+    StepGraphs graphs;
+    int last_cg_iterations = 0;
     int64_t bytes = 0;
     std::vector<void *> allocs;
 };
@@ -322,6 +373,8 @@ void prof_end(ibmdem_ctx *c, int kid, double bytes);
 // ---------------------------------------------------------------------------
 // launchers (one per translation unit)
 // ---------------------------------------------------------------------------
+// dem.hip: exclusive scan of n counts into start[0..n] (start[n] = total)
+int scan_counts(ibmdem_ctx *c, const int *cnt, int *start, long n);
 // fluid.hip
 int fluid_predictor(ibmdem_ctx *c, double a, double b, double cc);
 int fluid_bulk(ibmdem_ctx *c, bool apply_now);
@@ -331,6 +384,11 @@ int fluid_upload_bc(ibmdem_ctx *c);
 // poisson.hip
 int poisson_setup(ibmdem_ctx *c);
 int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info);
+int poisson_begin(ibmdem_ctx *c);
+int poisson_iterations(ibmdem_ctx *c, int first, int last);
+int poisson_iterate_to_convergence(ibmdem_ctx *c, int first);
+int poisson_check(ibmdem_ctx *c, ibmdem_solve_info *info);
+int poisson_gauge(ibmdem_ctx *c);
 // ibm.hip
 int ibm_begin_step(ibmdem_ctx *c);
 int ibm_interpolate(ibmdem_ctx *c);

@@ -5,6 +5,7 @@
 #include "common.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -29,7 +30,7 @@ void prof_end(ibmdem_ctx *c, int kid, double bytes) {
     (void)hipEventRecord(e, c->stream);
     c->prof.ev.push_back(e);
     c->prof.launches++;
-    if (bytes > 0) c->prof.bytes_per_launch = bytes;
+    c->prof.bytes_total += bytes;
 }
 
 namespace {
@@ -56,6 +57,10 @@ int validate(const ibmdem_case *cs) {
     if (cs->precision != IBMDEM_FP64 && cs->precision != IBMDEM_MIXED) return IBMDEM_ERR_ARG;
     if ((long)cs->nx * cs->ny * cs->nz >= (1L << 31)) return IBMDEM_ERR_ARG;
     if (cs->n_markers > 0 && cs->n_bodies == 0) return IBMDEM_ERR_ARG;
+    // a marker's 5x5x5 neighbourhood must not wrap onto itself, and marker
+    // ids are packed in 23 bits in the spread's entry lists
+    if (cs->n_markers > 0 && (cs->nx < 5 || cs->ny < 5 || cs->nz < 5)) return IBMDEM_ERR_ARG;
+    if (cs->n_markers >= (1 << 23)) return IBMDEM_ERR_UNSUPPORTED;
     return IBMDEM_OK;
 }
 
@@ -79,14 +84,23 @@ int alloc_all(ibmdem_ctx *c) {
     const int M = c->M, B = c->B;
     A(c->mx, 3L * M); A(c->mv, 3L * M); A(c->fdrag, 3L * M); A(c->fcol, 3L * M);
     A(c->mvol, M); A(c->mmass, M); A(c->mrad, M); A(c->mbody, M); A(c->mcell, M);
-    A(c->wts, 24L * M); A(c->fv, 4L * M);
+    A(c->mrec, 8L * M);
     A(c->bin_count, n); A(c->bin_start, n + 1); A(c->bin_items, M); A(c->bin_tmp, M);
     c->scan_nblocks = div_up(n, 4096) + 2;
     A(c->scan_blocks, c->scan_nblocks);
     const long ntiles = ibm_tile_count(c->g);
     A(c->act_flag, ntiles);
     c->act_cap = std::min<long>(ntiles, 8L * M + 8);   // a marker reaches at most 2x2x2 tiles
+    c->act_grid = (c->act_cap + 7) / 8 * 8;
     A(c->act_list, c->act_cap);
+    A(c->act_meta, 8);
+    A(c->tile_ecount, c->act_cap + 1);
+    A(c->tile_estart, c->act_cap + 1);
+    A(c->tile_mask, c->act_cap);
+    // a cell lies in the source window of at most 3 tiles per dimension
+    // (2 directly, 1 more through a wall mirror or periodic wrap)
+    c->ent_cap = 27L * M + 64;
+    A(c->tile_ent, c->ent_cap);
     A(c->fact, 3 * 64 * c->act_cap);
     A(c->body_start, B + 1);
     A(c->bx, 3L * B); A(c->bv, 3L * B); A(c->bvold, 3L * B); A(c->bom, 3L * B); A(c->bL, 3L * B);
@@ -119,6 +133,136 @@ int finish(ibmdem_ctx *c) {
 #define CTX_OR_ARG(c) if (!(c)) return IBMDEM_ERR_ARG
 #define TRY(expr) do { int rc_ = (expr); if (rc_) return rc_; } while (0)
 
+namespace {
+
+// ---------------------------------------------------------------------------
+// The coupled step. Eagerly it is a sequence of launches with two host waits
+// inside the Poisson solve (the lagged convergence test) and one at the end
+// (the DEM error flags). Once the number of CG iterations a step needs is
+// known, the step runs as two captured HIP graphs instead:
+//   pre  = IBM begin-step tables, the multi-direct-forcing loop, predictor,
+//          divergence, the start of the solve and K CG iterations;
+//   post = gauge, projection and the particle sub-steps.
+// The host waits once between them to read the convergence flag (and runs
+// further iterations eagerly if K did not suffice, raising K for the next
+// step) and once after post for the error flags. The captured launches are
+// the eager ones, so the results are bitwise the same. The predictor and the
+// projection swap ping-pong buffers, so a step's launches depend on the
+// (vcur, pcur) parity: each parity has its own pair of graphs.
+// ---------------------------------------------------------------------------
+HostState host_state(const ibmdem_ctx *c) {
+    return HostState{c->vcur, c->pcur, c->bulk_pending, c->first_mdf, c->spread_valid, c->rmax_dirty};
+}
+void set_host_state(ibmdem_ctx *c, const HostState &h) {
+    c->vcur = h.vcur; c->pcur = h.pcur;
+    c->bulk_pending = h.bulk_pending; c->first_mdf = h.first_mdf;
+    c->spread_valid = h.spread_valid; c->rmax_dirty = h.rmax_dirty;
+}
+
+int step_pre(ibmdem_ctx *c) {
+    TRY(ibm_begin_step(c));
+    for (int it = 0; it < c->cs.n_mdf; ++it) {
+        TRY(ibm_interpolate(c));
+        TRY(ibm_spread(c));
+    }
+    TRY(fluid_predictor(c, 1.0, 0.0, 1.0));
+    TRY(fluid_bulk(c, false));
+    TRY(fluid_divergence(c, 1.0));
+    return poisson_begin(c);
+}
+
+int step_post(ibmdem_ctx *c) {
+    TRY(poisson_gauge(c));
+    TRY(fluid_project(c, 1.0));
+    TRY(dem_body_hydro(c));
+    for (int s = 0; s < c->cs.couple_interval; ++s) TRY(dem_substep_fused(c, s == c->cs.couple_interval - 1));
+    return IBMDEM_OK;
+}
+
+void drop_part(CapturedPart &p) {
+    if (p.exec) (void)hipGraphExecDestroy(p.exec);
+    p = CapturedPart();
+}
+
+template <class F>
+int capture_part(ibmdem_ctx *c, CapturedPart &part, F launches) {
+    drop_part(part);
+    part.before = host_state(c);
+    hipGraph_t graph = nullptr;
+    HIPX(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launches();
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    part.after = host_state(c);
+    set_host_state(c, part.before);   // run_part applies `after`
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc ? rc : set_error(c, IBMDEM_ERR_HIP, std::string("step capture: ") + hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&part.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        part.exec = nullptr;
+        return set_error(c, IBMDEM_ERR_HIP, std::string("step graph instantiation: ") + hipGetErrorString(ei));
+    }
+    return IBMDEM_OK;
+}
+
+int run_part(ibmdem_ctx *c, CapturedPart &part) {
+    HIPX(c, hipGraphLaunch(part.exec, c->stream));
+    set_host_state(c, part.after);
+    c->graphs.launches++;
+    return IBMDEM_OK;
+}
+
+int step_graph(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    StepGraphs &G = c->graphs;
+    const int K = G.cg_k;
+    CapturedPart &pre = G.pre[c->vcur * 2 + c->pcur];
+    if (!pre.exec || !(pre.before == host_state(c)) || pre.cg_last != K) {
+        TRY(capture_part(c, pre, [&] {
+            int rc = step_pre(c);
+            return rc ? rc : poisson_iterations(c, 1, K);
+        }));
+        pre.cg_last = K;
+    }
+    TRY(run_part(c, pre));
+    HIPX(c, hipMemcpyAsync(c->conv_host, c->scal + SC_CONV, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    if (c->conv_host[0] == 0.0) TRY(poisson_iterate_to_convergence(c, K + 1));
+    TRY(poisson_check(c, info));
+    if (c->last_cg_iterations > G.cg_k) G.cg_k = c->last_cg_iterations;
+    CapturedPart &post = G.post[c->vcur * 2 + c->pcur];
+    if (!post.exec || !(post.before == host_state(c))) TRY(capture_part(c, post, [&] { return step_post(c); }));
+    TRY(run_part(c, post));
+    return dem_check_flags(c);
+}
+
+int step_eager(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    TRY(step_pre(c));
+    TRY(poisson_iterate_to_convergence(c, 1));
+    TRY(poisson_check(c, info));
+    if (c->last_cg_iterations > c->graphs.cg_k) c->graphs.cg_k = c->last_cg_iterations;
+    TRY(step_post(c));
+    return dem_check_flags(c);
+}
+
+int step_impl(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    // graphs need a known iteration count, no per-launch profiling events and
+    // the steady host state (a fresh marker upload recomputes body radii once)
+    if (c->graphs.enabled && c->graphs.cg_k >= 0 && c->prof.kernel == 0 && !c->rmax_dirty)
+        return step_graph(c, info);
+    return step_eager(c, info);
+}
+
+void drop_graphs(ibmdem_ctx *c) {
+    for (int q = 0; q < 4; ++q) {
+        drop_part(c->graphs.pre[q]);
+        drop_part(c->graphs.post[q]);
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
@@ -146,6 +290,8 @@ int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
     c->eta = eta_of(&c->cs);
     c->M = cs->n_markers;
     c->B = cs->n_bodies;
+    const char *ng = std::getenv("IBMDEM_NO_GRAPH");
+    c->graphs.enabled = !(ng && *ng && *ng != '0');
     if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return IBMDEM_ERR_HIP;
@@ -163,6 +309,7 @@ int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
 int ibmdem_destroy(ibmdem_ctx *c) {
     if (!c) return IBMDEM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     for (void *p : c->allocs) (void)hipFree(p);
     for (hipEvent_t e : c->prof.ev) (void)hipEventDestroy(e);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
@@ -343,22 +490,6 @@ int ibmdem_cloud_evolution(ibmdem_ctx *c) {
     return dem_check_flags(c);
 }
 
-static int step_impl(ibmdem_ctx *c, ibmdem_solve_info *info) {
-    TRY(ibm_begin_step(c));
-    for (int it = 0; it < c->cs.n_mdf; ++it) {
-        TRY(ibm_interpolate(c));
-        TRY(ibm_spread(c));
-    }
-    TRY(fluid_predictor(c, 1.0, 0.0, 1.0));
-    TRY(fluid_bulk(c, false));
-    TRY(fluid_divergence(c, 1.0));
-    TRY(poisson_solve(c, info));
-    TRY(fluid_project(c, 1.0));
-    TRY(dem_body_hydro(c));
-    for (int s = 0; s < c->cs.couple_interval; ++s) TRY(dem_substep_fused(c, s == c->cs.couple_interval - 1));
-    return dem_check_flags(c);
-}
-
 int ibmdem_step(ibmdem_ctx *c, ibmdem_solve_info *info) {
     CTX_OR_ARG(c);
     return step_impl(c, info);
@@ -421,7 +552,7 @@ int ibmdem_profile_read(ibmdem_ctx *c, double *total_ms, int64_t *launches, doub
     }
     if (total_ms) *total_ms = tot;
     if (launches) *launches = c->prof.launches;
-    if (bytes_per_launch) *bytes_per_launch = c->prof.bytes_per_launch;
+    if (bytes_per_launch) *bytes_per_launch = c->prof.launches ? c->prof.bytes_total / c->prof.launches : 0.0;
     return IBMDEM_OK;
 }
 

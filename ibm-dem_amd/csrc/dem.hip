@@ -554,6 +554,19 @@ ContactParams contact_params(ibmdem_ctx *c) {
 
 }  // namespace
 
+int scan_counts(ibmdem_ctx *c, const int *cnt, int *start, long n) {
+    hipStream_t s = c->stream;
+    const int nb = div_up(n, CHUNK);
+    if (nb + 1 > c->scan_nblocks) return set_error(c, IBMDEM_ERR_CAPACITY, "scan longer than its block buffer");
+    hipLaunchKernelGGL(k_chunk_sum, dim3(nb), dim3(T), 0, s, n, cnt, c->scan_blocks);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, nb, c->scan_blocks);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(nb), dim3(T), 0, s, n, cnt, c->scan_blocks, start, nb);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
 int dem_rebuild(ibmdem_ctx *c) {
     const int M = c->M;
     if (M == 0) return IBMDEM_OK;

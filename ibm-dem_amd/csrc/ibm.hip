@@ -3,26 +3,32 @@
 // interpolation.F90:2-33) with the multi-direct-forcing velocity update
 // (fluid.F90:135-153) fused into the spread.
 //
-// Markers do not move during the fluid part of a step, so at the start of a
-// step every marker gets its 24 one-dimensional delta factors (per
-// dimension: staggered-face and cell-centre variants at cell offsets
-// -1..+2 from its own cell), stored marker-major, and the spread footprint
-// is compacted into a sorted list of active 4x4x4 cell tiles.
+// Markers do not move during the fluid part of a step, so once per step:
+//   - every marker gets a 64-byte record (position, cell; the interpolation
+//     adds fdrag*vol to it each iteration), so the spread reads one line per
+//     marker instead of three SoA arrays;
+//   - the spread footprint is compacted into a sorted list of active 4x4x4
+//     cell tiles, and each active tile gets the list of its source entries:
+//     the markers of its 7x7x7 source cells in source order (kc,jc,ic) and
+//     ascending id inside a cell, each packed as marker << 9 | source cell.
+// The delta factors (per dimension: staggered-face and cell-centre variants
+// at cell offsets -1..+2) are recomputed from the record where they are used;
+// the arithmetic is the same each time, so the results are too.
 //
 // Interpolation is marker-parallel and sums the taps in the reference's
 // (kk,jj,ii) order, so it is bit-identical to the reference's 125-cell loop
 // (zero-weight taps add an exact zero there).
 //
 // Spreading is a deterministic gather, no float atomics. One wavefront owns
-// a 4x4x4 tile of target cells. It stages the markers of the tile's 7x7x7
-// source cells in LDS (their delta factors, fdrag*vol and cell), in source
-// order (kc,jc,ic) and ascending id inside a cell; the reference's periodic
-// ghost copies (x + coordShift) and wall-mirror copies (2*coordMirror - x,
-// fdrag unnegated; ZJS_MPI_DEM.F90:173-269) enter as wrapped or mirrored
-// source cells with their factors evaluated at the image position. Each lane
-// then sums, for its own cell, f = 0 - sum (fdrag*vol)*w over its 4x4x4
-// source cells in that order, and applies u += dt*(f - f_old). The order a
-// cell's contributions are summed in does not depend on the tile size.
+// a tile; it stages chunks of its entry list in LDS (delta factors, fdrag*vol
+// and cell of the marker or of its image) and each lane sums, for its own
+// cell, f = 0 - sum (fdrag*vol)*w over the entries in list order. The
+// reference's periodic ghost copies (x + coordShift) and wall-mirror copies
+// (2*coordMirror - x, fdrag unnegated; ZJS_MPI_DEM.F90:173-269) enter as
+// wrapped or mirrored source cells with their factors evaluated at the image
+// position. Each lane then applies u += dt*(f - f_old) on its cell; lanes
+// whose cell no entry reaches are skipped entirely (their f is structurally
+// zero in every iteration of the step).
 #include "common.hpp"
 
 namespace {
@@ -32,62 +38,84 @@ constexpr int CHUNK = 4096;   // flags per compaction block
 constexpr int TL = 4;         // tile edge (cells)
 constexpr int SRC = TL + 3;   // source cells per tile edge
 constexpr int NSRC = SRC * SRC * SRC;
-constexpr int CAP = 64;       // LDS marker entries per pass (one per lane)
+constexpr int CAP = 64;       // LDS entries per pass (one per lane)
+constexpr int WST2 = 13;      // LDS row stride of the 12 factor pairs (odd: fewer bank conflicts)
+constexpr int SBITS = 9;      // source-cell bits of an entry (NSRC <= 512)
+static_assert(NSRC <= (1 << SBITS), "source cell index must fit its bits");
 
-__device__ __forceinline__ double wt_face(double xp, int i, double d) {
+// one-dimensional delta factors of a marker at xp for cell i: staggered
+// face (x = (i-1)d) and cell centre (x = (i-1/2)d), GetWeight
+// (ZJS_INTERPOLATE_DEM.F90:12-14); id = 1/d, used when d is a power of two
+template <bool P2, bool BR = false>
+__device__ __forceinline__ double wt_face(double xp, int i, double d, double id) {
     const double xc = ((double)i - 0.5) * d;
     const double dl = xp - xc;
-    return phi1(fabs((dl + 0.5 * d) / d));
+    const double r = fabs(qdiv<P2>(dl + 0.5 * d, d, id));
+    return BR ? phi1_branch(r) : phi1(r);
 }
-__device__ __forceinline__ double wt_cent(double xp, int i, double d) {
+template <bool P2, bool BR = false>
+__device__ __forceinline__ double wt_cent(double xp, int i, double d, double id) {
     const double xc = ((double)i - 0.5) * d;
     const double dl = xp - xc;
-    return phi1(fabs(dl / d));
+    const double r = fabs(qdiv<P2>(dl, d, id));
+    return BR ? phi1_branch(r) : phi1(r);
 }
 
-// weights layout (marker-major): wts[m*24 + dim*8 + variant*4 + o],
-// variant 0 staggered face, 1 cell centre, o = 0..3 for cells ci-1..ci+2
-__global__ __launch_bounds__(T) void k_weights(int M, GridDev g, const double *__restrict__ mx,
-                                               const int *__restrict__ mcell, double *__restrict__ wts,
-                                               double *__restrict__ fdrag) {
+// Active-tile slot of block b. The hardware hands block b to XCD b % 8; XCD x
+// gets the contiguous slots [x*per, (x+1)*per) so tiles that share source
+// markers share one L2. Returns -1 for blocks beyond the active count.
+__device__ __forceinline__ int tile_slot(int b, int nact) {
+    const int per = (nact + 7) >> 3;
+    const int x = b & 7, q = b >> 3;
+    if (q >= per) return -1;
+    const int slot = x * per + q;
+    return slot < nact ? slot : -1;
+}
+
+// per-step marker record: position, cell; zero fdrag
+__global__ __launch_bounds__(T) void k_records(int M, const double *__restrict__ mx, const int *__restrict__ mcell,
+                                               double *__restrict__ rec, double *__restrict__ fdrag) {
     const int m = blockIdx.x * T + threadIdx.x;
     if (m >= M) return;
-    const int cl = mcell[m];
-    const int cc[3] = {cl % g.nx + 1, (cl / g.nx) % g.ny + 1, cl / (g.nx * g.ny) + 1};
-    const double dd[3] = {g.dx, g.dy, g.dz};
-    double *w = wts + 24L * m;
-    for (int d = 0; d < 3; ++d) {
-        const double xp = mx[(long)d * M + m];
-        for (int o = 0; o < 4; ++o) {
-            const int cell = cc[d] + o - 1;
-            w[d * 8 + o] = wt_face(xp, cell, dd[d]);
-            w[d * 8 + 4 + o] = wt_cent(xp, cell, dd[d]);
-        }
-        fdrag[(long)d * M + m] = 0.0;
-    }
+    double *r = rec + 8L * m;
+    reinterpret_cast<double2 *>(r)[0] = make_double2(mx[m], mx[(long)M + m]);
+    reinterpret_cast<double2 *>(r)[1] = make_double2(mx[2L * M + m], __longlong_as_double((long long)mcell[m]));
+    for (int d = 0; d < 3; ++d) fdrag[(long)d * M + m] = 0.0;
 }
 
 // UpdateFluidVelocityOnLagrangianPoints: U_p = sum w u dx dy dz,
-// fdrag += (U_p - v_p)/dt; also stores fdrag*vol for the spread
+// fdrag += (U_p - v_p)/dt; also stores fdrag*vol in the marker record
 template <bool P2>
 __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const double *__restrict__ u,
                                               const double *__restrict__ v, const double *__restrict__ w,
-                                              const int *__restrict__ mcell, const double *__restrict__ wts,
-                                              const double *__restrict__ mv, double *__restrict__ fdrag,
-                                              const double *__restrict__ mvol, double *__restrict__ fv, double dV,
+                                              double *__restrict__ rec, const double *__restrict__ mv,
+                                              double *__restrict__ fdrag, const double *__restrict__ mvol, double dV,
                                               double idV, double dt) {
     const int m = blockIdx.x * T + threadIdx.x;
     if (m >= M) return;
-    const int cl = mcell[m];
+    double *r = rec + 8L * m;
+    const double2 r01 = reinterpret_cast<const double2 *>(r)[0];
+    const double2 r23 = reinterpret_cast<const double2 *>(r)[1];
+    const double xp[3] = {r01.x, r01.y, r23.x};
+    const int cl = (int)__double_as_longlong(r23.y);
     const int ci = cl % g.nx + 1, cj = (cl / g.nx) % g.ny + 1, ck = cl / (g.nx * g.ny) + 1;
-    const double *wm = wts + 24L * m;
-    double fx[4], cx[4], fy[4], cy[4], fz[4], cz[4];
-    for (int o = 0; o < 4; ++o) {
-        fx[o] = wm[o]; cx[o] = wm[4 + o];
-        fy[o] = wm[8 + o]; cy[o] = wm[12 + o];
-        fz[o] = wm[16 + o]; cz[o] = wm[20 + o];
-    }
     const double dx = g.dx, dy = g.dy, dz = g.dz;
+    const double idx = 1.0 / dx, idy = 1.0 / dy, idz = 1.0 / dz;
+    // centre factors at offset +2 are always 0 (the marker lies in cell c,
+    // so that centre is at least 3/2 cells away) and are not used below
+    double fx[4], cx[3], fy[4], cy[3], fz[4], cz[3];
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        fx[o] = wt_face<P2, true>(xp[0], ci + o - 1, dx, idx);
+        fy[o] = wt_face<P2, true>(xp[1], cj + o - 1, dy, idy);
+        fz[o] = wt_face<P2, true>(xp[2], ck + o - 1, dz, idz);
+    }
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        cx[o] = wt_cent<P2, true>(xp[0], ci + o - 1, dx, idx);
+        cy[o] = wt_cent<P2, true>(xp[1], cj + o - 1, dy, idy);
+        cz[o] = wt_cent<P2, true>(xp[2], ck + o - 1, dz, idz);
+    }
     double ulp = 0.0, vlp = 0.0, wlp = 0.0;
     // u: x-face taps -1..2, y/z centre taps -1..1 (the nonzero ranges)
     for (int kk = 0; kk < 3; ++kk)
@@ -115,9 +143,8 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const double *__
     fdrag[(long)M + m] = f1;
     fdrag[2L * M + m] = f2;
     const double vol = mvol[m];
-    fv[4L * m + 0] = f0 * vol;
-    fv[4L * m + 1] = f1 * vol;
-    fv[4L * m + 2] = f2 * vol;
+    reinterpret_cast<double2 *>(r)[2] = make_double2(f0 * vol, f1 * vol);
+    r[6] = f2 * vol;
 }
 
 struct TileGeom {
@@ -134,11 +161,9 @@ __global__ __launch_bounds__(T) void k_mark_tiles(int M, GridDev g, TileGeom tg,
     const int n[3] = {g.nx, g.ny, g.nz};
     const int nt[3] = {tg.ntx, tg.nty, tg.ntz};
     int lo[3], hi[3];
-    bool wrap[3];
     for (int d = 0; d < 3; ++d) {
         int a = cc[d] - 1, b = cc[d] + 2;
-        wrap[d] = g.bt[2 * d] == IBMDEM_BC_PERIODIC;
-        if (!wrap[d]) { a = a < 1 ? 1 : a; b = b > n[d] ? n[d] : b; }
+        if (g.bt[2 * d] != IBMDEM_BC_PERIODIC) { a = a < 1 ? 1 : a; b = b > n[d] ? n[d] : b; }
         lo[d] = a; hi[d] = b;
     }
     for (int k = lo[2]; k <= hi[2]; ++k) {
@@ -169,8 +194,8 @@ __global__ __launch_bounds__(T) void k_flag_count(long n, const unsigned char *_
     if (threadIdx.x == 0) cnt[blockIdx.x] = red[0];
 }
 
-// exclusive scan of nb ints in one block; total into a[nb]
-__global__ __launch_bounds__(1024) void k_scan_small(int nb, int *__restrict__ a) {
+// exclusive scan of nb ints in one block; total into a[nb] and *total
+__global__ __launch_bounds__(1024) void k_scan_small(int nb, int *__restrict__ a, int *__restrict__ total) {
     __shared__ int buf[1024];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
@@ -191,7 +216,7 @@ __global__ __launch_bounds__(1024) void k_scan_small(int nb, int *__restrict__ a
         if (threadIdx.x == 1023) carry += buf[1023];
         __syncthreads();
     }
-    if (threadIdx.x == 0) a[nb] = carry;
+    if (threadIdx.x == 0) { a[nb] = carry; *total = carry; }
 }
 
 // ordered compaction of set flags into list; clears the flags
@@ -219,198 +244,333 @@ __global__ __launch_bounds__(T) void k_flag_compact(long n, unsigned char *__res
         }
 }
 
-__global__ __launch_bounds__(T) void k_zero(long n, double *__restrict__ a) {
-    for (long q = (long)blockIdx.x * T + threadIdx.x; q < n; q += (long)gridDim.x * T) a[q] = 0.0;
-}
-
 // one source-cell coordinate mapped to the interior with its image kind:
-// 0 direct, 1 periodic (image = x + sh), 2 mirror (image = 2*sh - x)
-__device__ __forceinline__ bool src_map(int &i, int n, int lo, int hi, double L, int &kind, double &sh) {
-    kind = 0; sh = 0.0;
+// 0 direct, 1 periodic (image = x + coordShift), 2 wall mirror
+// (image = 2*coordMirror - x)
+__device__ __forceinline__ bool src_map(int &i, int n, int lo, int hi, int &kind) {
+    kind = 0;
     if (i < 1) {
-        if (lo == IBMDEM_BC_PERIODIC) { i += n; kind = 1; sh = -L; return true; }
-        i = 1 - i; kind = 2; sh = 0.0; return i >= 1 && i <= n;
+        if (lo == IBMDEM_BC_PERIODIC) { i += n; kind = 1; return true; }
+        i = 1 - i; kind = 2; return i >= 1 && i <= n;
     }
     if (i > n) {
-        if (hi == IBMDEM_BC_PERIODIC) { i -= n; kind = 1; sh = L; return true; }
-        i = 2 * n + 1 - i; kind = 2; sh = L; return i >= 1 && i <= n;
+        if (hi == IBMDEM_BC_PERIODIC) { i -= n; kind = 1; return true; }
+        i = 2 * n + 1 - i; kind = 2; return i >= 1 && i <= n;
     }
     return true;
 }
 
-struct SpreadArgs {
-    GridDev g;
-    TileGeom tg;
-    int M;
-    const int *tiles;
-    const int *bin_start, *bin_items;
-    const double *mx, *wts, *fv;
-    double *fact;      // 3 x (64 * cap_tiles), tile slot major
-    long fstride;      // 64 * cap_tiles
-    double *u, *v, *w;
-    double dV, idV, dt;
-};
+// 1-based unwrapped coordinates of source cell s of the tile with origin
+// (ib, jb, kb) (the cell before the tile's first cell, per dimension)
+__device__ __forceinline__ void src_coords(int s, int ib, int jb, int kb, int scc[3]) {
+    scc[0] = ib + s % SRC;
+    scc[1] = jb + (s / SRC) % SRC;
+    scc[2] = kb + s / (SRC * SRC);
+}
 
-template <bool P2>
-__global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
-    __shared__ int s_off[NSRC + 1];
-    __shared__ int s_gstart[NSRC];       // first item of the source cell in bin_items
-    __shared__ int s_kind[NSRC];         // image kinds, 2 bits per dimension
-    __shared__ double s_w[CAP][24];
-    __shared__ double s_fv[CAP][3];
-    __shared__ int s_cc[CAP][3];
-    const GridDev &g = a.g;
+// Per-source-cell bin ranges of a tile: first item of each of the NSRC
+// source cells in s_gstart, the count prefix in s_off (s_off[NSRC] = total).
+__device__ __forceinline__ void tile_bins(const GridDev &g, int ib, int jb, int kb, const int *__restrict__ bin_start,
+                                          int *s_off, int *s_gstart) {
     const int lane = threadIdx.x;
-    const int tile = a.tiles[blockIdx.x];
-    const int tx = tile % a.tg.ntx, ty = (tile / a.tg.ntx) % a.tg.nty, tz = tile / (a.tg.ntx * a.tg.nty);
-    const int ib = tx * TL - 1, jb = ty * TL - 1, kb = tz * TL - 1;   // 1-based source origin
-    const double L3[3] = {g.L[0], g.L[1], g.L[2]};
-    // 1. source cells: first item, image kind, marker count. The bin
-    // boundaries of all of the lane's cells are loaded before any is used.
     constexpr int SPL = (NSRC + 63) / 64;
-    int sbin[SPL], skind[SPL], sfirst[SPL], slast[SPL];
+    int sbin[SPL];
 #pragma unroll
     for (int r = 0; r < SPL; ++r) {
         const int s = lane + 64 * r;
-        sbin[r] = -1; skind[r] = 0;
+        sbin[r] = -1;
         if (s < NSRC) {
-            int ic = ib + s % SRC, jc = jb + (s / SRC) % SRC, kc = kb + s / (SRC * SRC);
-            int ki, kj, kk; double si, sj, sk;
-            if (src_map(ic, g.nx, g.bt[0], g.bt[1], L3[0], ki, si) && src_map(jc, g.ny, g.bt[2], g.bt[3], L3[1], kj, sj) &&
-                src_map(kc, g.nz, g.bt[4], g.bt[5], L3[2], kk, sk)) {
-                sbin[r] = (int)cidx(g, ic, jc, kc);
-                skind[r] = ki | (kj << 2) | (kk << 4);
-            }
+            int scc[3];
+            src_coords(s, ib, jb, kb, scc);
+            int ki, kj, kk;
+            if (src_map(scc[0], g.nx, g.bt[0], g.bt[1], ki) && src_map(scc[1], g.ny, g.bt[2], g.bt[3], kj) &&
+                src_map(scc[2], g.nz, g.bt[4], g.bt[5], kk))
+                sbin[r] = (int)cidx(g, scc[0], scc[1], scc[2]);
         }
     }
+    // all bin boundaries are loaded before any is used
+    int sfirst[SPL], slast[SPL];
 #pragma unroll
     for (int r = 0; r < SPL; ++r) {
-        sfirst[r] = sbin[r] >= 0 ? a.bin_start[sbin[r]] : 0;
-        slast[r] = sbin[r] >= 0 ? a.bin_start[sbin[r] + 1] : 0;
+        sfirst[r] = sbin[r] >= 0 ? bin_start[sbin[r]] : 0;
+        slast[r] = sbin[r] >= 0 ? bin_start[sbin[r] + 1] : 0;
     }
 #pragma unroll
     for (int r = 0; r < SPL; ++r) {
         const int s = lane + 64 * r;
         if (s < NSRC) {
             s_gstart[s] = sfirst[r];
-            s_kind[s] = skind[r];
             s_off[s + 1] = slast[r] - sfirst[r];
         }
     }
     __syncthreads();
-    {
-        // prefix sum of the NSRC counts: PER consecutive per lane, then a
-        // wavefront scan of the lane totals
-        constexpr int PER = (NSRC + 63) / 64;
-        const int b0 = lane * PER;
-        int loc = 0;
-        for (int q = 0; q < PER; ++q)
-            if (b0 + q < NSRC) loc += s_off[b0 + q + 1];
-        int inc = loc;
-        for (int st = 1; st < 64; st <<= 1) {
-            const int t = __shfl_up(inc, st, 64);
-            if (lane >= st) inc += t;
-        }
-        int run = inc - loc;
-        for (int q = 0; q < PER; ++q)
-            if (b0 + q < NSRC) { run += s_off[b0 + q + 1]; s_off[b0 + q + 1] = run; }
-        if (lane == 0) s_off[0] = 0;
+    // prefix sum of the NSRC counts: PER consecutive per lane, then a
+    // wavefront scan of the lane totals
+    constexpr int PER = (NSRC + 63) / 64;
+    const int b0 = lane * PER;
+    int loc = 0;
+    for (int q = 0; q < PER; ++q)
+        if (b0 + q < NSRC) loc += s_off[b0 + q + 1];
+    int inc = loc;
+    for (int st = 1; st < 64; st <<= 1) {
+        const int t = __shfl_up(inc, st, 64);
+        if (lane >= st) inc += t;
     }
+    int run = inc - loc;
+    for (int q = 0; q < PER; ++q)
+        if (b0 + q < NSRC) { run += s_off[b0 + q + 1]; s_off[b0 + q + 1] = run; }
+    if (lane == 0) s_off[0] = 0;
     __syncthreads();
-    const int total = s_off[NSRC];
+}
+
+__device__ __forceinline__ void tile_origin(int tile, TileGeom tg, int &tx, int &ty, int &tz) {
+    tx = tile % tg.ntx;
+    ty = (tile / tg.ntx) % tg.nty;
+    tz = tile / (tg.ntx * tg.nty);
+}
+
+// entries per active tile; slots nact..cap-1 get 0 (slot q is written by
+// block q / 64 whatever that block's own tile is, so each slot has one writer)
+__global__ __launch_bounds__(64) void k_tile_count(GridDev g, TileGeom tg, const int *__restrict__ meta,
+                                                   const int *__restrict__ tiles, const int *__restrict__ bin_start,
+                                                   int *__restrict__ ecount, long cap) {
+    __shared__ int s_off[NSRC + 1];
+    __shared__ int s_gstart[NSRC];
+    const int nact = meta[0];
+    const long q = (long)blockIdx.x * 64 + threadIdx.x;
+    if (q >= nact && q < cap) ecount[q] = 0;
+    const int b = tile_slot(blockIdx.x, nact);
+    if (b < 0) return;
+    int tx, ty, tz;
+    tile_origin(tiles[b], tg, tx, ty, tz);
+    tile_bins(g, tx * TL - 1, ty * TL - 1, tz * TL - 1, bin_start, s_off, s_gstart);
+    if (threadIdx.x == 0) ecount[b] = s_off[NSRC];
+}
+
+// write each active tile's entries at its scanned offset
+__global__ __launch_bounds__(64) void k_tile_fill(GridDev g, TileGeom tg, const int *__restrict__ meta,
+                                                  const int *__restrict__ tiles, const int *__restrict__ bin_start,
+                                                  const int *__restrict__ bin_items, const int *__restrict__ estart,
+                                                  uint32_t *__restrict__ ent, long ent_cap, int *__restrict__ flags) {
+    __shared__ int s_off[NSRC + 1];
+    __shared__ int s_gstart[NSRC];
+    const int b = tile_slot(blockIdx.x, meta[0]);
+    if (b < 0) return;
+    int tx, ty, tz;
+    tile_origin(tiles[b], tg, tx, ty, tz);
+    tile_bins(g, tx * TL - 1, ty * TL - 1, tz * TL - 1, bin_start, s_off, s_gstart);
+    const long base = estart[b];
+    if (base + s_off[NSRC] > ent_cap) {
+        if (threadIdx.x == 0) atomicOr(&flags[FL_CAPACITY], 1);
+        return;
+    }
+    for (int s = threadIdx.x; s < NSRC; s += 64) {
+        const int n = s_off[s + 1] - s_off[s];
+        const int g0 = s_gstart[s];
+        for (int q = 0; q < n; ++q)
+            ent[base + s_off[s] + q] = ((uint32_t)bin_items[g0 + q] << SBITS) | (uint32_t)s;
+    }
+}
+
+struct SpreadArgs {
+    GridDev g;
+    TileGeom tg;
+    const int *meta;
+    const int *tiles;
+    const int *estart;
+    const uint32_t *ent;
+    long ent_cap;
+    const double *rec;
+    double *fact;      // 3 x (64 * cap_tiles), tile slot major
+    long fstride;      // 64 * cap_tiles
+    unsigned long long *mask;
+    double *u, *v, *w;
+    double dV, idV, dt;
+    int first;         // first MDF iteration of the step: f_old = 0
+};
+
+// The block is one wavefront: its lanes see each other's LDS writes in
+// program order, so a compiler-level barrier replaces __syncthreads (which
+// would also wait for the global loads the pipeline keeps in flight).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// one lane's input for one staged entry. The packed entry id is kept raw
+// until its records are loaded (one pipeline stage later), so issuing the
+// load never waits for it.
+struct EntryIn {
+    double x0, x1, x2, f0, f1, f2;
+    uint32_t en;    // marker << SBITS | source cell
+    bool valid;     // the lane stages an entry
+};
+
+__device__ __forceinline__ void load_entry_id(const SpreadArgs &a, int e, bool valid, EntryIn &in) {
+    in.valid = valid;
+    in.en = valid ? a.ent[e] : 0u;
+}
+__device__ __forceinline__ void load_entry_rec(const SpreadArgs &a, EntryIn &in) {
+    if (in.valid) {
+        const long m = (long)(in.en >> SBITS);
+        const double2 *r = reinterpret_cast<const double2 *>(a.rec + 8 * m);
+        const double2 r01 = r[0], r23 = r[1], r45 = r[2];
+        in.x0 = r01.x; in.x1 = r01.y; in.x2 = r23.x;
+        in.f0 = r45.x; in.f1 = r45.y; in.f2 = a.rec[8 * m + 6];
+    }
+}
+
+// delta factors, cell and fdrag*vol of the lane's entry into LDS row `lane`
+template <bool P2>
+__device__ __forceinline__ void stage_entry(const SpreadArgs &a, const EntryIn &in, int ib, int jb, int kb,
+                                            double2 (*s_w)[WST2], int4 *s_cc, double4 *s_fv) {
+    const GridDev &g = a.g;
+    const int lane = threadIdx.x;
+    if (!in.valid) return;
+    const int src = (int)(in.en & ((1u << SBITS) - 1));
+    const double L3[3] = {g.L[0], g.L[1], g.L[2]};
+    const double dd[3] = {g.dx, g.dy, g.dz};
+    const double id3[3] = {1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz};
+    const int n3[3] = {g.nx, g.ny, g.nz};
+    const double xm[3] = {in.x0, in.x1, in.x2};
+    // with a power-of-two cell volume the 1/dV of every weight is an exact
+    // scaling, folded into fdrag*vol here once per entry (bitwise the same
+    // products as fv * (w * (1/dV)))
+    if (P2) s_fv[lane] = make_double4(in.f0 * a.idV, in.f1 * a.idV, in.f2 * a.idV, 0.0);
+    else s_fv[lane] = make_double4(in.f0, in.f1, in.f2, 0.0);
+    int scc[3], cim[3];
+    src_coords(src, ib, jb, kb, scc);
+    for (int d = 0; d < 3; ++d) {
+        int ic = scc[d], kind;
+        src_map(ic, n3[d], g.bt[2 * d], g.bt[2 * d + 1], kind);
+        double xp = xm[d];
+        int cimg = scc[d];
+        if (kind == 1) { xp = xp + (scc[d] < 1 ? -L3[d] : L3[d]); cimg = cell_of(xp, dd[d]); }
+        else if (kind == 2) { xp = 2.0 * (scc[d] < 1 ? 0.0 : L3[d]) - xp; cimg = cell_of(xp, dd[d]); }
+        cim[d] = cimg;
+        // the centre factor at offset +2 is always 0 (see k_interp)
+        for (int o = 0; o < 4; ++o)
+            s_w[lane][d * 4 + o] = make_double2(wt_face<P2>(xp, cimg + o - 1, dd[d], id3[d]),
+                                                o < 3 ? wt_cent<P2>(xp, cimg + o - 1, dd[d], id3[d]) : 0.0);
+    }
+    s_cc[lane] = make_int4(cim[0], cim[1], cim[2], 0);
+}
+
+// the lane's cell: f -= fdrag*vol * w over the staged entries, in order
+template <bool P2>
+__device__ __forceinline__ void gather_entries(const SpreadArgs &a, int ne, int i, int j, int k,
+                                               const double2 (*s_w)[WST2], const int4 *s_cc, const double4 *s_fv,
+                                               double &f0, double &f1, double &f2, bool &has) {
+    for (int e = 0; e < ne; ++e) {
+        const int4 cc = s_cc[e];
+        const int ox = i - cc.x + 1, oy = j - cc.y + 1, oz = k - cc.z + 1;
+        if ((unsigned)ox > 3u || (unsigned)oy > 3u || (unsigned)oz > 3u) continue;
+        has = true;
+        const double2 wx = s_w[e][ox], wy = s_w[e][4 + oy], wz = s_w[e][8 + oz];
+        const double4 fv = s_fv[e];
+        // (face, centre): u takes the x face, v the y face, w the z face
+        const double pu = wx.x * wy.y * wz.y, pv = wx.y * wy.x * wz.y, pw = wx.y * wy.y * wz.x;
+        const double wu = P2 ? pu : pu / a.dV, wv = P2 ? pv : pv / a.dV, ww = P2 ? pw : pw / a.dV;
+        f0 = f0 - fv.x * wu;
+        f1 = f1 - fv.y * wv;
+        f2 = f2 - fv.z * ww;
+    }
+}
+
+// One wavefront per active tile (block b takes tile slot tile_slot(b)).
+template <bool P2>
+__global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
+    // per staged entry: (face, centre) factor pairs for 3 dimensions x 4
+    // offsets (padded to an odd number of 16-byte pairs per entry), the cell
+    // of the marker or image, and fdrag*vol
+    __shared__ double2 s_w[CAP][WST2];
+    __shared__ int4 s_cc[CAP];
+    __shared__ double4 s_fv[CAP];
+    const GridDev &g = a.g;
+    const int lane = threadIdx.x;
+    const int slot = tile_slot(blockIdx.x, a.meta[0]);
+    if (slot < 0) return;
+    const int elo = a.estart[slot], ehi = a.estart[slot + 1];
+    if (ehi > a.ent_cap) return;   // the fill flagged a capacity error
+    int tx, ty, tz;
+    tile_origin(a.tiles[slot], a.tg, tx, ty, tz);
+    const int ib = tx * TL - 1, jb = ty * TL - 1, kb = tz * TL - 1;   // 1-based source origin
     const int li = lane % TL, lj = (lane / TL) % TL, lk = lane / (TL * TL);
     const int i = tx * TL + 1 + li, j = ty * TL + 1 + lj, k = tz * TL + 1 + lk;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int e0 = 0; e0 < total; e0 += CAP) {
-        const int e1 = min(total, e0 + CAP);
-        // 2. stage entries [e0, e1), one lane per entry
-        for (int e = e0 + lane; e < e1; e += 64) {
-            int lo = 0, hi = NSRC;   // last source cell with s_off <= e
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_off[mid] <= e) lo = mid; else hi = mid - 1;
-            }
-            const int s = lo;
-            const int m = a.bin_items[s_gstart[s] + (e - s_off[s])];
-            const int kinds = s_kind[s];
-            const int scc[3] = {ib + s % SRC, jb + (s / SRC) % SRC, kb + s / (SRC * SRC)};
-            const int el = e - e0;
-            const double *fvm = a.fv + 4L * m;
-            s_fv[el][0] = fvm[0]; s_fv[el][1] = fvm[1]; s_fv[el][2] = fvm[2];
-            if (kinds == 0) {
-                const double2 *wm = reinterpret_cast<const double2 *>(a.wts + 24L * m);
-                double2 *dst = reinterpret_cast<double2 *>(&s_w[el][0]);
-                for (int r = 0; r < 12; ++r) dst[r] = wm[r];
-                for (int d = 0; d < 3; ++d) s_cc[el][d] = scc[d];
-            } else {
-                const double dd[3] = {g.dx, g.dy, g.dz};
-                for (int d = 0; d < 3; ++d) {
-                    const int kind = (kinds >> (2 * d)) & 3;
-                    double xp = a.mx[(long)d * a.M + m];
-                    if (kind == 1) xp = xp + (scc[d] < 1 ? -L3[d] : L3[d]);
-                    else if (kind == 2) xp = 2.0 * (scc[d] < 1 ? 0.0 : L3[d]) - xp;
-                    const int cimg = cell_of(xp, dd[d]);
-                    s_cc[el][d] = cimg;
-                    for (int o = 0; o < 4; ++o) {
-                        s_w[el][d * 8 + o] = wt_face(xp, cimg + o - 1, dd[d]);
-                        s_w[el][d * 8 + 4 + o] = wt_cent(xp, cimg + o - 1, dd[d]);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        // 3. gather for this lane's cell: 16 rows of 4 consecutive source
-        // cells, each a contiguous entry range, in (kc,jc,ic,id) order
-        for (int sk = lk; sk < lk + 4; ++sk)
-            for (int sj = lj; sj < lj + 4; ++sj) {
-                const int s0 = (sk * SRC + sj) * SRC + li;
-                const int q0 = max(s_off[s0], e0), q1 = min(s_off[s0 + 4], e1);
-                for (int q = q0; q < q1; ++q) {
-                    const int e = q - e0;
-                    const int ox = i - s_cc[e][0] + 1, oy = j - s_cc[e][1] + 1, oz = k - s_cc[e][2] + 1;
-                    if (ox < 0 || ox > 3 || oy < 0 || oy > 3 || oz < 0 || oz > 3) continue;
-                    const double wu = qdiv<P2>(s_w[e][ox] * s_w[e][12 + oy] * s_w[e][20 + oz], a.dV, a.idV);
-                    const double wv = qdiv<P2>(s_w[e][4 + ox] * s_w[e][8 + oy] * s_w[e][20 + oz], a.dV, a.idV);
-                    const double ww = qdiv<P2>(s_w[e][4 + ox] * s_w[e][12 + oy] * s_w[e][16 + oz], a.dV, a.idV);
-                    f0 = f0 - s_fv[e][0] * wu;
-                    f1 = f1 - s_fv[e][1] * wv;
-                    f2 = f2 - s_fv[e][2] * ww;
-                }
-            }
-        __syncthreads();
+    bool has = false;
+    for (int e0 = elo; e0 < ehi; e0 += CAP) {
+        EntryIn in;
+        load_entry_id(a, e0 + lane, lane < ehi - e0, in);
+        load_entry_rec(a, in);
+        stage_entry<P2>(a, in, ib, jb, kb, s_w, s_cc, s_fv);
+        wave_sync();
+        gather_entries<P2>(a, min(CAP, ehi - e0), i, j, k, s_w, s_cc, s_fv, f0, f1, f2, has);
+        wave_sync();   // the rows are rewritten by the next chunk
     }
-    // 4. multi-direct-forcing update u += dt*(f - f_old) on the lane's cell
-    if (i > g.nx || j > g.ny || k > g.nz) return;
+    if (a.first) {
+        const unsigned long long bits = __ballot(has ? 1 : 0);
+        if (lane == 0) a.mask[slot] = bits;
+    }
+    // multi-direct-forcing update u += dt*(f - f_old) on the lane's cell
+    if (!has || i > g.nx || j > g.ny || k > g.nz) return;
     const int tgt[3] = {i, j, k};
-    const long slot = (long)blockIdx.x * 64 + lane;
+    const long fs = (long)slot * 64 + lane;
     const long o = cidx(g, i, j, k);
     double *vel[3] = {a.u, a.v, a.w};
     const double fn[3] = {f0, f1, f2};
     for (int c = 0; c < 3; ++c) {
-        const double fo = a.fact[c * a.fstride + slot];
-        if (fn[c] == 0.0 && fo == 0.0) continue;
-        a.fact[c * a.fstride + slot] = fn[c];
+        const double fo = a.first ? 0.0 : a.fact[c * a.fstride + fs];
+        a.fact[c * a.fstride + fs] = fn[c];
         const bool face = tgt[c] == 1 && g.bt[2 * c] == IBMDEM_BC_WALL;
-        if (!face) vel[c][o] = vel[c][o] + a.dt * (fn[c] - fo);
+        if (!face && !(fn[c] == 0.0 && fo == 0.0)) vel[c][o] = vel[c][o] + a.dt * (fn[c] - fo);
     }
 }
 
-__global__ __launch_bounds__(64) void k_scatter_force(const int *__restrict__ tiles, TileGeom tg, GridDev g,
-                                                      const double *__restrict__ fact, long fstride,
+__global__ __launch_bounds__(64) void k_scatter_force(const int *__restrict__ meta, const int *__restrict__ tiles,
+                                                      TileGeom tg, GridDev g, const double *__restrict__ fact,
+                                                      long fstride, const unsigned long long *__restrict__ mask,
                                                       double *__restrict__ fx, double *__restrict__ fy,
                                                       double *__restrict__ fz) {
-    const int tile = tiles[blockIdx.x];
+    const int b = tile_slot(blockIdx.x, meta[0]);
+    if (b < 0) return;
     const int lane = threadIdx.x;
-    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+    if (!((mask[b] >> lane) & 1ull)) return;
+    int tx, ty, tz;
+    tile_origin(tiles[b], tg, tx, ty, tz);
     const int i = tx * TL + 1 + lane % TL, j = ty * TL + 1 + (lane / TL) % TL, k = tz * TL + 1 + lane / (TL * TL);
     if (i > g.nx || j > g.ny || k > g.nz) return;
-    const long slot = (long)blockIdx.x * 64 + lane, o = cidx(g, i, j, k);
+    const long slot = (long)b * 64 + lane, o = cidx(g, i, j, k);
     fx[o] = fact[slot];
     fy[o] = fact[fstride + slot];
     fz[o] = fact[2 * fstride + slot];
 }
 
+// cells with at least one contribution, summed over the active tiles
+__global__ __launch_bounds__(1024) void k_mask_count(const int *__restrict__ meta,
+                                                     const unsigned long long *__restrict__ mask,
+                                                     long long *__restrict__ out) {
+    __shared__ long long red[1024];
+    const int nact = meta[0];
+    long long s = 0;
+    for (int b = threadIdx.x; b < nact; b += 1024) s += __popcll(mask[b]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 512; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
 TileGeom tile_geom(const GridDev &g) { return TileGeom{div_up(g.nx, TL), div_up(g.ny, TL), div_up(g.nz, TL)}; }
+
+// dx, dy, dz and their product are powers of two: every division by them
+// is an exact multiplication by the reciprocal
+bool pow2_grid(const ibmdem_ctx *c) {
+    return is_pow2(c->g.dx) && is_pow2(c->g.dy) && is_pow2(c->g.dz) && is_pow2(c->dV);
+}
 
 }  // namespace
 
@@ -421,13 +581,13 @@ long ibm_tile_count(const GridDev &g) {
 
 int ibm_begin_step(ibmdem_ctx *c) {
     const int M = c->M;
-    c->nact = 0;
     c->first_mdf = true;
+    c->spread_valid = false;
     if (M == 0) return IBMDEM_OK;
     hipStream_t s = c->stream;
     const TileGeom tg = tile_geom(c->g);
     const long nt = (long)tg.ntx * tg.nty * tg.ntz;
-    hipLaunchKernelGGL(k_weights, dim3(div_up(M, T)), dim3(T), 0, s, M, c->g, c->mx, c->mcell, c->wts, c->fdrag);
+    hipLaunchKernelGGL(k_records, dim3(div_up(M, T)), dim3(T), 0, s, M, c->mx, c->mcell, c->mrec, c->fdrag);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_mark_tiles, dim3(div_up(M, T)), dim3(T), 0, s, M, c->g, tg, c->mcell, c->act_flag);
     CHECK_LAUNCH(c);
@@ -435,17 +595,19 @@ int ibm_begin_step(ibmdem_ctx *c) {
     int *cnt = c->scan_blocks;   // sized >= n/CHUNK + 2 at create
     hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(T), 0, s, nt, c->act_flag, cnt);
     CHECK_LAUNCH(c);
-    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, nb, cnt);
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, nb, cnt, c->act_meta);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_flag_compact, dim3(nb), dim3(T), 0, s, nt, c->act_flag, cnt, c->act_list);
     CHECK_LAUNCH(c);
-    HIPX(c, hipMemcpyAsync(c->flags_host + FL_COUNT, cnt + nb, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPX(c, hipStreamSynchronize(s));
-    const int nact = c->flags_host[FL_COUNT];
-    if (nact > c->act_cap) return set_error(c, IBMDEM_ERR_CAPACITY, "spread footprint exceeds its buffer");
-    c->nact = nact;
-    const long nz = 3L * 64 * std::max(nact, 1);
-    hipLaunchKernelGGL(k_zero, dim3(std::max(1, std::min(4096, div_up(nz, T)))), dim3(T), 0, s, nz, c->fact);
+    // the active count stays on the device: every tile kernel is launched
+    // act_grid wide and its blocks beyond the count return at once
+    hipLaunchKernelGGL(k_tile_count, dim3(c->act_grid), dim3(64), 0, s, c->g, tg, c->act_meta, c->act_list,
+                       c->bin_start, c->tile_ecount, c->act_cap);
+    CHECK_LAUNCH(c);
+    int rc = scan_counts(c, c->tile_ecount, c->tile_estart, c->act_cap);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_tile_fill, dim3(c->act_grid), dim3(64), 0, s, c->g, tg, c->act_meta, c->act_list,
+                       c->bin_start, c->bin_items, c->tile_estart, c->tile_ent, c->ent_cap, c->flags);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
@@ -454,46 +616,66 @@ int ibm_interpolate(ibmdem_ctx *c) {
     if (c->M == 0) return IBMDEM_OK;
     const int cur = c->vcur;
     prof_begin(c, IBMDEM_K_INTERP);
-    if (is_pow2(c->dV))
+    if (pow2_grid(c))
         hipLaunchKernelGGL((k_interp<true>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->u[cur],
-                           c->v[cur], c->w[cur], c->mcell, c->wts, c->mv, c->fdrag, c->mvol, c->fv, c->dV,
-                           1.0 / c->dV, c->cs.dt);
+                           c->v[cur], c->w[cur], c->mrec, c->mv, c->fdrag, c->mvol, c->dV, 1.0 / c->dV, c->cs.dt);
     else
         hipLaunchKernelGGL((k_interp<false>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->u[cur],
-                           c->v[cur], c->w[cur], c->mcell, c->wts, c->mv, c->fdrag, c->mvol, c->fv, c->dV,
-                           1.0 / c->dV, c->cs.dt);
-    prof_end(c, IBMDEM_K_INTERP, 0.0);
+                           c->v[cur], c->w[cur], c->mrec, c->mv, c->fdrag, c->mvol, c->dV, 1.0 / c->dV, c->cs.dt);
+    // per marker: record 32 B, v 24, fdrag 24 + 24, vol 8, fdrag*vol 24;
+    // one read of each footprint velocity (cell, component)
+    prof_end(c, IBMDEM_K_INTERP, 136.0 * c->M + 8.0 * 3.0 * (double)c->footprint_cells);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
 
 int ibm_spread(ibmdem_ctx *c) {
-    if (c->M == 0 || c->nact == 0) return IBMDEM_OK;
+    if (c->M == 0) return IBMDEM_OK;
     const int cur = c->vcur;
     SpreadArgs a;
     a.g = c->g;
     a.tg = tile_geom(c->g);
-    a.M = c->M;
+    a.meta = c->act_meta;
     a.tiles = c->act_list;
-    a.bin_start = c->bin_start;
-    a.bin_items = c->bin_items;
-    a.mx = c->mx;
-    a.wts = c->wts;
-    a.fv = c->fv;
+    a.estart = c->tile_estart;
+    a.ent = c->tile_ent;
+    a.ent_cap = c->ent_cap;
+    a.rec = c->mrec;
     a.fact = c->fact;
-    a.fstride = 64L * c->nact;
+    a.fstride = 64L * c->act_cap;
+    a.mask = c->tile_mask;
     a.u = c->u[cur]; a.v = c->v[cur]; a.w = c->w[cur];
     a.dV = c->dV;
     a.idV = 1.0 / c->dV;
     a.dt = c->cs.dt;
+    a.first = c->first_mdf ? 1 : 0;
     prof_begin(c, IBMDEM_K_SPREAD);
-    if (is_pow2(c->dV))
-        hipLaunchKernelGGL((k_spread_tiles<true>), dim3(c->nact), dim3(64), 0, c->stream, a);
+    const dim3 grid(c->act_grid);
+    if (pow2_grid(c))
+        hipLaunchKernelGGL((k_spread_tiles<true>), grid, dim3(64), 0, c->stream, a);
     else
-        hipLaunchKernelGGL((k_spread_tiles<false>), dim3(c->nact), dim3(64), 0, c->stream, a);
+        hipLaunchKernelGGL((k_spread_tiles<false>), grid, dim3(64), 0, c->stream, a);
     prof_end(c, IBMDEM_K_SPREAD, 0.0);
     CHECK_LAUNCH(c);
+    if (c->prof.kernel == IBMDEM_K_SPREAD || c->prof.kernel == IBMDEM_K_INTERP) {
+        if (a.first) {
+            // profiling only: count the written cells once per step (host wait)
+            long long *cnt = reinterpret_cast<long long *>(c->act_meta + 2);
+            hipLaunchKernelGGL(k_mask_count, dim3(1), dim3(1024), 0, c->stream, c->act_meta, c->tile_mask, cnt);
+            CHECK_LAUNCH(c);
+            long long h = 0;
+            HIPX(c, hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, c->stream));
+            HIPX(c, hipStreamSynchronize(c->stream));
+            c->footprint_cells = (long)h;
+        }
+        // per marker: its 64 B record; per written (cell, component): u read
+        // and written, f written, f_old read (not in the first iteration)
+        const double per = a.first ? 24.0 : 32.0;
+        if (c->prof.kernel == IBMDEM_K_SPREAD)
+            c->prof.bytes_total += 64.0 * c->M + per * 3.0 * (double)c->footprint_cells;
+    }
     c->first_mdf = false;
+    c->spread_valid = true;
     return IBMDEM_OK;
 }
 
@@ -504,9 +686,9 @@ int ibm_download_force(ibmdem_ctx *c, double *fx, double *fy, double *fz) {
     hipStream_t s = c->stream;
     int rc = IBMDEM_OK;
     if (hipMemsetAsync(buf, 0, 3 * n * sizeof(double), s) != hipSuccess) rc = IBMDEM_ERR_HIP;
-    if (!rc && c->nact > 0) {
-        hipLaunchKernelGGL(k_scatter_force, dim3(c->nact), dim3(64), 0, s, c->act_list, tile_geom(c->g), c->g,
-                           c->fact, 64L * c->nact, buf, buf + n, buf + 2 * n);
+    if (!rc && c->M > 0 && c->spread_valid) {
+        hipLaunchKernelGGL(k_scatter_force, dim3(c->act_grid), dim3(64), 0, s, c->act_meta, c->act_list,
+                           tile_geom(c->g), c->g, c->fact, 64L * c->act_cap, c->tile_mask, buf, buf + n, buf + 2 * n);
         if (hipGetLastError() != hipSuccess) rc = IBMDEM_ERR_HIP;
     }
     double *dst[3] = {fx, fy, fz};

@@ -360,7 +360,10 @@ __global__ __launch_bounds__(RED_T) void k_final(const double *__restrict__ part
         if (op == 1) scal[SC_ALPHA] = scal[SC_RZ] / t;
         if (op == 2) { scal[SC_BETA] = t / scal[SC_RZ]; scal[SC_RZ] = t; }
         if (op == 3 && (sqrt(t) <= scal[SC_TOL] || !(t == t))) { scal[SC_CONV] = 1.0; scal[SC_ITER] = it; }
-        if (op == 4) scal[SC_TOL] = rtol * sqrt(t);
+        if (op == 4) {
+            scal[SC_TOL] = rtol * sqrt(t);
+            if (t == 0.0) { scal[SC_CONV] = 1.0; scal[SC_ITER] = 0; scal[SC_RR] = 0.0; }
+        }
     }
 }
 
@@ -543,7 +546,43 @@ int poisson_setup(ibmdem_ctx *c) {
     return IBMDEM_OK;
 }
 
-int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
+namespace {
+
+// one CG iteration (it is its 1-based number), queued without any host wait
+int cg_iteration(ibmdem_ctx *c, int it) {
+    const long n = c->n;
+    hipStream_t s = c->stream;
+    int rc;
+    const LevelDev &L0 = c->lev64[0].d;
+    prof_begin(c, IBMDEM_K_MATVEC);
+    hipLaunchKernelGGL(k_matvec_dot, dim3(RED_BLOCKS), dim3(RED_T), 0, s, L0, c->cg_p, c->cg_Ap, c->partial,
+                       c->scal);
+    prof_end(c, IBMDEM_K_MATVEC, 2.0 * 8.0 * (double)n);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_PAP, 1, it,
+                       c->cs.poisson_rtol);
+    CHECK_LAUNCH(c);
+    prof_begin(c, IBMDEM_K_CG_UPDATE);
+    hipLaunchKernelGGL(k_cg_update, dim3(RED_BLOCKS), dim3(RED_T), 0, s, n, c->cg_x, c->cg_r, c->cg_p, c->cg_Ap,
+                       c->scal, c->partial);
+    prof_end(c, IBMDEM_K_CG_UPDATE, 6.0 * 8.0 * (double)n);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_RR, 3, it,
+                       c->cs.poisson_rtol);
+    CHECK_LAUNCH(c);
+    if ((rc = precondition(c))) return rc;
+    if ((rc = dot_into(c, c->cg_r, c->cg_z, SC_RZ_NEW, 2))) return rc;
+    hipLaunchKernelGGL(k_cg_p, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z, c->scal);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+}  // namespace
+
+// Start of the solve, queued without any host wait: pinned-row right-hand
+// side, r = b, x = 0, ||b|| and the tolerance (a zero right-hand side
+// converges at once), z = M^-1 r, p = z, r.z.
+int poisson_begin(ibmdem_ctx *c) {
     const long n = c->n;
     hipStream_t s = c->stream;
     int rc;
@@ -552,72 +591,75 @@ int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
     hipLaunchKernelGGL(k_cg_init, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->rhs, c->cg_r, c->cg_x, c->scal);
     CHECK_LAUNCH(c);
     if ((rc = dot_into(c, c->cg_r, c->cg_r, SC_BNORM2, 4))) return rc;
+    if ((rc = precondition(c))) return rc;
+    hipLaunchKernelGGL(k_copy, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z);
+    CHECK_LAUNCH(c);
+    return dot_into(c, c->cg_r, c->cg_z, SC_RZ, 0);
+}
+
+// Iterations first..last, queued without any host wait. Once the device sets
+// SC_CONV every later CG kernel returns at once, so x is the iterate of the
+// converging iteration however many iterations were queued.
+int poisson_iterations(ibmdem_ctx *c, int first, int last) {
+    for (int it = first; it <= last; ++it) {
+        int rc = cg_iteration(c, it);
+        if (rc) return rc;
+    }
+    return IBMDEM_OK;
+}
+
+// Iterations from `first` until convergence or the cap: the host keeps at
+// most two iterations in flight and reads the convergence flag of the older
+// one, so the stream never drains inside the solve.
+int poisson_iterate_to_convergence(ibmdem_ctx *c, int first) {
+    hipStream_t s = c->stream;
+    const int maxit = c->cs.poisson_max_iter;
+    for (int q = 0; q <= maxit + 1; ++q) c->conv_host[q] = 0.0;
+    HIPX(c, hipMemcpyAsync(c->conv_host + first - 1, c->scal + SC_CONV, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPX(c, hipEventRecord(c->iter_ev[(first - 1) & 1], s));
+    for (int it = first; it <= maxit; ++it) {
+        int rc = cg_iteration(c, it);
+        if (rc) return rc;
+        HIPX(c, hipMemcpyAsync(c->conv_host + it, c->scal + SC_CONV, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPX(c, hipEventRecord(c->iter_ev[it & 1], s));
+        // wait for the previous iteration, then look at its flag
+        HIPX(c, hipEventSynchronize(c->iter_ev[(it - 1) & 1]));
+        if (c->conv_host[it - 1] != 0.0) break;
+    }
+    return IBMDEM_OK;
+}
+
+// Wait for the solve, report it, fail on a non-converged or non-finite one.
+int poisson_check(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    int rc;
     if ((rc = read_scalars(c))) return rc;
     const double bnorm = std::sqrt(c->scal_host[SC_BNORM2]);
-    int it = 0, conv = 0;
-    double rr = bnorm * bnorm;
-    const LevelDev &L0 = c->lev64[0].d;
-    const int maxit = c->cs.poisson_max_iter;
-    if (bnorm == 0.0) {
-        conv = 1;
-    } else {
-        if ((rc = precondition(c))) return rc;
-        hipLaunchKernelGGL(k_copy, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z);
-        CHECK_LAUNCH(c);
-        if ((rc = dot_into(c, c->cg_r, c->cg_z, SC_RZ, 0))) return rc;
-        // Iterations are queued without waiting: the host keeps at most two
-        // in flight and reads the convergence flag of the older one. Once the
-        // device sets SC_CONV every later CG kernel returns immediately, so x
-        // is the iterate of the converging iteration however many extra
-        // iterations were queued.
-        for (int q = 0; q <= maxit + 1; ++q) c->conv_host[q] = 0.0;
-        int queued = 0;
-        bool done = false;
-        for (it = 1; it <= maxit && !done; ++it) {
-            prof_begin(c, IBMDEM_K_MATVEC);
-            hipLaunchKernelGGL(k_matvec_dot, dim3(RED_BLOCKS), dim3(RED_T), 0, s, L0, c->cg_p, c->cg_Ap, c->partial,
-                               c->scal);
-            prof_end(c, IBMDEM_K_MATVEC, 2.0 * 8.0 * (double)n);
-            CHECK_LAUNCH(c);
-            hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_PAP, 1, it,
-                               c->cs.poisson_rtol);
-            CHECK_LAUNCH(c);
-            prof_begin(c, IBMDEM_K_CG_UPDATE);
-            hipLaunchKernelGGL(k_cg_update, dim3(RED_BLOCKS), dim3(RED_T), 0, s, n, c->cg_x, c->cg_r, c->cg_p,
-                               c->cg_Ap, c->scal, c->partial);
-            prof_end(c, IBMDEM_K_CG_UPDATE, 6.0 * 8.0 * (double)n);
-            CHECK_LAUNCH(c);
-            hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_RR, 3, it,
-                               c->cs.poisson_rtol);
-            CHECK_LAUNCH(c);
-            HIPX(c, hipMemcpyAsync(c->conv_host + it, c->scal + SC_CONV, sizeof(double), hipMemcpyDeviceToHost, s));
-            if ((rc = precondition(c))) return rc;
-            if ((rc = dot_into(c, c->cg_r, c->cg_z, SC_RZ_NEW, 2))) return rc;
-            hipLaunchKernelGGL(k_cg_p, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z, c->scal);
-            CHECK_LAUNCH(c);
-            HIPX(c, hipEventRecord(c->iter_ev[it & 1], s));
-            ++queued;
-            if (queued >= 2) {
-                // wait for the previous iteration, then look at its flag
-                HIPX(c, hipEventSynchronize(c->iter_ev[(it - 1) & 1]));
-                if (c->conv_host[it - 1] != 0.0) done = true;
-            }
-        }
-        HIPX(c, hipStreamSynchronize(s));
-        if ((rc = read_scalars(c))) return rc;
-        conv = c->scal_host[SC_CONV] != 0.0;
-        rr = c->scal_host[SC_RR];
-        it = conv ? (int)c->scal_host[SC_ITER] : maxit;
-        if (!(rr == rr)) return set_error(c, IBMDEM_ERR_NAN, "non-finite residual in the Poisson solve");
-    }
-    hipLaunchKernelGGL(k_gauge, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_x, c->pcorr);
-    CHECK_LAUNCH(c);
+    const int conv = c->scal_host[SC_CONV] != 0.0;
+    const double rr = bnorm > 0 ? c->scal_host[SC_RR] : 0.0;
+    const int it = conv ? (int)c->scal_host[SC_ITER] : c->cs.poisson_max_iter;
+    c->last_cg_iterations = it;
     if (info) {
         info->iterations = it;
         info->converged = conv;
         info->rel_residual = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
         info->rhs_norm = bnorm;
     }
+    if (!(rr == rr)) return set_error(c, IBMDEM_ERR_NAN, "non-finite residual in the Poisson solve");
     if (!conv) return set_error(c, IBMDEM_ERR_NOT_CONVERGED, "Poisson solve did not converge");
     return IBMDEM_OK;
+}
+
+// P' = x - x(1,1,1): the reference's gauge (poisson.F90:78,97-100)
+int poisson_gauge(ibmdem_ctx *c) {
+    hipLaunchKernelGGL(k_gauge, dim3(stream_blocks(c->n)), dim3(RED_T), 0, c->stream, c->n, c->cg_x, c->pcorr);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    int rc;
+    if ((rc = poisson_begin(c))) return rc;
+    if ((rc = poisson_iterate_to_convergence(c, 1))) return rc;
+    if ((rc = poisson_check(c, info))) return rc;
+    return poisson_gauge(c);
 }
