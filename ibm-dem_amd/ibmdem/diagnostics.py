@@ -1,0 +1,71 @@
+"""Diagnostics and output of a run: force coefficients, per-body probe lines
+and Tecplot-readable field dumps.
+
+The formats follow the reference's post-processing:
+  - field dumps: OutputFluidTecplot (io.F90:43-77) writes
+    ``variables=x,y,z,u,v,w,vel,P`` and ``zone i=..,j=..,k=..`` followed by one
+    line per cell (cell centre, the staggered u, v, w as stored, |u|, P);
+    here the interior cells are written (the reference includes its halo);
+  - probes: PostProcessing (fluid.F90:368-404) appends per body
+    ``step, x(3), v(3), omega(3), fdrag/m*rhof (3), fmom*rhof (3), fcol/m (3)``;
+    the reference divides the torque by I, which is 0 for its cylinders
+    (fluid.F90:396, ZJS_INIT_DEM.F90:427); the torque itself is written here;
+  - force coefficients: C = 2 F / (rho_f U^2 D L) from the assembled
+    hydrodynamic force of a body (AssembleLocalForceMS, ZJS_MS_DEM.F90:494-519).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .abi import BodyArrays, Case
+
+
+def force_coefficients(bodies: BodyArrays, body: int, rho: float, U: float, D: float, L: float,
+                       drag_axis: int = 0, lift_axis: int = 1):
+    """(C_D, C_L) of one body: C = 2 F / (rho U^2 D L), F the body's
+    assembled hydrodynamic force (bodies.fdrag, component-major)."""
+    q = 0.5 * rho * U * U * D * L
+    if not q > 0:
+        raise ValueError("rho, U, D and L must be positive")
+    F = bodies.fdrag[:, body]
+    return float(F[drag_axis] / q), float(F[lift_axis] / q)
+
+
+def probe_line(step: int, bodies: BodyArrays, body: int, rhof: float) -> str:
+    """One line of the reference's per-body probe file (fluid.F90:398-401)."""
+    b = body
+    m = bodies.mass[b]
+    vals = [*bodies.x[:, b], *bodies.v[:, b], *bodies.omega[:, b],
+            *(bodies.fdrag[:, b] / m * rhof), *(bodies.fmom[:, b] * rhof), *(bodies.fcol[:, b] / m)]
+    return f"{step:d} " + " ".join(f"{x:.16E}" for x in vals)
+
+
+def write_tecplot(path, case: Case, fields) -> None:
+    """Field dump in the reference's Tecplot layout (io.F90:55-77) over the
+    interior cells, i fastest."""
+    u, v, w, p = (np.asarray(a, dtype=np.float64) for a in fields)
+    nz, ny, nx = u.shape
+    k, j, i = np.meshgrid(np.arange(1, nz + 1), np.arange(1, ny + 1), np.arange(1, nx + 1), indexing="ij")
+    xc = (i - 0.5) * case.dx
+    yc = (j - 0.5) * case.dy
+    zc = (k - 0.5) * case.dz
+    vel = np.sqrt(u * u + v * v + w * w)
+    cols = np.stack([a.reshape(-1) for a in (xc, yc, zc, u, v, w, vel, p)], axis=1)
+    with open(path, "w") as f:
+        f.write(" variables=x,y,z,u,v,w,vel,P\n")
+        f.write(f" zone i= {nx} j= {ny} k= {nz}\n")
+        np.savetxt(f, cols, fmt="%.16E")
+
+
+def read_tecplot(path):
+    """-> (nx, ny, nz, array of shape (nx*ny*nz, 8)) of a dump written above."""
+    with open(path) as f:
+        head = f.readline()
+        if "variables=x,y,z,u,v,w,vel,P" not in head:
+            raise ValueError(f"{path}: not a field dump")
+        zone = f.readline().replace("=", " ").split()
+        nx, ny, nz = int(zone[2]), int(zone[4]), int(zone[6])
+        data = np.loadtxt(f, ndmin=2)
+    if data.shape != (nx * ny * nz, 8):
+        raise ValueError(f"{path}: {data.shape[0]} rows for a {nx}x{ny}x{nz} zone")
+    return nx, ny, nz, data

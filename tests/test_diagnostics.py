@@ -1,0 +1,58 @@
+"""Diagnostics: force coefficients, probe lines, Tecplot field dumps."""
+import numpy as np
+import pytest
+
+from ibmdem import abi, cases, diagnostics
+
+
+def _bodies():
+    b = abi.BodyArrays.empty(2)
+    b.fdrag[:, 1] = [3.0, -1.5, 0.25]
+    b.mass[:] = [2.0, 4.0]
+    b.x[:, 1] = [1.0, 2.0, 3.0]
+    b.v[:, 1] = [0.1, 0.2, 0.3]
+    b.omega[:, 1] = [0.0, 0.0, 0.5]
+    b.fmom[:, 1] = [0.0, 1.0, 0.0]
+    b.fcol[:, 1] = [4.0, 0.0, 0.0]
+    return b
+
+
+def test_force_coefficients_closed_form():
+    b = _bodies()
+    cd, cl = diagnostics.force_coefficients(b, 1, rho=1.0, U=2.0, D=32.0, L=64.0)
+    q = 0.5 * 1.0 * 4.0 * 32.0 * 64.0
+    assert cd == 3.0 / q and cl == -1.5 / q
+    cd2, cl2 = diagnostics.force_coefficients(b, 1, rho=1.0, U=2.0, D=32.0, L=64.0, drag_axis=2, lift_axis=0)
+    assert (cd2, cl2) == (0.25 / q, 3.0 / q)
+    with pytest.raises(ValueError):
+        diagnostics.force_coefficients(b, 1, rho=1.0, U=0.0, D=1.0, L=1.0)
+
+
+def test_probe_line_fields():
+    b = _bodies()
+    vals = diagnostics.probe_line(7, b, 1, rhof=2.0).split()
+    assert vals[0] == "7"
+    got = np.array([float(x) for x in vals[1:]])
+    want = np.concatenate([b.x[:, 1], b.v[:, 1], b.omega[:, 1], b.fdrag[:, 1] / 4.0 * 2.0,
+                           b.fmom[:, 1] * 2.0, b.fcol[:, 1] / 4.0])
+    np.testing.assert_array_equal(got, want)
+
+
+def test_tecplot_round_trip(tmp_path):
+    c = cases.build_case(6, 5, 4, [1] * 6, h=0.5)
+    rng = np.random.default_rng(3)
+    fields = [rng.normal(size=(4, 5, 6)) for _ in range(4)]
+    p = tmp_path / "f.dat"
+    diagnostics.write_tecplot(p, c, fields)
+    nx, ny, nz, data = diagnostics.read_tecplot(p)
+    assert (nx, ny, nz) == (6, 5, 4)
+    # i fastest: row 1 is cell (2,1,1), row nx is cell (1,2,1)
+    np.testing.assert_array_equal(data[1, :3], [1.5 * 0.5, 0.25, 0.25])
+    np.testing.assert_array_equal(data[6, :3], [0.25, 0.75, 0.25])
+    for q in range(3):
+        np.testing.assert_array_equal(data[:, 3 + q], fields[q].reshape(-1))
+    np.testing.assert_allclose(data[:, 6], np.sqrt(sum(f.reshape(-1) ** 2 for f in fields[:3])), rtol=1e-15)
+    np.testing.assert_array_equal(data[:, 7], fields[3].reshape(-1))
+    head = open(p).read().splitlines()[:2]
+    assert head[0].strip() == "variables=x,y,z,u,v,w,vel,P"
+    assert head[1].split() == ["zone", "i=", "6", "j=", "5", "k=", "4"]
