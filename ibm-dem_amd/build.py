@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libibmdem.so")
-SOURCES = ["ctx.hip", "fluid.hip", "poisson.hip", "ibm.hip", "dem.hip"]
+SOURCES = ["ctx.hip", "fluid.hip", "poisson.hip", "ibm.hip", "dem.hip", "batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IBMDEM_ARCH", "gfx950")
 FLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -373,6 +373,11 @@ void prof_end(ibmdem_ctx *c, int kid, double bytes);
 // ---------------------------------------------------------------------------
 // launchers (one per translation unit)
 // ---------------------------------------------------------------------------
+// ctx.hip: the coupled step in phases (for batches of contexts)
+int step_phase1(ibmdem_ctx *c, ibmdem_solve_info *info, bool *done);
+int step_phase2(ibmdem_ctx *c, ibmdem_solve_info *info);
+int step_phase3(ibmdem_ctx *c);
+void drop_graphs(ibmdem_ctx *c);
 // dem.hip: exclusive scan of n counts into start[0..n] (start[n] = total)
 int scan_counts(ibmdem_ctx *c, const int *cnt, int *start, long n);
 // fluid.hip

@@ -214,7 +214,29 @@ int run_part(ibmdem_ctx *c, CapturedPart &part) {
     return IBMDEM_OK;
 }
 
-int step_graph(ibmdem_ctx *c, ibmdem_solve_info *info) {
+int step_eager(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    TRY(step_pre(c));
+    TRY(poisson_iterate_to_convergence(c, 1));
+    TRY(poisson_check(c, info));
+    if (c->last_cg_iterations > c->graphs.cg_k) c->graphs.cg_k = c->last_cg_iterations;
+    TRY(step_post(c));
+    return dem_check_flags(c);
+}
+
+}  // namespace
+
+// A step in three phases so that a batch can interleave many contexts (one
+// stream each): phase 1 queues the first captured part, phase 2 waits for
+// the solve and queues the second part, phase 3 waits for the step's error
+// flags. A context that cannot use its graphs yet (unknown iteration count,
+// profiling, fresh marker upload) runs the whole step eagerly in phase 1 and
+// reports it in *done.
+int step_phase1(ibmdem_ctx *c, ibmdem_solve_info *info, bool *done) {
+    if (!(c->graphs.enabled && c->graphs.cg_k >= 0 && c->prof.kernel == 0 && !c->rmax_dirty)) {
+        *done = true;
+        return step_eager(c, info);
+    }
+    *done = false;
     StepGraphs &G = c->graphs;
     const int K = G.cg_k;
     CapturedPart &pre = G.pre[c->vcur * 2 + c->pcur];
@@ -227,38 +249,37 @@ int step_graph(ibmdem_ctx *c, ibmdem_solve_info *info) {
     }
     TRY(run_part(c, pre));
     HIPX(c, hipMemcpyAsync(c->conv_host, c->scal + SC_CONV, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return IBMDEM_OK;
+}
+
+int step_phase2(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    StepGraphs &G = c->graphs;
     HIPX(c, hipStreamSynchronize(c->stream));
-    if (c->conv_host[0] == 0.0) TRY(poisson_iterate_to_convergence(c, K + 1));
+    if (c->conv_host[0] == 0.0) TRY(poisson_iterate_to_convergence(c, G.cg_k + 1));
     TRY(poisson_check(c, info));
     if (c->last_cg_iterations > G.cg_k) G.cg_k = c->last_cg_iterations;
     CapturedPart &post = G.post[c->vcur * 2 + c->pcur];
     if (!post.exec || !(post.before == host_state(c))) TRY(capture_part(c, post, [&] { return step_post(c); }));
-    TRY(run_part(c, post));
-    return dem_check_flags(c);
+    return run_part(c, post);
 }
 
-int step_eager(ibmdem_ctx *c, ibmdem_solve_info *info) {
-    TRY(step_pre(c));
-    TRY(poisson_iterate_to_convergence(c, 1));
-    TRY(poisson_check(c, info));
-    if (c->last_cg_iterations > c->graphs.cg_k) c->graphs.cg_k = c->last_cg_iterations;
-    TRY(step_post(c));
-    return dem_check_flags(c);
-}
-
-int step_impl(ibmdem_ctx *c, ibmdem_solve_info *info) {
-    // graphs need a known iteration count, no per-launch profiling events and
-    // the steady host state (a fresh marker upload recomputes body radii once)
-    if (c->graphs.enabled && c->graphs.cg_k >= 0 && c->prof.kernel == 0 && !c->rmax_dirty)
-        return step_graph(c, info);
-    return step_eager(c, info);
-}
+int step_phase3(ibmdem_ctx *c) { return dem_check_flags(c); }
 
 void drop_graphs(ibmdem_ctx *c) {
     for (int q = 0; q < 4; ++q) {
         drop_part(c->graphs.pre[q]);
         drop_part(c->graphs.post[q]);
     }
+}
+
+namespace {
+
+int step_impl(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    bool done = false;
+    TRY(step_phase1(c, info, &done));
+    if (done) return IBMDEM_OK;
+    TRY(step_phase2(c, info));
+    return step_phase3(c);
 }
 
 }  // namespace

@@ -255,6 +255,49 @@ module ibmdem_api
             type(c_ptr), value :: count, partner, shift
         end function
 
+        ! ---- multigrid cycle and the batched form ------------------------------------
+        integer(c_int) function ibmdem_get_mg_params(ctx, mg) bind(C, name="ibmdem_get_mg_params")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), intent(out) :: mg(3)
+        end function
+        integer(c_int) function ibmdem_set_mg_params(ctx, mg) bind(C, name="ibmdem_set_mg_params")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), intent(in) :: mg(3)
+        end function
+        integer(c_int) function ibmdem_batch_create(n, cases, batch) bind(C, name="ibmdem_batch_create")
+            import :: c_int, c_ptr, c_int32_t, ibmdem_case
+            integer(c_int32_t), value :: n
+            type(ibmdem_case), intent(in) :: cases(*)
+            type(c_ptr), intent(out) :: batch
+        end function
+        integer(c_int) function ibmdem_batch_destroy(batch) bind(C, name="ibmdem_batch_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+        end function
+        integer(c_int) function ibmdem_batch_context(batch, q, ctx) bind(C, name="ibmdem_batch_context")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: batch
+            integer(c_int32_t), value :: q
+            type(c_ptr), intent(out) :: ctx
+        end function
+        type(c_ptr) function ibmdem_batch_last_error(batch) bind(C, name="ibmdem_batch_last_error")
+            import :: c_ptr
+            type(c_ptr), value :: batch
+        end function
+        integer(c_int) function ibmdem_batch_run(batch, n_steps, info) bind(C, name="ibmdem_batch_run")
+            import :: c_int, c_ptr, c_int32_t, ibmdem_solve_info
+            type(c_ptr), value :: batch
+            integer(c_int32_t), value :: n_steps
+            type(ibmdem_solve_info), intent(inout) :: info(*)
+        end function
+        integer(c_int) function ibmdem_batch_autotune(batch, trial_steps) bind(C, name="ibmdem_batch_autotune")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: batch
+            integer(c_int32_t), value :: trial_steps
+        end function
+
         ! ---- measurement ------------------------------------------------------------
         integer(c_int) function ibmdem_profile_kernel(ctx, kernel_id) bind(C, name="ibmdem_profile_kernel")
             import :: c_int, c_ptr, c_int32_t

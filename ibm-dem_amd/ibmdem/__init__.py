@@ -8,9 +8,9 @@ from __future__ import annotations
 
 import os
 
-from .abi import (BC_PERIODIC, BC_WALL, EXPORTED, FP64, MIXED, BodyArrays,
-                  Case, IbmdemError, MarkerArrays, Simulation, SolveInfo,
-                  load_library)
+from .abi import (BC_PERIODIC, BC_WALL, EXPORTED, FP64, MIXED, Batch,
+                  BodyArrays, Case, IbmdemError, MarkerArrays, Simulation,
+                  SolveInfo, load_library)
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GPU_LIB = os.path.join(PKG_ROOT, "lib", "libibmdem.so")
@@ -35,6 +35,11 @@ def simulation(case: Case) -> Simulation:
     return Simulation(gpu_library(), case)
 
 
-__all__ = ["BC_PERIODIC", "BC_WALL", "FP64", "MIXED", "BodyArrays", "Case",
+def batch(cases) -> Batch:
+    """A batch of independent cases on the GPU library."""
+    return Batch(gpu_library(), cases)
+
+
+__all__ = ["BC_PERIODIC", "BC_WALL", "FP64", "MIXED", "Batch", "BodyArrays", "Case", "batch",
            "IbmdemError", "MarkerArrays", "Simulation", "SolveInfo",
            "EXPORTED", "gpu_library", "simulation", "GPU_LIB"]

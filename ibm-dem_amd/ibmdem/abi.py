@@ -131,8 +131,15 @@ _FUNCS = {
     "ibmdem_profile_kernel": [C.c_void_p, _i32],
     "ibmdem_profile_read": [C.c_void_p, _pd, C.POINTER(C.c_int64), _pd],
     "ibmdem_memory_bytes": [C.c_void_p, C.POINTER(C.c_int64)],
+    "ibmdem_get_mg_params": [C.c_void_p, _pi],
+    "ibmdem_set_mg_params": [C.c_void_p, _pi],
+    "ibmdem_batch_create": [_i32, C.POINTER(Case), C.POINTER(C.c_void_p)],
+    "ibmdem_batch_destroy": [C.c_void_p],
+    "ibmdem_batch_context": [C.c_void_p, _i32, C.POINTER(C.c_void_p)],
+    "ibmdem_batch_run": [C.c_void_p, _i32, C.POINTER(SolveInfo)],
+    "ibmdem_batch_autotune": [C.c_void_p, _i32],
 }
-EXPORTED = tuple(_FUNCS) + ("ibmdem_last_error",)
+EXPORTED = tuple(_FUNCS) + ("ibmdem_last_error", "ibmdem_batch_last_error")
 
 
 def load_library(path: str) -> C.CDLL:
@@ -143,6 +150,8 @@ def load_library(path: str) -> C.CDLL:
         fn.restype = C.c_int
     lib.ibmdem_last_error.argtypes = [C.c_void_p]
     lib.ibmdem_last_error.restype = C.c_char_p
+    lib.ibmdem_batch_last_error.argtypes = [C.c_void_p]
+    lib.ibmdem_batch_last_error.restype = C.c_char_p
     return lib
 
 
@@ -382,3 +391,86 @@ class Simulation:
         n = C.c_int64()
         self.call("ibmdem_memory_bytes", C.byref(n))
         return n.value
+
+    def mg_params(self):
+        mg = np.zeros(3, dtype=np.int32)
+        self.call("ibmdem_get_mg_params", _p(mg))
+        return tuple(int(x) for x in mg)
+
+    def set_mg_params(self, pre: int, post: int, coarse: int) -> None:
+        mg = np.array([pre, post, coarse], dtype=np.int32)
+        self.call("ibmdem_set_mg_params", _p(mg))
+
+
+class _BorrowedSimulation(Simulation):
+    """A context owned by a Batch (closing it is the batch's job)."""
+
+    def __init__(self, lib, case, handle):
+        self.lib = lib
+        self.case = case
+        self.shape = (case.nz, case.ny, case.nx)
+        self.n_markers = case.n_markers
+        self.n_bodies = case.n_bodies
+        self.h = handle
+
+    def close(self) -> None:
+        self.h = None
+
+
+class Batch:
+    """Independent cases of uneven size advanced together (one context and
+    HIP stream each). ``batch[q]`` is case q's context for uploads,
+    downloads and single operators."""
+
+    def __init__(self, lib: C.CDLL, cases):
+        self.lib = lib
+        self.cases = list(cases)
+        arr = (Case * len(self.cases))(*self.cases)
+        h = C.c_void_p()
+        rc = lib.ibmdem_batch_create(len(self.cases), arr, C.byref(h))
+        if rc != IBMDEM_OK:
+            raise IbmdemError(rc, "ibmdem_batch_create")
+        self.h = h
+        self.sims = []
+        for q, cs in enumerate(self.cases):
+            ch = C.c_void_p()
+            self._check(lib.ibmdem_batch_context(self.h, q, C.byref(ch)), "ibmdem_batch_context")
+            self.sims.append(_BorrowedSimulation(lib, cs, ch))
+
+    def _check(self, rc, where):
+        if rc != IBMDEM_OK:
+            detail = self.lib.ibmdem_batch_last_error(self.h)
+            raise IbmdemError(rc, where, detail.decode() if detail else "")
+
+    def __len__(self):
+        return len(self.sims)
+
+    def __getitem__(self, q) -> Simulation:
+        return self.sims[q]
+
+    def run(self, n: int):
+        infos = (SolveInfo * len(self.sims))()
+        self._check(self.lib.ibmdem_batch_run(self.h, int(n), infos), "ibmdem_batch_run")
+        return list(infos)
+
+    def autotune(self, trial_steps: int = 3) -> None:
+        self._check(self.lib.ibmdem_batch_autotune(self.h, int(trial_steps)), "ibmdem_batch_autotune")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            for s in self.sims:
+                s.close()
+            self.lib.ibmdem_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

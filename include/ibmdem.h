@@ -208,6 +208,32 @@ int ibmdem_get_cells(ibmdem_ctx *ctx, int32_t *cell_ijk);
 int ibmdem_get_pairs(ibmdem_ctx *ctx, int32_t cap, int32_t *count,
                      int32_t *partner, int32_t *shift);
 
+/* ---- multigrid cycle of a context --------------------------------------------
+ * mg[0..2] = pre-smoothing sweeps, post-smoothing sweeps, coarsest-level
+ * sweeps of the V-cycle preconditioner (no counterpart in the reference,
+ * whose solver is PETSc's, poisson.F90:133-158). Setting them drops the
+ * context's captured step graphs. */
+int ibmdem_get_mg_params(ibmdem_ctx *ctx, int32_t *mg);
+int ibmdem_set_mg_params(ibmdem_ctx *ctx, const int32_t *mg);
+
+/* ---- batched form -----------------------------------------------------------
+ * Many independent cases of uneven size advanced together on one GPU, each a
+ * full context with its own HIP stream; a batch step interleaves the cases'
+ * captured step graphs so their kernels overlap. No counterpart in the
+ * reference, which runs one case per MPI job (main.F90). */
+typedef struct ibmdem_batch ibmdem_batch;
+int ibmdem_batch_create(int32_t n, const ibmdem_case *cases, ibmdem_batch **out);
+int ibmdem_batch_destroy(ibmdem_batch *batch);
+/* the context of case q, for uploads, downloads and single operators */
+int ibmdem_batch_context(ibmdem_batch *batch, int32_t q, ibmdem_ctx **ctx);
+const char *ibmdem_batch_last_error(const ibmdem_batch *batch);
+/* Advance every case n_steps (FirstOrderForwardEuler + CloudEvolutionMS
+ * each); info (may be NULL) receives one solve record per case. */
+int ibmdem_batch_run(ibmdem_batch *batch, int32_t n_steps, ibmdem_solve_info *info);
+/* Per case, time trial_steps steps (<= 0: 3) with each candidate multigrid
+ * cycle from the case's current state, keep the fastest, restore the state. */
+int ibmdem_batch_autotune(ibmdem_batch *batch, int32_t trial_steps);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Kernel ids for ibmdem_profile_* (HIP events around every launch of that
  * kernel on the context stream). */

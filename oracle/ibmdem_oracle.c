@@ -1137,6 +1137,10 @@ int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
     for (int d = 0; d < 3; ++d)
         if ((cs->btype[2 * d] == IBMDEM_BC_PERIODIC) != (cs->btype[2 * d + 1] == IBMDEM_BC_PERIODIC))
             return IBMDEM_ERR_ARG;
+    /* the same limits as the GPU library: a 5x5x5 marker neighbourhood must
+     * not wrap onto itself */
+    if (cs->n_markers > 0 && (cs->nx < 5 || cs->ny < 5 || cs->nz < 5)) return IBMDEM_ERR_ARG;
+    if (cs->n_markers >= (1 << 23)) return IBMDEM_ERR_UNSUPPORTED;
     ibmdem_ctx *c = (ibmdem_ctx *)xcalloc(1, sizeof *c);
     c->cs = *cs;
     if (c->cs.n_mdf <= 0) c->cs.n_mdf = 5;
@@ -1474,6 +1478,80 @@ int ibmdem_memory_bytes(const ibmdem_ctx *c, int64_t *bytes) {
     if (!c || !bytes) return IBMDEM_ERR_ARG;
     *bytes = 0;
     return IBMDEM_OK;
+}
+
+/* ------------------------------------------------------------------------
+ * Multigrid cycle parameters and the batched form. The oracle runs a batch's
+ * cases one after another; there is nothing to tune on one core, so the
+ * autotuner keeps every case's cycle.
+ * ------------------------------------------------------------------------ */
+int ibmdem_get_mg_params(ibmdem_ctx *c, int32_t *mg) {
+    if (!c || !mg) return IBMDEM_ERR_ARG;
+    mg[0] = c->cs.mg_pre;
+    mg[1] = c->cs.mg_post;
+    mg[2] = c->cs.mg_coarse_sweeps;
+    return IBMDEM_OK;
+}
+
+int ibmdem_set_mg_params(ibmdem_ctx *c, const int32_t *mg) {
+    if (!c || !mg) return IBMDEM_ERR_ARG;
+    if (mg[0] < 1 || mg[0] > 8 || mg[1] < 1 || mg[1] > 8 || mg[2] < 1 || mg[2] > 256) return IBMDEM_ERR_ARG;
+    c->cs.mg_pre = mg[0];
+    c->cs.mg_post = mg[1];
+    c->cs.mg_coarse_sweeps = mg[2];
+    return IBMDEM_OK;
+}
+
+struct ibmdem_batch {
+    int n;
+    ibmdem_ctx **c;
+};
+
+int ibmdem_batch_destroy(ibmdem_batch *b) {
+    if (!b) return IBMDEM_OK;
+    for (int q = 0; q < b->n; ++q) ibmdem_destroy(b->c[q]);
+    free(b->c);
+    free(b);
+    return IBMDEM_OK;
+}
+
+int ibmdem_batch_create(int32_t n, const ibmdem_case *cases, ibmdem_batch **out) {
+    if (!out || !cases || n < 1) return IBMDEM_ERR_ARG;
+    *out = NULL;
+    ibmdem_batch *b = (ibmdem_batch *)xcalloc(1, sizeof *b);
+    b->n = n;
+    b->c = (ibmdem_ctx **)xcalloc((size_t)n, sizeof *b->c);
+    for (int q = 0; q < n; ++q) {
+        int rc = ibmdem_create(&cases[q], &b->c[q]);
+        if (rc) {
+            ibmdem_batch_destroy(b);
+            return rc;
+        }
+    }
+    *out = b;
+    return IBMDEM_OK;
+}
+
+int ibmdem_batch_context(ibmdem_batch *b, int32_t q, ibmdem_ctx **ctx) {
+    if (!b || !ctx || q < 0 || q >= b->n) return IBMDEM_ERR_ARG;
+    *ctx = b->c[q];
+    return IBMDEM_OK;
+}
+
+const char *ibmdem_batch_last_error(const ibmdem_batch *b) { return b ? "" : "no batch"; }
+
+int ibmdem_batch_run(ibmdem_batch *b, int32_t n_steps, ibmdem_solve_info *info) {
+    if (!b || n_steps < 0) return IBMDEM_ERR_ARG;
+    for (int q = 0; q < b->n; ++q) {
+        int rc = ibmdem_run(b->c[q], n_steps, info ? &info[q] : NULL);
+        if (rc) return rc;
+    }
+    return IBMDEM_OK;
+}
+
+int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {
+    (void)trial_steps;
+    return b ? IBMDEM_OK : IBMDEM_ERR_ARG;
 }
 
 /* ------------------------------------------------------------------------

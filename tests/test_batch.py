@@ -1,0 +1,90 @@
+"""The batched form: independent cases of uneven size advanced together.
+
+A batch step must give each case exactly what stepping it alone gives (the
+same kernels in the same order on the case's own stream); the autotuner may
+change a case's multigrid cycle, which changes only the solver iterates
+(the solution agrees to the solver tolerance), and must leave the state as
+it found it."""
+import numpy as np
+import pytest
+
+import oracle
+from ibmdem import abi, cases
+
+
+def _cases():
+    return [cases.small_periodic(gap=0.75),
+            cases.small_walls(),
+            cases.small_periodic(n=(40, 32, 24), n_spheres=4, D=7.0, seed=cases.SEED_BASE + 21, gap=1.0)]
+
+
+def _single(lib, setup, n):
+    s = abi.Simulation(lib, setup.case)
+    cases.load(s, setup)
+    info = s.run(n)
+    out = (s.download_fields(), s.download_markers().x, s.download_bodies().x, info.iterations)
+    s.close()
+    return out
+
+
+def _batched(lib, setups, n, tune=0):
+    with abi.Batch(lib, [s.case for s in setups]) as b:
+        for q, s in enumerate(setups):
+            cases.load(b[q], s)
+        if tune:
+            before = [b[q].download_fields() for q in range(len(b))]
+            b.autotune(tune)
+            for q in range(len(b)):
+                for x, y in zip(before[q], b[q].download_fields()):
+                    np.testing.assert_array_equal(x, y)
+        infos = b.run(n)
+        return [(b[q].download_fields(), b[q].download_markers().x, b[q].download_bodies().x,
+                 infos[q].iterations, b[q].mg_params()) for q in range(len(b))]
+
+
+def test_mg_params_round_trip_and_range(oracle_lib):
+    setup = cases.small_walls()
+    s = abi.Simulation(oracle_lib, setup.case)
+    assert s.mg_params() == (2, 2, 24)
+    s.set_mg_params(3, 1, 12)
+    assert s.mg_params() == (3, 1, 12)
+    with pytest.raises(abi.IbmdemError):
+        s.set_mg_params(0, 1, 12)
+    s.close()
+
+
+def test_batch_equals_single_cases_on_the_oracle(oracle_lib):
+    setups = _cases()
+    got = _batched(oracle_lib, setups, 2)
+    for s, g in zip(setups, got):
+        want = _single(oracle_lib, s, 2)
+        for a, b in zip(g[0], want[0]):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(g[1], want[1])
+        np.testing.assert_array_equal(g[2], want[2])
+
+
+@pytest.mark.gpu
+def test_batch_equals_single_cases_on_the_gpu(gpu_lib):
+    setups = _cases()
+    got = _batched(gpu_lib, setups, 4)
+    for s, g in zip(setups, got):
+        want = _single(gpu_lib, s, 4)
+        for a, b in zip(g[0], want[0]):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(g[1], want[1])
+        np.testing.assert_array_equal(g[2], want[2])
+        assert g[3] == want[3]
+
+
+@pytest.mark.gpu
+def test_batch_autotune_keeps_the_solution(gpu_lib, oracle_lib):
+    setups = _cases()
+    tuned = _batched(gpu_lib, setups, 3, tune=2)
+    for s, g in zip(setups, tuned):
+        want = _single(oracle_lib, s, 3)
+        assert g[4] in [(1, 1, 24), (2, 2, 24), (3, 3, 24), (2, 2, 12), (1, 2, 24)]
+        for a, b in zip(g[0], want[0]):
+            scale = max(np.abs(b).max(), 1e-300)
+            assert np.abs(a - b).max() <= 1e-8 * scale
+        assert np.abs(g[2] - want[2]).max() <= 1e-10 * np.abs(want[2]).max()
