@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=256, help="grid edge (256 is the workload)")
     ap.add_argument("--spheres", type=int, default=2000)
+    ap.add_argument("--batched", action="store_true",
+                    help="also time the batched form (64 uneven settling cases)")
     return ap.parse_args()
 
 
@@ -140,6 +142,61 @@ def measured_traffic():
                 out[name] = float(val)
     except OSError:
         pass
+    return out
+
+
+def batched_run(steps, warmup):
+    """The batched form on 64 independent settling cases of uneven size
+    (grids 32..96 x 32..96 x 64..192, 1..40 spheres of D in [8h, 16h]):
+    mixed precision with autotuned multigrid cycles, timed over `steps`
+    batch steps; the same cases in FP64 give the per-case error."""
+    import numpy as np
+
+    import ibmdem
+    from ibmdem import cases
+    from ibmdem.abi import FP64, MIXED
+
+    out = {}
+    states = {}
+    for name, prec in (("mixed", MIXED), ("fp64", FP64)):
+        setups = cases.config_batch(precision=prec)
+        with ibmdem.batch([s.case for s in setups]) as b:
+            for q, s in enumerate(setups):
+                cases.load(b[q], s)
+            t_tune = time.perf_counter()
+            if prec == MIXED:
+                b.autotune(2)
+            t_tune = time.perf_counter() - t_tune
+            b.run(warmup)
+            for q in range(len(b)):
+                b[q].sync()
+            t0 = time.perf_counter()
+            infos = b.run(steps)
+            for q in range(len(b)):
+                b[q].sync()
+            secs = time.perf_counter() - t0
+            states[name] = [(b[q].download_fields(), b[q].download_bodies().x) for q in range(len(b))]
+            cells = sum(s.cells for s in setups)
+            out[name] = {"seconds": secs, "batch_steps_per_s": steps / secs,
+                         "gcell_per_s": cells * steps / secs / 1e9,
+                         "cg_iterations_mean": float(np.mean([i.iterations for i in infos])),
+                         "mg_cycles": sorted({b[q].mg_params() for q in range(len(b))}),
+                         "autotune_s": t_tune}
+            out["cases"] = len(setups)
+            out["cells"] = cells
+            out["markers"] = sum(s.case.n_markers for s in setups)
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+    errs = {"u": [], "v": [], "w": [], "p": [], "body_x": []}
+    for (fm, bm), (ff, bf) in zip(states["mixed"], states["fp64"]):
+        for k, a, b in zip("uvwp", fm, ff):
+            errs[k].append(rel(a, b))
+        errs["body_x"].append(rel(bm, bf))
+    out["error_vs_fp64_max_over_cases"] = {k: max(v) for k, v in errs.items()}
+    out["steps"] = steps
+    out["warmup"] = warmup
     return out
 
 
@@ -245,6 +302,8 @@ def main():
                 "p": rel(um[3], uf[3]), "marker_x": rel(mm.x, mf.x), "body_x": rel(bm.x, bf.x),
             },
         }
+    if args.batched:
+        line["batched"] = batched_run(args.steps, args.warmup)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline()

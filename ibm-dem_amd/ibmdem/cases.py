@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .abi import (BC_PERIODIC, BC_WALL, FP64, BodyArrays, Case, MarkerArrays)
+from .abi import (BC_PERIODIC, BC_WALL, FP64, MIXED, BodyArrays, Case, MarkerArrays)
 
 PI_F32 = float(np.float32(3.1415926536))
 SEED_BASE = 20260000
@@ -236,6 +236,44 @@ def config_suspension(n_spheres=2000, n=256, precision=FP64, rtol=1e-10,
     return Setup(f"suspension-{n}^3-{n_spheres}x{int(D)}h", case, mk, bd,
                  zero_fields(case), {"steps": 500, "solid_fraction": phi,
                                      "seed": seed})
+
+
+def config_batch(n_cases=64, precision=MIXED, seed=SEED_BASE + 5, rtol=1e-10,
+                 max_spheres=40) -> list[Setup]:
+    """Independent settling cases of uneven size for the batched form: grids
+    with nx, ny drawn from 32..96 and nz from 64..192 (multiples of 16), six
+    no-slip walls, 1..max_spheres spheres of D uniform in [8h, 16h]
+    (rho_p/rho_f = 1.5, the single-sphere settling physics: nu = 0.04,
+    g from Re_g = 40 at D = 12h) placed by seeded random sequential addition
+    inside the walls (gap >= 2h; a case keeps the spheres that fit)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nu, rr, u_ref = 0.04, 1.5, 0.1
+    g = u_ref ** 2 / ((rr - 1.0) * 12.0)
+    out = []
+    for q in range(n_cases):
+        nx, ny = (int(16 * rng.integers(2, 7)) for _ in range(2))
+        nz = int(16 * rng.integers(4, 13))
+        want = int(rng.integers(1, max_spheres + 1))
+        diam = np.sort(rng.uniform(8.0, 16.0, size=want))[::-1]
+        L = np.array([nx, ny, nz], dtype=np.float64)
+        cen, dia = [], []
+        for D in diam:
+            R = 0.5 * D
+            lo, hi = R + 1.0, L - R - 1.0
+            if np.any(hi <= lo):
+                continue
+            for _ in range(400):
+                c = lo + rng.random(3) * (hi - lo)
+                if all(np.linalg.norm(c - c2) >= R + 0.5 * d2 + 2.0 for c2, d2 in zip(cen, dia)):
+                    cen.append(c)
+                    dia.append(D)
+                    break
+        case = build_case(nx, ny, nz, [BC_WALL] * 6, visc=nu, gravity=(0.0, 0.0, -g), u_max=2 * u_ref,
+                          precision=precision, rtol=rtol)
+        mk, bd = place_spheres(case, [Sphere(tuple(c), D, rr) for c, D in zip(cen, dia)])
+        out.append(Setup(f"batch-{q}-{nx}x{ny}x{nz}-{len(cen)}", case, mk, bd, zero_fields(case),
+                         {"seed": seed, "spheres": len(cen)}))
+    return out
 
 
 # ---------------------------------------------------------------------------
