@@ -256,7 +256,9 @@ struct ibmdem_ctx {
     double *P[2] = {};
     int pcur = 0;
     double *pcorr = nullptr, *rhs = nullptr;
-    double *rhsl[3] = {};        // RHSul etc. (allocated on demand)
+    double *rhsl[3] = {};        // RHSul, RHSvl, RHSwl (RK3 contexts)
+    double *uold[3] = {};        // uold, vold, wold (RK3 contexts)
+    double *ffull[3] = {};       // the IBM force as full fields (RK3 contexts)
     // PCG (FP64)
     double *cg_x = nullptr, *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_Ap = nullptr;
     int nlev = 0;
@@ -386,6 +388,8 @@ int fluid_bulk(ibmdem_ctx *c, bool apply_now);
 int fluid_divergence(ibmdem_ctx *c, double cc);
 int fluid_project(ibmdem_ctx *c, double cc);
 int fluid_upload_bc(ibmdem_ctx *c);
+int fluid_save_uold(ibmdem_ctx *c);
+int fluid_predictor_general(ibmdem_ctx *c, double a, double b, double cc);
 // poisson.hip
 int poisson_setup(ibmdem_ctx *c);
 int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info);
@@ -399,6 +403,8 @@ int ibm_begin_step(ibmdem_ctx *c);
 int ibm_interpolate(ibmdem_ctx *c);
 int ibm_spread(ibmdem_ctx *c);
 int ibm_download_force(ibmdem_ctx *c, double *fx, double *fy, double *fz);
+// the IBM force of the last spread as full device fields (zero elsewhere)
+int ibm_force_field(ibmdem_ctx *c, double *fx, double *fy, double *fz);
 long ibm_tile_count(const GridDev &g);
 // dem.hip
 int dem_setup(ibmdem_ctx *c);

@@ -55,6 +55,7 @@ int validate(const ibmdem_case *cs) {
         if ((cs->btype[2 * d] == IBMDEM_BC_PERIODIC) != (cs->btype[2 * d + 1] == IBMDEM_BC_PERIODIC))
             return IBMDEM_ERR_ARG;
     if (cs->precision != IBMDEM_FP64 && cs->precision != IBMDEM_MIXED) return IBMDEM_ERR_ARG;
+    if (cs->time_scheme != IBMDEM_SCHEME_EULER && cs->time_scheme != IBMDEM_SCHEME_RK3) return IBMDEM_ERR_ARG;
     if ((long)cs->nx * cs->ny * cs->nz >= (1L << 31)) return IBMDEM_ERR_ARG;
     if (cs->n_markers > 0 && cs->n_bodies == 0) return IBMDEM_ERR_ARG;
     // a marker's 5x5x5 neighbourhood must not wrap onto itself, and marker
@@ -83,6 +84,8 @@ int alloc_all(ibmdem_ctx *c) {
     HIPX(c, hipHostMalloc((void **)&c->conv_host, (c->cs.poisson_max_iter + 2) * sizeof(double), hipHostMallocDefault));
     const int M = c->M, B = c->B;
     A(c->mx, 3L * M); A(c->mv, 3L * M); A(c->fdrag, 3L * M); A(c->fcol, 3L * M);
+    if (c->cs.time_scheme == IBMDEM_SCHEME_RK3)
+        for (int q = 0; q < 3; ++q) { A(c->rhsl[q], n); A(c->uold[q], n); A(c->ffull[q], n); }
     A(c->mvol, M); A(c->mmass, M); A(c->mrad, M); A(c->mbody, M); A(c->mcell, M);
     A(c->mrec, 8L * M);
     A(c->bin_count, n); A(c->bin_start, n + 1); A(c->bin_items, M); A(c->bin_tmp, M);
@@ -160,6 +163,7 @@ void set_host_state(ibmdem_ctx *c, const HostState &h) {
 }
 
 int step_pre(ibmdem_ctx *c) {
+    TRY(fluid_save_uold(c));
     TRY(ibm_begin_step(c));
     for (int it = 0; it < c->cs.n_mdf; ++it) {
         TRY(ibm_interpolate(c));
@@ -214,6 +218,25 @@ int run_part(ibmdem_ctx *c, CapturedPart &part) {
     return IBMDEM_OK;
 }
 
+// ThirdOrderLowStorageRungeKutta (fluid.F90:329-366): Wray's coefficients,
+// c = a + b, three stages of predictor, divergence, solve and projection.
+// Each stage advances from the previous stage's velocity (uold <- u before
+// every VelocityGuess); the reference routine never sets uold, so as written
+// its stages would all start from a stale velocity.
+int step_rk3(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    const double aR[3] = {32.0 / 60.0, 25.0 / 60.0, 45.0 / 60.0};
+    const double bR[3] = {0.0, -17.0 / 60.0, -25.0 / 60.0};
+    for (int s = 0; s < 3; ++s) {
+        const double cR = aR[s] + bR[s];
+        TRY(fluid_save_uold(c));
+        TRY(fluid_predictor(c, aR[s], bR[s], cR));
+        TRY(fluid_divergence(c, cR));
+        TRY(poisson_solve(c, info));
+        TRY(fluid_project(c, cR));
+    }
+    return IBMDEM_OK;
+}
+
 int step_eager(ibmdem_ctx *c, ibmdem_solve_info *info) {
     TRY(step_pre(c));
     TRY(poisson_iterate_to_convergence(c, 1));
@@ -232,6 +255,10 @@ int step_eager(ibmdem_ctx *c, ibmdem_solve_info *info) {
 // profiling, fresh marker upload) runs the whole step eagerly in phase 1 and
 // reports it in *done.
 int step_phase1(ibmdem_ctx *c, ibmdem_solve_info *info, bool *done) {
+    if (c->cs.time_scheme == IBMDEM_SCHEME_RK3) {
+        *done = true;
+        return step_rk3(c, info);
+    }
     if (!(c->graphs.enabled && c->graphs.cg_k >= 0 && c->prof.kernel == 0 && !c->rmax_dirty)) {
         *done = true;
         return step_eager(c, info);
@@ -472,7 +499,11 @@ int ibmdem_download_bodies(ibmdem_ctx *c, ibmdem_bodies *b) {
     return finish(c);
 }
 
-int ibmdem_begin_step(ibmdem_ctx *c) { CTX_OR_ARG(c); return ibm_begin_step(c); }
+int ibmdem_begin_step(ibmdem_ctx *c) {
+    CTX_OR_ARG(c);
+    TRY(fluid_save_uold(c));
+    return ibm_begin_step(c);
+}
 int ibmdem_ibm_interpolate(ibmdem_ctx *c) { CTX_OR_ARG(c); return ibm_interpolate(c); }
 int ibmdem_ibm_spread(ibmdem_ctx *c) { CTX_OR_ARG(c); return ibm_spread(c); }
 int ibmdem_fluid_predictor(ibmdem_ctx *c, double a, double b, double cc) {

@@ -64,7 +64,7 @@ __device__ __forceinline__ CT pdiv(CT x, double d, double inv) {
 
 template <typename CT, bool P2, class LU, class LV, class LW, class LP>
 __device__ __forceinline__ void predictor_cell(const LU &u, const LV &v, const LW &w, const LP &P,
-                                               const GridDev &g, const PredParams &pp, CT out[3]) {
+                                               const GridDev &g, const PredParams &pp, CT R[3], CT Pg[3]) {
     const CT h = CT(0.5);
     const CT visc = CT(pp.visc);
     const CT u0 = CT(u(0, 0, 0)), uE = CT(u(1, 0, 0)), uW = CT(u(-1, 0, 0)), uN = CT(u(0, 1, 0)),
@@ -120,13 +120,10 @@ __device__ __forceinline__ void predictor_cell(const LU &u, const LV &v, const L
     CT Py = pdiv<P2>(pdiv<P2>(p0 - pS, gdy, pp.idy), pp.rhof, pp.irhof);
     CT Pz = pdiv<P2>(pdiv<P2>(p0 - pB, gdz, pp.idz), pp.rhof, pp.irhof);
 
-    CT Ru = -(TIux + TIuy + TIuz) + (TVux + TVuy + TVuz) + CT(pp.cp[0]);
-    CT Rv = -(TIvx + TIvy + TIvz) + (TVvx + TVvy + TVvz) + CT(pp.cp[1]);
-    CT Rw = -(TIwx + TIwy + TIwz) + (TVwx + TVwy + TVwz) + CT(pp.cp[2]);
-    const CT a = CT(pp.a), c = CT(pp.c);
-    out[0] = a * Ru - c * Px;
-    out[1] = a * Rv - c * Py;
-    out[2] = a * Rw - c * Pz;
+    R[0] = -(TIux + TIuy + TIuz) + (TVux + TVuy + TVuz) + CT(pp.cp[0]);
+    R[1] = -(TIvx + TIvy + TIvz) + (TVvx + TVvy + TVvz) + CT(pp.cp[1]);
+    R[2] = -(TIwx + TIwy + TIwz) + (TVwx + TVwy + TVwz) + CT(pp.cp[2]);
+    Pg[0] = Px; Pg[1] = Py; Pg[2] = Pz;
 }
 
 __device__ __forceinline__ void block_sum3(double s[3], double *part) {
@@ -159,13 +156,15 @@ __global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *_
     double s[3] = {0.0, 0.0, 0.0};
     if (i <= g.nx && j <= g.ny) {
         const long o = cidx(g, i, j, k);
-        CT inc[3];
+        CT R[3], Pg[3], inc[3];
         Map3 mx, my, mz;
         mx.init(i, g.nx, g.bt[0], g.bt[1]);
         my.init(j, g.ny, g.bt[2], g.bt[3]);
         mz.init(k, g.nz, g.bt[4], g.bt[5]);
         predictor_cell<CT, P2>(Tap<0>{u, &g, &mx, &my, &mz}, Tap<1>{v, &g, &mx, &my, &mz}, Tap<2>{w, &g, &mx, &my, &mz},
-                           TapP{P, &g, &mx, &my, &mz}, g, pp, inc);
+                           TapP{P, &g, &mx, &my, &mz}, g, pp, R, Pg);
+        const CT a = CT(pp.a), c = CT(pp.c);
+        for (int q = 0; q < 3; ++q) inc[q] = a * R[q] - c * Pg[q];
         const double dt = pp.dt;
         double nu_ = wall_face<0>(g, i, j, k) ? 0.0 : u[o] + dt * (double)inc[0];
         double nv_ = wall_face<1>(g, i, j, k) ? 0.0 : v[o] + dt * (double)inc[1];
@@ -178,6 +177,65 @@ __global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *_
     if (BULK) {
         const long blk = (long)bx + (long)gridDim.x * (by + (long)gridDim.y * bz);
         block_sum3(s, partial + 3 * blk);
+    }
+}
+
+// VelocityGuess(a,b,c) in the reference's own form (fluid.F90:116-131):
+// RHS = -sum TI + sum TV + CP + f; u = uold + dt*(a*RHS + b*RHSl - c*gradP);
+// RHSl <- RHS. Used by contexts created for the low-storage RK3 scheme, which
+// keep uold, RHSl and the full-field IBM force.
+struct GeneralArgs {
+    const double *uold[3];
+    const double *f[3];
+    double *rhsl[3];
+    double b;
+};
+template <typename CT, bool BULK, bool P2>
+__global__ __launch_bounds__(BX *BY) void k_predictor_general(GridDev g, const double *__restrict__ u,
+                                                              const double *__restrict__ v,
+                                                              const double *__restrict__ w,
+                                                              const double *__restrict__ P, double *__restrict__ un,
+                                                              double *__restrict__ vn, double *__restrict__ wn,
+                                                              PredParams pp, GeneralArgs ga,
+                                                              double *__restrict__ partial) {
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (i <= g.nx && j <= g.ny) {
+        const long o = cidx(g, i, j, k);
+        CT R[3], Pg[3];
+        Map3 mx, my, mz;
+        mx.init(i, g.nx, g.bt[0], g.bt[1]);
+        my.init(j, g.ny, g.bt[2], g.bt[3]);
+        mz.init(k, g.nz, g.bt[4], g.bt[5]);
+        predictor_cell<CT, P2>(Tap<0>{u, &g, &mx, &my, &mz}, Tap<1>{v, &g, &mx, &my, &mz}, Tap<2>{w, &g, &mx, &my, &mz},
+                           TapP{P, &g, &mx, &my, &mz}, g, pp, R, Pg);
+        const CT a = CT(pp.a), b = CT(ga.b), c = CT(pp.c);
+        const bool face[3] = {wall_face<0>(g, i, j, k), wall_face<1>(g, i, j, k), wall_face<2>(g, i, j, k)};
+        double *out[3] = {un, vn, wn};
+        for (int q = 0; q < 3; ++q) {
+            const CT rhs = R[q] + CT(ga.f[q][o]);
+            const CT val = a * rhs + b * CT(ga.rhsl[q][o]) - c * Pg[q];
+            const double nv = face[q] ? 0.0 : ga.uold[q][o] + pp.dt * (double)val;
+            out[q][o] = nv;
+            ga.rhsl[q][o] = (double)rhs;
+            s[q] = nv;
+        }
+    }
+    if (BULK) {
+        const long blk = (long)bx + (long)gridDim.x * (by + (long)gridDim.y * bz);
+        block_sum3(s, partial + 3 * blk);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy3(long n, const double *__restrict__ a, const double *__restrict__ b,
+                                               const double *__restrict__ c, double *__restrict__ x,
+                                               double *__restrict__ y, double *__restrict__ z) {
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        x[q] = a[q];
+        y[q] = b[q];
+        z[q] = c[q];
     }
 }
 
@@ -357,10 +415,23 @@ int reduce_sum(ibmdem_ctx *c, const double *a, const double *b, long n, int slot
     return IBMDEM_OK;
 }
 
+// uold <- u at the start of a step (FirstOrderForwardEuler, fluid.F90:258);
+// only contexts for the general VelocityGuess keep uold
+int fluid_save_uold(ibmdem_ctx *c) {
+    if (!c->uold[0]) return IBMDEM_OK;
+    const int cur = c->vcur;
+    hipLaunchKernelGGL(k_copy3, dim3(4096), dim3(256), 0, c->stream, c->n, c->u[cur], c->v[cur], c->w[cur],
+                       c->uold[0], c->uold[1], c->uold[2]);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
 int fluid_predictor(ibmdem_ctx *c, double a, double b, double cc) {
+    if (c->uold[0]) return fluid_predictor_general(c, a, b, cc);
     if (b != 0.0 || a != 1.0)
         return set_error(c, IBMDEM_ERR_UNSUPPORTED,
-                         "predictor supports the forward-Euler coefficients (a=1, b=0)");
+                         "general VelocityGuess coefficients need a context created with time_scheme = "
+                         "IBMDEM_SCHEME_RK3 (it keeps uold and RHSl)");
     const GridDev &g = c->g;
     dim3 grid = field_grid(g), blk(BX, BY);
     PredParams pp{c->cs.dt, c->cs.visc, c->cs.rhof, a, cc, {c->cs.cp[0], c->cs.cp[1], c->cs.cp[2]}};
@@ -390,6 +461,54 @@ int fluid_predictor(ibmdem_ctx *c, double a, double b, double cc) {
     c->bulk_pending = false;
     if (bulk) {
         // partials of this launch feed the bulk finalize
+        const long nparts = (long)grid.x * grid.y * grid.z;
+        hipLaunchKernelGGL(k_bulk_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nparts, c->scal,
+                           c->cs.bulk_switch[0], c->cs.bulk_switch[1], c->cs.bulk_switch[2],
+                           c->cs.bulk_target[0], c->cs.bulk_target[1], c->cs.bulk_target[2],
+                           (double)((long)g.nx * g.ny * g.nz), c->cs.dt);
+        CHECK_LAUNCH(c);
+    }
+    return IBMDEM_OK;
+}
+
+int fluid_predictor_general(ibmdem_ctx *c, double a, double b, double cc) {
+    const GridDev &g = c->g;
+    dim3 grid = field_grid(g), blk(BX, BY);
+    PredParams pp{c->cs.dt, c->cs.visc, c->cs.rhof, a, cc, {c->cs.cp[0], c->cs.cp[1], c->cs.cp[2]}};
+    pp.idx = 1.0 / g.dx; pp.idy = 1.0 / g.dy; pp.idz = 1.0 / g.dz;
+    pp.idx2 = 1.0 / (g.dx * g.dx); pp.idy2 = 1.0 / (g.dy * g.dy); pp.idz2 = 1.0 / (g.dz * g.dz);
+    pp.irhof = 1.0 / c->cs.rhof;
+    int rc = ibm_force_field(c, c->ffull[0], c->ffull[1], c->ffull[2]);
+    if (rc) return rc;
+    GeneralArgs ga;
+    for (int q = 0; q < 3; ++q) {
+        ga.uold[q] = c->uold[q];
+        ga.f[q] = c->ffull[q];
+        ga.rhsl[q] = c->rhsl[q];
+    }
+    ga.b = b;
+    const int cur = c->vcur, nxt = 1 - c->vcur;
+    const bool bulk = c->cs.bulk_switch[0] == 1 || c->cs.bulk_switch[1] == 1 || c->cs.bulk_switch[2] == 1;
+    const bool p2 = is_pow2(g.dx) && is_pow2(g.dy) && is_pow2(g.dz) && is_pow2(c->cs.rhof);
+    const bool mixed = c->cs.precision == IBMDEM_MIXED;
+#define LAUNCH_GEN(CT, BK, PT)                                                                                      \
+    hipLaunchKernelGGL((k_predictor_general<CT, BK, PT>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],          \
+                       c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, ga, c->partial)
+    prof_begin(c, IBMDEM_K_PREDICTOR);
+    if (mixed) {
+        if (bulk) { if (p2) LAUNCH_GEN(float, true, true); else LAUNCH_GEN(float, true, false); }
+        else { if (p2) LAUNCH_GEN(float, false, true); else LAUNCH_GEN(float, false, false); }
+    } else {
+        if (bulk) { if (p2) LAUNCH_GEN(double, true, true); else LAUNCH_GEN(double, true, false); }
+        else { if (p2) LAUNCH_GEN(double, false, true); else LAUNCH_GEN(double, false, false); }
+    }
+#undef LAUNCH_GEN
+    CHECK_LAUNCH(c);
+    // reads u, v, w, P, uold (3), f (3), RHSl (3); writes u*, v*, w*, RHSl (3)
+    prof_end(c, IBMDEM_K_PREDICTOR, 19.0 * 8.0 * (double)c->n);
+    c->vcur = nxt;
+    c->bulk_pending = false;
+    if (bulk) {
         const long nparts = (long)grid.x * grid.y * grid.z;
         hipLaunchKernelGGL(k_bulk_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nparts, c->scal,
                            c->cs.bulk_switch[0], c->cs.bulk_switch[1], c->cs.bulk_switch[2],

@@ -244,6 +244,10 @@ __global__ __launch_bounds__(T) void k_flag_compact(long n, unsigned char *__res
         }
 }
 
+__global__ __launch_bounds__(T) void k_zero(long n, double *__restrict__ a) {
+    for (long q = (long)blockIdx.x * T + threadIdx.x; q < n; q += (long)gridDim.x * T) a[q] = 0.0;
+}
+
 // one source-cell coordinate mapped to the interior with its image kind:
 // 0 direct, 1 periodic (image = x + coordShift), 2 wall mirror
 // (image = 2*coordMirror - x)
@@ -676,6 +680,22 @@ int ibm_spread(ibmdem_ctx *c) {
     }
     c->first_mdf = false;
     c->spread_valid = true;
+    return IBMDEM_OK;
+}
+
+int ibm_force_field(ibmdem_ctx *c, double *fx, double *fy, double *fz) {
+    hipStream_t s = c->stream;
+    const long n = c->n;
+    double *f[3] = {fx, fy, fz};
+    for (int q = 0; q < 3; ++q) {
+        hipLaunchKernelGGL(k_zero, dim3(4096), dim3(T), 0, s, n, f[q]);
+        CHECK_LAUNCH(c);
+    }
+    if (c->M > 0 && c->spread_valid) {
+        hipLaunchKernelGGL(k_scatter_force, dim3(c->act_grid), dim3(64), 0, s, c->act_meta, c->act_list,
+                           tile_geom(c->g), c->g, c->fact, 64L * c->act_cap, c->tile_mask, fx, fy, fz);
+        CHECK_LAUNCH(c);
+    }
     return IBMDEM_OK;
 }
 

@@ -26,6 +26,9 @@ module ibmdem_api
     ! precision forms
     integer(c_int32_t), parameter :: IBMDEM_FP64 = 0
     integer(c_int32_t), parameter :: IBMDEM_MIXED = 1
+    ! time schemes
+    integer(c_int32_t), parameter :: IBMDEM_SCHEME_EULER = 0
+    integer(c_int32_t), parameter :: IBMDEM_SCHEME_RK3 = 1
 
     ! every component defaults to 0: the library reads 0 as "use the default"
     ! for the multigrid and tile parameters
@@ -55,7 +58,8 @@ module ibmdem_api
         integer(c_int32_t) :: mg_coarse_sweeps = 0
         integer(c_int32_t) :: mg_max_levels = 0
         integer(c_int32_t) :: stencil_tile = 0
-        integer(c_int32_t) :: reserved(7) = 0
+        integer(c_int32_t) :: time_scheme = 0
+        integer(c_int32_t) :: reserved(6) = 0
     end type
 
     ! markers (reference type Parcel), component-major SoA

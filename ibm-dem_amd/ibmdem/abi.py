@@ -27,6 +27,8 @@ BC_PERIODIC = 0
 BC_WALL = 1
 FP64 = 0
 MIXED = 1
+SCHEME_EULER = 0
+SCHEME_RK3 = 1
 
 K_PREDICTOR, K_MG_SMOOTH, K_MATVEC, K_SPREAD, K_INTERP, K_CONTACT, K_PROJECT, \
     K_DIVERGENCE, K_CG_UPDATE = range(1, 10)
@@ -64,7 +66,8 @@ class Case(C.Structure):
         ("mg_coarse_sweeps", _i32),
         ("mg_max_levels", _i32),
         ("stencil_tile", _i32),
-        ("reserved", _i32 * 7),
+        ("time_scheme", _i32),
+        ("reserved", _i32 * 6),
     ]
 
 

@@ -93,7 +93,7 @@ def build_case(nx, ny, nz, btype, *, h=1.0, visc=0.04, rhof=1.0, dt=None,
                bulk_switch=(0, 0, 0), bulk_target=(0.0, 0.0, 0.0),
                precision=FP64, rtol=1e-10, max_iter=200, n_mdf=5,
                couple_interval=10, u_max=0.2, n_markers=0, n_bodies=0,
-               max_pairs=200, mg_pre=2, mg_post=2, mg_coarse=24) -> Case:
+               max_pairs=200, mg_pre=2, mg_post=2, mg_coarse=24, time_scheme=0) -> Case:
     c = Case()
     c.nx, c.ny, c.nz = nx, ny, nz
     for s in range(6):
@@ -123,6 +123,7 @@ def build_case(nx, ny, nz, btype, *, h=1.0, visc=0.04, rhof=1.0, dt=None,
     c.mg_pre, c.mg_post, c.mg_coarse_sweeps = mg_pre, mg_post, mg_coarse
     c.mg_max_levels = 0
     c.stencil_tile = 0
+    c.time_scheme = time_scheme
     return c
 
 

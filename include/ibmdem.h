@@ -50,6 +50,18 @@ extern "C" {
 #define IBMDEM_MIXED 1   /* FP32 stencil predictor and FP32 multigrid inside FP64
                             CG (residual, inner products, projection, DEM in FP64) */
 
+/* ---- time schemes --------------------------------------------------------
+ * EULER: FirstOrderForwardEuler (fluid.F90:252-327) with the IBM/DEM
+ *   coupling; the predictor takes the forward-Euler coefficients (1, 0, 1).
+ * RK3: the context also keeps uold, RHSl and the full-field IBM force, so
+ *   VelocityGuess(a, b, c) takes any coefficients, and ibmdem_step runs
+ *   ThirdOrderLowStorageRungeKutta (fluid.F90:329-366): three stages of
+ *   predictor, divergence, Poisson solve and projection, fluid only, as the
+ *   reference; each stage starts from the previous stage's velocity (Wray's
+ *   scheme; the reference routine never sets uold). */
+#define IBMDEM_SCHEME_EULER 0
+#define IBMDEM_SCHEME_RK3   1
+
 /* ---- case description --------------------------------------------------- */
 typedef struct ibmdem_case {
     int32_t nx, ny, nz;          /* interior cells                          */
@@ -77,7 +89,8 @@ typedef struct ibmdem_case {
     int32_t mg_coarse_sweeps;    /* sweeps on the coarsest level (0 -> 24)  */
     int32_t mg_max_levels;       /* 0 -> as many as the grid allows         */
     int32_t stencil_tile;        /* predictor tile choice (0 -> default)    */
-    int32_t reserved[7];
+    int32_t time_scheme;         /* IBMDEM_SCHEME_*                         */
+    int32_t reserved[6];
 } ibmdem_case;
 
 /* Markers (reference type Parcel, ZJS_VARS.F90:6-12), SoA. Pointers may be

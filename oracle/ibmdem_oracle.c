@@ -1141,6 +1141,7 @@ int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
      * not wrap onto itself */
     if (cs->n_markers > 0 && (cs->nx < 5 || cs->ny < 5 || cs->nz < 5)) return IBMDEM_ERR_ARG;
     if (cs->n_markers >= (1 << 23)) return IBMDEM_ERR_UNSUPPORTED;
+    if (cs->time_scheme != IBMDEM_SCHEME_EULER && cs->time_scheme != IBMDEM_SCHEME_RK3) return IBMDEM_ERR_ARG;
     ibmdem_ctx *c = (ibmdem_ctx *)xcalloc(1, sizeof *c);
     c->cs = *cs;
     if (c->cs.n_mdf <= 0) c->cs.n_mdf = 5;
@@ -1409,8 +1410,33 @@ int ibmdem_cloud_evolution(ibmdem_ctx *c) {
     return IBMDEM_OK;
 }
 
+/* ThirdOrderLowStorageRungeKutta (fluid.F90:329-366): aRK, bRK as the
+ * reference sets them, cRK = aRK + bRK; fluid only. Each stage advances from
+ * the previous stage's velocity (uold <- u before every VelocityGuess), which
+ * is Wray's low-storage scheme; the reference routine never sets uold, so as
+ * written its stages would all start from a stale velocity. */
+static int step_rk3(ibmdem_ctx *c, ibmdem_solve_info *info) {
+    double aRK[3], bRK[3], cRK[3];
+    aRK[0] = 32.0 / 60.0; bRK[0] = 0.0;
+    aRK[1] = 25.0 / 60.0; bRK[1] = -17.0 / 60.0;
+    aRK[2] = 45.0 / 60.0; bRK[2] = -25.0 / 60.0;
+    for (int s = 0; s < 3; ++s) cRK[s] = aRK[s] + bRK[s];
+    for (int s = 0; s < 3; ++s) {
+        memcpy(c->uold, c->u, c->nbox * sizeof(double));
+        memcpy(c->vold, c->v, c->nbox * sizeof(double));
+        memcpy(c->wold, c->w, c->nbox * sizeof(double));
+        ibmdem_fluid_predictor(c, aRK[s], bRK[s], cRK[s]);
+        ibmdem_divergence(c, cRK[s]);
+        int rc = ibmdem_poisson_solve(c, info);
+        if (rc) return rc;
+        ibmdem_project(c, cRK[s]);
+    }
+    return IBMDEM_OK;
+}
+
 int ibmdem_step(ibmdem_ctx *c, ibmdem_solve_info *info) {
     if (!c) return IBMDEM_ERR_ARG;
+    if (c->cs.time_scheme == IBMDEM_SCHEME_RK3) return step_rk3(c, info);
     int rc;
     ibmdem_begin_step(c);
     for (int it = 0; it < c->cs.n_mdf; ++it) {

@@ -42,7 +42,7 @@ def test_oracle_exports_every_symbol():
 def test_struct_layouts():
     # sizes from the header: 3+6 ints, 3+3 doubles, ... checked by offsets
     assert abi.Case.dx.offset == 40
-    assert abi.Case.reserved.offset + C.sizeof(C.c_int32 * 7) <= C.sizeof(abi.Case)
+    assert abi.Case.reserved.offset + C.sizeof(C.c_int32 * 6) <= C.sizeof(abi.Case)
     assert C.sizeof(abi.Markers) == 8 * C.sizeof(C.c_void_p)
     assert C.sizeof(abi.Bodies) == 18 * C.sizeof(C.c_void_p)
     assert C.sizeof(abi.SolveInfo) == 24

@@ -362,6 +362,36 @@ def test_taylor_green_second_order():
     assert 1.7 < order < 2.5, (e1, e2, order)
 
 
+def taylor_green_run(n, dt, T, scheme):
+    nu = 0.1
+    h = 2 * np.pi / n
+    steps = int(round(T / dt))
+    case = cases.build_case(n, n, 2, [BC_PERIODIC] * 6, h=h, visc=nu, dt=T / steps, rtol=1e-13,
+                            max_iter=300, time_scheme=scheme)
+    case.n_markers = 0
+    case.n_bodies = 0
+    st = staggered_coords((n, n, 2), h)
+    u = np.cos(st["u"][0]) * np.sin(st["u"][1])
+    v = -np.sin(st["v"][0]) * np.cos(st["v"][1])
+    p = -0.25 * (np.cos(2 * st["w"][0]) + np.cos(2 * st["w"][1]))
+    with oracle.simulation(case) as s:
+        s.upload_fields(u, v, np.zeros_like(u), p)
+        s.run(steps)
+        return s.download_fields()[0]
+
+
+def test_time_order_euler_first_rk3_third():
+    """Temporal order on a fixed grid against a run with an 8x smaller step:
+    forward Euler (FirstOrderForwardEuler) is first order; the low-storage
+    RK3 with Wray's coefficients (fluid.F90:333-336) is third order, which a
+    wrong coefficient or a stale stage velocity would break."""
+    for scheme, lo, hi in ((0, 0.9, 1.4), (1, 2.7, 3.3)):
+        ref = taylor_green_run(16, 0.8 / 128, 0.8, scheme)
+        e = [np.abs(taylor_green_run(16, dt, 0.8, scheme) - ref).max() for dt in (0.1, 0.05)]
+        order = math.log(e[0] / e[1], 2)
+        assert lo < order < hi, (scheme, e, order)
+
+
 def test_projection_makes_velocity_divergence_free():
     setup = cases.small_periodic(n=(20, 16, 18), n_spheres=3, D=6.0, gap=0.7, rtol=1e-12)
     with oracle.simulation(setup.case) as s:
