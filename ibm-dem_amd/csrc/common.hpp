@@ -200,7 +200,32 @@ struct LevelDev {
     double idiag[27];            // 1/diag indexed by neighbour counts (cx,cy,cz)
     double diag[27];
     int rb_valid;                // red-black colouring consistent (no odd periodic dim)
+    int split;                   // colour-split storage (nx even), see lidx
+    int half;                    // nx*ny*nz / 2 when split
 };
+
+// Storage of a level's cell (i,j,k), 0-based. With nx even the level is
+// stored colour-split: the (i+j+k)-even cells first, then the odd ones, each
+// half in (k, j, i/2) order, so a red-black half sweep reads and writes
+// whole cache lines of one colour; otherwise x fastest.
+__host__ __device__ __forceinline__ int lidx(const LevelDev &L, int i, int j, int k) {
+    const int row = k * L.ny + j;
+    if (!L.split) return row * L.nx + i;
+    return ((i + j + k) & 1) * L.half + row * (L.nx >> 1) + (i >> 1);
+}
+// the cell stored at position q
+__host__ __device__ __forceinline__ void lcell(const LevelDev &L, int q, int &i, int &j, int &k) {
+    if (!L.split) {
+        i = q % L.nx; j = (q / L.nx) % L.ny; k = q / (L.nx * L.ny);
+        return;
+    }
+    const int c = q >= L.half ? 1 : 0;
+    const int r = q - c * L.half, h = L.nx >> 1;
+    const int row = r / h;
+    j = row % L.ny;
+    k = row / L.ny;
+    i = 2 * (r - row * h) + ((j + k + c) & 1);
+}
 
 template <typename T>
 struct MGLevel {

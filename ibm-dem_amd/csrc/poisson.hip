@@ -50,24 +50,21 @@ LaunchShape shape_for(int nx, int ny, int nz) {
 }
 
 // nbsum in the oracle's association: ((xw+xe)*ihx2 + (xs+xn)*ihy2) + (xb+xt)*ihz2,
-// wall neighbours 0, periodic neighbours wrapped; neighbour offsets are
-// selected per thread rather than taken on a divergent boundary path
+// wall neighbours 0, periodic neighbours wrapped
 template <typename T>
 __device__ __forceinline__ T nbsum(const T *__restrict__ x, const LevelDev &L, int i, int j, int k) {
     const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
-    const long sy = L.nx, sz = (long)L.nx * L.ny;
-    const long o = k * sz + j * sy + i;
     const bool pxw = L.bt[0] == IBMDEM_BC_PERIODIC, pyw = L.bt[2] == IBMDEM_BC_PERIODIC,
                pzw = L.bt[4] == IBMDEM_BC_PERIODIC;
     const bool hw = i > 0 || pxw, he = i < L.nx - 1 || pxw;
     const bool hs = j > 0 || pyw, hn = j < L.ny - 1 || pyw;
     const bool hb = k > 0 || pzw, ht = k < L.nz - 1 || pzw;
-    const long ow = i > 0 ? o - 1 : o + (L.nx - 1), oe = i < L.nx - 1 ? o + 1 : o - (L.nx - 1);
-    const long os = j > 0 ? o - sy : o + (L.ny - 1) * sy, on = j < L.ny - 1 ? o + sy : o - (L.ny - 1) * sy;
-    const long ob = k > 0 ? o - sz : o + (L.nz - 1) * sz, ot = k < L.nz - 1 ? o + sz : o - (L.nz - 1) * sz;
-    const T xw = hw ? x[ow] : T(0), xe = he ? x[oe] : T(0);
-    const T xs = hs ? x[os] : T(0), xn = hn ? x[on] : T(0);
-    const T xb = hb ? x[ob] : T(0), xt = ht ? x[ot] : T(0);
+    const int iw = i > 0 ? i - 1 : L.nx - 1, ie = i < L.nx - 1 ? i + 1 : 0;
+    const int js = j > 0 ? j - 1 : L.ny - 1, jn = j < L.ny - 1 ? j + 1 : 0;
+    const int kb = k > 0 ? k - 1 : L.nz - 1, kt = k < L.nz - 1 ? k + 1 : 0;
+    const T xw = hw ? x[lidx(L, iw, j, k)] : T(0), xe = he ? x[lidx(L, ie, j, k)] : T(0);
+    const T xs = hs ? x[lidx(L, i, js, k)] : T(0), xn = hn ? x[lidx(L, i, jn, k)] : T(0);
+    const T xb = hb ? x[lidx(L, i, j, kb)] : T(0), xt = ht ? x[lidx(L, i, j, kt)] : T(0);
     const T sx = (xw + xe) * ihx2;
     const T syy = (xs + xn) * ihy2;
     const T szz = (xb + xt) * ihz2;
@@ -92,10 +89,10 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
     const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     if (j >= L.ny) return;
     const int i = 2 * ip + ((j + k + color) & 1);
-    const long o = ((long)k * L.ny + j) * L.nx + i;
+    const int o = lidx(L, i, j, k);
     if (zero_init) {
         const int io = 2 * ip + ((j + k + color + 1) & 1);
-        if (io < L.nx) x[o - i + io] = T(0);
+        if (io < L.nx) x[lidx(L, io, j, k)] = T(0);
         if (i >= L.nx) return;
         const T id = T(L.idiag[diag_index(L, i, j, k)]);
         x[o] = (T(b[o]) + T(0)) * id;
@@ -114,7 +111,7 @@ __global__ __launch_bounds__(RB) void k_residual(LevelDev L, const T *__restrict
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     if (i >= L.nx || j >= L.ny) return;
-    const long o = ((long)k * L.ny + j) * L.nx + i;
+    const int o = lidx(L, i, j, k);
     const T dg = T(L.diag[diag_index(L, i, j, k)]);
     r[o] = T(b[o]) - (dg * x[o] - nbsum(x, L, i, j, k));
 }
@@ -142,14 +139,12 @@ __global__ __launch_bounds__(RB) void k_restrict(LevelDev F, const T *__restrict
     }
     T s = T(0);
     for (int c3 = 0; c3 < 4; ++c3)
-        for (int c2 = 0; c2 < 4; ++c2) {
-            const long row = ((long)fk[c3] * F.ny + fj[c2]) * F.nx;
+        for (int c2 = 0; c2 < 4; ++c2)
             for (int c1 = 0; c1 < 4; ++c1) {
                 T w = wt[c3] * wt[c2] * wt[c1];
-                s = s + w * rf[row + fi[c1]];
+                s = s + w * rf[lidx(F, fi[c1], fj[c2], fk[c3])];
             }
-        }
-    bc[((long)K * Cl.ny + J) * Cl.nx + I] = T(0.125) * s;
+    bc[lidx(Cl, I, J, K)] = T(0.125) * s;
 }
 
 __device__ __forceinline__ void parents(int f, int nc, int lo, int hi, int &p0, int &p1) {
@@ -178,9 +173,9 @@ __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, 
         for (int c2 = 0; c2 < 2; ++c2)
             for (int c1 = 0; c1 < 2; ++c1) {
                 T w = wt[c3] * wt[c2] * wt[c1];
-                s = s + w * xc[((long)pk[c3] * Cl.ny + pj[c2]) * Cl.nx + pi[c1]];
+                s = s + w * xc[lidx(Cl, pi[c1], pj[c2], pk[c3])];
             }
-    const long o = ((long)k * F.ny + j) * F.nx + i;
+    const int o = lidx(F, i, j, k);
     xf[o] = xf[o] + s;
 }
 
@@ -215,7 +210,7 @@ __global__ __launch_bounds__(64) void k_coarse_lds(LevelDev L, T *__restrict__ x
             }
         id[q] = T(L.idiag[diag_index(L, i, j, k)]);
         col[q] = (unsigned char)((i + j + k) & 1);
-        b[q] = bg[q];
+        b[q] = bg[lidx(L, i, j, k)];
         x[q] = T(0);
     }
     if (threadIdx.x == 0) x[n] = T(0);
@@ -243,7 +238,7 @@ __global__ __launch_bounds__(64) void k_coarse_lds(LevelDev L, T *__restrict__ x
             __syncthreads();
         }
     }
-    for (int q = threadIdx.x; q < n; q += 64) xg[q] = x[q];
+    for (int q = threadIdx.x; q < n; q += 64) xg[lidx(L, q % L.nx, (q / L.nx) % L.ny, q / (L.nx * L.ny))] = x[q];
 }
 
 template <typename T>
@@ -255,7 +250,8 @@ __global__ __launch_bounds__(COARSE_T) void k_coarse(LevelDev L, T *__restrict__
     __syncthreads();
     auto half = [&](int color) {
         for (long q = threadIdx.x; q < n; q += COARSE_T) {
-            const int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+            int i, j, k;
+            lcell(L, (int)q, i, j, k);
             if (((i + j + k) & 1) != color) continue;
             x[q] = (b[q] + nbsum(x, L, i, j, k)) * T(L.idiag[diag_index(L, i, j, k)]);
         }
@@ -268,7 +264,8 @@ __global__ __launch_bounds__(COARSE_T) void k_coarse(LevelDev L, T *__restrict__
         const T om = T(0.8);
         for (int s = 0; s < 2 * sweeps; ++s) {
             for (long q = threadIdx.x; q < n; q += COARSE_T) {
-                const int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+                int i, j, k;
+                lcell(L, (int)q, i, j, k);
                 T xn = (b[q] + nbsum(x, L, i, j, k)) * T(L.idiag[diag_index(L, i, j, k)]);
                 tmp[q] = x[q] + om * (xn - x[q]);
             }
@@ -303,7 +300,8 @@ __global__ __launch_bounds__(RED_T) void k_matvec_dot(LevelDev L, const double *
     const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
     double s = 0.0;
     for (long q = lo + threadIdx.x; q < hi; q += RED_T) {
-        const int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+        int i, j, k;
+        lcell(L, (int)q, i, j, k);
         const double a = L.diag[diag_index(L, i, j, k)] * p[q] - nbsum(p, L, i, j, k);
         Ap[q] = a;
         s += p[q] * a;
@@ -376,12 +374,16 @@ __global__ __launch_bounds__(RED_T) void k_cg_p(long n, double *__restrict__ p, 
         p[q] = z[q] + beta * p[q];
 }
 
-// r = rhs with the pinned row, x = 0
-__global__ __launch_bounds__(RED_T) void k_cg_init(long n, const double *__restrict__ rhs, double *__restrict__ r,
-                                                   double *__restrict__ x, double *__restrict__ scal) {
+// r = rhs with the pinned row, x = 0; r and x in level-0 storage, rhs in
+// field order (cell (0,0,0) is position 0 of both)
+__global__ __launch_bounds__(RED_T) void k_cg_init(LevelDev L, long n, const double *__restrict__ rhs,
+                                                   double *__restrict__ r, double *__restrict__ x,
+                                                   double *__restrict__ scal) {
     if (blockIdx.x == 0 && threadIdx.x == 0) { scal[SC_CONV] = 0.0; scal[SC_ITER] = 0.0; }
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
-        r[q] = q == 0 ? -scal[SC_PIN] : rhs[q];
+        int i, j, k;
+        lcell(L, (int)q, i, j, k);
+        r[q] = q == 0 ? -scal[SC_PIN] : rhs[((long)k * L.ny + j) * L.nx + i];
         x[q] = 0.0;
     }
 }
@@ -396,10 +398,16 @@ __global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, c
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) d[q] = TD(s[q]);
 }
 
-// gauge P'(1,1,1) = 0: pcorr = x - x[0]
-__global__ __launch_bounds__(RED_T) void k_gauge(long n, const double *__restrict__ x, double *__restrict__ pc) {
+// gauge P'(1,1,1) = 0: pcorr = x - x(1,1,1), x in level-0 storage, pcorr in
+// field order
+__global__ __launch_bounds__(RED_T) void k_gauge(LevelDev L, long n, const double *__restrict__ x,
+                                                 double *__restrict__ pc) {
     const double x0 = x[0];
-    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) pc[q] = x[q] - x0;
+    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
+        int i, j, k;
+        lcell(L, (int)q, i, j, k);
+        pc[((long)k * L.ny + j) * L.nx + i] = x[q] - x0;
+    }
 }
 
 int stream_blocks(long n) { return (int)std::min<long>(4096, std::max<long>(1, (n + RED_T - 1) / RED_T)); }
@@ -424,6 +432,8 @@ LevelDev make_level(int nx, int ny, int nz, const int bt[6], const double h[3]) 
     L.rb_valid = 1;
     for (int d = 0; d < 3; ++d)
         if (bt[2 * d] == IBMDEM_BC_PERIODIC && (n[d] % 2)) L.rb_valid = 0;
+    L.split = (nx % 2) == 0 ? 1 : 0;
+    L.half = L.split ? nx * ny * nz / 2 : 0;
     return L;
 }
 
@@ -588,7 +598,8 @@ int poisson_begin(ibmdem_ctx *c) {
     int rc;
     // pinned row: b[0] = -sum_{q>0} b[q]
     if ((rc = reduce_sum(c, c->rhs, nullptr, n, SC_PIN, 1))) return rc;
-    hipLaunchKernelGGL(k_cg_init, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->rhs, c->cg_r, c->cg_x, c->scal);
+    hipLaunchKernelGGL(k_cg_init, dim3(stream_blocks(n)), dim3(RED_T), 0, s, c->lev64[0].d, n, c->rhs, c->cg_r,
+                       c->cg_x, c->scal);
     CHECK_LAUNCH(c);
     if ((rc = dot_into(c, c->cg_r, c->cg_r, SC_BNORM2, 4))) return rc;
     if ((rc = precondition(c))) return rc;
@@ -651,7 +662,8 @@ int poisson_check(ibmdem_ctx *c, ibmdem_solve_info *info) {
 
 // P' = x - x(1,1,1): the reference's gauge (poisson.F90:78,97-100)
 int poisson_gauge(ibmdem_ctx *c) {
-    hipLaunchKernelGGL(k_gauge, dim3(stream_blocks(c->n)), dim3(RED_T), 0, c->stream, c->n, c->cg_x, c->pcorr);
+    hipLaunchKernelGGL(k_gauge, dim3(stream_blocks(c->n)), dim3(RED_T), 0, c->stream, c->lev64[0].d, c->n, c->cg_x,
+                       c->pcorr);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
