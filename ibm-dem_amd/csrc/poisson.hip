@@ -103,17 +103,22 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
     x[o] = (T(b[o]) + nbsum(x, L, i, j, k)) * id;
 }
 
-// r = b - A x
+// r = b - A x. Each thread owns the x-pair (2p, 2p+1): one cell of each
+// colour, at the same position of the two colour halves when split.
 template <typename T, typename TB>
 __global__ __launch_bounds__(RB) void k_residual(LevelDev L, const T *__restrict__ x, const TB *__restrict__ b,
                                                  T *__restrict__ r, const double *__restrict__ stop) {
     if (*stop != 0.0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
-    if (i >= L.nx || j >= L.ny) return;
-    const int o = lidx(L, i, j, k);
-    const T dg = T(L.diag[diag_index(L, i, j, k)]);
-    r[o] = T(b[o]) - (dg * x[o] - nbsum(x, L, i, j, k));
+    if (j >= L.ny) return;
+    for (int h = 0; h < 2; ++h) {
+        const int i = 2 * ip + h;
+        if (i >= L.nx) return;
+        const int o = lidx(L, i, j, k);
+        const T dg = T(L.diag[diag_index(L, i, j, k)]);
+        r[o] = T(b[o]) - (dg * x[o] - nbsum(x, L, i, j, k));
+    }
 }
 
 __device__ __forceinline__ int fine_index(int f, int n, int lo, int hi) {
@@ -127,24 +132,30 @@ template <typename T>
 __global__ __launch_bounds__(RB) void k_restrict(LevelDev F, const T *__restrict__ rf, LevelDev Cl,
                                                  T *__restrict__ bc, const double *__restrict__ stop) {
     if (*stop != 0.0) return;
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ip = blockIdx.x * blockDim.x + threadIdx.x;
     const int J = blockIdx.y * blockDim.y + threadIdx.y, K = blockIdx.z;
-    if (I >= Cl.nx || J >= Cl.ny) return;
+    if (J >= Cl.ny) return;
     const T wt[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
-    int fi[4], fj[4], fk[4];
+    int fj[4], fk[4];
     for (int a = 0; a < 4; ++a) {
-        fi[a] = fine_index(2 * I - 1 + a, F.nx, F.bt[0], F.bt[1]);
         fj[a] = fine_index(2 * J - 1 + a, F.ny, F.bt[2], F.bt[3]);
         fk[a] = fine_index(2 * K - 1 + a, F.nz, F.bt[4], F.bt[5]);
     }
-    T s = T(0);
-    for (int c3 = 0; c3 < 4; ++c3)
-        for (int c2 = 0; c2 < 4; ++c2)
-            for (int c1 = 0; c1 < 4; ++c1) {
-                T w = wt[c3] * wt[c2] * wt[c1];
-                s = s + w * rf[lidx(F, fi[c1], fj[c2], fk[c3])];
-            }
-    bc[lidx(Cl, I, J, K)] = T(0.125) * s;
+    // the thread's coarse x-pair (2p, 2p+1), one cell of each colour
+    for (int h = 0; h < 2; ++h) {
+        const int I = 2 * Ip + h;
+        if (I >= Cl.nx) return;
+        int fi[4];
+        for (int a = 0; a < 4; ++a) fi[a] = fine_index(2 * I - 1 + a, F.nx, F.bt[0], F.bt[1]);
+        T s = T(0);
+        for (int c3 = 0; c3 < 4; ++c3)
+            for (int c2 = 0; c2 < 4; ++c2)
+                for (int c1 = 0; c1 < 4; ++c1) {
+                    T w = wt[c3] * wt[c2] * wt[c1];
+                    s = s + w * rf[lidx(F, fi[c1], fj[c2], fk[c3])];
+                }
+        bc[lidx(Cl, I, J, K)] = T(0.125) * s;
+    }
 }
 
 __device__ __forceinline__ void parents(int f, int nc, int lo, int hi, int &p0, int &p1) {
@@ -160,23 +171,29 @@ template <typename T>
 __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, LevelDev Cl,
                                                 const T *__restrict__ xc, const double *__restrict__ stop) {
     if (*stop != 0.0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
-    if (i >= F.nx || j >= F.ny) return;
-    int pi[2], pj[2], pk[2];
-    parents(i, Cl.nx, Cl.bt[0], Cl.bt[1], pi[0], pi[1]);
+    if (j >= F.ny) return;
+    int pj[2], pk[2];
     parents(j, Cl.ny, Cl.bt[2], Cl.bt[3], pj[0], pj[1]);
     parents(k, Cl.nz, Cl.bt[4], Cl.bt[5], pk[0], pk[1]);
     const T wt[2] = {T(0.75), T(0.25)};
-    T s = T(0);
-    for (int c3 = 0; c3 < 2; ++c3)
-        for (int c2 = 0; c2 < 2; ++c2)
-            for (int c1 = 0; c1 < 2; ++c1) {
-                T w = wt[c3] * wt[c2] * wt[c1];
-                s = s + w * xc[lidx(Cl, pi[c1], pj[c2], pk[c3])];
-            }
-    const int o = lidx(F, i, j, k);
-    xf[o] = xf[o] + s;
+    // the thread's x-pair (2p, 2p+1), one cell of each colour
+    for (int h = 0; h < 2; ++h) {
+        const int i = 2 * ip + h;
+        if (i >= F.nx) return;
+        int pi[2];
+        parents(i, Cl.nx, Cl.bt[0], Cl.bt[1], pi[0], pi[1]);
+        T s = T(0);
+        for (int c3 = 0; c3 < 2; ++c3)
+            for (int c2 = 0; c2 < 2; ++c2)
+                for (int c1 = 0; c1 < 2; ++c1) {
+                    T w = wt[c3] * wt[c2] * wt[c1];
+                    s = s + w * xc[lidx(Cl, pi[c1], pj[c2], pk[c3])];
+                }
+        const int o = lidx(F, i, j, k);
+        xf[o] = xf[o] + s;
+    }
 }
 
 // coarsest level, one work-group. Small levels (<= COARSE_LDS cells, every
@@ -453,7 +470,6 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
         return IBMDEM_OK;
     }
     LaunchShape rb = shape_for(div_up(d.nx, 2), d.ny, d.nz);
-    LaunchShape full = shape_for(d.nx, d.ny, d.nz);
     // level-0 half-sweeps are the profiled (dominant) kernel: each reads the
     // right-hand side and the other colour of half the cells and writes them
     const double hb = l == 0 ? 1.5 * sizeof(T) * (double)L.n : 0.0;
@@ -467,15 +483,15 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
         half(1, 0);
         CHECK_LAUNCH(c);
     }
-    hipLaunchKernelGGL((k_residual<T, T>), full.grid, full.block, 0, s, d, L.x, L.b, L.r, c->scal + SC_CONV);
+    hipLaunchKernelGGL((k_residual<T, T>), rb.grid, rb.block, 0, s, d, L.x, L.b, L.r, c->scal + SC_CONV);
     CHECK_LAUNCH(c);
     MGLevel<T> &C = lev[l + 1];
-    LaunchShape cs = shape_for(C.d.nx, C.d.ny, C.d.nz);
+    LaunchShape cs = shape_for(div_up(C.d.nx, 2), C.d.ny, C.d.nz);
     hipLaunchKernelGGL((k_restrict<T>), cs.grid, cs.block, 0, s, d, L.r, C.d, C.b, c->scal + SC_CONV);
     CHECK_LAUNCH(c);
     int rc = vcycle<T>(c, lev, l + 1);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_prolong<T>), full.grid, full.block, 0, s, d, L.x, C.d, C.x, c->scal + SC_CONV);
+    hipLaunchKernelGGL((k_prolong<T>), rb.grid, rb.block, 0, s, d, L.x, C.d, C.x, c->scal + SC_CONV);
     CHECK_LAUNCH(c);
     for (int q = 0; q < c->cs.mg_post; ++q) {
         half(1, 0);
