@@ -466,19 +466,23 @@ template <bool P2>
 __device__ __forceinline__ void gather_entries(const SpreadArgs &a, int ne, int i, int j, int k,
                                                const double2 (*s_w)[WST2], const int4 *s_cc, const double4 *s_fv,
                                                double &f0, double &f1, double &f2, bool &has) {
+    // Branch-free: an entry whose footprint misses the lane's cell subtracts
+    // an exact 0 (f - 0 = f), read through an in-range index, so the loads
+    // of several entries can be in flight at once.
+#pragma unroll 4
     for (int e = 0; e < ne; ++e) {
         const int4 cc = s_cc[e];
         const int ox = i - cc.x + 1, oy = j - cc.y + 1, oz = k - cc.z + 1;
-        if ((unsigned)ox > 3u || (unsigned)oy > 3u || (unsigned)oz > 3u) continue;
-        has = true;
-        const double2 wx = s_w[e][ox], wy = s_w[e][4 + oy], wz = s_w[e][8 + oz];
+        const bool in = (unsigned)ox <= 3u && (unsigned)oy <= 3u && (unsigned)oz <= 3u;
+        has = has || in;
+        const double2 wx = s_w[e][ox & 3], wy = s_w[e][4 + (oy & 3)], wz = s_w[e][8 + (oz & 3)];
         const double4 fv = s_fv[e];
         // (face, centre): u takes the x face, v the y face, w the z face
         const double pu = wx.x * wy.y * wz.y, pv = wx.y * wy.x * wz.y, pw = wx.y * wy.y * wz.x;
         const double wu = P2 ? pu : pu / a.dV, wv = P2 ? pv : pv / a.dV, ww = P2 ? pw : pw / a.dV;
-        f0 = f0 - fv.x * wu;
-        f1 = f1 - fv.y * wv;
-        f2 = f2 - fv.z * ww;
+        f0 = f0 - (in ? fv.x * wu : 0.0);
+        f1 = f1 - (in ? fv.y * wv : 0.0);
+        f2 = f2 - (in ? fv.z * ww : 0.0);
     }
 }
 
