@@ -137,9 +137,9 @@ def measured_traffic():
     out = {}
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch "):
-                _, name, val = line.split()
-                out[name] = float(val)
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch":
+                out[f[1]] = float(f[2])
     except OSError:
         pass
     return out
