@@ -33,6 +33,7 @@ struct HostCopy {
     std::vector<int32_t> body;
     std::vector<double> bx, bv, bvold, bom, bL, bLold, brot, brotg, bI, bm, brho;
     std::vector<int32_t> bfixed;
+    std::vector<double> plane[3];   // outflow planes (sides 1, 3, 5)
 };
 
 ibmdem_markers marker_view(HostCopy &h) {
@@ -67,6 +68,12 @@ int save_state(ibmdem_ctx *c, HostCopy &h) {
         ibmdem_markers mk = marker_view(h);
         if ((rc = ibmdem_download_markers(c, &mk))) return rc;
     }
+    const long pn[3] = {(long)c->g.ny * c->g.nz, (long)c->g.nx * c->g.nz, (long)c->g.nx * c->g.ny};
+    for (int d = 0; d < 3; ++d) {
+        if (c->g.bt[2 * d + 1] != IBMDEM_BC_OUTFLOW) continue;
+        h.plane[d].resize(pn[d]);
+        if ((rc = ibmdem_download_outflow(c, 2 * d + 1, h.plane[d].data()))) return rc;
+    }
     return IBMDEM_OK;
 }
 
@@ -81,6 +88,9 @@ int restore_state(ibmdem_ctx *c, HostCopy &h) {
         if ((rc = ibmdem_upload_markers(c, &mk))) return rc;
     }
     if ((rc = ibmdem_upload_fields(c, h.f[0].data(), h.f[1].data(), h.f[2].data(), h.f[3].data()))) return rc;
+    // after the fields: their upload resets the outflow planes
+    for (int d = 0; d < 3; ++d)
+        if (!h.plane[d].empty() && (rc = ibmdem_upload_outflow(c, 2 * d + 1, h.plane[d].data()))) return rc;
     return ibmdem_sync(c);
 }
 

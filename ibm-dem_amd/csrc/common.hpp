@@ -13,6 +13,19 @@
 //   pressure               mirrored without negation (Neumann).
 // Corner ghosts compose the per-dimension rules, which is what the
 // reference's x-then-y-then-z halo passes produce.
+// Open sides (no counterpart in the reference; the same rules are in the
+// oracle):
+//   inflow (low sides)     normal component = U_n at the face and on every
+//                          ghost layer beyond it, whatever the transverse
+//                          index; tangential ghosts mirrored and negated
+//                          (zero tangential velocity); pressure Neumann;
+//   outflow (high sides)   normal component on the boundary face (index
+//                          n+1) and beyond = the stored outflow plane at the
+//                          mapped transverse index, times the transverse
+//                          mirror signs; tangential ghosts mirrored without
+//                          negation (zero gradient); pressure and its
+//                          correction mirrored and negated (zero at the
+//                          face).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,7 +43,16 @@ struct GridDev {
     int bt[6];
     double dx, dy, dz;
     double L[3];
+    double uin[3];      // normal velocity of an inflow side of dimension d
+    double *plane[3];   // outflow plane of dimension d (normal component on
+                        // the face n_d+1), first transverse index fastest
 };
+
+__host__ __device__ inline bool grid_open(const GridDev &g) {
+    for (int s = 0; s < 6; ++s)
+        if (g.bt[s] == IBMDEM_BC_INFLOW || g.bt[s] == IBMDEM_BC_OUTFLOW) return true;
+    return false;
+}
 
 // ---------------------------------------------------------------------------
 // boundary rules
@@ -42,7 +64,8 @@ __host__ __device__ inline long cidx(const GridDev &g, int i, int j, int k) {
 
 // Map one 1-based index of dimension d (size n) onto the interior.
 // Returns 0 if the value is a wall-normal ghost (forced zero), flips *sgn on a
-// wall mirror.
+// tangential mirror (wall or inflow side; an outflow side mirrors without
+// negation).
 template <bool NORMAL>
 __device__ __forceinline__ bool map_index(int &i, int n, int lo, int hi, double &sgn) {
     if (i < 1) {
@@ -53,15 +76,57 @@ __device__ __forceinline__ bool map_index(int &i, int n, int lo, int hi, double 
     if (i > n) {
         if (hi == IBMDEM_BC_PERIODIC) { i -= n; return true; }
         if (NORMAL) return false;
-        i = 2 * n + 1 - i; sgn = -sgn; return true;
+        i = 2 * n + 1 - i;
+        if (hi != IBMDEM_BC_OUTFLOW) sgn = -sgn;
+        return true;
     }
     return true;
 }
 
-// velocity component C (0=u,1=v,2=w) at logical 1-based (i,j,k), each index
-// in [-1, n+2], with the reference's post-Boundary ghost semantics.
+// the value every writer keeps on the low boundary face of component C (a
+// wall face zero, an inflow face the inflow velocity)
+template <int C>
+__device__ __forceinline__ double face_value(const GridDev &g) {
+    return g.bt[2 * C] == IBMDEM_BC_INFLOW ? g.uin[C] : 0.0;
+}
+
+// offset of the mapped transverse cell (a, b) (1-based, a the lower
+// dimension) in the outflow plane of dimension C
+template <int C>
+__device__ __forceinline__ long plane_off(const GridDev &g, int i, int j, int k) {
+    return C == 0 ? (long)(j - 1) + (long)g.ny * (k - 1)
+                  : (C == 1 ? (long)(i - 1) + (long)g.nx * (k - 1) : (long)(i - 1) + (long)g.nx * (j - 1));
+}
+
+// The open-side rules of velocity component C at (i,j,k): returns true and
+// sets *val when dimension C's index lies on the inflow face or beyond it, or
+// beyond an outflow face.
 template <int C, typename T>
+__device__ __forceinline__ bool open_value(const GridDev &g, int i, int j, int k, T &val) {
+    const int ic = C == 0 ? i : (C == 1 ? j : k);
+    const int nc = C == 0 ? g.nx : (C == 1 ? g.ny : g.nz);
+    if (ic <= 1 && g.bt[2 * C] == IBMDEM_BC_INFLOW) { val = T(g.uin[C]); return true; }
+    if (ic > nc && g.bt[2 * C + 1] == IBMDEM_BC_OUTFLOW) {
+        double s = 1.0;
+        if (C != 0) map_index<false>(i, g.nx, g.bt[0], g.bt[1], s);
+        if (C != 1) map_index<false>(j, g.ny, g.bt[2], g.bt[3], s);
+        if (C != 2) map_index<false>(k, g.nz, g.bt[4], g.bt[5], s);
+        const T p = T(g.plane[C][plane_off<C>(g, i, j, k)]);
+        val = s < 0 ? -p : p;
+        return true;
+    }
+    return false;
+}
+
+// velocity component C (0=u,1=v,2=w) at logical 1-based (i,j,k), each index
+// in [-1, n+2], with the reference's post-Boundary ghost semantics. OPEN
+// kernels (grids with an inflow or outflow side) apply the open-side rules.
+template <int C, bool OPEN = false, typename T>
 __device__ __forceinline__ T vel_at(const T *__restrict__ f, const GridDev &g, int i, int j, int k) {
+    if (OPEN) {
+        T ov;
+        if (open_value<C>(g, i, j, k, ov)) return ov;
+    }
     double s = 1.0;
     if (!map_index<C == 0>(i, g.nx, g.bt[0], g.bt[1], s)) return T(0);
     if (!map_index<C == 1>(j, g.ny, g.bt[2], g.bt[3], s)) return T(0);
@@ -82,71 +147,114 @@ __device__ __forceinline__ double vel_at_shift(const double *__restrict__ f, con
     if (!map_index<C == 1>(j, g.ny, g.bt[2], g.bt[3], s)) return 0.0;
     if (!map_index<C == 2>(k, g.nz, g.bt[4], g.bt[5], s)) return 0.0;
     int idx = C == 0 ? i : (C == 1 ? j : k);
-    if (idx == 1 && g.bt[2 * C] == IBMDEM_BC_WALL) return 0.0;
+    if (idx == 1 && g.bt[2 * C] != IBMDEM_BC_PERIODIC) return face_value<C>(g);
     double val = f[cidx(g, i, j, k)] + sh;
     return s < 0 ? -val : val;
 }
 
-// cell-centred scalar (P, P'): mirror without negation on walls
+// one index of a cell-centred scalar: wrap, or mirror (negated beyond an
+// outflow side, where the scalar is zero at the face)
+__device__ __forceinline__ void map_scal(int &i, int n, int lo, int hi, double &sgn) {
+    if (i < 1) { i = lo == IBMDEM_BC_PERIODIC ? i + n : 1 - i; return; }
+    if (i > n) {
+        if (hi == IBMDEM_BC_PERIODIC) { i -= n; return; }
+        i = 2 * n + 1 - i;
+        if (hi == IBMDEM_BC_OUTFLOW) sgn = -sgn;
+    }
+}
+
+// cell-centred scalar (P, P'): mirror without negation on walls and inflow
+// sides, negated beyond an outflow side
 template <typename T>
 __device__ __forceinline__ T scal_at(const T *__restrict__ f, const GridDev &g, int i, int j, int k) {
     double s = 1.0;
-    map_index<false>(i, g.nx, g.bt[0], g.bt[1], s);
-    map_index<false>(j, g.ny, g.bt[2], g.bt[3], s);
-    map_index<false>(k, g.nz, g.bt[4], g.bt[5], s);
-    return f[cidx(g, i, j, k)];
+    map_scal(i, g.nx, g.bt[0], g.bt[1], s);
+    map_scal(j, g.ny, g.bt[2], g.bt[3], s);
+    map_scal(k, g.nz, g.bt[4], g.bt[5], s);
+    const T v = f[cidx(g, i, j, k)];
+    return s < 0 ? -v : v;
 }
 
-// is (i,j,k) the low wall face of velocity component C (kept zero)?
+// is (i,j,k) the low boundary face of velocity component C (a wall face
+// kept zero, or an inflow face kept at the inflow velocity)?
 template <int C>
 __device__ __forceinline__ bool wall_face(const GridDev &g, int i, int j, int k) {
     int idx = C == 0 ? i : (C == 1 ? j : k);
-    return idx == 1 && g.bt[2 * C] == IBMDEM_BC_WALL;
+    return idx == 1 && g.bt[2 * C] != IBMDEM_BC_PERIODIC;
 }
 
 // Per-thread axis map: for offsets O0..O0+NO-1 around a 1-based index, the
-// 0-based interior index the ghost value comes from, whether it is a wall
-// mirror (negated for tangential components) and whether it lies beyond a
-// wall (forced zero for the wall-normal component). Evaluated once per
-// thread so every tap is one load, with no divergent boundary path.
+// 0-based interior index the ghost value comes from, whether it is a
+// tangential mirror that negates (wall or inflow side), whether it lies
+// beyond a wall (forced zero for the wall-normal component), on or beyond an
+// inflow face (the inflow velocity for the normal component) or beyond an
+// outflow face (the outflow plane for the normal component; the scalars
+// negated). Evaluated once per thread so every tap is one load, with no
+// divergent boundary path.
 template <int O0, int NO>
 struct AxisMap {
     int idx[NO];
     bool mir[NO];
     bool wall[NO];
+    bool cst[NO];
+    bool pln[NO];
     __device__ __forceinline__ void init(int i, int n, int lo, int hi) {
 #pragma unroll
         for (int q = 0; q < NO; ++q) {
             int j = i + O0 + q;
-            bool m = false;
+            bool m = false, w = false, p = false;
+            const bool c = j <= 1 && lo == IBMDEM_BC_INFLOW;
             if (j < 1) {
                 if (lo == IBMDEM_BC_PERIODIC) j += n;
-                else { j = 1 - j; m = true; }
+                else { j = 1 - j; m = true; w = lo == IBMDEM_BC_WALL; }
             } else if (j > n) {
                 if (hi == IBMDEM_BC_PERIODIC) j -= n;
-                else { j = 2 * n + 1 - j; m = true; }
+                else {
+                    j = 2 * n + 1 - j;
+                    p = hi == IBMDEM_BC_OUTFLOW;
+                    m = !p;
+                    w = hi == IBMDEM_BC_WALL;
+                }
             }
             idx[q] = j - 1;
             mir[q] = m;
-            wall[q] = m;
+            wall[q] = w;
+            cst[q] = c;
+            pln[q] = p;
         }
     }
 };
 
 // velocity component C at mapped offsets (a,b,c) of the three axis maps
-template <int C, class MX, class MY, class MZ, typename T>
+template <int C, bool OPEN = false, class MX, class MY, class MZ, typename T>
 __device__ __forceinline__ T vel_tap(const T *__restrict__ f, const GridDev &g, const MX &mx, const MY &my,
                                      const MZ &mz, int a, int b, int c) {
     const bool zero = (C == 0 && mx.wall[a]) || (C == 1 && my.wall[b]) || (C == 2 && mz.wall[c]);
     const bool neg = ((C != 0 && mx.mir[a]) ^ (C != 1 && my.mir[b])) ^ (C != 2 && mz.mir[c]);
-    const T v = f[(long)mx.idx[a] + (long)g.nx * ((long)my.idx[b] + (long)g.ny * mz.idx[c])];
+    const long fo = (long)mx.idx[a] + (long)g.nx * ((long)my.idx[b] + (long)g.ny * mz.idx[c]);
+    if (OPEN) {
+        const bool cst = (C == 0 && mx.cst[a]) || (C == 1 && my.cst[b]) || (C == 2 && mz.cst[c]);
+        const bool pln = (C == 0 && mx.pln[a]) || (C == 1 && my.pln[b]) || (C == 2 && mz.pln[c]);
+        const long po = C == 0 ? (long)my.idx[b] + (long)g.ny * mz.idx[c]
+                               : (C == 1 ? (long)mx.idx[a] + (long)g.nx * mz.idx[c]
+                                         : (long)mx.idx[a] + (long)g.nx * my.idx[b]);
+        const T v = pln ? T(g.plane[C][po]) : f[fo];
+        return cst ? T(g.uin[C]) : (zero ? T(0) : (neg ? -v : v));
+    }
+    const T v = f[fo];
     return zero ? T(0) : (neg ? -v : v);
 }
-// cell-centred scalar at mapped offsets (mirror without negation)
-template <class MX, class MY, class MZ, typename T>
+// cell-centred scalar at mapped offsets (mirror without negation; negated
+// beyond an outflow side in OPEN kernels)
+template <bool OPEN = false, class MX, class MY, class MZ, typename T>
 __device__ __forceinline__ T scal_tap(const T *__restrict__ f, const GridDev &g, const MX &mx, const MY &my,
                                       const MZ &mz, int a, int b, int c) {
-    return f[(long)mx.idx[a] + (long)g.nx * ((long)my.idx[b] + (long)g.ny * mz.idx[c])];
+    const T v = f[(long)mx.idx[a] + (long)g.nx * ((long)my.idx[b] + (long)g.ny * mz.idx[c])];
+    if (OPEN) {
+        const bool neg = (mx.pln[a] ^ my.pln[b]) ^ mz.pln[c];
+        return neg ? -v : v;
+    }
+    return v;
 }
 
 // x / d. When d is a power of two (P2), 1/d is exact and x * (1/d) rounds to
@@ -197,9 +305,10 @@ struct LevelDev {
     int nx, ny, nz;
     int bt[6];
     double ihx2, ihy2, ihz2;     // 1/h^2 per dimension
-    double idiag[27];            // 1/diag indexed by neighbour counts (cx,cy,cz)
-    double diag[27];
+    double idiag[64];            // 1/diag indexed by the per-dimension counts
+    double diag[64];             // (cx,cy,cz) in 0..3, see diag_index
     int rb_valid;                // red-black colouring consistent (no odd periodic dim)
+    int dirichlet;               // an outflow side: the operator is non-singular
     int split;                   // colour-split storage (nx even), see lidx
     int half;                    // nx*ny*nz / 2 when split
 };
@@ -412,7 +521,7 @@ int fluid_predictor(ibmdem_ctx *c, double a, double b, double cc);
 int fluid_bulk(ibmdem_ctx *c, bool apply_now);
 int fluid_divergence(ibmdem_ctx *c, double cc);
 int fluid_project(ibmdem_ctx *c, double cc);
-int fluid_upload_bc(ibmdem_ctx *c);
+int fluid_upload_bc(ibmdem_ctx *c, int reset_planes);
 int fluid_save_uold(ibmdem_ctx *c);
 int fluid_predictor_general(ibmdem_ctx *c, double a, double b, double cc);
 // poisson.hip

@@ -49,8 +49,17 @@ int validate(const ibmdem_case *cs) {
     if (cs->nx < 2 || cs->ny < 2 || cs->nz < 2 || cs->dx <= 0 || cs->dy <= 0 || cs->dz <= 0 || cs->dt <= 0 ||
         cs->n_markers < 0 || cs->n_bodies < 0 || cs->couple_interval < 1)
         return IBMDEM_ERR_ARG;
-    for (int s = 0; s < 6; ++s)
-        if (cs->btype[s] != IBMDEM_BC_PERIODIC && cs->btype[s] != IBMDEM_BC_WALL) return IBMDEM_ERR_UNSUPPORTED;
+    for (int s = 0; s < 6; ++s) {
+        const int b = cs->btype[s];
+        if (b != IBMDEM_BC_PERIODIC && b != IBMDEM_BC_WALL && b != IBMDEM_BC_INFLOW && b != IBMDEM_BC_OUTFLOW)
+            return IBMDEM_ERR_UNSUPPORTED;
+        // inflow on low sides only, outflow on high sides only
+        if ((b == IBMDEM_BC_INFLOW && (s & 1)) || (b == IBMDEM_BC_OUTFLOW && !(s & 1))) return IBMDEM_ERR_ARG;
+        // bulk forcing is a periodic-channel device
+        if ((b == IBMDEM_BC_INFLOW || b == IBMDEM_BC_OUTFLOW) &&
+            (cs->bulk_switch[0] == 1 || cs->bulk_switch[1] == 1 || cs->bulk_switch[2] == 1))
+            return IBMDEM_ERR_UNSUPPORTED;
+    }
     for (int d = 0; d < 3; ++d)
         if ((cs->btype[2 * d] == IBMDEM_BC_PERIODIC) != (cs->btype[2 * d + 1] == IBMDEM_BC_PERIODIC))
             return IBMDEM_ERR_ARG;
@@ -71,6 +80,11 @@ int alloc_all(ibmdem_ctx *c) {
 #define A(p, cnt) if ((rc = dev_alloc(c, &(p), (cnt)))) return rc
     for (int q = 0; q < 2; ++q) { A(c->u[q], n); A(c->v[q], n); A(c->w[q], n); A(c->P[q], n); }
     A(c->pcorr, n); A(c->rhs, n);
+    {
+        const long pn[3] = {(long)c->g.ny * c->g.nz, (long)c->g.nx * c->g.nz, (long)c->g.nx * c->g.ny};
+        for (int d = 0; d < 3; ++d)
+            if (c->g.bt[2 * d + 1] == IBMDEM_BC_OUTFLOW) A(c->g.plane[d], pn[d]);
+    }
     A(c->cg_x, n); A(c->cg_r, n); A(c->cg_z, n); A(c->cg_p, n); A(c->cg_Ap, n);
     const long pred_blocks = (long)div_up(c->g.nx, 64) * div_up(c->g.ny, 4) * c->g.nz;
     c->max_partial = (int)std::max<long>(4096, 3 * pred_blocks + 16);
@@ -333,6 +347,10 @@ int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
     for (int s = 0; s < 6; ++s) g.bt[s] = cs->btype[s];
     g.dx = cs->dx; g.dy = cs->dy; g.dz = cs->dz;
     g.L[0] = g.dx * g.nx; g.L[1] = g.dy * g.ny; g.L[2] = g.dz * g.nz;
+    for (int d = 0; d < 3; ++d) {
+        g.uin[d] = cs->btype[2 * d] == IBMDEM_BC_INFLOW ? cs->inflow[d] : 0.0;
+        g.plane[d] = nullptr;
+    }
     c->n = (long)g.nx * g.ny * g.nz;
     c->dV = g.dx * g.dy * g.dz;
     c->eta = eta_of(&c->cs);
@@ -345,6 +363,8 @@ int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
         return IBMDEM_ERR_HIP;
     }
     rc = alloc_all(c);
+    // inflow faces hold the inflow velocity from the start
+    if (rc == IBMDEM_OK) rc = fluid_upload_bc(c, 7);
     if (rc == IBMDEM_OK) rc = finish(c);
     if (rc) {
         ibmdem_destroy(c);
@@ -385,7 +405,35 @@ int ibmdem_upload_fields(ibmdem_ctx *c, const double *u, const double *v, const 
     TRY(h2d(c, c->w[c->vcur], w, b));
     TRY(h2d(c, c->P[c->pcur], p, b));
     c->bulk_pending = false;
-    TRY(fluid_upload_bc(c));
+    TRY(fluid_upload_bc(c, (u ? 1 : 0) | (v ? 2 : 0) | (w ? 4 : 0)));
+    return finish(c);
+}
+
+namespace {
+long outflow_plane(ibmdem_ctx *c, int side, double **plane) {
+    if (side != 1 && side != 3 && side != 5) return -1;
+    const int d = side / 2;
+    if (c->g.bt[side] != IBMDEM_BC_OUTFLOW) return -1;
+    *plane = c->g.plane[d];
+    return d == 0 ? (long)c->g.ny * c->g.nz : (d == 1 ? (long)c->g.nx * c->g.nz : (long)c->g.nx * c->g.ny);
+}
+}  // namespace
+
+int ibmdem_download_outflow(ibmdem_ctx *c, int32_t side, double *plane) {
+    CTX_OR_ARG(c);
+    double *dp = nullptr;
+    const long n = outflow_plane(c, side, &dp);
+    if (n < 0 || !plane) return set_error(c, IBMDEM_ERR_ARG, "not an outflow side (1, 3 or 5 of type OUTFLOW)");
+    TRY(d2h(c, plane, dp, n * sizeof(double)));
+    return finish(c);
+}
+
+int ibmdem_upload_outflow(ibmdem_ctx *c, int32_t side, const double *plane) {
+    CTX_OR_ARG(c);
+    double *dp = nullptr;
+    const long n = outflow_plane(c, side, &dp);
+    if (n < 0 || !plane) return set_error(c, IBMDEM_ERR_ARG, "not an outflow side (1, 3 or 5 of type OUTFLOW)");
+    TRY(h2d(c, dp, plane, n * sizeof(double)));
     return finish(c);
 }
 

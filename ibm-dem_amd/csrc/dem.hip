@@ -320,7 +320,7 @@ __global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *_
     // wall contact for markers in the first/last interior layer
     if (ci == 1 || ci == g.nx || cj == 1 || cj == g.ny || ck == 1 || ck == g.nz) {
         for (int s = 0; s < 6; ++s) {
-            if (g.bt[s] != IBMDEM_BC_WALL) continue;
+            if (g.bt[s] == IBMDEM_BC_PERIODIC) continue;   // walls, inflow and outflow screens
             const double p1 = (s % 2 == 0) ? 0.0 : g.L[0];
             const double p2 = (s % 2 == 0) ? 0.0 : g.L[1];
             const double p3 = (s % 2 == 0) ? 0.0 : g.L[2];

@@ -29,21 +29,22 @@ __device__ __forceinline__ void xcd_block(int &bx, int &by, int &bz) {
 }
 
 using Map3 = AxisMap<-1, 3>;
-template <int C>
+template <int C, bool OPEN>
 struct Tap {
     const double *__restrict__ p;
     const GridDev *g;
     const Map3 *mx, *my, *mz;
     __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
-        return vel_tap<C>(p, *g, *mx, *my, *mz, di + 1, dj + 1, dk + 1);
+        return vel_tap<C, OPEN>(p, *g, *mx, *my, *mz, di + 1, dj + 1, dk + 1);
     }
 };
+template <bool OPEN>
 struct TapP {
     const double *__restrict__ p;
     const GridDev *g;
     const Map3 *mx, *my, *mz;
     __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
-        return scal_tap(p, *g, *mx, *my, *mz, di + 1, dj + 1, dk + 1);
+        return scal_tap<OPEN>(p, *g, *mx, *my, *mz, di + 1, dj + 1, dk + 1);
     }
 };
 
@@ -143,7 +144,7 @@ __device__ __forceinline__ void block_sum3(double s[3], double *part) {
 // VelocityGuess fused form: u* = u~ + dt*(a*RHS_noF(u~) - c*gradP_old).
 // Because u~ = uold + dt*f after the multi-direct-forcing loop, this equals
 // the reference's uold + dt*(a*RHS(u~) + 0*RHSl - c*gradP) for a = 1.
-template <typename CT, bool BULK, bool P2>
+template <typename CT, bool BULK, bool P2, bool OPEN>
 __global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *__restrict__ u,
                                                       const double *__restrict__ v,
                                                       const double *__restrict__ w,
@@ -161,14 +162,14 @@ __global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *_
         mx.init(i, g.nx, g.bt[0], g.bt[1]);
         my.init(j, g.ny, g.bt[2], g.bt[3]);
         mz.init(k, g.nz, g.bt[4], g.bt[5]);
-        predictor_cell<CT, P2>(Tap<0>{u, &g, &mx, &my, &mz}, Tap<1>{v, &g, &mx, &my, &mz}, Tap<2>{w, &g, &mx, &my, &mz},
-                           TapP{P, &g, &mx, &my, &mz}, g, pp, R, Pg);
+        predictor_cell<CT, P2>(Tap<0, OPEN>{u, &g, &mx, &my, &mz}, Tap<1, OPEN>{v, &g, &mx, &my, &mz},
+                               Tap<2, OPEN>{w, &g, &mx, &my, &mz}, TapP<OPEN>{P, &g, &mx, &my, &mz}, g, pp, R, Pg);
         const CT a = CT(pp.a), c = CT(pp.c);
         for (int q = 0; q < 3; ++q) inc[q] = a * R[q] - c * Pg[q];
         const double dt = pp.dt;
-        double nu_ = wall_face<0>(g, i, j, k) ? 0.0 : u[o] + dt * (double)inc[0];
-        double nv_ = wall_face<1>(g, i, j, k) ? 0.0 : v[o] + dt * (double)inc[1];
-        double nw_ = wall_face<2>(g, i, j, k) ? 0.0 : w[o] + dt * (double)inc[2];
+        double nu_ = wall_face<0>(g, i, j, k) ? face_value<0>(g) : u[o] + dt * (double)inc[0];
+        double nv_ = wall_face<1>(g, i, j, k) ? face_value<1>(g) : v[o] + dt * (double)inc[1];
+        double nw_ = wall_face<2>(g, i, j, k) ? face_value<2>(g) : w[o] + dt * (double)inc[2];
         un[o] = nu_;
         vn[o] = nv_;
         wn[o] = nw_;
@@ -190,7 +191,7 @@ struct GeneralArgs {
     double *rhsl[3];
     double b;
 };
-template <typename CT, bool BULK, bool P2>
+template <typename CT, bool BULK, bool P2, bool OPEN>
 __global__ __launch_bounds__(BX *BY) void k_predictor_general(GridDev g, const double *__restrict__ u,
                                                               const double *__restrict__ v,
                                                               const double *__restrict__ w,
@@ -209,15 +210,16 @@ __global__ __launch_bounds__(BX *BY) void k_predictor_general(GridDev g, const d
         mx.init(i, g.nx, g.bt[0], g.bt[1]);
         my.init(j, g.ny, g.bt[2], g.bt[3]);
         mz.init(k, g.nz, g.bt[4], g.bt[5]);
-        predictor_cell<CT, P2>(Tap<0>{u, &g, &mx, &my, &mz}, Tap<1>{v, &g, &mx, &my, &mz}, Tap<2>{w, &g, &mx, &my, &mz},
-                           TapP{P, &g, &mx, &my, &mz}, g, pp, R, Pg);
+        predictor_cell<CT, P2>(Tap<0, OPEN>{u, &g, &mx, &my, &mz}, Tap<1, OPEN>{v, &g, &mx, &my, &mz},
+                               Tap<2, OPEN>{w, &g, &mx, &my, &mz}, TapP<OPEN>{P, &g, &mx, &my, &mz}, g, pp, R, Pg);
         const CT a = CT(pp.a), b = CT(ga.b), c = CT(pp.c);
         const bool face[3] = {wall_face<0>(g, i, j, k), wall_face<1>(g, i, j, k), wall_face<2>(g, i, j, k)};
+        const double fval[3] = {face_value<0>(g), face_value<1>(g), face_value<2>(g)};
         double *out[3] = {un, vn, wn};
         for (int q = 0; q < 3; ++q) {
             const CT rhs = R[q] + CT(ga.f[q][o]);
             const CT val = a * rhs + b * CT(ga.rhsl[q][o]) - c * Pg[q];
-            const double nv = face[q] ? 0.0 : ga.uold[q][o] + pp.dt * (double)val;
+            const double nv = face[q] ? fval[q] : ga.uold[q][o] + pp.dt * (double)val;
             out[q][o] = nv;
             ga.rhsl[q][o] = (double)rhs;
             s[q] = nv;
@@ -294,7 +296,11 @@ __global__ __launch_bounds__(BX *BY) void k_bulk_apply(GridDev g, double *u, dou
     if (!wall_face<2>(g, i, j, k)) w[o] = w[o] + scal[SC_MODP + 2];
 }
 
-// VelocityDivergence(c) (fluid.F90:97-100) on the (pending-shifted) velocity
+// VelocityDivergence(c) (fluid.F90:97-100) on the (pending-shifted) velocity.
+// OPEN: the outflow planes are first set to the predicted velocity of the
+// last interior face (zero normal gradient), by the thread of the last cell,
+// which reads that face as its own.
+template <bool OPEN>
 __global__ __launch_bounds__(BX *BY) void k_divergence(GridDev g, const double *__restrict__ u,
                                                        const double *__restrict__ v,
                                                        const double *__restrict__ w, double *__restrict__ rhs,
@@ -307,7 +313,18 @@ __global__ __launch_bounds__(BX *BY) void k_divergence(GridDev g, const double *
     double su = 0.0, sv = 0.0, sw = 0.0;
     if (shifted) { su = scal[SC_MODP]; sv = scal[SC_MODP + 1]; sw = scal[SC_MODP + 2]; }
     double u1, u0, v1, v0, w1, w0;
-    if (shifted) {
+    if (OPEN) {
+        u0 = vel_at<0, true>(u, g, i, j, k);
+        v0 = vel_at<1, true>(v, g, i, j, k);
+        w0 = vel_at<2, true>(w, g, i, j, k);
+        const long o = cidx(g, i, j, k);
+        if (i == g.nx && g.bt[1] == IBMDEM_BC_OUTFLOW) { u1 = u[o]; g.plane[0][plane_off<0>(g, i, j, k)] = u1; }
+        else u1 = vel_at<0, true>(u, g, i + 1, j, k);
+        if (j == g.ny && g.bt[3] == IBMDEM_BC_OUTFLOW) { v1 = v[o]; g.plane[1][plane_off<1>(g, i, j, k)] = v1; }
+        else v1 = vel_at<1, true>(v, g, i, j + 1, k);
+        if (k == g.nz && g.bt[5] == IBMDEM_BC_OUTFLOW) { w1 = w[o]; g.plane[2][plane_off<2>(g, i, j, k)] = w1; }
+        else w1 = vel_at<2, true>(w, g, i, j, k + 1);
+    } else if (shifted) {
         u1 = vel_at_shift<0>(u, g, i + 1, j, k, su); u0 = vel_at_shift<0>(u, g, i, j, k, su);
         v1 = vel_at_shift<1>(v, g, i, j + 1, k, sv); v0 = vel_at_shift<1>(v, g, i, j, k, sv);
         w1 = vel_at_shift<2>(w, g, i, j, k + 1, sw); w0 = vel_at_shift<2>(w, g, i, j, k, sw);
@@ -323,6 +340,9 @@ __global__ __launch_bounds__(BX *BY) void k_divergence(GridDev g, const double *
 // PressureCorrection + VelocityCorrection(c) (fluid.F90:206, 221-223):
 // P_new = P + P' and u = u* + c*Px_old*dt - c*(P_new(i)-P_new(i-1))/dx*dt/rhof,
 // with Px_old = (P(i)-P(i-1))/dx/rhof from the pressure before correction.
+// OPEN: the thread of the last cell before an outflow side corrects the
+// outflow plane by the same formula, with the ghost pressures -P and -P_new.
+template <bool OPEN>
 __global__ __launch_bounds__(BX *BY) void k_project(GridDev g, double *u, double *v, double *w,
                                                     const double *__restrict__ P, const double *__restrict__ Pc,
                                                     double *__restrict__ Pn, const double *__restrict__ scal,
@@ -342,23 +362,43 @@ __global__ __launch_bounds__(BX *BY) void k_project(GridDev g, double *u, double
         const double pm = scal_at(P, g, im, jm, km), qm = scal_at(Pc, g, im, jm, km);
         const double Pold_grad = (p0 - pm) / d[c] / rhof;
         const double pnm = pm + qm;
-        const bool face = (c == 0 ? i : c == 1 ? j : k) == 1 && g.bt[2 * c] == IBMDEM_BC_WALL;
-        if (face) { vel[c][o] = 0.0; continue; }
+        const bool face = (c == 0 ? i : c == 1 ? j : k) == 1 && g.bt[2 * c] != IBMDEM_BC_PERIODIC;
+        if (face) {
+            vel[c][o] = g.bt[2 * c] == IBMDEM_BC_INFLOW ? g.uin[c] : 0.0;
+            continue;
+        }
         double us = vel[c][o];
         if (shifted) us = us + scal[SC_MODP + c];
         vel[c][o] = us + cRK * Pold_grad * dt - cRK * (pn0 - pnm) / (d[c]) * dt / rhof;
     }
+    if (OPEN) {
+        const int ijk[3] = {i, j, k}, n3[3] = {g.nx, g.ny, g.nz};
+        for (int c = 0; c < 3; ++c) {
+            if (ijk[c] != n3[c] || g.bt[2 * c + 1] != IBMDEM_BC_OUTFLOW) continue;
+            const double pg = -p0, png = -pn0;
+            const double Pold_grad = (pg - p0) / d[c] / rhof;
+            const long po = c == 0 ? plane_off<0>(g, i, j, k) : (c == 1 ? plane_off<1>(g, i, j, k) : plane_off<2>(g, i, j, k));
+            const double us = g.plane[c][po];
+            g.plane[c][po] = us + cRK * Pold_grad * dt - cRK * (png - pn0) / (d[c]) * dt / rhof;
+        }
+    }
 }
 
-// force the wall-face normal components to zero (upload-time Boundary)
-__global__ __launch_bounds__(BX *BY) void k_wall_faces(GridDev g, double *u, double *v, double *w) {
+// upload-time Boundary: the low boundary faces take their fixed values (zero
+// on walls, the inflow velocity on inflow sides); the outflow plane of each
+// uploaded normal component (bit c of `reset`) starts from the last interior
+// face (zero normal gradient)
+__global__ __launch_bounds__(BX *BY) void k_wall_faces(GridDev g, double *u, double *v, double *w, int reset) {
     const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1,
               k = blockIdx.z + 1;
     if (i > g.nx || j > g.ny) return;
     const long o = cidx(g, i, j, k);
-    if (wall_face<0>(g, i, j, k)) u[o] = 0.0;
-    if (wall_face<1>(g, i, j, k)) v[o] = 0.0;
-    if (wall_face<2>(g, i, j, k)) w[o] = 0.0;
+    if (wall_face<0>(g, i, j, k)) u[o] = face_value<0>(g);
+    if (wall_face<1>(g, i, j, k)) v[o] = face_value<1>(g);
+    if (wall_face<2>(g, i, j, k)) w[o] = face_value<2>(g);
+    if ((reset & 1) && i == g.nx && g.bt[1] == IBMDEM_BC_OUTFLOW) g.plane[0][plane_off<0>(g, i, j, k)] = u[o];
+    if ((reset & 2) && j == g.ny && g.bt[3] == IBMDEM_BC_OUTFLOW) g.plane[1][plane_off<1>(g, i, j, k)] = v[o];
+    if ((reset & 4) && k == g.nz && g.bt[5] == IBMDEM_BC_OUTFLOW) g.plane[2][plane_off<2>(g, i, j, k)] = w[o];
 }
 
 // ---------------------------------------------------------------------------
@@ -443,16 +483,20 @@ int fluid_predictor(ibmdem_ctx *c, double a, double b, double cc) {
     pp.irhof = 1.0 / c->cs.rhof;
     const bool p2 = is_pow2(g.dx) && is_pow2(g.dy) && is_pow2(g.dz) && is_pow2(c->cs.rhof);
     const bool mixed = c->cs.precision == IBMDEM_MIXED;
-#define LAUNCH_PRED(CT, BK, PT)                                                                            \
-    hipLaunchKernelGGL((k_predictor<CT, BK, PT>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur], c->w[cur], \
+#define LAUNCH_PRED(CT, BK, PT, OP)                                                                        \
+    hipLaunchKernelGGL((k_predictor<CT, BK, PT, OP>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur], c->w[cur], \
                        c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial)
     prof_begin(c, IBMDEM_K_PREDICTOR);
-    if (mixed) {
-        if (bulk) { if (p2) LAUNCH_PRED(float, true, true); else LAUNCH_PRED(float, true, false); }
-        else { if (p2) LAUNCH_PRED(float, false, true); else LAUNCH_PRED(float, false, false); }
+    // open sides exclude bulk forcing (rejected at creation)
+    if (grid_open(g)) {
+        if (mixed) { if (p2) LAUNCH_PRED(float, false, true, true); else LAUNCH_PRED(float, false, false, true); }
+        else { if (p2) LAUNCH_PRED(double, false, true, true); else LAUNCH_PRED(double, false, false, true); }
+    } else if (mixed) {
+        if (bulk) { if (p2) LAUNCH_PRED(float, true, true, false); else LAUNCH_PRED(float, true, false, false); }
+        else { if (p2) LAUNCH_PRED(float, false, true, false); else LAUNCH_PRED(float, false, false, false); }
     } else {
-        if (bulk) { if (p2) LAUNCH_PRED(double, true, true); else LAUNCH_PRED(double, true, false); }
-        else { if (p2) LAUNCH_PRED(double, false, true); else LAUNCH_PRED(double, false, false); }
+        if (bulk) { if (p2) LAUNCH_PRED(double, true, true, false); else LAUNCH_PRED(double, true, false, false); }
+        else { if (p2) LAUNCH_PRED(double, false, true, false); else LAUNCH_PRED(double, false, false, false); }
     }
 #undef LAUNCH_PRED
     CHECK_LAUNCH(c);
@@ -491,16 +535,19 @@ int fluid_predictor_general(ibmdem_ctx *c, double a, double b, double cc) {
     const bool bulk = c->cs.bulk_switch[0] == 1 || c->cs.bulk_switch[1] == 1 || c->cs.bulk_switch[2] == 1;
     const bool p2 = is_pow2(g.dx) && is_pow2(g.dy) && is_pow2(g.dz) && is_pow2(c->cs.rhof);
     const bool mixed = c->cs.precision == IBMDEM_MIXED;
-#define LAUNCH_GEN(CT, BK, PT)                                                                                      \
-    hipLaunchKernelGGL((k_predictor_general<CT, BK, PT>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],          \
+#define LAUNCH_GEN(CT, BK, PT, OP)                                                                                  \
+    hipLaunchKernelGGL((k_predictor_general<CT, BK, PT, OP>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],      \
                        c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, ga, c->partial)
     prof_begin(c, IBMDEM_K_PREDICTOR);
-    if (mixed) {
-        if (bulk) { if (p2) LAUNCH_GEN(float, true, true); else LAUNCH_GEN(float, true, false); }
-        else { if (p2) LAUNCH_GEN(float, false, true); else LAUNCH_GEN(float, false, false); }
+    if (grid_open(g)) {
+        if (mixed) { if (p2) LAUNCH_GEN(float, false, true, true); else LAUNCH_GEN(float, false, false, true); }
+        else { if (p2) LAUNCH_GEN(double, false, true, true); else LAUNCH_GEN(double, false, false, true); }
+    } else if (mixed) {
+        if (bulk) { if (p2) LAUNCH_GEN(float, true, true, false); else LAUNCH_GEN(float, true, false, false); }
+        else { if (p2) LAUNCH_GEN(float, false, true, false); else LAUNCH_GEN(float, false, false, false); }
     } else {
-        if (bulk) { if (p2) LAUNCH_GEN(double, true, true); else LAUNCH_GEN(double, true, false); }
-        else { if (p2) LAUNCH_GEN(double, false, true); else LAUNCH_GEN(double, false, false); }
+        if (bulk) { if (p2) LAUNCH_GEN(double, true, true, false); else LAUNCH_GEN(double, true, false, false); }
+        else { if (p2) LAUNCH_GEN(double, false, true, false); else LAUNCH_GEN(double, false, false, false); }
     }
 #undef LAUNCH_GEN
     CHECK_LAUNCH(c);
@@ -551,8 +598,14 @@ int fluid_bulk(ibmdem_ctx *c, bool apply_now) {
 int fluid_divergence(ibmdem_ctx *c, double cc) {
     const GridDev &g = c->g;
     prof_begin(c, IBMDEM_K_DIVERGENCE);
-    hipLaunchKernelGGL(k_divergence, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur], c->v[c->vcur],
-                       c->w[c->vcur], c->rhs, c->scal, c->bulk_pending ? 1 : 0, c->cs.dt, cc, c->cs.rhof);
+    if (grid_open(g))
+        hipLaunchKernelGGL(k_divergence<true>, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
+                           c->v[c->vcur], c->w[c->vcur], c->rhs, c->scal, c->bulk_pending ? 1 : 0, c->cs.dt, cc,
+                           c->cs.rhof);
+    else
+        hipLaunchKernelGGL(k_divergence<false>, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
+                           c->v[c->vcur], c->w[c->vcur], c->rhs, c->scal, c->bulk_pending ? 1 : 0, c->cs.dt, cc,
+                           c->cs.rhof);
     CHECK_LAUNCH(c);
     prof_end(c, IBMDEM_K_DIVERGENCE, 4.0 * 8.0 * (double)c->n);
     return IBMDEM_OK;
@@ -562,9 +615,14 @@ int fluid_project(ibmdem_ctx *c, double cc) {
     const GridDev &g = c->g;
     const int pn = 1 - c->pcur;
     prof_begin(c, IBMDEM_K_PROJECT);
-    hipLaunchKernelGGL(k_project, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur], c->v[c->vcur],
-                       c->w[c->vcur], c->P[c->pcur], c->pcorr, c->P[pn], c->scal, c->bulk_pending ? 1 : 0,
-                       c->cs.dt, cc, c->cs.rhof);
+    if (grid_open(g))
+        hipLaunchKernelGGL(k_project<true>, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
+                           c->v[c->vcur], c->w[c->vcur], c->P[c->pcur], c->pcorr, c->P[pn], c->scal,
+                           c->bulk_pending ? 1 : 0, c->cs.dt, cc, c->cs.rhof);
+    else
+        hipLaunchKernelGGL(k_project<false>, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
+                           c->v[c->vcur], c->w[c->vcur], c->P[c->pcur], c->pcorr, c->P[pn], c->scal,
+                           c->bulk_pending ? 1 : 0, c->cs.dt, cc, c->cs.rhof);
     CHECK_LAUNCH(c);
     prof_end(c, IBMDEM_K_PROJECT, 9.0 * 8.0 * (double)c->n);
     c->pcur = pn;
@@ -572,10 +630,10 @@ int fluid_project(ibmdem_ctx *c, double cc) {
     return IBMDEM_OK;
 }
 
-int fluid_upload_bc(ibmdem_ctx *c) {
+int fluid_upload_bc(ibmdem_ctx *c, int reset_planes) {
     const GridDev &g = c->g;
     hipLaunchKernelGGL(k_wall_faces, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
-                       c->v[c->vcur], c->w[c->vcur]);
+                       c->v[c->vcur], c->w[c->vcur], reset_planes);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }

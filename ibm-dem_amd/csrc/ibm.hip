@@ -85,7 +85,7 @@ __global__ __launch_bounds__(T) void k_records(int M, const double *__restrict__
 
 // UpdateFluidVelocityOnLagrangianPoints: U_p = sum w u dx dy dz,
 // fdrag += (U_p - v_p)/dt; also stores fdrag*vol in the marker record
-template <bool P2>
+template <bool P2, bool OPEN>
 __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const double *__restrict__ u,
                                               const double *__restrict__ v, const double *__restrict__ w,
                                               double *__restrict__ rec, const double *__restrict__ mv,
@@ -122,19 +122,19 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const double *__
         for (int jj = 0; jj < 3; ++jj)
             for (int ii = 0; ii < 4; ++ii) {
                 const double wt = qdiv<P2>(fx[ii] * cy[jj] * cz[kk], dV, idV);
-                ulp = ulp + wt * vel_at<0>(u, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
+                ulp = ulp + wt * vel_at<0, OPEN>(u, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
             }
     for (int kk = 0; kk < 3; ++kk)
         for (int jj = 0; jj < 4; ++jj)
             for (int ii = 0; ii < 3; ++ii) {
                 const double wt = qdiv<P2>(cx[ii] * fy[jj] * cz[kk], dV, idV);
-                vlp = vlp + wt * vel_at<1>(v, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
+                vlp = vlp + wt * vel_at<1, OPEN>(v, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
             }
     for (int kk = 0; kk < 4; ++kk)
         for (int jj = 0; jj < 3; ++jj)
             for (int ii = 0; ii < 3; ++ii) {
                 const double wt = qdiv<P2>(cx[ii] * cy[jj] * fz[kk], dV, idV);
-                wlp = wlp + wt * vel_at<2>(w, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
+                wlp = wlp + wt * vel_at<2, OPEN>(w, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
             }
     const double f0 = fdrag[m] + 1.0 * (ulp - mv[m]) / dt;
     const double f1 = fdrag[(long)M + m] + 1.0 * (vlp - mv[(long)M + m]) / dt;
@@ -531,7 +531,8 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     for (int c = 0; c < 3; ++c) {
         const double fo = a.first ? 0.0 : a.fact[c * a.fstride + fs];
         a.fact[c * a.fstride + fs] = fn[c];
-        const bool face = tgt[c] == 1 && g.bt[2 * c] == IBMDEM_BC_WALL;
+        // the low boundary faces (wall, inflow) keep their fixed values
+        const bool face = tgt[c] == 1 && g.bt[2 * c] != IBMDEM_BC_PERIODIC;
         if (!face && !(fn[c] == 0.0 && fo == 0.0)) vel[c][o] = vel[c][o] + a.dt * (fn[c] - fo);
     }
 }
@@ -624,12 +625,13 @@ int ibm_interpolate(ibmdem_ctx *c) {
     if (c->M == 0) return IBMDEM_OK;
     const int cur = c->vcur;
     prof_begin(c, IBMDEM_K_INTERP);
-    if (pow2_grid(c))
-        hipLaunchKernelGGL((k_interp<true>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->u[cur],
-                           c->v[cur], c->w[cur], c->mrec, c->mv, c->fdrag, c->mvol, c->dV, 1.0 / c->dV, c->cs.dt);
-    else
-        hipLaunchKernelGGL((k_interp<false>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->u[cur],
-                           c->v[cur], c->w[cur], c->mrec, c->mv, c->fdrag, c->mvol, c->dV, 1.0 / c->dV, c->cs.dt);
+#define LAUNCH_INTERP(P2, OP)                                                                                      \
+    hipLaunchKernelGGL((k_interp<P2, OP>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->u[cur],      \
+                       c->v[cur], c->w[cur], c->mrec, c->mv, c->fdrag, c->mvol, c->dV, 1.0 / c->dV, c->cs.dt)
+    const bool open = grid_open(c->g);
+    if (pow2_grid(c)) { if (open) LAUNCH_INTERP(true, true); else LAUNCH_INTERP(true, false); }
+    else { if (open) LAUNCH_INTERP(false, true); else LAUNCH_INTERP(false, false); }
+#undef LAUNCH_INTERP
     // per marker: record 32 B, v 24, fdrag 24 + 24, vol 8, fdrag*vol 24;
     // one read of each footprint velocity (cell, component)
     prof_end(c, IBMDEM_K_INTERP, 136.0 * c->M + 8.0 * 3.0 * (double)c->footprint_cells);

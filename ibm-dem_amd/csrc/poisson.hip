@@ -11,6 +11,10 @@
 // (poisson.F90:78,97-100,126); that system's solution is the solution of
 // the singular system with b[0] := -sum_{i>0} b[i], shifted so that
 // P'(1,1,1) = 0, which is what is solved here.
+// Inflow sides are Neumann like walls. An outflow side (no counterpart in
+// the reference) holds P' = 0 on the face: the ghost is -P', which adds
+// 2/h^2 to the diagonal of the last cell; the operator is then
+// non-singular, and there is no pinned row and no gauge shift.
 //
 // Preconditioner: one symmetric V-cycle. Smoother red-black Gauss-Seidel
 // (pre: red then black, post: black then red); coarsest level: symmetric
@@ -71,11 +75,16 @@ __device__ __forceinline__ T nbsum(const T *__restrict__ x, const LevelDev &L, i
     return sx + syy + szz;
 }
 
+// per-dimension diagonal count: 1 per present neighbour, 2 for an outflow
+// (Dirichlet) face
+__device__ __forceinline__ int diag_count(int i, int n, int lo, int hi) {
+    int c = 2 - ((i == 0 && lo != IBMDEM_BC_PERIODIC) ? 1 : 0) - ((i == n - 1 && hi != IBMDEM_BC_PERIODIC) ? 1 : 0);
+    if (i == n - 1 && hi == IBMDEM_BC_OUTFLOW) c += 2;
+    return c;
+}
 __device__ __forceinline__ int diag_index(const LevelDev &L, int i, int j, int k) {
-    int cx = 2 - ((i == 0 && L.bt[0] == IBMDEM_BC_WALL) ? 1 : 0) - ((i == L.nx - 1 && L.bt[1] == IBMDEM_BC_WALL) ? 1 : 0);
-    int cy = 2 - ((j == 0 && L.bt[2] == IBMDEM_BC_WALL) ? 1 : 0) - ((j == L.ny - 1 && L.bt[3] == IBMDEM_BC_WALL) ? 1 : 0);
-    int cz = 2 - ((k == 0 && L.bt[4] == IBMDEM_BC_WALL) ? 1 : 0) - ((k == L.nz - 1 && L.bt[5] == IBMDEM_BC_WALL) ? 1 : 0);
-    return cx * 9 + cy * 3 + cz;
+    return diag_count(i, L.nx, L.bt[0], L.bt[1]) * 16 + diag_count(j, L.ny, L.bt[2], L.bt[3]) * 4 +
+           diag_count(k, L.nz, L.bt[4], L.bt[5]);
 }
 
 // one red-black half sweep; each thread owns the cell of the colour in a
@@ -122,8 +131,8 @@ __global__ __launch_bounds__(RB) void k_residual(LevelDev L, const T *__restrict
 }
 
 __device__ __forceinline__ int fine_index(int f, int n, int lo, int hi) {
-    if (f < 0) return lo == IBMDEM_BC_WALL ? 0 : f + n;
-    if (f >= n) return hi == IBMDEM_BC_WALL ? n - 1 : f - n;
+    if (f < 0) return lo != IBMDEM_BC_PERIODIC ? 0 : f + n;
+    if (f >= n) return hi != IBMDEM_BC_PERIODIC ? n - 1 : f - n;
     return f;
 }
 
@@ -161,8 +170,8 @@ __global__ __launch_bounds__(RB) void k_restrict(LevelDev F, const T *__restrict
 __device__ __forceinline__ void parents(int f, int nc, int lo, int hi, int &p0, int &p1) {
     int cc = f / 2;
     int nb = (f & 1) ? cc + 1 : cc - 1;
-    if (nb < 0) nb = lo == IBMDEM_BC_WALL ? cc : nb + nc;
-    else if (nb >= nc) nb = hi == IBMDEM_BC_WALL ? cc : nb - nc;
+    if (nb < 0) nb = lo != IBMDEM_BC_PERIODIC ? cc : nb + nc;
+    else if (nb >= nc) nb = hi != IBMDEM_BC_PERIODIC ? cc : nb - nc;
     p0 = cc; p1 = nb;
 }
 
@@ -219,8 +228,8 @@ __global__ __launch_bounds__(64) void k_coarse_lds(LevelDev L, T *__restrict__ x
                 int p[3] = {a[0], a[1], a[2]};
                 int v = p[d] + (dir ? 1 : -1);
                 int idx;
-                if (v < 0) idx = L.bt[2 * d] == IBMDEM_BC_WALL ? -1 : v + m[d];
-                else if (v >= m[d]) idx = L.bt[2 * d + 1] == IBMDEM_BC_WALL ? -1 : v - m[d];
+                if (v < 0) idx = L.bt[2 * d] != IBMDEM_BC_PERIODIC ? -1 : v + m[d];
+                else if (v >= m[d]) idx = L.bt[2 * d + 1] != IBMDEM_BC_PERIODIC ? -1 : v - m[d];
                 else idx = v;
                 p[d] = idx;
                 nb[2 * d + dir][q] = (short)(idx < 0 ? n : (p[2] * L.ny + p[1]) * L.nx + p[0]);
@@ -400,7 +409,7 @@ __global__ __launch_bounds__(RED_T) void k_cg_init(LevelDev L, long n, const dou
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
         int i, j, k;
         lcell(L, (int)q, i, j, k);
-        r[q] = q == 0 ? -scal[SC_PIN] : rhs[((long)k * L.ny + j) * L.nx + i];
+        r[q] = (q == 0 && !L.dirichlet) ? -scal[SC_PIN] : rhs[((long)k * L.ny + j) * L.nx + i];
         x[q] = 0.0;
     }
 }
@@ -416,10 +425,10 @@ __global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, c
 }
 
 // gauge P'(1,1,1) = 0: pcorr = x - x(1,1,1), x in level-0 storage, pcorr in
-// field order
+// field order (no shift with an outflow side, where P' = 0 on the face)
 __global__ __launch_bounds__(RED_T) void k_gauge(LevelDev L, long n, const double *__restrict__ x,
                                                  double *__restrict__ pc) {
-    const double x0 = x[0];
+    const double x0 = L.dirichlet ? 0.0 : x[0];
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
         int i, j, k;
         lcell(L, (int)q, i, j, k);
@@ -437,14 +446,17 @@ LevelDev make_level(int nx, int ny, int nz, const int bt[6], const double h[3]) 
     L.ihx2 = 1.0 / (h[0] * h[0]);
     L.ihy2 = 1.0 / (h[1] * h[1]);
     L.ihz2 = 1.0 / (h[2] * h[2]);
-    for (int cx = 0; cx < 3; ++cx)
-        for (int cy = 0; cy < 3; ++cy)
-            for (int cz = 0; cz < 3; ++cz) {
+    for (int cx = 0; cx < 4; ++cx)
+        for (int cy = 0; cy < 4; ++cy)
+            for (int cz = 0; cz < 4; ++cz) {
                 double dsum0 = cx * L.ihx2, dsum1 = cy * L.ihy2, dsum2 = cz * L.ihz2;
                 double d = dsum0 + dsum1 + dsum2;
-                L.diag[cx * 9 + cy * 3 + cz] = d;
-                L.idiag[cx * 9 + cy * 3 + cz] = d > 0 ? 1.0 / d : 0.0;
+                L.diag[cx * 16 + cy * 4 + cz] = d;
+                L.idiag[cx * 16 + cy * 4 + cz] = d > 0 ? 1.0 / d : 0.0;
             }
+    L.dirichlet = 0;
+    for (int s = 0; s < 6; ++s)
+        if (bt[s] == IBMDEM_BC_OUTFLOW) L.dirichlet = 1;
     const int n[3] = {nx, ny, nz};
     L.rb_valid = 1;
     for (int d = 0; d < 3; ++d)
@@ -612,8 +624,8 @@ int poisson_begin(ibmdem_ctx *c) {
     const long n = c->n;
     hipStream_t s = c->stream;
     int rc;
-    // pinned row: b[0] = -sum_{q>0} b[q]
-    if ((rc = reduce_sum(c, c->rhs, nullptr, n, SC_PIN, 1))) return rc;
+    // pinned row: b[0] = -sum_{q>0} b[q] (none with an outflow side)
+    if (!c->lev64[0].d.dirichlet && (rc = reduce_sum(c, c->rhs, nullptr, n, SC_PIN, 1))) return rc;
     hipLaunchKernelGGL(k_cg_init, dim3(stream_blocks(n)), dim3(RED_T), 0, s, c->lev64[0].d, n, c->rhs, c->cg_r,
                        c->cg_x, c->scal);
     CHECK_LAUNCH(c);

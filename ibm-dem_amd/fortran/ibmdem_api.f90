@@ -23,6 +23,8 @@ module ibmdem_api
     ! boundary types: the reference's bType (0 periodic, 1 solid wall)
     integer(c_int32_t), parameter :: IBMDEM_BC_PERIODIC = 0
     integer(c_int32_t), parameter :: IBMDEM_BC_WALL = 1
+    integer(c_int32_t), parameter :: IBMDEM_BC_INFLOW = 2
+    integer(c_int32_t), parameter :: IBMDEM_BC_OUTFLOW = 3
     ! precision forms
     integer(c_int32_t), parameter :: IBMDEM_FP64 = 0
     integer(c_int32_t), parameter :: IBMDEM_MIXED = 1
@@ -59,6 +61,7 @@ module ibmdem_api
         integer(c_int32_t) :: mg_max_levels = 0
         integer(c_int32_t) :: stencil_tile = 0
         integer(c_int32_t) :: time_scheme = 0
+        real(c_double)     :: inflow(3) = 0d0
         integer(c_int32_t) :: reserved(6) = 0
     end type
 
@@ -119,6 +122,18 @@ module ibmdem_api
                 bind(C, name="ibmdem_download_fields")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, u, v, w, p
+        end function
+        integer(c_int) function ibmdem_download_outflow(ctx, side, plane) &
+                bind(C, name="ibmdem_download_outflow")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx, plane
+            integer(c_int32_t), value :: side
+        end function
+        integer(c_int) function ibmdem_upload_outflow(ctx, side, plane) &
+                bind(C, name="ibmdem_upload_outflow")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx, plane
+            integer(c_int32_t), value :: side
         end function
         integer(c_int) function ibmdem_download_force(ctx, fx, fy, fz) &
                 bind(C, name="ibmdem_download_force")

@@ -15,6 +15,9 @@
 !   partX, partV, ...  (M, 3)         component-major marker vectors
 !   msX, msV, ...      (B, 3)         component-major body vectors
 !   msRot, ...         (9, B)         row-major 3x3 per body (mat(1:9))
+!   outflowX/Y/Z       (ny,nz) (nx,nz) (nx,ny)  outflow planes of sides 2,
+!                                     4, 6 (size 0 where the side is not
+!                                     an outflow)
 !
 ! Errors: the reference checks no return code. Here every call's status is
 ! kept in ibmStatus; with stopOnError (the default) a failing call prints
@@ -44,6 +47,7 @@ module ibmdem_host
     integer(c_int32_t), allocatable, target, save :: msFixed(:)
     real(c_double), allocatable, target, save :: msFdrag(:,:), msFmom(:,:), msFcol(:,:)
     real(c_double), allocatable, target, save :: msFmomCol(:,:), msFhydro(:,:), msFmomHydro(:,:)
+    real(c_double), allocatable, target, save :: outflowX(:,:), outflowY(:,:), outflowZ(:,:)
 
     character(len=8), parameter :: SNAPSHOT_MAGIC = "IBMDEMS1"
 
@@ -101,6 +105,10 @@ contains
         msX = 0d0; msV = 0d0; msVold = 0d0; msOmega = 0d0; msAngMom = 0d0; msAngMomOld = 0d0
         msRot = 0d0; msRotGlo = 0d0; msInertia = 0d0; msM = 0d0; msRho = 0d0; msFixed = 0
         msFdrag = 0d0; msFmom = 0d0; msFcol = 0d0; msFmomCol = 0d0; msFhydro = 0d0; msFmomHydro = 0d0
+        allocate(outflowX(merge(ny, 0, ibmCase%btype(2) == IBMDEM_BC_OUTFLOW), nz))
+        allocate(outflowY(merge(nx, 0, ibmCase%btype(4) == IBMDEM_BC_OUTFLOW), nz))
+        allocate(outflowZ(merge(nx, 0, ibmCase%btype(6) == IBMDEM_BC_OUTFLOW), ny))
+        outflowX = 0d0; outflowY = 0d0; outflowZ = 0d0
     end subroutine
 
     subroutine FreeHostMirrors()
@@ -109,6 +117,7 @@ contains
         deallocate(msX, msV, msVold, msOmega, msAngMom, msAngMomOld)
         deallocate(msRot, msRotGlo, msInertia, msM, msRho, msFixed)
         deallocate(msFdrag, msFmom, msFcol, msFmomCol, msFhydro, msFmomHydro)
+        deallocate(outflowX, outflowY, outflowZ)
     end subroutine
 
     ! C-ABI descriptors over the host mirrors. An empty array (no markers or
@@ -143,6 +152,10 @@ contains
         if (ibmCase%n_markers > 0) call check(ibmdem_upload_markers(ibmCtx, MarkerView()), "ibmdem_upload_markers")
         call check(ibmdem_upload_fields(ibmCtx, c_loc(u), c_loc(v), c_loc(w), c_loc(P)), &
                    "ibmdem_upload_fields")
+        ! after the fields, whose upload resets the outflow planes
+        if (size(outflowX) > 0) call check(ibmdem_upload_outflow(ibmCtx, 1, c_loc(outflowX)), "ibmdem_upload_outflow")
+        if (size(outflowY) > 0) call check(ibmdem_upload_outflow(ibmCtx, 3, c_loc(outflowY)), "ibmdem_upload_outflow")
+        if (size(outflowZ) > 0) call check(ibmdem_upload_outflow(ibmCtx, 5, c_loc(outflowZ)), "ibmdem_upload_outflow")
     end subroutine
 
     ! device -> host mirrors (the reference's output copies, io.F90)
@@ -151,6 +164,9 @@ contains
         if (ibmCase%n_markers > 0) call check(ibmdem_download_markers(ibmCtx, MarkerView()), "ibmdem_download_markers")
         call check(ibmdem_download_fields(ibmCtx, c_loc(u), c_loc(v), c_loc(w), c_loc(P)), &
                    "ibmdem_download_fields")
+        if (size(outflowX) > 0) call check(ibmdem_download_outflow(ibmCtx, 1, c_loc(outflowX)), "ibmdem_download_outflow")
+        if (size(outflowY) > 0) call check(ibmdem_download_outflow(ibmCtx, 3, c_loc(outflowY)), "ibmdem_download_outflow")
+        if (size(outflowZ) > 0) call check(ibmdem_download_outflow(ibmCtx, 5, c_loc(outflowZ)), "ibmdem_download_outflow")
     end subroutine
 
     !--------------------------------------------------------------------------
@@ -348,6 +364,7 @@ contains
         read(un) msRot, msRotGlo, msInertia, msM, msRho, msFixed
         read(un) msFdrag, msFmom, msFcol, msFmomCol, msFhydro, msFmomHydro
         read(un) u, v, w, P
+        read(un) outflowX, outflowY, outflowZ
         read(un) solveInfo%iterations, solveInfo%converged, solveInfo%rel_residual, solveInfo%rhs_norm
         close(un)
         call UploadState()
@@ -368,6 +385,7 @@ contains
         write(un) msRot, msRotGlo, msInertia, msM, msRho, msFixed
         write(un) msFdrag, msFmom, msFcol, msFmomCol, msFhydro, msFmomHydro
         write(un) u, v, w, P
+        write(un) outflowX, outflowY, outflowZ
         write(un) solveInfo%iterations, solveInfo%converged, solveInfo%rel_residual, solveInfo%rhs_norm
         close(un)
     end subroutine

@@ -25,6 +25,8 @@ ERRORS = {
 }
 BC_PERIODIC = 0
 BC_WALL = 1
+BC_INFLOW = 2
+BC_OUTFLOW = 3
 FP64 = 0
 MIXED = 1
 SCHEME_EULER = 0
@@ -67,6 +69,7 @@ class Case(C.Structure):
         ("mg_max_levels", _i32),
         ("stencil_tile", _i32),
         ("time_scheme", _i32),
+        ("inflow", _f64 * 3),
         ("reserved", _i32 * 6),
     ]
 
@@ -103,6 +106,8 @@ _FUNCS = {
     "ibmdem_sync": [C.c_void_p],
     "ibmdem_upload_fields": [C.c_void_p, _pd, _pd, _pd, _pd],
     "ibmdem_download_fields": [C.c_void_p, _pd, _pd, _pd, _pd],
+    "ibmdem_download_outflow": [C.c_void_p, _i32, _pd],
+    "ibmdem_upload_outflow": [C.c_void_p, _i32, _pd],
     "ibmdem_download_force": [C.c_void_p, _pd, _pd, _pd],
     "ibmdem_download_poisson": [C.c_void_p, _pd, _pd],
     "ibmdem_upload_poisson_rhs": [C.c_void_p, _pd],
@@ -296,6 +301,27 @@ class Simulation:
         out = [np.zeros(self.shape) for _ in range(4)]
         self.call("ibmdem_download_fields", *[_p(a) for a in out])
         return out
+
+    def outflow_shape(self, side: int):
+        """Array shape of the outflow plane of side 1, 3 or 5 (slowest
+        transverse dimension first)."""
+        nz, ny, nx = self.shape
+        shapes = {1: (nz, ny), 3: (nz, nx), 5: (ny, nx)}
+        if side not in shapes:
+            raise ValueError("outflow planes exist on sides 1, 3 and 5")
+        return shapes[side]
+
+    def download_outflow(self, side: int):
+        """Normal velocity on the outflow face of `side` (the outflow plane)."""
+        out = np.zeros(self.outflow_shape(side))
+        self.call("ibmdem_download_outflow", side, _p(out))
+        return out
+
+    def upload_outflow(self, side: int, plane) -> None:
+        a = np.ascontiguousarray(plane, dtype=np.float64)
+        if a.shape != self.outflow_shape(side):
+            raise ValueError(f"plane shape {a.shape} != {self.outflow_shape(side)}")
+        self.call("ibmdem_upload_outflow", side, _p(a))
 
     def download_force(self):
         out = [np.zeros(self.shape) for _ in range(3)]

@@ -22,7 +22,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .abi import (BC_PERIODIC, BC_WALL, FP64, MIXED, BodyArrays, Case, MarkerArrays)
+from .abi import (BC_INFLOW, BC_OUTFLOW, BC_PERIODIC, BC_WALL, FP64, MIXED, BodyArrays, Case,
+                  MarkerArrays)
 
 PI_F32 = float(np.float32(3.1415926536))
 SEED_BASE = 20260000
@@ -93,7 +94,8 @@ def build_case(nx, ny, nz, btype, *, h=1.0, visc=0.04, rhof=1.0, dt=None,
                bulk_switch=(0, 0, 0), bulk_target=(0.0, 0.0, 0.0),
                precision=FP64, rtol=1e-10, max_iter=200, n_mdf=5,
                couple_interval=10, u_max=0.2, n_markers=0, n_bodies=0,
-               max_pairs=200, mg_pre=2, mg_post=2, mg_coarse=24, time_scheme=0) -> Case:
+               max_pairs=200, mg_pre=2, mg_post=2, mg_coarse=24, time_scheme=0,
+               inflow=(0.0, 0.0, 0.0)) -> Case:
     c = Case()
     c.nx, c.ny, c.nz = nx, ny, nz
     for s in range(6):
@@ -124,6 +126,8 @@ def build_case(nx, ny, nz, btype, *, h=1.0, visc=0.04, rhof=1.0, dt=None,
     c.mg_max_levels = 0
     c.stencil_tile = 0
     c.time_scheme = time_scheme
+    for d in range(3):
+        c.inflow[d] = float(inflow[d])
     return c
 
 
@@ -356,6 +360,54 @@ def small_walls(n=(24, 20, 32), D=8.0, rho=1.5, seed=SEED_BASE + 12,
     w[:] = 0.01 * np.sin(np.pi * (x + 0.5) / nx) * np.sin(np.pi * (y + 0.5) / ny)
     p[:] = 0.001 * rng.normal(size=p.shape)
     return Setup(f"walls-{nx}x{ny}x{nz}", case, mk, bd, (u, v, w, p), {"seed": seed})
+
+
+def small_open(n=(32, 20, 24), axis=0, U=0.05, D=8.0, rho=1.5, seed=SEED_BASE + 13,
+               precision=FP64, rtol=1e-12, periodic=(), time_scheme=0, gap=0.75,
+               spheres=True) -> Setup:
+    """Channel with an inflow side (low `axis`, normal velocity U) and an
+    outflow side (high `axis`); the other sides walls, or periodic for the
+    dimensions in `periodic`. Two spheres in near contact sit against the
+    inflow face (their markers reach the inflow ghost layers and their mirror
+    images spread into the domain), a third near the outflow face; random
+    smooth initial fields with the inflow face at U (spheres=False: the
+    fluid alone)."""
+    nx, ny, nz = n
+    rng = np.random.Generator(np.random.PCG64(seed))
+    bt = [BC_WALL] * 6
+    for d in periodic:
+        bt[2 * d] = bt[2 * d + 1] = BC_PERIODIC
+    bt[2 * axis], bt[2 * axis + 1] = BC_INFLOW, BC_OUTFLOW
+    inflow = [0.0, 0.0, 0.0]
+    inflow[axis] = U
+    case = build_case(nx, ny, nz, bt, visc=0.04, gravity=(0.0, 0.0, -1e-3), u_max=0.3,
+                      precision=precision, rtol=rtol, inflow=inflow, time_scheme=time_scheme)
+    L = np.array([nx, ny, nz], dtype=np.float64)
+    R = 0.5 * D
+    o = (axis + 1) % 3
+    c1 = 0.5 * L + np.array([0.31, -0.27, 0.19])
+    c1[axis] = R + 0.4
+    c1[o] = R + 0.6
+    c2 = c1.copy()
+    c2[o] += D + gap            # marker shells inside the 1.65h candidate distance
+    c2[axis] += 0.35
+    c3 = 0.5 * L + np.array([-0.23, 0.41, -0.17])
+    c3[axis] = L[axis] - R - 0.7
+    sph = [Sphere(tuple(c1), D, rho), Sphere(tuple(c2), D, rho), Sphere(tuple(c3), D, rho)]
+    mk, bd = place_spheres(case, sph if spheres else [])
+    bd.v[:] = rng.normal(0.0, 0.01, size=bd.v.shape)
+    bd.vold[:] = bd.v
+    for i in range(mk.body.size):
+        mk.v[:, i] = bd.v[:, mk.body[i]]
+    u, v, w, p = zero_fields(case)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    f = [u, v, w]
+    f[axis][:] = U
+    u += 0.01 * np.sin(np.pi * (y + 0.5) / ny + 0.2) * np.cos(np.pi * (z + 0.5) / nz)
+    v += 0.01 * np.sin(np.pi * (x + 0.5) / nx) * np.cos(np.pi * (z + 0.5) / nz + 0.1)
+    w += 0.01 * np.cos(np.pi * (x + 0.5) / nx) * np.sin(np.pi * (y + 0.5) / ny)
+    p[:] = 0.001 * rng.normal(size=p.shape)
+    return Setup(f"open{'xyz'[axis]}-{nx}x{ny}x{nz}", case, mk, bd, (u, v, w, p), {"seed": seed})
 
 
 def load(sim, setup: Setup) -> None:

@@ -41,9 +41,20 @@ extern "C" {
 #define IBMDEM_ERR_ALLOC          7  /* device allocation failed             */
 #define IBMDEM_ERR_UNSUPPORTED    8  /* feature not available in this form   */
 
-/* ---- boundary types (reference bType: 0 periodic, 1 solid wall) -------- */
+/* ---- boundary types (reference bType: 0 periodic, 1 solid wall) --------
+ * INFLOW and OUTFLOW have no counterpart in the reference (its fluid
+ * boundaries are periodic or no-slip, initialization.F90:178-237):
+ *   INFLOW  (low sides only): the velocity is cs.inflow at the face, the
+ *           normal component stored on the face, the tangential ones
+ *           through the ghost 2*U_t - u; pressure Neumann;
+ *   OUTFLOW (high sides only): zero normal gradient of every velocity
+ *           component, pressure (and its correction) zero at the face; the
+ *           Poisson problem is then non-singular (no pinned cell).
+ * Markers see neither as a wall (no mirror images, no wall contact). */
 #define IBMDEM_BC_PERIODIC 0
 #define IBMDEM_BC_WALL     1
+#define IBMDEM_BC_INFLOW   2
+#define IBMDEM_BC_OUTFLOW  3
 
 /* ---- precision forms ---------------------------------------------------- */
 #define IBMDEM_FP64  0   /* everything in FP64                                   */
@@ -90,6 +101,8 @@ typedef struct ibmdem_case {
     int32_t mg_max_levels;       /* 0 -> as many as the grid allows         */
     int32_t stencil_tile;        /* predictor tile choice (0 -> default)    */
     int32_t time_scheme;         /* IBMDEM_SCHEME_*                         */
+    double  inflow[3];           /* normal velocity of the inflow side of
+                                  * dimension d (x, y, z)                  */
     int32_t reserved[6];
 } ibmdem_case;
 
@@ -137,11 +150,20 @@ int ibmdem_destroy(ibmdem_ctx *ctx);
 const char *ibmdem_last_error(const ibmdem_ctx *ctx);
 int ibmdem_sync(ibmdem_ctx *ctx);
 /* u, v, w, p: nx*ny*nz each; NULL skips. Upload applies the boundary
- * rules (wall-normal face velocities forced to zero, fluid.F90:243-248). */
+ * rules (wall-normal face velocities forced to zero, fluid.F90:243-248;
+ * inflow faces set to the inflow velocity; the outflow plane of an uploaded
+ * normal component reset to its last interior face). */
 int ibmdem_upload_fields(ibmdem_ctx *ctx, const double *u, const double *v,
                          const double *w, const double *p);
 int ibmdem_download_fields(ibmdem_ctx *ctx, double *u, double *v, double *w,
                            double *p);
+/* Outflow plane of side 1, 3 or 5 (type IBMDEM_BC_OUTFLOW): the normal
+ * velocity on the boundary face, which the interior-only field arrays do not
+ * hold; ny*nz (side 1), nx*nz (side 3) or nx*ny (side 5) values, the lower
+ * transverse index fastest. No counterpart in the reference (its fluid has
+ * no outflow side); part of the state a snapshot must carry. */
+int ibmdem_download_outflow(ibmdem_ctx *ctx, int32_t side, double *plane);
+int ibmdem_upload_outflow(ibmdem_ctx *ctx, int32_t side, const double *plane);
 /* IBM momentum source f = (fx,fy,fz) on the interior (fluid.F90 fx,fy,fz). */
 int ibmdem_download_force(ibmdem_ctx *ctx, double *fx, double *fy, double *fz);
 /* Pressure correction P' of the last solve and the Poisson RHS velDiv. */

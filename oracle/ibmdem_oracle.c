@@ -37,6 +37,20 @@
  *     multigrid-preconditioned CG the GPU runs (below); the pinned row
  *     (poisson.F90:78,97-100,126) is reproduced exactly by solving the
  *     singular system with b[0] := -sum_{i>0} b[i] and shifting P'(1,1,1)=0.
+ * Inflow and outflow sides have no counterpart in the reference (its fluid
+ * is periodic or walled). Their rules here are this library's own, the same
+ * in the GPU library, and are pinned by closed forms, not by the reference:
+ *   - inflow (low sides): the normal velocity is U_n on the face and on every
+ *     ghost layer beyond it; tangential ghosts mirror with negation;
+ *     pressure ghosts mirror (Neumann);
+ *   - outflow (high sides): the normal velocity on the boundary face n+1 is
+ *     state of its own (the outflow plane): set to the last interior face
+ *     before the divergence, corrected by the projection with P' = 0 on the
+ *     face, and copied, with the transverse mirror signs, into ghost layers
+ *     n+1 and n+2; tangential ghosts mirror without negation; pressure ghosts
+ *     mirror with negation (P = 0 on the face), which makes the Poisson
+ *     operator non-singular (no pinned row, no gauge shift);
+ *   - markers treat both like walls (mirror images, wall contact).
  * No OpenMP: one core, deterministic.
  */
 #include "../include/ibmdem.h"
@@ -78,6 +92,10 @@ struct ibmdem_ctx {
     double *fx, *fy, *fz, *fxo, *fyo, *fzo, *RHSul, *RHSvl, *RHSwl;
     double *Px, *Py, *Pz;
     double modP[3];
+    /* open sides: inflow normal velocity per dimension; outflow planes
+     * (interior transverse cells, lower index fastest) */
+    double uin[3];
+    double *plane[3];
     /* parcels: locals [0,M), images [M,np) */
     int M, B, np, cap;
     double *x, *vel, *fdrag, *fcol;   /* 3*cap, component-major by cap */
@@ -131,7 +149,11 @@ static void fill_ghost_one(ibmdem_ctx *c, double *f, int is_vel) {
     for (int s = 1; s <= 6; ++s) {
         int recv = (s % 2 == 0) ? s - 1 : s + 1;
         int dim = (s - 1) / 2;
-        int wall = c->btype[recv - 1] == IBMDEM_BC_WALL;
+        const int bt = c->btype[recv - 1];
+        /* every non-periodic side mirrors; velocities negate on walls and
+         * inflow sides, scalars on outflow sides */
+        int wall = bt != IBMDEM_BC_PERIODIC;
+        int neg = is_vel ? (bt == IBMDEM_BC_WALL || bt == IBMDEM_BC_INFLOW) : bt == IBMDEM_BC_OUTFLOW;
         int n = dim == 0 ? nx : dim == 1 ? ny : nz;
         /* index ranges of the other two dims */
         int lo1, hi1, lo2, hi2;
@@ -156,10 +178,39 @@ static void fill_ghost_one(ibmdem_ctx *c, double *f, int is_vel) {
                     else if (dim == 1) { is = ID(c, a, src, b); id = ID(c, a, dst, b); }
                     else { is = ID(c, a, b, src); id = ID(c, a, b, dst); }
                     double val = f[is];
-                    if (wall && is_vel) val = -val;
+                    if (wall && neg) val = -val;
                     f[id] = val;
                 }
     }
+}
+
+/* a transverse halo index of the outflow plane mapped onto the interior,
+ * with the velocity mirror sign of its side (outflow sides do not negate) */
+static int plane_map(const ibmdem_ctx *c, int a, int d, double *sg) {
+    const int n = d == 0 ? c->nx : d == 1 ? c->ny : c->nz;
+    const int lo = c->btype[2 * d], hi = c->btype[2 * d + 1];
+    if (a < 1) {
+        if (lo == IBMDEM_BC_PERIODIC) return a + n;
+        *sg = -*sg;
+        return 1 - a;
+    }
+    if (a > n) {
+        if (hi == IBMDEM_BC_PERIODIC) return a - n;
+        if (hi != IBMDEM_BC_OUTFLOW) *sg = -*sg;
+        return 2 * n + 1 - a;
+    }
+    return a;
+}
+/* outflow plane value of dimension d seen at halo cell (i,j,k) */
+static double plane_at(const ibmdem_ctx *c, int d, int i, int j, int k) {
+    double sg = 1.0;
+    int a[3] = {i, j, k};
+    for (int e = 0; e < 3; ++e)
+        if (e != d) a[e] = plane_map(c, a[e], e, &sg);
+    size_t off = d == 0 ? (size_t)(a[1] - 1) + (size_t)c->ny * (a[2] - 1)
+               : d == 1 ? (size_t)(a[0] - 1) + (size_t)c->nx * (a[2] - 1)
+                        : (size_t)(a[0] - 1) + (size_t)c->nx * (a[1] - 1);
+    return sg * c->plane[d][off];
 }
 
 static void boundary_vel(ibmdem_ctx *c) {
@@ -174,7 +225,29 @@ static void boundary_vel(ibmdem_ctx *c) {
                 if (c->btype[3] == 1 && j >= ny + 1) c->v[id] = 0.0;
                 if (c->btype[4] == 1 && k <= 1) c->w[id] = 0.0;
                 if (c->btype[5] == 1 && k >= nz + 1) c->w[id] = 0.0;
+                /* open sides */
+                if (c->btype[0] == IBMDEM_BC_INFLOW && i <= 1) c->u[id] = c->uin[0];
+                if (c->btype[2] == IBMDEM_BC_INFLOW && j <= 1) c->v[id] = c->uin[1];
+                if (c->btype[4] == IBMDEM_BC_INFLOW && k <= 1) c->w[id] = c->uin[2];
+                if (c->btype[1] == IBMDEM_BC_OUTFLOW && i >= nx + 1) c->u[id] = plane_at(c, 0, i, j, k);
+                if (c->btype[3] == IBMDEM_BC_OUTFLOW && j >= ny + 1) c->v[id] = plane_at(c, 1, i, j, k);
+                if (c->btype[5] == IBMDEM_BC_OUTFLOW && k >= nz + 1) c->w[id] = plane_at(c, 2, i, j, k);
             }
+}
+
+/* the normal velocity of the last interior face into the outflow plane of
+ * dimension d (zero normal gradient) */
+static void plane_from_interior(ibmdem_ctx *c, int d) {
+    double *f = d == 0 ? c->u : d == 1 ? c->v : c->w;
+    if (d == 0)
+        for (int k = 1; k <= c->nz; ++k)
+            for (int j = 1; j <= c->ny; ++j) c->plane[0][(size_t)(j - 1) + (size_t)c->ny * (k - 1)] = f[ID(c, c->nx, j, k)];
+    else if (d == 1)
+        for (int k = 1; k <= c->nz; ++k)
+            for (int i = 1; i <= c->nx; ++i) c->plane[1][(size_t)(i - 1) + (size_t)c->nx * (k - 1)] = f[ID(c, i, c->ny, k)];
+    else
+        for (int j = 1; j <= c->ny; ++j)
+            for (int i = 1; i <= c->nx; ++i) c->plane[2][(size_t)(i - 1) + (size_t)c->nx * (j - 1)] = f[ID(c, i, j, c->nz)];
 }
 
 static void fill_ghost_fluid(ibmdem_ctx *c) {
@@ -297,9 +370,10 @@ static int build_images(ibmdem_ctx *c) {
                 c->body[p] = c->body[s]; c->ptype[p] = c->ptype[s];
                 refresh_image(c, p);
             }
-        } else if (c->btype[idxRecv - 1] == IBMDEM_BC_WALL) {
+        } else if (c->btype[idxRecv - 1] != IBMDEM_BC_PERIODIC) {
             /* UpdateFullInfoForSolidOutletBoundary(dim, idxRecv):
-             * coordMirror = 0 on low sides, L on high sides */
+             * coordMirror = 0 on low sides, L on high sides; open sides
+             * mirror like the reference's outlet (ZJS_MPI_DEM.F90:203-209) */
             double mirror = (idxRecv % 2 == 1) ? 0.0 : Lg[dim];
             for (int q = 0; q < c->nsend[idxRecv - 1]; ++q) {
                 int s = c->send[idxRecv - 1][q];
@@ -522,6 +596,12 @@ static void velocity_guess(ibmdem_ctx *c, double aRK, double bRK, double cRK) {
                 c->Py[o] = (P[o] - P[o - sy]) / (dy) / rhof;
                 c->Pz[o] = (P[o] - P[o - sz]) / (dz) / rhof;
 
+                /* the pressure gradient on an outflow face, for the
+                 * projection of the outflow plane */
+                if (i == c->nx && c->btype[1] == IBMDEM_BC_OUTFLOW) c->Px[o + sx] = (P[o + sx] - P[o]) / (dx) / rhof;
+                if (j == c->ny && c->btype[3] == IBMDEM_BC_OUTFLOW) c->Py[o + sy] = (P[o + sy] - P[o]) / (dy) / rhof;
+                if (k == c->nz && c->btype[5] == IBMDEM_BC_OUTFLOW) c->Pz[o + sz] = (P[o + sz] - P[o]) / (dz) / rhof;
+
                 RHSu[o] = -(TIux + TIuy + TIuz) + (TVux + TVuy + TVuz) + c->cs.cp[0] + c->fx[o];
                 RHSv[o] = -(TIvx + TIvy + TIvz) + (TVvx + TVvy + TVvz) + c->cs.cp[1] + c->fy[o];
                 RHSw[o] = -(TIwx + TIwy + TIwz) + (TVwx + TVwy + TVwz) + c->cs.cp[2] + c->fz[o];
@@ -567,9 +647,14 @@ static void modify_bulk(ibmdem_ctx *c) {
             }
 }
 
-/* VelocityDivergence(c) (fluid.F90:89-105) */
+/* VelocityDivergence(c) (fluid.F90:89-105); outflow planes first take the
+ * last interior face */
 static void velocity_divergence(ibmdem_ctx *c, double cRK) {
     const size_t sx = 1, sy = (size_t)c->px, sz = (size_t)c->px * c->py;
+    int refill = 0;
+    for (int d = 0; d < 3; ++d)
+        if (c->btype[2 * d + 1] == IBMDEM_BC_OUTFLOW) { plane_from_interior(c, d); refill = 1; }
+    if (refill) boundary_vel(c);
     for (int k = 1; k <= c->nz; ++k)
         for (int j = 1; j <= c->ny; ++j)
             for (int i = 1; i <= c->nx; ++i) {
@@ -600,6 +685,19 @@ static void velocity_correction(ibmdem_ctx *c, double cRK) {
                 c->u[o] = c->u[o] + cRK * c->Px[o] * dt - cRK * (P[o] - P[o - sx]) / (c->dx) * dt / rhof;
                 c->v[o] = c->v[o] + cRK * c->Py[o] * dt - cRK * (P[o] - P[o - sy]) / (c->dy) * dt / rhof;
                 c->w[o] = c->w[o] + cRK * c->Pz[o] * dt - cRK * (P[o] - P[o - sz]) / (c->dz) * dt / rhof;
+                /* the outflow planes, by the same formula on face n+1 */
+                if (i == c->nx && c->btype[1] == IBMDEM_BC_OUTFLOW) {
+                    double *q = &c->plane[0][(size_t)(j - 1) + (size_t)c->ny * (k - 1)];
+                    *q = *q + cRK * c->Px[o + sx] * dt - cRK * (P[o + sx] - P[o]) / (c->dx) * dt / rhof;
+                }
+                if (j == c->ny && c->btype[3] == IBMDEM_BC_OUTFLOW) {
+                    double *q = &c->plane[1][(size_t)(i - 1) + (size_t)c->nx * (k - 1)];
+                    *q = *q + cRK * c->Py[o + sy] * dt - cRK * (P[o + sy] - P[o]) / (c->dy) * dt / rhof;
+                }
+                if (k == c->nz && c->btype[5] == IBMDEM_BC_OUTFLOW) {
+                    double *q = &c->plane[2][(size_t)(i - 1) + (size_t)c->nx * (j - 1)];
+                    *q = *q + cRK * c->Pz[o + sz] * dt - cRK * (P[o + sz] - P[o]) / (c->dz) * dt / rhof;
+                }
             }
 }
 
@@ -626,8 +724,8 @@ static inline double nb_val(const mg_level *L, const double *x, int i, int j, in
     int n = d == 0 ? L->nx : d == 1 ? L->ny : L->nz;
     int *t = d == 0 ? &i : d == 1 ? &j : &k;
     int a = *t + (side ? 1 : -1);
-    if (a < 0) { if (L->btype[2 * d] == IBMDEM_BC_WALL) return 0.0; a += n; }
-    else if (a >= n) { if (L->btype[2 * d + 1] == IBMDEM_BC_WALL) return 0.0; a -= n; }
+    if (a < 0) { if (L->btype[2 * d] != IBMDEM_BC_PERIODIC) return 0.0; a += n; }
+    else if (a >= n) { if (L->btype[2 * d + 1] != IBMDEM_BC_PERIODIC) return 0.0; a -= n; }
     *t = a;
     return x[((size_t)k * L->ny + j) * L->nx + i];
 }
@@ -663,8 +761,10 @@ static void mg_setup_level(mg_level *L) {
                 double dsum[3];
                 for (int d = 0; d < 3; ++d) {
                     int cnt = 0;
-                    if (pos[d] > 0 || L->btype[2 * d] != IBMDEM_BC_WALL) cnt++;
-                    if (pos[d] < nn[d] - 1 || L->btype[2 * d + 1] != IBMDEM_BC_WALL) cnt++;
+                    if (pos[d] > 0 || L->btype[2 * d] == IBMDEM_BC_PERIODIC) cnt++;
+                    if (pos[d] < nn[d] - 1 || L->btype[2 * d + 1] == IBMDEM_BC_PERIODIC) cnt++;
+                    /* outflow: ghost = -x, (x_g - x)/h^2 = -2x/h^2 */
+                    if (pos[d] == nn[d] - 1 && L->btype[2 * d + 1] == IBMDEM_BC_OUTFLOW) cnt += 2;
                     dsum[d] = cnt * ih[d];
                 }
                 double diag = dsum[0] + dsum[1] + dsum[2];
@@ -719,12 +819,12 @@ static void restrict_to(const mg_level *F, const double *rf, mg_level *C) {
                             int fk = wrap_or_mirror(2 * K - 1 + c3, F->nz, 0, 0, &ok);
                             /* walls mirror the out-of-range fine index */
                             int ti = 2 * I - 1 + c1, tj = 2 * J - 1 + c2, tk = 2 * K - 1 + c3;
-                            if (ti < 0 && F->btype[0] == IBMDEM_BC_WALL) fi = 0;
-                            if (ti >= F->nx && F->btype[1] == IBMDEM_BC_WALL) fi = F->nx - 1;
-                            if (tj < 0 && F->btype[2] == IBMDEM_BC_WALL) fj = 0;
-                            if (tj >= F->ny && F->btype[3] == IBMDEM_BC_WALL) fj = F->ny - 1;
-                            if (tk < 0 && F->btype[4] == IBMDEM_BC_WALL) fk = 0;
-                            if (tk >= F->nz && F->btype[5] == IBMDEM_BC_WALL) fk = F->nz - 1;
+                            if (ti < 0 && F->btype[0] != IBMDEM_BC_PERIODIC) fi = 0;
+                            if (ti >= F->nx && F->btype[1] != IBMDEM_BC_PERIODIC) fi = F->nx - 1;
+                            if (tj < 0 && F->btype[2] != IBMDEM_BC_PERIODIC) fj = 0;
+                            if (tj >= F->ny && F->btype[3] != IBMDEM_BC_PERIODIC) fj = F->ny - 1;
+                            if (tk < 0 && F->btype[4] != IBMDEM_BC_PERIODIC) fk = 0;
+                            if (tk >= F->nz && F->btype[5] != IBMDEM_BC_PERIODIC) fk = F->nz - 1;
                             double w = wt[c3] * wt[c2] * wt[c1];
                             s = s + w * rf[((size_t)fk * F->ny + fj) * F->nx + fi];
                         }
@@ -743,8 +843,8 @@ static void prolong_add(const mg_level *C, mg_level *F) {
                 for (int d = 0; d < 3; ++d) {
                     int cc = f[d] / 2;
                     int nb = (f[d] & 1) ? cc + 1 : cc - 1;
-                    if (nb < 0) nb = (C->btype[2 * d] == IBMDEM_BC_WALL) ? cc : nb + nc[d];
-                    else if (nb >= nc[d]) nb = (C->btype[2 * d + 1] == IBMDEM_BC_WALL) ? cc : nb - nc[d];
+                    if (nb < 0) nb = (C->btype[2 * d] != IBMDEM_BC_PERIODIC) ? cc : nb + nc[d];
+                    else if (nb >= nc[d]) nb = (C->btype[2 * d + 1] != IBMDEM_BC_PERIODIC) ? cc : nb - nc[d];
                     pp[d][0] = cc; pp[d][1] = nb;
                 }
                 static const double wt[2] = {0.75, 0.25};
@@ -819,10 +919,14 @@ static int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
     for (int k = 1; k <= c->nz; ++k)
         for (int j = 1; j <= c->ny; ++j)
             for (int i = 1; i <= c->nx; ++i) rhs[q++] = c->velDiv[ID(c, i, j, k)];
-    /* pinned row 0 of the reference: b[0] = -sum_{i>0} b[i] */
+    /* pinned row 0 of the reference: b[0] = -sum_{i>0} b[i]; none with an
+     * outflow side (P' = 0 there makes the operator non-singular) */
+    int dirichlet = 0;
+    for (int sd = 0; sd < 6; ++sd)
+        if (c->btype[sd] == IBMDEM_BC_OUTFLOW) dirichlet = 1;
     double s = 0.0;
     for (q = 1; q < n; ++q) s += rhs[q];
-    rhs[0] = -s;
+    if (!dirichlet) rhs[0] = -s;
     (void)b;
     double *x = c->cg_x, *r = c->cg_r, *z = c->cg_z, *p = c->cg_p, *Ap = c->cg_Ap;
     memset(x, 0, n * sizeof(double));
@@ -855,8 +959,8 @@ static int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
         }
         if (it > c->cs.poisson_max_iter) it = c->cs.poisson_max_iter;
     }
-    /* gauge P'(1,1,1) = 0 and scatter into Pcorr */
-    double x0 = x[0];
+    /* gauge P'(1,1,1) = 0 (none with an outflow side) and scatter into Pcorr */
+    double x0 = dirichlet ? 0.0 : x[0];
     q = 0;
     for (int k = 1; k <= c->nz; ++k)
         for (int j = 1; j <= c->ny; ++j)
@@ -912,7 +1016,7 @@ static void contact(ibmdem_ctx *c) {
         if (!(ci == 1 || ci == c->nx || cj == 1 || cj == c->ny || ck == 1 || ck == c->nz)) continue;
         double x1 = XC(c, i, 0), x2 = XC(c, i, 1), x3 = XC(c, i, 2);
         for (int s = 0; s < 6; ++s) {
-            if (c->btype[s] != IBMDEM_BC_WALL) continue;
+            if (c->btype[s] == IBMDEM_BC_PERIODIC) continue;   /* walls, inflow and outflow screens */
             double p1, p2, p3;
             if (s % 2 == 0) { p1 = 0.0; p2 = 0.0; p3 = 0.0; }
             else { p1 = c->L[0]; p2 = c->L[1]; p3 = c->L[2]; }
@@ -1132,8 +1236,15 @@ int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
     if (cs->nx < 2 || cs->ny < 2 || cs->nz < 2 || cs->dx <= 0 || cs->dy <= 0 || cs->dz <= 0 ||
         cs->dt <= 0 || cs->n_markers < 0 || cs->n_bodies < 0 || cs->couple_interval < 1)
         return IBMDEM_ERR_ARG;
-    for (int s = 0; s < 6; ++s)
-        if (cs->btype[s] != IBMDEM_BC_PERIODIC && cs->btype[s] != IBMDEM_BC_WALL) return IBMDEM_ERR_UNSUPPORTED;
+    for (int s = 0; s < 6; ++s) {
+        const int b = cs->btype[s];
+        if (b != IBMDEM_BC_PERIODIC && b != IBMDEM_BC_WALL && b != IBMDEM_BC_INFLOW && b != IBMDEM_BC_OUTFLOW)
+            return IBMDEM_ERR_UNSUPPORTED;
+        if ((b == IBMDEM_BC_INFLOW && (s & 1)) || (b == IBMDEM_BC_OUTFLOW && !(s & 1))) return IBMDEM_ERR_ARG;
+        if ((b == IBMDEM_BC_INFLOW || b == IBMDEM_BC_OUTFLOW) &&
+            (cs->bulk_switch[0] == 1 || cs->bulk_switch[1] == 1 || cs->bulk_switch[2] == 1))
+            return IBMDEM_ERR_UNSUPPORTED;
+    }
     for (int d = 0; d < 3; ++d)
         if ((cs->btype[2 * d] == IBMDEM_BC_PERIODIC) != (cs->btype[2 * d + 1] == IBMDEM_BC_PERIODIC))
             return IBMDEM_ERR_ARG;
@@ -1162,6 +1273,12 @@ int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
                          &c->RHSwl, &c->Px, &c->Py, &c->Pz};
     for (size_t q = 0; q < sizeof fields / sizeof fields[0]; ++q)
         *fields[q] = (double *)xcalloc(c->nbox, sizeof(double));
+    for (int d = 0; d < 3; ++d) {
+        c->uin[d] = cs->btype[2 * d] == IBMDEM_BC_INFLOW ? cs->inflow[d] : 0.0;
+        const size_t pn = d == 0 ? (size_t)c->ny * c->nz : d == 1 ? (size_t)c->nx * c->nz : (size_t)c->nx * c->ny;
+        c->plane[d] = cs->btype[2 * d + 1] == IBMDEM_BC_OUTFLOW ? (double *)xcalloc(pn, sizeof(double)) : NULL;
+    }
+    fill_ghost_fluid(c);
     c->M = cs->n_markers; c->B = cs->n_bodies;
     c->cap = 8 * c->M + 64;
     c->x = (double *)xcalloc(3 * (size_t)c->cap, sizeof(double));
@@ -1211,6 +1328,7 @@ int ibmdem_destroy(ibmdem_ctx *c) {
                     c->bfmomhydro, c->brot, c->brotg, c->bI, c->bm, c->brho, c->cg_x, c->cg_r, c->cg_z,
                     c->cg_p, c->cg_Ap};
     for (size_t q = 0; q < sizeof fl / sizeof fl[0]; ++q) free(fl[q]);
+    for (int d = 0; d < 3; ++d) free(c->plane[d]);
     free(c->body); free(c->ptype); free(c->cell); free(c->img); free(c->cell_start);
     free(c->cell_items); free(c->npairs); free(c->pairs); free(c->bfixed);
     for (int s = 0; s < 6; ++s) free(c->send[s]);
@@ -1243,6 +1361,31 @@ int ibmdem_upload_fields(ibmdem_ctx *c, const double *u, const double *v, const 
     if (v) copy_in(c, c->v, v);
     if (w) copy_in(c, c->w, w);
     if (p) copy_in(c, c->P, p);
+    /* an uploaded normal component resets its outflow plane */
+    const double *up[3] = {u, v, w};
+    for (int d = 0; d < 3; ++d)
+        if (up[d] && c->btype[2 * d + 1] == IBMDEM_BC_OUTFLOW) plane_from_interior(c, d);
+    fill_ghost_fluid(c);
+    return IBMDEM_OK;
+}
+static size_t outflow_size(const ibmdem_ctx *c, int side) {
+    if (side != 1 && side != 3 && side != 5) return 0;
+    if (c->btype[side] != IBMDEM_BC_OUTFLOW) return 0;
+    const int d = side / 2;
+    return d == 0 ? (size_t)c->ny * c->nz : d == 1 ? (size_t)c->nx * c->nz : (size_t)c->nx * c->ny;
+}
+int ibmdem_download_outflow(ibmdem_ctx *c, int32_t side, double *plane) {
+    if (!c) return IBMDEM_ERR_ARG;
+    const size_t n = outflow_size(c, side);
+    if (!n || !plane) return set_err(c, IBMDEM_ERR_ARG, "not an outflow side (1, 3 or 5 of type OUTFLOW)");
+    memcpy(plane, c->plane[side / 2], n * sizeof(double));
+    return IBMDEM_OK;
+}
+int ibmdem_upload_outflow(ibmdem_ctx *c, int32_t side, const double *plane) {
+    if (!c) return IBMDEM_ERR_ARG;
+    const size_t n = outflow_size(c, side);
+    if (!n || !plane) return set_err(c, IBMDEM_ERR_ARG, "not an outflow side (1, 3 or 5 of type OUTFLOW)");
+    memcpy(c->plane[side / 2], plane, n * sizeof(double));
     fill_ghost_fluid(c);
     return IBMDEM_OK;
 }

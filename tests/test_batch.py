@@ -15,7 +15,12 @@ from ibmdem import abi, cases
 def _cases():
     return [cases.small_periodic(gap=0.75),
             cases.small_walls(),
-            cases.small_periodic(n=(40, 32, 24), n_spheres=4, D=7.0, seed=cases.SEED_BASE + 21, gap=1.0)]
+            cases.small_periodic(n=(40, 32, 24), n_spheres=4, D=7.0, seed=cases.SEED_BASE + 21, gap=1.0),
+            cases.small_open(axis=2)]
+
+
+def _planes(s):
+    return [s.download_outflow(sd) for sd in (1, 3, 5) if s.case.btype[sd] == abi.BC_OUTFLOW]
 
 
 def _single(lib, setup, n):
@@ -32,10 +37,10 @@ def _batched(lib, setups, n, tune=0):
         for q, s in enumerate(setups):
             cases.load(b[q], s)
         if tune:
-            before = [b[q].download_fields() for q in range(len(b))]
+            before = [b[q].download_fields() + _planes(b[q]) for q in range(len(b))]
             b.autotune(tune)
             for q in range(len(b)):
-                for x, y in zip(before[q], b[q].download_fields()):
+                for x, y in zip(before[q], b[q].download_fields() + _planes(b[q])):
                     np.testing.assert_array_equal(x, y)
         infos = b.run(n)
         return [(b[q].download_fields(), b[q].download_markers().x, b[q].download_bodies().x,

@@ -82,11 +82,17 @@ def test_bind_c_sizes_match_c(oracle_program):
                      C.sizeof(abi.Bodies), C.sizeof(abi.SolveInfo)]
 
 
+def _named(name):
+    return {"periodic": lambda: cases.small_periodic(gap=0.75),
+            "walls": lambda: cases.small_walls(),
+            "open": lambda: cases.small_open(axis=2)}[name]()
+
+
 @needs_flang
-@pytest.mark.parametrize("name", ["periodic", "walls"])
+@pytest.mark.parametrize("name", ["periodic", "walls", "open"])
 @pytest.mark.parametrize("mode", ["ops", "step"])
 def test_fortran_driver_matches_whole_step(oracle_program, tmp_path, name, mode):
-    setup = (cases.small_periodic(gap=0.75) if name == "periodic" else cases.small_walls())
+    setup = _named(name)
     got = _run(oracle_program, setup, tmp_path, 2, mode)
     want = _python_run(oracle.library(), setup, 2)
     _assert_same_state(got, want)
@@ -132,9 +138,11 @@ def test_snapshot_round_trip(tmp_path):
     assert (info2.iterations, info2.converged, info2.rel_residual, info2.rhs_norm) == (7, 1, 1.5e-11, 2.0)
 
 
-def test_snapshot_resume_is_bitwise(tmp_path, oracle_lib):
-    """2 steps == 1 step, snapshot, restore in a fresh context, 1 step."""
-    setup = cases.small_periodic(gap=0.75)
+@pytest.mark.parametrize("name", ["periodic", "open"])
+def test_snapshot_resume_is_bitwise(tmp_path, oracle_lib, name):
+    """2 steps == 1 step, snapshot, restore in a fresh context, 1 step (an
+    open case needs its outflow plane from the snapshot)."""
+    setup = _named(name)
     want = _python_run(oracle_lib, setup, 2)
     sim = abi.Simulation(oracle_lib, setup.case)
     cases.load(sim, setup)
@@ -142,11 +150,9 @@ def test_snapshot_resume_is_bitwise(tmp_path, oracle_lib):
     p = str(tmp_path / "mid.snap")
     snapshot.write_simulation(p, sim, info)
     sim.close()
-    case, mk, bd, f, _ = snapshot.read(p)
+    case = snapshot.read(p)[0]
     sim = abi.Simulation(oracle_lib, case)
-    sim.upload_bodies(bd)
-    sim.upload_markers(mk)
-    sim.upload_fields(*f)
+    snapshot.load(sim, p)
     info = sim.step()
     got = (None, sim.download_markers(), sim.download_bodies(), sim.download_fields(), info)
     sim.close()
