@@ -23,6 +23,7 @@ interpolation 136 * markers + 8 * 3 * footprint cells; predictor 7 * s *
 cells. The CPU baseline is the oracle (one core) timed on a bounded sample.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--precision fp64|mixed|both]
+                [--batched] [--configs]
 """
 from __future__ import annotations
 
@@ -49,6 +50,10 @@ def parse():
     ap.add_argument("--spheres", type=int, default=2000)
     ap.add_argument("--batched", action="store_true",
                     help="also time the batched form (64 uneven settling cases)")
+    ap.add_argument("--configs", action="store_true",
+                    help="also time the open-boundary configurations (fluidized bed "
+                         "192x192x768 with 10,000 spheres; cylinder in crossflow 512x256x64) "
+                         "in FP64 and mixed precision")
     return ap.parse_args()
 
 
@@ -200,6 +205,76 @@ def batched_run(steps, warmup):
     return out
 
 
+def config_runs(steps, warmup):
+    """The open-boundary configurations, each in FP64 and mixed precision
+    from the same initial state: steps/s, Gcell/s, and the mixed form's
+    error against FP64 after the run (velocity, pressure, divergence per
+    cell relative to the velocity scale, body positions; for the cylinder,
+    its drag and lift coefficients in both forms)."""
+    import numpy as np
+
+    import ibmdem
+    from ibmdem import cases, diagnostics
+    from ibmdem.abi import BC_OUTFLOW, FP64, MIXED
+
+    out = {}
+    for name, make in (("fluidized_bed", cases.config_fluidized_bed), ("cylinder", cases.config_cylinder)):
+        t_gen = time.perf_counter()
+        setup = make(precision=FP64)
+        t_gen = time.perf_counter() - t_gen
+        print(f"[bench] {name}: {setup.cells} cells, {setup.case.n_markers} markers, "
+              f"generated in {t_gen:.1f} s", file=sys.stderr, flush=True)
+        res = {}
+        for prec_name, prec in (("fp64", FP64), ("mixed", MIXED)):
+            setup.case.precision = prec
+            with ibmdem.simulation(setup.case) as sim:
+                cases.load(sim, setup)
+                for _ in range(warmup):
+                    sim.step()
+                sim.sync()
+                t0 = time.perf_counter()
+                iters = []
+                for _ in range(steps):
+                    iters.append(sim.step().iterations)
+                sim.sync()
+                secs = time.perf_counter() - t0
+                planes = {sd: sim.download_outflow(sd) for sd in (1, 3, 5)
+                          if setup.case.btype[sd] == BC_OUTFLOW}
+                res[prec_name] = dict(secs=secs, iters=iters, fields=sim.download_fields(),
+                                      bodies=sim.download_bodies(), planes=planes)
+            print(f"[bench] {name} {prec_name}: {steps} steps in {secs:.3f} s", file=sys.stderr, flush=True)
+        f64, mx = res["fp64"], res["mixed"]
+        vscale = max(float(np.abs(q).max()) for q in f64["fields"][:3])
+        h = setup.case.dx
+
+        def div_max(r):
+            return float(np.abs(diagnostics.divergence(setup.case, r["fields"][:3], r["planes"])).max() * h / vscale)
+
+        row = {
+            "cells": setup.cells, "markers": int(setup.case.n_markers), "bodies": int(setup.case.n_bodies),
+            "dt": setup.case.dt, "steps": steps, "warmup": warmup,
+            "meta": {k: v for k, v in setup.meta.items() if isinstance(v, (int, float))},
+        }
+        for prec_name in ("fp64", "mixed"):
+            r = res[prec_name]
+            row[prec_name] = {"steps_per_s": steps / r["secs"], "gcell_per_s": setup.cells * steps / r["secs"] / 1e9,
+                              "cg_iterations_mean": float(np.mean(r["iters"])), "divergence": div_max(r)}
+        row["mixed_speedup"] = row["mixed"]["steps_per_s"] / row["fp64"]["steps_per_s"]
+        row["mixed_error_vs_fp64"] = {
+            "velocity": max(float(np.abs(a - b).max()) for a, b in zip(mx["fields"][:3], f64["fields"][:3])) / vscale,
+            "pressure": float(np.abs(mx["fields"][3] - f64["fields"][3]).max() / max(np.abs(f64["fields"][3]).max(), 1e-300)),
+            "body_x": float(np.abs(mx["bodies"].x - f64["bodies"].x).max() / np.abs(f64["bodies"].x).max()),
+        }
+        if name == "cylinder":
+            b = setup.meta["cylinder_body"]
+            m = setup.meta
+            row["force_coefficients"] = {
+                p: dict(zip(("C_D", "C_L"), diagnostics.force_coefficients(res[p]["bodies"], b, 1.0, m["U"], m["D"], m["span"])))
+                for p in ("fp64", "mixed")}
+        out[name] = row
+    return out
+
+
 def cpu_baseline():
     """The oracle (one core) on a bounded sample: one coupled step of a
     128^3 sub-box holding 250 spheres (the workload's solid fraction), taken
@@ -304,6 +379,8 @@ def main():
         }
     if args.batched:
         line["batched"] = batched_run(args.steps, args.warmup)
+    if args.configs:
+        line["configs"] = config_runs(args.steps, args.warmup)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline()

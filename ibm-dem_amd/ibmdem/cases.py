@@ -22,8 +22,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .abi import (BC_INFLOW, BC_OUTFLOW, BC_PERIODIC, BC_WALL, FP64, MIXED, BodyArrays, Case,
-                  MarkerArrays)
+from .abi import (BC_INFLOW, BC_OUTFLOW, BC_PERIODIC, BC_WALL, BODY_MAT, FP64, MIXED, BodyArrays,
+                  Case, MarkerArrays)
 
 PI_F32 = float(np.float32(3.1415926536))
 SEED_BASE = 20260000
@@ -241,6 +241,145 @@ def config_suspension(n_spheres=2000, n=256, precision=FP64, rtol=1e-10,
     return Setup(f"suspension-{n}^3-{n_spheres}x{int(D)}h", case, mk, bd,
                  zero_fields(case), {"steps": 500, "solid_fraction": phi,
                                      "seed": seed})
+
+
+def fill_upward(rng, diameters, box, gap=1.0, wall_gap=1.0, slab=24.0, batch=256, rise=1.0):
+    """Seeded bottom-up random sequential addition in a walled box: each
+    sphere (in the given order) takes the lowest of `batch` random candidate
+    positions in the slab [zcap - slab, zcap] that keeps a surface gap
+    >= `gap` to every placed sphere and `wall_gap` to the walls; the slab
+    rises by `rise` whenever no candidate fits. Returns (centres (n, 3),
+    fill height = highest sphere top)."""
+    L = np.asarray(box, dtype=np.float64)
+    n = len(diameters)
+    cen = np.zeros((n, 3))
+    rad = 0.5 * np.asarray(diameters, dtype=np.float64)
+    rmax = rad.max()
+    zcap = 2.0 * rmax + wall_gap + slab
+    for q in range(n):
+        R = rad[q]
+        lo = np.array([R + wall_gap, R + wall_gap, R + wall_gap])
+        hi_xy = L[:2] - R - wall_gap
+        while True:
+            zlo = max(lo[2], zcap - slab)
+            if zlo + R + wall_gap > L[2]:
+                raise RuntimeError(f"fill_upward placed only {q} of {n} spheres")
+            c = np.empty((batch, 3))
+            c[:, :2] = lo[:2] + rng.random((batch, 2)) * (hi_xy - lo[:2])
+            c[:, 2] = zlo + rng.random(batch) * (min(zcap, L[2] - R - wall_gap) - zlo)
+            ok = c[:, 2] + R + wall_gap <= L[2]
+            if q:
+                near = np.nonzero(cen[:q, 2] >= zlo - R - rmax - gap)[0]
+                if near.size:
+                    d = c[:, None, :] - cen[near][None, :, :]
+                    reach = rad[near][None, :] + R + gap
+                    ok &= np.all(np.einsum("ijk,ijk->ij", d, d) >= reach * reach, axis=1)
+            if ok.any():
+                k = np.nonzero(ok)[0]
+                cen[q] = c[k[np.argmin(c[k, 2])]]
+                break
+            zcap += rise
+    return cen, float(np.max(cen[:, 2] + rad))
+
+
+def config_fluidized_bed(precision=MIXED, rtol=1e-10, seed=SEED_BASE + 3, n_spheres=10000,
+                         n=(192, 192, 768)) -> Setup:
+    """Polydisperse fluidized-bed channel: 192x192x768, inflow w = U_in at
+    z = 0, outflow at the top, no-slip side walls; 10,000 spheres with
+    ln D ~ N(ln 10, 0.25^2) clipped to [8h, 16h], placed bottom-up
+    (fill_upward; the fill height is recorded: random addition cannot put
+    them in the lower 40 %; surface gaps >= 1h, so marker shells start
+    outside the contact distance), rho_p/rho_f = 2.5, about one marker per
+    h^2.
+    Re_g = 40 at D = 10h with nu = 0.04 fixes g; U_in = U_ref =
+    sqrt((rho_r - 1) g D) = 0.16; knCoeff = 1e5, e = 0.9. The fluid starts as
+    the uniform upflow w = U_in."""
+    nx, ny, nz = n
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nu, rr, Dref = 0.04, 2.5, 10.0
+    g = (40.0 * nu) ** 2 / ((rr - 1.0) * Dref ** 3)
+    u_ref = math.sqrt((rr - 1.0) * g * Dref)
+    diam = np.clip(np.exp(rng.normal(math.log(10.0), 0.25, size=n_spheres)), 8.0, 16.0)
+    cen, height = fill_upward(rng, diam, (nx, ny, nz))
+    bt = [BC_WALL, BC_WALL, BC_WALL, BC_WALL, BC_INFLOW, BC_OUTFLOW]
+    case = build_case(nx, ny, nz, bt, visc=nu, gravity=(0.0, 0.0, -g), kn_coeff=1e5, restitution=0.9,
+                      u_max=2 * u_ref, precision=precision, rtol=rtol, inflow=(0.0, 0.0, u_ref))
+    mk, bd = place_spheres(case, [Sphere(tuple(c), float(D), rr) for c, D in zip(cen, diam)])
+    u, v, w, p = zero_fields(case)
+    w[:] = u_ref
+    phi = float(np.sum(np.pi * diam ** 3 / 6) / (nx * ny * height))
+    return Setup(f"fluidized-bed-{nx}x{ny}x{nz}-{n_spheres}", case, mk, bd, (u, v, w, p),
+                 {"steps": 300, "seed": seed, "fill_height": height, "fill_fraction": phi,
+                  "u_in": u_ref, "g": g})
+
+
+def cylinder_markers(centre, D, span, n_around, n_span, h=1.0):
+    """Structured circumferential x spanwise lattice on a cylinder (axis
+    along z) retracted by 0.3h like the sphere template; each marker carries
+    the area it represents times h as its volume."""
+    rs = 0.5 * D - 0.3 * h
+    th = 2.0 * np.pi * (np.arange(n_around) + 0.5) / n_around
+    zz = (np.arange(n_span) + 0.5) * span / n_span
+    T, Z = np.meshgrid(th, zz, indexing="ij")
+    x = np.stack([centre[0] + rs * np.cos(T).ravel(), centre[1] + rs * np.sin(T).ravel(), Z.ravel()])
+    vol = 2.0 * np.pi * rs * span / (n_around * n_span) * h
+    return x, vol
+
+
+def config_cylinder(precision=MIXED, rtol=1e-10, seed=SEED_BASE + 4, n_free=200) -> Setup:
+    """Fixed spanwise-extruded cylinder in crossflow: 512x256x64, uniform
+    inflow u = 1 at x = 0, outflow at x = 512, periodic in y and in the span
+    z. Cylinder D = 32h on the z axis through (128, 128), fixed, 2,295
+    markers on a 51 x 45 circumferential x spanwise lattice (1.9h x 1.4h).
+    Re = U D / nu = 200 -> nu = 0.16; forward Euler needs dt = 0.072
+    (advective FTCS limit at U_max = 2). 200 free spheres of D = 8h
+    (201 markers, rho_p/rho_f = 2) by seeded random addition with centres in
+    x in [16, 96]; knCoeff = 1e5, e = 0.9, no gravity."""
+    nx, ny, nz = 512, 256, 64
+    rng = np.random.Generator(np.random.PCG64(seed))
+    D, U, Re = 32.0, 1.0, 200.0
+    nu = U * D / Re
+    bt = [BC_INFLOW, BC_OUTFLOW, BC_PERIODIC, BC_PERIODIC, BC_PERIODIC, BC_PERIODIC]
+    case = build_case(nx, ny, nz, bt, visc=nu, kn_coeff=1e5, restitution=0.9, u_max=2 * U,
+                      precision=precision, rtol=rtol, inflow=(U, 0.0, 0.0))
+    free = rsa_centres(rng, n_free, 8.0, (112.0, ny, nz), 2.0, [False, True, True], margin=16.0)
+    mk, bd = place_spheres(case, [Sphere(tuple(c), 8.0, 2.0) for c in free])
+    # the cylinder is body n_free, after the free spheres' markers
+    cx, cvol = cylinder_markers((128.0, 128.0), D, float(nz), 51, 45)
+    nc = cx.shape[1]
+    m0, b0 = mk.x.shape[1], bd.mass.size
+    mk2, bd2 = MarkerArrays.empty(m0 + nc), BodyArrays.empty(b0 + 1)
+    for k in mk.__dataclass_fields__:
+        getattr(mk2, k)[..., :m0] = getattr(mk, k)          # (3, M) or (M,)
+    for k in bd.__dataclass_fields__:
+        a, b = getattr(bd, k), getattr(bd2, k)
+        if k in BODY_MAT:
+            b[:b0] = a                                        # (B, 9)
+        else:
+            b[..., :b0] = a                                   # (3, B) or (B,)
+    rho_c = 2.0
+    mass = rho_c * math.pi * (0.5 * D) ** 2 * nz
+    mk2.x[:, m0:] = cx
+    mk2.vol[m0:] = cvol
+    mk2.mass[m0:] = rho_c * cvol
+    mk2.radeff[m0:] = 0.75
+    mk2.body[m0:] = b0
+    bd2.x[:, b0] = (128.0, 128.0, 0.5 * nz)
+    bd2.mass[b0] = mass
+    bd2.rho[b0] = rho_c
+    Ixx = mass * (3 * (0.5 * D) ** 2 + nz ** 2) / 12.0
+    Izz = 0.5 * mass * (0.5 * D) ** 2
+    bd2.inertia[b0] = [Ixx, 0, 0, 0, Ixx, 0, 0, 0, Izz]
+    bd2.rot[b0] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    bd2.rot_glo[b0] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    bd2.fixed[b0] = 1
+    case.n_markers = m0 + nc
+    case.n_bodies = b0 + 1
+    u, v, w, p = zero_fields(case)
+    u[:] = U
+    return Setup(f"cylinder-crossflow-{nx}x{ny}x{nz}", case, mk2, bd2, (u, v, w, p),
+                 {"steps": 2000, "seed": seed, "Re": Re, "D": D, "U": U, "span": float(nz),
+                  "cylinder_body": b0, "cylinder_markers": nc})
 
 
 def config_batch(n_cases=64, precision=MIXED, seed=SEED_BASE + 5, rtol=1e-10,

@@ -11,13 +11,15 @@ The formats follow the reference's post-processing:
     the reference divides the torque by I, which is 0 for its cylinders
     (fluid.F90:396, ZJS_INIT_DEM.F90:427); the torque itself is written here;
   - force coefficients: C = 2 F / (rho_f U^2 D L) from the assembled
-    hydrodynamic force of a body (AssembleLocalForceMS, ZJS_MS_DEM.F90:494-519).
+    hydrodynamic force of a body (AssembleLocalForceMS, ZJS_MS_DEM.F90:494-519);
+  - the discrete divergence of a velocity state (VelocityDivergence's
+    stencil, fluid.F90:97-100), outflow planes included.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from .abi import BodyArrays, Case
+from .abi import BC_OUTFLOW, BC_PERIODIC, BodyArrays, Case
 
 
 def force_coefficients(bodies: BodyArrays, body: int, rho: float, U: float, D: float, L: float,
@@ -29,6 +31,28 @@ def force_coefficients(bodies: BodyArrays, body: int, rho: float, U: float, D: f
         raise ValueError("rho, U, D and L must be positive")
     F = bodies.fdrag[:, body]
     return float(F[drag_axis] / q), float(F[lift_axis] / q)
+
+
+def divergence(case: Case, fields, planes=None):
+    """Cell divergence of the staggered velocity (u, v, w of shape
+    (nz, ny, nx)) with the boundary rules: the high face of a dimension is
+    the wrapped first face (periodic), zero (wall) or the outflow plane
+    (planes[side] for side 1, 3, 5; the array Simulation.download_outflow
+    returns)."""
+    h = (case.dx, case.dy, case.dz)
+    d = 0.0
+    for c in range(3):
+        a = np.asarray(fields[c], dtype=np.float64)
+        na = 2 - c
+        side = 2 * c + 1
+        if case.btype[side] == BC_OUTFLOW:
+            hi = np.expand_dims(np.asarray(planes[side], dtype=np.float64), na)
+        elif case.btype[side] == BC_PERIODIC:
+            hi = np.take(a, [0], axis=na)
+        else:
+            hi = np.zeros_like(np.take(a, [0], axis=na))
+        d = d + (np.concatenate([np.take(a, range(1, a.shape[na]), axis=na), hi], axis=na) - a) / h[c]
+    return d
 
 
 def probe_line(step: int, bodies: BodyArrays, body: int, rhof: float) -> str:

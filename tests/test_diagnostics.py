@@ -56,3 +56,21 @@ def test_tecplot_round_trip(tmp_path):
     head = open(p).read().splitlines()[:2]
     assert head[0].strip() == "variables=x,y,z,u,v,w,vel,P"
     assert head[1].split() == ["zone", "i=", "6", "j=", "5", "k=", "4"]
+
+
+def test_divergence_matches_the_library_rhs():
+    """diagnostics.divergence equals the library's own divergence (the
+    Poisson right-hand side is -div/dt*rhof with c = 1) on an open case."""
+    import oracle
+    from ibmdem.abi import Simulation
+    st = cases.small_open(axis=0, periodic=(2,))
+    with Simulation(oracle.library(), st.case) as s:
+        cases.load(s, st)
+        s.begin_step()
+        s.predictor(1.0, 0.0, 1.0)
+        s.divergence(1.0)
+        _, rhs = s.download_poisson()
+        f = s.download_fields()
+        planes = {1: s.download_outflow(1)}
+    d = diagnostics.divergence(st.case, f[:3], planes)
+    np.testing.assert_allclose(-d / st.case.dt * st.case.rhof, rhs, rtol=1e-12, atol=1e-15)
