@@ -65,6 +65,7 @@ int validate(const ibmdem_case *cs) {
             return IBMDEM_ERR_ARG;
     if (cs->precision != IBMDEM_FP64 && cs->precision != IBMDEM_MIXED) return IBMDEM_ERR_ARG;
     if (cs->time_scheme != IBMDEM_SCHEME_EULER && cs->time_scheme != IBMDEM_SCHEME_RK3) return IBMDEM_ERR_ARG;
+    if (cs->stencil_tile < 0 || cs->stencil_tile > 3) return IBMDEM_ERR_ARG;
     if ((long)cs->nx * cs->ny * cs->nz >= (1L << 31)) return IBMDEM_ERR_ARG;
     if (cs->n_markers > 0 && cs->n_bodies == 0) return IBMDEM_ERR_ARG;
     // a marker's 5x5x5 neighbourhood must not wrap onto itself, and marker
@@ -86,6 +87,7 @@ int alloc_all(ibmdem_ctx *c) {
             if (c->g.bt[2 * d + 1] == IBMDEM_BC_OUTFLOW) A(c->g.plane[d], pn[d]);
     }
     A(c->cg_x, n); A(c->cg_r, n); A(c->cg_z, n); A(c->cg_p, n); A(c->cg_Ap, n);
+    // bulk-forcing partials: one per 64 x 4 row group and plane
     const long pred_blocks = (long)div_up(c->g.nx, 64) * div_up(c->g.ny, 4) * c->g.nz;
     c->max_partial = (int)std::max<long>(4096, 3 * pred_blocks + 16);
     A(c->partial, c->max_partial);
@@ -365,6 +367,11 @@ int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
     rc = alloc_all(c);
     // inflow faces hold the inflow velocity from the start
     if (rc == IBMDEM_OK) rc = fluid_upload_bc(c, 7);
+    // the bin order is a permutation of the markers from the start (the
+    // interpolation walks markers in it)
+    std::vector<int> iota(c->M);
+    for (int m = 0; m < c->M; ++m) iota[m] = m;
+    if (rc == IBMDEM_OK) rc = h2d(c, c->bin_items, iota.data(), (size_t)c->M * sizeof(int));
     if (rc == IBMDEM_OK) rc = finish(c);
     if (rc) {
         ibmdem_destroy(c);

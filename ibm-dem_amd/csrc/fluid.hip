@@ -127,18 +127,28 @@ __device__ __forceinline__ void predictor_cell(const LU &u, const LV &v, const L
     Pg[0] = Px; Pg[1] = Py; Pg[2] = Pz;
 }
 
-__device__ __forceinline__ void block_sum3(double s[3], double *part) {
-    __shared__ double red[3][BX * BY];
-    const int t = threadIdx.x + BX * threadIdx.y;
-    for (int q = 0; q < 3; ++q) red[q][t] = s[q];
-    __syncthreads();
-    for (int st = BX * BY / 2; st > 0; st >>= 1) {
-        if (t < st)
-            for (int q = 0; q < 3; ++q) red[q][t] += red[q][t + st];
-        __syncthreads();
-    }
-    if (t == 0)
-        for (int q = 0; q < 3; ++q) part[q] = red[q][0];
+// Bulk-forcing partial sums with a fixed association, whatever the
+// predictor schedule: each 64-cell x row is summed by a butterfly over its
+// wavefront (lane 0's value), and each group of four rows (j = 4g+1..4g+4 at
+// one plane) by ((r0 + r1) + (r2 + r3)); partial[3 q + c], q = ((k-1) *
+// ceil(ny/4) + g) * ceil(nx/64) + i-tile.
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+// rows of this thread's wave into red[row][c] (row = threadIdx.y)
+__device__ __forceinline__ void row_sums(const double s[3], double (*red)[3]) {
+    double r[3];
+    for (int q = 0; q < 3; ++q) r[q] = wave_sum64(s[q]);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) red[threadIdx.y][q] = r[q];
+}
+__device__ __forceinline__ void group_partial(double (*red)[3], int g0, double *part) {
+    for (int q = 0; q < 3; ++q) part[q] = (red[g0][q] + red[g0 + 1][q]) + (red[g0 + 2][q] + red[g0 + 3][q]);
+}
+__device__ __forceinline__ long group_index(const GridDev &g, int bx, int jg, int k) {
+    return ((long)(k - 1) * ((g.ny + 3) / 4) + jg) * ((g.nx + BX - 1) / BX) + bx;
 }
 
 // VelocityGuess fused form: u* = u~ + dt*(a*RHS_noF(u~) - c*gradP_old).
@@ -176,8 +186,10 @@ __global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *_
         s[0] = nu_; s[1] = nv_; s[2] = nw_;
     }
     if (BULK) {
-        const long blk = (long)bx + (long)gridDim.x * (by + (long)gridDim.y * bz);
-        block_sum3(s, partial + 3 * blk);
+        __shared__ double red[BY][3];
+        row_sums(s, red);
+        __syncthreads();
+        if (threadIdx.x == 0 && threadIdx.y == 0) group_partial(red, 0, partial + 3 * group_index(g, bx, by, bz + 1));
     }
 }
 
@@ -226,8 +238,134 @@ __global__ __launch_bounds__(BX *BY) void k_predictor_general(GridDev g, const d
         }
     }
     if (BULK) {
-        const long blk = (long)bx + (long)gridDim.x * (by + (long)gridDim.y * bz);
-        block_sum3(s, partial + 3 * blk);
+        __shared__ double red[BY][3];
+        row_sums(s, red);
+        __syncthreads();
+        if (threadIdx.x == 0 && threadIdx.y == 0) group_partial(red, 0, partial + 3 * group_index(g, bx, by, bz + 1));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS z-marching predictor. A block of 64 x TY threads owns a 64 x TY cell
+// column tile and marches it through KZ planes. The three planes k-1, k,
+// k+1 of u, v, w (with a one-cell x/y halo) and plane k of P sit in LDS in
+// rotating slots, each element loaded once per block through the ghost
+// rules (vel_at / scal_at: the same values the direct kernel's taps read),
+// so every stencil tap is an LDS read and each field element crosses HBM
+// about (66 (TY+2) / (64 TY)) (KZ+2)/KZ times instead of the direct
+// kernel's plane re-reads. The cell arithmetic is predictor_cell's, so the
+// result is the direct kernel's bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int LX = BX + 2;   // tile row with halo
+
+template <typename LT, int TY>
+struct LdsPlanes {
+    LT v[3][3][TY + 2][LX];   // [component][slot][y][x]
+    LT p[TY + 2][LX];         // P, plane k
+};
+
+template <typename LT, int TY, int C>
+struct LTap {
+    const LdsPlanes<LT, TY> *L;
+    int sm, s0, sp, tx, ty;   // slots of planes k-1, k, k+1
+    __device__ __forceinline__ LT operator()(int di, int dj, int dk) const {
+        const int sl = dk < 0 ? sm : (dk > 0 ? sp : s0);
+        return L->v[C][sl][ty + 1 + dj][tx + 1 + di];
+    }
+};
+template <typename LT, int TY>
+struct LTapP {
+    const LdsPlanes<LT, TY> *L;
+    int tx, ty;
+    LT below;                  // P(i, j, k-1)
+    __device__ __forceinline__ LT operator()(int di, int dj, int dk) const {
+        return dk < 0 ? below : L->p[ty + 1 + dj][tx + 1 + di];
+    }
+};
+
+// plane k of component C (or of P when C == 3) into an LDS slot; elements
+// beyond the halo of the valid cells are not needed and not loaded
+template <typename LT, int TY, int C, bool OPEN>
+__device__ __forceinline__ void load_plane(LT (*dst)[LX], const double *__restrict__ f, const GridDev &g, int i0,
+                                           int j0, int k) {
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    for (int e = tid; e < (TY + 2) * LX; e += BX * TY) {
+        const int ly = e / LX, lx = e - ly * LX;
+        const int i = i0 - 1 + lx, j = j0 - 1 + ly;
+        LT val = LT(0);
+        if (i <= g.nx + 1 && j <= g.ny + 1) {
+            if (C == 3) val = LT(scal_at(f, g, i, j, k));
+            else val = LT(vel_at<C == 3 ? 0 : C, OPEN>(f, g, i, j, k));
+        }
+        dst[ly][lx] = val;
+    }
+}
+
+template <typename CT, bool BULK, bool P2, bool OPEN, int TY, bool GEN>
+__global__ __launch_bounds__(BX *TY) void k_predictor_lds(GridDev g, const double *__restrict__ u,
+                                                          const double *__restrict__ v,
+                                                          const double *__restrict__ w,
+                                                          const double *__restrict__ P, double *__restrict__ un,
+                                                          double *__restrict__ vn, double *__restrict__ wn,
+                                                          PredParams pp, GeneralArgs ga, int kz,
+                                                          double *__restrict__ partial) {
+    __shared__ LdsPlanes<CT, TY> L;
+    __shared__ double red[TY][3];
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = bx * BX + 1, j0 = by * TY + 1;
+    const int i = i0 + tx, j = j0 + ty;
+    const int k0 = bz * kz + 1, k1 = min(g.nz, k0 + kz - 1);
+    const bool mine = i <= g.nx && j <= g.ny;
+    // planes k0-1 and k0 of u, v, w
+    load_plane<CT, TY, 0, OPEN>(L.v[0][(k0 - 1) % 3], u, g, i0, j0, k0 - 1);
+    load_plane<CT, TY, 1, OPEN>(L.v[1][(k0 - 1) % 3], v, g, i0, j0, k0 - 1);
+    load_plane<CT, TY, 2, OPEN>(L.v[2][(k0 - 1) % 3], w, g, i0, j0, k0 - 1);
+    load_plane<CT, TY, 0, OPEN>(L.v[0][k0 % 3], u, g, i0, j0, k0);
+    load_plane<CT, TY, 1, OPEN>(L.v[1][k0 % 3], v, g, i0, j0, k0);
+    load_plane<CT, TY, 2, OPEN>(L.v[2][k0 % 3], w, g, i0, j0, k0);
+    CT below = mine ? CT(scal_at(P, g, i, j, k0 - 1)) : CT(0);
+    for (int k = k0; k <= k1; ++k) {
+        const int sm = (k - 1) % 3, s0 = k % 3, sp = (k + 1) % 3;
+        load_plane<CT, TY, 0, OPEN>(L.v[0][sp], u, g, i0, j0, k + 1);
+        load_plane<CT, TY, 1, OPEN>(L.v[1][sp], v, g, i0, j0, k + 1);
+        load_plane<CT, TY, 2, OPEN>(L.v[2][sp], w, g, i0, j0, k + 1);
+        load_plane<CT, TY, 3, OPEN>(L.p, P, g, i0, j0, k);
+        __syncthreads();
+        double s[3] = {0.0, 0.0, 0.0};
+        if (mine) {
+            const long o = cidx(g, i, j, k);
+            CT R[3], Pg[3];
+            LTapP<CT, TY> tp{&L, tx, ty, below};
+            predictor_cell<CT, P2>(LTap<CT, TY, 0>{&L, sm, s0, sp, tx, ty}, LTap<CT, TY, 1>{&L, sm, s0, sp, tx, ty},
+                                   LTap<CT, TY, 2>{&L, sm, s0, sp, tx, ty}, tp, g, pp, R, Pg);
+            below = L.p[ty + 1][tx + 1];
+            const bool face[3] = {wall_face<0>(g, i, j, k), wall_face<1>(g, i, j, k), wall_face<2>(g, i, j, k)};
+            const double fval[3] = {face_value<0>(g), face_value<1>(g), face_value<2>(g)};
+            double *out[3] = {un, vn, wn};
+            const double *cur[3] = {u, v, w};
+            const CT a = CT(pp.a), c = CT(pp.c);
+            for (int q = 0; q < 3; ++q) {
+                double nv;
+                if (GEN) {
+                    const CT rhs = R[q] + CT(ga.f[q][o]);
+                    const CT val = a * rhs + CT(ga.b) * CT(ga.rhsl[q][o]) - c * Pg[q];
+                    nv = face[q] ? fval[q] : ga.uold[q][o] + pp.dt * (double)val;
+                    ga.rhsl[q][o] = (double)rhs;
+                } else {
+                    const CT inc = a * R[q] - c * Pg[q];
+                    nv = face[q] ? fval[q] : cur[q][o] + pp.dt * (double)inc;
+                }
+                out[q][o] = nv;
+                s[q] = nv;
+            }
+        }
+        if (BULK) row_sums(s, red);
+        __syncthreads();   // slot sm is reloaded next iteration; red is complete
+        // a group of rows beyond ny (64x8 tiles over the last rows) has no partial
+        if (BULK && tx == 0 && (ty & 3) == 0 && j0 + ty <= g.ny)
+            group_partial(red, ty, partial + 3 * group_index(g, bx, (j0 - 1) / 4 + ty / 4, k));
     }
 }
 
@@ -466,62 +604,112 @@ int fluid_save_uold(ibmdem_ctx *c) {
     return IBMDEM_OK;
 }
 
+namespace {
+
+// Predictor launch shape from case.stencil_tile: 0 (default) LDS z-marching
+// 64x4 tiles over 32 planes, 1 the same over 16 planes, 2 LDS 64x8 tiles
+// over 32 planes, 3 the direct kernel (one thread per cell, taps through
+// L1/L2). All give the same result bit for bit.
+struct PredShape {
+    dim3 grid, blk;
+    int kz, ty;   // ty == 0: direct kernel
+};
+PredShape pred_shape(const ibmdem_ctx *c) {
+    const GridDev &g = c->g;
+    int ty = 4, kz = 32;
+    switch (c->cs.stencil_tile) {
+        case 1: kz = 16; break;
+        case 2: ty = 8; break;
+        case 3: return {field_grid(g), dim3(BX, BY), 0, 0};
+        default: break;
+    }
+    return {dim3(div_up(g.nx, BX), div_up(g.ny, ty), div_up(g.nz, kz)), dim3(BX, ty), kz, ty};
+}
+
+template <typename CT, bool BK, bool PT, bool OP, bool GEN>
+void launch_pred(ibmdem_ctx *c, const PredShape &S, const PredParams &pp, const GeneralArgs &ga) {
+    const GridDev &g = c->g;
+    const int cur = c->vcur, nxt = 1 - c->vcur;
+    const double *u = c->u[cur], *v = c->v[cur], *w = c->w[cur], *P = c->P[c->pcur];
+    double *un = c->u[nxt], *vn = c->v[nxt], *wn = c->w[nxt];
+    if (S.ty == 0) {
+        if (GEN)
+            hipLaunchKernelGGL((k_predictor_general<CT, BK, PT, OP>), S.grid, S.blk, 0, c->stream, g, u, v, w, P, un,
+                               vn, wn, pp, ga, c->partial);
+        else
+            hipLaunchKernelGGL((k_predictor<CT, BK, PT, OP>), S.grid, S.blk, 0, c->stream, g, u, v, w, P, un, vn, wn,
+                               pp, c->partial);
+    } else if (S.ty == 8) {
+        hipLaunchKernelGGL((k_predictor_lds<CT, BK, PT, OP, 8, GEN>), S.grid, S.blk, 0, c->stream, g, u, v, w, P, un,
+                           vn, wn, pp, ga, S.kz, c->partial);
+    } else {
+        hipLaunchKernelGGL((k_predictor_lds<CT, BK, PT, OP, 4, GEN>), S.grid, S.blk, 0, c->stream, g, u, v, w, P, un,
+                           vn, wn, pp, ga, S.kz, c->partial);
+    }
+}
+
+template <bool GEN>
+void dispatch_pred(ibmdem_ctx *c, const PredShape &S, const PredParams &pp, const GeneralArgs &ga, bool bulk) {
+    const GridDev &g = c->g;
+    const bool p2 = is_pow2(g.dx) && is_pow2(g.dy) && is_pow2(g.dz) && is_pow2(c->cs.rhof);
+    const bool mixed = c->cs.precision == IBMDEM_MIXED;
+    // open sides exclude bulk forcing (rejected at creation)
+    if (grid_open(g)) {
+        if (mixed) { if (p2) launch_pred<float, false, true, true, GEN>(c, S, pp, ga); else launch_pred<float, false, false, true, GEN>(c, S, pp, ga); }
+        else { if (p2) launch_pred<double, false, true, true, GEN>(c, S, pp, ga); else launch_pred<double, false, false, true, GEN>(c, S, pp, ga); }
+    } else if (mixed) {
+        if (bulk) { if (p2) launch_pred<float, true, true, false, GEN>(c, S, pp, ga); else launch_pred<float, true, false, false, GEN>(c, S, pp, ga); }
+        else { if (p2) launch_pred<float, false, true, false, GEN>(c, S, pp, ga); else launch_pred<float, false, false, false, GEN>(c, S, pp, ga); }
+    } else {
+        if (bulk) { if (p2) launch_pred<double, true, true, false, GEN>(c, S, pp, ga); else launch_pred<double, true, false, false, GEN>(c, S, pp, ga); }
+        else { if (p2) launch_pred<double, false, true, false, GEN>(c, S, pp, ga); else launch_pred<double, false, false, false, GEN>(c, S, pp, ga); }
+    }
+}
+
+PredParams pred_params(const ibmdem_ctx *c, double a, double cc) {
+    const GridDev &g = c->g;
+    PredParams pp{c->cs.dt, c->cs.visc, c->cs.rhof, a, cc, {c->cs.cp[0], c->cs.cp[1], c->cs.cp[2]}};
+    pp.idx = 1.0 / g.dx; pp.idy = 1.0 / g.dy; pp.idz = 1.0 / g.dz;
+    pp.idx2 = 1.0 / (g.dx * g.dx); pp.idy2 = 1.0 / (g.dy * g.dy); pp.idz2 = 1.0 / (g.dz * g.dz);
+    pp.irhof = 1.0 / c->cs.rhof;
+    return pp;
+}
+
+// the row-group partials of the predictor feed the bulk finalize
+int bulk_after_predictor(ibmdem_ctx *c) {
+    const GridDev &g = c->g;
+    const long nparts = (long)div_up(g.nx, BX) * div_up(g.ny, 4) * g.nz;
+    hipLaunchKernelGGL(k_bulk_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nparts, c->scal,
+                       c->cs.bulk_switch[0], c->cs.bulk_switch[1], c->cs.bulk_switch[2], c->cs.bulk_target[0],
+                       c->cs.bulk_target[1], c->cs.bulk_target[2], (double)((long)g.nx * g.ny * g.nz), c->cs.dt);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+}  // namespace
+
 int fluid_predictor(ibmdem_ctx *c, double a, double b, double cc) {
     if (c->uold[0]) return fluid_predictor_general(c, a, b, cc);
     if (b != 0.0 || a != 1.0)
         return set_error(c, IBMDEM_ERR_UNSUPPORTED,
                          "general VelocityGuess coefficients need a context created with time_scheme = "
                          "IBMDEM_SCHEME_RK3 (it keeps uold and RHSl)");
-    const GridDev &g = c->g;
-    dim3 grid = field_grid(g), blk(BX, BY);
-    PredParams pp{c->cs.dt, c->cs.visc, c->cs.rhof, a, cc, {c->cs.cp[0], c->cs.cp[1], c->cs.cp[2]}};
-    const int cur = c->vcur, nxt = 1 - c->vcur;
+    const PredParams pp = pred_params(c, a, cc);
+    const PredShape S = pred_shape(c);
     const bool bulk = c->cs.bulk_switch[0] == 1 || c->cs.bulk_switch[1] == 1 || c->cs.bulk_switch[2] == 1;
-    const double bytes = 7.0 * 8.0 * (double)c->n;
-    pp.idx = 1.0 / g.dx; pp.idy = 1.0 / g.dy; pp.idz = 1.0 / g.dz;
-    pp.idx2 = 1.0 / (g.dx * g.dx); pp.idy2 = 1.0 / (g.dy * g.dy); pp.idz2 = 1.0 / (g.dz * g.dz);
-    pp.irhof = 1.0 / c->cs.rhof;
-    const bool p2 = is_pow2(g.dx) && is_pow2(g.dy) && is_pow2(g.dz) && is_pow2(c->cs.rhof);
-    const bool mixed = c->cs.precision == IBMDEM_MIXED;
-#define LAUNCH_PRED(CT, BK, PT, OP)                                                                        \
-    hipLaunchKernelGGL((k_predictor<CT, BK, PT, OP>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur], c->w[cur], \
-                       c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, c->partial)
+    GeneralArgs ga{};
     prof_begin(c, IBMDEM_K_PREDICTOR);
-    // open sides exclude bulk forcing (rejected at creation)
-    if (grid_open(g)) {
-        if (mixed) { if (p2) LAUNCH_PRED(float, false, true, true); else LAUNCH_PRED(float, false, false, true); }
-        else { if (p2) LAUNCH_PRED(double, false, true, true); else LAUNCH_PRED(double, false, false, true); }
-    } else if (mixed) {
-        if (bulk) { if (p2) LAUNCH_PRED(float, true, true, false); else LAUNCH_PRED(float, true, false, false); }
-        else { if (p2) LAUNCH_PRED(float, false, true, false); else LAUNCH_PRED(float, false, false, false); }
-    } else {
-        if (bulk) { if (p2) LAUNCH_PRED(double, true, true, false); else LAUNCH_PRED(double, true, false, false); }
-        else { if (p2) LAUNCH_PRED(double, false, true, false); else LAUNCH_PRED(double, false, false, false); }
-    }
-#undef LAUNCH_PRED
+    dispatch_pred<false>(c, S, pp, ga, bulk);
     CHECK_LAUNCH(c);
-    prof_end(c, IBMDEM_K_PREDICTOR, bytes);
-    c->vcur = nxt;
+    // reads u, v, w, P; writes u*, v*, w*
+    prof_end(c, IBMDEM_K_PREDICTOR, 7.0 * 8.0 * (double)c->n);
+    c->vcur = 1 - c->vcur;
     c->bulk_pending = false;
-    if (bulk) {
-        // partials of this launch feed the bulk finalize
-        const long nparts = (long)grid.x * grid.y * grid.z;
-        hipLaunchKernelGGL(k_bulk_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nparts, c->scal,
-                           c->cs.bulk_switch[0], c->cs.bulk_switch[1], c->cs.bulk_switch[2],
-                           c->cs.bulk_target[0], c->cs.bulk_target[1], c->cs.bulk_target[2],
-                           (double)((long)g.nx * g.ny * g.nz), c->cs.dt);
-        CHECK_LAUNCH(c);
-    }
-    return IBMDEM_OK;
+    return bulk ? bulk_after_predictor(c) : IBMDEM_OK;
 }
 
 int fluid_predictor_general(ibmdem_ctx *c, double a, double b, double cc) {
-    const GridDev &g = c->g;
-    dim3 grid = field_grid(g), blk(BX, BY);
-    PredParams pp{c->cs.dt, c->cs.visc, c->cs.rhof, a, cc, {c->cs.cp[0], c->cs.cp[1], c->cs.cp[2]}};
-    pp.idx = 1.0 / g.dx; pp.idy = 1.0 / g.dy; pp.idz = 1.0 / g.dz;
-    pp.idx2 = 1.0 / (g.dx * g.dx); pp.idy2 = 1.0 / (g.dy * g.dy); pp.idz2 = 1.0 / (g.dz * g.dz);
-    pp.irhof = 1.0 / c->cs.rhof;
+    const PredParams pp = pred_params(c, a, cc);
     int rc = ibm_force_field(c, c->ffull[0], c->ffull[1], c->ffull[2]);
     if (rc) return rc;
     GeneralArgs ga;
@@ -531,39 +719,16 @@ int fluid_predictor_general(ibmdem_ctx *c, double a, double b, double cc) {
         ga.rhsl[q] = c->rhsl[q];
     }
     ga.b = b;
-    const int cur = c->vcur, nxt = 1 - c->vcur;
+    const PredShape S = pred_shape(c);
     const bool bulk = c->cs.bulk_switch[0] == 1 || c->cs.bulk_switch[1] == 1 || c->cs.bulk_switch[2] == 1;
-    const bool p2 = is_pow2(g.dx) && is_pow2(g.dy) && is_pow2(g.dz) && is_pow2(c->cs.rhof);
-    const bool mixed = c->cs.precision == IBMDEM_MIXED;
-#define LAUNCH_GEN(CT, BK, PT, OP)                                                                                  \
-    hipLaunchKernelGGL((k_predictor_general<CT, BK, PT, OP>), grid, blk, 0, c->stream, g, c->u[cur], c->v[cur],      \
-                       c->w[cur], c->P[c->pcur], c->u[nxt], c->v[nxt], c->w[nxt], pp, ga, c->partial)
     prof_begin(c, IBMDEM_K_PREDICTOR);
-    if (grid_open(g)) {
-        if (mixed) { if (p2) LAUNCH_GEN(float, false, true, true); else LAUNCH_GEN(float, false, false, true); }
-        else { if (p2) LAUNCH_GEN(double, false, true, true); else LAUNCH_GEN(double, false, false, true); }
-    } else if (mixed) {
-        if (bulk) { if (p2) LAUNCH_GEN(float, true, true, false); else LAUNCH_GEN(float, true, false, false); }
-        else { if (p2) LAUNCH_GEN(float, false, true, false); else LAUNCH_GEN(float, false, false, false); }
-    } else {
-        if (bulk) { if (p2) LAUNCH_GEN(double, true, true, false); else LAUNCH_GEN(double, true, false, false); }
-        else { if (p2) LAUNCH_GEN(double, false, true, false); else LAUNCH_GEN(double, false, false, false); }
-    }
-#undef LAUNCH_GEN
+    dispatch_pred<true>(c, S, pp, ga, bulk);
     CHECK_LAUNCH(c);
     // reads u, v, w, P, uold (3), f (3), RHSl (3); writes u*, v*, w*, RHSl (3)
     prof_end(c, IBMDEM_K_PREDICTOR, 19.0 * 8.0 * (double)c->n);
-    c->vcur = nxt;
+    c->vcur = 1 - c->vcur;
     c->bulk_pending = false;
-    if (bulk) {
-        const long nparts = (long)grid.x * grid.y * grid.z;
-        hipLaunchKernelGGL(k_bulk_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nparts, c->scal,
-                           c->cs.bulk_switch[0], c->cs.bulk_switch[1], c->cs.bulk_switch[2],
-                           c->cs.bulk_target[0], c->cs.bulk_target[1], c->cs.bulk_target[2],
-                           (double)((long)g.nx * g.ny * g.nz), c->cs.dt);
-        CHECK_LAUNCH(c);
-    }
-    return IBMDEM_OK;
+    return bulk ? bulk_after_predictor(c) : IBMDEM_OK;
 }
 
 int fluid_bulk(ibmdem_ctx *c, bool apply_now) {

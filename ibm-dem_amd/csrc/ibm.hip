@@ -83,16 +83,97 @@ __global__ __launch_bounds__(T) void k_records(int M, const double *__restrict__
     for (int d = 0; d < 3; ++d) fdrag[(long)d * M + m] = 0.0;
 }
 
+// Ghost rules of the four taps at offsets -1..+2 of one dimension around a
+// marker's cell (the rules of vel_at): mapped 0-based index, and bit masks
+// (bit q for offset q - 1) of tangential mirrors that negate, wall-normal
+// zeros, inflow constants and outflow-plane reads.
+struct TapMap4 {
+    int idx[4];
+    unsigned mir, wall, cst, pln;
+    __device__ __forceinline__ void init(int i, int n, int lo, int hi) {
+        mir = wall = cst = pln = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int j = i - 1 + q;
+            if (j <= 1 && lo == IBMDEM_BC_INFLOW) cst |= 1u << q;
+            if (j < 1) {
+                if (lo == IBMDEM_BC_PERIODIC) j += n;
+                else {
+                    j = 1 - j;
+                    mir |= 1u << q;
+                    if (lo == IBMDEM_BC_WALL) wall |= 1u << q;
+                }
+            } else if (j > n) {
+                if (hi == IBMDEM_BC_PERIODIC) j -= n;
+                else {
+                    j = 2 * n + 1 - j;
+                    if (hi == IBMDEM_BC_OUTFLOW) pln |= 1u << q;
+                    else mir |= 1u << q;
+                    if (hi == IBMDEM_BC_WALL) wall |= 1u << q;
+                }
+            }
+            idx[q] = j - 1;
+        }
+    }
+};
+
+__device__ __forceinline__ long tap_off(const GridDev &g, const TapMap4 &mx, const TapMap4 &my, const TapMap4 &mz,
+                                       int a, int b, int c) {
+    return (long)mx.idx[a] + (long)g.nx * ((long)my.idx[b] + (long)g.ny * mz.idx[c]);
+}
+
+// One interpolation term of the reference, wt * u * dx * dy * dz with
+// wt = (w1 w2 w3) / (dx dy dz) (delta3d, interpolation.F90:7-30), given
+// q = w1 w2 w3. When the spacings and the cell volume are powers of two the
+// divisions and multiplications by them are exact scalings, so the term is
+// q * u rounded once, bit for bit (in the normal range of doubles).
+template <bool P2>
+__device__ __forceinline__ double interp_term(double q, double val, double dV, double idV, double dx, double dy,
+                                              double dz) {
+    if (P2) return q * val;
+    return q / dV * val * dx * dy * dz;
+}
+
+// velocity component C at the taps (a, b, c) of three TapMap4 (vel_at's
+// value: tested bit for bit against the oracle's halo arrays)
+template <int C, bool OPEN>
+__device__ __forceinline__ double tap4(const double *__restrict__ f, const GridDev &g, const TapMap4 &mx,
+                                       const TapMap4 &my, const TapMap4 &mz, int a, int b, int c) {
+    const bool zero = C == 0 ? (mx.wall >> a) & 1u : (C == 1 ? (my.wall >> b) & 1u : (mz.wall >> c) & 1u);
+    const unsigned negbits = (C != 0 ? (mx.mir >> a) & 1u : 0u) ^ (C != 1 ? (my.mir >> b) & 1u : 0u) ^
+                             (C != 2 ? (mz.mir >> c) & 1u : 0u);
+    const long fo = (long)mx.idx[a] + (long)g.nx * ((long)my.idx[b] + (long)g.ny * mz.idx[c]);
+    double val;
+    if (OPEN) {
+        const bool cst = C == 0 ? (mx.cst >> a) & 1u : (C == 1 ? (my.cst >> b) & 1u : (mz.cst >> c) & 1u);
+        const bool pln = C == 0 ? (mx.pln >> a) & 1u : (C == 1 ? (my.pln >> b) & 1u : (mz.pln >> c) & 1u);
+        const long po = C == 0 ? (long)my.idx[b] + (long)g.ny * mz.idx[c]
+                               : (C == 1 ? (long)mx.idx[a] + (long)g.nx * mz.idx[c]
+                                         : (long)mx.idx[a] + (long)g.nx * my.idx[b]);
+        val = pln ? g.plane[C][po] : f[fo];
+        val = negbits ? -val : val;
+        return cst ? g.uin[C] : (zero ? 0.0 : val);
+    }
+    val = f[fo];
+    val = negbits ? -val : val;
+    return zero ? 0.0 : val;
+}
+
 // UpdateFluidVelocityOnLagrangianPoints: U_p = sum w u dx dy dz,
-// fdrag += (U_p - v_p)/dt; also stores fdrag*vol in the marker record
+// fdrag += (U_p - v_p)/dt; also stores fdrag*vol in the marker record.
+// Markers are taken in cell-bin order (a permutation of 0..M-1), so the 64
+// markers of a wavefront are neighbours and share their velocity taps in
+// L1/L2; each marker's arithmetic is independent of the order.
 template <bool P2, bool OPEN>
-__global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const double *__restrict__ u,
+__global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__restrict__ order,
+                                              const double *__restrict__ u,
                                               const double *__restrict__ v, const double *__restrict__ w,
                                               double *__restrict__ rec, const double *__restrict__ mv,
                                               double *__restrict__ fdrag, const double *__restrict__ mvol, double dV,
                                               double idV, double dt) {
-    const int m = blockIdx.x * T + threadIdx.x;
-    if (m >= M) return;
+    const int t = blockIdx.x * T + threadIdx.x;
+    if (t >= M) return;
+    const int m = order[t];
     double *r = rec + 8L * m;
     const double2 r01 = reinterpret_cast<const double2 *>(r)[0];
     const double2 r23 = reinterpret_cast<const double2 *>(r)[1];
@@ -116,26 +197,62 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const double *__
         cy[o] = wt_cent<P2, true>(xp[1], cj + o - 1, dy, idy);
         cz[o] = wt_cent<P2, true>(xp[2], ck + o - 1, dz, idz);
     }
+    // the ghost rules of the taps -1..+2 around the cell, once per marker
+    TapMap4 mx, my, mz;
+    mx.init(ci, g.nx, g.bt[0], g.bt[1]);
+    my.init(cj, g.ny, g.bt[2], g.bt[3]);
+    mz.init(ck, g.nz, g.bt[4], g.bt[5]);
     double ulp = 0.0, vlp = 0.0, wlp = 0.0;
-    // u: x-face taps -1..2, y/z centre taps -1..1 (the nonzero ranges)
-    for (int kk = 0; kk < 3; ++kk)
-        for (int jj = 0; jj < 3; ++jj)
-            for (int ii = 0; ii < 4; ++ii) {
-                const double wt = qdiv<P2>(fx[ii] * cy[jj] * cz[kk], dV, idV);
-                ulp = ulp + wt * vel_at<0, OPEN>(u, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
-            }
-    for (int kk = 0; kk < 3; ++kk)
-        for (int jj = 0; jj < 4; ++jj)
-            for (int ii = 0; ii < 3; ++ii) {
-                const double wt = qdiv<P2>(cx[ii] * fy[jj] * cz[kk], dV, idV);
-                vlp = vlp + wt * vel_at<1, OPEN>(v, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
-            }
-    for (int kk = 0; kk < 4; ++kk)
-        for (int jj = 0; jj < 3; ++jj)
-            for (int ii = 0; ii < 3; ++ii) {
-                const double wt = qdiv<P2>(cx[ii] * cy[jj] * fz[kk], dV, idV);
-                wlp = wlp + wt * vel_at<2, OPEN>(w, g, ci + ii - 1, cj + jj - 1, ck + kk - 1) * dx * dy * dz;
-            }
+    // A marker whose taps need no mirror, zero or open-side value (all of
+    // them away from walls and open sides; periodic wraps are only index
+    // arithmetic) reads every tap directly.
+    const bool plain = !OPEN && ((mx.mir | mx.wall | my.mir | my.wall | mz.mir | mz.wall) == 0u);
+    if (plain) {
+        // u: x-face taps -1..2, y/z centre taps -1..1 (the nonzero ranges)
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii)
+                    ulp = ulp + interp_term<P2>(fx[ii] * cy[jj] * cz[kk], u[tap_off(g, mx, my, mz, ii, jj, kk)],
+                                                dV, idV, dx, dy, dz);
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii)
+                    vlp = vlp + interp_term<P2>(cx[ii] * fy[jj] * cz[kk], v[tap_off(g, mx, my, mz, ii, jj, kk)],
+                                                dV, idV, dx, dy, dz);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii)
+                    wlp = wlp + interp_term<P2>(cx[ii] * cy[jj] * fz[kk], w[tap_off(g, mx, my, mz, ii, jj, kk)],
+                                                dV, idV, dx, dy, dz);
+    } else {
+        for (int kk = 0; kk < 3; ++kk)
+            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii)
+                    ulp = ulp + interp_term<P2>(fx[ii] * cy[jj] * cz[kk], tap4<0, OPEN>(u, g, mx, my, mz, ii, jj, kk),
+                                                dV, idV, dx, dy, dz);
+        for (int kk = 0; kk < 3; ++kk)
+            for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii)
+                    vlp = vlp + interp_term<P2>(cx[ii] * fy[jj] * cz[kk], tap4<1, OPEN>(v, g, mx, my, mz, ii, jj, kk),
+                                                dV, idV, dx, dy, dz);
+        for (int kk = 0; kk < 4; ++kk)
+            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii)
+                    wlp = wlp + interp_term<P2>(cx[ii] * cy[jj] * fz[kk], tap4<2, OPEN>(w, g, mx, my, mz, ii, jj, kk),
+                                                dV, idV, dx, dy, dz);
+    }
     const double f0 = fdrag[m] + 1.0 * (ulp - mv[m]) / dt;
     const double f1 = fdrag[(long)M + m] + 1.0 * (vlp - mv[(long)M + m]) / dt;
     const double f2 = fdrag[2L * M + m] + 1.0 * (wlp - mv[2L * M + m]) / dt;
@@ -626,7 +743,8 @@ int ibm_interpolate(ibmdem_ctx *c) {
     const int cur = c->vcur;
     prof_begin(c, IBMDEM_K_INTERP);
 #define LAUNCH_INTERP(P2, OP)                                                                                      \
-    hipLaunchKernelGGL((k_interp<P2, OP>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->u[cur],      \
+    hipLaunchKernelGGL((k_interp<P2, OP>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->bin_items,  \
+                       c->u[cur],                                                                                     \
                        c->v[cur], c->w[cur], c->mrec, c->mv, c->fdrag, c->mvol, c->dV, 1.0 / c->dV, c->cs.dt)
     const bool open = grid_open(c->g);
     if (pow2_grid(c)) { if (open) LAUNCH_INTERP(true, true); else LAUNCH_INTERP(true, false); }

@@ -99,7 +99,9 @@ typedef struct ibmdem_case {
     int32_t mg_pre, mg_post;     /* red-black GS sweeps (0 -> 2,2)          */
     int32_t mg_coarse_sweeps;    /* sweeps on the coarsest level (0 -> 24)  */
     int32_t mg_max_levels;       /* 0 -> as many as the grid allows         */
-    int32_t stencil_tile;        /* predictor tile choice (0 -> default)    */
+    int32_t stencil_tile;        /* predictor schedule, same result: 0 LDS  *
+                                  * z-marching 64x4 x 32 planes (default),  *
+                                  * 1 64x4 x 16, 2 64x8 x 32, 3 direct      */
     int32_t time_scheme;         /* IBMDEM_SCHEME_*                         */
     double  inflow[3];           /* normal velocity of the inflow side of
                                   * dimension d (x, y, z)                  */

@@ -1253,6 +1253,7 @@ int ibmdem_create(const ibmdem_case *cs, ibmdem_ctx **out) {
     if (cs->n_markers > 0 && (cs->nx < 5 || cs->ny < 5 || cs->nz < 5)) return IBMDEM_ERR_ARG;
     if (cs->n_markers >= (1 << 23)) return IBMDEM_ERR_UNSUPPORTED;
     if (cs->time_scheme != IBMDEM_SCHEME_EULER && cs->time_scheme != IBMDEM_SCHEME_RK3) return IBMDEM_ERR_ARG;
+    if (cs->stencil_tile < 0 || cs->stencil_tile > 3) return IBMDEM_ERR_ARG;   /* a schedule choice only */
     ibmdem_ctx *c = (ibmdem_ctx *)xcalloc(1, sizeof *c);
     c->cs = *cs;
     if (c->cs.n_mdf <= 0) c->cs.n_mdf = 5;
