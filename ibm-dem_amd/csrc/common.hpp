@@ -417,6 +417,7 @@ struct ibmdem_ctx {
     // per-marker record, 8 doubles (64 B): x, y, z, bits of the cell index
     // (set once per step), fdrag*vol (3, set by every interpolation), pad
     double *mrec = nullptr;
+    double *mfac = nullptr;              // per marker: 12 (face, centre) delta-factor pairs
     // bins over interior cells
     int *bin_count = nullptr, *bin_start = nullptr, *bin_items = nullptr, *bin_tmp = nullptr;
     int *scan_blocks = nullptr;
@@ -432,7 +433,7 @@ struct ibmdem_ctx {
     int *tile_estart = nullptr;          // act_cap + 1 (exclusive scan)
     uint32_t *tile_ent = nullptr;        // ent_cap
     long ent_cap = 0;
-    unsigned long long *tile_mask = nullptr;  // lanes with at least one contribution
+    unsigned long long *tile_mask = nullptr;  // cells with a contribution, IBM_MASK_WORDS per tile
     double *fact = nullptr;              // f of the tiles' cells, 3 * 64 * act_cap
     bool first_mdf = true;
     bool spread_valid = false;           // fact/tile_mask hold the last spread
@@ -540,6 +541,10 @@ int ibm_download_force(ibmdem_ctx *c, double *fx, double *fy, double *fz);
 // the IBM force of the last spread as full device fields (zero elsewhere)
 int ibm_force_field(ibmdem_ctx *c, double *fx, double *fy, double *fz);
 long ibm_tile_count(const GridDev &g);
+// spread tiles are IBM_TL^3 cells; each keeps IBM_TL^3 / 64 mask words
+constexpr int IBM_TL = 8;
+constexpr int IBM_TC = IBM_TL * IBM_TL * IBM_TL;
+constexpr int IBM_MASK_WORDS = IBM_TC / 64;
 // dem.hip
 int dem_setup(ibmdem_ctx *c);
 int dem_rebuild(ibmdem_ctx *c);

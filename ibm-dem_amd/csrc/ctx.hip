@@ -104,6 +104,7 @@ int alloc_all(ibmdem_ctx *c) {
         for (int q = 0; q < 3; ++q) { A(c->rhsl[q], n); A(c->uold[q], n); A(c->ffull[q], n); }
     A(c->mvol, M); A(c->mmass, M); A(c->mrad, M); A(c->mbody, M); A(c->mcell, M);
     A(c->mrec, 8L * M);
+    A(c->mfac, 24L * M);
     A(c->bin_count, n); A(c->bin_start, n + 1); A(c->bin_items, M); A(c->bin_tmp, M);
     c->scan_nblocks = div_up(n, 4096) + 2;
     A(c->scan_blocks, c->scan_nblocks);
@@ -115,12 +116,12 @@ int alloc_all(ibmdem_ctx *c) {
     A(c->act_meta, 8);
     A(c->tile_ecount, c->act_cap + 1);
     A(c->tile_estart, c->act_cap + 1);
-    A(c->tile_mask, c->act_cap);
+    A(c->tile_mask, IBM_MASK_WORDS * c->act_cap);
     // a cell lies in the source window of at most 3 tiles per dimension
     // (2 directly, 1 more through a wall mirror or periodic wrap)
     c->ent_cap = 27L * M + 64;
     A(c->tile_ent, c->ent_cap);
-    A(c->fact, 3 * 64 * c->act_cap);
+    A(c->fact, 3L * IBM_TC * c->act_cap);
     A(c->body_start, B + 1);
     A(c->bx, 3L * B); A(c->bv, 3L * B); A(c->bvold, 3L * B); A(c->bom, 3L * B); A(c->bL, 3L * B);
     A(c->bLold, 3L * B); A(c->brot, 9L * B); A(c->brotg, 9L * B); A(c->bI, 9L * B); A(c->bm, B);

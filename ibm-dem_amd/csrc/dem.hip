@@ -109,16 +109,26 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int nb, int *__restrict__ 
     if (threadIdx.x == 0) a[nb] = carry;
 }
 
-// exclusive scan of counts into start (CHUNK cells per block, 16 per thread)
+// exclusive scan of counts into start (CHUNK cells per block, 16 per
+// thread). Counts and starts cross HBM in coalesced rows through a padded
+// LDS tile (element e at e + e/32, so a thread's 16 consecutive elements
+// fall in distinct banks across the wavefront).
 __global__ __launch_bounds__(T) void k_chunk_scan(long n, const int *__restrict__ cnt, const int *__restrict__ off,
                                                   int *__restrict__ start, int nb) {
+    __shared__ int tile[CHUNK + CHUNK / 32];
     __shared__ int buf[T];
     constexpr int PER = CHUNK / T;
-    const long lo = (long)blockIdx.x * CHUNK + (long)threadIdx.x * PER;
+    const long base = (long)blockIdx.x * CHUNK;
+    for (int q = 0; q < PER; ++q) {
+        const int e = q * T + threadIdx.x;
+        tile[e + (e >> 5)] = base + e < n ? cnt[base + e] : 0;
+    }
+    __syncthreads();
     int loc[PER];
     int s = 0;
     for (int q = 0; q < PER; ++q) {
-        loc[q] = (lo + q < n) ? cnt[lo + q] : 0;
+        const int e = threadIdx.x * PER + q;
+        loc[q] = tile[e + (e >> 5)];
         s += loc[q];
     }
     buf[threadIdx.x] = s;
@@ -131,8 +141,14 @@ __global__ __launch_bounds__(T) void k_chunk_scan(long n, const int *__restrict_
     }
     int pos = off[blockIdx.x] + buf[threadIdx.x] - s;
     for (int q = 0; q < PER; ++q) {
-        if (lo + q < n) start[lo + q] = pos;
+        const int e = threadIdx.x * PER + q;
+        tile[e + (e >> 5)] = pos;
         pos += loc[q];
+    }
+    __syncthreads();
+    for (int q = 0; q < PER; ++q) {
+        const int e = q * T + threadIdx.x;
+        if (base + e < n) start[base + e] = tile[e + (e >> 5)];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) start[n] = off[nb];
 }

@@ -7,10 +7,12 @@
 //   - every marker gets a 64-byte record (position, cell; the interpolation
 //     adds fdrag*vol to it each iteration), so the spread reads one line per
 //     marker instead of three SoA arrays;
-//   - the spread footprint is compacted into a sorted list of active 4x4x4
+//   - the spread footprint is compacted into a sorted list of active 8x8x8
 //     cell tiles, and each active tile gets the list of its source entries:
-//     the markers of its 7x7x7 source cells in source order (kc,jc,ic) and
-//     ascending id inside a cell, each packed as marker << 9 | source cell.
+//     the markers of its 11x11x11 source cells in source order (kc,jc,ic)
+//     and ascending id inside a cell, each packed as marker << 9 | image
+//     code (3 bits per dimension: direct, periodic copy below or above,
+//     mirror copy below or above).
 // The delta factors (per dimension: staggered-face and cell-centre variants
 // at cell offsets -1..+2) are recomputed from the record where they are used;
 // the arithmetic is the same each time, so the results are too.
@@ -19,29 +21,34 @@
 // (kk,jj,ii) order, so it is bit-identical to the reference's 125-cell loop
 // (zero-weight taps add an exact zero there).
 //
-// Spreading is a deterministic gather, no float atomics. One wavefront owns
-// a tile; it stages chunks of its entry list in LDS (delta factors, fdrag*vol
-// and cell of the marker or of its image) and each lane sums, for its own
-// cell, f = 0 - sum (fdrag*vol)*w over the entries in list order. The
-// reference's periodic ghost copies (x + coordShift) and wall-mirror copies
-// (2*coordMirror - x, fdrag unnegated; ZJS_MPI_DEM.F90:173-269) enter as
-// wrapped or mirrored source cells with their factors evaluated at the image
-// position. Each lane then applies u += dt*(f - f_old) on its cell; lanes
-// whose cell no entry reaches are skipped entirely (their f is structurally
-// zero in every iteration of the step).
+// Spreading is deterministic, with no float atomics. One wavefront owns a
+// tile and keeps the tile's force (3 x 512 doubles) in LDS. It stages its
+// entry list in chunks (delta factors, fdrag*vol and the cell of the marker
+// or of its image) and takes the entries one after another: lane l adds the
+// entry's contribution -(fdrag*vol)*w to cell l of the entry's 4x4x4
+// support (when that cell is in the tile). The 64 lanes of one entry write
+// 64 different cells, and a wavefront's LDS operations complete in program
+// order, so every cell receives its contributions in entry-list order: the
+// sum does not depend on timing. The reference's periodic ghost copies
+// (x + coordShift) and wall-mirror copies (2*coordMirror - x, fdrag
+// unnegated; ZJS_MPI_DEM.F90:173-269) enter as image entries with their
+// factors evaluated at the image position. The wavefront then applies
+// u += dt*(f - f_old) on the tile's cells that some entry reached; the
+// others are skipped (their f is structurally zero in every iteration of
+// the step).
 #include "common.hpp"
 
 namespace {
 
 constexpr int T = 256;
 constexpr int CHUNK = 4096;   // flags per compaction block
-constexpr int TL = 4;         // tile edge (cells)
+constexpr int TL = IBM_TL;    // tile edge (cells)
+constexpr int TC = IBM_TC;    // cells per tile
 constexpr int SRC = TL + 3;   // source cells per tile edge
 constexpr int NSRC = SRC * SRC * SRC;
-constexpr int CAP = 64;       // LDS entries per pass (one per lane)
+constexpr int CAP = 32;       // entries staged per pass
 constexpr int WST2 = 13;      // LDS row stride of the 12 factor pairs (odd: fewer bank conflicts)
-constexpr int SBITS = 9;      // source-cell bits of an entry (NSRC <= 512)
-static_assert(NSRC <= (1 << SBITS), "source cell index must fit its bits");
+constexpr int SBITS = 9;      // image-code bits of an entry (3 per dimension)
 
 // one-dimensional delta factors of a marker at xp for cell i: staggered
 // face (x = (i-1)d) and cell centre (x = (i-1/2)d), GetWeight
@@ -72,15 +79,34 @@ __device__ __forceinline__ int tile_slot(int b, int nact) {
     return slot < nact ? slot : -1;
 }
 
-// per-step marker record: position, cell; zero fdrag
-__global__ __launch_bounds__(T) void k_records(int M, const double *__restrict__ mx, const int *__restrict__ mcell,
-                                               double *__restrict__ rec, double *__restrict__ fdrag) {
+// per-step marker record: position, cell; zero fdrag; and the marker's
+// one-dimensional delta factors, fac[d*4 + o] = (face, centre) for cell
+// offset o - 1 of dimension d (the centre factor at offset +2 is always 0:
+// the marker lies in its cell, so that centre is at least 3/2 cells away).
+// Markers do not move during the five interpolate/spread iterations of a
+// step, so these are computed once here and read by both.
+template <bool P2>
+__global__ __launch_bounds__(T) void k_records(int M, GridDev g, const double *__restrict__ mx,
+                                               const int *__restrict__ mcell, double *__restrict__ rec,
+                                               double *__restrict__ fac, double *__restrict__ fdrag) {
     const int m = blockIdx.x * T + threadIdx.x;
     if (m >= M) return;
     double *r = rec + 8L * m;
-    reinterpret_cast<double2 *>(r)[0] = make_double2(mx[m], mx[(long)M + m]);
-    reinterpret_cast<double2 *>(r)[1] = make_double2(mx[2L * M + m], __longlong_as_double((long long)mcell[m]));
+    const double xp[3] = {mx[m], mx[(long)M + m], mx[2L * M + m]};
+    const int cl = mcell[m];
+    reinterpret_cast<double2 *>(r)[0] = make_double2(xp[0], xp[1]);
+    reinterpret_cast<double2 *>(r)[1] = make_double2(xp[2], __longlong_as_double((long long)cl));
     for (int d = 0; d < 3; ++d) fdrag[(long)d * M + m] = 0.0;
+    const int cc[3] = {cl % g.nx + 1, (cl / g.nx) % g.ny + 1, cl / (g.nx * g.ny) + 1};
+    const double dd[3] = {g.dx, g.dy, g.dz};
+    double2 *f = reinterpret_cast<double2 *>(fac + 24L * m);
+    for (int d = 0; d < 3; ++d) {
+        const double id = 1.0 / dd[d];
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+            f[d * 4 + o] = make_double2(wt_face<P2, true>(xp[d], cc[d] + o - 1, dd[d], id),
+                                        o < 3 ? wt_cent<P2, true>(xp[d], cc[d] + o - 1, dd[d], id) : 0.0);
+    }
 }
 
 // Ghost rules of the four taps at offsets -1..+2 of one dimension around a
@@ -166,7 +192,7 @@ __device__ __forceinline__ double tap4(const double *__restrict__ f, const GridD
 // L1/L2; each marker's arithmetic is independent of the order.
 template <bool P2, bool OPEN>
 __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__restrict__ order,
-                                              const double *__restrict__ u,
+                                              const double *__restrict__ fac, const double *__restrict__ u,
                                               const double *__restrict__ v, const double *__restrict__ w,
                                               double *__restrict__ rec, const double *__restrict__ mv,
                                               double *__restrict__ fdrag, const double *__restrict__ mvol, double dV,
@@ -181,21 +207,16 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__res
     const int cl = (int)__double_as_longlong(r23.y);
     const int ci = cl % g.nx + 1, cj = (cl / g.nx) % g.ny + 1, ck = cl / (g.nx * g.ny) + 1;
     const double dx = g.dx, dy = g.dy, dz = g.dz;
-    const double idx = 1.0 / dx, idy = 1.0 / dy, idz = 1.0 / dz;
-    // centre factors at offset +2 are always 0 (the marker lies in cell c,
-    // so that centre is at least 3/2 cells away) and are not used below
+    (void)xp;
+    // the step's delta factors of the marker (k_records); centre factors at
+    // offset +2 are 0 and not used below
     double fx[4], cx[3], fy[4], cy[3], fz[4], cz[3];
+    const double2 *fp = reinterpret_cast<const double2 *>(fac + 24L * m);
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
-        fx[o] = wt_face<P2, true>(xp[0], ci + o - 1, dx, idx);
-        fy[o] = wt_face<P2, true>(xp[1], cj + o - 1, dy, idy);
-        fz[o] = wt_face<P2, true>(xp[2], ck + o - 1, dz, idz);
-    }
-#pragma unroll
-    for (int o = 0; o < 3; ++o) {
-        cx[o] = wt_cent<P2, true>(xp[0], ci + o - 1, dx, idx);
-        cy[o] = wt_cent<P2, true>(xp[1], cj + o - 1, dy, idy);
-        cz[o] = wt_cent<P2, true>(xp[2], ck + o - 1, dz, idz);
+        const double2 a = fp[o], b = fp[4 + o], c = fp[8 + o];
+        fx[o] = a.x; fy[o] = b.x; fz[o] = c.x;
+        if (o < 3) { cx[o] = a.y; cy[o] = b.y; cz[o] = c.y; }
     }
     // the ghost rules of the taps -1..+2 around the cell, once per marker
     TapMap4 mx, my, mz;
@@ -389,6 +410,21 @@ __device__ __forceinline__ void src_coords(int s, int ib, int jb, int kb, int sc
     scc[2] = kb + s / (SRC * SRC);
 }
 
+// image code of an unwrapped source cell, 3 bits per dimension: 0 the
+// marker itself, 1/2 its periodic copy from below/above (x -/+ L), 3/4 its
+// mirror copy below/above (-x, 2L - x)
+__device__ __forceinline__ uint32_t image_code(const GridDev &g, const int scc[3]) {
+    const int n3[3] = {g.nx, g.ny, g.nz};
+    uint32_t code = 0;
+    for (int d = 0; d < 3; ++d) {
+        uint32_t c = 0;
+        if (scc[d] < 1) c = g.bt[2 * d] == IBMDEM_BC_PERIODIC ? 1u : 3u;
+        else if (scc[d] > n3[d]) c = g.bt[2 * d + 1] == IBMDEM_BC_PERIODIC ? 2u : 4u;
+        code |= c << (3 * d);
+    }
+    return code;
+}
+
 // Per-source-cell bin ranges of a tile: first item of each of the NSRC
 // source cells in s_gstart, the count prefix in s_off (s_off[NSRC] = total).
 __device__ __forceinline__ void tile_bins(const GridDev &g, int ib, int jb, int kb, const int *__restrict__ bin_start,
@@ -485,11 +521,16 @@ __global__ __launch_bounds__(64) void k_tile_fill(GridDev g, TileGeom tg, const 
         if (threadIdx.x == 0) atomicOr(&flags[FL_CAPACITY], 1);
         return;
     }
+    const int ib = tx * TL - 1, jb = ty * TL - 1, kb = tz * TL - 1;
     for (int s = threadIdx.x; s < NSRC; s += 64) {
         const int n = s_off[s + 1] - s_off[s];
+        if (n == 0) continue;
         const int g0 = s_gstart[s];
+        int scc[3];
+        src_coords(s, ib, jb, kb, scc);
+        const uint32_t code = image_code(g, scc);
         for (int q = 0; q < n; ++q)
-            ent[base + s_off[s] + q] = ((uint32_t)bin_items[g0 + q] << SBITS) | (uint32_t)s;
+            ent[base + s_off[s] + q] = ((uint32_t)bin_items[g0 + q] << SBITS) | code;
     }
 }
 
@@ -502,8 +543,9 @@ struct SpreadArgs {
     const uint32_t *ent;
     long ent_cap;
     const double *rec;
-    double *fact;      // 3 x (64 * cap_tiles), tile slot major
-    long fstride;      // 64 * cap_tiles
+    const double *fac;  // per marker: 12 (face, centre) factor pairs
+    double *fact;      // 3 x (TC * cap_tiles), tile slot major
+    long fstride;      // TC * cap_tiles
     unsigned long long *mask;
     double *u, *v, *w;
     double dV, idV, dt;
@@ -524,7 +566,8 @@ __device__ __forceinline__ void wave_sync() {
 // load never waits for it.
 struct EntryIn {
     double x0, x1, x2, f0, f1, f2;
-    uint32_t en;    // marker << SBITS | source cell
+    int cell;       // the marker's cell (linear)
+    uint32_t en;    // marker << SBITS | image code
     bool valid;     // the lane stages an entry
 };
 
@@ -538,39 +581,46 @@ __device__ __forceinline__ void load_entry_rec(const SpreadArgs &a, EntryIn &in)
         const double2 *r = reinterpret_cast<const double2 *>(a.rec + 8 * m);
         const double2 r01 = r[0], r23 = r[1], r45 = r[2];
         in.x0 = r01.x; in.x1 = r01.y; in.x2 = r23.x;
+        in.cell = (int)__double_as_longlong(r23.y);
         in.f0 = r45.x; in.f1 = r45.y; in.f2 = a.rec[8 * m + 6];
     }
 }
 
 // delta factors, cell and fdrag*vol of the lane's entry into LDS row `lane`
 template <bool P2>
-__device__ __forceinline__ void stage_entry(const SpreadArgs &a, const EntryIn &in, int ib, int jb, int kb,
-                                            double2 (*s_w)[WST2], int4 *s_cc, double4 *s_fv) {
+__device__ __forceinline__ void stage_entry(const SpreadArgs &a, const EntryIn &in, double2 (*s_w)[WST2],
+                                            int4 *s_cc, double4 *s_fv) {
     const GridDev &g = a.g;
     const int lane = threadIdx.x;
     if (!in.valid) return;
-    const int src = (int)(in.en & ((1u << SBITS) - 1));
+    const uint32_t code = in.en & ((1u << SBITS) - 1);
     const double L3[3] = {g.L[0], g.L[1], g.L[2]};
     const double dd[3] = {g.dx, g.dy, g.dz};
     const double id3[3] = {1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz};
-    const int n3[3] = {g.nx, g.ny, g.nz};
     const double xm[3] = {in.x0, in.x1, in.x2};
+    const int cc[3] = {in.cell % g.nx + 1, (in.cell / g.nx) % g.ny + 1, in.cell / (g.nx * g.ny) + 1};
     // with a power-of-two cell volume the 1/dV of every weight is an exact
     // scaling, folded into fdrag*vol here once per entry (bitwise the same
     // products as fv * (w * (1/dV)))
     if (P2) s_fv[lane] = make_double4(in.f0 * a.idV, in.f1 * a.idV, in.f2 * a.idV, 0.0);
     else s_fv[lane] = make_double4(in.f0, in.f1, in.f2, 0.0);
-    int scc[3], cim[3];
-    src_coords(src, ib, jb, kb, scc);
+    int cim[3];
+    const long m = (long)(in.en >> SBITS);
+    const double2 *mf = reinterpret_cast<const double2 *>(a.fac + 24 * m);
     for (int d = 0; d < 3; ++d) {
-        int ic = scc[d], kind;
-        src_map(ic, n3[d], g.bt[2 * d], g.bt[2 * d + 1], kind);
+        const uint32_t c = (code >> (3 * d)) & 7u;
+        if (c == 0u) {
+            // the marker itself: its factors from k_records
+            cim[d] = cc[d];
+            for (int o = 0; o < 4; ++o) s_w[lane][d * 4 + o] = mf[d * 4 + o];
+            continue;
+        }
         double xp = xm[d];
-        int cimg = scc[d];
-        if (kind == 1) { xp = xp + (scc[d] < 1 ? -L3[d] : L3[d]); cimg = cell_of(xp, dd[d]); }
-        else if (kind == 2) { xp = 2.0 * (scc[d] < 1 ? 0.0 : L3[d]) - xp; cimg = cell_of(xp, dd[d]); }
+        if (c == 1u || c == 2u) xp = xp + (c == 1u ? -L3[d] : L3[d]);
+        else xp = 2.0 * (c == 3u ? 0.0 : L3[d]) - xp;
+        const int cimg = cell_of(xp, dd[d]);
         cim[d] = cimg;
-        // the centre factor at offset +2 is always 0 (see k_interp)
+        // the centre factor at offset +2 is always 0 (see k_records)
         for (int o = 0; o < 4; ++o)
             s_w[lane][d * 4 + o] = make_double2(wt_face<P2>(xp, cimg + o - 1, dd[d], id3[d]),
                                                 o < 3 ? wt_cent<P2>(xp, cimg + o - 1, dd[d], id3[d]) : 0.0);
@@ -578,34 +628,11 @@ __device__ __forceinline__ void stage_entry(const SpreadArgs &a, const EntryIn &
     s_cc[lane] = make_int4(cim[0], cim[1], cim[2], 0);
 }
 
-// the lane's cell: f -= fdrag*vol * w over the staged entries, in order
-template <bool P2>
-__device__ __forceinline__ void gather_entries(const SpreadArgs &a, int ne, int i, int j, int k,
-                                               const double2 (*s_w)[WST2], const int4 *s_cc, const double4 *s_fv,
-                                               double &f0, double &f1, double &f2, bool &has) {
-    // Branch-free: an entry whose footprint misses the lane's cell subtracts
-    // an exact 0 (f - 0 = f), read through an in-range index, so the loads
-    // of several entries can be in flight at once.
-#pragma unroll 4
-    for (int e = 0; e < ne; ++e) {
-        const int4 cc = s_cc[e];
-        const int ox = i - cc.x + 1, oy = j - cc.y + 1, oz = k - cc.z + 1;
-        const bool in = (unsigned)ox <= 3u && (unsigned)oy <= 3u && (unsigned)oz <= 3u;
-        has = has || in;
-        const double2 wx = s_w[e][ox & 3], wy = s_w[e][4 + (oy & 3)], wz = s_w[e][8 + (oz & 3)];
-        const double4 fv = s_fv[e];
-        // (face, centre): u takes the x face, v the y face, w the z face
-        const double pu = wx.x * wy.y * wz.y, pv = wx.y * wy.x * wz.y, pw = wx.y * wy.y * wz.x;
-        const double wu = P2 ? pu : pu / a.dV, wv = P2 ? pv : pv / a.dV, ww = P2 ? pw : pw / a.dV;
-        f0 = f0 - (in ? fv.x * wu : 0.0);
-        f1 = f1 - (in ? fv.y * wv : 0.0);
-        f2 = f2 - (in ? fv.z * ww : 0.0);
-    }
-}
-
 // One wavefront per active tile (block b takes tile slot tile_slot(b)).
 template <bool P2>
 __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
+    __shared__ double s_acc[3][TC];            // the tile's force
+    __shared__ unsigned s_touch[TC / 32];      // cells some entry reached
     // per staged entry: (face, centre) factor pairs for 3 dimensions x 4
     // offsets (padded to an odd number of 16-byte pairs per entry), the cell
     // of the marker or image, and fdrag*vol
@@ -620,37 +647,61 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     if (ehi > a.ent_cap) return;   // the fill flagged a capacity error
     int tx, ty, tz;
     tile_origin(a.tiles[slot], a.tg, tx, ty, tz);
-    const int ib = tx * TL - 1, jb = ty * TL - 1, kb = tz * TL - 1;   // 1-based source origin
-    const int li = lane % TL, lj = (lane / TL) % TL, lk = lane / (TL * TL);
-    const int i = tx * TL + 1 + li, j = ty * TL + 1 + lj, k = tz * TL + 1 + lk;
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    bool has = false;
+    const int i0 = tx * TL + 1, j0 = ty * TL + 1, k0 = tz * TL + 1;
+    for (int q = lane; q < TC; q += 64) {
+        s_acc[0][q] = 0.0;
+        s_acc[1][q] = 0.0;
+        s_acc[2][q] = 0.0;
+    }
+    if (lane < TC / 32) s_touch[lane] = 0u;
+    // the lane's offset in an entry's 4x4x4 support
+    const int oa = lane & 3, ob = (lane >> 2) & 3, oc = lane >> 4;
+    wave_sync();
     for (int e0 = elo; e0 < ehi; e0 += CAP) {
+        const int ne = min(CAP, ehi - e0);
         EntryIn in;
-        load_entry_id(a, e0 + lane, lane < ehi - e0, in);
+        load_entry_id(a, e0 + lane, lane < ne, in);
         load_entry_rec(a, in);
-        stage_entry<P2>(a, in, ib, jb, kb, s_w, s_cc, s_fv);
+        stage_entry<P2>(a, in, s_w, s_cc, s_fv);
         wave_sync();
-        gather_entries<P2>(a, min(CAP, ehi - e0), i, j, k, s_w, s_cc, s_fv, f0, f1, f2, has);
+        for (int e = 0; e < ne; ++e) {
+            const int4 cc = s_cc[e];
+            const int lx = cc.x - 1 + oa - i0, ly = cc.y - 1 + ob - j0, lz = cc.z - 1 + oc - k0;
+            if ((unsigned)lx < (unsigned)TL && (unsigned)ly < (unsigned)TL && (unsigned)lz < (unsigned)TL) {
+                const double2 wx = s_w[e][oa], wy = s_w[e][4 + ob], wz = s_w[e][8 + oc];
+                const double4 fv = s_fv[e];
+                // (face, centre): u takes the x face, v the y face, w the z face
+                const double pu = wx.x * wy.y * wz.y, pv = wx.y * wy.x * wz.y, pw = wx.y * wy.y * wz.x;
+                const double wu = P2 ? pu : pu / a.dV, wv = P2 ? pv : pv / a.dV, ww = P2 ? pw : pw / a.dV;
+                const int q = lx + TL * (ly + TL * lz);
+                s_acc[0][q] = s_acc[0][q] - fv.x * wu;
+                s_acc[1][q] = s_acc[1][q] - fv.y * wv;
+                s_acc[2][q] = s_acc[2][q] - fv.z * ww;
+                atomicOr(&s_touch[q >> 5], 1u << (q & 31));
+            }
+        }
         wave_sync();   // the rows are rewritten by the next chunk
     }
-    if (a.first) {
-        const unsigned long long bits = __ballot(has ? 1 : 0);
-        if (lane == 0) a.mask[slot] = bits;
-    }
-    // multi-direct-forcing update u += dt*(f - f_old) on the lane's cell
-    if (!has || i > g.nx || j > g.ny || k > g.nz) return;
-    const int tgt[3] = {i, j, k};
-    const long fs = (long)slot * 64 + lane;
-    const long o = cidx(g, i, j, k);
+    if (a.first && lane < IBM_MASK_WORDS)
+        a.mask[(long)slot * IBM_MASK_WORDS + lane] =
+            ((unsigned long long)s_touch[2 * lane + 1] << 32) | (unsigned long long)s_touch[2 * lane];
+    // multi-direct-forcing update u += dt*(f - f_old) on the reached cells
     double *vel[3] = {a.u, a.v, a.w};
-    const double fn[3] = {f0, f1, f2};
-    for (int c = 0; c < 3; ++c) {
-        const double fo = a.first ? 0.0 : a.fact[c * a.fstride + fs];
-        a.fact[c * a.fstride + fs] = fn[c];
-        // the low boundary faces (wall, inflow) keep their fixed values
-        const bool face = tgt[c] == 1 && g.bt[2 * c] != IBMDEM_BC_PERIODIC;
-        if (!face && !(fn[c] == 0.0 && fo == 0.0)) vel[c][o] = vel[c][o] + a.dt * (fn[c] - fo);
+    const long fbase = (long)slot * TC;
+    for (int q = lane; q < TC; q += 64) {
+        if (!((s_touch[q >> 5] >> (q & 31)) & 1u)) continue;
+        const int i = i0 + (q & (TL - 1)), j = j0 + ((q / TL) & (TL - 1)), k = k0 + q / (TL * TL);
+        if (i > g.nx || j > g.ny || k > g.nz) continue;
+        const int tgt[3] = {i, j, k};
+        const long o = cidx(g, i, j, k);
+        for (int c = 0; c < 3; ++c) {
+            const double fn = s_acc[c][q];
+            const double fo = a.first ? 0.0 : a.fact[c * a.fstride + fbase + q];
+            a.fact[c * a.fstride + fbase + q] = fn;
+            // the low boundary faces (wall, inflow) keep their fixed values
+            const bool face = tgt[c] == 1 && g.bt[2 * c] != IBMDEM_BC_PERIODIC;
+            if (!face && !(fn == 0.0 && fo == 0.0)) vel[c][o] = vel[c][o] + a.dt * (fn - fo);
+        }
     }
 }
 
@@ -661,16 +712,18 @@ __global__ __launch_bounds__(64) void k_scatter_force(const int *__restrict__ me
                                                       double *__restrict__ fz) {
     const int b = tile_slot(blockIdx.x, meta[0]);
     if (b < 0) return;
-    const int lane = threadIdx.x;
-    if (!((mask[b] >> lane) & 1ull)) return;
     int tx, ty, tz;
     tile_origin(tiles[b], tg, tx, ty, tz);
-    const int i = tx * TL + 1 + lane % TL, j = ty * TL + 1 + (lane / TL) % TL, k = tz * TL + 1 + lane / (TL * TL);
-    if (i > g.nx || j > g.ny || k > g.nz) return;
-    const long slot = (long)b * 64 + lane, o = cidx(g, i, j, k);
-    fx[o] = fact[slot];
-    fy[o] = fact[fstride + slot];
-    fz[o] = fact[2 * fstride + slot];
+    for (int q = threadIdx.x; q < TC; q += 64) {
+        if (!((mask[(long)b * IBM_MASK_WORDS + q / 64] >> (q & 63)) & 1ull)) continue;
+        const int i = tx * TL + 1 + (q & (TL - 1)), j = ty * TL + 1 + ((q / TL) & (TL - 1)),
+                  k = tz * TL + 1 + q / (TL * TL);
+        if (i > g.nx || j > g.ny || k > g.nz) continue;
+        const long slot = (long)b * TC + q, o = cidx(g, i, j, k);
+        fx[o] = fact[slot];
+        fy[o] = fact[fstride + slot];
+        fz[o] = fact[2 * fstride + slot];
+    }
 }
 
 // cells with at least one contribution, summed over the active tiles
@@ -680,7 +733,7 @@ __global__ __launch_bounds__(1024) void k_mask_count(const int *__restrict__ met
     __shared__ long long red[1024];
     const int nact = meta[0];
     long long s = 0;
-    for (int b = threadIdx.x; b < nact; b += 1024) s += __popcll(mask[b]);
+    for (long b = threadIdx.x; b < (long)nact * IBM_MASK_WORDS; b += 1024) s += __popcll(mask[b]);
     red[threadIdx.x] = s;
     __syncthreads();
     for (int st = 512; st > 0; st >>= 1) {
@@ -713,7 +766,12 @@ int ibm_begin_step(ibmdem_ctx *c) {
     hipStream_t s = c->stream;
     const TileGeom tg = tile_geom(c->g);
     const long nt = (long)tg.ntx * tg.nty * tg.ntz;
-    hipLaunchKernelGGL(k_records, dim3(div_up(M, T)), dim3(T), 0, s, M, c->mx, c->mcell, c->mrec, c->fdrag);
+    if (pow2_grid(c))
+        hipLaunchKernelGGL(k_records<true>, dim3(div_up(M, T)), dim3(T), 0, s, M, c->g, c->mx, c->mcell, c->mrec,
+                           c->mfac, c->fdrag);
+    else
+        hipLaunchKernelGGL(k_records<false>, dim3(div_up(M, T)), dim3(T), 0, s, M, c->g, c->mx, c->mcell, c->mrec,
+                           c->mfac, c->fdrag);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_mark_tiles, dim3(div_up(M, T)), dim3(T), 0, s, M, c->g, tg, c->mcell, c->act_flag);
     CHECK_LAUNCH(c);
@@ -744,15 +802,15 @@ int ibm_interpolate(ibmdem_ctx *c) {
     prof_begin(c, IBMDEM_K_INTERP);
 #define LAUNCH_INTERP(P2, OP)                                                                                      \
     hipLaunchKernelGGL((k_interp<P2, OP>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->bin_items,  \
-                       c->u[cur],                                                                                     \
+                       c->mfac, c->u[cur],                                                                            \
                        c->v[cur], c->w[cur], c->mrec, c->mv, c->fdrag, c->mvol, c->dV, 1.0 / c->dV, c->cs.dt)
     const bool open = grid_open(c->g);
     if (pow2_grid(c)) { if (open) LAUNCH_INTERP(true, true); else LAUNCH_INTERP(true, false); }
     else { if (open) LAUNCH_INTERP(false, true); else LAUNCH_INTERP(false, false); }
 #undef LAUNCH_INTERP
-    // per marker: record 32 B, v 24, fdrag 24 + 24, vol 8, fdrag*vol 24;
-    // one read of each footprint velocity (cell, component)
-    prof_end(c, IBMDEM_K_INTERP, 136.0 * c->M + 8.0 * 3.0 * (double)c->footprint_cells);
+    // per marker: record 32 B, factors 192, v 24, fdrag 24 + 24, vol 8,
+    // fdrag*vol 24; one read of each footprint velocity (cell, component)
+    prof_end(c, IBMDEM_K_INTERP, 328.0 * c->M + 8.0 * 3.0 * (double)c->footprint_cells);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
@@ -769,8 +827,9 @@ int ibm_spread(ibmdem_ctx *c) {
     a.ent = c->tile_ent;
     a.ent_cap = c->ent_cap;
     a.rec = c->mrec;
+    a.fac = c->mfac;
     a.fact = c->fact;
-    a.fstride = 64L * c->act_cap;
+    a.fstride = (long)TC * c->act_cap;
     a.mask = c->tile_mask;
     a.u = c->u[cur]; a.v = c->v[cur]; a.w = c->w[cur];
     a.dV = c->dV;
@@ -796,11 +855,12 @@ int ibm_spread(ibmdem_ctx *c) {
             HIPX(c, hipStreamSynchronize(c->stream));
             c->footprint_cells = (long)h;
         }
-        // per marker: its 64 B record; per written (cell, component): u read
-        // and written, f written, f_old read (not in the first iteration)
+        // per marker: its 64 B record and 192 B of delta factors; per written
+        // (cell, component): u read and written, f written, f_old read (not
+        // in the first iteration)
         const double per = a.first ? 24.0 : 32.0;
         if (c->prof.kernel == IBMDEM_K_SPREAD)
-            c->prof.bytes_total += 64.0 * c->M + per * 3.0 * (double)c->footprint_cells;
+            c->prof.bytes_total += 256.0 * c->M + per * 3.0 * (double)c->footprint_cells;
     }
     c->first_mdf = false;
     c->spread_valid = true;
@@ -817,7 +877,7 @@ int ibm_force_field(ibmdem_ctx *c, double *fx, double *fy, double *fz) {
     }
     if (c->M > 0 && c->spread_valid) {
         hipLaunchKernelGGL(k_scatter_force, dim3(c->act_grid), dim3(64), 0, s, c->act_meta, c->act_list,
-                           tile_geom(c->g), c->g, c->fact, 64L * c->act_cap, c->tile_mask, fx, fy, fz);
+                           tile_geom(c->g), c->g, c->fact, (long)TC * c->act_cap, c->tile_mask, fx, fy, fz);
         CHECK_LAUNCH(c);
     }
     return IBMDEM_OK;
@@ -832,7 +892,7 @@ int ibm_download_force(ibmdem_ctx *c, double *fx, double *fy, double *fz) {
     if (hipMemsetAsync(buf, 0, 3 * n * sizeof(double), s) != hipSuccess) rc = IBMDEM_ERR_HIP;
     if (!rc && c->M > 0 && c->spread_valid) {
         hipLaunchKernelGGL(k_scatter_force, dim3(c->act_grid), dim3(64), 0, s, c->act_meta, c->act_list,
-                           tile_geom(c->g), c->g, c->fact, 64L * c->act_cap, c->tile_mask, buf, buf + n, buf + 2 * n);
+                           tile_geom(c->g), c->g, c->fact, (long)TC * c->act_cap, c->tile_mask, buf, buf + n, buf + 2 * n);
         if (hipGetLastError() != hipSuccess) rc = IBMDEM_ERR_HIP;
     }
     double *dst[3] = {fx, fy, fz};
