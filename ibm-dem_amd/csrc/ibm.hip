@@ -566,6 +566,7 @@ __device__ __forceinline__ void wave_sync() {
 // load never waits for it.
 struct EntryIn {
     double x0, x1, x2, f0, f1, f2;
+    double2 fw[12]; // the marker's delta factors (k_records)
     int cell;       // the marker's cell (linear)
     uint32_t en;    // marker << SBITS | image code
     bool valid;     // the lane stages an entry
@@ -583,6 +584,9 @@ __device__ __forceinline__ void load_entry_rec(const SpreadArgs &a, EntryIn &in)
         in.x0 = r01.x; in.x1 = r01.y; in.x2 = r23.x;
         in.cell = (int)__double_as_longlong(r23.y);
         in.f0 = r45.x; in.f1 = r45.y; in.f2 = a.rec[8 * m + 6];
+        const double2 *mf = reinterpret_cast<const double2 *>(a.fac + 24 * m);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) in.fw[q] = mf[q];
     }
 }
 
@@ -605,14 +609,14 @@ __device__ __forceinline__ void stage_entry(const SpreadArgs &a, const EntryIn &
     if (P2) s_fv[lane] = make_double4(in.f0 * a.idV, in.f1 * a.idV, in.f2 * a.idV, 0.0);
     else s_fv[lane] = make_double4(in.f0, in.f1, in.f2, 0.0);
     int cim[3];
-    const long m = (long)(in.en >> SBITS);
-    const double2 *mf = reinterpret_cast<const double2 *>(a.fac + 24 * m);
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
         const uint32_t c = (code >> (3 * d)) & 7u;
         if (c == 0u) {
             // the marker itself: its factors from k_records
             cim[d] = cc[d];
-            for (int o = 0; o < 4; ++o) s_w[lane][d * 4 + o] = mf[d * 4 + o];
+#pragma unroll
+            for (int o = 0; o < 4; ++o) s_w[lane][d * 4 + o] = in.fw[d * 4 + o];
             continue;
         }
         double xp = xm[d];
@@ -657,13 +661,19 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     // the lane's offset in an entry's 4x4x4 support
     const int oa = lane & 3, ob = (lane >> 2) & 3, oc = lane >> 4;
     wave_sync();
+    // Two-stage prefetch: while chunk c is summed, the records and factors
+    // of chunk c+1 and the entry ids of chunk c+2 are in flight.
+    EntryIn in, nx;
+    load_entry_id(a, elo + lane, lane < min(CAP, ehi - elo), in);
+    load_entry_rec(a, in);
+    load_entry_id(a, elo + CAP + lane, lane < min(CAP, ehi - elo - CAP), nx);
     for (int e0 = elo; e0 < ehi; e0 += CAP) {
         const int ne = min(CAP, ehi - e0);
-        EntryIn in;
-        load_entry_id(a, e0 + lane, lane < ne, in);
-        load_entry_rec(a, in);
         stage_entry<P2>(a, in, s_w, s_cc, s_fv);
         wave_sync();
+        in = nx;
+        load_entry_rec(a, in);
+        load_entry_id(a, e0 + 2 * CAP + lane, lane < min(CAP, ehi - e0 - 2 * CAP), nx);
         for (int e = 0; e < ne; ++e) {
             const int4 cc = s_cc[e];
             const int lx = cc.x - 1 + oa - i0, ly = cc.y - 1 + ob - j0, lz = cc.z - 1 + oc - k0;
@@ -685,22 +695,43 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     if (a.first && lane < IBM_MASK_WORDS)
         a.mask[(long)slot * IBM_MASK_WORDS + lane] =
             ((unsigned long long)s_touch[2 * lane + 1] << 32) | (unsigned long long)s_touch[2 * lane];
-    // multi-direct-forcing update u += dt*(f - f_old) on the reached cells
+    // multi-direct-forcing update u += dt*(f - f_old) on the reached cells:
+    // the lane's 8 cells (one per z plane of the tile) in two passes, every
+    // load of a pass issued before any is used
     double *vel[3] = {a.u, a.v, a.w};
     const long fbase = (long)slot * TC;
-    for (int q = lane; q < TC; q += 64) {
-        if (!((s_touch[q >> 5] >> (q & 31)) & 1u)) continue;
-        const int i = i0 + (q & (TL - 1)), j = j0 + ((q / TL) & (TL - 1)), k = k0 + q / (TL * TL);
-        if (i > g.nx || j > g.ny || k > g.nz) continue;
-        const int tgt[3] = {i, j, k};
-        const long o = cidx(g, i, j, k);
-        for (int c = 0; c < 3; ++c) {
-            const double fn = s_acc[c][q];
-            const double fo = a.first ? 0.0 : a.fact[c * a.fstride + fbase + q];
-            a.fact[c * a.fstride + fbase + q] = fn;
-            // the low boundary faces (wall, inflow) keep their fixed values
-            const bool face = tgt[c] == 1 && g.bt[2 * c] != IBMDEM_BC_PERIODIC;
-            if (!face && !(fn == 0.0 && fo == 0.0)) vel[c][o] = vel[c][o] + a.dt * (fn - fo);
+    const int li = lane & (TL - 1), lj = lane >> 3;
+    const int i = i0 + li, j = j0 + lj;
+    constexpr int HALF = TC / 64 / 2;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        bool has[HALF];
+        double fo[HALF][3], vo[HALF][3];
+        long o[HALF];
+#pragma unroll
+        for (int r = 0; r < HALF; ++r) {
+            const int q = lane + 64 * (h * HALF + r), k = k0 + h * HALF + r;
+            has[r] = ((s_touch[q >> 5] >> (q & 31)) & 1u) && i <= g.nx && j <= g.ny && k <= g.nz;
+            o[r] = has[r] ? cidx(g, i, j, k) : 0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                fo[r][c] = (has[r] && !a.first) ? a.fact[c * a.fstride + fbase + q] : 0.0;
+                vo[r][c] = has[r] ? vel[c][o[r]] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < HALF; ++r) {
+            if (!has[r]) continue;
+            const int q = lane + 64 * (h * HALF + r), k = k0 + h * HALF + r;
+            const int tgt[3] = {i, j, k};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double fn = s_acc[c][q];
+                a.fact[c * a.fstride + fbase + q] = fn;
+                // the low boundary faces (wall, inflow) keep their fixed values
+                const bool face = tgt[c] == 1 && g.bt[2 * c] != IBMDEM_BC_PERIODIC;
+                if (!face && !(fn == 0.0 && fo[r][c] == 0.0)) vel[c][o[r]] = vo[r][c] + a.dt * (fn - fo[r][c]);
+            }
         }
     }
 }
