@@ -46,7 +46,7 @@ constexpr int TL = IBM_TL;    // tile edge (cells)
 constexpr int TC = IBM_TC;    // cells per tile
 constexpr int SRC = TL + 3;   // source cells per tile edge
 constexpr int NSRC = SRC * SRC * SRC;
-constexpr int CAP = 32;       // entries staged per pass
+constexpr int CAP = 16;       // entries staged per pass (fewer: less LDS, more resident waves)
 constexpr int WST2 = 13;      // LDS row stride of the 12 factor pairs (odd: fewer bank conflicts)
 constexpr int SBITS = 9;      // image-code bits of an entry (3 per dimension)
 
