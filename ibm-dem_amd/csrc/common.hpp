@@ -527,6 +527,7 @@ int fluid_save_uold(ibmdem_ctx *c);
 int fluid_predictor_general(ibmdem_ctx *c, double a, double b, double cc);
 // poisson.hip
 int poisson_setup(ibmdem_ctx *c);
+long poisson_partials(int nx, int ny, int nz);
 int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info);
 int poisson_begin(ibmdem_ctx *c);
 int poisson_iterations(ibmdem_ctx *c, int first, int last);

@@ -175,30 +175,59 @@ __device__ __forceinline__ void parents(int f, int nc, int lo, int hi, int &p0, 
     p0 = cc; p1 = nb;
 }
 
-// x_f += sum (wz*wy*wx) x_c over the 8 parents, weights (3/4 own, 1/4 neighbour)
+// coarse index of a virtual coarse coordinate: one step beyond a wall side
+// clamps to the edge (a fine cell's neighbour parent there is its own
+// parent, see parents), a periodic side wraps. Coordinates further out only
+// occur in the unused part of a tile and are mapped to some valid cell.
+__device__ __forceinline__ int tile_coord(int v, int n, int lo, int hi) {
+    if (v < 0) return lo != IBMDEM_BC_PERIODIC ? 0 : ((v % n) + n) % n;
+    if (v >= n) return hi != IBMDEM_BC_PERIODIC ? n - 1 : v % n;
+    return v;
+}
+
+// x_f += sum (wz*wy*wx) x_c over the 8 parents, weights (3/4 own, 1/4
+// neighbour). The block's coarse parents (its x-pairs' coarse columns +-1,
+// its rows' coarse rows +-1, the plane's two parent planes) are staged in LDS
+// once, so each coarse value is read from memory once per block instead of
+// by every fine cell it weighs into; the sum is the same 8 products in the
+// same order.
+constexpr int PRO_TILE = 1024;   // (bx + 2) * (by / 2 + 2) * 2 <= 528 for the shapes of shape_for
 template <typename T>
 __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, LevelDev Cl,
                                                 const T *__restrict__ xc, const double *__restrict__ stop) {
     if (*stop != 0.0) return;
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
-    if (j >= F.ny) return;
-    int pj[2], pk[2];
-    parents(j, Cl.ny, Cl.bt[2], Cl.bt[3], pj[0], pj[1]);
+    __shared__ T tile[PRO_TILE];
+    const int bx = blockDim.x, by = blockDim.y;
+    const int TX = bx + 2, TY = by / 2 + 2;
+    const int I0 = blockIdx.x * bx - 1, J0 = (blockIdx.y * by) / 2 - 1;
+    const int k = blockIdx.z;
+    int pk[2];
     parents(k, Cl.nz, Cl.bt[4], Cl.bt[5], pk[0], pk[1]);
+    const int t = threadIdx.y * bx + threadIdx.x;
+    for (int q = t; q < 2 * TY * TX; q += bx * by) {
+        const int x = q % TX, y = (q / TX) % TY, c = q / (TX * TY);
+        const int I = tile_coord(I0 + x, Cl.nx, Cl.bt[0], Cl.bt[1]);
+        const int J = tile_coord(J0 + y, Cl.ny, Cl.bt[2], Cl.bt[3]);
+        tile[q] = xc[lidx(Cl, I, J, pk[c])];
+    }
+    __syncthreads();
+    const int ip = blockIdx.x * bx + threadIdx.x;
+    const int j = blockIdx.y * by + threadIdx.y;
+    if (j >= F.ny) return;
     const T wt[2] = {T(0.75), T(0.25)};
+    // tile rows of the own and the neighbour coarse row
+    const int ty[2] = {j / 2 - J0, j / 2 + ((j & 1) ? 1 : -1) - J0};
     // the thread's x-pair (2p, 2p+1), one cell of each colour
     for (int h = 0; h < 2; ++h) {
         const int i = 2 * ip + h;
         if (i >= F.nx) return;
-        int pi[2];
-        parents(i, Cl.nx, Cl.bt[0], Cl.bt[1], pi[0], pi[1]);
+        const int tx[2] = {i / 2 - I0, i / 2 + ((i & 1) ? 1 : -1) - I0};
         T s = T(0);
         for (int c3 = 0; c3 < 2; ++c3)
             for (int c2 = 0; c2 < 2; ++c2)
                 for (int c1 = 0; c1 < 2; ++c1) {
                     T w = wt[c3] * wt[c2] * wt[c1];
-                    s = s + w * xc[lidx(Cl, pi[c1], pj[c2], pk[c3])];
+                    s = s + w * tile[(c3 * TY + ty[c2]) * TX + tx[c1]];
                 }
         const int o = lidx(F, i, j, k);
         xf[o] = xf[o] + s;
@@ -315,25 +344,35 @@ __device__ __forceinline__ double block_reduce(double s, double *red) {
     return red[0];
 }
 
-// Ap = A p, partial sums of p.Ap (chunked: block b owns a contiguous range)
-__global__ __launch_bounds__(RED_T) void k_matvec_dot(LevelDev L, const double *__restrict__ p,
-                                                      double *__restrict__ Ap, double *__restrict__ partial,
-                                                      const double *__restrict__ scal) {
+// Ap = A p and one partial sum of p.Ap per block. Blocks, x-pairs and
+// planes as k_residual: a block's plane neighbours are read by the block
+// gridDim.x * gridDim.y positions away, a multiple of the XCD count for the
+// level-0 shapes, so they are served by the same L2.
+__global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__restrict__ p,
+                                                   double *__restrict__ Ap, double *__restrict__ partial,
+                                                   const double *__restrict__ scal) {
     if (scal[SC_CONV] != 0.0) return;
-    __shared__ double red[RED_T];
-    const long n = (long)L.nx * L.ny * L.nz;
-    const long chunk = (n + gridDim.x - 1) / gridDim.x;
-    const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+    __shared__ double red[RB];
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
     double s = 0.0;
-    for (long q = lo + threadIdx.x; q < hi; q += RED_T) {
-        int i, j, k;
-        lcell(L, (int)q, i, j, k);
-        const double a = L.diag[diag_index(L, i, j, k)] * p[q] - nbsum(p, L, i, j, k);
-        Ap[q] = a;
-        s += p[q] * a;
+    if (j < L.ny)
+        for (int h = 0; h < 2; ++h) {
+            const int i = 2 * ip + h;
+            if (i >= L.nx) break;
+            const int o = lidx(L, i, j, k);
+            const double a = L.diag[diag_index(L, i, j, k)] * p[o] - nbsum(p, L, i, j, k);
+            Ap[o] = a;
+            s += p[o] * a;
+        }
+    const int nt = blockDim.x * blockDim.y, t = threadIdx.y * blockDim.x + threadIdx.x;
+    red[t] = s;
+    __syncthreads();
+    for (int st = nt / 2; st > 0; st >>= 1) {
+        if (t < st) red[t] += red[t + st];
+        __syncthreads();
     }
-    double t = block_reduce(s, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (t == 0) partial[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
 }
 
 // x += alpha p; r -= alpha Ap; partial sums of r.r
@@ -503,6 +542,10 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
     CHECK_LAUNCH(c);
     int rc = vcycle<T>(c, lev, l + 1);
     if (rc) return rc;
+    if (rb.block.y % 2 || (rb.block.x + 2) * (rb.block.y / 2 + 2) * 2 > (unsigned)PRO_TILE) {
+        return set_error(c, IBMDEM_ERR_ARG, "prolongation tile of " + std::to_string(rb.block.x) + "x" +
+                                                std::to_string(rb.block.y) + " x-pairs does not fit");
+    }
     hipLaunchKernelGGL((k_prolong<T>), rb.grid, rb.block, 0, s, d, L.x, C.d, C.x, c->scal + SC_CONV);
     CHECK_LAUNCH(c);
     for (int q = 0; q < c->cs.mg_post; ++q) {
@@ -549,6 +592,12 @@ int read_scalars(ibmdem_ctx *c) {
 
 }  // namespace
 
+// partial sums the level-0 matrix-vector product writes (one per block)
+long poisson_partials(int nx, int ny, int nz) {
+    const LaunchShape mv = shape_for(div_up(nx, 2), ny, nz);
+    return (long)mv.grid.x * mv.grid.y * mv.grid.z;
+}
+
 int poisson_setup(ibmdem_ctx *c) {
     int bt[6];
     for (int s = 0; s < 6; ++s) bt[s] = c->g.bt[s];
@@ -592,12 +641,13 @@ int cg_iteration(ibmdem_ctx *c, int it) {
     hipStream_t s = c->stream;
     int rc;
     const LevelDev &L0 = c->lev64[0].d;
+    const LaunchShape mv = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);
+    const int mv_blocks = (int)(mv.grid.x * mv.grid.y * mv.grid.z);
     prof_begin(c, IBMDEM_K_MATVEC);
-    hipLaunchKernelGGL(k_matvec_dot, dim3(RED_BLOCKS), dim3(RED_T), 0, s, L0, c->cg_p, c->cg_Ap, c->partial,
-                       c->scal);
+    hipLaunchKernelGGL(k_matvec_dot, mv.grid, mv.block, 0, s, L0, c->cg_p, c->cg_Ap, c->partial, c->scal);
     prof_end(c, IBMDEM_K_MATVEC, 2.0 * 8.0 * (double)n);
     CHECK_LAUNCH(c);
-    hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_PAP, 1, it,
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_T), 0, s, c->partial, mv_blocks, c->scal, SC_PAP, 1, it,
                        c->cs.poisson_rtol);
     CHECK_LAUNCH(c);
     prof_begin(c, IBMDEM_K_CG_UPDATE);
