@@ -23,6 +23,8 @@ ERRORS = {
     7: "device allocation failed",
     8: "unsupported",
 }
+(ERR_HIP, ERR_ARG, ERR_CAPACITY, ERR_OUTSIDE, ERR_NOT_CONVERGED, ERR_NAN, ERR_ALLOC,
+ ERR_UNSUPPORTED) = range(1, 9)
 BC_PERIODIC = 0
 BC_WALL = 1
 BC_INFLOW = 2

@@ -12,7 +12,8 @@ import numpy as np
 import pytest
 
 from ibmdem import cases
-from ibmdem.abi import BC_OUTFLOW
+from ibmdem.abi import (BC_INFLOW, BC_OUTFLOW, BC_PERIODIC, BC_WALL, ERR_NOT_CONVERGED, IbmdemError,
+                        Simulation)
 
 pytestmark = pytest.mark.gpu
 
@@ -147,6 +148,38 @@ def test_poisson_same_rhs(pair):
     pg, _ = g.download_poisson()
     pr, _ = r.download_poisson()
     assert rel(pg, pr) < 1e-9
+
+
+_P, _W, _I, _O = BC_PERIODIC, BC_WALL, BC_INFLOW, BC_OUTFLOW
+
+
+@pytest.mark.parametrize("n,bt", [
+    ((48, 40, 36), [_P] * 6),
+    ((40, 24, 20), [_W] * 6),
+    ((64, 16, 48), [_P, _P, _W, _W, _W, _W]),
+    ((36, 28, 40), [_W, _W, _W, _W, _I, _O]),
+    ((40, 32, 24), [_I, _O, _W, _W, _P, _P]),
+], ids=["periodic", "walls", "channel", "open-z", "open-x"])
+def test_one_preconditioned_iteration(gpu_lib, oracle_lib, n, bt):
+    """One CG iteration from x = 0 is x = (r.z / p.Ap) z with z = one
+    V-cycle applied to the right-hand side, so the iterate exposes the
+    V-cycle (smoother, residual, restriction, prolongation, coarsest solve)
+    up to the rounding of two inner products summed in different orders.
+    Level sizes here leave partial coarse tiles in every dimension."""
+    case = cases.build_case(*n, bt, rtol=1e-30, max_iter=1)
+    rng = np.random.default_rng(5)
+    rhs = rng.normal(size=(n[2], n[1], n[0]))
+    rhs -= rhs.mean()
+    out = []
+    for lib in (gpu_lib, oracle_lib):
+        with Simulation(lib, case) as s:
+            s.upload_poisson_rhs(rhs)
+            with pytest.raises(IbmdemError) as e:
+                s.poisson()
+            assert e.value.code == ERR_NOT_CONVERGED
+            out.append(s.download_poisson()[0])
+    assert np.abs(out[1]).max() > 0
+    assert rel(out[0], out[1]) < 1e-12
 
 
 def test_contact_and_bodies(pair):
