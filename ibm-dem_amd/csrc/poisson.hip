@@ -140,45 +140,52 @@ __device__ __forceinline__ T nbsum_pair(const T *__restrict__ oth, const PairGeo
 }
 
 // one red-black half sweep; each thread owns the cell of the colour in a
-// pair of x cells. zero_init: the level's x is implicitly 0 (first sweep of
-// a V-cycle visit); the other colour is written 0.
+// pair of x cells, in RB_ROWS rows blockDim.y apart (independent cells, so
+// their loads are in flight together). zero_init: the level's x is
+// implicitly 0 (first sweep of a V-cycle visit); the other colour is
+// written 0.
+constexpr int RB_ROWS = 2;
 template <typename T, typename TB>
 __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
                                                 int color, int zero_init, const double *__restrict__ stop) {
     if (*stop != 0.0) return;
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
-    if (j >= L.ny) return;
-    if (L.split) {
-        if (ip >= (L.nx >> 1)) return;
-        const PairGeo g = pair_geo(L, ip, j, k);
-        const int h = g.ca == color ? 0 : 1;   // which cell of the pair has this colour
-        T *own = x + (long)color * L.half;
-        T *oth = x + (long)(1 - color) * L.half;
-        const int o = g.pos;
-        const T id = T(L.idiag[(h ? g.cx1 : g.cx0) * 16 + g.cyz]);
-        const TB bo = b[(long)color * L.half + o];
-        if (zero_init) {
-            oth[o] = T(0);
-            own[o] = (T(bo) + T(0)) * id;
-        } else {
-            own[o] = (T(bo) + nbsum_pair(oth, g, h, L)) * id;
+    const int k = blockIdx.z;
+#pragma unroll
+    for (int rr = 0; rr < RB_ROWS; ++rr) {
+        const int j = (blockIdx.y * RB_ROWS + rr) * blockDim.y + threadIdx.y;
+        if (j >= L.ny) return;
+        if (L.split) {
+            if (ip >= (L.nx >> 1)) return;
+            const PairGeo g = pair_geo(L, ip, j, k);
+            const int h = g.ca == color ? 0 : 1;   // which cell of the pair has this colour
+            T *own = x + (long)color * L.half;
+            T *oth = x + (long)(1 - color) * L.half;
+            const int o = g.pos;
+            const T id = T(L.idiag[(h ? g.cx1 : g.cx0) * 16 + g.cyz]);
+            const TB bo = b[(long)color * L.half + o];
+            if (zero_init) {
+                oth[o] = T(0);
+                own[o] = (T(bo) + T(0)) * id;
+            } else {
+                own[o] = (T(bo) + nbsum_pair(oth, g, h, L)) * id;
+            }
+            continue;
         }
-        return;
-    }
-    const int i = 2 * ip + ((j + k + color) & 1);
-    const int o = lidx(L, i, j, k);
-    if (zero_init) {
-        const int io = 2 * ip + ((j + k + color + 1) & 1);
-        if (io < L.nx) x[lidx(L, io, j, k)] = T(0);
-        if (i >= L.nx) return;
+        const int i = 2 * ip + ((j + k + color) & 1);
+        const int o = lidx(L, i, j, k);
+        if (zero_init) {
+            const int io = 2 * ip + ((j + k + color + 1) & 1);
+            if (io < L.nx) x[lidx(L, io, j, k)] = T(0);
+            if (i >= L.nx) continue;
+            const T id = T(L.idiag[diag_index(L, i, j, k)]);
+            x[o] = (T(b[o]) + T(0)) * id;
+            continue;
+        }
+        if (i >= L.nx) continue;
         const T id = T(L.idiag[diag_index(L, i, j, k)]);
-        x[o] = (T(b[o]) + T(0)) * id;
-        return;
+        x[o] = (T(b[o]) + nbsum(x, L, i, j, k)) * id;
     }
-    if (i >= L.nx) return;
-    const T id = T(L.idiag[diag_index(L, i, j, k)]);
-    x[o] = (T(b[o]) + nbsum(x, L, i, j, k)) * id;
 }
 
 // r = b - A x. Each thread owns the x-pair (2p, 2p+1): one cell of each
@@ -655,9 +662,10 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
     // level-0 half-sweeps are the profiled (dominant) kernel: each reads the
     // right-hand side and the other colour of half the cells and writes them
     const double hb = l == 0 ? 1.5 * sizeof(T) * (double)L.n : 0.0;
+    const dim3 rbh(rb.grid.x, div_up(d.ny, RB_ROWS * (int)rb.block.y), rb.grid.z);
     auto half = [&](int color, int zero) {
         if (l == 0) prof_begin(c, IBMDEM_K_MG_SMOOTH);
-        hipLaunchKernelGGL((k_rb_half<T, T>), rb.grid, rb.block, 0, s, d, L.x, L.b, color, zero, c->scal + SC_CONV);
+        hipLaunchKernelGGL((k_rb_half<T, T>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero, c->scal + SC_CONV);
         if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, hb);
     };
     for (int q = 0; q < c->cs.mg_pre; ++q) {
