@@ -101,7 +101,7 @@ KERNEL_NAMES = {
     "spread": "k_spread_tiles (IBM spread + multi-direct-forcing update)",
     "mg_smooth": "k_rb_half (level-0 red-black Gauss-Seidel half-sweep)",
     "interp": "k_interp (IBM interpolation)",
-    "predictor": "k_predictor (fused advection-diffusion predictor)",
+    "predictor": "k_predictor_lds (fused advection-diffusion predictor, LDS z-marching)",
 }
 
 

@@ -197,7 +197,11 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__res
                                               double *__restrict__ rec, const double *__restrict__ mv,
                                               double *__restrict__ fdrag, const double *__restrict__ mvol, double dV,
                                               double idV, double dt) {
-    const int t = blockIdx.x * T + threadIdx.x;
+    // consecutive bin-ordered markers (one body's neighbourhood) stay on one
+    // XCD, so the field taps they share come from one L2
+    const int slot = tile_slot(blockIdx.x, (M + T - 1) / T);
+    if (slot < 0) return;
+    const int t = slot * T + threadIdx.x;
     if (t >= M) return;
     const int m = order[t];
     double *r = rec + 8L * m;
@@ -832,7 +836,8 @@ int ibm_interpolate(ibmdem_ctx *c) {
     const int cur = c->vcur;
     prof_begin(c, IBMDEM_K_INTERP);
 #define LAUNCH_INTERP(P2, OP)                                                                                      \
-    hipLaunchKernelGGL((k_interp<P2, OP>), dim3(div_up(c->M, T)), dim3(T), 0, c->stream, c->M, c->g, c->bin_items,  \
+    hipLaunchKernelGGL((k_interp<P2, OP>), dim3(8 * div_up(div_up(c->M, T), 8)), dim3(T), 0, c->stream, c->M, c->g, \
+                       c->bin_items,                                                                                  \
                        c->mfac, c->u[cur],                                                                            \
                        c->v[cur], c->w[cur], c->mrec, c->mv, c->fdrag, c->mvol, c->dV, 1.0 / c->dV, c->cs.dt)
     const bool open = grid_open(c->g);
