@@ -13,9 +13,10 @@
 //     and ascending id inside a cell, each packed as marker << 9 | image
 //     code (3 bits per dimension: direct, periodic copy below or above,
 //     mirror copy below or above).
-// The delta factors (per dimension: staggered-face and cell-centre variants
-// at cell offsets -1..+2) are recomputed from the record where they are used;
-// the arithmetic is the same each time, so the results are too.
+//   - every marker's one-dimensional delta factors (per dimension:
+//     staggered-face and cell-centre variants at cell offsets -1..+2) are
+//     computed once and read by the interpolation and the spread of all
+//     five iterations.
 //
 // Interpolation is marker-parallel and sums the taps in the reference's
 // (kk,jj,ii) order, so it is bit-identical to the reference's 125-cell loop
