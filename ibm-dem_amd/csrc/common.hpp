@@ -469,6 +469,7 @@ enum {
     SC_CONV = 15,     // 1 once ||r|| <= tol: every later CG kernel returns at once
     SC_TOL = 16,      // rtol * ||b||
     SC_ITER = 17,     // iteration at which SC_CONV was set
+    SC_DONE = 18,     // last CG iteration that ran (its x update is deferred)
     SC_COUNT = 32
 };
 enum { FL_CAPACITY = 0, FL_OUTSIDE = 1, FL_NAN = 2, FL_COUNT = 8 };

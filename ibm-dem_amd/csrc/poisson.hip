@@ -31,6 +31,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace {
 
@@ -47,6 +48,9 @@ int pow2ceil(int v) {
     while (p < v) p <<= 1;
     return p;
 }
+LaunchShape shape_for(int nx, int ny, int nz);
+// grid of a red-black half sweep (RB_ROWS rows per thread) on an nx*ny*nz level
+dim3 half_grid(int nx, int ny, int nz);
 LaunchShape shape_for(int nx, int ny, int nz) {
     int bx = std::min(64, pow2ceil(nx));
     int by = std::min(RB / bx, pow2ceil(ny));
@@ -54,9 +58,10 @@ LaunchShape shape_for(int nx, int ny, int nz) {
 }
 
 // nbsum in the oracle's association: ((xw+xe)*ihx2 + (xs+xn)*ihy2) + (xb+xt)*ihz2,
-// wall neighbours 0, periodic neighbours wrapped
-template <typename T>
-__device__ __forceinline__ T nbsum(const T *__restrict__ x, const LevelDev &L, int i, int j, int k) {
+// wall neighbours 0, periodic neighbours wrapped; x(o) is the value at
+// storage position o
+template <typename T, typename F>
+__device__ __forceinline__ T nbsum_f(const F &x, const LevelDev &L, int i, int j, int k) {
     const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
     const bool pxw = L.bt[0] == IBMDEM_BC_PERIODIC, pyw = L.bt[2] == IBMDEM_BC_PERIODIC,
                pzw = L.bt[4] == IBMDEM_BC_PERIODIC;
@@ -66,13 +71,17 @@ __device__ __forceinline__ T nbsum(const T *__restrict__ x, const LevelDev &L, i
     const int iw = i > 0 ? i - 1 : L.nx - 1, ie = i < L.nx - 1 ? i + 1 : 0;
     const int js = j > 0 ? j - 1 : L.ny - 1, jn = j < L.ny - 1 ? j + 1 : 0;
     const int kb = k > 0 ? k - 1 : L.nz - 1, kt = k < L.nz - 1 ? k + 1 : 0;
-    const T xw = hw ? x[lidx(L, iw, j, k)] : T(0), xe = he ? x[lidx(L, ie, j, k)] : T(0);
-    const T xs = hs ? x[lidx(L, i, js, k)] : T(0), xn = hn ? x[lidx(L, i, jn, k)] : T(0);
-    const T xb = hb ? x[lidx(L, i, j, kb)] : T(0), xt = ht ? x[lidx(L, i, j, kt)] : T(0);
+    const T xw = hw ? x(lidx(L, iw, j, k)) : T(0), xe = he ? x(lidx(L, ie, j, k)) : T(0);
+    const T xs = hs ? x(lidx(L, i, js, k)) : T(0), xn = hn ? x(lidx(L, i, jn, k)) : T(0);
+    const T xb = hb ? x(lidx(L, i, j, kb)) : T(0), xt = ht ? x(lidx(L, i, j, kt)) : T(0);
     const T sx = (xw + xe) * ihx2;
     const T syy = (xs + xn) * ihy2;
     const T szz = (xb + xt) * ihz2;
     return sx + syy + szz;
+}
+template <typename T>
+__device__ __forceinline__ T nbsum(const T *__restrict__ x, const LevelDev &L, int i, int j, int k) {
+    return nbsum_f<T>([&](int o) { return x[o]; }, L, i, j, k);
 }
 
 // per-dimension diagonal count: 1 per present neighbour, 2 for an outflow
@@ -117,46 +126,54 @@ __device__ __forceinline__ PairGeo pair_geo(const LevelDev &L, int p, int j, int
     g.cyz = diag_count(j, L.ny, L.bt[2], L.bt[3]) * 4 + diag_count(k, L.nz, L.bt[4], L.bt[5]);
     return g;
 }
-// nbsum of the pair's even (h = 0) or odd (h = 1) cell; oth is the other
-// cell's colour half
-template <typename T>
-__device__ __forceinline__ T nbsum_pair(const T *__restrict__ oth, const PairGeo &g, int h, const LevelDev &L) {
+// nbsum of the pair's even (h = 0) or odd (h = 1) cell; q(d) is the value
+// at offset d from the pair's position in the other cell's colour half
+template <typename T, typename F>
+__device__ __forceinline__ T nbsum_pair_f(const F &q, const PairGeo &g, int h, const LevelDev &L) {
     const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
-    const T *q = oth + g.pos;
     T xw, xe;
     if (h == 0) {
-        xw = g.hw ? q[g.dw] : T(0);
-        xe = q[0];
+        xw = g.hw ? q(g.dw) : T(0);
+        xe = q(0);
     } else {
-        xw = q[0];
-        xe = g.he ? q[g.de] : T(0);
+        xw = q(0);
+        xe = g.he ? q(g.de) : T(0);
     }
-    const T xs = g.hs ? q[g.ds] : T(0), xn = g.hn ? q[g.dn] : T(0);
-    const T xb = g.hb ? q[g.db] : T(0), xt = g.ht ? q[g.dt] : T(0);
+    const T xs = g.hs ? q(g.ds) : T(0), xn = g.hn ? q(g.dn) : T(0);
+    const T xb = g.hb ? q(g.db) : T(0), xt = g.ht ? q(g.dt) : T(0);
     const T sx = (xw + xe) * ihx2;
     const T syy = (xs + xn) * ihy2;
     const T szz = (xb + xt) * ihz2;
     return sx + syy + szz;
+}
+template <typename T>
+__device__ __forceinline__ T nbsum_pair(const T *__restrict__ oth, const PairGeo &g, int h, const LevelDev &L) {
+    const T *q = oth + g.pos;
+    return nbsum_pair_f<T>([&](int d) { return q[d]; }, g, h, L);
 }
 
 // one red-black half sweep; each thread owns the cell of the colour in a
 // pair of x cells, in RB_ROWS rows blockDim.y apart (independent cells, so
 // their loads are in flight together). zero_init: the level's x is
 // implicitly 0 (first sweep of a V-cycle visit); the other colour is
-// written 0.
+// written 0. DOT (the last two level-0 sweeps of the preconditioner, whose
+// b is the CG residual r and whose x is z = M^-1 r): also one partial sum of
+// b.x over the block's updated cells into partial[pofs + block].
 constexpr int RB_ROWS = 2;
-template <typename T, typename TB>
+template <typename T, typename TB, bool DOT = false>
 __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
-                                                int color, int zero_init, const double *__restrict__ stop) {
+                                                int color, int zero_init, const double *__restrict__ stop,
+                                                double *__restrict__ partial = nullptr, long pofs = 0) {
     if (*stop != 0.0) return;
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.z;
+    double dsum = 0.0;
 #pragma unroll
     for (int rr = 0; rr < RB_ROWS; ++rr) {
         const int j = (blockIdx.y * RB_ROWS + rr) * blockDim.y + threadIdx.y;
-        if (j >= L.ny) return;
+        if (j >= L.ny) continue;
         if (L.split) {
-            if (ip >= (L.nx >> 1)) return;
+            if (ip >= (L.nx >> 1)) continue;
             const PairGeo g = pair_geo(L, ip, j, k);
             const int h = g.ca == color ? 0 : 1;   // which cell of the pair has this colour
             T *own = x + (long)color * L.half;
@@ -168,7 +185,9 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
                 oth[o] = T(0);
                 own[o] = (T(bo) + T(0)) * id;
             } else {
-                own[o] = (T(bo) + nbsum_pair(oth, g, h, L)) * id;
+                const T xn = (T(bo) + nbsum_pair(oth, g, h, L)) * id;
+                own[o] = xn;
+                if (DOT) dsum += double(bo) * double(xn);
             }
             continue;
         }
@@ -184,8 +203,26 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
         }
         if (i >= L.nx) continue;
         const T id = T(L.idiag[diag_index(L, i, j, k)]);
-        x[o] = (T(b[o]) + nbsum(x, L, i, j, k)) * id;
+        const T xn = (T(b[o]) + nbsum(x, L, i, j, k)) * id;
+        x[o] = xn;
+        if (DOT) dsum += double(b[o]) * double(xn);
     }
+    if (DOT) {
+        __shared__ double red[RB];
+        const int nt = blockDim.x * blockDim.y, t = threadIdx.y * blockDim.x + threadIdx.x;
+        red[t] = dsum;
+        __syncthreads();
+        for (int st = nt / 2; st > 0; st >>= 1) {
+            if (t < st) red[t] += red[t + st];
+            __syncthreads();
+        }
+        if (t == 0) partial[pofs + ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+    }
+}
+
+dim3 half_grid(int nx, int ny, int nz) {
+    const LaunchShape rb = shape_for(div_up(nx, 2), ny, nz);
+    return dim3(rb.grid.x, div_up(ny, RB_ROWS * (int)rb.block.y), rb.grid.z);
 }
 
 // r = b - A x. Each thread owns the x-pair (2p, 2p+1): one cell of each
@@ -501,10 +538,23 @@ __global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__r
     if (t == 0) partial[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
 }
 
-// x += alpha p; r -= alpha Ap; partial sums of r.r
-__global__ __launch_bounds__(RED_T) void k_cg_update(long n, double *__restrict__ x, double *__restrict__ r,
-                                                     const double *__restrict__ p, const double *__restrict__ Ap,
-                                                     const double *__restrict__ scal, double *__restrict__ partial) {
+// CG iteration k >= 2, first kernel: p_k = z_k + beta_{k-1} p_{k-1} in place,
+// and the deferred x_k = x_{k-1} + alpha_{k-1} p_{k-1}
+__global__ __launch_bounds__(RED_T) void k_cg_px(long n, double *__restrict__ p, const double *__restrict__ z,
+                                                 double *__restrict__ x, const double *__restrict__ scal) {
+    if (scal[SC_CONV] != 0.0) return;
+    const double beta = scal[SC_BETA], alpha = scal[SC_ALPHA];
+    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
+        const double po = p[q];
+        p[q] = z[q] + beta * po;
+        x[q] = x[q] + alpha * po;
+    }
+}
+
+// r -= alpha Ap; partial sums of r.r (x is updated one iteration later, by
+// k_cg_px or, after the last iteration, by k_gauge)
+__global__ __launch_bounds__(RED_T) void k_cg_update_r(long n, double *__restrict__ r, const double *__restrict__ Ap,
+                                                       const double *__restrict__ scal, double *__restrict__ partial) {
     if (scal[SC_CONV] != 0.0) return;
     __shared__ double red[RED_T];
     const double alpha = scal[SC_ALPHA];
@@ -512,7 +562,6 @@ __global__ __launch_bounds__(RED_T) void k_cg_update(long n, double *__restrict_
     const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
     double s = 0.0;
     for (long q = lo + threadIdx.x; q < hi; q += RED_T) {
-        x[q] = x[q] + alpha * p[q];
         double rq = r[q] - alpha * Ap[q];
         r[q] = rq;
         s += rq * rq;
@@ -560,6 +609,7 @@ __global__ __launch_bounds__(FIN_T) void k_final(const double *__restrict__ part
         scal[slot] = t;
         if (op == 1) scal[SC_ALPHA] = scal[SC_RZ] / t;
         if (op == 2) { scal[SC_BETA] = t / scal[SC_RZ]; scal[SC_RZ] = t; }
+        if (op == 3) scal[SC_DONE] = it;
         if (op == 3 && (sqrt(t) <= scal[SC_TOL] || !(t == t))) { scal[SC_CONV] = 1.0; scal[SC_ITER] = it; }
         if (op == 4) {
             scal[SC_TOL] = rtol * sqrt(t);
@@ -568,21 +618,13 @@ __global__ __launch_bounds__(FIN_T) void k_final(const double *__restrict__ part
     }
 }
 
-// p = z + beta p
-__global__ __launch_bounds__(RED_T) void k_cg_p(long n, double *__restrict__ p, const double *__restrict__ z,
-                                                const double *__restrict__ scal) {
-    if (scal[SC_CONV] != 0.0) return;
-    const double beta = scal[SC_BETA];
-    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T)
-        p[q] = z[q] + beta * p[q];
-}
 
 // r = rhs with the pinned row, x = 0; r and x in level-0 storage, rhs in
 // field order (cell (0,0,0) is position 0 of both)
 __global__ __launch_bounds__(RED_T) void k_cg_init(LevelDev L, long n, const double *__restrict__ rhs,
                                                    double *__restrict__ r, double *__restrict__ x,
                                                    double *__restrict__ scal) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[SC_CONV] = 0.0; scal[SC_ITER] = 0.0; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[SC_CONV] = 0.0; scal[SC_ITER] = 0.0; scal[SC_DONE] = 0.0; }
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
         int i, j, k;
         lcell(L, (int)q, i, j, k);
@@ -603,13 +645,19 @@ __global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, c
 
 // gauge P'(1,1,1) = 0: pcorr = x - x(1,1,1), x in level-0 storage, pcorr in
 // field order (no shift with an outflow side, where P' = 0 on the face)
+// P' = x - x(1,1,1) in field order, x taking the last completed CG
+// iteration's deferred update x + alpha p
 __global__ __launch_bounds__(RED_T) void k_gauge(LevelDev L, long n, const double *__restrict__ x,
+                                                 const double *__restrict__ p, const double *__restrict__ scal,
                                                  double *__restrict__ pc) {
-    const double x0 = L.dirichlet ? 0.0 : x[0];
+    const int done = (int)scal[SC_DONE];
+    const double alpha = done > 0 ? scal[SC_ALPHA] : 0.0;
+    auto xv = [&](long q) { return done > 0 ? x[q] + alpha * p[q] : x[q]; };
+    const double x0 = L.dirichlet ? 0.0 : xv(0);
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
         int i, j, k;
         lcell(L, (int)q, i, j, k);
-        pc[((long)k * L.ny + j) * L.nx + i] = x[q] - x0;
+        pc[((long)k * L.ny + j) * L.nx + i] = xv(q) - x0;
     }
 }
 
@@ -644,7 +692,7 @@ LevelDev make_level(int nx, int ny, int nz, const int bt[6], const double h[3]) 
 }
 
 template <typename T>
-int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
+int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     MGLevel<T> &L = lev[l];
     const LevelDev &d = L.d;
     hipStream_t s = c->stream;
@@ -662,10 +710,20 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
     // level-0 half-sweeps are the profiled (dominant) kernel: each reads the
     // right-hand side and the other colour of half the cells and writes them
     const double hb = l == 0 ? 1.5 * sizeof(T) * (double)L.n : 0.0;
-    const dim3 rbh(rb.grid.x, div_up(d.ny, RB_ROWS * (int)rb.block.y), rb.grid.z);
-    auto half = [&](int color, int zero) {
+    const dim3 rbh = half_grid(d.nx, d.ny, d.nz);
+    const long nbh = (long)rbh.x * rbh.y * rbh.z;
+    auto half = [&](int color, int zero, int dot_slot = -1) {
         if (l == 0) prof_begin(c, IBMDEM_K_MG_SMOOTH);
-        hipLaunchKernelGGL((k_rb_half<T, T>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero, c->scal + SC_CONV);
+        if constexpr (std::is_same<T, double>::value) {
+            if (dot_slot >= 0) {
+                hipLaunchKernelGGL((k_rb_half<T, T, true>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero,
+                                   c->scal + SC_CONV, c->partial, dot_slot * nbh);
+                if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, hb);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_rb_half<T, T>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero, c->scal + SC_CONV,
+                           nullptr, 0L);
         if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, hb);
     };
     for (int q = 0; q < c->cs.mg_pre; ++q) {
@@ -688,12 +746,17 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l) {
     hipLaunchKernelGGL((k_prolong<T>), rb.grid, rb.block, 0, s, d, L.x, C.d, C.x, c->scal + SC_CONV);
     CHECK_LAUNCH(c);
     for (int q = 0; q < c->cs.mg_post; ++q) {
-        half(1, 0);
-        half(0, 0);
+        // the last post sweep at level 0 finalizes z: with dot, it also
+        // leaves the partial sums of r.z (colour 1, then colour 0)
+        const bool last = dot && l == 0 && q == c->cs.mg_post - 1;
+        half(1, 0, last ? 0 : -1);
+        half(0, 0, last ? 1 : -1);
         CHECK_LAUNCH(c);
     }
     return IBMDEM_OK;
 }
+
+int dot_into(ibmdem_ctx *c, const double *a, const double *b, int slot, int op);
 
 // z = M^{-1} r
 int precondition(ibmdem_ctx *c) {
@@ -714,6 +777,28 @@ int precondition(ibmdem_ctx *c) {
     return vcycle<double>(c, c->lev64, 0);
 }
 
+// z = M^{-1} r, then r.z into scal[slot] with k_final's op. In the FP64 form
+// with post-smoothing the level-0 sweeps that finalize z also leave the
+// partial sums of r.z (one per block and colour), so z is not read again.
+int precondition_dot(ibmdem_ctx *c, int slot, int op) {
+    if (c->cs.precision == IBMDEM_MIXED || c->cs.mg_post < 1 || c->nlev < 2) {
+        int rc = precondition(c);
+        if (rc) return rc;
+        return dot_into(c, c->cg_r, c->cg_z, slot, op);
+    }
+    c->lev64[0].b = c->cg_r;
+    c->lev64[0].x = c->cg_z;
+    int rc = vcycle<double>(c, c->lev64, 0, true);
+    if (rc) return rc;
+    const LevelDev &d = c->lev64[0].d;
+    const dim3 g = half_grid(d.nx, d.ny, d.nz);
+    const int np = 2 * (int)(g.x * g.y * g.z);
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(FIN_T), 0, c->stream, c->partial, np, c->scal, slot, op, 0,
+                       c->cs.poisson_rtol);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
 int dot_into(ibmdem_ctx *c, const double *a, const double *b, int slot, int op) {
     hipLaunchKernelGGL(k_dot, dim3(RED_BLOCKS), dim3(RED_T), 0, c->stream, c->n, a, b, c->partial, c->scal);
     CHECK_LAUNCH(c);
@@ -731,10 +816,12 @@ int read_scalars(ibmdem_ctx *c) {
 
 }  // namespace
 
-// partial sums the level-0 matrix-vector product writes (one per block)
+// partial sums the level-0 matrix-vector product (one per block) and the
+// last level-0 sweeps with r.z (one per block and colour) write
 long poisson_partials(int nx, int ny, int nz) {
     const LaunchShape mv = shape_for(div_up(nx, 2), ny, nz);
-    return (long)mv.grid.x * mv.grid.y * mv.grid.z;
+    const dim3 h = half_grid(nx, ny, nz);
+    return std::max<long>((long)mv.grid.x * mv.grid.y * mv.grid.z, 2L * h.x * h.y * h.z);
 }
 
 int poisson_setup(ibmdem_ctx *c) {
@@ -782,6 +869,11 @@ int cg_iteration(ibmdem_ctx *c, int it) {
     const LevelDev &L0 = c->lev64[0].d;
     const LaunchShape mv = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);
     const int mv_blocks = (int)(mv.grid.x * mv.grid.y * mv.grid.z);
+    if (it > 1) {
+        hipLaunchKernelGGL(k_cg_px, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z, c->cg_x,
+                           c->scal);
+        CHECK_LAUNCH(c);
+    }
     prof_begin(c, IBMDEM_K_MATVEC);
     hipLaunchKernelGGL(k_matvec_dot, mv.grid, mv.block, 0, s, L0, c->cg_p, c->cg_Ap, c->partial, c->scal);
     prof_end(c, IBMDEM_K_MATVEC, 2.0 * 8.0 * (double)n);
@@ -790,18 +882,14 @@ int cg_iteration(ibmdem_ctx *c, int it) {
                        c->cs.poisson_rtol);
     CHECK_LAUNCH(c);
     prof_begin(c, IBMDEM_K_CG_UPDATE);
-    hipLaunchKernelGGL(k_cg_update, dim3(RED_BLOCKS), dim3(RED_T), 0, s, n, c->cg_x, c->cg_r, c->cg_p, c->cg_Ap,
-                       c->scal, c->partial);
-    prof_end(c, IBMDEM_K_CG_UPDATE, 6.0 * 8.0 * (double)n);
+    hipLaunchKernelGGL(k_cg_update_r, dim3(RED_BLOCKS), dim3(RED_T), 0, s, n, c->cg_r, c->cg_Ap, c->scal,
+                       c->partial);
+    prof_end(c, IBMDEM_K_CG_UPDATE, 3.0 * 8.0 * (double)n);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_final, dim3(1), dim3(FIN_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_RR, 3, it,
                        c->cs.poisson_rtol);
     CHECK_LAUNCH(c);
-    if ((rc = precondition(c))) return rc;
-    if ((rc = dot_into(c, c->cg_r, c->cg_z, SC_RZ_NEW, 2))) return rc;
-    hipLaunchKernelGGL(k_cg_p, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z, c->scal);
-    CHECK_LAUNCH(c);
-    return IBMDEM_OK;
+    return precondition_dot(c, SC_RZ_NEW, 2);
 }
 
 }  // namespace
@@ -819,10 +907,10 @@ int poisson_begin(ibmdem_ctx *c) {
                        c->cg_x, c->scal);
     CHECK_LAUNCH(c);
     if ((rc = dot_into(c, c->cg_r, c->cg_r, SC_BNORM2, 4))) return rc;
-    if ((rc = precondition(c))) return rc;
+    if ((rc = precondition_dot(c, SC_RZ, 0))) return rc;
     hipLaunchKernelGGL(k_copy, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z);
     CHECK_LAUNCH(c);
-    return dot_into(c, c->cg_r, c->cg_z, SC_RZ, 0);
+    return IBMDEM_OK;
 }
 
 // Iterations first..last, queued without any host wait. Once the device sets
@@ -880,7 +968,7 @@ int poisson_check(ibmdem_ctx *c, ibmdem_solve_info *info) {
 // P' = x - x(1,1,1): the reference's gauge (poisson.F90:78,97-100)
 int poisson_gauge(ibmdem_ctx *c) {
     hipLaunchKernelGGL(k_gauge, dim3(stream_blocks(c->n)), dim3(RED_T), 0, c->stream, c->lev64[0].d, c->n, c->cg_x,
-                       c->pcorr);
+                       c->cg_p, c->scal, c->pcorr);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
