@@ -638,9 +638,15 @@ __device__ __forceinline__ void stage_entry(const SpreadArgs &a, const EntryIn &
 }
 
 // One wavefront per active tile (block b takes tile slot tile_slot(b)).
+// LDS position of tile cell (x, y, z): z planes 68 doubles apart, so the 32
+// lanes of a half-wavefront (support offsets 4 x 4 x 2) hit 32 different
+// 8-byte bank pairs
+constexpr int ACC_Z = TL * TL + 4, ACC_N = ACC_Z * TL;
+__device__ __forceinline__ int acc_idx(int q) { return q + 4 * (q >> 6); }
+
 template <bool P2>
 __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
-    __shared__ double s_acc[3][TC];            // the tile's force
+    __shared__ double s_acc[3][ACC_N];         // the tile's force
     __shared__ unsigned s_touch[TC / 32];      // cells some entry reached
     // per staged entry: (face, centre) factor pairs for 3 dimensions x 4
     // offsets (padded to an odd number of 16-byte pairs per entry), the cell
@@ -657,12 +663,15 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     int tx, ty, tz;
     tile_origin(a.tiles[slot], a.tg, tx, ty, tz);
     const int i0 = tx * TL + 1, j0 = ty * TL + 1, k0 = tz * TL + 1;
-    for (int q = lane; q < TC; q += 64) {
+    for (int q = lane; q < ACC_N; q += 64) {
         s_acc[0][q] = 0.0;
         s_acc[1][q] = 0.0;
         s_acc[2][q] = 0.0;
     }
-    if (lane < TC / 32) s_touch[lane] = 0u;
+    // the reached cells are found in the first iteration of a step; the
+    // markers (so the entries and their supports) do not move until the next
+    if (lane < TC / 32)
+        s_touch[lane] = a.first ? 0u : (unsigned)(a.mask[(long)slot * IBM_MASK_WORDS + (lane >> 1)] >> (32 * (lane & 1)));
     // the lane's offset in an entry's 4x4x4 support
     const int oa = lane & 3, ob = (lane >> 2) & 3, oc = lane >> 4;
     wave_sync();
@@ -688,11 +697,11 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
                 // (face, centre): u takes the x face, v the y face, w the z face
                 const double pu = wx.x * wy.y * wz.y, pv = wx.y * wy.x * wz.y, pw = wx.y * wy.y * wz.x;
                 const double wu = P2 ? pu : pu / a.dV, wv = P2 ? pv : pv / a.dV, ww = P2 ? pw : pw / a.dV;
-                const int q = lx + TL * (ly + TL * lz);
-                s_acc[0][q] = s_acc[0][q] - fv.x * wu;
-                s_acc[1][q] = s_acc[1][q] - fv.y * wv;
-                s_acc[2][q] = s_acc[2][q] - fv.z * ww;
-                atomicOr(&s_touch[q >> 5], 1u << (q & 31));
+                const int q = lx + TL * (ly + TL * lz), qa = lx + TL * ly + ACC_Z * lz;
+                s_acc[0][qa] = s_acc[0][qa] - fv.x * wu;
+                s_acc[1][qa] = s_acc[1][qa] - fv.y * wv;
+                s_acc[2][qa] = s_acc[2][qa] - fv.z * ww;
+                if (a.first) atomicOr(&s_touch[q >> 5], 1u << (q & 31));
             }
         }
         wave_sync();   // the rows are rewritten by the next chunk
@@ -731,7 +740,7 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
             const int tgt[3] = {i, j, k};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double fn = s_acc[c][q];
+                const double fn = s_acc[c][acc_idx(q)];
                 a.fact[c * a.fstride + fbase + q] = fn;
                 // the low boundary faces (wall, inflow) keep their fixed values
                 const bool face = tgt[c] == 1 && g.bt[2 * c] != IBMDEM_BC_PERIODIC;
