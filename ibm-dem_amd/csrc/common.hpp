@@ -396,6 +396,8 @@ struct ibmdem_ctx {
     // PCG (FP64)
     double *cg_x = nullptr, *cg_r = nullptr, *cg_z = nullptr, *cg_p = nullptr, *cg_Ap = nullptr;
     int nlev = 0;
+    int tail_first = IBMDEM_MAX_LEVELS;  // levels from here down run as one work-group
+    LevelDev *dlev = nullptr;            // level descriptors in device memory (for that work-group)
     MGLevel<double> lev64[IBMDEM_MAX_LEVELS];
     MGLevel<float> lev32[IBMDEM_MAX_LEVELS];
     // reductions

@@ -30,6 +30,7 @@
 #include "common.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 
@@ -43,6 +44,14 @@ constexpr int COARSE_T = 1024;
 struct LaunchShape {
     dim3 grid, block;
 };
+
+// Barrier of one wavefront over LDS: its LDS operations complete in program
+// order, so a compiler-level fence suffices.
+__device__ __forceinline__ void wave_barrier_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 int pow2ceil(int v) {
     int p = 1;
     while (p < v) p <<= 1;
@@ -483,6 +492,183 @@ __global__ __launch_bounds__(COARSE_T) void k_coarse(LevelDev L, T *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// The small levels of the hierarchy as one work-group: from level `first`
+// down to the coarsest, the whole V-cycle (smoothing, residual, restriction,
+// coarse sweeps, prolongation) runs in LDS with a barrier between phases, in
+// place of a few dozen launches that each cost more than their work. The
+// coarsest level sweeps on one wavefront from neighbour tables, as
+// k_coarse_lds does. Every cell's arithmetic is the multi-launch path's, so
+// the result is the same bit for bit; b of level `first` comes from global
+// memory and its x goes back there for the prolongation to the level above.
+// ---------------------------------------------------------------------------
+constexpr int TAIL_T = 1024;
+constexpr int TAIL_BYTES = 112 * 1024;   // x (one extra zero slot on the coarsest level), b, r of the levels
+
+template <typename T>
+__device__ __forceinline__ void tail_half(const LevelDev &L, T *x, const T *b, int color, bool zero) {
+    const int n = L.nx * L.ny * L.nz;
+    for (int q = threadIdx.x; q < n; q += TAIL_T) {
+        int i, j, k;
+        lcell(L, q, i, j, k);
+        if (((i + j + k) & 1) != color) {
+            if (zero) x[q] = T(0);
+            continue;
+        }
+        const T id = T(L.idiag[diag_index(L, i, j, k)]);
+        x[q] = zero ? (T(b[q]) + T(0)) * id : (T(b[q]) + nbsum(x, L, i, j, k)) * id;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_residual(const LevelDev &L, const T *x, const T *b, T *r) {
+    const int n = L.nx * L.ny * L.nz;
+    for (int q = threadIdx.x; q < n; q += TAIL_T) {
+        int i, j, k;
+        lcell(L, q, i, j, k);
+        const T dg = T(L.diag[diag_index(L, i, j, k)]);
+        r[q] = T(b[q]) - (dg * x[q] - nbsum(x, L, i, j, k));
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_restrict(const LevelDev &F, const T *rf, const LevelDev &Cl, T *bc) {
+    const T wt[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
+    const int n = Cl.nx * Cl.ny * Cl.nz;
+    for (int q = threadIdx.x; q < n; q += TAIL_T) {
+        int I, J, K;
+        lcell(Cl, q, I, J, K);
+        int fi[4], fj[4], fk[4];
+        for (int a = 0; a < 4; ++a) {
+            fi[a] = tile_coord(2 * I - 1 + a, F.nx, F.bt[0], F.bt[1]);
+            fj[a] = tile_coord(2 * J - 1 + a, F.ny, F.bt[2], F.bt[3]);
+            fk[a] = tile_coord(2 * K - 1 + a, F.nz, F.bt[4], F.bt[5]);
+        }
+        T s = T(0);
+        for (int c3 = 0; c3 < 4; ++c3)
+            for (int c2 = 0; c2 < 4; ++c2)
+                for (int c1 = 0; c1 < 4; ++c1) {
+                    T w = wt[c3] * wt[c2] * wt[c1];
+                    s = s + w * rf[lidx(F, fi[c1], fj[c2], fk[c3])];
+                }
+        bc[q] = T(0.125) * s;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_prolong(const LevelDev &F, T *xf, const LevelDev &Cl, const T *xc) {
+    const T wt[2] = {T(0.75), T(0.25)};
+    const int n = F.nx * F.ny * F.nz;
+    for (int q = threadIdx.x; q < n; q += TAIL_T) {
+        int i, j, k;
+        lcell(F, q, i, j, k);
+        int pi[2], pj[2], pk[2];
+        parents(i, Cl.nx, Cl.bt[0], Cl.bt[1], pi[0], pi[1]);
+        parents(j, Cl.ny, Cl.bt[2], Cl.bt[3], pj[0], pj[1]);
+        parents(k, Cl.nz, Cl.bt[4], Cl.bt[5], pk[0], pk[1]);
+        T s = T(0);
+        for (int c3 = 0; c3 < 2; ++c3)
+            for (int c2 = 0; c2 < 2; ++c2)
+                for (int c1 = 0; c1 < 2; ++c1) {
+                    T w = wt[c3] * wt[c2] * wt[c1];
+                    s = s + w * xc[lidx(Cl, pi[c1], pj[c2], pk[c3])];
+                }
+        xf[q] = xf[q] + s;
+    }
+    __syncthreads();
+}
+
+// The coarsest level (n <= COARSE_LDS cells, consistent red-black colouring)
+// on wavefront 0: neighbour tables whose missing (wall) neighbours point at
+// the zero slot x[n], then the symmetric sweeps of k_coarse_lds.
+template <typename T>
+__device__ __forceinline__ void tail_coarse(const LevelDev &L, T *x, const T *b, int sweeps,
+                                            short (*nb)[COARSE_LDS], T *id, unsigned char *col) {
+    const int n = L.nx * L.ny * L.nz;
+    const int m[3] = {L.nx, L.ny, L.nz};
+    for (int q = threadIdx.x; q < n; q += 64) {
+        int a[3];
+        lcell(L, q, a[0], a[1], a[2]);
+        for (int d = 0; d < 3; ++d)
+            for (int dir = 0; dir < 2; ++dir) {
+                int p[3] = {a[0], a[1], a[2]};
+                const int v = p[d] + (dir ? 1 : -1);
+                int idx;
+                if (v < 0) idx = L.bt[2 * d] != IBMDEM_BC_PERIODIC ? -1 : v + m[d];
+                else if (v >= m[d]) idx = L.bt[2 * d + 1] != IBMDEM_BC_PERIODIC ? -1 : v - m[d];
+                else idx = v;
+                p[d] = idx;
+                nb[2 * d + dir][q] = (short)(idx < 0 ? n : lidx(L, p[0], p[1], p[2]));
+            }
+        id[q] = T(L.idiag[diag_index(L, a[0], a[1], a[2])]);
+        col[q] = (unsigned char)((a[0] + a[1] + a[2]) & 1);
+        x[q] = T(0);
+    }
+    if (threadIdx.x == 0) x[n] = T(0);
+    wave_barrier_lds();
+    const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+    auto half = [&](int color) {
+        for (int q = threadIdx.x; q < n; q += 64)
+            if (col[q] == color) {
+                const T sx = (x[nb[0][q]] + x[nb[1][q]]) * ihx2;
+                const T sy = (x[nb[2][q]] + x[nb[3][q]]) * ihy2;
+                const T sz = (x[nb[4][q]] + x[nb[5][q]]) * ihz2;
+                x[q] = (b[q] + (sx + sy + sz)) * id[q];
+            }
+        wave_barrier_lds();
+    };
+    for (int s = 0; s < sweeps; ++s) { half(0); half(1); }
+    for (int s = 0; s < sweeps; ++s) { half(1); half(0); }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TAIL_T) void k_vcycle_tail(const LevelDev *__restrict__ lev, int first, int last,
+                                                        const T *__restrict__ bg, T *__restrict__ xg, int pre,
+                                                        int post, int coarse_sweeps,
+                                                        const double *__restrict__ stop) {
+    if (*stop != 0.0) return;
+    __shared__ T mem[TAIL_BYTES / sizeof(T)];
+    __shared__ short s_nb[6][COARSE_LDS];
+    __shared__ T s_id[COARSE_LDS];
+    __shared__ unsigned char s_col[COARSE_LDS];
+    __shared__ int s_off[IBMDEM_MAX_LEVELS + 1];   // level l: x at off, then b, then r
+    auto cells = [&](int l) { return lev[l].nx * lev[l].ny * lev[l].nz; };
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int l = first; l <= last; ++l) {
+            s_off[l] = off;
+            off += 3 * cells(l) + (l == last ? 1 : 0);
+        }
+    }
+    __syncthreads();
+    auto X = [&](int l) { return mem + s_off[l]; };
+    auto B = [&](int l) { return mem + s_off[l] + cells(l) + (l == last ? 1 : 0); };
+    auto R = [&](int l) { return B(l) + cells(l); };
+    for (int q = threadIdx.x; q < cells(first); q += TAIL_T) B(first)[q] = bg[q];
+    __syncthreads();
+    for (int l = first; l < last; ++l) {
+        for (int q = 0; q < pre; ++q) {
+            tail_half(lev[l], X(l), B(l), 0, q == 0);
+            tail_half(lev[l], X(l), B(l), 1, false);
+        }
+        tail_residual(lev[l], X(l), B(l), R(l));
+        tail_restrict(lev[l], R(l), lev[l + 1], B(l + 1));
+    }
+    if (threadIdx.x < 64) tail_coarse(lev[last], X(last), B(last), coarse_sweeps, s_nb, s_id, s_col);
+    __syncthreads();
+    for (int l = last - 1; l >= first; --l) {
+        tail_prolong(lev[l], X(l), lev[l + 1], X(l + 1));
+        for (int q = 0; q < post; ++q) {
+            tail_half(lev[l], X(l), B(l), 1, false);
+            tail_half(lev[l], X(l), B(l), 0, false);
+        }
+    }
+    for (int q = threadIdx.x; q < cells(first); q += TAIL_T) xg[q] = X(first)[q];
+}
+
+// ---------------------------------------------------------------------------
 // CG kernels (FP64)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double block_reduce(double s, double *red) {
@@ -696,6 +882,12 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     MGLevel<T> &L = lev[l];
     const LevelDev &d = L.d;
     hipStream_t s = c->stream;
+    if (l >= c->tail_first) {
+        hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(TAIL_T), 0, s, c->dlev, l, c->nlev - 1, L.b, L.x,
+                           c->cs.mg_pre, c->cs.mg_post, c->cs.mg_coarse_sweeps, c->scal + SC_CONV);
+        CHECK_LAUNCH(c);
+        return IBMDEM_OK;
+    }
     if (l == c->nlev - 1) {
         if (L.n <= COARSE_LDS)
             hipLaunchKernelGGL((k_coarse_lds<T>), dim3(1), dim3(64), 0, s, d, L.x, L.b, c->cs.mg_coarse_sweeps,
@@ -856,6 +1048,30 @@ int poisson_setup(ibmdem_ctx *c) {
         nx /= 2; ny /= 2; nz /= 2;
         h[0] *= 2; h[1] *= 2; h[2] *= 2;
     }
+    // the small levels whose x, b and r fit the tail work-group's LDS
+    // together run as one launch per V-cycle, when the coarsest level suits
+    // the one-wavefront table sweeps (IBMDEM_NO_TAIL=1 keeps one launch per
+    // operation, for comparison)
+    const size_t elem = c->cs.precision == IBMDEM_MIXED ? sizeof(float) : sizeof(double);
+    const LevelDev &lc = c->lev64[c->nlev - 1].d;
+    c->tail_first = c->nlev;
+    if (c->nlev >= 2 && lc.rb_valid && c->lev64[c->nlev - 1].n <= COARSE_LDS) {
+        size_t need = elem;   // the zero slot
+        for (int l = c->nlev - 1; l >= 1; --l) {
+            need += 3 * (size_t)c->lev64[l].n * elem;
+            if (need > (size_t)TAIL_BYTES) break;
+            c->tail_first = l;
+        }
+        if (c->tail_first == c->nlev - 1) c->tail_first = c->nlev;   // the coarsest alone: k_coarse_lds
+    }
+    const char *nt = std::getenv("IBMDEM_NO_TAIL");
+    if (nt && *nt && *nt != '0') c->tail_first = c->nlev;
+    int rc;
+    if ((rc = dev_alloc(c, &c->dlev, (size_t)c->nlev))) return rc;
+    LevelDev host[IBMDEM_MAX_LEVELS];
+    for (int l = 0; l < c->nlev; ++l) host[l] = c->lev64[l].d;
+    HIPX(c, hipMemcpyAsync(c->dlev, host, c->nlev * sizeof(LevelDev), hipMemcpyHostToDevice, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
     return IBMDEM_OK;
 }
 

@@ -388,3 +388,31 @@ def test_predictor_schedules_agree_bitwise(gpu_lib, name):
             np.testing.assert_array_equal(a, b)
         for a, b in zip(after, out[0][1]):
             np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("name,precision", [("periodic", 0), ("periodic32", 0), ("periodic32", 1), ("walls", 0),
+                                            ("open-z", 0), ("walls", 1), ("odd", 0)])
+def test_small_level_work_group_matches_launches(gpu_lib, monkeypatch, name, precision):
+    """The small multigrid levels run as one work-group in LDS (one launch
+    per V-cycle); the result equals the one-launch-per-operation path bit
+    for bit, in both precisions; a hierarchy whose coarsest level has an odd
+    periodic dimension keeps the launches."""
+    def setup():
+        if name == "odd":
+            return cases.small_periodic(n=(36, 24, 20), n_spheres=3)
+        if name == "periodic32":   # levels 32^3 ... 2^3: the work-group takes 16^3 and below
+            return cases.small_periodic(n=(32, 32, 32), n_spheres=4)
+        return _setups()[name]()
+    outs = []
+    for no_tail in ("1", "0"):
+        monkeypatch.setenv("IBMDEM_NO_TAIL", no_tail)
+        st = setup()
+        st.case.precision = precision
+        with Simulation(gpu_lib, st.case) as s:
+            cases.load(s, st)
+            infos = [s.step() for _ in range(3)]
+            outs.append((s.download_fields(), [(i.iterations, i.rel_residual) for i in infos]))
+    monkeypatch.delenv("IBMDEM_NO_TAIL")
+    assert outs[0][1] == outs[1][1]
+    for a, b in zip(outs[0][0], outs[1][0]):
+        np.testing.assert_array_equal(a, b)
