@@ -507,6 +507,25 @@ constexpr int TAIL_BYTES = 112 * 1024;   // x (one extra zero slot on the coarse
 template <typename T>
 __device__ __forceinline__ void tail_half(const LevelDev &L, T *x, const T *b, int color, bool zero) {
     const int n = L.nx * L.ny * L.nz;
+    if (L.split) {   // by x-pairs, as k_rb_half
+        const int rl = L.nx >> 1;
+        T *own = x + color * L.half, *oth = x + (1 - color) * L.half;
+        for (int p = threadIdx.x; p < L.half; p += TAIL_T) {
+            const int row = p / rl;
+            const PairGeo g = pair_geo(L, p - row * rl, row % L.ny, row / L.ny);
+            const int h = g.ca == color ? 0 : 1;
+            const T id = T(L.idiag[(h ? g.cx1 : g.cx0) * 16 + g.cyz]);
+            const T bo = T(b[color * L.half + p]);
+            if (zero) {
+                oth[p] = T(0);
+                own[p] = (bo + T(0)) * id;
+            } else {
+                own[p] = (bo + nbsum_pair(oth, g, h, L)) * id;
+            }
+        }
+        __syncthreads();
+        return;
+    }
     for (int q = threadIdx.x; q < n; q += TAIL_T) {
         int i, j, k;
         lcell(L, q, i, j, k);
@@ -523,6 +542,20 @@ __device__ __forceinline__ void tail_half(const LevelDev &L, T *x, const T *b, i
 template <typename T>
 __device__ __forceinline__ void tail_residual(const LevelDev &L, const T *x, const T *b, T *r) {
     const int n = L.nx * L.ny * L.nz;
+    if (L.split) {   // by x-pairs, as k_residual
+        const int rl = L.nx >> 1;
+        for (int p = threadIdx.x; p < L.half; p += TAIL_T) {
+            const int row = p / rl;
+            const PairGeo g = pair_geo(L, p - row * rl, row % L.ny, row / L.ny);
+            for (int h = 0; h < 2; ++h) {
+                const int own = (h ? 1 - g.ca : g.ca) * L.half, oth = (h ? g.ca : 1 - g.ca) * L.half;
+                const T dg = T(L.diag[(h ? g.cx1 : g.cx0) * 16 + g.cyz]);
+                r[own + p] = T(b[own + p]) - (dg * x[own + p] - nbsum_pair(x + oth, g, h, L));
+            }
+        }
+        __syncthreads();
+        return;
+    }
     for (int q = threadIdx.x; q < n; q += TAIL_T) {
         int i, j, k;
         lcell(L, q, i, j, k);
