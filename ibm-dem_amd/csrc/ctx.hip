@@ -90,7 +90,7 @@ int alloc_all(ibmdem_ctx *c) {
     // bulk-forcing partials: one per 64 x 4 row group and plane; the
     // matrix-vector product's: one per block
     const long pred_blocks = (long)div_up(c->g.nx, 64) * div_up(c->g.ny, 4) * c->g.nz;
-    c->max_partial = (int)std::max<long>(std::max<long>(4096, 3 * pred_blocks + 16),
+    c->max_partial = (int)std::max<long>(std::max<long>(4096, 3 * pred_blocks + 3 * 256 + 16),
                                          poisson_partials(c->g.nx, c->g.ny, c->g.nz));
     A(c->partial, c->max_partial);
     A(c->scal, SC_COUNT);

@@ -379,6 +379,28 @@ __global__ __launch_bounds__(256) void k_copy3(long n, const double *__restrict_
     }
 }
 
+// First stage of the bulk sums: block b adds the row-group partials of its
+// contiguous chunk (thread t takes every 256th, then a tree) into out[3b + c].
+constexpr int BULK_BLOCKS = 256;
+__global__ __launch_bounds__(256) void k_bulk_reduce(const double *__restrict__ partial, long nparts,
+                                                     double *__restrict__ out) {
+    __shared__ double red[3][256];
+    const int t = threadIdx.x;
+    const long chunk = (nparts + gridDim.x - 1) / gridDim.x;
+    const long lo = (long)blockIdx.x * chunk, hi = min(nparts, lo + chunk);
+    double s[3] = {0, 0, 0};
+    for (long q = lo + t; q < hi; q += 256)
+        for (int c = 0; c < 3; ++c) s[c] += partial[3 * q + c];
+    for (int c = 0; c < 3; ++c) red[c][t] = s[c];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (t < st)
+            for (int c = 0; c < 3; ++c) red[c][t] += red[c][t + st];
+        __syncthreads();
+    }
+    if (t < 3) out[3 * blockIdx.x + t] = red[t][0];
+}
+
 // ModifyBulkVelocity finalize (fluid.F90:177-183): one block sums the block
 // partials in a fixed order and stores the shifts dt*modifyP.
 __global__ __launch_bounds__(256) void k_bulk_finalize(const double *__restrict__ partial, long nparts,
@@ -679,7 +701,11 @@ PredParams pred_params(const ibmdem_ctx *c, double a, double cc) {
 int bulk_after_predictor(ibmdem_ctx *c) {
     const GridDev &g = c->g;
     const long nparts = (long)div_up(g.nx, BX) * div_up(g.ny, 4) * g.nz;
-    hipLaunchKernelGGL(k_bulk_finalize, dim3(1), dim3(256), 0, c->stream, c->partial, nparts, c->scal,
+    // two stages: BULK_BLOCKS chunk sums after the row-group partials, then one block
+    double *second = c->partial + 3 * nparts;
+    hipLaunchKernelGGL(k_bulk_reduce, dim3(BULK_BLOCKS), dim3(256), 0, c->stream, c->partial, nparts, second);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_bulk_finalize, dim3(1), dim3(256), 0, c->stream, second, (long)BULK_BLOCKS, c->scal,
                        c->cs.bulk_switch[0], c->cs.bulk_switch[1], c->cs.bulk_switch[2], c->cs.bulk_target[0],
                        c->cs.bulk_target[1], c->cs.bulk_target[2], (double)((long)g.nx * g.ny * g.nz), c->cs.dt);
     CHECK_LAUNCH(c);
