@@ -168,11 +168,15 @@ __device__ __forceinline__ T nbsum_pair(const T *__restrict__ oth, const PairGeo
 // written 0. DOT (the last two level-0 sweeps of the preconditioner, whose
 // b is the CG residual r and whose x is z = M^-1 r): also one partial sum of
 // b.x over the block's updated cells into partial[pofs + block].
+// RES = 1 (the last pre-smoothing sweep): also the residual of the updated
+// cells, r = b - (diag x - nbsum) with the sweep's own nbsum (the other
+// colour does not change in between); RES = 2: only that residual, for the
+// cells of `color`, x unchanged. Together they give r = b - A x.
 constexpr int RB_ROWS = 2;
-template <typename T, typename TB, bool DOT = false>
+template <typename T, typename TB, bool DOT = false, int RES = 0>
 __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
                                                 int color, int zero_init, const double *__restrict__ stop,
-                                                double *__restrict__ partial = nullptr, long pofs = 0) {
+                                                double *__restrict__ partial, long pofs, T *__restrict__ r) {
     if (*stop != 0.0) return;
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.z;
@@ -188,14 +192,21 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
             T *own = x + (long)color * L.half;
             T *oth = x + (long)(1 - color) * L.half;
             const int o = g.pos;
-            const T id = T(L.idiag[(h ? g.cx1 : g.cx0) * 16 + g.cyz]);
+            const int di = (h ? g.cx1 : g.cx0) * 16 + g.cyz;
             const TB bo = b[(long)color * L.half + o];
+            if (RES == 2) {
+                r[(long)color * L.half + o] = T(bo) - (T(L.diag[di]) * own[o] - nbsum_pair(oth, g, h, L));
+                continue;
+            }
+            const T id = T(L.idiag[di]);
             if (zero_init) {
                 oth[o] = T(0);
                 own[o] = (T(bo) + T(0)) * id;
             } else {
-                const T xn = (T(bo) + nbsum_pair(oth, g, h, L)) * id;
+                const T sn = nbsum_pair(oth, g, h, L);
+                const T xn = (T(bo) + sn) * id;
                 own[o] = xn;
+                if (RES == 1) r[(long)color * L.half + o] = T(bo) - (T(L.diag[di]) * xn - sn);
                 if (DOT) dsum += double(bo) * double(xn);
             }
             continue;
@@ -211,9 +222,15 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
             continue;
         }
         if (i >= L.nx) continue;
-        const T id = T(L.idiag[diag_index(L, i, j, k)]);
-        const T xn = (T(b[o]) + nbsum(x, L, i, j, k)) * id;
+        const int di = diag_index(L, i, j, k);
+        const T sn = nbsum(x, L, i, j, k);
+        if (RES == 2) {
+            r[o] = T(b[o]) - (T(L.diag[di]) * x[o] - sn);
+            continue;
+        }
+        const T xn = (T(b[o]) + sn) * T(L.idiag[di]);
         x[o] = xn;
+        if (RES == 1) r[o] = T(b[o]) - (T(L.diag[di]) * xn - sn);
         if (DOT) dsum += double(b[o]) * double(xn);
     }
     if (DOT) {
@@ -232,35 +249,6 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
 dim3 half_grid(int nx, int ny, int nz) {
     const LaunchShape rb = shape_for(div_up(nx, 2), ny, nz);
     return dim3(rb.grid.x, div_up(ny, RB_ROWS * (int)rb.block.y), rb.grid.z);
-}
-
-// r = b - A x. Each thread owns the x-pair (2p, 2p+1): one cell of each
-// colour, at the same position of the two colour halves when split.
-template <typename T, typename TB>
-__global__ __launch_bounds__(RB) void k_residual(LevelDev L, const T *__restrict__ x, const TB *__restrict__ b,
-                                                 T *__restrict__ r, const double *__restrict__ stop) {
-    if (*stop != 0.0) return;
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
-    if (j >= L.ny) return;
-    if (L.split) {
-        if (ip >= (L.nx >> 1)) return;
-        const PairGeo g = pair_geo(L, ip, j, k);
-        for (int h = 0; h < 2; ++h) {
-            const long own = (long)(h ? 1 - g.ca : g.ca) * L.half, oth = (long)(h ? g.ca : 1 - g.ca) * L.half;
-            const int o = g.pos;
-            const T dg = T(L.diag[(h ? g.cx1 : g.cx0) * 16 + g.cyz]);
-            r[own + o] = T(b[own + o]) - (dg * x[own + o] - nbsum_pair(x + oth, g, h, L));
-        }
-        return;
-    }
-    for (int h = 0; h < 2; ++h) {
-        const int i = 2 * ip + h;
-        if (i >= L.nx) return;
-        const int o = lidx(L, i, j, k);
-        const T dg = T(L.diag[diag_index(L, i, j, k)]);
-        r[o] = T(b[o]) - (dg * x[o] - nbsum(x, L, i, j, k));
-    }
 }
 
 // cell index of a virtual coordinate one step beyond the level: a wall side
@@ -542,7 +530,7 @@ __device__ __forceinline__ void tail_half(const LevelDev &L, T *x, const T *b, i
 template <typename T>
 __device__ __forceinline__ void tail_residual(const LevelDev &L, const T *x, const T *b, T *r) {
     const int n = L.nx * L.ny * L.nz;
-    if (L.split) {   // by x-pairs, as k_residual
+    if (L.split) {   // by x-pairs, as the residual of k_rb_half
         const int rl = L.nx >> 1;
         for (int p = threadIdx.x; p < L.half; p += TAIL_T) {
             const int row = p / rl;
@@ -714,8 +702,9 @@ __device__ __forceinline__ double block_reduce(double s, double *red) {
     return red[0];
 }
 
-// Ap = A p and one partial sum of p.Ap per block. Blocks, x-pairs and
-// planes as k_residual: a block's plane neighbours are read by the block
+// Ap = A p and one partial sum of p.Ap per block. A thread owns an x-pair
+// (both colours), a block 64 x 4 pairs of one plane: a block's plane
+// neighbours are read by the block
 // gridDim.x * gridDim.y positions away, a multiple of the XCD count for the
 // level-0 shapes, so they are served by the same L2.
 __global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__restrict__ p,
@@ -937,26 +926,34 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     const double hb = l == 0 ? 1.5 * sizeof(T) * (double)L.n : 0.0;
     const dim3 rbh = half_grid(d.nx, d.ny, d.nz);
     const long nbh = (long)rbh.x * rbh.y * rbh.z;
-    auto half = [&](int color, int zero, int dot_slot = -1) {
+    // res: 1 = the sweep also leaves the residual of its cells
+    auto half = [&](int color, int zero, int dot_slot = -1, int res = 0) {
         if (l == 0) prof_begin(c, IBMDEM_K_MG_SMOOTH);
         if constexpr (std::is_same<T, double>::value) {
             if (dot_slot >= 0) {
                 hipLaunchKernelGGL((k_rb_half<T, T, true>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero,
-                                   c->scal + SC_CONV, c->partial, dot_slot * nbh);
+                                   c->scal + SC_CONV, c->partial, dot_slot * nbh, (T *)nullptr);
                 if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, hb);
                 return;
             }
         }
-        hipLaunchKernelGGL((k_rb_half<T, T>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero, c->scal + SC_CONV,
-                           nullptr, 0L);
+        if (res)
+            hipLaunchKernelGGL((k_rb_half<T, T, false, 1>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero,
+                               c->scal + SC_CONV, (double *)nullptr, 0L, L.r);
+        else
+            hipLaunchKernelGGL((k_rb_half<T, T>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero, c->scal + SC_CONV,
+                               (double *)nullptr, 0L, (T *)nullptr);
         if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, hb);
     };
+    // pre-smoothing (red, black); the last black sweep also leaves r of the
+    // black cells, then r of the red cells
     for (int q = 0; q < c->cs.mg_pre; ++q) {
         half(0, q == 0 ? 1 : 0);
-        half(1, 0);
+        half(1, 0, -1, q == c->cs.mg_pre - 1 ? 1 : 0);
         CHECK_LAUNCH(c);
     }
-    hipLaunchKernelGGL((k_residual<T, T>), rb.grid, rb.block, 0, s, d, L.x, L.b, L.r, c->scal + SC_CONV);
+    hipLaunchKernelGGL((k_rb_half<T, T, false, 2>), rbh, rb.block, 0, s, d, L.x, L.b, 0, 0, c->scal + SC_CONV,
+                       (double *)nullptr, 0L, L.r);
     CHECK_LAUNCH(c);
     MGLevel<T> &C = lev[l + 1];
     const dim3 cs(div_up(C.d.nx, RS_CX), div_up(C.d.ny, RS_CY), C.d.nz);
