@@ -167,7 +167,9 @@ __device__ __forceinline__ T nbsum_pair(const T *__restrict__ oth, const PairGeo
 // implicitly 0 (first sweep of a V-cycle visit); the other colour is
 // written 0. DOT (the last two level-0 sweeps of the preconditioner, whose
 // b is the CG residual r and whose x is z = M^-1 r): also one partial sum of
-// b.x over the block's updated cells into partial[pofs + block].
+// r.z over the block's updated cells into partial[pofs + block]; in the
+// mixed form (z64 set) r is read from the FP64 r64 and the sweep also
+// writes z64 = double(x), the FP64 z of its cells.
 // RES = 1 (the last pre-smoothing sweep): also the residual of the updated
 // cells, r = b - (diag x - nbsum) with the sweep's own nbsum (the other
 // colour does not change in between); RES = 2: only that residual, for the
@@ -176,7 +178,9 @@ constexpr int RB_ROWS = 2;
 template <typename T, typename TB, bool DOT = false, int RES = 0>
 __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
                                                 int color, int zero_init, const double *__restrict__ stop,
-                                                double *__restrict__ partial, long pofs, T *__restrict__ r) {
+                                                double *__restrict__ partial, long pofs, T *__restrict__ r,
+                                                const double *__restrict__ r64 = nullptr,
+                                                double *__restrict__ z64 = nullptr) {
     if (*stop != 0.0) return;
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.z;
@@ -207,7 +211,15 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
                 const T xn = (T(bo) + sn) * id;
                 own[o] = xn;
                 if (RES == 1) r[(long)color * L.half + o] = T(bo) - (T(L.diag[di]) * xn - sn);
-                if (DOT) dsum += double(bo) * double(xn);
+                if (DOT) {
+                    const long go = (long)color * L.half + o;
+                    if (z64) {
+                        z64[go] = double(xn);
+                        dsum += r64[go] * double(xn);
+                    } else {
+                        dsum += double(bo) * double(xn);
+                    }
+                }
             }
             continue;
         }
@@ -231,7 +243,14 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
         const T xn = (T(b[o]) + sn) * T(L.idiag[di]);
         x[o] = xn;
         if (RES == 1) r[o] = T(b[o]) - (T(L.diag[di]) * xn - sn);
-        if (DOT) dsum += double(b[o]) * double(xn);
+        if (DOT) {
+            if (z64) {
+                z64[o] = double(xn);
+                dsum += r64[o] * double(xn);
+            } else {
+                dsum += double(b[o]) * double(xn);
+            }
+        }
     }
     if (DOT) {
         __shared__ double red[RB];
@@ -761,8 +780,10 @@ __global__ __launch_bounds__(RED_T) void k_cg_px(long n, double *__restrict__ p,
 
 // r -= alpha Ap; partial sums of r.r (x is updated one iteration later, by
 // k_cg_px or, after the last iteration, by k_gauge)
+// (and, in the mixed form, the FP32 copy of r the V-cycle reads)
 __global__ __launch_bounds__(RED_T) void k_cg_update_r(long n, double *__restrict__ r, const double *__restrict__ Ap,
-                                                       const double *__restrict__ scal, double *__restrict__ partial) {
+                                                       const double *__restrict__ scal, double *__restrict__ partial,
+                                                       float *__restrict__ r32) {
     if (scal[SC_CONV] != 0.0) return;
     __shared__ double red[RED_T];
     const double alpha = scal[SC_ALPHA];
@@ -772,6 +793,7 @@ __global__ __launch_bounds__(RED_T) void k_cg_update_r(long n, double *__restric
     for (long q = lo + threadIdx.x; q < hi; q += RED_T) {
         double rq = r[q] - alpha * Ap[q];
         r[q] = rq;
+        if (r32) r32[q] = float(rq);
         s += rq * rq;
     }
     double t = block_reduce(s, red);
@@ -831,12 +853,14 @@ __global__ __launch_bounds__(FIN_T) void k_final(const double *__restrict__ part
 // field order (cell (0,0,0) is position 0 of both)
 __global__ __launch_bounds__(RED_T) void k_cg_init(LevelDev L, long n, const double *__restrict__ rhs,
                                                    double *__restrict__ r, double *__restrict__ x,
-                                                   double *__restrict__ scal) {
+                                                   double *__restrict__ scal, float *__restrict__ r32) {
     if (blockIdx.x == 0 && threadIdx.x == 0) { scal[SC_CONV] = 0.0; scal[SC_ITER] = 0.0; scal[SC_DONE] = 0.0; }
     for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
         int i, j, k;
         lcell(L, (int)q, i, j, k);
-        r[q] = (q == 0 && !L.dirichlet) ? -scal[SC_PIN] : rhs[((long)k * L.ny + j) * L.nx + i];
+        const double rq = (q == 0 && !L.dirichlet) ? -scal[SC_PIN] : rhs[((long)k * L.ny + j) * L.nx + i];
+        r[q] = rq;
+        if (r32) r32[q] = float(rq);   // the mixed form's FP32 copy
         x[q] = 0.0;
     }
 }
@@ -852,9 +876,8 @@ __global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, c
 }
 
 // gauge P'(1,1,1) = 0: pcorr = x - x(1,1,1), x in level-0 storage, pcorr in
-// field order (no shift with an outflow side, where P' = 0 on the face)
-// P' = x - x(1,1,1) in field order, x taking the last completed CG
-// iteration's deferred update x + alpha p
+// field order (no shift with an outflow side, where P' = 0 on the face), x
+// taking the last completed CG iteration's deferred update x + alpha p
 __global__ __launch_bounds__(RED_T) void k_gauge(LevelDev L, long n, const double *__restrict__ x,
                                                  const double *__restrict__ p, const double *__restrict__ scal,
                                                  double *__restrict__ pc) {
@@ -929,13 +952,14 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     // res: 1 = the sweep also leaves the residual of its cells
     auto half = [&](int color, int zero, int dot_slot = -1, int res = 0) {
         if (l == 0) prof_begin(c, IBMDEM_K_MG_SMOOTH);
-        if constexpr (std::is_same<T, double>::value) {
-            if (dot_slot >= 0) {
-                hipLaunchKernelGGL((k_rb_half<T, T, true>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero,
-                                   c->scal + SC_CONV, c->partial, dot_slot * nbh, (T *)nullptr);
-                if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, hb);
-                return;
-            }
+        if (dot_slot >= 0) {
+            // the mixed form reads the FP64 r and writes the FP64 z as well
+            const bool mixed = !std::is_same<T, double>::value;
+            hipLaunchKernelGGL((k_rb_half<T, T, true>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero,
+                               c->scal + SC_CONV, c->partial, dot_slot * nbh, (T *)nullptr,
+                               mixed ? c->cg_r : nullptr, mixed ? c->cg_z : nullptr);
+            if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, hb);
+            return;
         }
         if (res)
             hipLaunchKernelGGL((k_rb_half<T, T, false, 1>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero,
@@ -1003,14 +1027,21 @@ int precondition(ibmdem_ctx *c) {
 // with post-smoothing the level-0 sweeps that finalize z also leave the
 // partial sums of r.z (one per block and colour), so z is not read again.
 int precondition_dot(ibmdem_ctx *c, int slot, int op) {
-    if (c->cs.precision == IBMDEM_MIXED || c->cs.mg_post < 1 || c->nlev < 2) {
+    if (c->cs.mg_post < 1 || c->nlev < 2) {
         int rc = precondition(c);
         if (rc) return rc;
         return dot_into(c, c->cg_r, c->cg_z, slot, op);
     }
-    c->lev64[0].b = c->cg_r;
-    c->lev64[0].x = c->cg_z;
-    int rc = vcycle<double>(c, c->lev64, 0, true);
+    int rc;
+    if (c->cs.precision == IBMDEM_MIXED) {
+        // the FP32 copy of r was written with r (k_cg_init, k_cg_update_r);
+        // the last level-0 sweeps write the FP64 z
+        rc = vcycle<float>(c, c->lev32, 0, true);
+    } else {
+        c->lev64[0].b = c->cg_r;
+        c->lev64[0].x = c->cg_z;
+        rc = vcycle<double>(c, c->lev64, 0, true);
+    }
     if (rc) return rc;
     const LevelDev &d = c->lev64[0].d;
     const dim3 g = half_grid(d.nx, d.ny, d.nz);
@@ -1129,7 +1160,7 @@ int cg_iteration(ibmdem_ctx *c, int it) {
     CHECK_LAUNCH(c);
     prof_begin(c, IBMDEM_K_CG_UPDATE);
     hipLaunchKernelGGL(k_cg_update_r, dim3(RED_BLOCKS), dim3(RED_T), 0, s, n, c->cg_r, c->cg_Ap, c->scal,
-                       c->partial);
+                       c->partial, c->cs.precision == IBMDEM_MIXED ? c->lev32[0].b : (float *)nullptr);
     prof_end(c, IBMDEM_K_CG_UPDATE, 3.0 * 8.0 * (double)n);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_final, dim3(1), dim3(FIN_T), 0, s, c->partial, RED_BLOCKS, c->scal, SC_RR, 3, it,
@@ -1150,7 +1181,7 @@ int poisson_begin(ibmdem_ctx *c) {
     // pinned row: b[0] = -sum_{q>0} b[q] (none with an outflow side)
     if (!c->lev64[0].d.dirichlet && (rc = reduce_sum(c, c->rhs, nullptr, n, SC_PIN, 1))) return rc;
     hipLaunchKernelGGL(k_cg_init, dim3(stream_blocks(n)), dim3(RED_T), 0, s, c->lev64[0].d, n, c->rhs, c->cg_r,
-                       c->cg_x, c->scal);
+                       c->cg_x, c->scal, c->cs.precision == IBMDEM_MIXED ? c->lev32[0].b : (float *)nullptr);
     CHECK_LAUNCH(c);
     if ((rc = dot_into(c, c->cg_r, c->cg_r, SC_BNORM2, 4))) return rc;
     if ((rc = precondition_dot(c, SC_RZ, 0))) return rc;
