@@ -232,7 +232,9 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__res
     // A marker whose taps need no mirror, zero or open-side value (all of
     // them away from walls and open sides; periodic wraps are only index
     // arithmetic) reads every tap directly.
-    const bool plain = !OPEN && ((mx.mir | mx.wall | my.mir | my.wall | mz.mir | mz.wall) == 0u);
+    const unsigned special = mx.mir | mx.wall | my.mir | my.wall | mz.mir | mz.wall |
+                             (OPEN ? (mx.cst | mx.pln | my.cst | my.pln | mz.cst | mz.pln) : 0u);
+    const bool plain = special == 0u;
     if (plain) {
         // u: x-face taps -1..2, y/z centre taps -1..1 (the nonzero ranges)
 #pragma unroll
