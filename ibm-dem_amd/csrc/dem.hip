@@ -218,27 +218,47 @@ __global__ __launch_bounds__(T) void k_body_rmax(int M, int B, GridDev g, const 
 }
 
 // near[b] = 1 if some other body's markers can come within the candidate
-// distance of b's: |X_b - X_c| < rmax_b + rmax_c + lim (+ a safety margin)
+// distance of b's: |X_b - X_c| < rmax_b + rmax_c + lim (+ a safety margin).
+// Thread b of block (x, y) tests body b = 64x + t against the NEAR_CH bodies
+// c of chunk y, staged in LDS (every block would otherwise read all B
+// positions from L2), and ORs a hit into near[b], which k_body_near_clear
+// zeroed. The periodic minimum image takes rint(d * (1/L)): it can differ
+// from rint(d / L) only where both images are L/2 away, within rounding,
+// which the safety margin covers, so the flags never miss a body in reach.
+constexpr int NEAR_CH = 64;
+__global__ __launch_bounds__(64) void k_body_near_clear(int B, int *__restrict__ near) {
+    const int b = blockIdx.x * 64 + threadIdx.x;
+    if (b < B) near[b] = 0;
+}
 __global__ __launch_bounds__(64) void k_body_near(int B, GridDev g, const double *__restrict__ bx,
                                                   const double *__restrict__ rmax, double lim, int *__restrict__ near) {
-    const int b = blockIdx.x;
+    __shared__ double sx[NEAR_CH], sy[NEAR_CH], sz[NEAR_CH], sr[NEAR_CH];
+    const int c0 = blockIdx.y * NEAR_CH, nc = min(NEAR_CH, B - c0);
+    for (int q = threadIdx.x; q < nc; q += 64) {
+        sx[q] = bx[c0 + q];
+        sy[q] = bx[(long)B + c0 + q];
+        sz[q] = bx[2L * B + c0 + q];
+        sr[q] = rmax[c0 + q];
+    }
+    __syncthreads();
+    const int b = blockIdx.x * 64 + threadIdx.x;
+    if (b >= B) return;
     const double xb0 = bx[b], xb1 = bx[(long)B + b], xb2 = bx[2L * B + b];
     const double rb = rmax[b] + lim + 1e-6 * (g.dx + lim);
+    const double iL[3] = {1.0 / g.L[0], 1.0 / g.L[1], 1.0 / g.L[2]};
     int hit = 0;
-    for (int c = threadIdx.x; c < B && !hit; c += 64) {
-        if (c == b) continue;
-        double d[3] = {bx[c] - xb0, bx[(long)B + c] - xb1, bx[2L * B + c] - xb2};
+    for (int q = 0; q < nc && !hit; ++q) {
+        if (c0 + q == b) continue;
+        double d[3] = {sx[q] - xb0, sy[q] - xb1, sz[q] - xb2};
         double d2 = 0.0;
-        for (int q = 0; q < 3; ++q) {
-            if (g.bt[2 * q] == IBMDEM_BC_PERIODIC) d[q] -= g.L[q] * rint(d[q] / g.L[q]);
-            d2 += d[q] * d[q];
+        for (int k = 0; k < 3; ++k) {
+            if (g.bt[2 * k] == IBMDEM_BC_PERIODIC) d[k] -= g.L[k] * rint(d[k] * iL[k]);
+            d2 += d[k] * d[k];
         }
-        const double lim_bc = rb + rmax[c];
+        const double lim_bc = rb + sr[q];
         if (d2 < lim_bc * lim_bc) hit = 1;
     }
-    // any-reduction over the wavefront
-    hit = __any(hit) ? 1 : 0;
-    if (threadIdx.x == 0) near[b] = hit;
+    if (hit) atomicOr(&near[b], 1);
 }
 
 // fcol of marker i (ZJS_COLLISION_DEM.F90:31-72, 93-146)
@@ -616,7 +636,10 @@ int dem_near(ibmdem_ctx *c) {
         c->rmax_dirty = false;
     }
     const double lim = c->cs.collision_dist * (1.0 + c->cs.collision_buffer);
-    hipLaunchKernelGGL(k_body_near, dim3(c->B), dim3(64), 0, c->stream, c->B, c->g, c->bx, c->brmax, lim, c->bnear);
+    hipLaunchKernelGGL(k_body_near_clear, dim3(div_up(c->B, 64)), dim3(64), 0, c->stream, c->B, c->bnear);
+    CHECK_LAUNCH(c);
+    hipLaunchKernelGGL(k_body_near, dim3(div_up(c->B, 64), div_up(c->B, NEAR_CH)), dim3(64), 0, c->stream, c->B,
+                       c->g, c->bx, c->brmax, lim, c->bnear);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
