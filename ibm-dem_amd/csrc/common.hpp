@@ -448,7 +448,8 @@ struct ibmdem_ctx {
     double *bfdrag = nullptr, *bfmom = nullptr, *bfcol = nullptr, *bfmomcol = nullptr;
     double *bfhydro = nullptr, *bfmomhydro = nullptr;
     double *brmax = nullptr;     // max |x_m - X_b| over the body's markers
-    int *bnear = nullptr;        // 1 if another body is within contact reach
+    int *bnear = nullptr;        // number of other bodies within contact reach
+    int *blist = nullptr;        // their ids, up to 32 per body
     bool rmax_dirty = true;
     // pair list for parity (allocated on demand)
     int *pair_count = nullptr, *pair_partner = nullptr, *pair_shift = nullptr;

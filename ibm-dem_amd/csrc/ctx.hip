@@ -130,7 +130,7 @@ int alloc_all(ibmdem_ctx *c) {
     A(c->brho, B); A(c->bfixed, B);
     A(c->bfdrag, 3L * B); A(c->bfmom, 3L * B); A(c->bfcol, 3L * B); A(c->bfmomcol, 3L * B);
     A(c->bfhydro, 3L * B); A(c->bfmomhydro, 3L * B);
-    A(c->brmax, B); A(c->bnear, B);
+    A(c->brmax, B); A(c->bnear, B); A(c->blist, 32L * B);
 #undef A
     return poisson_setup(c);
 }

@@ -217,21 +217,25 @@ __global__ __launch_bounds__(T) void k_body_rmax(int M, int B, GridDev g, const 
     if (threadIdx.x == 0) rmax[b] = red[0];
 }
 
-// near[b] = 1 if some other body's markers can come within the candidate
-// distance of b's: |X_b - X_c| < rmax_b + rmax_c + lim (+ a safety margin).
-// Thread b of block (x, y) tests body b = 64x + t against the NEAR_CH bodies
-// c of chunk y, staged in LDS (every block would otherwise read all B
-// positions from L2), and ORs a hit into near[b], which k_body_near_clear
-// zeroed. The periodic minimum image takes rint(d * (1/L)): it can differ
-// from rint(d / L) only where both images are L/2 away, within rounding,
-// which the safety margin covers, so the flags never miss a body in reach.
+// The bodies c whose markers can come within the candidate distance of body
+// b's: |X_b - X_c| < rmax_b + rmax_c + lim (+ a safety margin). near[b]
+// counts them and list[b * NEAR_MAX + s] holds the first NEAR_MAX (in no
+// particular order: the list only culls, see k_contact). Thread b of block
+// (x, y) tests body b = 64x + t against the NEAR_CH bodies c of chunk y,
+// staged in LDS (every block would otherwise read all B positions from L2);
+// k_body_near_clear zeroed the counts. The periodic minimum image takes
+// rint(d * (1/L)): it can differ from rint(d / L) only where both images are
+// L/2 away, within rounding, which the safety margin covers, so no body in
+// reach is missed.
 constexpr int NEAR_CH = 64;
+constexpr int NEAR_MAX = 32;
 __global__ __launch_bounds__(64) void k_body_near_clear(int B, int *__restrict__ near) {
     const int b = blockIdx.x * 64 + threadIdx.x;
     if (b < B) near[b] = 0;
 }
 __global__ __launch_bounds__(64) void k_body_near(int B, GridDev g, const double *__restrict__ bx,
-                                                  const double *__restrict__ rmax, double lim, int *__restrict__ near) {
+                                                  const double *__restrict__ rmax, double lim, int *__restrict__ near,
+                                                  int *__restrict__ list) {
     __shared__ double sx[NEAR_CH], sy[NEAR_CH], sz[NEAR_CH], sr[NEAR_CH];
     const int c0 = blockIdx.y * NEAR_CH, nc = min(NEAR_CH, B - c0);
     for (int q = threadIdx.x; q < nc; q += 64) {
@@ -246,8 +250,7 @@ __global__ __launch_bounds__(64) void k_body_near(int B, GridDev g, const double
     const double xb0 = bx[b], xb1 = bx[(long)B + b], xb2 = bx[2L * B + b];
     const double rb = rmax[b] + lim + 1e-6 * (g.dx + lim);
     const double iL[3] = {1.0 / g.L[0], 1.0 / g.L[1], 1.0 / g.L[2]};
-    int hit = 0;
-    for (int q = 0; q < nc && !hit; ++q) {
+    for (int q = 0; q < nc; ++q) {
         if (c0 + q == b) continue;
         double d[3] = {sx[q] - xb0, sy[q] - xb1, sz[q] - xb2};
         double d2 = 0.0;
@@ -256,9 +259,11 @@ __global__ __launch_bounds__(64) void k_body_near(int B, GridDev g, const double
             d2 += d[k] * d[k];
         }
         const double lim_bc = rb + sr[q];
-        if (d2 < lim_bc * lim_bc) hit = 1;
+        if (d2 < lim_bc * lim_bc) {
+            const int slot = atomicAdd(&near[b], 1);
+            if (slot < NEAR_MAX) list[(long)b * NEAR_MAX + slot] = c0 + q;
+        }
     }
-    if (hit) atomicOr(&near[b], 1);
 }
 
 // fcol of marker i (ZJS_COLLISION_DEM.F90:31-72, 93-146)
@@ -270,7 +275,9 @@ __global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *_
                                                const int *__restrict__ bin_items, double *__restrict__ fcol,
                                                ContactParams cp, int max_pairs, int *__restrict__ flags,
                                                int cap, int *__restrict__ pcount, int *__restrict__ ppartner,
-                                               int *__restrict__ pshift, const int *__restrict__ bnear) {
+                                               int *__restrict__ pshift, const int *__restrict__ bnear,
+                                               const int *__restrict__ blist, const double *__restrict__ bx, int B,
+                                               const double *__restrict__ brmax, double lim) {
     const int i = blockIdx.x * T + threadIdx.x;
     if (i >= M) return;
     const int cl = mcell[i];
@@ -281,9 +288,31 @@ __global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *_
     const int bi = mbody[i];
     double f0 = 0, f1 = 0, f2 = 0;
     int cnt = 0;
-    // a body with no other body in reach has no candidate pairs: skipping
-    // its scan adds nothing, exactly as the scan would
-    const int kstart = bnear[bi] ? -2 : 3;
+    // a marker with no other body in reach has no candidate pairs: skipping
+    // its scan adds nothing, exactly as the scan would. A partner j of body c
+    // within lim of x_i puts x_i within lim + rmax_c of c's centre (nearest
+    // image), so the marker is scanned only if some body on its body's near
+    // list is that close (with the body test's margin), or if the list
+    // overflowed.
+    const int nnear = bnear[bi];
+    bool scan = nnear > NEAR_MAX;
+    if (nnear > 0 && !scan) {
+        const double iL[3] = {1.0 / g.L[0], 1.0 / g.L[1], 1.0 / g.L[2]};
+        const double xi[3] = {xi0, xi1, xi2};
+        const double rl = lim + 1e-6 * (g.dx + lim);
+        for (int s = 0; s < nnear && !scan; ++s) {
+            const int cb = blist[(long)bi * NEAR_MAX + s];
+            double d2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                double d = xi[k] - bx[(long)k * B + cb];
+                if (g.bt[2 * k] == IBMDEM_BC_PERIODIC) d -= g.L[k] * rint(d * iL[k]);
+                d2 += d * d;
+            }
+            const double r = rl + brmax[cb];
+            scan = d2 < r * r;
+        }
+    }
+    const int kstart = scan ? -2 : 3;
     for (int kk = kstart; kk <= 2; ++kk) {
         int kc = ck + kk, kcode; double ksh;
         if (!nb_cell(kc, g.nz, g.bt[4], g.bt[5], g.L[2], ksh, kcode)) continue;
@@ -627,6 +656,9 @@ int dem_rebuild(ibmdem_ctx *c) {
     return IBMDEM_OK;
 }
 
+// the candidate distance of the contact scan, as the body culling uses it
+double near_lim(const ibmdem_ctx *c) { return c->cs.collision_dist * (1.0 + c->cs.collision_buffer); }
+
 int dem_near(ibmdem_ctx *c) {
     if (c->B == 0) return IBMDEM_OK;
     if (c->rmax_dirty) {
@@ -635,11 +667,11 @@ int dem_near(ibmdem_ctx *c) {
         CHECK_LAUNCH(c);
         c->rmax_dirty = false;
     }
-    const double lim = c->cs.collision_dist * (1.0 + c->cs.collision_buffer);
+    const double lim = near_lim(c);
     hipLaunchKernelGGL(k_body_near_clear, dim3(div_up(c->B, 64)), dim3(64), 0, c->stream, c->B, c->bnear);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_body_near, dim3(div_up(c->B, 64), div_up(c->B, NEAR_CH)), dim3(64), 0, c->stream, c->B,
-                       c->g, c->bx, c->brmax, lim, c->bnear);
+                       c->g, c->bx, c->brmax, lim, c->bnear, c->blist);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
@@ -652,7 +684,8 @@ int dem_contact(ibmdem_ctx *c) {
     prof_begin(c, IBMDEM_K_CONTACT);
     hipLaunchKernelGGL((k_contact<false>), dim3(div_up(M, T)), dim3(T), 0, c->stream, M, c->g, c->mx, c->mv,
                        c->mmass, c->mrad, c->mbody, c->mcell, c->bin_start, c->bin_items, c->fcol,
-                       contact_params(c), c->cs.max_pairs, c->flags, 0, nullptr, nullptr, nullptr, c->bnear);
+                       contact_params(c), c->cs.max_pairs, c->flags, 0, nullptr, nullptr, nullptr, c->bnear, c->blist,
+                       c->bx, c->B, c->brmax, near_lim(c));
     prof_end(c, IBMDEM_K_CONTACT, 0.0);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
@@ -672,7 +705,7 @@ int dem_pairs(ibmdem_ctx *c, int cap) {
     hipLaunchKernelGGL((k_contact<true>), dim3(div_up(M, T)), dim3(T), 0, c->stream, M, c->g, c->mx, c->mv,
                        c->mmass, c->mrad, c->mbody, c->mcell, c->bin_start, c->bin_items, c->fcol,
                        contact_params(c), c->cs.max_pairs, c->flags, c->pair_cap, c->pair_count, c->pair_partner,
-                       c->pair_shift, c->bnear);
+                       c->pair_shift, c->bnear, c->blist, c->bx, c->B, c->brmax, near_lim(c));
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
