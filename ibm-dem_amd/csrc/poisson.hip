@@ -100,6 +100,9 @@ __device__ __forceinline__ int diag_count(int i, int n, int lo, int hi) {
     if (i == n - 1 && hi == IBMDEM_BC_OUTFLOW) c += 2;
     return c;
 }
+// the diagonal table entry of a cell with both neighbours in all three
+// dimensions (counts 2, 2, 2)
+constexpr int DIAG_INTERIOR = 2 * 16 + 2 * 4 + 2;
 __device__ __forceinline__ int diag_index(const LevelDev &L, int i, int j, int k) {
     return diag_count(i, L.nx, L.bt[0], L.bt[1]) * 16 + diag_count(j, L.ny, L.bt[2], L.bt[3]) * 4 +
            diag_count(k, L.nz, L.bt[4], L.bt[5]);
@@ -182,6 +185,7 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
                                                 const double *__restrict__ r64 = nullptr,
                                                 double *__restrict__ z64 = nullptr) {
     if (*stop != 0.0) return;
+    const T id_int = T(L.idiag[DIAG_INTERIOR]), dg_int = T(L.diag[DIAG_INTERIOR]);
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.z;
     double dsum = 0.0;
@@ -199,10 +203,17 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
             const int di = (h ? g.cx1 : g.cx0) * 16 + g.cyz;
             const TB bo = b[(long)color * L.half + o];
             if (RES == 2) {
-                r[(long)color * L.half + o] = T(bo) - (T(L.diag[di]) * own[o] - nbsum_pair(oth, g, h, L));
+                T dg;
+                if (di == DIAG_INTERIOR) dg = dg_int;
+                else dg = T(L.diag[di]);
+                r[(long)color * L.half + o] = T(bo) - (dg * own[o] - nbsum_pair(oth, g, h, L));
                 continue;
             }
-            const T id = T(L.idiag[di]);
+            // cells with both neighbours in every dimension (all but the
+            // boundary layers) share table entry DIAG_INTERIOR, read once
+            T id;
+            if (di == DIAG_INTERIOR) id = id_int;
+            else id = T(L.idiag[di]);
             if (zero_init) {
                 oth[o] = T(0);
                 own[o] = (T(bo) + T(0)) * id;
@@ -210,7 +221,12 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
                 const T sn = nbsum_pair(oth, g, h, L);
                 const T xn = (T(bo) + sn) * id;
                 own[o] = xn;
-                if (RES == 1) r[(long)color * L.half + o] = T(bo) - (T(L.diag[di]) * xn - sn);
+                if (RES == 1) {
+                    T dg;
+                    if (di == DIAG_INTERIOR) dg = dg_int;
+                    else dg = T(L.diag[di]);
+                    r[(long)color * L.half + o] = T(bo) - (dg * xn - sn);
+                }
                 if (DOT) {
                     const long go = (long)color * L.half + o;
                     if (z64) {
@@ -737,10 +753,15 @@ __global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__r
     if (j < L.ny && L.split) {
         if (ip < (L.nx >> 1)) {
             const PairGeo g = pair_geo(L, ip, j, k);
+            const double dg_int = L.diag[DIAG_INTERIOR];
             for (int h = 0; h < 2; ++h) {
                 const long own = (long)(h ? 1 - g.ca : g.ca) * L.half, oth = (long)(h ? g.ca : 1 - g.ca) * L.half;
                 const long o = own + g.pos;
-                const double a = L.diag[(h ? g.cx1 : g.cx0) * 16 + g.cyz] * p[o] - nbsum_pair(p + oth, g, h, L);
+                const int di = (h ? g.cx1 : g.cx0) * 16 + g.cyz;
+                double dg;
+                if (di == DIAG_INTERIOR) dg = dg_int;
+                else dg = L.diag[di];
+                const double a = dg * p[o] - nbsum_pair(p + oth, g, h, L);
                 Ap[o] = a;
                 s += p[o] * a;
             }
