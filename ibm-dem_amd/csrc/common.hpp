@@ -509,7 +509,7 @@ int dev_alloc(ibmdem_ctx *c, T **p, size_t count) {
 
 // launch-time profiling of one kernel id
 void prof_begin(ibmdem_ctx *c, int kid);
-void prof_end(ibmdem_ctx *c, int kid, double bytes);
+void prof_end(ibmdem_ctx *c, int kid, double bytes, int launches = 1);
 
 // ---------------------------------------------------------------------------
 // launchers (one per translation unit)

@@ -23,13 +23,15 @@ void prof_begin(ibmdem_ctx *c, int kid) {
     (void)hipEventRecord(e, c->stream);
     c->prof.ev.push_back(e);
 }
-void prof_end(ibmdem_ctx *c, int kid, double bytes) {
+// the interval since the matching prof_begin held `launches` launches of
+// the profiled kernel moving `bytes` together
+void prof_end(ibmdem_ctx *c, int kid, double bytes, int launches) {
     if (c->prof.kernel != kid) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
     (void)hipEventRecord(e, c->stream);
     c->prof.ev.push_back(e);
-    c->prof.launches++;
+    c->prof.launches += launches;
     c->prof.bytes_total += bytes;
 }
 

@@ -966,20 +966,36 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     }
     LaunchShape rb = shape_for(div_up(d.nx, 2), d.ny, d.nz);
     // level-0 half-sweeps are the profiled (dominant) kernel: each reads the
-    // right-hand side and the other colour of half the cells and writes them
+    // right-hand side and the other colour of half the cells and writes them;
+    // the residual variant also writes r of its half, the mixed form's r.z
+    // variant also reads the FP64 r and writes the FP64 z of its half
     const double hb = l == 0 ? 1.5 * sizeof(T) * (double)L.n : 0.0;
+    const double hb_res = hb + (l == 0 ? 0.5 * sizeof(T) * (double)L.n : 0.0);
+    const double hb_dot = hb + (l == 0 && !std::is_same<T, double>::value ? 8.0 * (double)L.n : 0.0);
     const dim3 rbh = half_grid(d.nx, d.ny, d.nz);
     const long nbh = (long)rbh.x * rbh.y * rbh.z;
+    // The pre- and the post-smoothing sweeps at level 0 are each timed as one
+    // group of back-to-back launches (one event pair per group).
+    double grp_bytes = 0.0;
+    int grp_n = 0;
+    auto group_begin = [&]() {
+        if (l == 0) prof_begin(c, IBMDEM_K_MG_SMOOTH);
+        grp_bytes = 0.0;
+        grp_n = 0;
+    };
+    auto group_end = [&]() {
+        if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, grp_bytes, grp_n);
+    };
     // res: 1 = the sweep also leaves the residual of its cells
     auto half = [&](int color, int zero, int dot_slot = -1, int res = 0) {
-        if (l == 0) prof_begin(c, IBMDEM_K_MG_SMOOTH);
+        ++grp_n;
         if (dot_slot >= 0) {
             // the mixed form reads the FP64 r and writes the FP64 z as well
             const bool mixed = !std::is_same<T, double>::value;
             hipLaunchKernelGGL((k_rb_half<T, T, true>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero,
                                c->scal + SC_CONV, c->partial, dot_slot * nbh, (T *)nullptr,
                                mixed ? c->cg_r : nullptr, mixed ? c->cg_z : nullptr);
-            if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, hb);
+            grp_bytes += hb_dot;
             return;
         }
         if (res)
@@ -988,15 +1004,17 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
         else
             hipLaunchKernelGGL((k_rb_half<T, T>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero, c->scal + SC_CONV,
                                (double *)nullptr, 0L, (T *)nullptr);
-        if (l == 0) prof_end(c, IBMDEM_K_MG_SMOOTH, hb);
+        grp_bytes += res ? hb_res : hb;
     };
     // pre-smoothing (red, black); the last black sweep also leaves r of the
     // black cells, then r of the red cells
+    group_begin();
     for (int q = 0; q < c->cs.mg_pre; ++q) {
         half(0, q == 0 ? 1 : 0);
         half(1, 0, -1, q == c->cs.mg_pre - 1 ? 1 : 0);
         CHECK_LAUNCH(c);
     }
+    group_end();
     hipLaunchKernelGGL((k_rb_half<T, T, false, 2>), rbh, rb.block, 0, s, d, L.x, L.b, 0, 0, c->scal + SC_CONV,
                        (double *)nullptr, 0L, L.r);
     CHECK_LAUNCH(c);
@@ -1012,6 +1030,7 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     }
     hipLaunchKernelGGL((k_prolong<T>), rb.grid, rb.block, 0, s, d, L.x, C.d, C.x, c->scal + SC_CONV);
     CHECK_LAUNCH(c);
+    group_begin();
     for (int q = 0; q < c->cs.mg_post; ++q) {
         // the last post sweep at level 0 finalizes z: with dot, it also
         // leaves the partial sums of r.z (colour 1, then colour 0)
@@ -1020,6 +1039,7 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
         half(0, 0, last ? 1 : -1);
         CHECK_LAUNCH(c);
     }
+    group_end();
     return IBMDEM_OK;
 }
 
