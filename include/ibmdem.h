@@ -273,16 +273,17 @@ int ibmdem_batch_autotune(ibmdem_batch *batch, int32_t trial_steps);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Kernel ids for ibmdem_profile_* (HIP events around every launch of that
- * kernel on the context stream). */
+ * kernel on the context stream; the level-0 smoother's pre- and post-smoothing
+ * sweeps are timed as two groups of back-to-back launches). */
 #define IBMDEM_K_PREDICTOR   1
-#define IBMDEM_K_MG_SMOOTH   2
-#define IBMDEM_K_MATVEC      3
+#define IBMDEM_K_MG_SMOOTH   2   /* level-0 red-black half sweeps      */
+#define IBMDEM_K_MATVEC      3   /* CG matrix-vector product           */
 #define IBMDEM_K_SPREAD      4
 #define IBMDEM_K_INTERP      5
 #define IBMDEM_K_CONTACT     6
 #define IBMDEM_K_PROJECT     7
 #define IBMDEM_K_DIVERGENCE  8
-#define IBMDEM_K_CG_UPDATE   9
+#define IBMDEM_K_CG_UPDATE   9   /* CG r -= alpha Ap with r.r          */
 int ibmdem_profile_kernel(ibmdem_ctx *ctx, int32_t kernel_id);
 int ibmdem_profile_read(ibmdem_ctx *ctx, double *total_ms, int64_t *launches,
                         double *bytes_per_launch);
