@@ -871,15 +871,25 @@ __global__ __launch_bounds__(FIN_T) void k_final(const double *__restrict__ part
 
 
 // r = rhs with the pinned row, x = 0; r and x in level-0 storage, rhs in
-// field order (cell (0,0,0) is position 0 of both)
-__global__ __launch_bounds__(RED_T) void k_cg_init(LevelDev L, long n, const double *__restrict__ rhs,
-                                                   double *__restrict__ r, double *__restrict__ x,
-                                                   double *__restrict__ scal, float *__restrict__ r32) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[SC_CONV] = 0.0; scal[SC_ITER] = 0.0; scal[SC_DONE] = 0.0; }
-    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
-        int i, j, k;
-        lcell(L, (int)q, i, j, k);
-        const double rq = (q == 0 && !L.dirichlet) ? -scal[SC_PIN] : rhs[((long)k * L.ny + j) * L.nx + i];
+// field order (cell (0,0,0) is position 0 of both). A thread takes the
+// x-pair (2p, 2p+1) of row (j, k), blocks as k_matvec_dot.
+__global__ __launch_bounds__(RB) void k_cg_init(LevelDev L, const double *__restrict__ rhs,
+                                                double *__restrict__ r, double *__restrict__ x,
+                                                double *__restrict__ scal, float *__restrict__ r32) {
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    const double pin = scal[SC_PIN];
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+        scal[SC_CONV] = 0.0;
+        scal[SC_ITER] = 0.0;
+        scal[SC_DONE] = 0.0;
+    }
+    if (j >= L.ny) return;
+    for (int h = 0; h < 2; ++h) {
+        const int i = 2 * ip + h;
+        if (i >= L.nx) return;
+        const int q = lidx(L, i, j, k);
+        const double rq = (q == 0 && !L.dirichlet) ? -pin : rhs[((long)k * L.ny + j) * L.nx + i];
         r[q] = rq;
         if (r32) r32[q] = float(rq);   // the mixed form's FP32 copy
         x[q] = 0.0;
@@ -899,17 +909,21 @@ __global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, c
 // gauge P'(1,1,1) = 0: pcorr = x - x(1,1,1), x in level-0 storage, pcorr in
 // field order (no shift with an outflow side, where P' = 0 on the face), x
 // taking the last completed CG iteration's deferred update x + alpha p
-__global__ __launch_bounds__(RED_T) void k_gauge(LevelDev L, long n, const double *__restrict__ x,
-                                                 const double *__restrict__ p, const double *__restrict__ scal,
-                                                 double *__restrict__ pc) {
+// (thread per x-pair of a row, blocks as k_matvec_dot)
+__global__ __launch_bounds__(RB) void k_gauge(LevelDev L, const double *__restrict__ x,
+                                              const double *__restrict__ p, const double *__restrict__ scal,
+                                              double *__restrict__ pc) {
     const int done = (int)scal[SC_DONE];
     const double alpha = done > 0 ? scal[SC_ALPHA] : 0.0;
     auto xv = [&](long q) { return done > 0 ? x[q] + alpha * p[q] : x[q]; };
     const double x0 = L.dirichlet ? 0.0 : xv(0);
-    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
-        int i, j, k;
-        lcell(L, (int)q, i, j, k);
-        pc[((long)k * L.ny + j) * L.nx + i] = xv(q) - x0;
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (j >= L.ny) return;
+    for (int h = 0; h < 2; ++h) {
+        const int i = 2 * ip + h;
+        if (i >= L.nx) return;
+        pc[((long)k * L.ny + j) * L.nx + i] = xv(lidx(L, i, j, k)) - x0;
     }
 }
 
@@ -1221,8 +1235,10 @@ int poisson_begin(ibmdem_ctx *c) {
     int rc;
     // pinned row: b[0] = -sum_{q>0} b[q] (none with an outflow side)
     if (!c->lev64[0].d.dirichlet && (rc = reduce_sum(c, c->rhs, nullptr, n, SC_PIN, 1))) return rc;
-    hipLaunchKernelGGL(k_cg_init, dim3(stream_blocks(n)), dim3(RED_T), 0, s, c->lev64[0].d, n, c->rhs, c->cg_r,
-                       c->cg_x, c->scal, c->cs.precision == IBMDEM_MIXED ? c->lev32[0].b : (float *)nullptr);
+    const LevelDev &L0 = c->lev64[0].d;
+    const LaunchShape fs = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);
+    hipLaunchKernelGGL(k_cg_init, fs.grid, fs.block, 0, s, L0, c->rhs, c->cg_r, c->cg_x, c->scal,
+                       c->cs.precision == IBMDEM_MIXED ? c->lev32[0].b : (float *)nullptr);
     CHECK_LAUNCH(c);
     if ((rc = dot_into(c, c->cg_r, c->cg_r, SC_BNORM2, 4))) return rc;
     if ((rc = precondition_dot(c, SC_RZ, 0))) return rc;
@@ -1285,8 +1301,9 @@ int poisson_check(ibmdem_ctx *c, ibmdem_solve_info *info) {
 
 // P' = x - x(1,1,1): the reference's gauge (poisson.F90:78,97-100)
 int poisson_gauge(ibmdem_ctx *c) {
-    hipLaunchKernelGGL(k_gauge, dim3(stream_blocks(c->n)), dim3(RED_T), 0, c->stream, c->lev64[0].d, c->n, c->cg_x,
-                       c->cg_p, c->scal, c->pcorr);
+    const LevelDev &L0 = c->lev64[0].d;
+    const LaunchShape fs = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);
+    hipLaunchKernelGGL(k_gauge, fs.grid, fs.block, 0, c->stream, L0, c->cg_x, c->cg_p, c->scal, c->pcorr);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
