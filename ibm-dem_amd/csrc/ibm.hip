@@ -644,6 +644,13 @@ __device__ __forceinline__ void stage_entry(const SpreadArgs &a, const EntryIn &
 // lanes of a half-wavefront (support offsets 4 x 4 x 2) hit 32 different
 // 8-byte bank pairs
 constexpr int ACC_Z = TL * TL + 4, ACC_N = ACC_Z * TL;
+// s += v as one LDS add without return (ds_add_f64: round to nearest even,
+// the same result as s - p for v = -p). The LDS executes a wavefront's
+// operations in issue order, so successive adds to one cell land in program
+// order and the lane does not wait for the old value.
+__device__ __forceinline__ void lds_add(double *s, double v) {
+    __hip_atomic_fetch_add(s, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
 __device__ __forceinline__ int acc_idx(int q) { return q + 4 * (q >> 6); }
 
 template <bool P2>
@@ -700,9 +707,9 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
                 const double pu = wx.x * wy.y * wz.y, pv = wx.y * wy.x * wz.y, pw = wx.y * wy.y * wz.x;
                 const double wu = P2 ? pu : pu / a.dV, wv = P2 ? pv : pv / a.dV, ww = P2 ? pw : pw / a.dV;
                 const int q = lx + TL * (ly + TL * lz), qa = lx + TL * ly + ACC_Z * lz;
-                s_acc[0][qa] = s_acc[0][qa] - fv.x * wu;
-                s_acc[1][qa] = s_acc[1][qa] - fv.y * wv;
-                s_acc[2][qa] = s_acc[2][qa] - fv.z * ww;
+                lds_add(&s_acc[0][qa], -(fv.x * wu));
+                lds_add(&s_acc[1][qa], -(fv.y * wv));
+                lds_add(&s_acc[2][qa], -(fv.z * ww));
                 if (a.first) atomicOr(&s_touch[q >> 5], 1u << (q & 31));
             }
         }
