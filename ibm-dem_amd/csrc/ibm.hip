@@ -22,15 +22,15 @@
 // (kk,jj,ii) order, so it is bit-identical to the reference's 125-cell loop
 // (zero-weight taps add an exact zero there).
 //
-// Spreading is deterministic, with no float atomics. One wavefront owns a
-// tile and keeps the tile's force (3 x 512 doubles) in LDS. It stages its
-// entry list in chunks (delta factors, fdrag*vol and the cell of the marker
-// or of its image) and takes the entries one after another: lane l adds the
-// entry's contribution -(fdrag*vol)*w to cell l of the entry's 4x4x4
-// support (when that cell is in the tile). The 64 lanes of one entry write
-// 64 different cells, and a wavefront's LDS operations complete in program
-// order, so every cell receives its contributions in entry-list order: the
-// sum does not depend on timing. The reference's periodic ghost copies
+// Spreading is deterministic: no two wavefronts ever add to one value. One
+// wavefront owns a tile and keeps the tile's force (3 x 512 doubles) in LDS.
+// It stages its entry list in chunks (delta factors, fdrag*vol and the cell
+// of the marker or of its image) and takes the entries in list order: lane l
+// adds the entry's contribution -(fdrag*vol)*w to cell l of the entry's
+// 4x4x4 support (when that cell is in the tile) with an LDS add. The 64
+// lanes of one entry add to 64 different cells, and the LDS applies a
+// wavefront's operations in issue order, so every cell receives its
+// contributions in entry-list order: the sum does not depend on timing. The reference's periodic ghost copies
 // (x + coordShift) and wall-mirror copies (2*coordMirror - x, fdrag
 // unnegated; ZJS_MPI_DEM.F90:173-269) enter as image entries with their
 // factors evaluated at the image position. The wavefront then applies
@@ -47,8 +47,12 @@ constexpr int TL = IBM_TL;    // tile edge (cells)
 constexpr int TC = IBM_TC;    // cells per tile
 constexpr int SRC = TL + 3;   // source cells per tile edge
 constexpr int NSRC = SRC * SRC * SRC;
-constexpr int CAP = 16;       // entries staged per pass (fewer: less LDS, more resident waves)
+// entries staged per pass: one pass's fixed costs (the staging, the wait for
+// its records) are spread over 32 entries; 16 or 48 measured slower
+constexpr int CAP = 32;
 constexpr int WST2 = 13;      // LDS row stride of the 12 factor pairs (odd: fewer bank conflicts)
+constexpr int SU = 4;         // entries whose terms are formed before their adds (2 or 8: slower)
+static_assert(CAP % SU == 0, "entry groups tile the staged rows");
 constexpr int SBITS = 9;      // image-code bits of an entry (3 per dimension)
 
 // one-dimensional delta factors of a marker at xp for cell i: staggered
@@ -583,24 +587,44 @@ __device__ __forceinline__ void load_entry_id(const SpreadArgs &a, int e, bool v
     in.valid = valid;
     in.en = valid ? a.ent[e] : 0u;
 }
+// The entry id is consumed here on every path, also where no lane of the
+// wavefront is valid: the register then cannot look pending at the summing
+// loop, which would otherwise wait for every load in flight before reusing
+// it. (The empty asm only reads the register; it emits no instruction.)
 __device__ __forceinline__ void load_entry_rec(const SpreadArgs &a, EntryIn &in) {
+    asm volatile("" ::"v"(in.en));
     if (in.valid) {
         const long m = (long)(in.en >> SBITS);
-        const double2 *r = reinterpret_cast<const double2 *>(a.rec + 8 * m);
-        const double2 r01 = r[0], r23 = r[1], r45 = r[2];
-        in.x0 = r01.x; in.x1 = r01.y; in.x2 = r23.x;
-        in.cell = (int)__double_as_longlong(r23.y);
-        in.f0 = r45.x; in.f1 = r45.y; in.f2 = a.rec[8 * m + 6];
+        // each load has exactly the width of what is used: no dead
+        // registers that the summing loop could reuse while a load is in
+        // flight (that would make it wait for the next chunk's records)
+        const double *r = a.rec + 8 * m;
+        const double2 r01 = reinterpret_cast<const double2 *>(r)[0];
+        const double2 r45 = reinterpret_cast<const double2 *>(r)[2];
+        in.x0 = r01.x; in.x1 = r01.y; in.x2 = r[2];
+        in.cell = reinterpret_cast<const int *>(r + 3)[0];   // low word of the cell slot
+        in.f0 = r45.x; in.f1 = r45.y; in.f2 = r[6];
         const double2 *mf = reinterpret_cast<const double2 *>(a.fac + 24 * m);
 #pragma unroll
         for (int q = 0; q < 12; ++q) in.fw[q] = mf[q];
     }
 }
 
-// delta factors, cell and fdrag*vol of the lane's entry into LDS row `lane`
+// What the summing loop needs of a staged entry that is the same for every
+// lane: the support's corner relative to the tile (packed, 5 bits per
+// dimension, biased by 8) and fdrag*vol. The staging lane keeps it in
+// registers and the loop reads it with readlane, so it costs no LDS
+// bandwidth.
+struct EntryUni {
+    int corner;
+    double f0, f1, f2;
+};
+
+// delta factors of the lane's entry into LDS row `lane`; its cell and
+// fdrag*vol into u
 template <bool P2>
 __device__ __forceinline__ void stage_entry(const SpreadArgs &a, const EntryIn &in, double2 (*s_w)[WST2],
-                                            int4 *s_cc, double4 *s_fv) {
+                                            int i0, int j0, int k0, EntryUni &u) {
     const GridDev &g = a.g;
     const int lane = threadIdx.x;
     if (!in.valid) return;
@@ -613,8 +637,11 @@ __device__ __forceinline__ void stage_entry(const SpreadArgs &a, const EntryIn &
     // with a power-of-two cell volume the 1/dV of every weight is an exact
     // scaling, folded into fdrag*vol here once per entry (bitwise the same
     // products as fv * (w * (1/dV)))
-    if (P2) s_fv[lane] = make_double4(in.f0 * a.idV, in.f1 * a.idV, in.f2 * a.idV, 0.0);
-    else s_fv[lane] = make_double4(in.f0, in.f1, in.f2, 0.0);
+    if (P2) {
+        u.f0 = in.f0 * a.idV; u.f1 = in.f1 * a.idV; u.f2 = in.f2 * a.idV;
+    } else {
+        u.f0 = in.f0; u.f1 = in.f1; u.f2 = in.f2;
+    }
     int cim[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -636,7 +663,13 @@ __device__ __forceinline__ void stage_entry(const SpreadArgs &a, const EntryIn &
             s_w[lane][d * 4 + o] = make_double2(wt_face<P2>(xp, cimg + o - 1, dd[d], id3[d]),
                                                 o < 3 ? wt_cent<P2>(xp, cimg + o - 1, dd[d], id3[d]) : 0.0);
     }
-    s_cc[lane] = make_int4(cim[0], cim[1], cim[2], 0);
+    // support cells cim - 1 .. cim + 2 reach the tile, so each corner offset
+    // lies in [-3, 7]; clamping to [-4, 8] changes nothing (such a support
+    // misses the tile either way) and keeps the packed fields in range
+    const int o3[3] = {i0, j0, k0};
+    u.corner = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u.corner |= (min(max(cim[d] - 1 - o3[d], -4), 8) + 8) << (5 * d);
 }
 
 // One wavefront per active tile (block b takes tile slot tile_slot(b)).
@@ -652,17 +685,22 @@ __device__ __forceinline__ void lds_add(double *s, double v) {
     __hip_atomic_fetch_add(s, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 __device__ __forceinline__ int acc_idx(int q) { return q + 4 * (q >> 6); }
+// lane e's double, on every lane (e uniform)
+__device__ __forceinline__ double readlane_f64(double v, int e) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, e);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), e);
+    return __longlong_as_double((long long)((unsigned long long)hi << 32 | lo));
+}
 
 template <bool P2>
 __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     __shared__ double s_acc[3][ACC_N];         // the tile's force
     __shared__ unsigned s_touch[TC / 32];      // cells some entry reached
     // per staged entry: (face, centre) factor pairs for 3 dimensions x 4
-    // offsets (padded to an odd number of 16-byte pairs per entry), the cell
-    // of the marker or image, and fdrag*vol
+    // offsets (padded to an odd number of 16-byte pairs per entry); its
+    // support corner and fdrag*vol stay in the staging lane's registers
     __shared__ double2 s_w[CAP][WST2];
-    __shared__ int4 s_cc[CAP];
-    __shared__ double4 s_fv[CAP];
     const GridDev &g = a.g;
     const int lane = threadIdx.x;
     const int slot = tile_slot(blockIdx.x, a.meta[0]);
@@ -683,6 +721,8 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
         s_touch[lane] = a.first ? 0u : (unsigned)(a.mask[(long)slot * IBM_MASK_WORDS + (lane >> 1)] >> (32 * (lane & 1)));
     // the lane's offset in an entry's 4x4x4 support
     const int oa = lane & 3, ob = (lane >> 2) & 3, oc = lane >> 4;
+    const int loff = oa + TL * ob + ACC_Z * oc;   // its accumulator offset from the support corner
+    EntryUni uni{};
     wave_sync();
     // Two-stage prefetch: while chunk c is summed, the records and factors
     // of chunk c+1 and the entry ids of chunk c+2 are in flight.
@@ -692,25 +732,47 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     load_entry_id(a, elo + CAP + lane, lane < min(CAP, ehi - elo - CAP), nx);
     for (int e0 = elo; e0 < ehi; e0 += CAP) {
         const int ne = min(CAP, ehi - e0);
-        stage_entry<P2>(a, in, s_w, s_cc, s_fv);
+        stage_entry<P2>(a, in, s_w, i0, j0, k0, uni);
         wave_sync();
         in = nx;
         load_entry_rec(a, in);
         load_entry_id(a, e0 + 2 * CAP + lane, lane < min(CAP, ehi - e0 - 2 * CAP), nx);
-        for (int e = 0; e < ne; ++e) {
-            const int4 cc = s_cc[e];
-            const int lx = cc.x - 1 + oa - i0, ly = cc.y - 1 + ob - j0, lz = cc.z - 1 + oc - k0;
-            if ((unsigned)lx < (unsigned)TL && (unsigned)ly < (unsigned)TL && (unsigned)lz < (unsigned)TL) {
+        // SU entries at a time: their factor rows are read and their terms
+        // formed first, then their adds are issued back to back in entry
+        // order (rows at or past ne hold stale data and are not added)
+        for (int g0 = 0; g0 < ne; g0 += SU) {
+            double t0[SU], t1[SU], t2[SU];
+            int qa[SU], qt[SU];
+            bool hit[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int e = g0 + u;
+                // the entry's uniform values from its staging lane (e is the
+                // same on every lane, so these are scalars)
+                const int cp = __builtin_amdgcn_readlane(uni.corner, e);
+                const int bx = (cp & 31) - 8, by = ((cp >> 5) & 31) - 8, bz = ((cp >> 10) & 31) - 8;
+                const int lx = bx + oa, ly = by + ob, lz = bz + oc;
+                hit[u] = e < ne && (unsigned)lx < (unsigned)TL && (unsigned)ly < (unsigned)TL &&
+                         (unsigned)lz < (unsigned)TL;
+                qa[u] = (bx + TL * by + ACC_Z * bz) + loff;
+                qt[u] = lx + TL * (ly + TL * lz);
+                const double f0 = readlane_f64(uni.f0, e), f1 = readlane_f64(uni.f1, e),
+                             f2 = readlane_f64(uni.f2, e);
                 const double2 wx = s_w[e][oa], wy = s_w[e][4 + ob], wz = s_w[e][8 + oc];
-                const double4 fv = s_fv[e];
                 // (face, centre): u takes the x face, v the y face, w the z face
                 const double pu = wx.x * wy.y * wz.y, pv = wx.y * wy.x * wz.y, pw = wx.y * wy.y * wz.x;
                 const double wu = P2 ? pu : pu / a.dV, wv = P2 ? pv : pv / a.dV, ww = P2 ? pw : pw / a.dV;
-                const int q = lx + TL * (ly + TL * lz), qa = lx + TL * ly + ACC_Z * lz;
-                lds_add(&s_acc[0][qa], -(fv.x * wu));
-                lds_add(&s_acc[1][qa], -(fv.y * wv));
-                lds_add(&s_acc[2][qa], -(fv.z * ww));
-                if (a.first) atomicOr(&s_touch[q >> 5], 1u << (q & 31));
+                t0[u] = f0 * wu;
+                t1[u] = f1 * wv;
+                t2[u] = f2 * ww;
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                if (!hit[u]) continue;
+                lds_add(&s_acc[0][qa[u]], -t0[u]);
+                lds_add(&s_acc[1][qa[u]], -t1[u]);
+                lds_add(&s_acc[2][qa[u]], -t2[u]);
+                if (a.first) atomicOr(&s_touch[qt[u] >> 5], 1u << (qt[u] & 31));
             }
         }
         wave_sync();   // the rows are rewritten by the next chunk
