@@ -313,25 +313,39 @@ __global__ __launch_bounds__(RB) void k_restrict(LevelDev F, const T *__restrict
     __shared__ T tile[RS_TILE];
     const int t = threadIdx.x;
     const int I0 = blockIdx.x * RS_CX, J0 = blockIdx.y * RS_CY, K = blockIdx.z;
-    // all of a thread's loads are issued before the first LDS store waits
-    constexpr int NL = (RS_TILE + RB - 1) / RB;
-    T v[NL];
+    // The tile's rows r = z RS_RY + y: wavefront w loads columns 0..63 of
+    // rows w, w + RB/64, ...; a row's (j, k) and its part of the storage
+    // index are the same on every lane (scalars), the lane's part (its
+    // column's i) is computed once. Columns 64.. of every row are one more
+    // load for the first threads. All loads are issued before the first LDS
+    // store waits.
+    constexpr int NW = RB / 64, NROW = RS_RY * 4, RPW = NROW / NW, NEXTRA = (RS_RX - 64) * NROW;
+    static_assert(NROW % NW == 0 && RS_RX >= 64 && NEXTRA <= RB, "row split of the restriction tile");
+    const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int half = F.split ? F.half : 0, rl = F.split ? F.nx >> 1 : F.nx;
+    const int il = tile_coord(2 * I0 - 1 + lane, F.nx, F.bt[0], F.bt[1]);
+    // the lane's offset within a row for row parity 0 and 1
+    const int lo0 = F.split ? (il & 1) * half + (il >> 1) : il;
+    const int lo1 = F.split ? ((il & 1) ^ 1) * half + (il >> 1) : il;
+    T v[RPW], ve = T(0);
 #pragma unroll
-    for (int u = 0; u < NL; ++u) {
-        const int q = t + u * RB;
-        if (q < RS_TILE) {
-            const int x = q % RS_RX, y = (q / RS_RX) % RS_RY, z = q / (RS_RX * RS_RY);
-            const int i = tile_coord(2 * I0 - 1 + x, F.nx, F.bt[0], F.bt[1]);
-            const int j = tile_coord(2 * J0 - 1 + y, F.ny, F.bt[2], F.bt[3]);
-            const int k = tile_coord(2 * K - 1 + z, F.nz, F.bt[4], F.bt[5]);
-            v[u] = rf[lidx(F, i, j, k)];
-        }
+    for (int u = 0; u < RPW; ++u) {
+        const int r = w + NW * u, y = r % RS_RY, z = r / RS_RY;
+        const int j = tile_coord(2 * J0 - 1 + y, F.ny, F.bt[2], F.bt[3]);
+        const int k = tile_coord(2 * K - 1 + z, F.nz, F.bt[4], F.bt[5]);
+        const int rowoff = (k * F.ny + j) * rl;
+        v[u] = rf[rowoff + (((j + k) & 1) ? lo1 : lo0)];
+    }
+    if (t < NEXTRA) {
+        const int r = t / (RS_RX - 64), x = 64 + t % (RS_RX - 64), y = r % RS_RY, z = r / RS_RY;
+        const int i = tile_coord(2 * I0 - 1 + x, F.nx, F.bt[0], F.bt[1]);
+        const int j = tile_coord(2 * J0 - 1 + y, F.ny, F.bt[2], F.bt[3]);
+        const int k = tile_coord(2 * K - 1 + z, F.nz, F.bt[4], F.bt[5]);
+        ve = rf[lidx(F, i, j, k)];
     }
 #pragma unroll
-    for (int u = 0; u < NL; ++u) {
-        const int q = t + u * RB;
-        if (q < RS_TILE) tile[q] = v[u];
-    }
+    for (int u = 0; u < RPW; ++u) tile[(w + NW * u) * RS_RX + lane] = v[u];
+    if (t < NEXTRA) tile[(t / (RS_RX - 64)) * RS_RX + 64 + t % (RS_RX - 64)] = ve;
     __syncthreads();
     const int I = I0 + t % RS_CX, J = J0 + t / RS_CX;
     if (I >= Cl.nx || J >= Cl.ny) return;
