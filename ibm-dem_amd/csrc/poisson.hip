@@ -158,17 +158,30 @@ __device__ __forceinline__ T nbsum_pair_f(const F &q, const PairGeo &g, int h, c
     const T szz = (xb + xt) * ihz2;
     return sx + syy + szz;
 }
+// nbsum_pair_f's value from an array. Every neighbour is loaded without a
+// branch (an absent one from offset 0, the pair's own position, and then
+// replaced by 0), so a thread's loads can all be in flight at once.
 template <typename T>
 __device__ __forceinline__ T nbsum_pair(const T *__restrict__ oth, const PairGeo &g, int h, const LevelDev &L) {
     const T *q = oth + g.pos;
-    return nbsum_pair_f<T>([&](int d) { return q[d]; }, g, h, L);
+    const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+    const bool pw = h == 0 ? g.hw : true, pe = h == 0 ? true : g.he;
+    const T vw = q[h == 0 && g.hw ? g.dw : 0], ve = q[h != 0 && g.he ? g.de : 0];
+    const T vs = q[g.hs ? g.ds : 0], vn = q[g.hn ? g.dn : 0];
+    const T vb = q[g.hb ? g.db : 0], vt = q[g.ht ? g.dt : 0];
+    const T xw = pw ? vw : T(0), xe = pe ? ve : T(0);
+    const T xs = g.hs ? vs : T(0), xn = g.hn ? vn : T(0);
+    const T xb = g.hb ? vb : T(0), xt = g.ht ? vt : T(0);
+    const T sx = (xw + xe) * ihx2;
+    const T syy = (xs + xn) * ihy2;
+    const T szz = (xb + xt) * ihz2;
+    return sx + syy + szz;
 }
 
 // one red-black half sweep; each thread owns the cell of the colour in a
 // pair of x cells, in RB_ROWS rows blockDim.y apart (independent cells, so
-// their loads are in flight together). zero_init: the level's x is
-// implicitly 0 (first sweep of a V-cycle visit); the other colour is
-// written 0. DOT (the last two level-0 sweeps of the preconditioner, whose
+// their loads are in flight together). ZI: the level's x is implicitly 0
+// (first sweep of a V-cycle visit); the other colour is written 0. DOT (the last two level-0 sweeps of the preconditioner, whose
 // b is the CG residual r and whose x is z = M^-1 r): also one partial sum of
 // r.z over the block's updated cells into partial[pofs + block]; in the
 // mixed form (z64 set) r is read from the FP64 r64 and the sweep also
@@ -178,9 +191,9 @@ __device__ __forceinline__ T nbsum_pair(const T *__restrict__ oth, const PairGeo
 // colour does not change in between); RES = 2: only that residual, for the
 // cells of `color`, x unchanged. Together they give r = b - A x.
 constexpr int RB_ROWS = 2;
-template <typename T, typename TB, bool DOT = false, int RES = 0>
+template <typename T, typename TB, bool DOT = false, int RES = 0, bool ZI = false>
 __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
-                                                int color, int zero_init, const double *__restrict__ stop,
+                                                int color, const double *__restrict__ stop,
                                                 double *__restrict__ partial, long pofs, T *__restrict__ r,
                                                 const double *__restrict__ r64 = nullptr,
                                                 double *__restrict__ z64 = nullptr) {
@@ -189,24 +202,42 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.z;
     double dsum = 0.0;
+    if (L.split) {
+        // Two phases over the thread's rows: every load (b, the other
+        // colour's neighbours, the own value of the residual pass) is issued
+        // before the first store, so the rows' loads are in flight together.
+        // Stores go to this colour's half and x is read from the other half
+        // (the residual pass reads its own cells but does not write x), so
+        // the order changes no value.
+        T *own = x + (long)color * L.half;
+        T *oth = x + (long)(1 - color) * L.half;
+        bool ok[RB_ROWS];
+        int pos[RB_ROWS], dix[RB_ROWS];
+        TB bo[RB_ROWS];
+        T sn[RB_ROWS], ov[RB_ROWS];
 #pragma unroll
-    for (int rr = 0; rr < RB_ROWS; ++rr) {
-        const int j = (blockIdx.y * RB_ROWS + rr) * blockDim.y + threadIdx.y;
-        if (j >= L.ny) continue;
-        if (L.split) {
-            if (ip >= (L.nx >> 1)) continue;
-            const PairGeo g = pair_geo(L, ip, j, k);
+        for (int rr = 0; rr < RB_ROWS; ++rr) {
+            // rows and pairs past the level's end load from the last one
+            // (valid addresses, no branch) and are dropped in phase two
+            const int j = (blockIdx.y * RB_ROWS + rr) * blockDim.y + threadIdx.y;
+            ok[rr] = j < L.ny && ip < (L.nx >> 1);
+            const PairGeo g = pair_geo(L, min(ip, (L.nx >> 1) - 1), min(j, L.ny - 1), k);
             const int h = g.ca == color ? 0 : 1;   // which cell of the pair has this colour
-            T *own = x + (long)color * L.half;
-            T *oth = x + (long)(1 - color) * L.half;
-            const int o = g.pos;
-            const int di = (h ? g.cx1 : g.cx0) * 16 + g.cyz;
-            const TB bo = b[(long)color * L.half + o];
+            pos[rr] = g.pos;
+            dix[rr] = (h ? g.cx1 : g.cx0) * 16 + g.cyz;
+            bo[rr] = b[(long)color * L.half + g.pos];
+            sn[rr] = (ZI && RES != 2) ? T(0) : nbsum_pair(oth, g, h, L);
+            if (RES == 2) ov[rr] = own[g.pos];
+        }
+#pragma unroll
+        for (int rr = 0; rr < RB_ROWS; ++rr) {
+            if (!ok[rr]) continue;
+            const int o = pos[rr], di = dix[rr];
             if (RES == 2) {
                 T dg;
                 if (di == DIAG_INTERIOR) dg = dg_int;
                 else dg = T(L.diag[di]);
-                r[(long)color * L.half + o] = T(bo) - (dg * own[o] - nbsum_pair(oth, g, h, L));
+                r[(long)color * L.half + o] = T(bo[rr]) - (dg * ov[rr] - sn[rr]);
                 continue;
             }
             // cells with both neighbours in every dimension (all but the
@@ -214,18 +245,17 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
             T id;
             if (di == DIAG_INTERIOR) id = id_int;
             else id = T(L.idiag[di]);
-            if (zero_init) {
+            if (ZI) {
                 oth[o] = T(0);
-                own[o] = (T(bo) + T(0)) * id;
+                own[o] = (T(bo[rr]) + T(0)) * id;
             } else {
-                const T sn = nbsum_pair(oth, g, h, L);
-                const T xn = (T(bo) + sn) * id;
+                const T xn = (T(bo[rr]) + sn[rr]) * id;
                 own[o] = xn;
                 if (RES == 1) {
                     T dg;
                     if (di == DIAG_INTERIOR) dg = dg_int;
                     else dg = T(L.diag[di]);
-                    r[(long)color * L.half + o] = T(bo) - (dg * xn - sn);
+                    r[(long)color * L.half + o] = T(bo[rr]) - (dg * xn - sn[rr]);
                 }
                 if (DOT) {
                     const long go = (long)color * L.half + o;
@@ -233,38 +263,43 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
                         z64[go] = double(xn);
                         dsum += r64[go] * double(xn);
                     } else {
-                        dsum += double(bo) * double(xn);
+                        dsum += double(bo[rr]) * double(xn);
                     }
                 }
             }
-            continue;
         }
-        const int i = 2 * ip + ((j + k + color) & 1);
-        const int o = lidx(L, i, j, k);
-        if (zero_init) {
-            const int io = 2 * ip + ((j + k + color + 1) & 1);
-            if (io < L.nx) x[lidx(L, io, j, k)] = T(0);
+    } else {
+#pragma unroll
+        for (int rr = 0; rr < RB_ROWS; ++rr) {
+            const int j = (blockIdx.y * RB_ROWS + rr) * blockDim.y + threadIdx.y;
+            if (j >= L.ny) continue;
+            const int i = 2 * ip + ((j + k + color) & 1);
+            const int o = lidx(L, i, j, k);
+            if (ZI) {
+                const int io = 2 * ip + ((j + k + color + 1) & 1);
+                if (io < L.nx) x[lidx(L, io, j, k)] = T(0);
+                if (i >= L.nx) continue;
+                const T id = T(L.idiag[diag_index(L, i, j, k)]);
+                x[o] = (T(b[o]) + T(0)) * id;
+                continue;
+            }
             if (i >= L.nx) continue;
-            const T id = T(L.idiag[diag_index(L, i, j, k)]);
-            x[o] = (T(b[o]) + T(0)) * id;
-            continue;
-        }
-        if (i >= L.nx) continue;
-        const int di = diag_index(L, i, j, k);
-        const T sn = nbsum(x, L, i, j, k);
-        if (RES == 2) {
-            r[o] = T(b[o]) - (T(L.diag[di]) * x[o] - sn);
-            continue;
-        }
-        const T xn = (T(b[o]) + sn) * T(L.idiag[di]);
-        x[o] = xn;
-        if (RES == 1) r[o] = T(b[o]) - (T(L.diag[di]) * xn - sn);
-        if (DOT) {
-            if (z64) {
-                z64[o] = double(xn);
-                dsum += r64[o] * double(xn);
-            } else {
-                dsum += double(b[o]) * double(xn);
+            const int di = diag_index(L, i, j, k);
+            const T sn = nbsum(x, L, i, j, k);
+            if (RES == 2) {
+                r[o] = T(b[o]) - (T(L.diag[di]) * x[o] - sn);
+                continue;
+            }
+            const T xn = (T(b[o]) + sn) * T(L.idiag[di]);
+            x[o] = xn;
+            if (RES == 1) r[o] = T(b[o]) - (T(L.diag[di]) * xn - sn);
+            if (DOT) {
+                if (z64) {
+                    z64[o] = double(xn);
+                    dsum += r64[o] * double(xn);
+                } else {
+                    dsum += double(b[o]) * double(xn);
+                }
             }
         }
     }
@@ -388,6 +423,14 @@ __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, 
     int pk[2];
     parents(k, Cl.nz, Cl.bt[4], Cl.bt[5], pk[0], pk[1]);
     const int t = threadIdx.y * bx + threadIdx.x;
+    // the thread's two fine values are loaded first, so their latency
+    // overlaps the staging of the coarse tile
+    const int ip = blockIdx.x * bx + threadIdx.x;
+    const int j = blockIdx.y * by + threadIdx.y;
+    T xo[2] = {T(0), T(0)};
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+        if (j < F.ny && 2 * ip + h < F.nx) xo[h] = xf[lidx(F, 2 * ip + h, j, k)];
     // up to four loads of a thread in flight before the LDS stores wait
     for (int q0 = t; q0 < 2 * TY * TX; q0 += 4 * bx * by) {
         T v[4];
@@ -408,8 +451,6 @@ __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, 
         }
     }
     __syncthreads();
-    const int ip = blockIdx.x * bx + threadIdx.x;
-    const int j = blockIdx.y * by + threadIdx.y;
     if (j >= F.ny) return;
     const T wt[2] = {T(0.75), T(0.25)};
     // tile rows of the own and the neighbour coarse row
@@ -426,8 +467,7 @@ __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, 
                     T w = wt[c3] * wt[c2] * wt[c1];
                     s = s + w * tile[(c3 * TY + ty[c2]) * TX + tx[c1]];
                 }
-        const int o = lidx(F, i, j, k);
-        xf[o] = xf[o] + s;
+        xf[lidx(F, i, j, k)] = xo[h] + s;
     }
 }
 
@@ -768,16 +808,25 @@ __global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__r
         if (ip < (L.nx >> 1)) {
             const PairGeo g = pair_geo(L, ip, j, k);
             const double dg_int = L.diag[DIAG_INTERIOR];
+            // both cells' loads are issued before either store (Ap and p are
+            // distinct arrays; the order changes no value)
+            double pv[2], a[2];
+            long oo[2];
+#pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const long own = (long)(h ? 1 - g.ca : g.ca) * L.half, oth = (long)(h ? g.ca : 1 - g.ca) * L.half;
-                const long o = own + g.pos;
+                oo[h] = own + g.pos;
                 const int di = (h ? g.cx1 : g.cx0) * 16 + g.cyz;
                 double dg;
                 if (di == DIAG_INTERIOR) dg = dg_int;
                 else dg = L.diag[di];
-                const double a = dg * p[o] - nbsum_pair(p + oth, g, h, L);
-                Ap[o] = a;
-                s += p[o] * a;
+                pv[h] = p[oo[h]];
+                a[h] = dg * pv[h] - nbsum_pair(p + oth, g, h, L);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                Ap[oo[h]] = a[h];
+                s += pv[h] * a[h];
             }
         }
     } else if (j < L.ny) {
@@ -1020,17 +1069,20 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
         if (dot_slot >= 0) {
             // the mixed form reads the FP64 r and writes the FP64 z as well
             const bool mixed = !std::is_same<T, double>::value;
-            hipLaunchKernelGGL((k_rb_half<T, T, true>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero,
+            hipLaunchKernelGGL((k_rb_half<T, T, true>), rbh, rb.block, 0, s, d, L.x, L.b, color,
                                c->scal + SC_CONV, c->partial, dot_slot * nbh, (T *)nullptr,
                                mixed ? c->cg_r : nullptr, mixed ? c->cg_z : nullptr);
             grp_bytes += hb_dot;
             return;
         }
         if (res)
-            hipLaunchKernelGGL((k_rb_half<T, T, false, 1>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero,
+            hipLaunchKernelGGL((k_rb_half<T, T, false, 1>), rbh, rb.block, 0, s, d, L.x, L.b, color,
                                c->scal + SC_CONV, (double *)nullptr, 0L, L.r);
+        else if (zero)
+            hipLaunchKernelGGL((k_rb_half<T, T, false, 0, true>), rbh, rb.block, 0, s, d, L.x, L.b, color,
+                               c->scal + SC_CONV, (double *)nullptr, 0L, (T *)nullptr);
         else
-            hipLaunchKernelGGL((k_rb_half<T, T>), rbh, rb.block, 0, s, d, L.x, L.b, color, zero, c->scal + SC_CONV,
+            hipLaunchKernelGGL((k_rb_half<T, T>), rbh, rb.block, 0, s, d, L.x, L.b, color, c->scal + SC_CONV,
                                (double *)nullptr, 0L, (T *)nullptr);
         grp_bytes += res ? hb_res : hb;
     };
@@ -1043,7 +1095,7 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
         CHECK_LAUNCH(c);
     }
     group_end();
-    hipLaunchKernelGGL((k_rb_half<T, T, false, 2>), rbh, rb.block, 0, s, d, L.x, L.b, 0, 0, c->scal + SC_CONV,
+    hipLaunchKernelGGL((k_rb_half<T, T, false, 2>), rbh, rb.block, 0, s, d, L.x, L.b, 0, c->scal + SC_CONV,
                        (double *)nullptr, 0L, L.r);
     CHECK_LAUNCH(c);
     MGLevel<T> &C = lev[l + 1];
