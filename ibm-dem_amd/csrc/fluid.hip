@@ -7,6 +7,8 @@
 // a contiguous slab of z planes (neighbouring planes share its L2).
 #include "common.hpp"
 
+#include <type_traits>
+
 #include <cmath>
 
 namespace {
@@ -301,6 +303,81 @@ __device__ __forceinline__ void load_plane(LT (*dst)[LX], const double *__restri
     }
 }
 
+// The same plane loads on grids without open sides, where every ghost rule
+// is a product of per-axis maps (map_index, map_scal: mapped index, sign,
+// forced zero). A thread's plane elements (its slots) are the same in every
+// plane, so their x and y maps are evaluated once per thread; a plane adds
+// only its z map, the same for the whole block.
+template <int TY>
+struct PlaneSlots {
+    static constexpr int NE = (TY + 2) * LX, NT = BX * TY, NS = (NE + NT - 1) / NT;
+    int off[4][NS];    // x/y part of the linear index per field (u, v, w, P)
+    unsigned neg;      // bit 4 q + f: the x/y maps negate
+    unsigned zero;     // bit 4 q + f: forced zero (a wall-normal ghost or an element not loaded)
+    bool has[NS];      // the slot is one of the plane's elements
+};
+template <int TY>
+__device__ __forceinline__ void plane_slots(PlaneSlots<TY> &S, const GridDev &g, int i0, int j0) {
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    S.neg = 0u;
+    S.zero = 0u;
+#pragma unroll
+    for (int q = 0; q < PlaneSlots<TY>::NS; ++q) {
+        const int e = tid + q * PlaneSlots<TY>::NT;
+        S.has[q] = e < PlaneSlots<TY>::NE;
+        const int ly = e / LX, lx = e - ly * LX;
+        const int i = i0 - 1 + lx, j = j0 - 1 + ly;
+        const bool inside = S.has[q] && i <= g.nx + 1 && j <= g.ny + 1;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            int ii = i, jj = j;
+            double sg = 1.0;
+            bool ok = true;
+            if (f == 3) {
+                map_scal(ii, g.nx, g.bt[0], g.bt[1], sg);
+                map_scal(jj, g.ny, g.bt[2], g.bt[3], sg);
+            } else {
+                ok = (f == 0 ? map_index<true>(ii, g.nx, g.bt[0], g.bt[1], sg)
+                             : map_index<false>(ii, g.nx, g.bt[0], g.bt[1], sg)) &&
+                     (f == 1 ? map_index<true>(jj, g.ny, g.bt[2], g.bt[3], sg)
+                             : map_index<false>(jj, g.ny, g.bt[2], g.bt[3], sg));
+            }
+            ok = ok && inside;
+            S.off[f][q] = ok ? (jj - 1) * g.nx + (ii - 1) : 0;
+            if (!ok) S.zero |= 1u << (4 * q + f);
+            if (sg < 0) S.neg |= 1u << (4 * q + f);
+        }
+    }
+}
+// plane k of field F (0..2 velocity components, 3 P) through the slots
+template <typename LT, int TY, int F>
+__device__ __forceinline__ void load_plane_slots(LT (*dst)[LX], const double *__restrict__ f, const GridDev &g,
+                                                 const PlaneSlots<TY> &S, int k) {
+    int kk = k;
+    double sg = 1.0;
+    bool ok = true;
+    if (F == 3) map_scal(kk, g.nz, g.bt[4], g.bt[5], sg);
+    else if (F == 2) ok = map_index<true>(kk, g.nz, g.bt[4], g.bt[5], sg);
+    else ok = map_index<false>(kk, g.nz, g.bt[4], g.bt[5], sg);
+    const long zoff = ok ? (long)(kk - 1) * g.nx * g.ny : 0;
+    const unsigned zneg = sg < 0 ? 1u : 0u;
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    double v[PlaneSlots<TY>::NS];
+#pragma unroll
+    for (int q = 0; q < PlaneSlots<TY>::NS; ++q) {
+        const bool z = !ok || ((S.zero >> (4 * q + F)) & 1u);
+        v[q] = f[z ? 0 : zoff + S.off[F][q]];
+        const unsigned neg = ((S.neg >> (4 * q + F)) & 1u) ^ zneg;
+        v[q] = neg ? -v[q] : v[q];
+        v[q] = z ? 0.0 : v[q];
+    }
+#pragma unroll
+    for (int q = 0; q < PlaneSlots<TY>::NS; ++q) {
+        const int e = tid + q * PlaneSlots<TY>::NT;
+        if (S.has[q]) (&dst[0][0])[e] = LT(v[q]);
+    }
+}
+
 template <typename CT, bool BULK, bool P2, bool OPEN, int TY, bool GEN>
 __global__ __launch_bounds__(BX *TY) void k_predictor_lds(GridDev g, const double *__restrict__ u,
                                                           const double *__restrict__ v,
@@ -318,20 +395,32 @@ __global__ __launch_bounds__(BX *TY) void k_predictor_lds(GridDev g, const doubl
     const int i = i0 + tx, j = j0 + ty;
     const int k0 = bz * kz + 1, k1 = min(g.nz, k0 + kz - 1);
     const bool mine = i <= g.nx && j <= g.ny;
+    PlaneSlots<TY> S;
+    if (!OPEN) plane_slots<TY>(S, g, i0, j0);
+    // plane kk of field F into dst
+    auto plane = [&](auto fld, CT (*dst)[LX], const double *__restrict__ f, int kk) {
+        constexpr int F = decltype(fld)::value;
+        if (OPEN) load_plane<CT, TY, F, OPEN>(dst, f, g, i0, j0, kk);
+        else load_plane_slots<CT, TY, F>(dst, f, g, S, kk);
+    };
+    using F0 = std::integral_constant<int, 0>;
+    using F1 = std::integral_constant<int, 1>;
+    using F2 = std::integral_constant<int, 2>;
+    using F3 = std::integral_constant<int, 3>;
     // planes k0-1 and k0 of u, v, w
-    load_plane<CT, TY, 0, OPEN>(L.v[0][(k0 - 1) % 3], u, g, i0, j0, k0 - 1);
-    load_plane<CT, TY, 1, OPEN>(L.v[1][(k0 - 1) % 3], v, g, i0, j0, k0 - 1);
-    load_plane<CT, TY, 2, OPEN>(L.v[2][(k0 - 1) % 3], w, g, i0, j0, k0 - 1);
-    load_plane<CT, TY, 0, OPEN>(L.v[0][k0 % 3], u, g, i0, j0, k0);
-    load_plane<CT, TY, 1, OPEN>(L.v[1][k0 % 3], v, g, i0, j0, k0);
-    load_plane<CT, TY, 2, OPEN>(L.v[2][k0 % 3], w, g, i0, j0, k0);
+    plane(F0{}, L.v[0][(k0 - 1) % 3], u, k0 - 1);
+    plane(F1{}, L.v[1][(k0 - 1) % 3], v, k0 - 1);
+    plane(F2{}, L.v[2][(k0 - 1) % 3], w, k0 - 1);
+    plane(F0{}, L.v[0][k0 % 3], u, k0);
+    plane(F1{}, L.v[1][k0 % 3], v, k0);
+    plane(F2{}, L.v[2][k0 % 3], w, k0);
     CT below = mine ? CT(scal_at(P, g, i, j, k0 - 1)) : CT(0);
     for (int k = k0; k <= k1; ++k) {
         const int sm = (k - 1) % 3, s0 = k % 3, sp = (k + 1) % 3;
-        load_plane<CT, TY, 0, OPEN>(L.v[0][sp], u, g, i0, j0, k + 1);
-        load_plane<CT, TY, 1, OPEN>(L.v[1][sp], v, g, i0, j0, k + 1);
-        load_plane<CT, TY, 2, OPEN>(L.v[2][sp], w, g, i0, j0, k + 1);
-        load_plane<CT, TY, 3, OPEN>(L.p, P, g, i0, j0, k);
+        plane(F0{}, L.v[0][sp], u, k + 1);
+        plane(F1{}, L.v[1][sp], v, k + 1);
+        plane(F2{}, L.v[2][sp], w, k + 1);
+        plane(F3{}, L.p, P, k);
         __syncthreads();
         double s[3] = {0.0, 0.0, 0.0};
         if (mine) {
