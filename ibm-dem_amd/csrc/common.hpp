@@ -152,6 +152,39 @@ __device__ __forceinline__ double vel_at_shift(const double *__restrict__ f, con
     return s < 0 ? -val : val;
 }
 
+// A tap of vel_at / vel_at_shift split into its reference (computed
+// without touching memory) and its value, so that a kernel can compute all
+// its taps' references, issue all their loads together and then form the
+// values: the same numbers as vel_at and vel_at_shift.
+struct VelRef {
+    long idx;    // linear index to load (0 when the tap is a forced zero)
+    bool zero;   // wall-normal ghost: the value is 0
+    bool neg;    // tangential mirror: the value is negated
+    bool face;   // the mapped cell is the low face of a non-periodic side
+};
+template <int C>
+__device__ __forceinline__ VelRef vel_ref(const GridDev &g, int i, int j, int k) {
+    double s = 1.0;
+    const bool ok = map_index<C == 0>(i, g.nx, g.bt[0], g.bt[1], s) &&
+                    map_index<C == 1>(j, g.ny, g.bt[2], g.bt[3], s) &&
+                    map_index<C == 2>(k, g.nz, g.bt[4], g.bt[5], s);
+    const int idx = C == 0 ? i : (C == 1 ? j : k);
+    return VelRef{ok ? cidx(g, i, j, k) : 0L, !ok, s < 0, idx == 1 && g.bt[2 * C] != IBMDEM_BC_PERIODIC};
+}
+// vel_at's value from the tap's loaded element
+__device__ __forceinline__ double vel_val(const VelRef &r, double loaded) {
+    const double v = r.neg ? -loaded : loaded;
+    return r.zero ? 0.0 : v;
+}
+// vel_at_shift's value from the tap's loaded element
+template <int C>
+__device__ __forceinline__ double vel_val_shift(const GridDev &g, const VelRef &r, double loaded, double sh) {
+    if (r.zero) return 0.0;
+    if (r.face) return face_value<C>(g);
+    const double val = loaded + sh;
+    return r.neg ? -val : val;
+}
+
 // one index of a cell-centred scalar: wrap, or mirror (negated beyond an
 // outflow side, where the scalar is zero at the face)
 __device__ __forceinline__ void map_scal(int &i, int n, int lo, int hi, double &sgn) {
@@ -173,6 +206,16 @@ __device__ __forceinline__ T scal_at(const T *__restrict__ f, const GridDev &g, 
     map_scal(k, g.nz, g.bt[4], g.bt[5], s);
     const T v = f[cidx(g, i, j, k)];
     return s < 0 ? -v : v;
+}
+
+// scal_at split into reference and value, as VelRef
+__device__ __forceinline__ long scal_ref(const GridDev &g, int i, int j, int k, bool &neg) {
+    double s = 1.0;
+    map_scal(i, g.nx, g.bt[0], g.bt[1], s);
+    map_scal(j, g.ny, g.bt[2], g.bt[3], s);
+    map_scal(k, g.nz, g.bt[4], g.bt[5], s);
+    neg = s < 0;
+    return cidx(g, i, j, k);
 }
 
 // is (i,j,k) the low boundary face of velocity component C (a wall face

@@ -549,12 +549,26 @@ __global__ __launch_bounds__(BX *BY) void k_bulk_apply(GridDev g, double *u, dou
 // OPEN: the outflow planes are first set to the predicted velocity of the
 // last interior face (zero normal gradient), by the thread of the last cell,
 // which reads that face as its own.
+// A divisor with its reciprocal: when it is a power of two, x / d is
+// exactly x * (1/d) (a scaling by a power of two rounds once, to the same
+// value), and the kernel multiplies instead of running the division sequence.
+struct Divisor {
+    double d, inv;
+    int p2;
+};
+inline Divisor divisor(double d) { return Divisor{d, 1.0 / d, is_pow2(d) ? 1 : 0}; }
+__device__ __forceinline__ double dv(double x, const Divisor &D) {
+    if (D.p2) return x * D.inv;
+    return x / D.d;
+}
+
 template <bool OPEN>
 __global__ __launch_bounds__(BX *BY) void k_divergence(GridDev g, const double *__restrict__ u,
                                                        const double *__restrict__ v,
                                                        const double *__restrict__ w, double *__restrict__ rhs,
-                                                       const double *__restrict__ scal, int shifted, double dt,
-                                                       double cRK, double rhof) {
+                                                       const double *__restrict__ scal, int shifted, Divisor ddx,
+                                                       Divisor ddy, Divisor ddz, Divisor ddt, Divisor dcrk,
+                                                       double rhof) {
     int bx, by, bz;
     xcd_block(bx, by, bz);
     const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
@@ -573,17 +587,25 @@ __global__ __launch_bounds__(BX *BY) void k_divergence(GridDev g, const double *
         else v1 = vel_at<1, true>(v, g, i, j + 1, k);
         if (k == g.nz && g.bt[5] == IBMDEM_BC_OUTFLOW) { w1 = w[o]; g.plane[2][plane_off<2>(g, i, j, k)] = w1; }
         else w1 = vel_at<2, true>(w, g, i, j, k + 1);
-    } else if (shifted) {
-        u1 = vel_at_shift<0>(u, g, i + 1, j, k, su); u0 = vel_at_shift<0>(u, g, i, j, k, su);
-        v1 = vel_at_shift<1>(v, g, i, j + 1, k, sv); v0 = vel_at_shift<1>(v, g, i, j, k, sv);
-        w1 = vel_at_shift<2>(w, g, i, j, k + 1, sw); w0 = vel_at_shift<2>(w, g, i, j, k, sw);
     } else {
-        u1 = vel_at<0>(u, g, i + 1, j, k); u0 = vel_at<0>(u, g, i, j, k);
-        v1 = vel_at<1>(v, g, i, j + 1, k); v0 = vel_at<1>(v, g, i, j, k);
-        w1 = vel_at<2>(w, g, i, j, k + 1); w0 = vel_at<2>(w, g, i, j, k);
+        // the six taps' references first, then their loads together
+        const VelRef a1 = vel_ref<0>(g, i + 1, j, k), a0 = vel_ref<0>(g, i, j, k);
+        const VelRef b1 = vel_ref<1>(g, i, j + 1, k), b0 = vel_ref<1>(g, i, j, k);
+        const VelRef c1 = vel_ref<2>(g, i, j, k + 1), c0 = vel_ref<2>(g, i, j, k);
+        const double la1 = u[a1.idx], la0 = u[a0.idx], lb1 = v[b1.idx], lb0 = v[b0.idx];
+        const double lc1 = w[c1.idx], lc0 = w[c0.idx];
+        if (shifted) {
+            u1 = vel_val_shift<0>(g, a1, la1, su); u0 = vel_val_shift<0>(g, a0, la0, su);
+            v1 = vel_val_shift<1>(g, b1, lb1, sv); v0 = vel_val_shift<1>(g, b0, lb0, sv);
+            w1 = vel_val_shift<2>(g, c1, lc1, sw); w0 = vel_val_shift<2>(g, c0, lc0, sw);
+        } else {
+            u1 = vel_val(a1, la1); u0 = vel_val(a0, la0);
+            v1 = vel_val(b1, lb1); v0 = vel_val(b0, lb0);
+            w1 = vel_val(c1, lc1); w0 = vel_val(c0, lc0);
+        }
     }
-    double d = (u1 - u0) / g.dx + (v1 - v0) / g.dy + (w1 - w0) / g.dz;
-    rhs[cidx(g, i, j, k)] = -d / dt / cRK * rhof;
+    double d = dv(u1 - u0, ddx) + dv(v1 - v0, ddy) + dv(w1 - w0, ddz);
+    rhs[cidx(g, i, j, k)] = dv(dv(-d, ddt), dcrk) * rhof;
 }
 
 // PressureCorrection + VelocityCorrection(c) (fluid.F90:206, 221-223):
@@ -595,40 +617,55 @@ template <bool OPEN>
 __global__ __launch_bounds__(BX *BY) void k_project(GridDev g, double *u, double *v, double *w,
                                                     const double *__restrict__ P, const double *__restrict__ Pc,
                                                     double *__restrict__ Pn, const double *__restrict__ scal,
-                                                    int shifted, double dt, double cRK, double rhof) {
+                                                    int shifted, double dt, double cRK, Divisor ddx, Divisor ddy,
+                                                    Divisor ddz, Divisor drho) {
     int bx, by, bz;
     xcd_block(bx, by, bz);
     const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
     if (i > g.nx || j > g.ny) return;
     const long o = cidx(g, i, j, k);
+    double *vel[3] = {u, v, w};
+    // every load first (the cell's own u, v, w are read and written by this
+    // thread only): the lower neighbours' P and P' through the scalar ghost
+    // rule, the cell's P, P' and velocities
+    long nb[3];
+    bool ng[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nb[c] = scal_ref(g, c == 0 ? i - 1 : i, c == 1 ? j - 1 : j, c == 2 ? k - 1 : k, ng[c]);
     const double p0 = P[o], q0 = Pc[o];
+    double pml[3], qml[3], usl[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        pml[c] = P[nb[c]];
+        qml[c] = Pc[nb[c]];
+        usl[c] = vel[c][o];
+    }
     const double pn0 = p0 + q0;
     Pn[o] = pn0;
-    const double d[3] = {g.dx, g.dy, g.dz};
-    double *vel[3] = {u, v, w};
+    const Divisor d[3] = {ddx, ddy, ddz};
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const int im = c == 0 ? i - 1 : i, jm = c == 1 ? j - 1 : j, km = c == 2 ? k - 1 : k;
-        const double pm = scal_at(P, g, im, jm, km), qm = scal_at(Pc, g, im, jm, km);
-        const double Pold_grad = (p0 - pm) / d[c] / rhof;
+        const double pm = ng[c] ? -pml[c] : pml[c], qm = ng[c] ? -qml[c] : qml[c];
+        const double Pold_grad = dv(dv(p0 - pm, d[c]), drho);
         const double pnm = pm + qm;
         const bool face = (c == 0 ? i : c == 1 ? j : k) == 1 && g.bt[2 * c] != IBMDEM_BC_PERIODIC;
         if (face) {
             vel[c][o] = g.bt[2 * c] == IBMDEM_BC_INFLOW ? g.uin[c] : 0.0;
             continue;
         }
-        double us = vel[c][o];
+        double us = usl[c];
         if (shifted) us = us + scal[SC_MODP + c];
-        vel[c][o] = us + cRK * Pold_grad * dt - cRK * (pn0 - pnm) / (d[c]) * dt / rhof;
+        vel[c][o] = us + cRK * Pold_grad * dt - dv(dv(cRK * (pn0 - pnm), d[c]) * dt, drho);
     }
     if (OPEN) {
         const int ijk[3] = {i, j, k}, n3[3] = {g.nx, g.ny, g.nz};
         for (int c = 0; c < 3; ++c) {
             if (ijk[c] != n3[c] || g.bt[2 * c + 1] != IBMDEM_BC_OUTFLOW) continue;
             const double pg = -p0, png = -pn0;
-            const double Pold_grad = (pg - p0) / d[c] / rhof;
+            const double Pold_grad = dv(dv(pg - p0, d[c]), drho);
             const long po = c == 0 ? plane_off<0>(g, i, j, k) : (c == 1 ? plane_off<1>(g, i, j, k) : plane_off<2>(g, i, j, k));
             const double us = g.plane[c][po];
-            g.plane[c][po] = us + cRK * Pold_grad * dt - cRK * (png - pn0) / (d[c]) * dt / rhof;
+            g.plane[c][po] = us + cRK * Pold_grad * dt - dv(dv(cRK * (png - pn0), d[c]) * dt, drho);
         }
     }
 }
@@ -880,12 +917,12 @@ int fluid_divergence(ibmdem_ctx *c, double cc) {
     prof_begin(c, IBMDEM_K_DIVERGENCE);
     if (grid_open(g))
         hipLaunchKernelGGL(k_divergence<true>, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
-                           c->v[c->vcur], c->w[c->vcur], c->rhs, c->scal, c->bulk_pending ? 1 : 0, c->cs.dt, cc,
-                           c->cs.rhof);
+                           c->v[c->vcur], c->w[c->vcur], c->rhs, c->scal, c->bulk_pending ? 1 : 0, divisor(g.dx),
+                           divisor(g.dy), divisor(g.dz), divisor(c->cs.dt), divisor(cc), c->cs.rhof);
     else
         hipLaunchKernelGGL(k_divergence<false>, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
-                           c->v[c->vcur], c->w[c->vcur], c->rhs, c->scal, c->bulk_pending ? 1 : 0, c->cs.dt, cc,
-                           c->cs.rhof);
+                           c->v[c->vcur], c->w[c->vcur], c->rhs, c->scal, c->bulk_pending ? 1 : 0, divisor(g.dx),
+                           divisor(g.dy), divisor(g.dz), divisor(c->cs.dt), divisor(cc), c->cs.rhof);
     CHECK_LAUNCH(c);
     prof_end(c, IBMDEM_K_DIVERGENCE, 4.0 * 8.0 * (double)c->n);
     return IBMDEM_OK;
@@ -898,11 +935,13 @@ int fluid_project(ibmdem_ctx *c, double cc) {
     if (grid_open(g))
         hipLaunchKernelGGL(k_project<true>, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
                            c->v[c->vcur], c->w[c->vcur], c->P[c->pcur], c->pcorr, c->P[pn], c->scal,
-                           c->bulk_pending ? 1 : 0, c->cs.dt, cc, c->cs.rhof);
+                           c->bulk_pending ? 1 : 0, c->cs.dt, cc, divisor(g.dx), divisor(g.dy), divisor(g.dz),
+                           divisor(c->cs.rhof));
     else
         hipLaunchKernelGGL(k_project<false>, field_grid(g), dim3(BX, BY), 0, c->stream, g, c->u[c->vcur],
                            c->v[c->vcur], c->w[c->vcur], c->P[c->pcur], c->pcorr, c->P[pn], c->scal,
-                           c->bulk_pending ? 1 : 0, c->cs.dt, cc, c->cs.rhof);
+                           c->bulk_pending ? 1 : 0, c->cs.dt, cc, divisor(g.dx), divisor(g.dy), divisor(g.dz),
+                           divisor(c->cs.rhof));
     CHECK_LAUNCH(c);
     prof_end(c, IBMDEM_K_PROJECT, 9.0 * 8.0 * (double)c->n);
     c->pcur = pn;
