@@ -240,31 +240,56 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__res
                              (OPEN ? (mx.cst | mx.pln | my.cst | my.pln | mz.cst | mz.pln) : 0u);
     const bool plain = special == 0u;
     if (plain) {
+        // Each component's 36 taps are loaded before its sum starts, so a
+        // wavefront keeps them in flight together; the sum then runs in the
+        // reference's (kk, jj, ii) order.
         // u: x-face taps -1..2, y/z centre taps -1..1 (the nonzero ranges)
+        double t[36];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii) t[(kk * 3 + jj) * 4 + ii] = u[tap_off(g, mx, my, mz, ii, jj, kk)];
+        __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the sum
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk)
 #pragma unroll
             for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
                 for (int ii = 0; ii < 4; ++ii)
-                    ulp = ulp + interp_term<P2>(fx[ii] * cy[jj] * cz[kk], u[tap_off(g, mx, my, mz, ii, jj, kk)],
-                                                dV, idV, dx, dy, dz);
+                    ulp = ulp + interp_term<P2>(fx[ii] * cy[jj] * cz[kk], t[(kk * 3 + jj) * 4 + ii], dV, idV, dx, dy,
+                                                dz);
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii) t[(kk * 4 + jj) * 3 + ii] = v[tap_off(g, mx, my, mz, ii, jj, kk)];
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk)
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
                 for (int ii = 0; ii < 3; ++ii)
-                    vlp = vlp + interp_term<P2>(cx[ii] * fy[jj] * cz[kk], v[tap_off(g, mx, my, mz, ii, jj, kk)],
-                                                dV, idV, dx, dy, dz);
+                    vlp = vlp + interp_term<P2>(cx[ii] * fy[jj] * cz[kk], t[(kk * 4 + jj) * 3 + ii], dV, idV, dx, dy,
+                                                dz);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii) t[(kk * 3 + jj) * 3 + ii] = w[tap_off(g, mx, my, mz, ii, jj, kk)];
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
             for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
                 for (int ii = 0; ii < 3; ++ii)
-                    wlp = wlp + interp_term<P2>(cx[ii] * cy[jj] * fz[kk], w[tap_off(g, mx, my, mz, ii, jj, kk)],
-                                                dV, idV, dx, dy, dz);
+                    wlp = wlp + interp_term<P2>(cx[ii] * cy[jj] * fz[kk], t[(kk * 3 + jj) * 3 + ii], dV, idV, dx, dy,
+                                                dz);
     } else {
         for (int kk = 0; kk < 3; ++kk)
             for (int jj = 0; jj < 3; ++jj)
