@@ -85,28 +85,47 @@ __global__ __launch_bounds__(T) void k_chunk_sum(long n, const int *__restrict__
     if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
+// exclusive scan of the nb chunk sums in place (a[nb] = total), one
+// work-group: 4096 values per pass, four consecutive per thread, a scan
+// across each wavefront by shuffles and the 16 wavefront totals from LDS
 __global__ __launch_bounds__(1024) void k_scan_blocks(int nb, int *__restrict__ a) {
-    __shared__ int buf[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nb; base += 1024) {
-        const int q = base + threadIdx.x;
-        int v = q < nb ? a[q] : 0;
-        buf[threadIdx.x] = v;
-        __syncthreads();
-        for (int st = 1; st < 1024; st <<= 1) {
-            int t = threadIdx.x >= st ? buf[threadIdx.x - st] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += t;
-            __syncthreads();
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int carry = 0;
+    for (int base = 0; base < nb; base += 4096) {
+        int v[4], s = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = base + 4 * t + q;
+            v[q] = idx < nb ? a[idx] : 0;
+            s += v[q];
         }
-        if (q < nb) a[q] = carry + buf[threadIdx.x] - v;
+        int x = s;   // inclusive scan of s over the wavefront
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
         __syncthreads();
-        if (threadIdx.x == 1023) carry += buf[1023];
-        __syncthreads();
+        int before = 0, tot = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int ws = wsum[q];
+            before += q < w ? ws : 0;
+            tot += ws;
+        }
+        int run = carry + before + x - s;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = base + 4 * t + q;
+            if (idx < nb) a[idx] = run;
+            run += v[q];
+        }
+        carry += tot;
+        __syncthreads();   // wsum is rewritten by the next pass
     }
-    if (threadIdx.x == 0) a[nb] = carry;
+    if (t == 0) a[nb] = carry;
 }
 
 // exclusive scan of counts into start (CHUNK cells per block, 16 per
