@@ -472,19 +472,32 @@ __device__ __forceinline__ void body_image(const GridDev &g, const double *bx, i
 }
 
 // EvolutionMS for one body (couplesolverCylinder.F90:14-40)
+// One thread per body: every body value is loaded before the first store
+// (the state arrays are written through the same pointers, so loads after a
+// store would each wait for the one before), then the update, then the
+// stores. The arithmetic is EvolutionMS's in its order.
 __device__ void evolve_body(const BodyState &s, int B, int b, double dtD, double rhof, const double grav[3]) {
     if (s.fixed[b]) return;
-    const double ratio = (s.rho[b] - rhof) / s.rho[b];
-    double *rot = s.rot + 9L * b, *rg = s.rotg + 9L * b;
-    for (int d = 0; d < 3; ++d)
-        s.v[(long)d * B + b] = s.v[(long)d * B + b] + s.fhydro[(long)d * B + b] * dtD +
-                               s.fcol[(long)d * B + b] / s.m[b] * dtD + ratio * grav[d] * dtD;
+    double v[3], fh[3], fc[3], om[3], L[3], fmh[3], fmc[3], x[3], Iv[9], rgl[9];
+    const double rho = s.rho[b], mb = s.m[b];
+    for (int d = 0; d < 3; ++d) {
+        const long o = (long)d * B + b;
+        v[d] = s.v[o]; fh[d] = s.fhydro[o]; fc[d] = s.fcol[o]; om[d] = s.om[o];
+        L[d] = s.L[o]; fmh[d] = s.fmomhydro[o]; fmc[d] = s.fmomcol[o]; x[d] = s.x[o];
+    }
+    for (int q = 0; q < 9; ++q) {
+        Iv[q] = s.I[9L * b + q];
+        rgl[q] = s.rotg[9L * b + q];
+    }
+    const double ratio = (rho - rhof) / rho;
+    for (int d = 0; d < 3; ++d) v[d] = v[d] + fh[d] * dtD + fc[d] / mb * dtD + ratio * grav[d] * dtD;
     // CalculateRotateMatrix(omega*dtDEM) (couplesolverCylinder.F90:76-97)
-    const double th0 = s.om[b] * dtD, th1 = s.om[(long)B + b] * dtD, th2 = s.om[2L * B + b] * dtD;
+    const double th0 = om[0] * dtD, th1 = om[1] * dtD, th2 = om[2] * dtD;
     double mag = sqrt(th0 * th0 + th1 * th1 + th2 * th2);
     mag = mag > 1e-10 ? mag : 1e-10;
     const double ux = th0 / mag, uy = th1 / mag, uz = th2 / mag;
     const double cm = cos(mag), sm = sin(mag);
+    double rot[9];
     rot[0] = (1 - cm) * ux * ux + cm;
     rot[1] = (1 - cm) * ux * uy - sm * uz;
     rot[2] = (1 - cm) * ux * uz + sm * uy;
@@ -495,26 +508,27 @@ __device__ void evolve_body(const BodyState &s, int B, int b, double dtD, double
     rot[7] = (1 - cm) * uz * uy + sm * ux;
     rot[8] = (1 - cm) * uz * uz + cm;
     double ITInv[9];
-    for (int q = 0; q < 9; ++q) ITInv[q] = s.I[9L * b + q];
+    for (int q = 0; q < 9; ++q) ITInv[q] = Iv[q];
     ITInv[0] = 1.0 / ITInv[0]; ITInv[4] = 1.0 / ITInv[4]; ITInv[8] = 1.0 / ITInv[8];
-    double rgl[9];
-    for (int q = 0; q < 9; ++q) rgl[q] = rg[q];
     tmul(rgl, rot, rgl);
-    for (int q = 0; q < 9; ++q) rg[q] = rgl[q];
     const double RMInv[9] = {rgl[0], rgl[3], rgl[6], rgl[1], rgl[4], rgl[7], rgl[2], rgl[5], rgl[8]};
     double T1[9], TM[9];
     tmul(rgl, ITInv, T1);
     tmul(T1, RMInv, TM);
-    double Lv[3];
-    for (int d = 0; d < 3; ++d) {
-        s.L[(long)d * B + b] = s.L[(long)d * B + b] + (s.fmomhydro[(long)d * B + b] + s.fmomcol[(long)d * B + b]) * dtD;
-        Lv[d] = s.L[(long)d * B + b];
+    for (int d = 0; d < 3; ++d) L[d] = L[d] + (fmh[d] + fmc[d]) * dtD;
+    const double o0 = TM[0] * L[0] + TM[1] * L[1] + TM[2] * L[2];
+    const double o1 = TM[3] * L[0] + TM[4] * L[1] + TM[5] * L[2];
+    const double o2 = TM[6] * L[0] + TM[7] * L[1] + TM[8] * L[2];
+    for (int d = 0; d < 3; ++d) x[d] = x[d] + v[d] * dtD;
+    for (int q = 0; q < 9; ++q) {
+        s.rot[9L * b + q] = rot[q];
+        s.rotg[9L * b + q] = rgl[q];
     }
-    const double o0 = TM[0] * Lv[0] + TM[1] * Lv[1] + TM[2] * Lv[2];
-    const double o1 = TM[3] * Lv[0] + TM[4] * Lv[1] + TM[5] * Lv[2];
-    const double o2 = TM[6] * Lv[0] + TM[7] * Lv[1] + TM[8] * Lv[2];
-    s.om[b] = o0; s.om[(long)B + b] = o1; s.om[2L * B + b] = o2;
-    for (int d = 0; d < 3; ++d) s.x[(long)d * B + b] = s.x[(long)d * B + b] + s.v[(long)d * B + b] * dtD;
+    const double omn[3] = {o0, o1, o2};
+    for (int d = 0; d < 3; ++d) {
+        const long o = (long)d * B + b;
+        s.v[o] = v[d]; s.L[o] = L[d]; s.om[o] = omn[d]; s.x[o] = x[d];
+    }
 }
 
 // mode 0: fcol, fmomcol only; 1: all four sums (AssembleLocalForceMS);
@@ -569,15 +583,23 @@ __global__ __launch_bounds__(BT) void k_body_reduce(int M, int B, GridDev g, con
         }
     }
     if (mode == 2) {
-        const double ratio = (s.rho[b] - rhof) / s.rho[b];
+        // the sums just stored are still in red; the rest is loaded before
+        // any store
+        const double rho = s.rho[b], mb = s.m[b];
+        double v[3], vo[3], L[3], Lo[3];
         for (int d = 0; d < 3; ++d) {
             const long o = (long)d * B + b;
-            double fh = (s.v[o] - s.vold[o]) / dt * (1.0 - ratio);
-            double fmh = (s.L[o] - s.Lold[o]) / dt * (1.0 - ratio);
-            s.vold[o] = s.v[o];
-            s.Lold[o] = s.L[o];
-            s.fhydro[o] = fh + s.fdrag[o] / s.m[b];
-            s.fmomhydro[o] = fmh + s.fmom[o];
+            v[d] = s.v[o]; vo[d] = s.vold[o]; L[d] = s.L[o]; Lo[d] = s.Lold[o];
+        }
+        const double ratio = (rho - rhof) / rho;
+        for (int d = 0; d < 3; ++d) {
+            const long o = (long)d * B + b;
+            const double fh = (v[d] - vo[d]) / dt * (1.0 - ratio);
+            const double fmh = (L[d] - Lo[d]) / dt * (1.0 - ratio);
+            s.vold[o] = v[d];
+            s.Lold[o] = L[d];
+            s.fhydro[o] = fh + red[d][0] / mb;
+            s.fmomhydro[o] = fmh + red[6 + d][0];
         }
     }
     if (integrate) {
