@@ -44,8 +44,10 @@ __global__ __launch_bounds__(T) void k_wrap_cells_count(int M, GridDev g, double
     const int n[3] = {g.nx, g.ny, g.nz};
     const double dd[3] = {g.dx, g.dy, g.dz};
     int c[3];
+    // the three coordinates are loaded before a wrapped one is stored back
+    const double x3[3] = {mx[m], mx[(long)M + m], mx[2L * M + m]};
     for (int d = 0; d < 3; ++d) {
-        double x = mx[(long)d * M + m];
+        double x = x3[d];
         int ci = cell_of(x, dd[d]);
         if (ci > n[d] && g.bt[2 * d + 1] == IBMDEM_BC_PERIODIC) {
             x = x + (-g.L[d]);
@@ -623,21 +625,36 @@ __global__ __launch_bounds__(T) void k_follow(int M, int B, GridDev g, double *_
     if (m >= M) return;
     const int b = mbody[m];
     double xm[3] = {mx[m], mx[(long)M + m], mx[2L * M + m]};
-    if (!s.fixed[b]) {
+    // the body's values are loaded together, before the fixed test
+    const bool fixed = s.fixed[b] != 0;
+    double bxv[3], bv[3], om[3], mat[9];
+    for (int d = 0; d < 3; ++d) {
+        bxv[d] = s.x[(long)d * B + b];
+        bv[d] = s.v[(long)d * B + b];
+        om[d] = s.om[(long)d * B + b];
+    }
+    for (int q = 0; q < 9; ++q) mat[q] = s.rot[9L * b + q];
+    if (!fixed) {
+        // body_image's rule on the loaded centre
         double xb[3];
-        body_image(g, s.x, B, b, xm, xb);
-        for (int d = 0; d < 3; ++d) xm[d] = xm[d] + s.v[(long)d * B + b] * dtD;
-        const double *mat = s.rot + 9L * b;
+        for (int d = 0; d < 3; ++d) {
+            xb[d] = bxv[d];
+            if (g.bt[2 * d] == IBMDEM_BC_PERIODIC) {
+                const double dd = xm[d] - xb[d];
+                if (dd > 0.5 * g.L[d]) xb[d] = xb[d] + g.L[d];
+                else if (dd < -0.5 * g.L[d]) xb[d] = xb[d] + (-g.L[d]);
+            }
+        }
+        for (int d = 0; d < 3; ++d) xm[d] = xm[d] + bv[d] * dtD;
         const double t0 = xm[0] - xb[0], t1 = xm[1] - xb[1], t2 = xm[2] - xb[2];
         xm[0] = mat[0] * t0 + mat[1] * t1 + mat[2] * t2;
         xm[1] = mat[3] * t0 + mat[4] * t1 + mat[5] * t2;
         xm[2] = mat[6] * t0 + mat[7] * t1 + mat[8] * t2;
         for (int d = 0; d < 3; ++d) xm[d] = xm[d] + xb[d];
-        const double om[3] = {s.om[b], s.om[(long)B + b], s.om[2L * B + b]};
         const double rr[3] = {xm[0] - xb[0], xm[1] - xb[1], xm[2] - xb[2]};
         double vv[3];
         cross3(om, rr, vv);
-        for (int d = 0; d < 3; ++d) mv[(long)d * M + m] = vv[d] + s.v[(long)d * B + b];
+        for (int d = 0; d < 3; ++d) mv[(long)d * M + m] = vv[d] + bv[d];
     }
     const double e[3] = {g.dx * 1e-20, g.dy * 1e-20, g.dz * 1e-20};
     for (int d = 0; d < 3; ++d) {
