@@ -378,6 +378,46 @@ __device__ __forceinline__ void load_plane_slots(LT (*dst)[LX], const double *__
     }
 }
 
+// The same split in two: the loads of a plane into registers (issued one
+// plane ahead, so they are in flight during the current plane's arithmetic)
+// and their ghost rule and LDS store.
+template <int TY>
+struct PlaneFetch {
+    double v[PlaneSlots<TY>::NS];
+    bool zok;        // the plane's z map is not a forced zero
+    unsigned zneg;   // the plane's z map negates
+};
+template <int TY, int F>
+__device__ __forceinline__ void plane_fetch(PlaneFetch<TY> &P, const double *__restrict__ f, const GridDev &g,
+                                            const PlaneSlots<TY> &S, int k) {
+    int kk = k;
+    double sg = 1.0;
+    bool ok = true;
+    if (F == 3) map_scal(kk, g.nz, g.bt[4], g.bt[5], sg);
+    else if (F == 2) ok = map_index<true>(kk, g.nz, g.bt[4], g.bt[5], sg);
+    else ok = map_index<false>(kk, g.nz, g.bt[4], g.bt[5], sg);
+    const long zoff = ok ? (long)(kk - 1) * g.nx * g.ny : 0;
+    P.zok = ok;
+    P.zneg = sg < 0 ? 1u : 0u;
+#pragma unroll
+    for (int q = 0; q < PlaneSlots<TY>::NS; ++q) {
+        const bool z = !ok || ((S.zero >> (4 * q + F)) & 1u);
+        P.v[q] = f[z ? 0 : zoff + S.off[F][q]];
+    }
+}
+template <typename LT, int TY, int F>
+__device__ __forceinline__ void plane_commit(LT (*dst)[LX], const PlaneFetch<TY> &P, const PlaneSlots<TY> &S) {
+    const int tid = threadIdx.x + BX * threadIdx.y;
+#pragma unroll
+    for (int q = 0; q < PlaneSlots<TY>::NS; ++q) {
+        const bool z = !P.zok || ((S.zero >> (4 * q + F)) & 1u);
+        const unsigned neg = ((S.neg >> (4 * q + F)) & 1u) ^ P.zneg;
+        double v = neg ? -P.v[q] : P.v[q];
+        v = z ? 0.0 : v;
+        if (S.has[q]) (&dst[0][0])[tid + q * PlaneSlots<TY>::NT] = LT(v);
+    }
+}
+
 template <typename CT, bool BULK, bool P2, bool OPEN, int TY, bool GEN>
 __global__ __launch_bounds__(BX *TY) void k_predictor_lds(GridDev g, const double *__restrict__ u,
                                                           const double *__restrict__ v,
@@ -415,12 +455,34 @@ __global__ __launch_bounds__(BX *TY) void k_predictor_lds(GridDev g, const doubl
     plane(F1{}, L.v[1][k0 % 3], v, k0);
     plane(F2{}, L.v[2][k0 % 3], w, k0);
     CT below = mine ? CT(scal_at(P, g, i, j, k0 - 1)) : CT(0);
+    // grids without open sides: planes k+1 of u, v, w and k of P are in
+    // registers when iteration k starts (the first from here)
+    PlaneFetch<TY> fu, fv, fw, fp;
+    if (!OPEN) {
+        plane_fetch<TY, 0>(fu, u, g, S, k0 + 1);
+        plane_fetch<TY, 1>(fv, v, g, S, k0 + 1);
+        plane_fetch<TY, 2>(fw, w, g, S, k0 + 1);
+        plane_fetch<TY, 3>(fp, P, g, S, k0);
+    }
     for (int k = k0; k <= k1; ++k) {
         const int sm = (k - 1) % 3, s0 = k % 3, sp = (k + 1) % 3;
-        plane(F0{}, L.v[0][sp], u, k + 1);
-        plane(F1{}, L.v[1][sp], v, k + 1);
-        plane(F2{}, L.v[2][sp], w, k + 1);
-        plane(F3{}, L.p, P, k);
+        if (OPEN) {
+            plane(F0{}, L.v[0][sp], u, k + 1);
+            plane(F1{}, L.v[1][sp], v, k + 1);
+            plane(F2{}, L.v[2][sp], w, k + 1);
+            plane(F3{}, L.p, P, k);
+        } else {
+            plane_commit<CT, TY, 0>(L.v[0][sp], fu, S);
+            plane_commit<CT, TY, 1>(L.v[1][sp], fv, S);
+            plane_commit<CT, TY, 2>(L.v[2][sp], fw, S);
+            plane_commit<CT, TY, 3>(L.p, fp, S);
+            if (k < k1) {
+                plane_fetch<TY, 0>(fu, u, g, S, k + 2);
+                plane_fetch<TY, 1>(fv, v, g, S, k + 2);
+                plane_fetch<TY, 2>(fw, w, g, S, k + 2);
+                plane_fetch<TY, 3>(fp, P, g, S, k + 1);
+            }
+        }
         __syncthreads();
         double s[3] = {0.0, 0.0, 0.0};
         if (mine) {
