@@ -558,15 +558,20 @@ __global__ __launch_bounds__(64) void k_tile_fill(GridDev g, TileGeom tg, const 
         return;
     }
     const int ib = tx * TL - 1, jb = ty * TL - 1, kb = tz * TL - 1;
-    for (int s = threadIdx.x; s < NSRC; s += 64) {
-        const int n = s_off[s + 1] - s_off[s];
-        if (n == 0) continue;
-        const int g0 = s_gstart[s];
+    // consecutive lanes write consecutive entries: entry e comes from the
+    // last source cell s with s_off[s] <= e (a binary search over the LDS
+    // offsets; empty cells share their offset with the next cell)
+    const int total = s_off[NSRC];
+    for (int e = threadIdx.x; e < total; e += 64) {
+        int lo = 0, hi = NSRC - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
         int scc[3];
-        src_coords(s, ib, jb, kb, scc);
-        const uint32_t code = image_code(g, scc);
-        for (int q = 0; q < n; ++q)
-            ent[base + s_off[s] + q] = ((uint32_t)bin_items[g0 + q] << SBITS) | code;
+        src_coords(lo, ib, jb, kb, scc);
+        ent[base + e] = ((uint32_t)bin_items[s_gstart[lo] + (e - s_off[lo])] << SBITS) | image_code(g, scc);
     }
 }
 
