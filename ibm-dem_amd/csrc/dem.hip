@@ -73,18 +73,26 @@ __global__ __launch_bounds__(T) void k_wrap_cells_count(int M, GridDev g, double
     atomicAdd(&bin_count[cl], 1);
 }
 
+// sum of the counts of each CHUNK cells (integers: any order is exact), by
+// wavefront shuffles and the T/64 wavefront totals
+__device__ __forceinline__ int wave_sum_int(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
 __global__ __launch_bounds__(T) void k_chunk_sum(long n, const int *__restrict__ a, int *__restrict__ out) {
-    __shared__ int red[T];
+    __shared__ int red[T / 64];
     const long lo = (long)blockIdx.x * CHUNK;
     int s = 0;
     for (long q = lo + threadIdx.x; q < min(n, lo + CHUNK); q += T) s += a[q];
-    red[threadIdx.x] = s;
+    s = wave_sum_int(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    for (int st = T / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < T / 64; ++w) t += red[w];
+        out[blockIdx.x] = t;
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 // exclusive scan of the nb chunk sums in place (a[nb] = total), one
@@ -137,7 +145,7 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int nb, int *__restrict__ 
 __global__ __launch_bounds__(T) void k_chunk_scan(long n, const int *__restrict__ cnt, const int *__restrict__ off,
                                                   int *__restrict__ start, int nb) {
     __shared__ int tile[CHUNK + CHUNK / 32];
-    __shared__ int buf[T];
+    __shared__ int buf[T / 64];
     constexpr int PER = CHUNK / T;
     const long base = (long)blockIdx.x * CHUNK;
     for (int q = 0; q < PER; ++q) {
@@ -152,15 +160,20 @@ __global__ __launch_bounds__(T) void k_chunk_scan(long n, const int *__restrict_
         loc[q] = tile[e + (e >> 5)];
         s += loc[q];
     }
-    buf[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = 1; st < T; st <<= 1) {
-        int t = threadIdx.x >= st ? buf[threadIdx.x - st] : 0;
-        __syncthreads();
-        buf[threadIdx.x] += t;
-        __syncthreads();
+    // exclusive prefix of the thread totals: a scan across each wavefront
+    // by shuffles, then the T/64 wavefront totals
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = s;
+#pragma unroll
+    for (int st = 1; st < 64; st <<= 1) {
+        const int y = __shfl_up(x, st, 64);
+        if (lane >= st) x += y;
     }
-    int pos = off[blockIdx.x] + buf[threadIdx.x] - s;
+    if (lane == 63) buf[wv] = x;
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < T / 64; ++w) before += w < wv ? buf[w] : 0;
+    int pos = off[blockIdx.x] + before + x - s;
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x * PER + q;
         tile[e + (e >> 5)] = pos;
