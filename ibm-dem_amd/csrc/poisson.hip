@@ -431,6 +431,41 @@ __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, 
 #pragma unroll
     for (int h = 0; h < 2; ++h)
         if (j < F.ny && 2 * ip + h < F.nx) xo[h] = xf[lidx(F, 2 * ip + h, j, k)];
+    if (bx == 64) {
+        // one wavefront per threadIdx.y: wavefront w stages columns 0..63 of
+        // tile rows w, w + by, ... (row r = c TY + y); a row's (J, plane) and
+        // its part of the storage index are scalars, the lane's column index
+        // is computed once; columns 64 and 65 are one more load for the first
+        // threads. All loads are issued before the LDS stores wait.
+        const int lane = threadIdx.x, w = threadIdx.y, NROW = 2 * TY;
+        const int half = Cl.split ? Cl.half : 0, rl = Cl.split ? Cl.nx >> 1 : Cl.nx;
+        const int Il = tile_coord(I0 + lane, Cl.nx, Cl.bt[0], Cl.bt[1]);
+        const int lo0 = Cl.split ? (Il & 1) * half + (Il >> 1) : Il;
+        const int lo1 = Cl.split ? ((Il & 1) ^ 1) * half + (Il >> 1) : Il;
+        T v[4], ve = T(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = w + u * by;
+            v[u] = T(0);
+            if (r < NROW) {
+                const int y = r % TY, c = r / TY;
+                const int J = tile_coord(J0 + y, Cl.ny, Cl.bt[2], Cl.bt[3]), K = pk[c];
+                v[u] = xc[(K * Cl.ny + J) * rl + (((J + K) & 1) ? lo1 : lo0)];
+            }
+        }
+        if (t < 2 * NROW) {
+            const int r = t >> 1, x = 64 + (t & 1), y = r % TY, c = r / TY;
+            const int I = tile_coord(I0 + x, Cl.nx, Cl.bt[0], Cl.bt[1]);
+            const int J = tile_coord(J0 + y, Cl.ny, Cl.bt[2], Cl.bt[3]);
+            ve = xc[lidx(Cl, I, J, pk[c])];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = w + u * by;
+            if (r < NROW) tile[r * TX + lane] = v[u];
+        }
+        if (t < 2 * NROW) tile[(t >> 1) * TX + 64 + (t & 1)] = ve;
+    } else
     // up to four loads of a thread in flight before the LDS stores wait
     for (int q0 = t; q0 < 2 * TY * TX; q0 += 4 * bx * by) {
         T v[4];
