@@ -982,12 +982,15 @@ __global__ __launch_bounds__(RB) void k_cg_init(LevelDev L, const double *__rest
         scal[SC_ITER] = 0.0;
         scal[SC_DONE] = 0.0;
     }
-    if (j >= L.ny) return;
-    for (int h = 0; h < 2; ++h) {
-        const int i = 2 * ip + h;
-        if (i >= L.nx) return;
-        const int q = lidx(L, i, j, k);
-        const double rq = (q == 0 && !L.dirichlet) ? -pin : rhs[((long)k * L.ny + j) * L.nx + i];
+    if (j >= L.ny || 2 * ip >= L.nx) return;
+    // both cells' right-hand sides are loaded before the stores (the second
+    // from the first's place when the row ends at an odd length)
+    const bool two = 2 * ip + 1 < L.nx;
+    const long f0 = ((long)k * L.ny + j) * L.nx + 2 * ip;
+    const double v[2] = {rhs[f0], rhs[f0 + (two ? 1 : 0)]};
+    for (int h = 0; h < (two ? 2 : 1); ++h) {
+        const int q = lidx(L, 2 * ip + h, j, k);
+        const double rq = (q == 0 && !L.dirichlet) ? -pin : v[h];
         r[q] = rq;
         if (r32) r32[q] = float(rq);   // the mixed form's FP32 copy
         x[q] = 0.0;
@@ -1017,12 +1020,14 @@ __global__ __launch_bounds__(RB) void k_gauge(LevelDev L, const double *__restri
     const double x0 = L.dirichlet ? 0.0 : xv(0);
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
-    if (j >= L.ny) return;
-    for (int h = 0; h < 2; ++h) {
-        const int i = 2 * ip + h;
-        if (i >= L.nx) return;
-        pc[((long)k * L.ny + j) * L.nx + i] = xv(lidx(L, i, j, k)) - x0;
-    }
+    if (j >= L.ny || 2 * ip >= L.nx) return;
+    // both cells' values are loaded before the stores (the second from the
+    // first's place when the row ends at an odd length)
+    const bool two = 2 * ip + 1 < L.nx;
+    const double v0 = xv(lidx(L, 2 * ip, j, k)), v1 = xv(lidx(L, two ? 2 * ip + 1 : 2 * ip, j, k));
+    const long f0 = ((long)k * L.ny + j) * L.nx + 2 * ip;
+    pc[f0] = v0 - x0;
+    if (two) pc[f0 + 1] = v1 - x0;
 }
 
 int stream_blocks(long n) { return (int)std::min<long>(4096, std::max<long>(1, (n + RED_T - 1) / RED_T)); }
