@@ -178,10 +178,29 @@ __device__ __forceinline__ T nbsum_pair(const T *__restrict__ oth, const PairGeo
     return sx + syy + szz;
 }
 
+// nbsum_pair from the loaded values: the pair value, the other x neighbour
+// (west for h = 0, east for h = 1), south, north, below, above (each read
+// at offset 0 when absent, and replaced by 0 here)
+template <typename T>
+__device__ __forceinline__ T nbsum_pair_vals(const PairGeo &g, int h, T part, T xo, T ys, T yn, T zb, T zt,
+                                             const LevelDev &L) {
+    const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+    const bool pw = h == 0 ? g.hw : true, pe = h == 0 ? true : g.he;
+    const T vw = h == 0 ? xo : part, ve = h == 0 ? part : xo;
+    const T xw = pw ? vw : T(0), xe = pe ? ve : T(0);
+    const T xs = g.hs ? ys : T(0), xn = g.hn ? yn : T(0);
+    const T xb = g.hb ? zb : T(0), xt = g.ht ? zt : T(0);
+    const T sx = (xw + xe) * ihx2;
+    const T syy = (xs + xn) * ihy2;
+    const T szz = (xb + xt) * ihz2;
+    return sx + syy + szz;
+}
+
 // one red-black half sweep; each thread owns the cell of the colour in a
-// pair of x cells, in RB_ROWS rows blockDim.y apart (independent cells, so
-// their loads are in flight together). ZI: the level's x is implicitly 0
-// (first sweep of a V-cycle visit); the other colour is written 0. DOT (the last two level-0 sweeps of the preconditioner, whose
+// pair of x cells, in RB_ROWS = 2 adjacent rows (independent cells, so
+// their loads are in flight together, two of them shared). ZI: the level's
+// x is implicitly 0 (first sweep of a V-cycle visit); the other colour is
+// written 0. DOT (the last two level-0 sweeps of the preconditioner, whose
 // b is the CG residual r and whose x is z = M^-1 r): also one partial sum of
 // r.z over the block's updated cells into partial[pofs + block]; in the
 // mixed form (z64 set) r is read from the FP64 r64 and the sweep also
@@ -191,6 +210,7 @@ __device__ __forceinline__ T nbsum_pair(const T *__restrict__ oth, const PairGeo
 // colour does not change in between); RES = 2: only that residual, for the
 // cells of `color`, x unchanged. Together they give r = b - A x.
 constexpr int RB_ROWS = 2;
+static_assert(RB_ROWS == 2, "the half sweep shares loads between its two adjacent rows");
 template <typename T, typename TB, bool DOT = false, int RES = 0, bool ZI = false>
 __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
                                                 int color, const double *__restrict__ stop,
@@ -215,19 +235,41 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
         int pos[RB_ROWS], dix[RB_ROWS];
         TB bo[RB_ROWS];
         T sn[RB_ROWS], ov[RB_ROWS];
+        // the thread's two rows are adjacent (j0, j0 + 1); rows and pairs
+        // past the level's end load from the last one (valid addresses, no
+        // branch) and are dropped in phase two
+        const int j0 = (blockIdx.y * blockDim.y + threadIdx.y) * RB_ROWS;
+        const int ipc = min(ip, (L.nx >> 1) - 1);
+        PairGeo g[RB_ROWS];
+        int h[RB_ROWS];
 #pragma unroll
         for (int rr = 0; rr < RB_ROWS; ++rr) {
-            // rows and pairs past the level's end load from the last one
-            // (valid addresses, no branch) and are dropped in phase two
-            const int j = (blockIdx.y * RB_ROWS + rr) * blockDim.y + threadIdx.y;
+            const int j = j0 + rr;
             ok[rr] = j < L.ny && ip < (L.nx >> 1);
-            const PairGeo g = pair_geo(L, min(ip, (L.nx >> 1) - 1), min(j, L.ny - 1), k);
-            const int h = g.ca == color ? 0 : 1;   // which cell of the pair has this colour
-            pos[rr] = g.pos;
-            dix[rr] = (h ? g.cx1 : g.cx0) * 16 + g.cyz;
-            bo[rr] = b[(long)color * L.half + g.pos];
-            sn[rr] = (ZI && RES != 2) ? T(0) : nbsum_pair(oth, g, h, L);
-            if (RES == 2) ov[rr] = own[g.pos];
+            g[rr] = pair_geo(L, ipc, min(j, L.ny - 1), k);
+            h[rr] = g[rr].ca == color ? 0 : 1;   // which cell of the pair has this colour
+            pos[rr] = g[rr].pos;
+            dix[rr] = (h[rr] ? g[rr].cx1 : g[rr].cx0) * 16 + g[rr].cyz;
+            bo[rr] = b[(long)color * L.half + g[rr].pos];
+            if (RES == 2) ov[rr] = own[g[rr].pos];
+        }
+        if (ZI && RES != 2) {
+            sn[0] = sn[1] = T(0);
+        } else {
+            // nbsum_pair of both rows with two loads shared: when both rows
+            // exist, row 0's north neighbour is row 1's pair value and row
+            // 1's south neighbour is row 0's
+            const T *q0 = oth + g[0].pos, *q1 = oth + g[1].pos;
+            const T a = q0[0], c = q1[0];
+            const T ys0 = q0[g[0].hs ? g[0].ds : 0], yn1 = q1[g[1].hn ? g[1].dn : 0];
+            const T xo0 = q0[h[0] == 0 ? (g[0].hw ? g[0].dw : 0) : (g[0].he ? g[0].de : 0)];
+            const T xo1 = q1[h[1] == 0 ? (g[1].hw ? g[1].dw : 0) : (g[1].he ? g[1].de : 0)];
+            const T zb0 = q0[g[0].hb ? g[0].db : 0], zt0 = q0[g[0].ht ? g[0].dt : 0];
+            const T zb1 = q1[g[1].hb ? g[1].db : 0], zt1 = q1[g[1].ht ? g[1].dt : 0];
+            T yn0 = c, ys1 = a;
+            if (j0 + 1 >= L.ny) yn0 = q0[g[0].hn ? g[0].dn : 0];   // row 1 does not exist
+            sn[0] = nbsum_pair_vals(g[0], h[0], a, xo0, ys0, yn0, zb0, zt0, L);
+            sn[1] = nbsum_pair_vals(g[1], h[1], c, xo1, ys1, yn1, zb1, zt1, L);
         }
 #pragma unroll
         for (int rr = 0; rr < RB_ROWS; ++rr) {
@@ -271,7 +313,7 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
     } else {
 #pragma unroll
         for (int rr = 0; rr < RB_ROWS; ++rr) {
-            const int j = (blockIdx.y * RB_ROWS + rr) * blockDim.y + threadIdx.y;
+            const int j = (blockIdx.y * blockDim.y + threadIdx.y) * RB_ROWS + rr;
             if (j >= L.ny) continue;
             const int i = 2 * ip + ((j + k + color) & 1);
             const int o = lidx(L, i, j, k);
