@@ -878,42 +878,77 @@ __global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__r
                                                    const double *__restrict__ scal) {
     if (scal[SC_CONV] != 0.0) return;
     __shared__ double red[RB];
+    // a thread takes the x-pair ip of two adjacent rows j0, j0 + 1 (blocks as
+    // the half sweep); when both rows exist, a cell's north neighbour in row
+    // j0 is row j0 + 1's value of the same cell of its pair, and a cell's
+    // south neighbour in row j0 + 1 is row j0's: those loads are shared
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    const int j0 = (blockIdx.y * blockDim.y + threadIdx.y) * 2, k = blockIdx.z;
     double s = 0.0;
-    if (j < L.ny && L.split) {
-        if (ip < (L.nx >> 1)) {
-            const PairGeo g = pair_geo(L, ip, j, k);
-            const double dg_int = L.diag[DIAG_INTERIOR];
-            // both cells' loads are issued before either store (Ap and p are
-            // distinct arrays; the order changes no value)
-            double pv[2], a[2];
-            long oo[2];
+    if (L.split) {
+        const int ipc = min(ip, (L.nx >> 1) - 1);
+        const double dg_int = L.diag[DIAG_INTERIOR];
+        const bool adj = j0 + 1 < L.ny;
+        PairGeo g[2];
+        bool ok[2];
+        long oo[2][2];
+        double pv[2][2], dgv[2][2], a[2][2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            ok[r] = j0 + r < L.ny && ip < (L.nx >> 1);
+            g[r] = pair_geo(L, ipc, min(j0 + r, L.ny - 1), k);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const long own = (long)(h ? 1 - g.ca : g.ca) * L.half, oth = (long)(h ? g.ca : 1 - g.ca) * L.half;
-                oo[h] = own + g.pos;
-                const int di = (h ? g.cx1 : g.cx0) * 16 + g.cyz;
+                oo[r][h] = (long)(h ? 1 - g[r].ca : g[r].ca) * L.half + g[r].pos;
+                const int di = (h ? g[r].cx1 : g[r].cx0) * 16 + g[r].cyz;
                 double dg;
                 if (di == DIAG_INTERIOR) dg = dg_int;
                 else dg = L.diag[di];
-                pv[h] = p[oo[h]];
-                a[h] = dg * pv[h] - nbsum_pair(p + oth, g, h, L);
-            }
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                Ap[oo[h]] = a[h];
-                s += pv[h] * a[h];
+                dgv[r][h] = dg;
+                pv[r][h] = p[oo[r][h]];
             }
         }
-    } else if (j < L.ny) {
-        for (int h = 0; h < 2; ++h) {
-            const int i = 2 * ip + h;
-            if (i >= L.nx) break;
-            const int o = lidx(L, i, j, k);
-            const double a = L.diag[diag_index(L, i, j, k)] * p[o] - nbsum(p, L, i, j, k);
-            Ap[o] = a;
-            s += p[o] * a;
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const PairGeo &G = g[r];
+                const double *q = p + (long)(h ? G.ca : 1 - G.ca) * L.half + G.pos;
+                const double xo = q[h == 0 ? (G.hw ? G.dw : 0) : (G.he ? G.de : 0)];
+                const double zb = q[G.hb ? G.db : 0], zt = q[G.ht ? G.dt : 0];
+                double ys, yn;
+                if (r == 0) {
+                    ys = q[G.hs ? G.ds : 0];
+                    if (adj) yn = pv[1][h];
+                    else yn = q[G.hn ? G.dn : 0];
+                } else {
+                    yn = q[G.hn ? G.dn : 0];
+                    if (adj) ys = pv[0][h];
+                    else ys = q[G.hs ? G.ds : 0];
+                }
+                a[r][h] = dgv[r][h] * pv[r][h] - nbsum_pair_vals(G, h, pv[r][1 - h], xo, ys, yn, zb, zt, L);
+            }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (!ok[r]) continue;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                Ap[oo[r][h]] = a[r][h];
+                s += pv[r][h] * a[r][h];
+            }
+        }
+    } else {
+        for (int r = 0; r < 2; ++r) {
+            const int j = j0 + r;
+            if (j >= L.ny) break;
+            for (int h = 0; h < 2; ++h) {
+                const int i = 2 * ip + h;
+                if (i >= L.nx) break;
+                const int o = lidx(L, i, j, k);
+                const double av = L.diag[diag_index(L, i, j, k)] * p[o] - nbsum(p, L, i, j, k);
+                Ap[o] = av;
+                s += p[o] * av;
+            }
         }
     }
     const int nt = blockDim.x * blockDim.y, t = threadIdx.y * blockDim.x + threadIdx.x;
@@ -1347,15 +1382,16 @@ int cg_iteration(ibmdem_ctx *c, int it) {
     hipStream_t s = c->stream;
     int rc;
     const LevelDev &L0 = c->lev64[0].d;
-    const LaunchShape mv = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);
-    const int mv_blocks = (int)(mv.grid.x * mv.grid.y * mv.grid.z);
+    const LaunchShape mv = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);   // block shape
+    const dim3 mvg = half_grid(L0.nx, L0.ny, L0.nz);                     // two rows per thread
+    const int mv_blocks = (int)(mvg.x * mvg.y * mvg.z);
     if (it > 1) {
         hipLaunchKernelGGL(k_cg_px, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z, c->cg_x,
                            c->scal);
         CHECK_LAUNCH(c);
     }
     prof_begin(c, IBMDEM_K_MATVEC);
-    hipLaunchKernelGGL(k_matvec_dot, mv.grid, mv.block, 0, s, L0, c->cg_p, c->cg_Ap, c->partial, c->scal);
+    hipLaunchKernelGGL(k_matvec_dot, mvg, mv.block, 0, s, L0, c->cg_p, c->cg_Ap, c->partial, c->scal);
     prof_end(c, IBMDEM_K_MATVEC, 2.0 * 8.0 * (double)n);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_final, dim3(1), dim3(FIN_T), 0, s, c->partial, mv_blocks, c->scal, SC_PAP, 1, it,
