@@ -1047,7 +1047,7 @@ __global__ __launch_bounds__(FIN_T) void k_final(const double *__restrict__ part
 
 // r = rhs with the pinned row, x = 0; r and x in level-0 storage, rhs in
 // field order (cell (0,0,0) is position 0 of both). A thread takes the
-// x-pair (2p, 2p+1) of row (j, k), blocks as k_matvec_dot.
+// x-pair (2p, 2p+1) of row (j, k), blocks of shape_for over the level's x-pairs.
 __global__ __launch_bounds__(RB) void k_cg_init(LevelDev L, const double *__restrict__ rhs,
                                                 double *__restrict__ r, double *__restrict__ x,
                                                 double *__restrict__ scal, float *__restrict__ r32) {
@@ -1087,7 +1087,7 @@ __global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, c
 // gauge P'(1,1,1) = 0: pcorr = x - x(1,1,1), x in level-0 storage, pcorr in
 // field order (no shift with an outflow side, where P' = 0 on the face), x
 // taking the last completed CG iteration's deferred update x + alpha p
-// (thread per x-pair of a row, blocks as k_matvec_dot)
+// (thread per x-pair of a row, blocks of shape_for over the x-pairs)
 __global__ __launch_bounds__(RB) void k_gauge(LevelDev L, const double *__restrict__ x,
                                               const double *__restrict__ p, const double *__restrict__ scal,
                                               double *__restrict__ pc) {
