@@ -167,31 +167,40 @@ int ibmdem_batch_context(ibmdem_batch *b, int32_t q, ibmdem_ctx **ctx) {
 
 const char *ibmdem_batch_last_error(const ibmdem_batch *b) { return b ? b->err.c_str() : "no batch"; }
 
+// A case that fails stops there (its context keeps its own message, as a
+// single context's failed step does); every other case still completes the
+// step in progress, so no context is left between its two step graphs. The
+// run then stops and returns the first failure (lowest case index).
 int ibmdem_batch_run(ibmdem_batch *b, int32_t n_steps, ibmdem_solve_info *info) {
     if (!b || n_steps < 0) return IBMDEM_ERR_ARG;
     const int n = (int)b->ctx.size();
-    std::vector<char> done(n);
+    std::vector<char> done(n), failed(n, 0);
     std::vector<ibmdem_solve_info> tmp(n);
     ibmdem_solve_info *inf = info ? info : tmp.data();
-    for (int s = 0; s < n_steps; ++s) {
+    int first_rc = IBMDEM_OK, first_q = -1;
+    auto fail = [&](int q, int rc) {
+        failed[q] = 1;
+        if (first_q < 0 || q < first_q) { first_q = q; first_rc = rc; }
+    };
+    for (int s = 0; s < n_steps && first_q < 0; ++s) {
         for (int q = 0; q < n; ++q) {
             bool d = false;
             const int rc = step_phase1(b->ctx[q], &inf[q], &d);
-            if (rc) return batch_error(b, q, rc);
             done[q] = d;
+            if (rc) fail(q, rc);
         }
         for (int q = 0; q < n; ++q) {
-            if (done[q]) continue;
+            if (done[q] || failed[q]) continue;
             const int rc = step_phase2(b->ctx[q], &inf[q]);
-            if (rc) return batch_error(b, q, rc);
+            if (rc) fail(q, rc);
         }
         for (int q = 0; q < n; ++q) {
-            if (done[q]) continue;
+            if (done[q] || failed[q]) continue;
             const int rc = step_phase3(b->ctx[q]);
-            if (rc) return batch_error(b, q, rc);
+            if (rc) fail(q, rc);
         }
     }
-    return IBMDEM_OK;
+    return first_q < 0 ? IBMDEM_OK : batch_error(b, first_q, first_rc);
 }
 
 int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {

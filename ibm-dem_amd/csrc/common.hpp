@@ -518,7 +518,10 @@ enum {
     SC_DONE = 18,     // last CG iteration that ran (its x update is deferred)
     SC_COUNT = 32
 };
-enum { FL_CAPACITY = 0, FL_OUTSIDE = 1, FL_NAN = 2, FL_COUNT = 8 };
+// device error flags: contact candidates beyond maxCollisionNo, a marker
+// outside the domain, a non-finite marker or body state, the IBM spread's
+// tile entry list beyond its capacity
+enum { FL_CAPACITY = 0, FL_OUTSIDE = 1, FL_NAN = 2, FL_SPREAD = 3, FL_COUNT = 8 };
 
 // ---------------------------------------------------------------------------
 // host helpers

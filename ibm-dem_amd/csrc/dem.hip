@@ -58,13 +58,17 @@ __global__ __launch_bounds__(T) void k_wrap_cells_count(int M, GridDev g, double
             mx[(long)d * M + m] = x;
             ci = cell_of(x, dd[d]);
         }
-        // a marker through a wall (or a non-finite position) is an error;
-        // it is still binned in the nearest boundary cell so that every
-        // list stays consistent until the caller sees the status
-        if (!(ci >= 1 && ci <= n[d])) {
+        // a marker through a wall, or a non-finite position, is an error;
+        // it is still binned in the nearest boundary cell (a non-finite one
+        // in cell 1) so that every list stays consistent until the caller
+        // sees the status
+        const bool finite = x - x == 0.0;
+        if (!finite) {
+            flags[FL_NAN] = 1;
+            ci = 1;
+        } else if (!(ci >= 1 && ci <= n[d])) {
             flags[FL_OUTSIDE] = 1;
             ci = ci < 1 ? 1 : n[d];
-            if (!(x == x)) ci = 1;
         }
         c[d] = ci;
     }
@@ -451,6 +455,7 @@ struct BodyState {
     double *x, *v, *vold, *om, *L, *Lold, *rot, *rotg, *I, *m, *rho;
     int *fixed;
     double *fdrag, *fmom, *fcol, *fmomcol, *fhydro, *fmomhydro;
+    int *flags;     // FL_NAN is raised for a non-finite updated state
 };
 
 __device__ __forceinline__ void cross3(const double a[3], const double b[3], double d[3]) {
@@ -540,10 +545,13 @@ __device__ void evolve_body(const BodyState &s, int B, int b, double dtD, double
         s.rotg[9L * b + q] = rgl[q];
     }
     const double omn[3] = {o0, o1, o2};
+    double chk = 0.0;   // finite iff every updated value is
     for (int d = 0; d < 3; ++d) {
         const long o = (long)d * B + b;
         s.v[o] = v[d]; s.L[o] = L[d]; s.om[o] = omn[d]; s.x[o] = x[d];
+        chk += (v[d] - v[d]) + (L[d] - L[d]) + (omn[d] - omn[d]) + (x[d] - x[d]);
     }
+    if (!(chk == 0.0)) s.flags[FL_NAN] = 1;
 }
 
 // mode 0: fcol, fmomcol only; 1: all four sums (AssembleLocalForceMS);
@@ -680,7 +688,7 @@ __global__ __launch_bounds__(T) void k_follow(int M, int B, GridDev g, double *_
 
 BodyState body_state(ibmdem_ctx *c) {
     return BodyState{c->bx, c->bv, c->bvold, c->bom, c->bL, c->bLold, c->brot, c->brotg, c->bI, c->bm, c->brho,
-                     c->bfixed, c->bfdrag, c->bfmom, c->bfcol, c->bfmomcol, c->bfhydro, c->bfmomhydro};
+                     c->bfixed, c->bfdrag, c->bfmom, c->bfcol, c->bfmomcol, c->bfhydro, c->bfmomhydro, c->flags};
 }
 
 ContactParams contact_params(ibmdem_ctx *c) {
@@ -834,15 +842,14 @@ int dem_substep_fused(ibmdem_ctx *c, bool last) {
 int dem_check_flags(ibmdem_ctx *c) {
     HIPX(c, hipMemcpyAsync(c->flags_host, c->flags, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPX(c, hipStreamSynchronize(c->stream));
-    if (c->flags_host[FL_OUTSIDE]) {
-        HIPX(c, hipMemsetAsync(c->flags, 0, FL_COUNT * sizeof(int), c->stream));
-        return set_error(c, IBMDEM_ERR_OUTSIDE, "a marker left the domain through a wall");
-    }
-    if (c->flags_host[FL_CAPACITY]) {
-        HIPX(c, hipMemsetAsync(c->flags, 0, FL_COUNT * sizeof(int), c->stream));
-        return set_error(c, IBMDEM_ERR_CAPACITY, "contact candidates exceed maxCollisionNo");
-    }
-    return IBMDEM_OK;
+    const int *f = c->flags_host;
+    if (!(f[FL_NAN] || f[FL_OUTSIDE] || f[FL_CAPACITY] || f[FL_SPREAD])) return IBMDEM_OK;
+    HIPX(c, hipMemsetAsync(c->flags, 0, FL_COUNT * sizeof(int), c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    if (f[FL_NAN]) return set_error(c, IBMDEM_ERR_NAN, "non-finite marker position or body state");
+    if (f[FL_OUTSIDE]) return set_error(c, IBMDEM_ERR_OUTSIDE, "a marker left the domain through a wall");
+    if (f[FL_CAPACITY]) return set_error(c, IBMDEM_ERR_CAPACITY, "contact candidates exceed maxCollisionNo");
+    return set_error(c, IBMDEM_ERR_CAPACITY, "IBM spread entry list exceeds its capacity");
 }
 
 int dem_setup(ibmdem_ctx *c) { (void)c; return IBMDEM_OK; }

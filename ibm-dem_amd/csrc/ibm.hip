@@ -554,7 +554,7 @@ __global__ __launch_bounds__(64) void k_tile_fill(GridDev g, TileGeom tg, const 
     tile_bins(g, tx * TL - 1, ty * TL - 1, tz * TL - 1, bin_start, s_off, s_gstart);
     const long base = estart[b];
     if (base + s_off[NSRC] > ent_cap) {
-        if (threadIdx.x == 0) atomicOr(&flags[FL_CAPACITY], 1);
+        if (threadIdx.x == 0) atomicOr(&flags[FL_SPREAD], 1);
         return;
     }
     const int ib = tx * TL - 1, jb = ty * TL - 1, kb = tz * TL - 1;

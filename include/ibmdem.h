@@ -50,7 +50,9 @@ extern "C" {
  *   OUTFLOW (high sides only): zero normal gradient of every velocity
  *           component, pressure (and its correction) zero at the face; the
  *           Poisson problem is then non-singular (no pinned cell).
- * Markers see neither as a wall (no mirror images, no wall contact). */
+ * Markers treat both like walls: the spread adds their mirror images across
+ * the side (as the reference does for its outlet ghosts, ZJS_MPI_DEM.F90:
+ * 203-209) and markers in the last cell layer get wall contact from it. */
 #define IBMDEM_BC_PERIODIC 0
 #define IBMDEM_BC_WALL     1
 #define IBMDEM_BC_INFLOW   2

@@ -51,7 +51,11 @@
  *     mirror with negation (P = 0 on the face), which makes the Poisson
  *     operator non-singular (no pinned row, no gauge shift);
  *   - markers treat both like walls (mirror images, wall contact).
- * No OpenMP: one core, deterministic.
+ * OpenMP: loops whose iterations write disjoint data run in parallel (the
+ * spread gives each thread a slab of z planes and walks the parcels in
+ * list order, so every cell still receives its terms in that order), and
+ * every reduction sums fixed chunks of 4096 terms serially. Results are
+ * therefore bitwise independent of the thread count (OMP_NUM_THREADS).
  */
 #include "../include/ibmdem.h"
 
@@ -59,6 +63,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define LAP 2
 
@@ -134,6 +141,15 @@ static int set_err(ibmdem_ctx *c, int code, const char *msg) {
 
 static void *xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
 
+/* Deterministic reductions: partial sums over fixed chunks of RCHUNK terms
+ * (in parallel), then the chunk sums in order. */
+#define RCHUNK 4096
+static double chunk_sum(const double *part, int nc) {
+    double s = 0.0;
+    for (int q = 0; q < nc; ++q) s += part[q];
+    return s;
+}
+
 /* ------------------------------------------------------------------------
  * FillGhostRegionForFluid + DealWithBoundaryFluid + Boundary for one rank
  * (boundary.F90:2-247, fluid.F90:240-250). Side order 1..6; the x passes
@@ -160,6 +176,7 @@ static void fill_ghost_one(ibmdem_ctx *c, double *f, int is_vel) {
         if (dim == 0) { lo1 = 1; hi1 = ny; lo2 = 1; hi2 = nz; }
         else if (dim == 1) { lo1 = 1 - LAP; hi1 = nx + LAP; lo2 = 1; hi2 = nz; }
         else { lo1 = 1 - LAP; hi1 = nx + LAP; lo2 = 1 - LAP; hi2 = ny + LAP; }
+#pragma omp parallel for schedule(static)
         for (int b = lo2; b <= hi2; ++b)
             for (int a = lo1; a <= hi1; ++a)
                 for (int l = 0; l < LAP; ++l) {
@@ -215,6 +232,7 @@ static double plane_at(const ibmdem_ctx *c, int d, int i, int j, int k) {
 
 static void boundary_vel(ibmdem_ctx *c) {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
+#pragma omp parallel for schedule(static)
     for (int k = 1 - LAP; k <= nz + LAP; ++k)
         for (int j = 1 - LAP; j <= ny + LAP; ++j)
             for (int i = 1 - LAP; i <= nx + LAP; ++i) {
@@ -403,29 +421,38 @@ static int collision_flag(const ibmdem_ctx *c, int i, int j) {
 static int rebuild_lists(ibmdem_ctx *c) {
     /* CompressDEM: markers whose cell left the interior through a periodic
      * face take the position of their ghost copy, x + coordShift */
+    int outside = 0, nonfinite = 0;
+#pragma omp parallel for schedule(static) reduction(| : outside, nonfinite)
     for (int p = 0; p < c->M; ++p) {
         double xp[3] = {XC(c, p, 0), XC(c, p, 1), XC(c, p, 2)};
         int ijk[3];
+        /* a non-finite position (no counterpart: the reference would bin it
+         * wherever the conversion lands) is its own error */
+        if (!(isfinite(xp[0]) && isfinite(xp[1]) && isfinite(xp[2]))) {
+            nonfinite = 1;
+            c->ptype[p] = -1;
+            continue;
+        }
         cell_of(c, xp, ijk);
         int n[3] = {c->nx, c->ny, c->nz};
         for (int d = 0; d < 3; ++d) {
             if (ijk[d] > n[d]) {
-                if (c->btype[2 * d + 1] != IBMDEM_BC_PERIODIC)
-                    return set_err(c, IBMDEM_ERR_OUTSIDE, "marker left the domain");
-                XC(c, p, d) = XC(c, p, d) + (-c->L[d]);
+                if (c->btype[2 * d + 1] != IBMDEM_BC_PERIODIC) outside = 1;
+                else XC(c, p, d) = XC(c, p, d) + (-c->L[d]);
             } else if (ijk[d] < 1) {
-                if (c->btype[2 * d] != IBMDEM_BC_PERIODIC)
-                    return set_err(c, IBMDEM_ERR_OUTSIDE, "marker left the domain");
-                XC(c, p, d) = XC(c, p, d) + c->L[d];
+                if (c->btype[2 * d] != IBMDEM_BC_PERIODIC) outside = 1;
+                else XC(c, p, d) = XC(c, p, d) + c->L[d];
             }
         }
         xp[0] = XC(c, p, 0); xp[1] = XC(c, p, 1); xp[2] = XC(c, p, 2);
         cell_of(c, xp, ijk);
         if (ijk[0] < 1 || ijk[0] > c->nx || ijk[1] < 1 || ijk[1] > c->ny ||
             ijk[2] < 1 || ijk[2] > c->nz)
-            return set_err(c, IBMDEM_ERR_OUTSIDE, "marker outside the domain");
+            outside = 1;
         c->ptype[p] = -1;
     }
+    if (nonfinite) return set_err(c, IBMDEM_ERR_NAN, "non-finite marker position");
+    if (outside) return set_err(c, IBMDEM_ERR_OUTSIDE, "marker outside the domain");
     c->np = c->M;
     for (int s = 0; s < 6; ++s) c->nsend[s] = 0;
     put_into_cells(c, 0, c->M, 0);
@@ -442,6 +469,8 @@ static int rebuild_lists(ibmdem_ctx *c) {
     }
     free(cur);
     /* RebuildParcelCollisionListDEM (ZJS_UPDATE_LIST_DEM.F90:248-286) */
+    int over = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : over)
     for (int i = 0; i < c->M; ++i) {
         int ci, cj, ck, cnt = 0;
         unpack_cell(c, c->cell[i], &ci, &cj, &ck);
@@ -454,14 +483,14 @@ static int rebuild_lists(ibmdem_ctx *c) {
                         int j = c->cell_items[q];
                         if (c->body[i] != c->body[j] && collision_flag(c, i, j) &&
                             c->ptype[j] == -1 && i != j) {
-                            if (cnt >= c->cs.max_pairs)
-                                return set_err(c, IBMDEM_ERR_CAPACITY, "pair list capacity");
+                            if (cnt >= c->cs.max_pairs) { over = 1; continue; }
                             c->pairs[(size_t)i * c->cs.max_pairs + cnt++] = j;
                         }
                     }
                 }
         c->npairs[i] = cnt;
     }
+    if (over) return set_err(c, IBMDEM_ERR_CAPACITY, "pair list capacity");
     return IBMDEM_OK;
 }
 
@@ -476,6 +505,7 @@ static void cell_centre(const ibmdem_ctx *c, int i, int j, int k, double xc[3]) 
 
 static void ibm_interpolate(ibmdem_ctx *c) {
     const double dV1 = c->dx, dV2 = c->dy, dV3 = c->dz;
+#pragma omp parallel for schedule(dynamic, 256)
     for (int p = 0; p < c->M; ++p) {
         int ci, cj, ck;
         unpack_cell(c, c->cell[p], &ci, &cj, &ck);
@@ -501,20 +531,34 @@ static void ibm_interpolate(ibmdem_ctx *c) {
 }
 
 static void ibm_spread(ibmdem_ctx *c) {
-    memcpy(c->fxo, c->fx, c->nbox * sizeof(double));
-    memcpy(c->fyo, c->fy, c->nbox * sizeof(double));
-    memcpy(c->fzo, c->fz, c->nbox * sizeof(double));
-    memset(c->fx, 0, c->nbox * sizeof(double));
-    memset(c->fy, 0, c->nbox * sizeof(double));
-    memset(c->fz, 0, c->nbox * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < c->nbox; ++q) {
+        c->fxo[q] = c->fx[q]; c->fyo[q] = c->fy[q]; c->fzo[q] = c->fz[q];
+        c->fx[q] = 0.0; c->fy[q] = 0.0; c->fz[q] = 0.0;
+    }
+    /* Thread t owns the halo planes [k0, k1] and walks every parcel in list
+     * order, adding only into its own planes: each cell receives its terms
+     * in parcel-list order, exactly as the serial loop. */
+    const int kmin = 1 - LAP, kmax = c->nz + LAP;
+#pragma omp parallel
+    {
+        int nt = 1, t = 0;
+#ifdef _OPENMP
+        nt = omp_get_num_threads();
+        t = omp_get_thread_num();
+#endif
+        const int span = kmax - kmin + 1;
+        const int k0 = kmin + (int)((long)span * t / nt), k1 = kmin + (int)((long)span * (t + 1) / nt) - 1;
     for (int p = 0; p < c->np; ++p) {
         int ci, cj, ck;
         unpack_cell(c, c->cell[p], &ci, &cj, &ck);
+        if (ck + 2 < k0 || ck - 2 > k1) continue;
         double xp[3] = {XC(c, p, 0), XC(c, p, 1), XC(c, p, 2)};
         for (int kk = -2; kk <= 2; ++kk)
             for (int jj = -2; jj <= 2; ++jj)
                 for (int ii = -2; ii <= 2; ++ii) {
                     int i = ci + ii, j = cj + jj, k = ck + kk;
+                    if (k < k0 || k > k1) continue;
                     if (!in_box(c, i, j, k)) continue;
                     double xc[3], wt[3];
                     cell_centre(c, i, j, k, xc);
@@ -525,11 +569,13 @@ static void ibm_spread(ibmdem_ctx *c) {
                     c->fz[id] = c->fz[id] - FD(c, p, 2) * c->vol[p] * wt[2];
                 }
     }
+    }
 }
 
 /* VelocityPostGuess (fluid.F90:135-153) */
 static void velocity_post_guess(ibmdem_ctx *c) {
     const double dt = c->cs.dt;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= c->nz; ++k)
         for (int j = 1; j <= c->ny; ++j)
             for (int i = 1; i <= c->nx; ++i) {
@@ -553,6 +599,7 @@ static void velocity_guess(ibmdem_ctx *c, double aRK, double bRK, double cRK) {
     double *RHSv = (double *)xcalloc(c->nbox, sizeof(double));
     double *RHSw = (double *)xcalloc(c->nbox, sizeof(double));
     const double *u = c->u, *v = c->v, *w = c->w, *P = c->P;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= c->nz; ++k)
         for (int j = 1; j <= c->ny; ++j)
             for (int i = 1; i <= c->nx; ++i) {
@@ -606,6 +653,7 @@ static void velocity_guess(ibmdem_ctx *c, double aRK, double bRK, double cRK) {
                 RHSv[o] = -(TIvx + TIvy + TIvz) + (TVvx + TVvy + TVvz) + c->cs.cp[1] + c->fy[o];
                 RHSw[o] = -(TIwx + TIwy + TIwz) + (TVwx + TVwy + TVwz) + c->cs.cp[2] + c->fz[o];
             }
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= c->nz; ++k)
         for (int j = 1; j <= c->ny; ++j)
             for (int i = 1; i <= c->nx; ++i) {
@@ -624,19 +672,29 @@ static void velocity_guess(ibmdem_ctx *c, double aRK, double bRK, double cRK) {
 static void modify_bulk(ibmdem_ctx *c) {
     const int *sw = c->cs.bulk_switch;
     if (!(sw[0] == 1 || sw[1] == 1 || sw[2] == 1)) return;
-    double us = 0.0, vs = 0.0, ws = 0.0;
-    for (int k = 1; k <= c->nz; ++k)
-        for (int j = 1; j <= c->ny; ++j)
-            for (int i = 1; i <= c->nx; ++i) {
-                size_t o = ID(c, i, j, k);
-                us += c->u[o]; vs += c->v[o]; ws += c->w[o];
-            }
+    /* SUM in array element order, as fixed chunks of RCHUNK elements */
+    const int nTot0 = c->nx * c->ny * c->nz, nc = (nTot0 + RCHUNK - 1) / RCHUNK;
+    double *part = (double *)xcalloc(3 * (size_t)nc, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int ch = 0; ch < nc; ++ch) {
+        double a = 0.0, b = 0.0, e = 0.0;
+        const int hi = (ch + 1) * RCHUNK < nTot0 ? (ch + 1) * RCHUNK : nTot0;
+        for (int q = ch * RCHUNK; q < hi; ++q) {
+            const int i = q % c->nx + 1, j = (q / c->nx) % c->ny + 1, k = q / (c->nx * c->ny) + 1;
+            size_t o = ID(c, i, j, k);
+            a += c->u[o]; b += c->v[o]; e += c->w[o];
+        }
+        part[ch] = a; part[nc + ch] = b; part[2 * (size_t)nc + ch] = e;
+    }
+    double us = chunk_sum(part, nc), vs = chunk_sum(part + nc, nc), ws = chunk_sum(part + 2 * (size_t)nc, nc);
+    free(part);
     int nTot = c->nx * c->ny * c->nz;
     c->modP[0] = c->modP[1] = c->modP[2] = 0.0;
     const double dt = c->cs.dt;
     if (sw[0] == 1) c->modP[0] = -(us / nTot - c->cs.bulk_target[0]) / dt;
     if (sw[1] == 1) c->modP[1] = -(vs / nTot - c->cs.bulk_target[1]) / dt;
     if (sw[2] == 1) c->modP[2] = -(ws / nTot - c->cs.bulk_target[2]) / dt;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= c->nz; ++k)
         for (int j = 1; j <= c->ny; ++j)
             for (int i = 1; i <= c->nx; ++i) {
@@ -655,6 +713,7 @@ static void velocity_divergence(ibmdem_ctx *c, double cRK) {
     for (int d = 0; d < 3; ++d)
         if (c->btype[2 * d + 1] == IBMDEM_BC_OUTFLOW) { plane_from_interior(c, d); refill = 1; }
     if (refill) boundary_vel(c);
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= c->nz; ++k)
         for (int j = 1; j <= c->ny; ++j)
             for (int i = 1; i <= c->nx; ++i) {
@@ -667,6 +726,7 @@ static void velocity_divergence(ibmdem_ctx *c, double cRK) {
 
 /* PressureCorrection + VelocityCorrection(c) (fluid.F90:199-238) */
 static void pressure_correction(ibmdem_ctx *c) {
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= c->nz; ++k)
         for (int j = 1; j <= c->ny; ++j)
             for (int i = 1; i <= c->nx; ++i) {
@@ -677,6 +737,7 @@ static void pressure_correction(ibmdem_ctx *c) {
 static void velocity_correction(ibmdem_ctx *c, double cRK) {
     const size_t sx = 1, sy = (size_t)c->px, sz = (size_t)c->px * c->py;
     const double dt = c->cs.dt, rhof = c->cs.rhof;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= c->nz; ++k)
         for (int j = 1; j <= c->ny; ++j)
             for (int i = 1; i <= c->nx; ++i) {
@@ -753,6 +814,7 @@ static void mg_setup_level(mg_level *L) {
     for (int d = 0; d < 3; ++d)
         if (L->btype[2 * d] == IBMDEM_BC_PERIODIC && (nn[d] % 2))
             L->rb_valid = 0;
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < L->nz; ++k)
         for (int j = 0; j < L->ny; ++j)
             for (int i = 0; i < L->nx; ++i) {
@@ -774,6 +836,7 @@ static void mg_setup_level(mg_level *L) {
 }
 
 static void mg_apply(const mg_level *L, const double *x, double *ax) {
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < L->nz; ++k)
         for (int j = 0; j < L->ny; ++j)
             for (int i = 0; i < L->nx; ++i) {
@@ -783,6 +846,8 @@ static void mg_apply(const mg_level *L, const double *x, double *ax) {
 }
 
 static void rb_half(mg_level *L, int color) {
+    /* cells of one colour only read the other colour: any order */
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < L->nz; ++k)
         for (int j = 0; j < L->ny; ++j)
             for (int i = 0; i < L->nx; ++i) {
@@ -793,6 +858,7 @@ static void rb_half(mg_level *L, int color) {
 }
 static void jacobi_sweep(mg_level *L, double omega) {
     double *t = (double *)xcalloc(L->n, sizeof(double));
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < L->nz; ++k)
         for (int j = 0; j < L->ny; ++j)
             for (int i = 0; i < L->nx; ++i) {
@@ -806,6 +872,7 @@ static void jacobi_sweep(mg_level *L, double omega) {
 
 static void restrict_to(const mg_level *F, const double *rf, mg_level *C) {
     static const double wt[4] = {0.25, 0.75, 0.75, 0.25};
+#pragma omp parallel for schedule(static)
     for (int K = 0; K < C->nz; ++K)
         for (int J = 0; J < C->ny; ++J)
             for (int I = 0; I < C->nx; ++I) {
@@ -833,6 +900,7 @@ static void restrict_to(const mg_level *F, const double *rf, mg_level *C) {
 }
 
 static void prolong_add(const mg_level *C, mg_level *F) {
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < F->nz; ++k)
         for (int j = 0; j < F->ny; ++j)
             for (int i = 0; i < F->nx; ++i) {
@@ -875,6 +943,7 @@ static void vcycle(ibmdem_ctx *c, int l) {
     }
     for (int s = 0; s < c->cs.mg_pre; ++s) { rb_half(L, 0); rb_half(L, 1); }
     mg_apply(L, L->x, L->r);
+#pragma omp parallel for schedule(static)
     for (int q = 0; q < L->n; ++q) L->r[q] = L->b[q] - L->r[q];
     restrict_to(L, L->r, &c->lev[l + 1]);
     vcycle(c, l + 1);
@@ -883,8 +952,17 @@ static void vcycle(ibmdem_ctx *c, int l) {
 }
 
 static double dot(const double *a, const double *b, int n) {
-    double s = 0.0;
-    for (int q = 0; q < n; ++q) s += a[q] * b[q];
+    const int nc = (n + RCHUNK - 1) / RCHUNK;
+    double *part = (double *)xcalloc((size_t)nc, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int ch = 0; ch < nc; ++ch) {
+        double s = 0.0;
+        const int hi = (ch + 1) * RCHUNK < n ? (ch + 1) * RCHUNK : n;
+        for (int q = ch * RCHUNK; q < hi; ++q) s += a[q] * b[q];
+        part[ch] = s;
+    }
+    const double s = chunk_sum(part, nc);
+    free(part);
     return s;
 }
 
@@ -924,8 +1002,21 @@ static int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
     int dirichlet = 0;
     for (int sd = 0; sd < 6; ++sd)
         if (c->btype[sd] == IBMDEM_BC_OUTFLOW) dirichlet = 1;
-    double s = 0.0;
-    for (q = 1; q < n; ++q) s += rhs[q];
+    /* sum_{i>0} b[i] in fixed chunks (the first chunk starts at element 1) */
+    double s;
+    {
+        const int nc = (n + RCHUNK - 1) / RCHUNK;
+        double *part = (double *)xcalloc((size_t)nc, sizeof(double));
+#pragma omp parallel for schedule(static)
+        for (int ch = 0; ch < nc; ++ch) {
+            double a = 0.0;
+            const int hi = (ch + 1) * RCHUNK < n ? (ch + 1) * RCHUNK : n;
+            for (int e = ch == 0 ? 1 : ch * RCHUNK; e < hi; ++e) a += rhs[e];
+            part[ch] = a;
+        }
+        s = chunk_sum(part, nc);
+        free(part);
+    }
     if (!dirichlet) rhs[0] = -s;
     (void)b;
     double *x = c->cg_x, *r = c->cg_r, *z = c->cg_z, *p = c->cg_p, *Ap = c->cg_Ap;
@@ -946,8 +1037,10 @@ static int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
             mg_apply(&c->lev[0], p, Ap);
             double pAp = dot(p, Ap, n);
             double alpha = rz / pAp;
-            for (q = 0; q < n; ++q) { x[q] = x[q] + alpha * p[q]; r[q] = r[q] - alpha * Ap[q]; }
+#pragma omp parallel for schedule(static)
+            for (int e = 0; e < n; ++e) { x[e] = x[e] + alpha * p[e]; r[e] = r[e] - alpha * Ap[e]; }
             rr = dot(r, r, n);
+            if (!(rr == rr)) break;
             if (sqrt(rr) <= c->cs.poisson_rtol * bnorm) { conv = 1; break; }
             memcpy(c->lev[0].b, r, n * sizeof(double));
             vcycle(c, 0);
@@ -955,7 +1048,8 @@ static int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
             double rzn = dot(r, z, n);
             double beta = rzn / rz;
             rz = rzn;
-            for (q = 0; q < n; ++q) p[q] = z[q] + beta * p[q];
+#pragma omp parallel for schedule(static)
+            for (int e = 0; e < n; ++e) p[e] = z[e] + beta * p[e];
         }
         if (it > c->cs.poisson_max_iter) it = c->cs.poisson_max_iter;
     }
@@ -972,6 +1066,7 @@ static int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
         info->rhs_norm = bnorm;
     }
     free(rhs);
+    if (!(rr == rr)) return set_err(c, IBMDEM_ERR_NAN, "non-finite residual in the Poisson solve");
     if (!conv) return set_err(c, IBMDEM_ERR_NOT_CONVERGED, "Poisson solve did not converge");
     return IBMDEM_OK;
 }
@@ -981,6 +1076,7 @@ static int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
  * ------------------------------------------------------------------------ */
 static void contact(ibmdem_ctx *c) {
     const double eta = c->eta, di = c->dx, knC = c->cs.kn_coeff;
+#pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < c->M; ++i) {
         FC(c, i, 0) = 0; FC(c, i, 1) = 0; FC(c, i, 2) = 0;
         for (int q = 0; q < c->npairs[i]; ++q) {
@@ -1010,6 +1106,7 @@ static void contact(ibmdem_ctx *c) {
     }
     /* CalculateSolidBoundaryCollisionForce */
     static const double nrm[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < c->M; ++i) {
         int ci, cj, ck;
         unpack_cell(c, c->cell[i], &ci, &cj, &ck);
@@ -1180,6 +1277,7 @@ static void body_integrate(ibmdem_ctx *c) {
 
 static void markers_follow(ibmdem_ctx *c) {
     const double dtD = c->cs.dt / c->cs.couple_interval;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < c->M; ++i) {
         int b = c->body[i];
         if (c->bfixed[b]) continue;
@@ -1202,6 +1300,7 @@ static void markers_follow(ibmdem_ctx *c) {
     }
     /* AdjustParcelPosition (ZJS_EVOLUTION_DEM.F90:121-150) */
     double e[3] = {c->dx * 1e-20, c->dy * 1e-20, c->dz * 1e-20};
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < c->M; ++i)
         for (int d = 0; d < 3; ++d) {
             if (fabs(XC(c, i, d) - 0.0) < e[d]) XC(c, i, d) = 0.0 + e[d];
@@ -1710,13 +1809,17 @@ int ibmdem_batch_context(ibmdem_batch *b, int32_t q, ibmdem_ctx **ctx) {
 
 const char *ibmdem_batch_last_error(const ibmdem_batch *b) { return b ? "" : "no batch"; }
 
+/* Step by step over the cases: a failing case stops, the others complete
+ * the step in progress, then the first failure (lowest case) is returned. */
 int ibmdem_batch_run(ibmdem_batch *b, int32_t n_steps, ibmdem_solve_info *info) {
     if (!b || n_steps < 0) return IBMDEM_ERR_ARG;
-    for (int q = 0; q < b->n; ++q) {
-        int rc = ibmdem_run(b->c[q], n_steps, info ? &info[q] : NULL);
-        if (rc) return rc;
-    }
-    return IBMDEM_OK;
+    int first = IBMDEM_OK;
+    for (int s = 0; s < n_steps && first == IBMDEM_OK; ++s)
+        for (int q = 0; q < b->n; ++q) {
+            int rc = ibmdem_step(b->c[q], info ? &info[q] : NULL);
+            if (rc && first == IBMDEM_OK) first = rc;
+        }
+    return first;
 }
 
 int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {

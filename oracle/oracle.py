@@ -11,7 +11,10 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-LIB = os.path.join(HERE, "_build", "libibmdem_oracle.so")
+# IBMDEM_ORACLE_LIB names another build of the oracle (a copy with a planted
+# mistake, to check that the tests catch it); the default is the Makefile's
+DEFAULT_LIB = os.path.join(HERE, "_build", "libibmdem_oracle.so")
+LIB = os.environ.get("IBMDEM_ORACLE_LIB", DEFAULT_LIB)
 sys.path.insert(0, os.path.join(ROOT, "ibm-dem_amd"))
 
 from ibmdem.abi import Simulation, load_library  # noqa: E402
@@ -27,7 +30,7 @@ def build() -> str:
 def library():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        if LIB == DEFAULT_LIB and not os.path.exists(LIB):
             build()
         _lib = load_library(LIB)
     return _lib

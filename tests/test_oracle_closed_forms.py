@@ -577,3 +577,187 @@ def test_body_force_assembly():
         np.testing.assert_allclose(b.fdrag[:, k], 1.7 * fv.sum(axis=1), rtol=1e-12)
         np.testing.assert_allclose(b.fmom[:, k], np.cross(r.T, fv.T).sum(axis=0), rtol=1e-10, atol=1e-12)
         np.testing.assert_allclose(b.fcol[:, k], mk.fcol[:, sel].sum(axis=1), rtol=1e-12, atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# a2-a5 in three dimensions: the ABC (Beltrami) flow
+# ---------------------------------------------------------------------------
+def abc_flow_error(n, A=1.0, B=0.7, Cc=0.5, nu=0.1, T=0.3):
+    """Arnold-Beltrami-Childress flow u = (A sin z + C cos y, B sin x + A cos z,
+    C sin y + B cos x) on the triply periodic [0, 2pi]^3. Its vorticity equals
+    the velocity, so (u.grad)u = grad(|u|^2/2) is absorbed by the pressure and
+    the exact solution decays as exp(-nu t): every flux of ConvectionTensor
+    (fluid.F90:15-44) including the z-fluxes and the cross terms, and every
+    direction of ViscousTensor (fluid.F90:58-66), must be second-order
+    accurate for the discrete solution to converge at second order. dt ~ h^2
+    keeps the first-order time error at the same order."""
+    h = 2 * np.pi / n
+    dt = 0.04 * h * h / nu
+    steps = int(round(T / dt))
+    dt = T / steps
+    case = cases.build_case(n, n, n, [BC_PERIODIC] * 6, h=h, visc=nu, dt=dt, rtol=1e-13, max_iter=300)
+    case.n_markers = 0
+    case.n_bodies = 0
+    st = staggered_coords((n, n, n), h)
+    def vel(t):
+        g = math.exp(-nu * t)
+        u = (A * np.sin(st["u"][2]) + Cc * np.cos(st["u"][1])) * g
+        v = (B * np.sin(st["v"][0]) + A * np.cos(st["v"][2])) * g
+        w = (Cc * np.sin(st["w"][1]) + B * np.cos(st["w"][0])) * g
+        return u, v, w
+    u0, v0, w0 = vel(0.0)
+    with oracle.simulation(case) as s:
+        s.upload_fields(u0, v0, w0, np.zeros_like(u0))
+        s.run(steps)
+        uo, vo, wo, _ = s.download_fields()
+    ue, ve, we = vel(steps * dt)
+    return max(np.abs(uo - ue).max(), np.abs(vo - ve).max(), np.abs(wo - we).max())
+
+
+def test_abc_flow_second_order_in_3d():
+    e1 = abc_flow_error(12)
+    e2 = abc_flow_error(24)
+    order = math.log(e1 / e2, 2)
+    assert 1.7 < order < 2.5, (e1, e2, order)
+    # the error is a discretisation error, far below the decay itself
+    assert e2 < 2e-3
+
+
+# ---------------------------------------------------------------------------
+# a29: CloudEvolutionMS, fhydro and fmomHydro (couplesolverCylinder.F90:194-211)
+# ---------------------------------------------------------------------------
+def test_cloud_evolution_recurrence():
+    """With no IBM or contact force, CloudEvolutionMS sets
+    fhydro = (v - vold)/dt (1 - ratio) and the ten sub-steps add fhydro dt
+    and ratio g dt, so the per-step increment d_n = v_{n+1} - v_n obeys
+    d_n = (1 - ratio) d_{n-1} + ratio g dt. From vold = v (d_{-1} = 0):
+    d_n = g dt (1 - (1 - ratio)^(n+1)). The angular momentum, with no torque,
+    has e_n = L_{n+1} - L_n = (1 - ratio)^(n+1) (L_0 - Lold_0)."""
+    n = (16, 16, 64)
+    g = (0.0, 0.0, -0.01)
+    case, mk, bd = single_marker_case(n, [BC_WALL] * 6, np.array([[8.0, 8.0, 40.0]]), gravity=g, dt=0.5)
+    case.rhof = 1.3
+    rho = 3.9
+    ratio = (rho - 1.3) / rho
+    bd.rho[0] = rho
+    bd.v[:, 0] = [0.01, -0.02, 0.005]
+    bd.vold[:, 0] = bd.v[:, 0]
+    bd.angmom[:, 0] = [0.3, -0.1, 0.2]
+    bd.angmom_old[:, 0] = [0.1, 0.05, 0.0]
+    mk.v[:, 0] = bd.v[:, 0]
+    dt = case.dt
+    steps = 12
+    with oracle.simulation(case) as s:
+        s.upload_bodies(bd)
+        s.upload_markers(mk)
+        vs = [s.download_bodies().v[:, 0].copy()]
+        Ls = [s.download_bodies().angmom[:, 0].copy()]
+        for _ in range(steps):
+            s.cloud_evolution()
+            b = s.download_bodies()
+            vs.append(b.v[:, 0].copy())
+            Ls.append(b.angmom[:, 0].copy())
+            assert np.all(b.fcol == 0.0) and np.all(b.fdrag == 0.0)
+    vs, Ls = np.array(vs), np.array(Ls)
+    q = 1.0 - ratio
+    # increments are differences of the state: compare to a few ulps of it
+    for k in range(steps):
+        d_ref = np.asarray(g) * dt * (1.0 - q ** (k + 1))
+        np.testing.assert_allclose(vs[k + 1] - vs[k], d_ref, rtol=1e-12, atol=8e-16 * np.abs(vs).max())
+        e_ref = q ** (k + 1) * (bd.angmom[:, 0] - bd.angmom_old[:, 0])
+        np.testing.assert_allclose(Ls[k + 1] - Ls[k], e_ref, rtol=1e-12, atol=8e-16 * np.abs(Ls).max())
+
+
+# ---------------------------------------------------------------------------
+# a36: the composed step. Momentum budget in a periodic box.
+# ---------------------------------------------------------------------------
+def momentum_budget_run(gravity, steps=4):
+    """Returns per-step (fluid momentum rho_f sum u dV, body momenta
+    sum_b (m_b v_b - rho_f V_b vold_b), gravity impulse) for a periodic box of
+    spheres in contact with random fluid and body velocities. Per step the
+    spread adds rho_f dt sum f dV = -dt sum_b F_b to the fluid (partition of
+    unity), the body update adds dt F_b + (m - rho_f V) g dt + contact
+    (pairwise antisymmetric) to m v - rho_f V vold (the fhydro term of
+    CloudEvolutionMS is rho_f V (v - vold)/dt since m (1 - ratio) = rho_f V),
+    and the convective, viscous and pressure terms sum to zero in a periodic
+    box. Bulk forcing is off."""
+    setup = cases.small_periodic(n=(24, 20, 22), n_spheres=3, D=6.0, gap=0.6, rtol=1e-13)
+    case = setup.case
+    for d in range(3):
+        case.bulk_switch[d] = 0
+        case.gravity[d] = gravity[d]
+    case.rhof = 1.3
+    setup.bodies.rho[:] *= 1.3
+    setup.bodies.mass[:] *= 1.3
+    setup.bodies.inertia[:] *= 1.3
+    setup.markers.mass[:] *= 1.3
+    rf = case.rhof
+    dV = case.dx * case.dy * case.dz
+    vol_b = setup.bodies.mass / setup.bodies.rho
+    out = []
+    with oracle.simulation(case) as s:
+        cases.load(s, setup)
+        for k in range(steps + 1):
+            u, v, w, _ = s.download_fields()
+            b = s.download_bodies()
+            pf = rf * dV * np.array([u.sum(), v.sum(), w.sum()])
+            pb = (b.mass[None, :] * b.v - rf * vol_b[None, :] * b.vold).sum(axis=1)
+            out.append((pf, pb, b.fcol.sum(axis=1)))
+            if k < steps:
+                s.step()
+    grav = ((setup.bodies.mass - rf * vol_b).sum() * np.asarray(gravity) * case.dt)
+    return out, grav
+
+
+@pytest.mark.parametrize("gravity", [(0.0, 0.0, 0.0), (0.0, 0.0, -2e-3)])
+def test_composed_step_momentum_budget(gravity):
+    out, grav = momentum_budget_run(gravity)
+    scale = max(np.abs(o[0]).max() + np.abs(o[1]).max() for o in out)
+    moved = False
+    for k in range(len(out) - 1):
+        (pf0, pb0, _), (pf1, pb1, _) = out[k], out[k + 1]
+        change = (pf1 + pb1) - (pf0 + pb0)
+        np.testing.assert_allclose(change, grav, rtol=0, atol=1e-11 * scale)
+        moved |= np.abs(pf1 - pf0).max() > 1e-6 * scale
+    assert moved, "the fluid and the bodies exchanged no momentum"
+
+
+def test_cloud_evolution_torque_term():
+    """One CloudEvolutionMS call on a lone body whose markers carry a fixed
+    fdrag: fmomHydro = (L - Lold)/dt (1 - ratio) + T with T = sum r x
+    (fdrag vol) (couplesolverCylinder.F90:198-210, ZJS_MS_DEM.F90:509-510),
+    constant over the ten sub-steps (no contact), so
+    L_1 - L_0 = (L_0 - Lold_0)(1 - ratio) + T dt; T by brute force from the
+    marker geometry."""
+    n = (24, 24, 24)
+    rng = np.random.default_rng(17)
+    centre = np.array([12.0, 11.5, 12.5])
+    m = 6
+    case = cases.build_case(*n, [BC_WALL] * 6, n_markers=m, n_bodies=1, dt=0.4)
+    case.rhof = 1.2
+    mk = abi.MarkerArrays.empty(m)
+    bd = abi.BodyArrays.empty(1)
+    mk.x[:] = centre[:, None] + rng.uniform(-3.0, 3.0, size=(3, m))
+    mk.fdrag[:] = rng.normal(size=(3, m))
+    mk.vol[:] = rng.uniform(0.5, 1.0, size=m)
+    mk.mass[:] = 0.8
+    mk.radeff[:] = 0.75
+    bd.x[:, 0] = centre
+    bd.mass[0] = 50.0
+    bd.rho[0] = 3.0
+    bd.inertia[0] = [40.0, 0, 0, 0, 45.0, 0, 0, 0, 50.0]
+    bd.rot[0] = bd.rot_glo[0] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    bd.angmom[:, 0] = [0.2, -0.1, 0.05]
+    bd.angmom_old[:, 0] = [0.1, 0.0, 0.1]
+    ratio = (3.0 - 1.2) / 3.0
+    r = mk.x - centre[:, None]
+    T = np.cross(r.T, (mk.fdrag * mk.vol[None, :]).T).sum(axis=0)
+    with oracle.simulation(case) as s:
+        s.upload_bodies(bd)
+        s.upload_markers(mk)
+        s.cloud_evolution()
+        b = s.download_bodies()
+    want = (bd.angmom[:, 0] - bd.angmom_old[:, 0]) * (1 - ratio) + T * case.dt
+    np.testing.assert_allclose(b.angmom[:, 0] - bd.angmom[:, 0], want, rtol=1e-12,
+                               atol=1e-14 * np.abs(bd.angmom).max())
+    assert np.abs(T).max() > 0.1
