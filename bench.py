@@ -14,16 +14,25 @@ The FP64 form is the headline (value = steps/s); the mixed-precision form
 state and reported beside it. The timed steps run without per-launch
 events; afterwards one step per profiled kernel (spread, level-0 smoother,
 interpolation, predictor) is run with HIP events around each launch of that
-kernel on the context stream, and the kernel with the largest time per step
-is the roofline headline. Algorithmic bytes per launch are computed by the
-library: level-0 smoother 1.5 * s * cells (half the cells read their
-right-hand side and the other colour, and are written); spread
-64 * markers + 32 * 3 * footprint cells (24 in the first iteration);
-interpolation 136 * markers + 8 * 3 * footprint cells; predictor 7 * s *
-cells. The CPU baseline is the oracle (one core) timed on a bounded sample.
+kernel on the context stream. The roofline headline is the single kernel
+(one launch shape) with the largest time per step; the level-0 smoother's
+row sums its sweep variants and is listed, not ranked. Algorithmic bytes
+per launch are computed by the library (s = 8 B, F = footprint cells x 3
+components): spread 60 M + 4 s F (3 s F in the first iteration, which
+reads no f_old); interpolation 72 M + s F; level-0 smoother 1.5 s cells;
+predictor 7 s cells. `traffic` is the measured HBM bytes per launch of the
+headline kernel (profiles/, calibrated FETCH_SIZE + WRITE_SIZE).
+
+The other configurations run by default after the headline (fluidized bed
+192x192x768 and cylinder in crossflow 512x256x64, each in FP64 and mixed
+precision from the same state; the batched form over 64 uneven cases), and
+the CPU baseline times the oracle on the workload itself: the GPU's FP64
+state after the timed steps is uploaded into the oracle, which runs two
+coupled steps on the host cores granted to this process (OpenMP) and one
+step on one core.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--precision fp64|mixed|both]
-                [--batched] [--configs]
+                [--no-batched] [--no-configs]
 """
 from __future__ import annotations
 
@@ -48,12 +57,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=256, help="grid edge (256 is the workload)")
     ap.add_argument("--spheres", type=int, default=2000)
-    ap.add_argument("--batched", action="store_true",
-                    help="also time the batched form (64 uneven settling cases)")
-    ap.add_argument("--configs", action="store_true",
-                    help="also time the open-boundary configurations (fluidized bed "
-                         "192x192x768 with 10,000 spheres; cylinder in crossflow 512x256x64) "
-                         "in FP64 and mixed precision")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched form (64 uneven settling cases)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the open-boundary configurations (fluidized bed 192x192x768 with "
+                         "10,000 spheres; cylinder in crossflow 512x256x64) in FP64 and mixed precision")
+    ap.add_argument("--config-steps", type=int, default=10,
+                    help="timed steps of the other configurations and the batch")
     return ap.parse_args()
 
 
@@ -99,15 +109,20 @@ def timed_run(setup, precision, steps, warmup, profile):
 
 KERNEL_NAMES = {
     "spread": "k_spread_tiles (IBM spread + multi-direct-forcing update)",
-    "mg_smooth": "k_rb_half (level-0 red-black Gauss-Seidel half-sweep)",
+    "mg_smooth": "k_rb_half, all level-0 variants (red-black Gauss-Seidel half-sweeps: plain, "
+                 "first sweep, residual, r.z)",
     "interp": "k_interp (IBM interpolation)",
     "predictor": "k_predictor_lds (fused advection-diffusion predictor, LDS z-marching)",
 }
 
 
+GROUPS = ("mg_smooth",)   # rows that sum several kernels (variants)
+
+
 def roofline_of(prof, traffic=None):
     """Per profiled kernel: average launch time and algorithmic GB/s; the
-    dominant one (largest time per step) is the headline."""
+    single kernel with the largest time per step is the headline (grouped
+    rows are listed only)."""
     rows = {}
     for name, (total_ms, launches, bpl) in prof.items():
         if not launches:
@@ -123,13 +138,23 @@ def roofline_of(prof, traffic=None):
             "achieved": achieved,
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
         }
-    if not rows:
+        if name in GROUPS:
+            rows[name]["group"] = True
+        t = (traffic or {}).get(name)
+        if t and bpl:
+            rows[name]["traffic"] = t
+            rows[name]["traffic_over_algorithmic"] = t / bpl
+    single = [k for k in rows if k not in GROUPS]
+    if not single:
         return None, rows
-    top = max(rows, key=lambda k: rows[k]["ms_per_step"])
+    top = max(single, key=lambda k: rows[k]["ms_per_step"])
     r = rows[top]
+    tr = (traffic or {}).get(top)
     head = {"kernel": r["kernel"], "bound": "hbm", "achieved": r["achieved"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": r["frac"], "traffic": (traffic or {}).get(top),
+            "unit": "GB/s", "frac": r["frac"], "traffic": tr,
+            "traffic_over_algorithmic": tr / r["algorithmic_bytes_per_launch"] if tr else None,
             "avg_launch_us": r["avg_launch_us"], "launches_per_step": r["launches_per_step"],
+            "ms_per_step": r["ms_per_step"],
             "algorithmic_bytes_per_launch": r["algorithmic_bytes_per_launch"]}
     return head, rows
 
@@ -275,33 +300,67 @@ def config_runs(steps, warmup):
     return out
 
 
-def cpu_baseline():
-    """The oracle (one core) on a bounded sample: one coupled step of a
-    128^3 sub-box holding 250 spheres (the workload's solid fraction), taken
-    after one warm-up step, reported in cell-steps per second and scaled to
-    steps/s of the 256^3 workload."""
+def host_cores():
+    """Cores this process may use (its CPU affinity set, capped by
+    OMP_NUM_THREADS when that is set: a GPU box grants a share of its host)
+    and the physical cores lscpu reports, for the record."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    phys = None
+    try:
+        import subprocess
+        info = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(line.split(":", 1) for line in info.splitlines() if ":" in line)
+        phys = int(kv["Core(s) per socket"].strip()) * int(kv["Socket(s)"].strip())
+    except Exception:
+        pass
+    return n, phys
+
+
+def cpu_baseline(setup, state, steps_multi=2, steps_one=1):
+    """The oracle on the workload itself: the GPU's FP64 state after the
+    timed steps (fields, markers, bodies) is uploaded into the oracle, which
+    then runs `steps_multi` coupled steps with OpenMP on the granted cores and
+    `steps_one` step on one core (results are the same bits either way)."""
+    import ctypes
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    from ibmdem import cases
+    from ibmdem.abi import FP64
 
-    setup = cases.config_suspension(n_spheres=250, n=128)
-    sim = oracle.simulation(setup.case)
-    cases.load(sim, setup)
-    sim.step()               # warm-up (the flow starts at rest)
-    t0 = time.perf_counter()
-    info = sim.step()
-    dt = time.perf_counter() - t0
-    sim.close()
-    cells = setup.cells
-    gcell = cells / dt / 1e9
+    lib = oracle.library()
+    lib.ibmdem_oracle_set_threads.restype = ctypes.c_int
+    lib.ibmdem_oracle_set_threads.argtypes = [ctypes.c_int]
+    cores, phys = host_cores()
+    fields, markers, bodies = state
+    setup.case.precision = FP64
+    out = {}
+    with oracle.simulation(setup.case) as sim:
+        sim.upload_bodies(bodies)
+        sim.upload_markers(markers)
+        sim.upload_fields(*fields)
+        for label, threads, steps in (("multi", cores, steps_multi), ("one", 1, steps_one)):
+            used = lib.ibmdem_oracle_set_threads(threads)
+            t0 = time.perf_counter()
+            its = [sim.step().iterations for _ in range(steps)]
+            secs = time.perf_counter() - t0
+            out[label] = dict(threads=used, steps=steps, secs=secs, its=its)
+            print(f"[bench] cpu oracle, {used} thread(s): {steps} steps in {secs:.1f} s, CG {its}",
+                  file=sys.stderr, flush=True)
+    m, o = out["multi"], out["one"]
     return {
-        "value": gcell * 1e9 / 256 ** 3,
-        "unit": "steps/s (256^3 workload, scaled by cell count)",
-        "gcell_per_s": gcell,
-        "cores": 1,
+        "value": m["steps"] / m["secs"],
+        "unit": "steps/s",
+        "gcell_per_s": setup.cells * m["steps"] / m["secs"] / 1e9,
+        "cores": m["threads"],
+        "physical_cores_lscpu": phys,
         "kind": "port",
-        "sample": f"1 step of a 128^3 periodic sub-box with 250 spheres (same solid fraction), "
-                  f"{dt:.1f} s, {info.iterations} CG iterations",
+        "sample": f"the 256^3 workload itself from the GPU's state after the timed steps: "
+                  f"{m['steps']} steps on {m['threads']} cores in {m['secs']:.1f} s (CG iterations {m['its']})",
+        "one_core": {"value": o["steps"] / o["secs"], "unit": "steps/s", "cores": 1,
+                     "sample": f"{o['steps']} step in {o['secs']:.1f} s (CG iterations {o['its']})"},
     }
 
 
@@ -377,13 +436,14 @@ def main():
                 "p": rel(um[3], uf[3]), "marker_x": rel(mm.x, mf.x), "body_x": rel(bm.x, bf.x),
             },
         }
-    if args.batched:
-        line["batched"] = batched_run(args.steps, args.warmup)
-    if args.configs:
-        line["configs"] = config_runs(args.steps, args.warmup)
+    csteps, cwarm = args.config_steps, min(args.warmup, 3)
+    if not args.no_configs:
+        line["configs"] = config_runs(csteps, cwarm)
+    if not args.no_batched:
+        line["batched"] = batched_run(csteps, cwarm)
     if not args.no_cpu:
         try:
-            line["cpu_baseline"] = cpu_baseline()
+            line["cpu_baseline"] = cpu_baseline(setup, results["fp64"]["state"] if "fp64" in results else r["state"])
         except Exception as e:  # the baseline must not hide the GPU line
             line["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(line), flush=True)

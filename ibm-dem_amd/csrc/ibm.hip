@@ -39,6 +39,8 @@
 // the step).
 #include "common.hpp"
 
+#include <cstdlib>
+
 namespace {
 
 constexpr int T = 256;
@@ -723,6 +725,58 @@ __device__ __forceinline__ double readlane_f64(double v, int e) {
     return __longlong_as_double((long long)((unsigned long long)hi << 32 | lo));
 }
 
+// End of a tile's spread: store the reached-cell mask (first iteration),
+// then the multi-direct-forcing update u += dt*(f - f_old) on the reached
+// cells: the lane's 8 cells (one per z plane of the tile) in two passes,
+// every load of a pass issued before any is used.
+__device__ __forceinline__ void tile_epilogue(const SpreadArgs &a, int slot, double (*s_acc)[ACC_N],
+                                              const unsigned *s_touch, int i0, int j0, int k0) {
+    const GridDev &g = a.g;
+    const int lane = threadIdx.x;
+    if (a.first && lane < IBM_MASK_WORDS)
+        a.mask[(long)slot * IBM_MASK_WORDS + lane] =
+            ((unsigned long long)s_touch[2 * lane + 1] << 32) | (unsigned long long)s_touch[2 * lane];
+    // multi-direct-forcing update u += dt*(f - f_old) on the reached cells:
+    // the lane's 8 cells (one per z plane of the tile) in two passes, every
+    // load of a pass issued before any is used
+    double *vel[3] = {a.u, a.v, a.w};
+    const long fbase = (long)slot * TC;
+    const int li = lane & (TL - 1), lj = lane >> 3;
+    const int i = i0 + li, j = j0 + lj;
+    constexpr int HALF = TC / 64 / 2;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        bool has[HALF];
+        double fo[HALF][3], vo[HALF][3];
+        long o[HALF];
+#pragma unroll
+        for (int r = 0; r < HALF; ++r) {
+            const int q = lane + 64 * (h * HALF + r), k = k0 + h * HALF + r;
+            has[r] = ((s_touch[q >> 5] >> (q & 31)) & 1u) && i <= g.nx && j <= g.ny && k <= g.nz;
+            o[r] = has[r] ? cidx(g, i, j, k) : 0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                fo[r][c] = (has[r] && !a.first) ? a.fact[c * a.fstride + fbase + q] : 0.0;
+                vo[r][c] = has[r] ? vel[c][o[r]] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < HALF; ++r) {
+            if (!has[r]) continue;
+            const int q = lane + 64 * (h * HALF + r), k = k0 + h * HALF + r;
+            const int tgt[3] = {i, j, k};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double fn = s_acc[c][acc_idx(q)];
+                a.fact[c * a.fstride + fbase + q] = fn;
+                // the low boundary faces (wall, inflow) keep their fixed values
+                const bool face = tgt[c] == 1 && g.bt[2 * c] != IBMDEM_BC_PERIODIC;
+                if (!face && !(fn == 0.0 && fo[r][c] == 0.0)) vel[c][o[r]] = vo[r][c] + a.dt * (fn - fo[r][c]);
+            }
+        }
+    }
+}
+
 template <bool P2>
 __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     __shared__ double s_acc[3][ACC_N];         // the tile's force
@@ -807,48 +861,203 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
         }
         wave_sync();   // the rows are rewritten by the next chunk
     }
-    if (a.first && lane < IBM_MASK_WORDS)
-        a.mask[(long)slot * IBM_MASK_WORDS + lane] =
-            ((unsigned long long)s_touch[2 * lane + 1] << 32) | (unsigned long long)s_touch[2 * lane];
-    // multi-direct-forcing update u += dt*(f - f_old) on the reached cells:
-    // the lane's 8 cells (one per z plane of the tile) in two passes, every
-    // load of a pass issued before any is used
-    double *vel[3] = {a.u, a.v, a.w};
-    const long fbase = (long)slot * TC;
-    const int li = lane & (TL - 1), lj = lane >> 3;
-    const int i = i0 + li, j = j0 + lj;
-    constexpr int HALF = TC / 64 / 2;
+    tile_epilogue(a, slot, s_acc, s_touch, i0, j0, k0);
+}
+
+// ---------------------------------------------------------------------------
+// Packed taps. Of an entry's 4x4x4 support, u has 36 nonzero taps (the x-face
+// factor at offsets 0..3 times the y and z centre factors at offsets 0..2; the
+// centre factor at offset 3 is always 0, see k_records), v and w likewise with
+// their own face dimension: 108 taps in all, numbered
+//   u (a, b, c): a + 4b + 12c        a in 0..3, b, c in 0..2
+//   v (a, b, c): 36 + a + 3b + 12c   b in 0..3, a, c in 0..2
+//   w (a, b, c): 72 + a + 3b + 9c    c in 0..3, a, b in 0..2
+// and summed as two 64-lane LDS adds per entry (tap s: lane s % 64 of add
+// s / 64) instead of one 64-lane add per component. Each tap's value is the
+// term of the one-add-per-component form, -(f * ((phi_x phi_y) phi_z)), and
+// every cell still receives its terms in entry order, so the force is the
+// same to the bit.
+// ---------------------------------------------------------------------------
+constexpr int NTAP = 108;
+constexpr int PCAP = 16;        // entries staged per chunk, four lanes each
+constexpr int PLANES = 64 / PCAP;
+constexpr int TROW = NTAP + 1;  // LDS row stride of an entry's taps (odd)
+static_assert(PLANES == 4, "each staged entry's taps are split four ways (9 per component per lane)");
+
+struct TapSlot {
+    int off;       // accumulator offset from the support corner
+    int a, b, c;   // support offsets
+    bool valid;
+};
+__device__ __forceinline__ TapSlot tap_slot(int s) {
+    int comp, a, b, c;
+    if (s < 36) { comp = 0; a = s % 4; b = (s / 4) % 3; c = s / 12; }
+    else if (s < 72) { const int t = s - 36; comp = 1; a = t % 3; b = (t / 3) % 4; c = t / 12; }
+    else { const int t = s - 72; comp = 2; a = t % 3; b = (t / 3) % 3; c = t / 9; }
+    TapSlot r;
+    r.valid = s < NTAP;
+    r.a = a; r.b = b; r.c = c;
+    r.off = comp * ACC_N + a + TL * b + ACC_Z * c;
+    return r;
+}
+
+// Stage one chunk: lane l works on entry l % PCAP and computes the nine u
+// taps with x offset q = l / PCAP, the nine v taps with y offset q and the
+// nine w taps with z offset q into the entry's LDS row. Every lane of an
+// entry forms the same support corner (packed as in stage_entry).
+template <bool P2>
+__device__ __forceinline__ void stage_taps(const SpreadArgs &a, const EntryIn &in, double (*s_tap)[TROW],
+                                           int i0, int j0, int k0, int &corner) {
+    const GridDev &g = a.g;
+    if (!in.valid) return;
+    const int e = threadIdx.x & (PCAP - 1), q = threadIdx.x / PCAP;
+    const uint32_t code = in.en & ((1u << SBITS) - 1);
+    const double L3[3] = {g.L[0], g.L[1], g.L[2]};
+    const double dd[3] = {g.dx, g.dy, g.dz};
+    const double id3[3] = {1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz};
+    const double xm[3] = {in.x0, in.x1, in.x2};
+    const int cc[3] = {in.cell % g.nx + 1, (in.cell / g.nx) % g.ny + 1, in.cell / (g.nx * g.ny) + 1};
+    double f0, f1, f2;
+    if (P2) {
+        f0 = in.f0 * a.idV; f1 = in.f1 * a.idV; f2 = in.f2 * a.idV;
+    } else {
+        f0 = in.f0; f1 = in.f1; f2 = in.f2;
+    }
+    double fa[3][4], ce[3][3];   // face factors at offsets 0..3, centre factors at 0..2
+    int cim[3];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        bool has[HALF];
-        double fo[HALF][3], vo[HALF][3];
-        long o[HALF];
+    for (int d = 0; d < 3; ++d) {
+        const uint32_t cd = (code >> (3 * d)) & 7u;
+        if (cd == 0u) {
+            cim[d] = cc[d];
 #pragma unroll
-        for (int r = 0; r < HALF; ++r) {
-            const int q = lane + 64 * (h * HALF + r), k = k0 + h * HALF + r;
-            has[r] = ((s_touch[q >> 5] >> (q & 31)) & 1u) && i <= g.nx && j <= g.ny && k <= g.nz;
-            o[r] = has[r] ? cidx(g, i, j, k) : 0;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                fo[r][c] = (has[r] && !a.first) ? a.fact[c * a.fstride + fbase + q] : 0.0;
-                vo[r][c] = has[r] ? vel[c][o[r]] : 0.0;
+            for (int o = 0; o < 4; ++o) {
+                fa[d][o] = in.fw[d * 4 + o].x;
+                if (o < 3) ce[d][o] = in.fw[d * 4 + o].y;
             }
+            continue;
         }
+        double xp = xm[d];
+        if (cd == 1u || cd == 2u) xp = xp + (cd == 1u ? -L3[d] : L3[d]);
+        else xp = 2.0 * (cd == 3u ? 0.0 : L3[d]) - xp;
+        const int cimg = cell_of(xp, dd[d]);
+        cim[d] = cimg;
 #pragma unroll
-        for (int r = 0; r < HALF; ++r) {
-            if (!has[r]) continue;
-            const int q = lane + 64 * (h * HALF + r), k = k0 + h * HALF + r;
-            const int tgt[3] = {i, j, k};
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double fn = s_acc[c][acc_idx(q)];
-                a.fact[c * a.fstride + fbase + q] = fn;
-                // the low boundary faces (wall, inflow) keep their fixed values
-                const bool face = tgt[c] == 1 && g.bt[2 * c] != IBMDEM_BC_PERIODIC;
-                if (!face && !(fn == 0.0 && fo[r][c] == 0.0)) vel[c][o[r]] = vo[r][c] + a.dt * (fn - fo[r][c]);
-            }
+        for (int o = 0; o < 4; ++o) {
+            fa[d][o] = wt_face<P2>(xp, cimg + o - 1, dd[d], id3[d]);
+            if (o < 3) ce[d][o] = wt_cent<P2>(xp, cimg + o - 1, dd[d], id3[d]);
         }
     }
+    const int o3[3] = {i0, j0, k0};
+    corner = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) corner |= (min(max(cim[d] - 1 - o3[d], -4), 8) + 8) << (5 * d);
+    // this lane's face factor of each dimension (offset q)
+    const double fq0 = q == 0 ? fa[0][0] : (q == 1 ? fa[0][1] : (q == 2 ? fa[0][2] : fa[0][3]));
+    const double fq1 = q == 0 ? fa[1][0] : (q == 1 ? fa[1][1] : (q == 2 ? fa[1][2] : fa[1][3]));
+    const double fq2 = q == 0 ? fa[2][0] : (q == 1 ? fa[2][1] : (q == 2 ? fa[2][2] : fa[2][3]));
+    double *row = s_tap[e];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            // u (q, b, c): (face_x * centre_y) * centre_z
+            const double pu = fq0 * ce[1][b] * ce[2][c];
+            row[q + 4 * b + 12 * c] = -(f0 * (P2 ? pu : pu / a.dV));
+        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int aa = 0; aa < 3; ++aa) {
+            // v (aa, q, c): (centre_x * face_y) * centre_z
+            const double pv = ce[0][aa] * fq1 * ce[2][c];
+            row[36 + aa + 3 * q + 12 * c] = -(f1 * (P2 ? pv : pv / a.dV));
+        }
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int aa = 0; aa < 3; ++aa) {
+            // w (aa, b, q): (centre_x * centre_y) * face_z
+            const double pw = ce[0][aa] * ce[1][b] * fq2;
+            row[72 + aa + 3 * b + 9 * q] = -(f2 * (P2 ? pw : pw / a.dV));
+        }
+}
+
+// One wavefront per active tile, as k_spread_tiles, with the packed taps:
+// chunks of PCAP entries are staged by all 64 lanes (four per entry) while
+// the next chunk's records are in flight; each entry then costs two LDS adds.
+template <bool P2>
+__global__ __launch_bounds__(64) void k_spread_packed(SpreadArgs a) {
+    __shared__ double s_acc[3][ACC_N];         // the tile's force
+    __shared__ unsigned s_touch[TC / 32];      // cells some entry reached
+    __shared__ double s_tap[PCAP][TROW];       // staged entries' taps
+    const int lane = threadIdx.x;
+    const int slot = tile_slot(blockIdx.x, a.meta[0]);
+    if (slot < 0) return;
+    const int elo = a.estart[slot], ehi = a.estart[slot + 1];
+    if (ehi > a.ent_cap) return;   // the fill flagged a capacity error
+    int tx, ty, tz;
+    tile_origin(a.tiles[slot], a.tg, tx, ty, tz);
+    const int i0 = tx * TL + 1, j0 = ty * TL + 1, k0 = tz * TL + 1;
+    for (int q = lane; q < ACC_N; q += 64) {
+        s_acc[0][q] = 0.0;
+        s_acc[1][q] = 0.0;
+        s_acc[2][q] = 0.0;
+    }
+    if (lane < TC / 32)
+        s_touch[lane] = a.first ? 0u : (unsigned)(a.mask[(long)slot * IBM_MASK_WORDS + (lane >> 1)] >> (32 * (lane & 1)));
+    // the lane's two tap slots, and its offset in a support (reached-cell mask)
+    const TapSlot sa = tap_slot(lane), sb = tap_slot(64 + lane);
+    const int oa = lane & 3, ob = (lane >> 2) & 3, oc = lane >> 4;
+    double *acc = &s_acc[0][0];
+    const int me = lane & (PCAP - 1);
+    int corner = 0;
+    wave_sync();
+    EntryIn in, nx;
+    load_entry_id(a, elo + me, me < min(PCAP, ehi - elo), in);
+    load_entry_rec(a, in);
+    load_entry_id(a, elo + PCAP + me, me < min(PCAP, ehi - elo - PCAP), nx);
+    for (int e0 = elo; e0 < ehi; e0 += PCAP) {
+        const int ne = min(PCAP, ehi - e0);
+        stage_taps<P2>(a, in, s_tap, i0, j0, k0, corner);
+        wave_sync();
+        in = nx;
+        load_entry_rec(a, in);
+        load_entry_id(a, e0 + 2 * PCAP + me, me < min(PCAP, ehi - e0 - 2 * PCAP), nx);
+        for (int g0 = 0; g0 < ne; g0 += SU) {
+            double ta[SU], tb[SU];
+            int qa[SU], qb[SU], qt[SU];
+            bool ha[SU], hb[SU], ht[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int e = g0 + u;
+                // the entry's corner from lane e (it staged the entry's x offset 0)
+                const int cp = __builtin_amdgcn_readlane(corner, e);
+                const int bx = (cp & 31) - 8, by = ((cp >> 5) & 31) - 8, bz = ((cp >> 10) & 31) - 8;
+                const int base = bx + TL * by + ACC_Z * bz;
+                const bool live = e < ne;
+                ha[u] = live && (unsigned)(bx + sa.a) < (unsigned)TL && (unsigned)(by + sa.b) < (unsigned)TL &&
+                        (unsigned)(bz + sa.c) < (unsigned)TL;
+                hb[u] = live && sb.valid && (unsigned)(bx + sb.a) < (unsigned)TL &&
+                        (unsigned)(by + sb.b) < (unsigned)TL && (unsigned)(bz + sb.c) < (unsigned)TL;
+                qa[u] = base + sa.off;
+                qb[u] = base + sb.off;
+                ta[u] = s_tap[e][lane];
+                tb[u] = sb.valid ? s_tap[e][64 + lane] : 0.0;
+                const int lx = bx + oa, ly = by + ob, lz = bz + oc;
+                ht[u] = live && (unsigned)lx < (unsigned)TL && (unsigned)ly < (unsigned)TL && (unsigned)lz < (unsigned)TL;
+                qt[u] = lx + TL * (ly + TL * lz);
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                if (ha[u]) lds_add(&acc[qa[u]], ta[u]);
+                if (hb[u]) lds_add(&acc[qb[u]], tb[u]);
+                if (a.first && ht[u]) atomicOr(&s_touch[qt[u] >> 5], 1u << (qt[u] & 31));
+            }
+        }
+        wave_sync();   // the rows are rewritten by the next chunk
+    }
+    tile_epilogue(a, slot, s_acc, s_touch, i0, j0, k0);
 }
 
 __global__ __launch_bounds__(64) void k_scatter_force(const int *__restrict__ meta, const int *__restrict__ tiles,
@@ -955,9 +1164,10 @@ int ibm_interpolate(ibmdem_ctx *c) {
     if (pow2_grid(c)) { if (open) LAUNCH_INTERP(true, true); else LAUNCH_INTERP(true, false); }
     else { if (open) LAUNCH_INTERP(false, true); else LAUNCH_INTERP(false, false); }
 #undef LAUNCH_INTERP
-    // per marker: record 32 B, factors 192, v 24, fdrag 24 + 24, vol 8,
-    // fdrag*vol 24; one read of each footprint velocity (cell, component)
-    prof_end(c, IBMDEM_K_INTERP, 328.0 * c->M + 8.0 * 3.0 * (double)c->footprint_cells);
+    // compulsory bytes: 72 per marker (its position, velocity and the
+    // updated fdrag) and one read of each footprint velocity (cell,
+    // component); the per-step delta-factor table is not compulsory
+    prof_end(c, IBMDEM_K_INTERP, 72.0 * c->M + 8.0 * 3.0 * (double)c->footprint_cells);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
@@ -985,10 +1195,17 @@ int ibm_spread(ibmdem_ctx *c) {
     a.first = c->first_mdf ? 1 : 0;
     prof_begin(c, IBMDEM_K_SPREAD);
     const dim3 grid(c->act_grid);
-    if (pow2_grid(c))
-        hipLaunchKernelGGL((k_spread_tiles<true>), grid, dim3(64), 0, c->stream, a);
-    else
-        hipLaunchKernelGGL((k_spread_tiles<false>), grid, dim3(64), 0, c->stream, a);
+    static const bool one_add_per_component = [] {
+        const char *e = std::getenv("IBMDEM_SPREAD_PER_COMPONENT");
+        return e && *e && *e != '0';
+    }();
+    if (one_add_per_component) {
+        if (pow2_grid(c)) hipLaunchKernelGGL((k_spread_tiles<true>), grid, dim3(64), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_spread_tiles<false>), grid, dim3(64), 0, c->stream, a);
+    } else {
+        if (pow2_grid(c)) hipLaunchKernelGGL((k_spread_packed<true>), grid, dim3(64), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_spread_packed<false>), grid, dim3(64), 0, c->stream, a);
+    }
     prof_end(c, IBMDEM_K_SPREAD, 0.0);
     CHECK_LAUNCH(c);
     if (c->prof.kernel == IBMDEM_K_SPREAD || c->prof.kernel == IBMDEM_K_INTERP) {
@@ -1002,12 +1219,13 @@ int ibm_spread(ibmdem_ctx *c) {
             HIPX(c, hipStreamSynchronize(c->stream));
             c->footprint_cells = (long)h;
         }
-        // per marker: its 64 B record and 192 B of delta factors; per written
-        // (cell, component): u read and written, f written, f_old read (not
-        // in the first iteration)
+        // compulsory bytes: 60 per marker (position, fdrag, volume, cell) and
+        // per written (cell, component) u read and written, f written and
+        // f_old read (not in the first iteration); the per-step delta-factor
+        // table and the records' re-reads by neighbouring tiles are not
         const double per = a.first ? 24.0 : 32.0;
         if (c->prof.kernel == IBMDEM_K_SPREAD)
-            c->prof.bytes_total += 256.0 * c->M + per * 3.0 * (double)c->footprint_cells;
+            c->prof.bytes_total += 60.0 * c->M + per * 3.0 * (double)c->footprint_cells;
     }
     c->first_mdf = false;
     c->spread_valid = true;

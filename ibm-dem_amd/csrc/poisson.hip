@@ -1478,7 +1478,9 @@ int poisson_check(ibmdem_ctx *c, ibmdem_solve_info *info) {
         info->rel_residual = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
         info->rhs_norm = bnorm;
     }
-    if (!(rr == rr)) return set_error(c, IBMDEM_ERR_NAN, "non-finite residual in the Poisson solve");
+    // a non-finite right-hand side (||b|| is then NaN or inf) or residual
+    if (!(bnorm - bnorm == 0.0) || !(rr == rr))
+        return set_error(c, IBMDEM_ERR_NAN, "non-finite right-hand side or residual in the Poisson solve");
     if (!conv) return set_error(c, IBMDEM_ERR_NOT_CONVERGED, "Poisson solve did not converge");
     return IBMDEM_OK;
 }

@@ -1832,6 +1832,17 @@ int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {
  * pin directly.
  * ------------------------------------------------------------------------ */
 double ibmdem_oracle_phi(double r) { return phi1(r); }
+/* OpenMP threads of the following calls (the CPU baseline times the oracle at
+ * several counts in one process); returns the count in effect */
+int ibmdem_oracle_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 double ibmdem_oracle_eta(double restitution, int textbook) {
     ibmdem_case cs;
     memset(&cs, 0, sizeof cs);

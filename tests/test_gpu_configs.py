@@ -90,9 +90,11 @@ def test_c1_settling_1000_step_trajectory(gpu_lib):
     ke_err = np.abs(ke - gold["ke"]).max() / np.abs(gold["ke"]).max()
     print(f"C1 1000 steps: displacement {dx_err:.3e}, velocity {v_err:.3e}, kinetic energy {ke_err:.3e}")
     assert np.abs(gold["x"][-1, 2] - x0[2]) > 50.0   # it settled a long way
-    assert dx_err < 1e-9
-    assert v_err < 1e-9
-    assert ke_err < 1e-9
+    # the same algorithm and CG iteration counts: only summation orders
+    # differ, so the trajectories agree far inside the 1e-10 FP64 target
+    assert dx_err < 1e-12
+    assert v_err < 1e-12
+    assert ke_err < 1e-12
 
 
 # ---------------------------------------------------------------------------
