@@ -39,8 +39,6 @@
 // the step).
 #include "common.hpp"
 
-#include <cstdlib>
-
 namespace {
 
 constexpr int T = 256;
@@ -199,7 +197,7 @@ __device__ __forceinline__ double tap4(const double *__restrict__ f, const GridD
 // L1/L2; each marker's arithmetic is independent of the order.
 template <bool P2, bool OPEN>
 __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__restrict__ order,
-                                              const double *__restrict__ fac, const double *__restrict__ u,
+                                              const double *__restrict__ u,
                                               const double *__restrict__ v, const double *__restrict__ w,
                                               double *__restrict__ rec, const double *__restrict__ mv,
                                               double *__restrict__ fdrag, const double *__restrict__ mvol, double dV,
@@ -218,17 +216,25 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__res
     const int cl = (int)__double_as_longlong(r23.y);
     const int ci = cl % g.nx + 1, cj = (cl / g.nx) % g.ny + 1, ck = cl / (g.nx * g.ny) + 1;
     const double dx = g.dx, dy = g.dy, dz = g.dz;
-    (void)xp;
-    // the step's delta factors of the marker (k_records); centre factors at
-    // offset +2 are 0 and not used below
+    // the marker's delta factors, computed here rather than read from the
+    // step's table (the same function as k_records, so the same values; 21
+    // evaluations cost less than streaming the 192-byte row every
+    // iteration); centre factors at offset +2 are 0 and not used below
     double fx[4], cx[3], fy[4], cy[3], fz[4], cz[3];
-    const double2 *fp = reinterpret_cast<const double2 *>(fac + 24L * m);
+    auto factors = [&]() {
+        const double ix = 1.0 / dx, iy = 1.0 / dy, iz = 1.0 / dz;
 #pragma unroll
-    for (int o = 0; o < 4; ++o) {
-        const double2 a = fp[o], b = fp[4 + o], c = fp[8 + o];
-        fx[o] = a.x; fy[o] = b.x; fz[o] = c.x;
-        if (o < 3) { cx[o] = a.y; cy[o] = b.y; cz[o] = c.y; }
-    }
+        for (int o = 0; o < 4; ++o) {
+            fx[o] = wt_face<P2, true>(xp[0], ci + o - 1, dx, ix);
+            fy[o] = wt_face<P2, true>(xp[1], cj + o - 1, dy, iy);
+            fz[o] = wt_face<P2, true>(xp[2], ck + o - 1, dz, iz);
+            if (o < 3) {
+                cx[o] = wt_cent<P2, true>(xp[0], ci + o - 1, dx, ix);
+                cy[o] = wt_cent<P2, true>(xp[1], cj + o - 1, dy, iy);
+                cz[o] = wt_cent<P2, true>(xp[2], ck + o - 1, dz, iz);
+            }
+        }
+    };
     // the ghost rules of the taps -1..+2 around the cell, once per marker
     TapMap4 mx, my, mz;
     mx.init(ci, g.nx, g.bt[0], g.bt[1]);
@@ -244,7 +250,8 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__res
     if (plain) {
         // Each component's 36 taps are loaded before its sum starts, so a
         // wavefront keeps them in flight together; the sum then runs in the
-        // reference's (kk, jj, ii) order.
+        // reference's (kk, jj, ii) order. The factors are computed while the
+        // u taps are in flight.
         // u: x-face taps -1..2, y/z centre taps -1..1 (the nonzero ranges)
         double t[36];
 #pragma unroll
@@ -253,6 +260,7 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__res
             for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
                 for (int ii = 0; ii < 4; ++ii) t[(kk * 3 + jj) * 4 + ii] = u[tap_off(g, mx, my, mz, ii, jj, kk)];
+        factors();
         __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the sum
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk)
@@ -293,6 +301,7 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__res
                     wlp = wlp + interp_term<P2>(cx[ii] * cy[jj] * fz[kk], t[(kk * 3 + jj) * 3 + ii], dV, idV, dx, dy,
                                                 dz);
     } else {
+        factors();
         for (int kk = 0; kk < 3; ++kk)
             for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
@@ -864,202 +873,6 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     tile_epilogue(a, slot, s_acc, s_touch, i0, j0, k0);
 }
 
-// ---------------------------------------------------------------------------
-// Packed taps. Of an entry's 4x4x4 support, u has 36 nonzero taps (the x-face
-// factor at offsets 0..3 times the y and z centre factors at offsets 0..2; the
-// centre factor at offset 3 is always 0, see k_records), v and w likewise with
-// their own face dimension: 108 taps in all, numbered
-//   u (a, b, c): a + 4b + 12c        a in 0..3, b, c in 0..2
-//   v (a, b, c): 36 + a + 3b + 12c   b in 0..3, a, c in 0..2
-//   w (a, b, c): 72 + a + 3b + 9c    c in 0..3, a, b in 0..2
-// and summed as two 64-lane LDS adds per entry (tap s: lane s % 64 of add
-// s / 64) instead of one 64-lane add per component. Each tap's value is the
-// term of the one-add-per-component form, -(f * ((phi_x phi_y) phi_z)), and
-// every cell still receives its terms in entry order, so the force is the
-// same to the bit.
-// ---------------------------------------------------------------------------
-constexpr int NTAP = 108;
-constexpr int PCAP = 16;        // entries staged per chunk, four lanes each
-constexpr int PLANES = 64 / PCAP;
-constexpr int TROW = NTAP + 1;  // LDS row stride of an entry's taps (odd)
-static_assert(PLANES == 4, "each staged entry's taps are split four ways (9 per component per lane)");
-
-struct TapSlot {
-    int off;       // accumulator offset from the support corner
-    int a, b, c;   // support offsets
-    bool valid;
-};
-__device__ __forceinline__ TapSlot tap_slot(int s) {
-    int comp, a, b, c;
-    if (s < 36) { comp = 0; a = s % 4; b = (s / 4) % 3; c = s / 12; }
-    else if (s < 72) { const int t = s - 36; comp = 1; a = t % 3; b = (t / 3) % 4; c = t / 12; }
-    else { const int t = s - 72; comp = 2; a = t % 3; b = (t / 3) % 3; c = t / 9; }
-    TapSlot r;
-    r.valid = s < NTAP;
-    r.a = a; r.b = b; r.c = c;
-    r.off = comp * ACC_N + a + TL * b + ACC_Z * c;
-    return r;
-}
-
-// Stage one chunk: lane l works on entry l % PCAP and computes the nine u
-// taps with x offset q = l / PCAP, the nine v taps with y offset q and the
-// nine w taps with z offset q into the entry's LDS row. Every lane of an
-// entry forms the same support corner (packed as in stage_entry).
-template <bool P2>
-__device__ __forceinline__ void stage_taps(const SpreadArgs &a, const EntryIn &in, double (*s_tap)[TROW],
-                                           int i0, int j0, int k0, int &corner) {
-    const GridDev &g = a.g;
-    if (!in.valid) return;
-    const int e = threadIdx.x & (PCAP - 1), q = threadIdx.x / PCAP;
-    const uint32_t code = in.en & ((1u << SBITS) - 1);
-    const double L3[3] = {g.L[0], g.L[1], g.L[2]};
-    const double dd[3] = {g.dx, g.dy, g.dz};
-    const double id3[3] = {1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz};
-    const double xm[3] = {in.x0, in.x1, in.x2};
-    const int cc[3] = {in.cell % g.nx + 1, (in.cell / g.nx) % g.ny + 1, in.cell / (g.nx * g.ny) + 1};
-    double f0, f1, f2;
-    if (P2) {
-        f0 = in.f0 * a.idV; f1 = in.f1 * a.idV; f2 = in.f2 * a.idV;
-    } else {
-        f0 = in.f0; f1 = in.f1; f2 = in.f2;
-    }
-    double fa[3][4], ce[3][3];   // face factors at offsets 0..3, centre factors at 0..2
-    int cim[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const uint32_t cd = (code >> (3 * d)) & 7u;
-        if (cd == 0u) {
-            cim[d] = cc[d];
-#pragma unroll
-            for (int o = 0; o < 4; ++o) {
-                fa[d][o] = in.fw[d * 4 + o].x;
-                if (o < 3) ce[d][o] = in.fw[d * 4 + o].y;
-            }
-            continue;
-        }
-        double xp = xm[d];
-        if (cd == 1u || cd == 2u) xp = xp + (cd == 1u ? -L3[d] : L3[d]);
-        else xp = 2.0 * (cd == 3u ? 0.0 : L3[d]) - xp;
-        const int cimg = cell_of(xp, dd[d]);
-        cim[d] = cimg;
-#pragma unroll
-        for (int o = 0; o < 4; ++o) {
-            fa[d][o] = wt_face<P2>(xp, cimg + o - 1, dd[d], id3[d]);
-            if (o < 3) ce[d][o] = wt_cent<P2>(xp, cimg + o - 1, dd[d], id3[d]);
-        }
-    }
-    const int o3[3] = {i0, j0, k0};
-    corner = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) corner |= (min(max(cim[d] - 1 - o3[d], -4), 8) + 8) << (5 * d);
-    // this lane's face factor of each dimension (offset q)
-    const double fq0 = q == 0 ? fa[0][0] : (q == 1 ? fa[0][1] : (q == 2 ? fa[0][2] : fa[0][3]));
-    const double fq1 = q == 0 ? fa[1][0] : (q == 1 ? fa[1][1] : (q == 2 ? fa[1][2] : fa[1][3]));
-    const double fq2 = q == 0 ? fa[2][0] : (q == 1 ? fa[2][1] : (q == 2 ? fa[2][2] : fa[2][3]));
-    double *row = s_tap[e];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            // u (q, b, c): (face_x * centre_y) * centre_z
-            const double pu = fq0 * ce[1][b] * ce[2][c];
-            row[q + 4 * b + 12 * c] = -(f0 * (P2 ? pu : pu / a.dV));
-        }
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int aa = 0; aa < 3; ++aa) {
-            // v (aa, q, c): (centre_x * face_y) * centre_z
-            const double pv = ce[0][aa] * fq1 * ce[2][c];
-            row[36 + aa + 3 * q + 12 * c] = -(f1 * (P2 ? pv : pv / a.dV));
-        }
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-        for (int aa = 0; aa < 3; ++aa) {
-            // w (aa, b, q): (centre_x * centre_y) * face_z
-            const double pw = ce[0][aa] * ce[1][b] * fq2;
-            row[72 + aa + 3 * b + 9 * q] = -(f2 * (P2 ? pw : pw / a.dV));
-        }
-}
-
-// One wavefront per active tile, as k_spread_tiles, with the packed taps:
-// chunks of PCAP entries are staged by all 64 lanes (four per entry) while
-// the next chunk's records are in flight; each entry then costs two LDS adds.
-template <bool P2>
-__global__ __launch_bounds__(64) void k_spread_packed(SpreadArgs a) {
-    __shared__ double s_acc[3][ACC_N];         // the tile's force
-    __shared__ unsigned s_touch[TC / 32];      // cells some entry reached
-    __shared__ double s_tap[PCAP][TROW];       // staged entries' taps
-    const int lane = threadIdx.x;
-    const int slot = tile_slot(blockIdx.x, a.meta[0]);
-    if (slot < 0) return;
-    const int elo = a.estart[slot], ehi = a.estart[slot + 1];
-    if (ehi > a.ent_cap) return;   // the fill flagged a capacity error
-    int tx, ty, tz;
-    tile_origin(a.tiles[slot], a.tg, tx, ty, tz);
-    const int i0 = tx * TL + 1, j0 = ty * TL + 1, k0 = tz * TL + 1;
-    for (int q = lane; q < ACC_N; q += 64) {
-        s_acc[0][q] = 0.0;
-        s_acc[1][q] = 0.0;
-        s_acc[2][q] = 0.0;
-    }
-    if (lane < TC / 32)
-        s_touch[lane] = a.first ? 0u : (unsigned)(a.mask[(long)slot * IBM_MASK_WORDS + (lane >> 1)] >> (32 * (lane & 1)));
-    // the lane's two tap slots, and its offset in a support (reached-cell mask)
-    const TapSlot sa = tap_slot(lane), sb = tap_slot(64 + lane);
-    const int oa = lane & 3, ob = (lane >> 2) & 3, oc = lane >> 4;
-    double *acc = &s_acc[0][0];
-    const int me = lane & (PCAP - 1);
-    int corner = 0;
-    wave_sync();
-    EntryIn in, nx;
-    load_entry_id(a, elo + me, me < min(PCAP, ehi - elo), in);
-    load_entry_rec(a, in);
-    load_entry_id(a, elo + PCAP + me, me < min(PCAP, ehi - elo - PCAP), nx);
-    for (int e0 = elo; e0 < ehi; e0 += PCAP) {
-        const int ne = min(PCAP, ehi - e0);
-        stage_taps<P2>(a, in, s_tap, i0, j0, k0, corner);
-        wave_sync();
-        in = nx;
-        load_entry_rec(a, in);
-        load_entry_id(a, e0 + 2 * PCAP + me, me < min(PCAP, ehi - e0 - 2 * PCAP), nx);
-        for (int g0 = 0; g0 < ne; g0 += SU) {
-            double ta[SU], tb[SU];
-            int qa[SU], qb[SU], qt[SU];
-            bool ha[SU], hb[SU], ht[SU];
-#pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const int e = g0 + u;
-                // the entry's corner from lane e (it staged the entry's x offset 0)
-                const int cp = __builtin_amdgcn_readlane(corner, e);
-                const int bx = (cp & 31) - 8, by = ((cp >> 5) & 31) - 8, bz = ((cp >> 10) & 31) - 8;
-                const int base = bx + TL * by + ACC_Z * bz;
-                const bool live = e < ne;
-                ha[u] = live && (unsigned)(bx + sa.a) < (unsigned)TL && (unsigned)(by + sa.b) < (unsigned)TL &&
-                        (unsigned)(bz + sa.c) < (unsigned)TL;
-                hb[u] = live && sb.valid && (unsigned)(bx + sb.a) < (unsigned)TL &&
-                        (unsigned)(by + sb.b) < (unsigned)TL && (unsigned)(bz + sb.c) < (unsigned)TL;
-                qa[u] = base + sa.off;
-                qb[u] = base + sb.off;
-                ta[u] = s_tap[e][lane];
-                tb[u] = sb.valid ? s_tap[e][64 + lane] : 0.0;
-                const int lx = bx + oa, ly = by + ob, lz = bz + oc;
-                ht[u] = live && (unsigned)lx < (unsigned)TL && (unsigned)ly < (unsigned)TL && (unsigned)lz < (unsigned)TL;
-                qt[u] = lx + TL * (ly + TL * lz);
-            }
-#pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                if (ha[u]) lds_add(&acc[qa[u]], ta[u]);
-                if (hb[u]) lds_add(&acc[qb[u]], tb[u]);
-                if (a.first && ht[u]) atomicOr(&s_touch[qt[u] >> 5], 1u << (qt[u] & 31));
-            }
-        }
-        wave_sync();   // the rows are rewritten by the next chunk
-    }
-    tile_epilogue(a, slot, s_acc, s_touch, i0, j0, k0);
-}
-
 __global__ __launch_bounds__(64) void k_scatter_force(const int *__restrict__ meta, const int *__restrict__ tiles,
                                                       TileGeom tg, GridDev g, const double *__restrict__ fact,
                                                       long fstride, const unsigned long long *__restrict__ mask,
@@ -1157,8 +970,7 @@ int ibm_interpolate(ibmdem_ctx *c) {
     prof_begin(c, IBMDEM_K_INTERP);
 #define LAUNCH_INTERP(P2, OP)                                                                                      \
     hipLaunchKernelGGL((k_interp<P2, OP>), dim3(8 * div_up(div_up(c->M, T), 8)), dim3(T), 0, c->stream, c->M, c->g, \
-                       c->bin_items,                                                                                  \
-                       c->mfac, c->u[cur],                                                                            \
+                       c->bin_items, c->u[cur],                                                                       \
                        c->v[cur], c->w[cur], c->mrec, c->mv, c->fdrag, c->mvol, c->dV, 1.0 / c->dV, c->cs.dt)
     const bool open = grid_open(c->g);
     if (pow2_grid(c)) { if (open) LAUNCH_INTERP(true, true); else LAUNCH_INTERP(true, false); }
@@ -1195,17 +1007,10 @@ int ibm_spread(ibmdem_ctx *c) {
     a.first = c->first_mdf ? 1 : 0;
     prof_begin(c, IBMDEM_K_SPREAD);
     const dim3 grid(c->act_grid);
-    static const bool one_add_per_component = [] {
-        const char *e = std::getenv("IBMDEM_SPREAD_PER_COMPONENT");
-        return e && *e && *e != '0';
-    }();
-    if (one_add_per_component) {
-        if (pow2_grid(c)) hipLaunchKernelGGL((k_spread_tiles<true>), grid, dim3(64), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_spread_tiles<false>), grid, dim3(64), 0, c->stream, a);
-    } else {
-        if (pow2_grid(c)) hipLaunchKernelGGL((k_spread_packed<true>), grid, dim3(64), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_spread_packed<false>), grid, dim3(64), 0, c->stream, a);
-    }
+    if (pow2_grid(c))
+        hipLaunchKernelGGL((k_spread_tiles<true>), grid, dim3(64), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_spread_tiles<false>), grid, dim3(64), 0, c->stream, a);
     prof_end(c, IBMDEM_K_SPREAD, 0.0);
     CHECK_LAUNCH(c);
     if (c->prof.kernel == IBMDEM_K_SPREAD || c->prof.kernel == IBMDEM_K_INTERP) {
