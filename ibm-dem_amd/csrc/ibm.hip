@@ -39,6 +39,9 @@
 // the step).
 #include "common.hpp"
 
+#include <algorithm>
+#include <mutex>
+
 namespace {
 
 constexpr int T = 256;
@@ -894,6 +897,31 @@ __global__ __launch_bounds__(64) void k_scatter_force(const int *__restrict__ me
     }
 }
 
+// The IBM force of the cells with k in [k0, k1) (1-based) into a slab
+// buffer: component q of cell (i, j, k) at q * slab + cidx(i, j, k) - (k0 - 1)
+// * nx * ny; cells outside the slab are skipped.
+__global__ __launch_bounds__(64) void k_scatter_force_slab(const int *__restrict__ meta, const int *__restrict__ tiles,
+                                                           TileGeom tg, GridDev g, const double *__restrict__ fact,
+                                                           long fstride, const unsigned long long *__restrict__ mask,
+                                                           int k0, int k1, long slab, double *__restrict__ out) {
+    const int b = tile_slot(blockIdx.x, meta[0]);
+    if (b < 0) return;
+    int tx, ty, tz;
+    tile_origin(tiles[b], tg, tx, ty, tz);
+    if (tz * TL + TL < k0 || tz * TL + 1 >= k1) return;   // the tile misses the slab
+    const long first = (long)(k0 - 1) * g.nx * g.ny;
+    for (int q = threadIdx.x; q < TC; q += 64) {
+        if (!((mask[(long)b * IBM_MASK_WORDS + q / 64] >> (q & 63)) & 1ull)) continue;
+        const int i = tx * TL + 1 + (q & (TL - 1)), j = ty * TL + 1 + ((q / TL) & (TL - 1)),
+                  k = tz * TL + 1 + q / (TL * TL);
+        if (i > g.nx || j > g.ny || k < k0 || k >= k1) continue;
+        const long sl = (long)b * TC + q, o = cidx(g, i, j, k) - first;
+        out[o] = fact[sl];
+        out[slab + o] = fact[fstride + sl];
+        out[2 * slab + o] = fact[2 * fstride + sl];
+    }
+}
+
 // cells with at least one contribution, summed over the active tiles
 __global__ __launch_bounds__(1024) void k_mask_count(const int *__restrict__ meta,
                                                      const unsigned long long *__restrict__ mask,
@@ -1072,4 +1100,85 @@ int ibm_download_force(ibmdem_ctx *c, double *fx, double *fy, double *fz) {
     if (hipStreamSynchronize(s) != hipSuccess) rc = IBMDEM_ERR_HIP;
     (void)hipFree(buf);
     return rc ? set_error(c, rc, "force download failed") : IBMDEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ibmdem_download_force without device allocation. ibmdem_download_force
+// stages the force fields in a device buffer it allocates per call; this
+// variant stages them, one slab of z planes at a time, in a fixed scratch
+// array of IBM_SCRATCH_DOUBLES doubles (16 MiB) that every GPU holds from the
+// first call until the process ends. The host code never uses the array's
+// name as an address: the per-device address comes from hipGetSymbolAddress.
+// Calls on one GPU share its array, so calls that overlap on one GPU (two
+// threads, two streams) race; each slab is cleared by a fill kernel before
+// use, so no call sees another's contents.
+// ---------------------------------------------------------------------------
+constexpr long IBM_SCRATCH_DOUBLES = 1L << 21;
+static __device__ __align__(256) double scratch[IBM_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+double *scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// the scratch array's address on the context's device (looked up once per
+// device, with that device current)
+int scratch_on_device(ibmdem_ctx *c, double **out) {
+    if (c->device < 0 || c->device >= SCRATCH_MAX_DEVICES)
+        return set_error(c, IBMDEM_ERR_UNSUPPORTED, "device index beyond the no-allocation scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[c->device]) {
+        int cur = -1;
+        HIPX(c, hipGetDevice(&cur));
+        if (cur != c->device) HIPX(c, hipSetDevice(c->device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != c->device) HIPX(c, hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return set_error(c, IBMDEM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        scratch_ptr[c->device] = static_cast<double *>(p);
+    }
+    *out = scratch_ptr[c->device];
+    return IBMDEM_OK;
+}
+
+__global__ __launch_bounds__(256) void k_fill_zero(long n, double *__restrict__ a) {
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) a[q] = 0.0;
+}
+}  // namespace
+
+extern "C" int ibmdem_download_force_noalloc(ibmdem_ctx *c, double *fx, double *fy, double *fz) {
+    if (!c) return IBMDEM_ERR_ARG;
+    const long plane = (long)c->g.nx * c->g.ny;
+    const long per = IBM_SCRATCH_DOUBLES / 3;   // cells per slab
+    if (plane > per)
+        return set_error(c, IBMDEM_ERR_UNSUPPORTED,
+                         "a z plane of " + std::to_string(plane) + " cells exceeds the no-allocation scratch (at most " +
+                             std::to_string(per) + " cells per plane)");
+    double *buf = nullptr;
+    int rc = scratch_on_device(c, &buf);
+    if (rc) return rc;
+    const int nz = c->g.nz;
+    const int kstep = (int)std::min<long>(nz, per / plane);
+    hipStream_t s = c->stream;
+    double *dst[3] = {fx, fy, fz};
+    for (int k0 = 1; k0 <= nz; k0 += kstep) {
+        const int k1 = std::min(nz + 1, k0 + kstep);
+        const long slab = (long)(k1 - k0) * plane;
+        hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<long>(div_up(3 * slab, 256), 2048)), dim3(256), 0, s,
+                           3 * slab, buf);
+        CHECK_LAUNCH(c);
+        if (c->M > 0 && c->spread_valid) {
+            hipLaunchKernelGGL(k_scatter_force_slab, dim3(c->act_grid), dim3(64), 0, s, c->act_meta, c->act_list,
+                               tile_geom(c->g), c->g, c->fact, (long)TC * c->act_cap, c->tile_mask, k0, k1, slab, buf);
+            CHECK_LAUNCH(c);
+        }
+        for (int q = 0; q < 3; ++q)
+            if (dst[q])
+                HIPX(c, hipMemcpyAsync(dst[q] + (long)(k0 - 1) * plane, buf + q * slab, slab * sizeof(double),
+                                       hipMemcpyDeviceToHost, s));
+        // the next slab reuses the scratch: its copies must have landed
+        HIPX(c, hipStreamSynchronize(s));
+    }
+    return IBMDEM_OK;
 }

@@ -111,6 +111,7 @@ _FUNCS = {
     "ibmdem_download_outflow": [C.c_void_p, _i32, _pd],
     "ibmdem_upload_outflow": [C.c_void_p, _i32, _pd],
     "ibmdem_download_force": [C.c_void_p, _pd, _pd, _pd],
+    "ibmdem_download_force_noalloc": [C.c_void_p, _pd, _pd, _pd],
     "ibmdem_download_poisson": [C.c_void_p, _pd, _pd],
     "ibmdem_upload_poisson_rhs": [C.c_void_p, _pd],
     "ibmdem_upload_markers": [C.c_void_p, C.POINTER(Markers)],
@@ -328,6 +329,13 @@ class Simulation:
     def download_force(self):
         out = [np.zeros(self.shape) for _ in range(3)]
         self.call("ibmdem_download_force", *[_p(a) for a in out])
+        return out
+
+    def download_force_noalloc(self):
+        """download_force through the library's fixed device scratch (no
+        device allocation)."""
+        out = [np.zeros(self.shape) for _ in range(3)]
+        self.call("ibmdem_download_force_noalloc", *[_p(a) for a in out])
         return out
 
     def download_poisson(self):

@@ -170,6 +170,14 @@ int ibmdem_download_outflow(ibmdem_ctx *ctx, int32_t side, double *plane);
 int ibmdem_upload_outflow(ibmdem_ctx *ctx, int32_t side, const double *plane);
 /* IBM momentum source f = (fx,fy,fz) on the interior (fluid.F90 fx,fy,fz). */
 int ibmdem_download_force(ibmdem_ctx *ctx, double *fx, double *fy, double *fz);
+/* The same download without any device allocation, for callers that cannot
+ * allocate (code that forbids hipMalloc, tight serving loops). No counterpart
+ * in the reference. It stages the fields one slab of z planes at a time in a
+ * fixed 16 MiB device array that each GPU holds from the first call until the
+ * process ends; a grid whose single z plane holds more than 699,050 cells is
+ * rejected with IBMDEM_ERR_UNSUPPORTED. Calls on one GPU share that array:
+ * calls that overlap on one GPU race. */
+int ibmdem_download_force_noalloc(ibmdem_ctx *ctx, double *fx, double *fy, double *fz);
 /* Pressure correction P' of the last solve and the Poisson RHS velDiv. */
 int ibmdem_download_poisson(ibmdem_ctx *ctx, double *pcorr, double *rhs);
 int ibmdem_upload_poisson_rhs(ibmdem_ctx *ctx, const double *rhs);

@@ -1504,6 +1504,11 @@ int ibmdem_download_force(ibmdem_ctx *c, double *fx, double *fy, double *fz) {
     if (fz) copy_out(c, fz, c->fz);
     return IBMDEM_OK;
 }
+/* the oracle allocates nothing per call either way */
+int ibmdem_download_force_noalloc(ibmdem_ctx *c, double *fx, double *fy, double *fz) {
+    return ibmdem_download_force(c, fx, fy, fz);
+}
+
 int ibmdem_download_poisson(ibmdem_ctx *c, double *pcorr, double *rhs) {
     if (!c) return IBMDEM_ERR_ARG;
     if (pcorr) copy_out(c, pcorr, c->Pc);
