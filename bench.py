@@ -64,6 +64,9 @@ def parse():
                          "10,000 spheres; cylinder in crossflow 512x256x64) in FP64 and mixed precision")
     ap.add_argument("--config-steps", type=int, default=10,
                     help="timed steps of the other configurations and the batch")
+    ap.add_argument("--cylinder-steps", type=int, default=0,
+                    help="also run the cylinder in crossflow this many steps in FP64 and mixed precision and "
+                         "report its force coefficients over time (C_D, C_L, mean C_D, Strouhal number)")
     return ap.parse_args()
 
 
@@ -300,6 +303,59 @@ def config_runs(steps, warmup):
     return out
 
 
+def cylinder_shedding(steps, every=10):
+    """The cylinder in crossflow (512x256x64, Re 200) over `steps` steps in
+    FP64 and mixed precision from the same state. The cylinder's assembled
+    force is recorded after every step in the device probe ring (no host
+    wait inside the run); C_D = 2 F_x / (rho U^2 D L), C_L from F_y; mean
+    C_D, rms C_L and the Strouhal number over the second half of the run;
+    the histories every `every` steps."""
+    import numpy as np
+
+    import ibmdem
+    from ibmdem import cases, diagnostics
+    from ibmdem.abi import FP64, MIXED
+
+    setup = cases.config_cylinder(precision=FP64)
+    m = setup.meta
+    body = m["cylinder_body"]
+    q = 0.5 * setup.case.rhof * m["U"] ** 2 * m["D"] * m["span"]
+    out = {"steps": steps, "dt": setup.case.dt, "sample_every": every}
+    hist = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        setup.case.precision = prec
+        with ibmdem.simulation(setup.case) as sim:
+            cases.load(sim, setup)
+            sim.probe_bodies([body], steps)
+            sim.sync()
+            t0 = time.perf_counter()
+            done = 0
+            while done < steps:
+                k = min(2000, steps - done)
+                sim.run(k)
+                done += k
+                print(f"[bench] cylinder {name}: {done}/{steps} steps, {time.perf_counter() - t0:.0f} s",
+                      file=sys.stderr, flush=True)
+            sim.sync()
+            secs = time.perf_counter() - t0
+            first, rec = sim.probe_read()
+        n = rec.shape[0]
+        t = setup.case.dt * (first + np.arange(n) + 1)
+        cd, cl = rec[:, 0, 9] / q, rec[:, 0, 10] / q
+        stats = diagnostics.shedding_statistics(t, cd, cl, m["D"], m["U"])
+        hist[name] = (cd, cl)
+        out[name] = dict(stats, steps_per_s=steps / secs,
+                         cd_history=[round(float(v), 6) for v in cd[every - 1::every]],
+                         cl_history=[round(float(v), 6) for v in cl[every - 1::every]])
+    (cd64, cl64), (cd32, cl32) = hist["fp64"], hist["mixed"]
+    out["mixed_vs_fp64"] = {"max_abs_cd": float(np.abs(cd32 - cd64).max()),
+                            "max_abs_cl": float(np.abs(cl32 - cl64).max()),
+                            "mean_cd_rel": abs(out["mixed"]["mean_cd"] - out["fp64"]["mean_cd"]) / abs(out["fp64"]["mean_cd"]),
+                            "strouhal_abs": (abs(out["mixed"]["strouhal"] - out["fp64"]["strouhal"])
+                                             if out["mixed"]["strouhal"] and out["fp64"]["strouhal"] else None)}
+    return out
+
+
 def host_cores():
     """Cores this process may use (its CPU affinity set, capped by
     OMP_NUM_THREADS when that is set: a GPU box grants a share of its host)
@@ -441,6 +497,8 @@ def main():
         line["configs"] = config_runs(csteps, cwarm)
     if not args.no_batched:
         line["batched"] = batched_run(csteps, cwarm)
+    if args.cylinder_steps > 0:
+        line["cylinder_shedding"] = cylinder_shedding(args.cylinder_steps)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(setup, results["fp64"]["state"] if "fp64" in results else r["state"])

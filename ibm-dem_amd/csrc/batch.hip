@@ -7,7 +7,8 @@
 // graph, then collect each case's error flags), so the cases' kernels overlap
 // on the GPU and the host never waits for one case while another could be
 // queued. The autotuner picks, per case, the multigrid cycle (pre/post
-// smoothing sweeps, coarsest-level sweeps) with the shortest measured step.
+// smoothing sweeps, coarsest-level sweeps) with the shortest measured step,
+// then the predictor's tile schedule (case.stencil_tile) under that cycle.
 #include "common.hpp"
 
 #include <chrono>
@@ -135,6 +136,22 @@ int ibmdem_set_mg_params(ibmdem_ctx *c, const int32_t *mg) {
     return IBMDEM_OK;
 }
 
+int ibmdem_get_stencil_tile(ibmdem_ctx *c, int32_t *tile) {
+    if (!c || !tile) return IBMDEM_ERR_ARG;
+    *tile = c->cs.stencil_tile;
+    return IBMDEM_OK;
+}
+
+int ibmdem_set_stencil_tile(ibmdem_ctx *c, int32_t tile) {
+    if (!c) return IBMDEM_ERR_ARG;
+    if (tile < 0 || tile > 3) return set_error(c, IBMDEM_ERR_ARG, "stencil_tile must be 0..3");
+    int rc = ibmdem_sync(c);
+    if (rc) return rc;
+    c->cs.stencil_tile = tile;
+    drop_graphs(c);   // captured graphs hold the old schedule
+    return IBMDEM_OK;
+}
+
 int ibmdem_batch_create(int32_t n, const ibmdem_case *cases, ibmdem_batch **out) {
     if (!out || !cases || n < 1) return IBMDEM_ERR_ARG;
     *out = nullptr;
@@ -227,6 +244,18 @@ int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {
         }
         if ((rc = ibmdem_set_mg_params(c, best_i >= 0 ? cand[best_i] : orig))) return batch_error(b, q, rc);
         if ((rc = restore_state(c, h))) return batch_error(b, q, rc);
+        // then the predictor schedule (every schedule gives the same bits)
+        const int tile0 = c->cs.stencil_tile;
+        double best_t = 1e300;
+        int best_tile = tile0;
+        for (int t = 0; t < 4; ++t) {
+            if ((rc = ibmdem_set_stencil_tile(c, t))) return batch_error(b, q, rc);
+            double sec = 0.0;
+            rc = time_steps(c, steps, &sec);
+            if (rc == IBMDEM_OK && sec < best_t) { best_t = sec; best_tile = t; }
+            if ((rc = restore_state(c, h))) return batch_error(b, q, rc);
+        }
+        if ((rc = ibmdem_set_stencil_tile(c, best_tile))) return batch_error(b, q, rc);
     }
     return IBMDEM_OK;
 }

@@ -498,6 +498,14 @@ struct ibmdem_ctx {
     int *pair_count = nullptr, *pair_partner = nullptr, *pair_shift = nullptr;
     int pair_cap = 0;
     Profile prof;
+    // per-body probe ring (ibmdem_probe_bodies): after every step the listed
+    // bodies' 18 values are appended on the device; probe_count (device) is
+    // the number of steps recorded so far
+    int probe_n = 0, probe_cap = 0, probe_list_cap = 0;
+    long probe_ring_cap = 0;
+    int *probe_list = nullptr;
+    double *probe_ring = nullptr;
+    long long *probe_count = nullptr;
     // the coupled step as two captured HIP graphs (see ctx.hip), one pair
     // per (vcur, pcur) parity of the step
     StepGraphs graphs;
@@ -606,6 +614,8 @@ int dem_markers_follow(ibmdem_ctx *c);
 int dem_body_hydro(ibmdem_ctx *c);
 int dem_pairs(ibmdem_ctx *c, int cap);
 int dem_check_flags(ibmdem_ctx *c);
+// the probe record of the step just completed (no host wait)
+int dem_probe_append(ibmdem_ctx *c);
 
 // deterministic reductions (fluid.hip)
 // sum over n values of f(i) into scal[slot]; fixed tree, independent of timing

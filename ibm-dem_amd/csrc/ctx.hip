@@ -201,7 +201,7 @@ int step_post(ibmdem_ctx *c) {
     TRY(fluid_project(c, 1.0));
     TRY(dem_body_hydro(c));
     for (int s = 0; s < c->cs.couple_interval; ++s) TRY(dem_substep_fused(c, s == c->cs.couple_interval - 1));
-    return IBMDEM_OK;
+    return dem_probe_append(c);
 }
 
 void drop_part(CapturedPart &p) {

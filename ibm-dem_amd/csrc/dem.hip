@@ -853,3 +853,93 @@ int dem_check_flags(ibmdem_ctx *c) {
 }
 
 int dem_setup(ibmdem_ctx *c) { (void)c; return IBMDEM_OK; }
+
+// ---------------------------------------------------------------------------
+// Per-body probe ring (the reference's PostProcessing probe files,
+// fluid.F90:368-404): after each step one small kernel copies the listed
+// bodies' position, velocity, angular velocity, hydrodynamic force and
+// torque and contact force into slot (count mod capacity) of a device ring
+// and advances the device-side count, so recording needs no host wait and
+// can be captured with the step.
+// ---------------------------------------------------------------------------
+constexpr int PROBE_VALUES = 18;
+
+namespace {
+__global__ __launch_bounds__(64) void k_probe_append(int n, int cap, int B, const int *__restrict__ list, BodyState s,
+                                                     double *__restrict__ ring, long long *__restrict__ count) {
+    __shared__ long long slot;
+    if (threadIdx.x == 0) slot = *count % cap;
+    __syncthreads();
+    for (int q = threadIdx.x; q < n * PROBE_VALUES; q += 64) {
+        const int i = q / PROBE_VALUES, v = q % PROBE_VALUES, b = list[i];
+        const int grp = v / 3, d = v % 3;
+        const double *src = grp == 0 ? s.x : grp == 1 ? s.v : grp == 2 ? s.om : grp == 3 ? s.fdrag
+                          : grp == 4 ? s.fmom : s.fcol;
+        ring[(slot * n + i) * PROBE_VALUES + v] = src[(long)d * B + b];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *count = *count + 1;
+}
+}  // namespace
+
+int dem_probe_append(ibmdem_ctx *c) {
+    if (c->probe_n == 0) return IBMDEM_OK;
+    hipLaunchKernelGGL(k_probe_append, dim3(1), dim3(64), 0, c->stream, c->probe_n, c->probe_cap, c->B,
+                       c->probe_list, body_state(c), c->probe_ring, c->probe_count);
+    CHECK_LAUNCH(c);
+    return IBMDEM_OK;
+}
+
+extern "C" int ibmdem_probe_bodies(ibmdem_ctx *c, int32_t n, const int32_t *bodies, int32_t capacity) {
+    if (!c || n < 0 || (n > 0 && (!bodies || capacity < 1))) return IBMDEM_ERR_ARG;
+    for (int i = 0; i < n; ++i)
+        if (bodies[i] < 0 || bodies[i] >= c->B) return set_error(c, IBMDEM_ERR_ARG, "probed body index out of range");
+    HIPX(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);   // the step's launches change
+    c->probe_n = 0;
+    if (n == 0) return IBMDEM_OK;
+    const long ring = (long)n * capacity * PROBE_VALUES;
+    if (ring > (1L << 30)) return set_error(c, IBMDEM_ERR_ARG, "probe ring larger than 8 GiB");
+    int rc;
+    if (!c->probe_count && (rc = dev_alloc(c, &c->probe_count, 1))) return rc;
+    // buffers are kept and reused by a later, no larger, probe list
+    if (n > c->probe_list_cap) {
+        if ((rc = dev_alloc(c, &c->probe_list, n))) return rc;
+        c->probe_list_cap = n;
+    }
+    if (ring > c->probe_ring_cap) {
+        if ((rc = dev_alloc(c, &c->probe_ring, (size_t)ring))) return rc;
+        c->probe_ring_cap = ring;
+    }
+    HIPX(c, hipMemcpyAsync(c->probe_list, bodies, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPX(c, hipMemsetAsync(c->probe_count, 0, sizeof(long long), c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    c->probe_n = n;
+    c->probe_cap = capacity;
+    return IBMDEM_OK;
+}
+
+extern "C" int ibmdem_probe_read(ibmdem_ctx *c, int32_t *n_steps, int64_t *first_step, double *records) {
+    if (!c || !n_steps || !first_step) return IBMDEM_ERR_ARG;
+    *n_steps = 0;
+    *first_step = 0;
+    if (c->probe_n == 0) return IBMDEM_OK;
+    long long count = 0;
+    HIPX(c, hipMemcpyAsync(&count, c->probe_count, sizeof count, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    const long long kept = std::min<long long>(count, c->probe_cap);
+    *n_steps = (int32_t)kept;
+    *first_step = count - kept;
+    if (!records || kept == 0) return IBMDEM_OK;
+    // oldest first: slots (count - kept .. count - 1) mod capacity
+    const size_t rec = (size_t)c->probe_n * PROBE_VALUES;
+    const long long s0 = (count - kept) % c->probe_cap;
+    const long long tail = std::min<long long>(kept, c->probe_cap - s0);
+    HIPX(c, hipMemcpyAsync(records, c->probe_ring + s0 * rec, tail * rec * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    if (kept > tail)
+        HIPX(c, hipMemcpyAsync(records + tail * rec, c->probe_ring, (kept - tail) * rec * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return IBMDEM_OK;
+}

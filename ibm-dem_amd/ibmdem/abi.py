@@ -112,6 +112,10 @@ _FUNCS = {
     "ibmdem_upload_outflow": [C.c_void_p, _i32, _pd],
     "ibmdem_download_force": [C.c_void_p, _pd, _pd, _pd],
     "ibmdem_download_force_noalloc": [C.c_void_p, _pd, _pd, _pd],
+    "ibmdem_probe_bodies": [C.c_void_p, _i32, _pi, _i32],
+    "ibmdem_get_stencil_tile": [C.c_void_p, _pi],
+    "ibmdem_set_stencil_tile": [C.c_void_p, _i32],
+    "ibmdem_probe_read": [C.c_void_p, C.POINTER(_i32), C.POINTER(C.c_int64), _pd],
     "ibmdem_download_poisson": [C.c_void_p, _pd, _pd],
     "ibmdem_upload_poisson_rhs": [C.c_void_p, _pd],
     "ibmdem_upload_markers": [C.c_void_p, C.POINTER(Markers)],
@@ -331,6 +335,26 @@ class Simulation:
         self.call("ibmdem_download_force", *[_p(a) for a in out])
         return out
 
+    PROBE_FIELDS = ("x", "v", "omega", "fdrag", "fmom", "fcol")
+
+    def probe_bodies(self, bodies, capacity: int) -> None:
+        """Record, after every step, the listed bodies' x, v, omega, fdrag,
+        fmom, fcol (18 values) in a device ring of `capacity` steps."""
+        b = np.ascontiguousarray(bodies, dtype=np.int32)
+        self._probe_n = b.size
+        self.call("ibmdem_probe_bodies", int(b.size), _p(b) if b.size else None, int(capacity))
+
+    def probe_read(self):
+        """(first_step, records) with records of shape (steps, bodies, 18),
+        oldest step first."""
+        n = _i32(0)
+        first = C.c_int64(0)
+        self.call("ibmdem_probe_read", C.byref(n), C.byref(first), None)
+        out = np.zeros((n.value, getattr(self, "_probe_n", 0), 18))
+        if n.value:
+            self.call("ibmdem_probe_read", C.byref(n), C.byref(first), _p(out))
+        return int(first.value), out
+
     def download_force_noalloc(self):
         """download_force through the library's fixed device scratch (no
         device allocation)."""
@@ -439,6 +463,14 @@ class Simulation:
     def set_mg_params(self, pre: int, post: int, coarse: int) -> None:
         mg = np.array([pre, post, coarse], dtype=np.int32)
         self.call("ibmdem_set_mg_params", _p(mg))
+
+    def stencil_tile(self) -> int:
+        t = np.zeros(1, dtype=np.int32)
+        self.call("ibmdem_get_stencil_tile", _p(t))
+        return int(t[0])
+
+    def set_stencil_tile(self, tile: int) -> None:
+        self.call("ibmdem_set_stencil_tile", int(tile))
 
 
 class _BorrowedSimulation(Simulation):

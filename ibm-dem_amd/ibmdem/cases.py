@@ -421,6 +421,127 @@ def config_batch(n_cases=64, precision=MIXED, seed=SEED_BASE + 5, rtol=1e-10,
 
 
 # ---------------------------------------------------------------------------
+# The reference's own case: 86 tilted cylinders (InitCylinderParcelsDEM)
+# ---------------------------------------------------------------------------
+def read_marker_template(path) -> np.ndarray:
+    """A marker template file of the reference (CYLINDER_LP*.dat: one marker
+    per line, three columns x y z, read list-directed at
+    ZJS_INIT_DEM.F90:371-373) as an (n, 3) float64 array."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if parts:
+                rows.append([float(v) for v in parts[:3]])
+    return np.asarray(rows, dtype=np.float64)
+
+
+def rotation_matrix(theta) -> np.ndarray:
+    """CalculateRotateMatrix (couplesolverCylinder.F90:76-97): Rodrigues'
+    formula for the rotation by |theta| about theta/|theta| (|theta| clamped
+    to >= 1e-10), as the 9 row-major entries, in the reference's operation
+    order."""
+    t0, t1, t2 = (float(v) for v in theta)
+    mag = math.sqrt(t0 * t0 + t1 * t1 + t2 * t2)
+    mag = mag if mag > 1e-10 else 1e-10
+    ux, uy, uz = t0 / mag, t1 / mag, t2 / mag
+    cm, sm = math.cos(mag), math.sin(mag)
+    return np.array([(1 - cm) * ux * ux + cm, (1 - cm) * ux * uy - sm * uz, (1 - cm) * ux * uz + sm * uy,
+                     (1 - cm) * uy * ux + sm * uz, (1 - cm) * uy * uy + cm, (1 - cm) * uy * uz - sm * ux,
+                     (1 - cm) * uz * ux - sm * uy, (1 - cm) * uz * uy + sm * ux, (1 - cm) * uz * uz + cm])
+
+
+def reference_cylinder_centres(n_bodies=86):
+    """Body centres of InitCylinderParcelsDEM (ZJS_INIT_DEM.F90:382-394), in
+    metres: a 9 x 2 x 9 lattice of pitch 1e-3 m visited with stride 7, the y
+    positions mapped onto the 1.2e-3 m span."""
+    delta = 3e-3 / 3
+    ntx, nty, ntz = 9, 2, 9
+    nt = ntx * nty * ntz
+    out = np.zeros((n_bodies, 3))
+    for i in range(1, n_bodies + 1):
+        idx = (7 * i - 1) % nt + 1
+        iz = (idx - 1) // (ntx * nty)
+        iy = (idx - 1 - iz * ntx * nty) // ntx
+        ix = (idx - 1) % ntx
+        x = ix * delta + delta * 0.5 + 1e-10
+        y = iy * delta + delta * 0.5 + 1e-10
+        y = y / delta * 1.2e-3 / 2 + 1e-10
+        z = iz * delta + delta * 0.5 + 1e-10
+        out[i - 1] = (x, y, z)
+    return out
+
+
+def config_reference_cylinders(template: np.ndarray, seed=SEED_BASE + 6, n_bodies=86, precision=FP64,
+                               rtol=1e-10) -> Setup:
+    """The reference's shipped case in its own SI units (main.F90:20,
+    initialization.F90:241-270,341-377, ZJS_INIT_DEM.F90:67-78,350-485): a
+    240 x 32 x 240 grid of 9e-3 x 1.2e-3 x 9e-3 m, periodic in x and z,
+    no-slip walls in y, water (nu = 1e-6, rho_f = 1e3), dt = 5e-5 s, bulk
+    forcing to a mean u of 0.1 m/s (and w = 0), gravity -9.81 m/s^2 along z,
+    knCoeff 4e4, e = 1e-4; 86 cylinders (rho = 2650) from the marker template
+    (template x, y scaled by (rad - 0.3 dx) height / rad, z by height - 0.6 dx,
+    rad = the binary32 literal 2.08e-4, height = 4 rad), each tilted about y
+    by pi/2 times a seeded uniform number (the reference draws it unseeded).
+    Marker volume pi dx / n (rad^2 + 2 rad height + dx^2 / 4); body volume
+    pi rad^2 height, inertia m (3 rad^2 + height^2) / 12 about x and y and
+    m rad^2 / 2 about the axis (ZJS_INIT_DEM.F90:422-431,477)."""
+    nx, ny, nz = 240, 32, 240
+    dx = 9e-3 / nx
+    case = build_case(nx, ny, nz, [BC_PERIODIC, BC_PERIODIC, BC_WALL, BC_WALL, BC_PERIODIC, BC_PERIODIC],
+                      h=dx, visc=1e-6, rhof=1e3, dt=1e-4 / 2, gravity=(0.0, 0.0, -9.81), kn_coeff=4e4,
+                      restitution=0.0001, bulk_switch=(1, 0, 1), bulk_target=(0.1, 0.0, 0.0),
+                      precision=precision, rtol=rtol)
+    case.kn_coeff = 1e4 * 4          # SI: the reference's own value
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rnd = rng.random(n_bodies)
+    pi = math.acos(-1.0)
+    rad = float(np.float32(2.08e-4))
+    height = rad * 4
+    radratio = (rad - 0.3 * dx) * height / rad
+    heightratio = height - 0.3 * 2 * dx
+    rho = 2.65e3
+    n = template.shape[0]
+    centres = reference_cylinder_centres(n_bodies)
+    mk = MarkerArrays.empty(n * n_bodies)
+    bd = BodyArrays.empty(n_bodies)
+    vol_m = pi * dx / n * (rad ** 2 + 2 * rad * height + 0.25 * dx ** 2)
+    for j in range(n_bodies):
+        c = centres[j]
+        R = rotation_matrix((0.0, pi / 2.0 * rnd[j], 0.0))
+        xt = np.empty((n, 3))
+        xt[:, 0] = radratio * template[:, 0] + c[0]
+        xt[:, 1] = radratio * template[:, 1] + c[1]
+        xt[:, 2] = heightratio * template[:, 2] + c[2]
+        # RotateLagrangianPoints (couplesolverCylinder.F90:60-74)
+        t = xt - c
+        xr = np.empty_like(xt)
+        xr[:, 0] = R[0] * t[:, 0] + R[1] * t[:, 1] + R[2] * t[:, 2]
+        xr[:, 1] = R[3] * t[:, 0] + R[4] * t[:, 1] + R[5] * t[:, 2]
+        xr[:, 2] = R[6] * t[:, 0] + R[7] * t[:, 1] + R[8] * t[:, 2]
+        xr = xr + c
+        o = j * n
+        mk.x[:, o:o + n] = xr.T
+        mk.vol[o:o + n] = vol_m
+        mk.mass[o:o + n] = rho * vol_m
+        mk.radeff[o:o + n] = dx * 0.75
+        mk.body[o:o + n] = j
+        vol_b = (pi * rad ** 2) * height
+        m = vol_b * rho
+        bd.x[:, j] = c
+        bd.mass[j] = m
+        bd.rho[j] = rho
+        i1 = m * (3.0 * rad ** 2 + height ** 2) / 12.0
+        bd.inertia[j] = [i1, 0, 0, 0, i1, 0, 0, 0, 0.5 * m * rad ** 2]
+        bd.rot[j] = R
+        bd.rot_glo[j] = R
+    case.n_markers = n * n_bodies
+    case.n_bodies = n_bodies
+    return Setup(f"reference-cylinders-{nx}x{ny}x{nz}-{n_bodies}", case, mk, bd, zero_fields(case),
+                 {"seed": seed, "rad": rad, "height": height, "dx": dx, "tilt": rnd})
+
+
+# ---------------------------------------------------------------------------
 # Small cases for parity tests
 # ---------------------------------------------------------------------------
 def small_periodic(n=(32, 24, 28), n_spheres=6, D=8.0, rho=2.0, seed=SEED_BASE + 11,

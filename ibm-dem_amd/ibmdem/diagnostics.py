@@ -55,6 +55,28 @@ def divergence(case: Case, fields, planes=None):
     return d
 
 
+def shedding_statistics(t, cd, cl, D: float, U: float, start_fraction: float = 0.5):
+    """Mean drag coefficient, rms lift coefficient and Strouhal number
+    St = f D / U of a force-coefficient history, over the samples from
+    start_fraction of the run on (past the start-up transient). f comes
+    from the upward zero crossings of C_L - mean(C_L), located by linear
+    interpolation between samples: (crossings - 1) periods between the
+    first and the last. St is None when fewer than two crossings fall in
+    the window."""
+    t, cd, cl = (np.asarray(a, dtype=np.float64) for a in (t, cd, cl))
+    k0 = int(len(t) * start_fraction)
+    t, cd, cl = t[k0:], cd[k0:], cl[k0:]
+    y = cl - cl.mean()
+    up = np.nonzero((y[:-1] < 0) & (y[1:] >= 0))[0]
+    times = t[up] + (t[up + 1] - t[up]) * (-y[up]) / (y[up + 1] - y[up])
+    st = None
+    if len(times) >= 2:
+        f = (len(times) - 1) / (times[-1] - times[0])
+        st = float(f * D / U)
+    return {"mean_cd": float(cd.mean()), "rms_cl": float(np.sqrt(np.mean(y * y))), "strouhal": st,
+            "periods": max(len(times) - 1, 0), "window": [float(t[0]), float(t[-1])]}
+
+
 def probe_line(step: int, bodies: BodyArrays, body: int, rhof: float) -> str:
     """One line of the reference's per-body probe file (fluid.F90:398-401)."""
     b = body

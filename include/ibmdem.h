@@ -243,6 +243,20 @@ int ibmdem_step(ibmdem_ctx *ctx, ibmdem_solve_info *info);
 /* Run n steps; info (may be NULL) receives the last solve. */
 int ibmdem_run(ibmdem_ctx *ctx, int32_t n_steps, ibmdem_solve_info *info);
 
+/* ---- per-body probes (PostProcessing, fluid.F90:368-404) -----------------
+ * After every coupled step the library appends, for each listed body, 18
+ * doubles -- position, velocity, angular velocity, assembled hydrodynamic
+ * force (rho_f sum fdrag vol) and torque, contact force, 3 each in that
+ * order -- to a device ring holding the last `capacity` steps. Recording
+ * takes no host wait (the step graphs include it); n_bodies = 0 stops it.
+ * The reference writes these probes to files every step; here the caller
+ * reads them when it wants. */
+int ibmdem_probe_bodies(ibmdem_ctx *ctx, int32_t n_bodies, const int32_t *bodies, int32_t capacity);
+/* *n_steps = steps held (at most capacity), *first_step = the 0-based index
+ * of the oldest held step since probing started; records (may be NULL)
+ * receives n_steps * n_bodies * 18 doubles, oldest step first. */
+int ibmdem_probe_read(ibmdem_ctx *ctx, int32_t *n_steps, int64_t *first_step, double *records);
+
 /* ---- integer lists for parity / diagnostics ------------------------------ */
 /* Per-marker cell as the reference's PutParcelsIntoCellsDEM computes it
  * (ZJS_UPDATE_LIST_DEM.F90:41-43): 1-based (i,j,k), 3*M int32,
@@ -262,6 +276,11 @@ int ibmdem_get_pairs(ibmdem_ctx *ctx, int32_t cap, int32_t *count,
  * context's captured step graphs. */
 int ibmdem_get_mg_params(ibmdem_ctx *ctx, int32_t *mg);
 int ibmdem_set_mg_params(ibmdem_ctx *ctx, const int32_t *mg);
+/* The predictor's schedule, case.stencil_tile (0..3; every schedule gives
+ * the same result bit for bit); setting it drops the captured step graphs.
+ * No counterpart in the reference. */
+int ibmdem_get_stencil_tile(ibmdem_ctx *ctx, int32_t *tile);
+int ibmdem_set_stencil_tile(ibmdem_ctx *ctx, int32_t tile);
 
 /* ---- batched form -----------------------------------------------------------
  * Many independent cases of uneven size advanced together on one GPU, each a
@@ -278,7 +297,9 @@ const char *ibmdem_batch_last_error(const ibmdem_batch *batch);
  * each); info (may be NULL) receives one solve record per case. */
 int ibmdem_batch_run(ibmdem_batch *batch, int32_t n_steps, ibmdem_solve_info *info);
 /* Per case, time trial_steps steps (<= 0: 3) with each candidate multigrid
- * cycle from the case's current state, keep the fastest, restore the state. */
+ * cycle from the case's current state and keep the fastest, then each
+ * predictor schedule (stencil_tile 0..3) under that cycle; the state is
+ * restored after every trial. */
 int ibmdem_batch_autotune(ibmdem_batch *batch, int32_t trial_steps);
 
 /* ---- measurement --------------------------------------------------------- */

@@ -125,6 +125,11 @@ struct ibmdem_ctx {
     mg_level lev[16];
     double *cg_p, *cg_Ap, *cg_z, *cg_x, *cg_r;
     double L[3];
+    /* per-body probe ring (ibmdem_probe_bodies) */
+    int probe_n, probe_cap;
+    int *probe_list;
+    double *probe_ring;
+    long long probe_count;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -1431,6 +1436,7 @@ int ibmdem_destroy(ibmdem_ctx *c) {
     for (int d = 0; d < 3; ++d) free(c->plane[d]);
     free(c->body); free(c->ptype); free(c->cell); free(c->img); free(c->cell_start);
     free(c->cell_items); free(c->npairs); free(c->pairs); free(c->bfixed);
+    free(c->probe_list); free(c->probe_ring);
     for (int s = 0; s < 6; ++s) free(c->send[s]);
     for (int l = 0; l < c->nlev; ++l) {
         free(c->lev[l].inv_diag); free(c->lev[l].diag); free(c->lev[l].x); free(c->lev[l].b); free(c->lev[l].r);
@@ -1682,6 +1688,49 @@ static int step_rk3(ibmdem_ctx *c, ibmdem_solve_info *info) {
     return IBMDEM_OK;
 }
 
+/* per-body probe record of the step just completed (x, v, omega, fdrag,
+ * fmom, fcol of each listed body; PostProcessing, fluid.F90:368-404) */
+static void probe_append(ibmdem_ctx *c) {
+    if (c->probe_n == 0) return;
+    const long long slot = c->probe_count % c->probe_cap;
+    const double *src[6] = {c->bx, c->bv, c->bom, c->bfdrag, c->bfmom, c->bfcol};
+    for (int i = 0; i < c->probe_n; ++i)
+        for (int v = 0; v < 18; ++v)
+            c->probe_ring[(slot * c->probe_n + i) * 18 + v] = src[v / 3][(size_t)(v % 3) * c->B + c->probe_list[i]];
+    c->probe_count++;
+}
+
+int ibmdem_probe_bodies(ibmdem_ctx *c, int32_t n, const int32_t *bodies, int32_t capacity) {
+    if (!c || n < 0 || (n > 0 && (!bodies || capacity < 1))) return IBMDEM_ERR_ARG;
+    for (int i = 0; i < n; ++i)
+        if (bodies[i] < 0 || bodies[i] >= c->B) return set_err(c, IBMDEM_ERR_ARG, "probed body index out of range");
+    free(c->probe_list); free(c->probe_ring);
+    c->probe_list = NULL; c->probe_ring = NULL;
+    c->probe_n = 0; c->probe_count = 0;
+    if (n == 0) return IBMDEM_OK;
+    c->probe_list = (int *)xcalloc((size_t)n, sizeof(int));
+    c->probe_ring = (double *)xcalloc((size_t)n * capacity * 18, sizeof(double));
+    memcpy(c->probe_list, bodies, (size_t)n * sizeof(int));
+    c->probe_n = n; c->probe_cap = capacity;
+    return IBMDEM_OK;
+}
+
+int ibmdem_probe_read(ibmdem_ctx *c, int32_t *n_steps, int64_t *first_step, double *records) {
+    if (!c || !n_steps || !first_step) return IBMDEM_ERR_ARG;
+    *n_steps = 0; *first_step = 0;
+    if (c->probe_n == 0) return IBMDEM_OK;
+    const long long kept = c->probe_count < c->probe_cap ? c->probe_count : c->probe_cap;
+    *n_steps = (int32_t)kept;
+    *first_step = c->probe_count - kept;
+    if (!records) return IBMDEM_OK;
+    const size_t rec = (size_t)c->probe_n * 18;
+    for (long long q = 0; q < kept; ++q) {
+        const long long slot = (c->probe_count - kept + q) % c->probe_cap;
+        memcpy(records + q * rec, c->probe_ring + slot * rec, rec * sizeof(double));
+    }
+    return IBMDEM_OK;
+}
+
 int ibmdem_step(ibmdem_ctx *c, ibmdem_solve_info *info) {
     if (!c) return IBMDEM_ERR_ARG;
     if (c->cs.time_scheme == IBMDEM_SCHEME_RK3) return step_rk3(c, info);
@@ -1697,7 +1746,10 @@ int ibmdem_step(ibmdem_ctx *c, ibmdem_solve_info *info) {
     rc = ibmdem_poisson_solve(c, info);
     if (rc) return rc;
     ibmdem_project(c, 1.0);
-    return ibmdem_cloud_evolution(c);
+    rc = ibmdem_cloud_evolution(c);
+    if (rc) return rc;
+    probe_append(c);
+    return IBMDEM_OK;
 }
 int ibmdem_run(ibmdem_ctx *c, int32_t n, ibmdem_solve_info *info) {
     for (int s = 0; s < n; ++s) {
@@ -1759,6 +1811,20 @@ int ibmdem_memory_bytes(const ibmdem_ctx *c, int64_t *bytes) {
  * cases one after another; there is nothing to tune on one core, so the
  * autotuner keeps every case's cycle.
  * ------------------------------------------------------------------------ */
+/* the predictor schedule is a GPU launch choice: the oracle only keeps the
+ * value */
+int ibmdem_get_stencil_tile(ibmdem_ctx *c, int32_t *tile) {
+    if (!c || !tile) return IBMDEM_ERR_ARG;
+    *tile = c->cs.stencil_tile;
+    return IBMDEM_OK;
+}
+int ibmdem_set_stencil_tile(ibmdem_ctx *c, int32_t tile) {
+    if (!c) return IBMDEM_ERR_ARG;
+    if (tile < 0 || tile > 3) return set_err(c, IBMDEM_ERR_ARG, "stencil_tile must be 0..3");
+    c->cs.stencil_tile = tile;
+    return IBMDEM_OK;
+}
+
 int ibmdem_get_mg_params(ibmdem_ctx *c, int32_t *mg) {
     if (!c || !mg) return IBMDEM_ERR_ARG;
     mg[0] = c->cs.mg_pre;

@@ -44,7 +44,7 @@ def _batched(lib, setups, n, tune=0):
                     np.testing.assert_array_equal(x, y)
         infos = b.run(n)
         return [(b[q].download_fields(), b[q].download_markers().x, b[q].download_bodies().x,
-                 infos[q].iterations, b[q].mg_params()) for q in range(len(b))]
+                 infos[q].iterations, b[q].mg_params(), b[q].stencil_tile()) for q in range(len(b))]
 
 
 def test_mg_params_round_trip_and_range(oracle_lib):
@@ -89,6 +89,7 @@ def test_batch_autotune_keeps_the_solution(gpu_lib, oracle_lib):
     for s, g in zip(setups, tuned):
         want = _single(oracle_lib, s, 3)
         assert g[4] in [(1, 1, 24), (2, 2, 24), (3, 3, 24), (2, 2, 12), (1, 2, 24)]
+        assert g[5] in (0, 1, 2, 3)
         for a, b in zip(g[0], want[0]):
             scale = max(np.abs(b).max(), 1e-300)
             assert np.abs(a - b).max() <= 1e-8 * scale

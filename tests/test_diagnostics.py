@@ -74,3 +74,66 @@ def test_divergence_matches_the_library_rhs():
         planes = {1: s.download_outflow(1)}
     d = diagnostics.divergence(st.case, f[:3], planes)
     np.testing.assert_allclose(-d / st.case.dt * st.case.rhof, rhs, rtol=1e-12, atol=1e-15)
+
+
+def _probe_run(lib, steps, cap, graphs=True):
+    import os
+    from ibmdem.abi import Simulation
+    setup = cases.small_walls()
+    old = os.environ.get("IBMDEM_NO_GRAPH")
+    if not graphs:
+        os.environ["IBMDEM_NO_GRAPH"] = "1"
+    try:
+        s = Simulation(lib, setup.case)
+    finally:
+        if not graphs:
+            if old is None:
+                del os.environ["IBMDEM_NO_GRAPH"]
+            else:
+                os.environ["IBMDEM_NO_GRAPH"] = old
+    with s:
+        cases.load(s, setup)
+        s.probe_bodies([1, 0], cap)
+        states = []
+        for _ in range(steps):
+            s.step()
+            b = s.download_bodies()
+            states.append(np.stack([np.concatenate([getattr(b, f)[:, q] for f in s.PROBE_FIELDS]) for q in (1, 0)]))
+        first, rec = s.probe_read()
+    return first, rec, np.array(states)
+
+
+def _check_probe(lib, graphs=True):
+    first, rec, states = _probe_run(lib, 6, 4, graphs)
+    assert first == 2 and rec.shape == (4, 2, 18)
+    np.testing.assert_array_equal(rec, states[2:])
+
+
+def test_probe_ring_on_the_oracle(oracle_lib):
+    """The probe ring keeps the last `capacity` steps, oldest first, each
+    record the listed bodies' state after its step."""
+    _check_probe(oracle_lib)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graphs", [True, False])
+def test_probe_ring_on_the_gpu(gpu_lib, graphs):
+    """The same on the GPU, recorded inside the captured step graphs or by
+    eager launches, with no host wait."""
+    _check_probe(gpu_lib, graphs)
+
+
+def test_shedding_statistics_of_a_known_signal():
+    """A lift signal A sin(2 pi f t) + c and a drag signal with mean 1.3:
+    St = f D / U, rms C_L = A / sqrt(2), mean C_D = 1.3, from the window past
+    the first half."""
+    D, U, f = 32.0, 1.0, 1.0 / 160.0
+    t = np.arange(0, 3000.0, 0.072)
+    cl = 0.7 * np.sin(2 * np.pi * f * t + 0.3) + 0.01
+    cd = 1.3 + 0.05 * np.sin(4 * np.pi * f * t)
+    r = diagnostics.shedding_statistics(t, cd, cl, D, U)
+    assert r["strouhal"] == pytest.approx(f * D / U, rel=1e-4)
+    assert r["rms_cl"] == pytest.approx(0.7 / np.sqrt(2), rel=2e-3)
+    assert r["mean_cd"] == pytest.approx(1.3, abs=2e-3)
+    assert r["periods"] >= 8
+    assert diagnostics.shedding_statistics(t[:100], cd[:100], cl[:100], D, U)["strouhal"] is None

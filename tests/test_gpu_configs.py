@@ -262,3 +262,31 @@ def test_c5_batch_of_64_equals_single_cases(gpu_lib):
         for a, c in zip(g[0], want[0]):
             np.testing.assert_array_equal(a, c)
         np.testing.assert_array_equal(g[1], want[1])
+
+
+# ---------------------------------------------------------------------------
+# The reference's own case: 86 tilted cylinders, 240 x 32 x 240, SI units
+# ---------------------------------------------------------------------------
+def test_reference_cylinders_against_the_oracle(gpu_lib, oracle_lib):
+    """The case the reference ships (InitCylinderParcelsDEM with
+    CYLINDER_LP1125.dat; walls in y, periodic in x and z, bulk forcing to
+    0.1 m/s), in its SI units, so every kernel takes its non-power-of-two
+    spacing path: three steps GPU against the oracle."""
+    t = np.load(os.path.join(os.path.dirname(GOLDEN), "cylinder_lp1125.npy"))
+    setup = cases.config_reference_cylinders(t)
+    out = []
+    for lib in (gpu_lib, oracle_lib):
+        with Simulation(lib, setup.case) as s:
+            cases.load(s, setup)
+            its = [s.step().iterations for _ in range(3)]
+            out.append((s.cells(), s.download_fields(), s.download_bodies(), s.download_markers(), its))
+    (cg, fg, bg, mg, ig), (cr, fr, br, mr, ir) = out
+    np.testing.assert_array_equal(cg, cr)
+    assert ig == ir
+    for a, b, name in zip(fg, fr, "uvwp"):
+        assert np.abs(b).max() > 0, name
+        assert rel(a, b) < 1e-10, name
+    for k in ("x", "v", "omega", "fdrag"):
+        assert rel(getattr(bg, k), getattr(br, k)) < 1e-10, k
+    assert rel(mg.x, mr.x) < 1e-12
+    assert rel(mg.fdrag, mr.fdrag) < 1e-10
