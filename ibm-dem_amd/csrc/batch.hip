@@ -3,21 +3,53 @@
 //
 // Each case is a full context with its own HIP stream. A batch step runs the
 // three phases of every case's step interleaved (queue the first captured
-// graph of all cases, then wait for each case's solve and queue its second
+// graph of all cases, then wait for the solves and queue each case's second
 // graph, then collect each case's error flags), so the cases' kernels overlap
 // on the GPU and the host never waits for one case while another could be
-// queued. The autotuner picks, per case, the multigrid cycle (pre/post
-// smoothing sweeps, coarsest-level sweeps) with the shortest measured step,
-// then the predictor's tile schedule (case.stencil_tile) under that cycle.
+// queued.
+//
+// The Poisson solve, most of a step's launches, runs batched: cases that
+// share the precision, the smoothing counts and the iteration cap form a
+// group whose solve is one captured graph of launches that each cover every
+// member (poisson_batch_*), on the group's own stream after the members'
+// first parts. The host reads all members' solver scalars at once, runs
+// further batched iterations while a member has not converged, and queues
+// the members' second parts. A group whose members cannot all run from
+// graphs this step (first step, fresh markers, profiling) steps them one by
+// one, as ibmdem_batch_set_fused(b, 0) does for every case.
+//
+// The autotuner picks the multigrid cycle (pre/post smoothing sweeps,
+// coarsest-level sweeps) with the shortest measured batch step, then, per
+// case, the predictor's tile schedule (case.stencil_tile).
 #include "common.hpp"
 
 #include <chrono>
+#include <map>
 #include <new>
+#include <tuple>
 #include <vector>
+
+namespace {
+
+struct BatchGroup {
+    std::vector<int> member;          // case indices, ascending
+    PoissonBatch *pb = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t exec = nullptr;    // begin + exec_k iterations
+    int exec_k = -1;
+    bool active = false;              // solves batched this step
+};
+
+}  // namespace
 
 struct ibmdem_batch {
     std::vector<ibmdem_ctx *> ctx;
     std::string err;
+    bool fused = true;
+    std::vector<BatchGroup> groups;
+    std::vector<int32_t> plan_key;   // the cases' parameters the groups were planned with
+    std::vector<hipEvent_t> ev;      // per case: end of its step's first part
+    int64_t fused_solves = 0;        // group solves run batched
 };
 
 namespace {
@@ -93,6 +125,207 @@ int restore_state(ibmdem_ctx *c, HostCopy &h) {
     for (int d = 0; d < 3; ++d)
         if (!h.plane[d].empty() && (rc = ibmdem_upload_outflow(c, 2 * d + 1, h.plane[d].data()))) return rc;
     return ibmdem_sync(c);
+}
+
+void free_groups(ibmdem_batch *b) {
+    for (BatchGroup &g : b->groups) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.stream) (void)hipStreamDestroy(g.stream);
+        poisson_batch_destroy(g.pb);
+    }
+    b->groups.clear();
+    b->plan_key.clear();
+}
+
+// a group failure is each member's failure, with the same message
+int group_error(ibmdem_batch *b, const BatchGroup &g, int rc, const std::string &msg) {
+    for (int q : g.member) set_error(b->ctx[q], rc, msg);
+    return rc;
+}
+
+// Groups of the cases whose solve can run batched, rebuilt when a case's
+// cycle, precision or iteration cap has changed since they were planned.
+int plan_groups(ibmdem_batch *b) {
+    const int n = (int)b->ctx.size();
+    std::vector<int32_t> key;
+    for (ibmdem_ctx *c : b->ctx)
+        key.insert(key.end(), {poisson_batchable(c) ? 1 : 0, c->cs.precision, c->cs.mg_pre, c->cs.mg_post,
+                               c->cs.mg_coarse_sweeps, c->cs.poisson_max_iter});
+    if (key == b->plan_key) return IBMDEM_OK;
+    free_groups(b);
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> by;
+    for (int q = 0; q < n; ++q) {
+        const ibmdem_ctx *c = b->ctx[q];
+        if (poisson_batchable(c))
+            by[std::make_tuple(c->cs.precision, c->cs.mg_pre, c->cs.mg_post, c->cs.poisson_max_iter)].push_back(q);
+    }
+    for (auto &kv : by) {
+        b->groups.emplace_back();
+        BatchGroup &g = b->groups.back();
+        g.member = kv.second;
+        std::vector<ibmdem_ctx *> cs;
+        for (int q : g.member) cs.push_back(b->ctx[q]);
+        std::string err;
+        int rc = poisson_batch_create(cs, &g.pb, err);
+        if (!rc && hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess) {
+            rc = IBMDEM_ERR_HIP;
+            err = "batch stream creation failed";
+        }
+        if (rc) {
+            b->err = err;
+            free_groups(b);
+            return rc;
+        }
+    }
+    if ((int)b->ev.size() < n) {
+        for (int q = (int)b->ev.size(); q < n; ++q) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                b->err = "event creation failed";
+                free_groups(b);
+                return IBMDEM_ERR_HIP;
+            }
+            b->ev.push_back(e);
+        }
+    }
+    b->plan_key = key;
+    return IBMDEM_OK;
+}
+
+// the group's solve as one graph: the begin part and k iterations
+int capture_group(ibmdem_batch *b, BatchGroup &g, int k) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    g.exec_k = -1;
+    hipGraph_t graph = nullptr;
+    std::string err;
+    if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return group_error(b, g, IBMDEM_ERR_HIP, "batched solve capture failed to begin");
+    int rc = poisson_batch_begin(g.pb, g.stream, err);
+    for (int it = 1; it <= k && !rc; ++it) rc = poisson_batch_iteration(g.pb, g.stream, it, err);
+    const hipError_t e = hipStreamEndCapture(g.stream, &graph);
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return group_error(b, g, rc ? rc : IBMDEM_ERR_HIP,
+                           rc ? err : std::string("batched solve capture: ") + hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        g.exec = nullptr;
+        return group_error(b, g, IBMDEM_ERR_HIP, std::string("batched solve instantiation: ") + hipGetErrorString(ei));
+    }
+    g.exec_k = k;
+    return IBMDEM_OK;
+}
+
+// Queue the group's solve after its members' first parts: k = the largest
+// iteration count a member has needed, then the scalars to the host.
+int group_solve_queue(ibmdem_batch *b, BatchGroup &g) {
+    int k = 0;
+    for (int q : g.member) {
+        if (hipStreamWaitEvent(g.stream, b->ev[q], 0) != hipSuccess)
+            return group_error(b, g, IBMDEM_ERR_HIP, "batched solve: stream wait failed");
+        k = std::max(k, b->ctx[q]->graphs.cg_k);
+    }
+    int rc;
+    if (!g.exec || g.exec_k != k)
+        if ((rc = capture_group(b, g, k))) return rc;
+    if (hipGraphLaunch(g.exec, g.stream) != hipSuccess)
+        return group_error(b, g, IBMDEM_ERR_HIP, "batched solve graph launch failed");
+    std::string err;
+    if ((rc = poisson_batch_gather(g.pb, g.stream, err))) return group_error(b, g, rc, err);
+    b->fused_solves++;
+    return IBMDEM_OK;
+}
+
+// Wait for the group's solve; while a member has not converged, one more
+// batched iteration (a member that has converged skips it), up to the cap.
+int group_solve_finish(ibmdem_batch *b, BatchGroup &g) {
+    std::string err;
+    int it = g.exec_k;
+    const int maxit = b->ctx[g.member[0]]->cs.poisson_max_iter;
+    while (true) {
+        if (hipStreamSynchronize(g.stream) != hipSuccess)
+            return group_error(b, g, IBMDEM_ERR_HIP, "batched solve: stream synchronization failed");
+        bool all = true;
+        for (size_t i = 0; i < g.member.size(); ++i) all = all && poisson_batch_scalars(g.pb, (int)i)[SC_CONV] != 0.0;
+        if (all || it >= maxit) return IBMDEM_OK;
+        ++it;
+        int rc = poisson_batch_iteration(g.pb, g.stream, it, err);
+        if (!rc) rc = poisson_batch_gather(g.pb, g.stream, err);
+        if (rc) return group_error(b, g, rc, err);
+    }
+}
+
+// `n_steps` batch steps. A case that fails stops there (its context keeps
+// its own message, as a single context's failed step does); every other case
+// still completes the step in progress, so no context is left between its
+// two step graphs. Returns the first failure (lowest case index).
+int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
+    const int n = (int)b->ctx.size();
+    int rc;
+    if (b->fused) {
+        if ((rc = plan_groups(b))) return rc;
+    } else {
+        free_groups(b);
+    }
+    std::vector<char> done(n), failed(n, 0), batched(n, 0);
+    int first_rc = IBMDEM_OK, first_q = -1;
+    auto fail = [&](int q, int r) {
+        failed[q] = 1;
+        if (first_q < 0 || q < first_q) { first_q = q; first_rc = r; }
+    };
+    for (int s = 0; s < n_steps && first_q < 0; ++s) {
+        std::fill(batched.begin(), batched.end(), 0);
+        for (BatchGroup &g : b->groups) {
+            g.active = true;
+            for (int q : g.member) g.active = g.active && step_batchable(b->ctx[q]);
+            if (g.active)
+                for (int q : g.member) batched[q] = 1;
+        }
+        for (int q = 0; q < n; ++q) {
+            ibmdem_ctx *c = b->ctx[q];
+            bool d = false;
+            if (batched[q]) {
+                rc = step_phase1_batched(c);
+                if (!rc && hipEventRecord(b->ev[q], c->stream) != hipSuccess)
+                    rc = set_error(c, IBMDEM_ERR_HIP, "event record failed");
+            } else {
+                rc = step_phase1(c, &inf[q], &d);
+            }
+            done[q] = d;
+            if (rc) fail(q, rc);
+        }
+        for (BatchGroup &g : b->groups) {
+            if (!g.active) continue;
+            if ((rc = group_solve_queue(b, g))) {
+                for (int q : g.member) if (!failed[q]) fail(q, rc);
+                g.active = false;
+            }
+        }
+        for (BatchGroup &g : b->groups) {
+            if (!g.active) continue;
+            if ((rc = group_solve_finish(b, g))) {
+                for (int q : g.member) if (!failed[q]) fail(q, rc);
+                continue;
+            }
+            for (size_t i = 0; i < g.member.size(); ++i) {
+                const int q = g.member[i];
+                if (failed[q]) continue;
+                if ((rc = step_phase2_batched(b->ctx[q], poisson_batch_scalars(g.pb, (int)i), &inf[q]))) fail(q, rc);
+            }
+        }
+        for (int q = 0; q < n; ++q) {
+            if (batched[q] || done[q] || failed[q]) continue;
+            if ((rc = step_phase2(b->ctx[q], &inf[q]))) fail(q, rc);
+        }
+        for (int q = 0; q < n; ++q) {
+            if (done[q] || failed[q]) continue;
+            if ((rc = step_phase3(b->ctx[q]))) fail(q, rc);
+        }
+    }
+    return first_q < 0 ? IBMDEM_OK : batch_error(b, first_q, first_rc);
 }
 
 // seconds per step of `steps` graph-replayed steps after one warm-up step
@@ -171,6 +404,10 @@ int ibmdem_batch_create(int32_t n, const ibmdem_case *cases, ibmdem_batch **out)
 
 int ibmdem_batch_destroy(ibmdem_batch *b) {
     if (!b) return IBMDEM_OK;
+    for (BatchGroup &g : b->groups)
+        if (g.stream) (void)hipStreamSynchronize(g.stream);
+    free_groups(b);
+    for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
     for (ibmdem_ctx *c : b->ctx) ibmdem_destroy(c);
     delete b;
     return IBMDEM_OK;
@@ -184,67 +421,76 @@ int ibmdem_batch_context(ibmdem_batch *b, int32_t q, ibmdem_ctx **ctx) {
 
 const char *ibmdem_batch_last_error(const ibmdem_batch *b) { return b ? b->err.c_str() : "no batch"; }
 
-// A case that fails stops there (its context keeps its own message, as a
-// single context's failed step does); every other case still completes the
-// step in progress, so no context is left between its two step graphs. The
-// run then stops and returns the first failure (lowest case index).
 int ibmdem_batch_run(ibmdem_batch *b, int32_t n_steps, ibmdem_solve_info *info) {
     if (!b || n_steps < 0) return IBMDEM_ERR_ARG;
-    const int n = (int)b->ctx.size();
-    std::vector<char> done(n), failed(n, 0);
-    std::vector<ibmdem_solve_info> tmp(n);
-    ibmdem_solve_info *inf = info ? info : tmp.data();
-    int first_rc = IBMDEM_OK, first_q = -1;
-    auto fail = [&](int q, int rc) {
-        failed[q] = 1;
-        if (first_q < 0 || q < first_q) { first_q = q; first_rc = rc; }
-    };
-    for (int s = 0; s < n_steps && first_q < 0; ++s) {
-        for (int q = 0; q < n; ++q) {
-            bool d = false;
-            const int rc = step_phase1(b->ctx[q], &inf[q], &d);
-            done[q] = d;
-            if (rc) fail(q, rc);
-        }
-        for (int q = 0; q < n; ++q) {
-            if (done[q] || failed[q]) continue;
-            const int rc = step_phase2(b->ctx[q], &inf[q]);
-            if (rc) fail(q, rc);
-        }
-        for (int q = 0; q < n; ++q) {
-            if (done[q] || failed[q]) continue;
-            const int rc = step_phase3(b->ctx[q]);
-            if (rc) fail(q, rc);
-        }
-    }
-    return first_q < 0 ? IBMDEM_OK : batch_error(b, first_q, first_rc);
+    std::vector<ibmdem_solve_info> tmp(b->ctx.size());
+    return run_steps(b, n_steps, info ? info : tmp.data());
+}
+
+int ibmdem_batch_set_fused(ibmdem_batch *b, int32_t on) {
+    if (!b) return IBMDEM_ERR_ARG;
+    for (BatchGroup &g : b->groups)
+        if (g.stream) (void)hipStreamSynchronize(g.stream);
+    b->fused = on != 0;
+    if (!b->fused) free_groups(b);
+    return IBMDEM_OK;
+}
+
+int ibmdem_batch_fused_solves(const ibmdem_batch *b, int64_t *count) {
+    if (!b || !count) return IBMDEM_ERR_ARG;
+    *count = b->fused_solves;
+    return IBMDEM_OK;
 }
 
 int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {
     if (!b) return IBMDEM_ERR_ARG;
     const int steps = trial_steps > 0 ? trial_steps : 3;
+    const int n = (int)b->ctx.size();
     // candidate cycles: pre, post, coarsest-level sweeps
     static const int32_t cand[][3] = {{1, 1, 24}, {2, 2, 24}, {3, 3, 24}, {2, 2, 12}, {1, 2, 24}};
-    for (int q = 0; q < (int)b->ctx.size(); ++q) {
-        ibmdem_ctx *c = b->ctx[q];
-        HostCopy h;
-        int rc = save_state(c, h);
-        if (rc) return batch_error(b, q, rc);
-        int32_t orig[3];
-        ibmdem_get_mg_params(c, orig);
-        double best = 1e300;
-        int best_i = -1;
-        for (int i = 0; i < (int)(sizeof cand / sizeof cand[0]); ++i) {
-            if ((rc = ibmdem_set_mg_params(c, cand[i]))) return batch_error(b, q, rc);
-            double sec = 0.0;
-            rc = time_steps(c, steps, &sec);
-            // a cycle that fails (e.g. no convergence within the cap) is not a candidate
-            if (rc == IBMDEM_OK && sec < best) { best = sec; best_i = i; }
-            if ((rc = restore_state(c, h))) return batch_error(b, q, rc);
+    const int ncand = (int)(sizeof cand / sizeof cand[0]);
+    std::vector<HostCopy> h(n);
+    std::vector<int32_t> orig(3 * n);
+    int rc;
+    for (int q = 0; q < n; ++q) {
+        if ((rc = save_state(b->ctx[q], h[q]))) return batch_error(b, q, rc);
+        ibmdem_get_mg_params(b->ctx[q], &orig[3 * q]);
+    }
+    auto restore_all = [&]() {
+        for (int q = 0; q < n; ++q) {
+            const int r = restore_state(b->ctx[q], h[q]);
+            if (r) return batch_error(b, q, r);
         }
-        if ((rc = ibmdem_set_mg_params(c, best_i >= 0 ? cand[best_i] : orig))) return batch_error(b, q, rc);
-        if ((rc = restore_state(c, h))) return batch_error(b, q, rc);
-        // then the predictor schedule (every schedule gives the same bits)
+        return (int)IBMDEM_OK;
+    };
+    // The cycle, for the whole batch (its solve runs batched): two steps
+    // under the candidate (the first learns the iteration counts, the
+    // second captures the graphs), then `steps` timed batch steps. A cycle
+    // under which a case fails (e.g. no convergence within the cap) is not a
+    // candidate.
+    std::vector<ibmdem_solve_info> info(n);
+    double best = 1e300;
+    int best_i = -1;
+    for (int i = 0; i < ncand; ++i) {
+        for (int q = 0; q < n; ++q)
+            if ((rc = ibmdem_set_mg_params(b->ctx[q], cand[i]))) return batch_error(b, q, rc);
+        double sec = 0.0;
+        rc = run_steps(b, 2, info.data());
+        if (rc == IBMDEM_OK) {
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = run_steps(b, steps, info.data());
+            sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / steps;
+        }
+        if (rc == IBMDEM_OK && sec < best) { best = sec; best_i = i; }
+        if ((rc = restore_all())) return rc;
+    }
+    for (int q = 0; q < n; ++q)
+        if ((rc = ibmdem_set_mg_params(b->ctx[q], best_i >= 0 ? cand[best_i] : &orig[3 * q])))
+            return batch_error(b, q, rc);
+    if ((rc = restore_all())) return rc;
+    // then, per case, the predictor schedule (every schedule gives the same bits)
+    for (int q = 0; q < n; ++q) {
+        ibmdem_ctx *c = b->ctx[q];
         const int tile0 = c->cs.stencil_tile;
         double best_t = 1e300;
         int best_tile = tile0;
@@ -253,10 +499,11 @@ int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {
             double sec = 0.0;
             rc = time_steps(c, steps, &sec);
             if (rc == IBMDEM_OK && sec < best_t) { best_t = sec; best_tile = t; }
-            if ((rc = restore_state(c, h))) return batch_error(b, q, rc);
+            if ((rc = restore_state(c, h[q]))) return batch_error(b, q, rc);
         }
         if ((rc = ibmdem_set_stencil_tile(c, best_tile))) return batch_error(b, q, rc);
     }
+    b->err.clear();
     return IBMDEM_OK;
 }
 

@@ -415,6 +415,7 @@ struct StepGraphs {
     bool enabled = true;         // IBMDEM_NO_GRAPH=1 in the environment turns them off
     int cg_k = -1;               // CG iterations to capture (-1: not yet known)
     CapturedPart pre[4], post[4];
+    CapturedPart pre_rhs[4];     // pre without the solve's preconditioning and iterations (batched solve)
     int64_t launches = 0;        // graph launches (for tests)
 };
 
@@ -573,6 +574,12 @@ int step_phase1(ibmdem_ctx *c, ibmdem_solve_info *info, bool *done);
 int step_phase2(ibmdem_ctx *c, ibmdem_solve_info *info);
 int step_phase3(ibmdem_ctx *c);
 void drop_graphs(ibmdem_ctx *c);
+// a batch step whose solve runs batched: phase 1 queues the step up to the
+// solve's right-hand side, phase 2 takes the solver scalars of the batched
+// solve and queues the rest of the step (phase 3 as above)
+bool step_batchable(const ibmdem_ctx *c);
+int step_phase1_batched(ibmdem_ctx *c);
+int step_phase2_batched(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info);
 // dem.hip: exclusive scan of n counts into start[0..n] (start[n] = total)
 int scan_counts(ibmdem_ctx *c, const int *cnt, int *start, long n);
 // fluid.hip
@@ -588,10 +595,29 @@ int poisson_setup(ibmdem_ctx *c);
 long poisson_partials(int nx, int ny, int nz);
 int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info);
 int poisson_begin(ibmdem_ctx *c);
+int poisson_begin_rhs(ibmdem_ctx *c);
+int poisson_begin_precondition(ibmdem_ctx *c);
 int poisson_iterations(ibmdem_ctx *c, int first, int last);
 int poisson_iterate_to_convergence(ibmdem_ctx *c, int first);
 int poisson_check(ibmdem_ctx *c, ibmdem_solve_info *info);
+int poisson_check_scalars(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info);
 int poisson_gauge(ibmdem_ctx *c);
+
+// The batched solve of a group of contexts (same precision, smoothing counts
+// and iteration cap): one launch per solver operation over all of them.
+struct PoissonBatch;
+bool poisson_batchable(const ibmdem_ctx *c);
+int poisson_batch_create(const std::vector<ibmdem_ctx *> &cs, PoissonBatch **out, std::string &err);
+void poisson_batch_destroy(PoissonBatch *pb);
+// z = M^-1 r, r.z, p = z of every case (after each case's poisson_begin_rhs)
+int poisson_batch_begin(PoissonBatch *pb, hipStream_t s, std::string &err);
+// CG iteration it (1-based) of every case
+int poisson_batch_iteration(PoissonBatch *pb, hipStream_t s, int it, std::string &err);
+// every case's solver scalars to the host (read after s has drained)
+int poisson_batch_gather(PoissonBatch *pb, hipStream_t s, std::string &err);
+const double *poisson_batch_scalars(const PoissonBatch *pb, int q);
+// launches of the begin part and k iterations
+int poisson_batch_launches(const PoissonBatch *pb, int k);
 // ibm.hip
 int ibm_begin_step(ibmdem_ctx *c);
 int ibm_interpolate(ibmdem_ctx *c);

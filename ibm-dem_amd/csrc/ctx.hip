@@ -183,7 +183,8 @@ void set_host_state(ibmdem_ctx *c, const HostState &h) {
     c->spread_valid = h.spread_valid; c->rmax_dirty = h.rmax_dirty;
 }
 
-int step_pre(ibmdem_ctx *c) {
+// the step up to the solve's right-hand side (r = b, ||b||)
+int step_pre_rhs(ibmdem_ctx *c) {
     TRY(fluid_save_uold(c));
     TRY(ibm_begin_step(c));
     for (int it = 0; it < c->cs.n_mdf; ++it) {
@@ -193,7 +194,12 @@ int step_pre(ibmdem_ctx *c) {
     TRY(fluid_predictor(c, 1.0, 0.0, 1.0));
     TRY(fluid_bulk(c, false));
     TRY(fluid_divergence(c, 1.0));
-    return poisson_begin(c);
+    return poisson_begin_rhs(c);
+}
+
+int step_pre(ibmdem_ctx *c) {
+    TRY(step_pre_rhs(c));
+    return poisson_begin_precondition(c);
 }
 
 int step_post(ibmdem_ctx *c) {
@@ -313,10 +319,35 @@ int step_phase2(ibmdem_ctx *c, ibmdem_solve_info *info) {
 
 int step_phase3(ibmdem_ctx *c) { return dem_check_flags(c); }
 
+// The batched form of phases 1 and 2: the solve's preconditioning and
+// iterations are left to the batch (poisson_batch_*), which runs them for a
+// group of contexts at once; the launches of each context are those of its
+// own step, so the results are bitwise the same.
+bool step_batchable(const ibmdem_ctx *c) {
+    return c->cs.time_scheme != IBMDEM_SCHEME_RK3 && c->graphs.enabled && c->graphs.cg_k >= 0 &&
+           c->prof.kernel == 0 && !c->rmax_dirty && poisson_batchable(c);
+}
+
+int step_phase1_batched(ibmdem_ctx *c) {
+    CapturedPart &part = c->graphs.pre_rhs[c->vcur * 2 + c->pcur];
+    if (!part.exec || !(part.before == host_state(c))) TRY(capture_part(c, part, [&] { return step_pre_rhs(c); }));
+    return run_part(c, part);
+}
+
+int step_phase2_batched(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info) {
+    StepGraphs &G = c->graphs;
+    TRY(poisson_check_scalars(c, scal, info));
+    if (c->last_cg_iterations > G.cg_k) G.cg_k = c->last_cg_iterations;
+    CapturedPart &post = G.post[c->vcur * 2 + c->pcur];
+    if (!post.exec || !(post.before == host_state(c))) TRY(capture_part(c, post, [&] { return step_post(c); }));
+    return run_part(c, post);
+}
+
 void drop_graphs(ibmdem_ctx *c) {
     for (int q = 0; q < 4; ++q) {
         drop_part(c->graphs.pre[q]);
         drop_part(c->graphs.post[q]);
+        drop_part(c->graphs.pre_rhs[q]);
     }
 }
 

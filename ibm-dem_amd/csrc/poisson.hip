@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <type_traits>
 
 namespace {
@@ -44,6 +45,21 @@ constexpr int COARSE_T = 1024;
 struct LaunchShape {
     dim3 grid, block;
 };
+
+// Block coordinates a kernel body works with: the hardware's in a launch for
+// one case (hw_blk), or the case's own in a launch over a batch of cases,
+// where each case's blocks are a range of one flattened grid and its 2-D
+// block shape is laid over the launch's 1-D blocks (see k_batched below).
+struct Blk {
+    int bx, by, bz;   // block index
+    int gx, gy, gz;   // grid dimensions
+    int tx, ty;       // thread index in the block
+    int dx, dy;       // block dimensions
+};
+__device__ __forceinline__ Blk hw_blk() {
+    return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z,
+               (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.x, (int)blockDim.y};
+}
 
 // Barrier of one wavefront over LDS: its LDS operations complete in program
 // order, so a compiler-level fence suffices.
@@ -60,9 +76,13 @@ int pow2ceil(int v) {
 LaunchShape shape_for(int nx, int ny, int nz);
 // grid of a red-black half sweep (RB_ROWS rows per thread) on an nx*ny*nz level
 dim3 half_grid(int nx, int ny, int nz);
+// Blocks always hold RB threads (bx * by = RB, rows past the level masked),
+// so a launch over a batch of cases can run every case's blocks with one
+// block size and the same block decomposition as the case's own launch.
 LaunchShape shape_for(int nx, int ny, int nz) {
+    (void)ny;
     int bx = std::min(64, pow2ceil(nx));
-    int by = std::min(RB / bx, pow2ceil(ny));
+    int by = RB / bx;
     return {dim3(div_up(nx, bx), div_up(ny, by), nz), dim3(bx, by, 1)};
 }
 
@@ -212,15 +232,15 @@ __device__ __forceinline__ T nbsum_pair_vals(const PairGeo &g, int h, T part, T 
 constexpr int RB_ROWS = 2;
 static_assert(RB_ROWS == 2, "the half sweep shares loads between its two adjacent rows");
 template <typename T, typename TB, bool DOT = false, int RES = 0, bool ZI = false>
-__global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
+__device__ __forceinline__ void rb_half_body(const Blk &kb, const LevelDev &L, T *__restrict__ x, const TB *__restrict__ b,
                                                 int color, const double *__restrict__ stop,
                                                 double *__restrict__ partial, long pofs, T *__restrict__ r,
-                                                const double *__restrict__ r64 = nullptr,
-                                                double *__restrict__ z64 = nullptr) {
+                                                const double *__restrict__ r64,
+                                                double *__restrict__ z64) {
     if (*stop != 0.0) return;
     const T id_int = T(L.idiag[DIAG_INTERIOR]), dg_int = T(L.diag[DIAG_INTERIOR]);
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.z;
+    const int ip = kb.bx * kb.dx + kb.tx;
+    const int k = kb.bz;
     double dsum = 0.0;
     if (L.split) {
         // Two phases over the thread's rows: every load (b, the other
@@ -238,7 +258,7 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
         // the thread's two rows are adjacent (j0, j0 + 1); rows and pairs
         // past the level's end load from the last one (valid addresses, no
         // branch) and are dropped in phase two
-        const int j0 = (blockIdx.y * blockDim.y + threadIdx.y) * RB_ROWS;
+        const int j0 = (kb.by * kb.dy + kb.ty) * RB_ROWS;
         const int ipc = min(ip, (L.nx >> 1) - 1);
         PairGeo g[RB_ROWS];
         int h[RB_ROWS];
@@ -313,7 +333,7 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
     } else {
 #pragma unroll
         for (int rr = 0; rr < RB_ROWS; ++rr) {
-            const int j = (blockIdx.y * blockDim.y + threadIdx.y) * RB_ROWS + rr;
+            const int j = (kb.by * kb.dy + kb.ty) * RB_ROWS + rr;
             if (j >= L.ny) continue;
             const int i = 2 * ip + ((j + k + color) & 1);
             const int o = lidx(L, i, j, k);
@@ -347,15 +367,23 @@ __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, c
     }
     if (DOT) {
         __shared__ double red[RB];
-        const int nt = blockDim.x * blockDim.y, t = threadIdx.y * blockDim.x + threadIdx.x;
+        const int nt = kb.dx * kb.dy, t = kb.ty * kb.dx + kb.tx;
         red[t] = dsum;
         __syncthreads();
         for (int st = nt / 2; st > 0; st >>= 1) {
             if (t < st) red[t] += red[t + st];
             __syncthreads();
         }
-        if (t == 0) partial[pofs + ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+        if (t == 0) partial[pofs + ((long)kb.bz * kb.gy + kb.by) * kb.gx + kb.bx] = red[0];
     }
+}
+template <typename T, typename TB, bool DOT = false, int RES = 0, bool ZI = false>
+__global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
+                                                int color, const double *__restrict__ stop,
+                                                double *__restrict__ partial, long pofs, T *__restrict__ r,
+                                                const double *__restrict__ r64 = nullptr,
+                                                double *__restrict__ z64 = nullptr) {
+    rb_half_body<T, TB, DOT, RES, ZI>(hw_blk(), L, x, b, color, stop, partial, pofs, r, r64, z64);
 }
 
 dim3 half_grid(int nx, int ny, int nz) {
@@ -384,12 +412,12 @@ constexpr int RS_CX = 32, RS_CY = 8;
 constexpr int RS_RX = 2 * RS_CX + 2, RS_RY = 2 * RS_CY + 2, RS_TILE = RS_RX * RS_RY * 4;
 static_assert(RS_CX * RS_CY == RB, "one coarse cell per thread");
 template <typename T>
-__global__ __launch_bounds__(RB) void k_restrict(LevelDev F, const T *__restrict__ rf, LevelDev Cl,
+__device__ __forceinline__ void restrict_body(const Blk &kb, const LevelDev &F, const T *__restrict__ rf, const LevelDev &Cl,
                                                  T *__restrict__ bc, const double *__restrict__ stop) {
     if (*stop != 0.0) return;
     __shared__ T tile[RS_TILE];
-    const int t = threadIdx.x;
-    const int I0 = blockIdx.x * RS_CX, J0 = blockIdx.y * RS_CY, K = blockIdx.z;
+    const int t = kb.tx;
+    const int I0 = kb.bx * RS_CX, J0 = kb.by * RS_CY, K = kb.bz;
     // The tile's rows r = z RS_RY + y: wavefront w loads columns 0..63 of
     // rows w, w + RB/64, ...; a row's (j, k) and its part of the storage
     // index are the same on every lane (scalars), the lane's part (its
@@ -437,6 +465,11 @@ __global__ __launch_bounds__(RB) void k_restrict(LevelDev F, const T *__restrict
             }
     bc[lidx(Cl, I, J, K)] = T(0.125) * s;
 }
+template <typename T>
+__global__ __launch_bounds__(RB) void k_restrict(LevelDev F, const T *__restrict__ rf, LevelDev Cl,
+                                                 T *__restrict__ bc, const double *__restrict__ stop) {
+    restrict_body<T>(hw_blk(), F, rf, Cl, bc, stop);
+}
 
 __device__ __forceinline__ void parents(int f, int nc, int lo, int hi, int &p0, int &p1) {
     int cc = f / 2;
@@ -454,32 +487,32 @@ __device__ __forceinline__ void parents(int f, int nc, int lo, int hi, int &p0, 
 // same order.
 constexpr int PRO_TILE = 1024;   // (bx + 2) * (by / 2 + 2) * 2 <= 528 for the shapes of shape_for
 template <typename T>
-__global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, LevelDev Cl,
+__device__ __forceinline__ void prolong_body(const Blk &kb, const LevelDev &F, T *__restrict__ xf, const LevelDev &Cl,
                                                 const T *__restrict__ xc, const double *__restrict__ stop) {
     if (*stop != 0.0) return;
     __shared__ T tile[PRO_TILE];
-    const int bx = blockDim.x, by = blockDim.y;
+    const int bx = kb.dx, by = kb.dy;
     const int TX = bx + 2, TY = by / 2 + 2;
-    const int I0 = blockIdx.x * bx - 1, J0 = (blockIdx.y * by) / 2 - 1;
-    const int k = blockIdx.z;
+    const int I0 = kb.bx * bx - 1, J0 = (kb.by * by) / 2 - 1;
+    const int k = kb.bz;
     int pk[2];
     parents(k, Cl.nz, Cl.bt[4], Cl.bt[5], pk[0], pk[1]);
-    const int t = threadIdx.y * bx + threadIdx.x;
+    const int t = kb.ty * bx + kb.tx;
     // the thread's two fine values are loaded first, so their latency
     // overlaps the staging of the coarse tile
-    const int ip = blockIdx.x * bx + threadIdx.x;
-    const int j = blockIdx.y * by + threadIdx.y;
+    const int ip = kb.bx * bx + kb.tx;
+    const int j = kb.by * by + kb.ty;
     T xo[2] = {T(0), T(0)};
 #pragma unroll
     for (int h = 0; h < 2; ++h)
         if (j < F.ny && 2 * ip + h < F.nx) xo[h] = xf[lidx(F, 2 * ip + h, j, k)];
     if (bx == 64) {
-        // one wavefront per threadIdx.y: wavefront w stages columns 0..63 of
+        // one wavefront per kb.ty: wavefront w stages columns 0..63 of
         // tile rows w, w + by, ... (row r = c TY + y); a row's (J, plane) and
         // its part of the storage index are scalars, the lane's column index
         // is computed once; columns 64 and 65 are one more load for the first
         // threads. All loads are issued before the LDS stores wait.
-        const int lane = threadIdx.x, w = threadIdx.y, NROW = 2 * TY;
+        const int lane = kb.tx, w = kb.ty, NROW = 2 * TY;
         const int half = Cl.split ? Cl.half : 0, rl = Cl.split ? Cl.nx >> 1 : Cl.nx;
         const int Il = tile_coord(I0 + lane, Cl.nx, Cl.bt[0], Cl.bt[1]);
         const int lo0 = Cl.split ? (Il & 1) * half + (Il >> 1) : Il;
@@ -547,6 +580,11 @@ __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, 
         xf[lidx(F, i, j, k)] = xo[h] + s;
     }
 }
+template <typename T>
+__global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, LevelDev Cl,
+                                                const T *__restrict__ xc, const double *__restrict__ stop) {
+    prolong_body<T>(hw_blk(), F, xf, Cl, xc, stop);
+}
 
 // coarsest level, one work-group. Small levels (<= COARSE_LDS cells, every
 // level the grids here coarsen to) live in LDS with a precomputed neighbour
@@ -555,7 +593,7 @@ __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, 
 // coarsest levels run the same sweeps in global memory.
 constexpr int COARSE_LDS = 512;
 template <typename T>
-__global__ __launch_bounds__(64) void k_coarse_lds(LevelDev L, T *__restrict__ xg, const T *__restrict__ bg,
+__device__ __forceinline__ void coarse_lds_body(const Blk &kb, const LevelDev &L, T *__restrict__ xg, const T *__restrict__ bg,
                                                    int sweeps, const double *__restrict__ stop) {
     if (*stop != 0.0) return;
     __shared__ T x[COARSE_LDS + 1], b[COARSE_LDS], id[COARSE_LDS], t[COARSE_LDS];
@@ -563,7 +601,7 @@ __global__ __launch_bounds__(64) void k_coarse_lds(LevelDev L, T *__restrict__ x
     __shared__ unsigned char col[COARSE_LDS];
     const int n = L.nx * L.ny * L.nz;
     const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
-    for (int q = threadIdx.x; q < n; q += 64) {
+    for (int q = kb.tx; q < n; q += 64) {
         const int i = q % L.nx, j = (q / L.nx) % L.ny, k = q / (L.nx * L.ny);
         const int a[3] = {i, j, k}, m[3] = {L.nx, L.ny, L.nz};
         for (int d = 0; d < 3; ++d)
@@ -582,7 +620,7 @@ __global__ __launch_bounds__(64) void k_coarse_lds(LevelDev L, T *__restrict__ x
         b[q] = bg[lidx(L, i, j, k)];
         x[q] = T(0);
     }
-    if (threadIdx.x == 0) x[n] = T(0);
+    if (kb.tx == 0) x[n] = T(0);
     __syncthreads();
     auto upd = [&](int q) -> T {
         const T sx = (x[nb[0][q]] + x[nb[1][q]]) * ihx2;
@@ -591,7 +629,7 @@ __global__ __launch_bounds__(64) void k_coarse_lds(LevelDev L, T *__restrict__ x
         return (b[q] + (sx + sy + sz)) * id[q];
     };
     auto half = [&](int color) {
-        for (int q = threadIdx.x; q < n; q += 64)
+        for (int q = kb.tx; q < n; q += 64)
             if (col[q] == color) x[q] = upd(q);
         __syncthreads();
     };
@@ -601,24 +639,29 @@ __global__ __launch_bounds__(64) void k_coarse_lds(LevelDev L, T *__restrict__ x
     } else {
         const T om = T(0.8);
         for (int s = 0; s < 2 * sweeps; ++s) {
-            for (int q = threadIdx.x; q < n; q += 64) t[q] = x[q] + om * (upd(q) - x[q]);
+            for (int q = kb.tx; q < n; q += 64) t[q] = x[q] + om * (upd(q) - x[q]);
             __syncthreads();
-            for (int q = threadIdx.x; q < n; q += 64) x[q] = t[q];
+            for (int q = kb.tx; q < n; q += 64) x[q] = t[q];
             __syncthreads();
         }
     }
-    for (int q = threadIdx.x; q < n; q += 64) xg[lidx(L, q % L.nx, (q / L.nx) % L.ny, q / (L.nx * L.ny))] = x[q];
+    for (int q = kb.tx; q < n; q += 64) xg[lidx(L, q % L.nx, (q / L.nx) % L.ny, q / (L.nx * L.ny))] = x[q];
+}
+template <typename T>
+__global__ __launch_bounds__(64) void k_coarse_lds(LevelDev L, T *__restrict__ xg, const T *__restrict__ bg,
+                                                   int sweeps, const double *__restrict__ stop) {
+    coarse_lds_body<T>(hw_blk(), L, xg, bg, sweeps, stop);
 }
 
 template <typename T>
-__global__ __launch_bounds__(COARSE_T) void k_coarse(LevelDev L, T *__restrict__ x, const T *__restrict__ b,
+__device__ __forceinline__ void coarse_body(const Blk &kb, const LevelDev &L, T *__restrict__ x, const T *__restrict__ b,
                                                      T *__restrict__ tmp, int sweeps, const double *__restrict__ stop) {
     if (*stop != 0.0) return;
     const long n = (long)L.nx * L.ny * L.nz;
-    for (long q = threadIdx.x; q < n; q += COARSE_T) x[q] = T(0);
+    for (long q = kb.tx; q < n; q += COARSE_T) x[q] = T(0);
     __syncthreads();
     auto half = [&](int color) {
-        for (long q = threadIdx.x; q < n; q += COARSE_T) {
+        for (long q = kb.tx; q < n; q += COARSE_T) {
             int i, j, k;
             lcell(L, (int)q, i, j, k);
             if (((i + j + k) & 1) != color) continue;
@@ -632,17 +675,22 @@ __global__ __launch_bounds__(COARSE_T) void k_coarse(LevelDev L, T *__restrict__
     } else {
         const T om = T(0.8);
         for (int s = 0; s < 2 * sweeps; ++s) {
-            for (long q = threadIdx.x; q < n; q += COARSE_T) {
+            for (long q = kb.tx; q < n; q += COARSE_T) {
                 int i, j, k;
                 lcell(L, (int)q, i, j, k);
                 T xn = (b[q] + nbsum(x, L, i, j, k)) * T(L.idiag[diag_index(L, i, j, k)]);
                 tmp[q] = x[q] + om * (xn - x[q]);
             }
             __syncthreads();
-            for (long q = threadIdx.x; q < n; q += COARSE_T) x[q] = tmp[q];
+            for (long q = kb.tx; q < n; q += COARSE_T) x[q] = tmp[q];
             __syncthreads();
         }
     }
+}
+template <typename T>
+__global__ __launch_bounds__(COARSE_T) void k_coarse(LevelDev L, T *__restrict__ x, const T *__restrict__ b,
+                                                     T *__restrict__ tmp, int sweeps, const double *__restrict__ stop) {
+    coarse_body<T>(hw_blk(), L, x, b, tmp, sweeps, stop);
 }
 
 // ---------------------------------------------------------------------------
@@ -811,7 +859,7 @@ __device__ __forceinline__ void tail_coarse(const LevelDev &L, T *x, const T *b,
 }
 
 template <typename T>
-__global__ __launch_bounds__(TAIL_T) void k_vcycle_tail(const LevelDev *__restrict__ lev, int first, int last,
+__device__ __forceinline__ void vcycle_tail_body(const Blk &kb, const LevelDev *__restrict__ lev, int first, int last,
                                                         const T *__restrict__ bg, T *__restrict__ xg, int pre,
                                                         int post, int coarse_sweeps,
                                                         const double *__restrict__ stop) {
@@ -822,7 +870,7 @@ __global__ __launch_bounds__(TAIL_T) void k_vcycle_tail(const LevelDev *__restri
     __shared__ unsigned char s_col[COARSE_LDS];
     __shared__ int s_off[IBMDEM_MAX_LEVELS + 1];   // level l: x at off, then b, then r
     auto cells = [&](int l) { return lev[l].nx * lev[l].ny * lev[l].nz; };
-    if (threadIdx.x == 0) {
+    if (kb.tx == 0) {
         int off = 0;
         for (int l = first; l <= last; ++l) {
             s_off[l] = off;
@@ -833,7 +881,7 @@ __global__ __launch_bounds__(TAIL_T) void k_vcycle_tail(const LevelDev *__restri
     auto X = [&](int l) { return mem + s_off[l]; };
     auto B = [&](int l) { return mem + s_off[l] + cells(l) + (l == last ? 1 : 0); };
     auto R = [&](int l) { return B(l) + cells(l); };
-    for (int q = threadIdx.x; q < cells(first); q += TAIL_T) B(first)[q] = bg[q];
+    for (int q = kb.tx; q < cells(first); q += TAIL_T) B(first)[q] = bg[q];
     __syncthreads();
     for (int l = first; l < last; ++l) {
         for (int q = 0; q < pre; ++q) {
@@ -843,7 +891,7 @@ __global__ __launch_bounds__(TAIL_T) void k_vcycle_tail(const LevelDev *__restri
         tail_residual(lev[l], X(l), B(l), R(l));
         tail_restrict(lev[l], R(l), lev[l + 1], B(l + 1));
     }
-    if (threadIdx.x < 64) tail_coarse(lev[last], X(last), B(last), coarse_sweeps, s_nb, s_id, s_col);
+    if (kb.tx < 64) tail_coarse(lev[last], X(last), B(last), coarse_sweeps, s_nb, s_id, s_col);
     __syncthreads();
     for (int l = last - 1; l >= first; --l) {
         tail_prolong(lev[l], X(l), lev[l + 1], X(l + 1));
@@ -852,7 +900,14 @@ __global__ __launch_bounds__(TAIL_T) void k_vcycle_tail(const LevelDev *__restri
             tail_half(lev[l], X(l), B(l), 0, false);
         }
     }
-    for (int q = threadIdx.x; q < cells(first); q += TAIL_T) xg[q] = X(first)[q];
+    for (int q = kb.tx; q < cells(first); q += TAIL_T) xg[q] = X(first)[q];
+}
+template <typename T>
+__global__ __launch_bounds__(TAIL_T) void k_vcycle_tail(const LevelDev *__restrict__ lev, int first, int last,
+                                                        const T *__restrict__ bg, T *__restrict__ xg, int pre,
+                                                        int post, int coarse_sweeps,
+                                                        const double *__restrict__ stop) {
+    vcycle_tail_body<T>(hw_blk(), lev, first, last, bg, xg, pre, post, coarse_sweeps, stop);
 }
 
 // ---------------------------------------------------------------------------
@@ -873,7 +928,7 @@ __device__ __forceinline__ double block_reduce(double s, double *red) {
 // neighbours are read by the block
 // gridDim.x * gridDim.y positions away, a multiple of the XCD count for the
 // level-0 shapes, so they are served by the same L2.
-__global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__restrict__ p,
+__device__ __forceinline__ void matvec_dot_body(const Blk &kb, const LevelDev &L, const double *__restrict__ p,
                                                    double *__restrict__ Ap, double *__restrict__ partial,
                                                    const double *__restrict__ scal) {
     if (scal[SC_CONV] != 0.0) return;
@@ -882,8 +937,8 @@ __global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__r
     // the half sweep); when both rows exist, a cell's north neighbour in row
     // j0 is row j0 + 1's value of the same cell of its pair, and a cell's
     // south neighbour in row j0 + 1 is row j0's: those loads are shared
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j0 = (blockIdx.y * blockDim.y + threadIdx.y) * 2, k = blockIdx.z;
+    const int ip = kb.bx * kb.dx + kb.tx;
+    const int j0 = (kb.by * kb.dy + kb.ty) * 2, k = kb.bz;
     double s = 0.0;
     if (L.split) {
         const int ipc = min(ip, (L.nx >> 1) - 1);
@@ -951,49 +1006,63 @@ __global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__r
             }
         }
     }
-    const int nt = blockDim.x * blockDim.y, t = threadIdx.y * blockDim.x + threadIdx.x;
+    const int nt = kb.dx * kb.dy, t = kb.ty * kb.dx + kb.tx;
     red[t] = s;
     __syncthreads();
     for (int st = nt / 2; st > 0; st >>= 1) {
         if (t < st) red[t] += red[t + st];
         __syncthreads();
     }
-    if (t == 0) partial[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+    if (t == 0) partial[((long)kb.bz * kb.gy + kb.by) * kb.gx + kb.bx] = red[0];
+}
+__global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__restrict__ p,
+                                                   double *__restrict__ Ap, double *__restrict__ partial,
+                                                   const double *__restrict__ scal) {
+    matvec_dot_body(hw_blk(), L, p, Ap, partial, scal);
 }
 
 // CG iteration k >= 2, first kernel: p_k = z_k + beta_{k-1} p_{k-1} in place,
 // and the deferred x_k = x_{k-1} + alpha_{k-1} p_{k-1}
-__global__ __launch_bounds__(RED_T) void k_cg_px(long n, double *__restrict__ p, const double *__restrict__ z,
+__device__ __forceinline__ void cg_px_body(const Blk &kb, long n, double *__restrict__ p, const double *__restrict__ z,
                                                  double *__restrict__ x, const double *__restrict__ scal) {
     if (scal[SC_CONV] != 0.0) return;
     const double beta = scal[SC_BETA], alpha = scal[SC_ALPHA];
-    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) {
+    for (long q = (long)kb.bx * RED_T + kb.tx; q < n; q += (long)kb.gx * RED_T) {
         const double po = p[q];
         p[q] = z[q] + beta * po;
         x[q] = x[q] + alpha * po;
     }
 }
+__global__ __launch_bounds__(RED_T) void k_cg_px(long n, double *__restrict__ p, const double *__restrict__ z,
+                                                 double *__restrict__ x, const double *__restrict__ scal) {
+    cg_px_body(hw_blk(), n, p, z, x, scal);
+}
 
 // r -= alpha Ap; partial sums of r.r (x is updated one iteration later, by
 // k_cg_px or, after the last iteration, by k_gauge)
 // (and, in the mixed form, the FP32 copy of r the V-cycle reads)
-__global__ __launch_bounds__(RED_T) void k_cg_update_r(long n, double *__restrict__ r, const double *__restrict__ Ap,
+__device__ __forceinline__ void cg_update_r_body(const Blk &kb, long n, double *__restrict__ r, const double *__restrict__ Ap,
                                                        const double *__restrict__ scal, double *__restrict__ partial,
                                                        float *__restrict__ r32) {
     if (scal[SC_CONV] != 0.0) return;
     __shared__ double red[RED_T];
     const double alpha = scal[SC_ALPHA];
-    const long chunk = (n + gridDim.x - 1) / gridDim.x;
-    const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+    const long chunk = (n + kb.gx - 1) / kb.gx;
+    const long lo = (long)kb.bx * chunk, hi = min(n, lo + chunk);
     double s = 0.0;
-    for (long q = lo + threadIdx.x; q < hi; q += RED_T) {
+    for (long q = lo + kb.tx; q < hi; q += RED_T) {
         double rq = r[q] - alpha * Ap[q];
         r[q] = rq;
         if (r32) r32[q] = float(rq);
         s += rq * rq;
     }
     double t = block_reduce(s, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (kb.tx == 0) partial[kb.bx] = t;
+}
+__global__ __launch_bounds__(RED_T) void k_cg_update_r(long n, double *__restrict__ r, const double *__restrict__ Ap,
+                                                       const double *__restrict__ scal, double *__restrict__ partial,
+                                                       float *__restrict__ r32) {
+    cg_update_r_body(hw_blk(), n, r, Ap, scal, partial, r32);
 }
 
 // partial sums of a.b
@@ -1015,23 +1084,23 @@ __global__ __launch_bounds__(RED_T) void k_dot(long n, const double *__restrict_
 // Each thread sums its strided partials with four independent accumulators
 // (the loads of a round are in flight together), then a tree.
 constexpr int FIN_T = 1024;
-__global__ __launch_bounds__(FIN_T) void k_final(const double *__restrict__ partial, int np, double *__restrict__ scal,
+__device__ __forceinline__ void final_body(const Blk &kb, const double *__restrict__ partial, int np, double *__restrict__ scal,
                                                  int slot, int op, int it, double rtol) {
     if (scal[SC_CONV] != 0.0) return;
     __shared__ double red[FIN_T];
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    int q = threadIdx.x;
+    int q = kb.tx;
     for (; q + 3 * FIN_T < np; q += 4 * FIN_T)
         for (int u = 0; u < 4; ++u) a[u] += partial[q + u * FIN_T];
     for (int u = 0; q < np; q += FIN_T, ++u) a[u] += partial[q];
-    red[threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
+    red[kb.tx] = (a[0] + a[1]) + (a[2] + a[3]);
     __syncthreads();
     for (int st = FIN_T / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        if (kb.tx < st) red[kb.tx] += red[kb.tx + st];
         __syncthreads();
     }
     const double t = red[0];
-    if (threadIdx.x == 0) {
+    if (kb.tx == 0) {
         scal[slot] = t;
         if (op == 1) scal[SC_ALPHA] = scal[SC_RZ] / t;
         if (op == 2) { scal[SC_BETA] = t / scal[SC_RZ]; scal[SC_RZ] = t; }
@@ -1042,6 +1111,10 @@ __global__ __launch_bounds__(FIN_T) void k_final(const double *__restrict__ part
             if (t == 0.0) { scal[SC_CONV] = 1.0; scal[SC_ITER] = 0; scal[SC_RR] = 0.0; }
         }
     }
+}
+__global__ __launch_bounds__(FIN_T) void k_final(const double *__restrict__ partial, int np, double *__restrict__ scal,
+                                                 int slot, int op, int it, double rtol) {
+    final_body(hw_blk(), partial, np, scal, slot, op, it, rtol);
 }
 
 
@@ -1074,8 +1147,11 @@ __global__ __launch_bounds__(RB) void k_cg_init(LevelDev L, const double *__rest
     }
 }
 
+__device__ __forceinline__ void copy_body(const Blk &kb, long n, double *__restrict__ d, const double *__restrict__ s) {
+    for (long q = (long)kb.bx * RED_T + kb.tx; q < n; q += (long)kb.gx * RED_T) d[q] = s[q];
+}
 __global__ __launch_bounds__(RED_T) void k_copy(long n, double *__restrict__ d, const double *__restrict__ s) {
-    for (long q = (long)blockIdx.x * RED_T + threadIdx.x; q < n; q += (long)gridDim.x * RED_T) d[q] = s[q];
+    copy_body(hw_blk(), n, d, s);
 }
 template <typename TD, typename TS>
 __global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, const TS *__restrict__ s,
@@ -1410,10 +1486,11 @@ int cg_iteration(ibmdem_ctx *c, int it) {
 
 }  // namespace
 
-// Start of the solve, queued without any host wait: pinned-row right-hand
-// side, r = b, x = 0, ||b|| and the tolerance (a zero right-hand side
-// converges at once), z = M^-1 r, p = z, r.z.
-int poisson_begin(ibmdem_ctx *c) {
+// Start of the solve, queued without any host wait, in two parts: first
+// the pinned-row right-hand side, r = b, x = 0, ||b|| and the tolerance (a
+// zero right-hand side converges at once); then z = M^-1 r, p = z, r.z
+// (which a batch runs for all its cases at once).
+int poisson_begin_rhs(ibmdem_ctx *c) {
     const long n = c->n;
     hipStream_t s = c->stream;
     int rc;
@@ -1424,11 +1501,21 @@ int poisson_begin(ibmdem_ctx *c) {
     hipLaunchKernelGGL(k_cg_init, fs.grid, fs.block, 0, s, L0, c->rhs, c->cg_r, c->cg_x, c->scal,
                        c->cs.precision == IBMDEM_MIXED ? c->lev32[0].b : (float *)nullptr);
     CHECK_LAUNCH(c);
-    if ((rc = dot_into(c, c->cg_r, c->cg_r, SC_BNORM2, 4))) return rc;
+    return dot_into(c, c->cg_r, c->cg_r, SC_BNORM2, 4);
+}
+
+int poisson_begin_precondition(ibmdem_ctx *c) {
+    int rc;
     if ((rc = precondition_dot(c, SC_RZ, 0))) return rc;
-    hipLaunchKernelGGL(k_copy, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z);
+    hipLaunchKernelGGL(k_copy, dim3(stream_blocks(c->n)), dim3(RED_T), 0, c->stream, c->n, c->cg_p, c->cg_z);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
+}
+
+int poisson_begin(ibmdem_ctx *c) {
+    int rc;
+    if ((rc = poisson_begin_rhs(c))) return rc;
+    return poisson_begin_precondition(c);
 }
 
 // Iterations first..last, queued without any host wait. Once the device sets
@@ -1467,6 +1554,13 @@ int poisson_iterate_to_convergence(ibmdem_ctx *c, int first) {
 int poisson_check(ibmdem_ctx *c, ibmdem_solve_info *info) {
     int rc;
     if ((rc = read_scalars(c))) return rc;
+    return poisson_check_scalars(c, nullptr, info);
+}
+
+// The same report from solver scalars already on the host (scal, or the
+// context's own copy when null).
+int poisson_check_scalars(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info) {
+    if (scal) std::memcpy(c->scal_host, scal, SC_COUNT * sizeof(double));
     const double bnorm = std::sqrt(c->scal_host[SC_BNORM2]);
     const int conv = c->scal_host[SC_CONV] != 0.0;
     const double rr = bnorm > 0 ? c->scal_host[SC_RR] : 0.0;
@@ -1503,4 +1597,625 @@ int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
     if (chk && chk != IBMDEM_ERR_NOT_CONVERGED) return chk;
     if ((rc = poisson_gauge(c))) return rc;
     return chk;
+}
+
+// ---------------------------------------------------------------------------
+// The batched solve: one launch per solver operation over a group of cases.
+//
+// A group's cases share the precision, the smoothing counts and the
+// iteration cap; grids, level counts, where the one-work-group tail starts
+// and the coarsest-level sweeps are per case. A launch runs each member's
+// blocks with the body, block decomposition and arguments of the member's
+// own launch, and a member sees its operations in the order its own solve
+// queues them, so its arithmetic is bitwise that of solving it alone. A
+// block finds its case in a table of first blocks (each a multiple of 8, so
+// a case's blocks keep the XCD placement of its own launch); a member that
+// has converged returns at once, as it does alone.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct BGrid {
+    int gx, gy, gz;   // the case's grid of blocks
+    int dx, dy;       // its block shape: dx a power of two, dx * dy the launch's block size
+    int lgdx;         // log2(dx)
+};
+
+BGrid bgrid(dim3 grid, dim3 block) {
+    int lg = 0;
+    while ((1 << lg) < (int)block.x) ++lg;
+    return BGrid{(int)grid.x, (int)grid.y, (int)grid.z, (int)block.x, (int)block.y, lg};
+}
+
+template <class A>
+struct BTable {
+    const int *first;   // first block of each case, ascending
+    const A *arg;       // the case's arguments
+    int ncase;
+};
+
+// the block's case (the number of cases whose first block is at most
+// blockIdx.x, less one: a wavefront compares 64 first blocks per round) and
+// its coordinates in that case's own grid; null for a padding block
+template <class A>
+__device__ __forceinline__ const A *batch_arg(const BTable<A> &tab, Blk &kb) {
+    const int bid = blockIdx.x, lane = threadIdx.x & 63;
+    int q = -1;
+    for (int c0 = 0; c0 < tab.ncase; c0 += 64) {
+        const int cq = c0 + lane;
+        const int cnt = __popcll(__ballot(cq < tab.ncase && tab.first[cq] <= bid));
+        q += cnt;
+        if (cnt < 64) break;
+    }
+    q = __builtin_amdgcn_readfirstlane(q);
+    const A *a = tab.arg + q;
+    const BGrid g = a->g;
+    const int lb = bid - tab.first[q];
+    if (lb >= g.gx * g.gy * g.gz) return nullptr;
+    kb.gx = g.gx; kb.gy = g.gy; kb.gz = g.gz;
+    kb.bx = lb % g.gx;
+    kb.by = (lb / g.gx) % g.gy;
+    kb.bz = lb / (g.gx * g.gy);
+    kb.dx = g.dx; kb.dy = g.dy;
+    kb.tx = threadIdx.x & (g.dx - 1);
+    kb.ty = threadIdx.x >> g.lgdx;
+    return a;
+}
+
+template <typename T>
+struct RbArg {
+    BGrid g;
+    const LevelDev *L;
+    T *x;
+    const T *b;
+    const double *stop;
+    double *partial;
+    long pofs;
+    T *r;
+    const double *r64;
+    double *z64;
+};
+template <typename T, bool DOT, int RES, bool ZI>
+__global__ __launch_bounds__(RB) void kb_rb_half(BTable<RbArg<T>> tab, int color) {
+    Blk kb;
+    const RbArg<T> *a = batch_arg(tab, kb);
+    if (!a) return;
+    rb_half_body<T, T, DOT, RES, ZI>(kb, *a->L, a->x, a->b, color, a->stop, a->partial, a->pofs, a->r, a->r64,
+                                     a->z64);
+}
+
+template <typename T>
+struct RsArg {
+    BGrid g;
+    const LevelDev *F, *C;
+    const T *rf;
+    T *bc;
+    const double *stop;
+};
+template <typename T>
+__global__ __launch_bounds__(RB) void kb_restrict(BTable<RsArg<T>> tab) {
+    Blk kb;
+    const RsArg<T> *a = batch_arg(tab, kb);
+    if (!a) return;
+    restrict_body<T>(kb, *a->F, a->rf, *a->C, a->bc, a->stop);
+}
+
+template <typename T>
+struct PrArg {
+    BGrid g;
+    const LevelDev *F, *C;
+    T *xf;
+    const T *xc;
+    const double *stop;
+};
+template <typename T>
+__global__ __launch_bounds__(RB) void kb_prolong(BTable<PrArg<T>> tab) {
+    Blk kb;
+    const PrArg<T> *a = batch_arg(tab, kb);
+    if (!a) return;
+    prolong_body<T>(kb, *a->F, a->xf, *a->C, a->xc, a->stop);
+}
+
+// the bottom of a case's V-cycle: the one-work-group tail (first < last),
+// or the coarsest level alone from LDS tables or in global memory
+template <typename T>
+struct BtArg {
+    BGrid g;
+    const LevelDev *lev;
+    int first, last, pre, post, sweeps;
+    T *x;
+    const T *b;
+    T *tmp;
+    const double *stop;
+};
+template <typename T>
+__global__ __launch_bounds__(TAIL_T) void kb_tail(BTable<BtArg<T>> tab) {
+    Blk kb;
+    const BtArg<T> *a = batch_arg(tab, kb);
+    if (!a) return;
+    vcycle_tail_body<T>(kb, a->lev, a->first, a->last, a->b, a->x, a->pre, a->post, a->sweeps, a->stop);
+}
+template <typename T>
+__global__ __launch_bounds__(64) void kb_coarse_lds(BTable<BtArg<T>> tab) {
+    Blk kb;
+    const BtArg<T> *a = batch_arg(tab, kb);
+    if (!a) return;
+    coarse_lds_body<T>(kb, a->lev[a->last], a->x, a->b, a->sweeps, a->stop);
+}
+template <typename T>
+__global__ __launch_bounds__(COARSE_T) void kb_coarse(BTable<BtArg<T>> tab) {
+    Blk kb;
+    const BtArg<T> *a = batch_arg(tab, kb);
+    if (!a) return;
+    coarse_body<T>(kb, a->lev[a->last], a->x, a->b, a->tmp, a->sweeps, a->stop);
+}
+
+// the CG vector operations and the scalar step
+struct CgArg {
+    BGrid g;
+    const LevelDev *L;
+    long n;
+    double *p, *z, *x, *r, *Ap;
+    double *partial;
+    double *scal;
+    float *r32;
+    int np;           // partial sums k_final adds
+    double rtol;
+};
+__global__ __launch_bounds__(RB) void kb_matvec_dot(BTable<CgArg> tab) {
+    Blk kb;
+    const CgArg *a = batch_arg(tab, kb);
+    if (!a) return;
+    matvec_dot_body(kb, *a->L, a->p, a->Ap, a->partial, a->scal);
+}
+__global__ __launch_bounds__(RED_T) void kb_cg_px(BTable<CgArg> tab) {
+    Blk kb;
+    const CgArg *a = batch_arg(tab, kb);
+    if (!a) return;
+    cg_px_body(kb, a->n, a->p, a->z, a->x, a->scal);
+}
+__global__ __launch_bounds__(RED_T) void kb_cg_update_r(BTable<CgArg> tab) {
+    Blk kb;
+    const CgArg *a = batch_arg(tab, kb);
+    if (!a) return;
+    cg_update_r_body(kb, a->n, a->r, a->Ap, a->scal, a->partial, a->r32);
+}
+__global__ __launch_bounds__(RED_T) void kb_copy(BTable<CgArg> tab) {
+    Blk kb;
+    const CgArg *a = batch_arg(tab, kb);
+    if (!a) return;
+    copy_body(kb, a->n, a->p, a->z);
+}
+__global__ __launch_bounds__(FIN_T) void kb_final(BTable<CgArg> tab, int slot, int op, int it) {
+    Blk kb;
+    const CgArg *a = batch_arg(tab, kb);
+    if (!a) return;
+    final_body(kb, a->partial, a->np, a->scal, slot, op, it, a->rtol);
+}
+
+// every case's solver scalars into one array (case q at q * SC_COUNT)
+__global__ __launch_bounds__(64) void kb_gather_scalars(const double *const *__restrict__ scal, int ncase,
+                                                        double *__restrict__ out) {
+    const int q = blockIdx.x;
+    if (q < ncase && threadIdx.x < SC_COUNT) out[(long)q * SC_COUNT + threadIdx.x] = scal[q][threadIdx.x];
+}
+
+// One batched launch: its kernel (launched by fn with the uniform arguments
+// below) and its table in the group's device arena.
+struct BOp;
+using BLaunch = void (*)(const BOp &, hipStream_t, int it);
+struct BOp {
+    BLaunch fn = nullptr;
+    size_t first_off = 0, arg_off = 0;   // table offsets in the arena
+    const int *first = nullptr;
+    const void *arg = nullptr;
+    int ncase = 0, blocks = 0;
+    int color = 0, slot = 0, op = 0;
+    bool with_it = false;      // k_final's iteration argument is the iteration's number
+    bool skip_first = false;   // not in iteration 1 (the deferred x update)
+};
+template <class A>
+BTable<A> table(const BOp &o) {
+    return BTable<A>{o.first, (const A *)o.arg, o.ncase};
+}
+
+template <typename T, bool DOT, int RES, bool ZI>
+void launch_rb(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL((kb_rb_half<T, DOT, RES, ZI>), dim3(o.blocks), dim3(RB), 0, s, table<RbArg<T>>(o), o.color);
+}
+template <typename T>
+void launch_restrict(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_restrict<T>, dim3(o.blocks), dim3(RB), 0, s, table<RsArg<T>>(o));
+}
+template <typename T>
+void launch_prolong(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_prolong<T>, dim3(o.blocks), dim3(RB), 0, s, table<PrArg<T>>(o));
+}
+template <typename T>
+void launch_tail(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_tail<T>, dim3(o.blocks), dim3(TAIL_T), 0, s, table<BtArg<T>>(o));
+}
+template <typename T>
+void launch_coarse_lds(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_coarse_lds<T>, dim3(o.blocks), dim3(64), 0, s, table<BtArg<T>>(o));
+}
+template <typename T>
+void launch_coarse(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_coarse<T>, dim3(o.blocks), dim3(COARSE_T), 0, s, table<BtArg<T>>(o));
+}
+void launch_matvec(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_matvec_dot, dim3(o.blocks), dim3(RB), 0, s, table<CgArg>(o));
+}
+void launch_px(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_cg_px, dim3(o.blocks), dim3(RED_T), 0, s, table<CgArg>(o));
+}
+void launch_update_r(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_cg_update_r, dim3(o.blocks), dim3(RED_T), 0, s, table<CgArg>(o));
+}
+void launch_copy(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_copy, dim3(o.blocks), dim3(RED_T), 0, s, table<CgArg>(o));
+}
+void launch_final(const BOp &o, hipStream_t s, int it) {
+    hipLaunchKernelGGL(kb_final, dim3(o.blocks), dim3(FIN_T), 0, s, table<CgArg>(o), o.slot, o.op,
+                       o.with_it ? it : 0);
+}
+
+// Host bytes of the tables, copied to the device once the plan is built.
+struct Arena {
+    std::vector<char> bytes;
+    size_t put(const void *p, size_t n) {
+        const size_t off = (bytes.size() + 15) & ~size_t(15);
+        bytes.resize(off + n);
+        std::memcpy(bytes.data() + off, p, n);
+        return off;
+    }
+};
+
+// one launch of `block` threads per block over the cases with arguments
+// args (a case's grid and block shape in args[i].g)
+template <class A>
+int add_op(Arena &ar, std::vector<BOp> &ops, BLaunch fn, int block, const std::vector<A> &args, BOp o,
+           std::string &err) {
+    if (args.empty()) return IBMDEM_OK;
+    std::vector<int> first(args.size());
+    long tot = 0;
+    for (size_t i = 0; i < args.size(); ++i) {
+        const BGrid &g = args[i].g;
+        if (g.dx * g.dy != block || (1 << g.lgdx) != g.dx) {
+            err = "batched launch: a case's block shape does not fill the launch's blocks";
+            return IBMDEM_ERR_ARG;
+        }
+        tot = (tot + 7) & ~7L;
+        first[i] = (int)tot;
+        tot += (long)args[i].g.gx * args[i].g.gy * args[i].g.gz;
+        if (tot >= (1L << 31)) {
+            err = "batched launch of more than 2^31 blocks";
+            return IBMDEM_ERR_ARG;
+        }
+    }
+    o.fn = fn;
+    o.ncase = (int)args.size();
+    o.blocks = (int)tot;
+    o.first_off = ar.put(first.data(), first.size() * sizeof(int));
+    o.arg_off = ar.put(args.data(), args.size() * sizeof(A));
+    ops.push_back(o);
+    return IBMDEM_OK;
+}
+
+template <typename T>
+MGLevel<T> *levels(ibmdem_ctx *c);
+template <>
+MGLevel<double> *levels<double>(ibmdem_ctx *c) { return c->lev64; }
+template <>
+MGLevel<float> *levels<float>(ibmdem_ctx *c) { return c->lev32; }
+
+// the level where a case's V-cycle turns: its tail's first level, else its coarsest
+int bottom_level(const ibmdem_ctx *c) { return std::min(c->tail_first, c->nlev - 1); }
+
+// z = M^-1 r for every case, with r.z into scal[slot] (k_final's op): the
+// batched form of vcycle<T>(c, lev, 0, true) and precondition_dot
+template <typename T>
+int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ibmdem_ctx *> &cs, int slot,
+                          int op, std::string &err) {
+    const bool mixed = !std::is_same<T, double>::value;
+    const int pre = cs[0]->cs.mg_pre, post = cs[0]->cs.mg_post;
+    int depth = 0;
+    for (ibmdem_ctx *c : cs) depth = std::max(depth, bottom_level(c));
+    int rc;
+    // a half sweep of level l of every case whose V-cycle goes below l
+    auto half = [&](int l, int color, int zero, int dot_slot, int res) {
+        std::vector<RbArg<T>> args;
+        for (ibmdem_ctx *c : cs) {
+            if (l >= bottom_level(c)) continue;
+            MGLevel<T> &L = levels<T>(c)[l];
+            const LevelDev &d = L.d;
+            const LaunchShape sh = shape_for(div_up(d.nx, 2), d.ny, d.nz);
+            const dim3 hg = half_grid(d.nx, d.ny, d.nz);
+            RbArg<T> a{};
+            a.g = bgrid(hg, sh.block);
+            a.L = c->dlev + l;
+            a.x = L.x;
+            a.b = L.b;
+            a.stop = c->scal + SC_CONV;
+            a.partial = c->partial;
+            a.pofs = dot_slot >= 0 ? dot_slot * (long)hg.x * hg.y * hg.z : 0;
+            a.r = res ? L.r : nullptr;
+            a.r64 = dot_slot >= 0 && mixed ? c->cg_r : nullptr;
+            a.z64 = dot_slot >= 0 && mixed ? c->cg_z : nullptr;
+            args.push_back(a);
+        }
+        BOp o;
+        o.color = color;
+        BLaunch fn = dot_slot >= 0 ? launch_rb<T, true, 0, false>
+                     : res == 2    ? launch_rb<T, false, 2, false>
+                     : res == 1    ? launch_rb<T, false, 1, false>
+                     : zero        ? launch_rb<T, false, 0, true>
+                                   : launch_rb<T, false, 0, false>;
+        return add_op(ar, ops, fn, RB, args, o, err);
+    };
+    // the bottoms at level l
+    auto bottoms = [&](int l) {
+        std::vector<BtArg<T>> tail, lds, glob;
+        for (ibmdem_ctx *c : cs) {
+            if (bottom_level(c) != l) continue;
+            MGLevel<T> &L = levels<T>(c)[l];
+            BtArg<T> a{};
+            a.lev = c->dlev;
+            a.first = l;
+            a.last = c->nlev - 1;
+            a.pre = c->cs.mg_pre;
+            a.post = c->cs.mg_post;
+            a.sweeps = c->cs.mg_coarse_sweeps;
+            a.x = L.x;
+            a.b = L.b;
+            a.tmp = L.r;
+            a.stop = c->scal + SC_CONV;
+            if (l >= c->tail_first) {
+                a.g = bgrid(dim3(1), dim3(TAIL_T));
+                tail.push_back(a);
+            } else if (L.n <= COARSE_LDS) {
+                a.g = bgrid(dim3(1), dim3(64));
+                lds.push_back(a);
+            } else {
+                a.g = bgrid(dim3(1), dim3(COARSE_T));
+                glob.push_back(a);
+            }
+        }
+        int r;
+        if ((r = add_op(ar, ops, launch_tail<T>, TAIL_T, tail, BOp(), err))) return r;
+        if ((r = add_op(ar, ops, launch_coarse_lds<T>, 64, lds, BOp(), err))) return r;
+        return add_op(ar, ops, launch_coarse<T>, COARSE_T, glob, BOp(), err);
+    };
+    for (int l = 0; l < depth; ++l) {
+        for (int q = 0; q < pre; ++q) {
+            if ((rc = half(l, 0, q == 0, -1, 0))) return rc;
+            if ((rc = half(l, 1, 0, -1, q == pre - 1 ? 1 : 0))) return rc;
+        }
+        if ((rc = half(l, 0, 0, -1, 2))) return rc;
+        std::vector<RsArg<T>> rs;
+        for (ibmdem_ctx *c : cs) {
+            if (l >= bottom_level(c)) continue;
+            MGLevel<T> &F = levels<T>(c)[l], &C = levels<T>(c)[l + 1];
+            RsArg<T> a{};
+            a.g = bgrid(dim3(div_up(C.d.nx, RS_CX), div_up(C.d.ny, RS_CY), C.d.nz), dim3(RB));
+            a.F = c->dlev + l;
+            a.C = c->dlev + l + 1;
+            a.rf = F.r;
+            a.bc = C.b;
+            a.stop = c->scal + SC_CONV;
+            rs.push_back(a);
+        }
+        if ((rc = add_op(ar, ops, launch_restrict<T>, RB, rs, BOp(), err))) return rc;
+        if ((rc = bottoms(l + 1))) return rc;
+    }
+    for (int l = depth - 1; l >= 0; --l) {
+        std::vector<PrArg<T>> pr;
+        for (ibmdem_ctx *c : cs) {
+            if (l >= bottom_level(c)) continue;
+            MGLevel<T> &F = levels<T>(c)[l], &C = levels<T>(c)[l + 1];
+            const LaunchShape sh = shape_for(div_up(F.d.nx, 2), F.d.ny, F.d.nz);
+            if (sh.block.y % 2 || (sh.block.x + 2) * (sh.block.y / 2 + 2) * 2 > (unsigned)PRO_TILE) {
+                err = "prolongation tile does not fit";
+                return IBMDEM_ERR_ARG;
+            }
+            PrArg<T> a{};
+            a.g = bgrid(sh.grid, sh.block);
+            a.F = c->dlev + l;
+            a.C = c->dlev + l + 1;
+            a.xf = F.x;
+            a.xc = C.x;
+            a.stop = c->scal + SC_CONV;
+            pr.push_back(a);
+        }
+        if ((rc = add_op(ar, ops, launch_prolong<T>, RB, pr, BOp(), err))) return rc;
+        for (int q = 0; q < post; ++q) {
+            const bool last = l == 0 && q == post - 1;
+            if ((rc = half(l, 1, 0, last ? 0 : -1, 0))) return rc;
+            if ((rc = half(l, 0, 0, last ? 1 : -1, 0))) return rc;
+        }
+    }
+    std::vector<CgArg> fin;
+    for (ibmdem_ctx *c : cs) {
+        const LevelDev &d = c->lev64[0].d;
+        const dim3 g = half_grid(d.nx, d.ny, d.nz);
+        CgArg a{};
+        a.g = bgrid(dim3(1), dim3(FIN_T));
+        a.partial = c->partial;
+        a.scal = c->scal;
+        a.np = 2 * (int)(g.x * g.y * g.z);
+        a.rtol = c->cs.poisson_rtol;
+        fin.push_back(a);
+    }
+    BOp o;
+    o.slot = slot;
+    o.op = op;
+    return add_op(ar, ops, launch_final, FIN_T, fin, o, err);
+}
+
+// the CG vectors of every case (the mixed form's FP32 copy of r with them)
+CgArg cg_arg(ibmdem_ctx *c, dim3 grid, dim3 block, int np) {
+    CgArg a{};
+    a.g = bgrid(grid, block);
+    a.L = c->dlev;
+    a.n = c->n;
+    a.p = c->cg_p; a.z = c->cg_z; a.x = c->cg_x; a.r = c->cg_r; a.Ap = c->cg_Ap;
+    a.partial = c->partial;
+    a.scal = c->scal;
+    a.r32 = c->cs.precision == IBMDEM_MIXED ? c->lev32[0].b : nullptr;
+    a.np = np;
+    a.rtol = c->cs.poisson_rtol;
+    return a;
+}
+
+template <typename T>
+int plan_all(Arena &ar, std::vector<BOp> &begin, std::vector<BOp> &iter, const std::vector<ibmdem_ctx *> &cs,
+             std::string &err) {
+    int rc;
+    // poisson_begin_precondition: z = M^-1 r with r.z, p = z
+    if ((rc = plan_precondition_dot<T>(ar, begin, cs, SC_RZ, 0, err))) return rc;
+    std::vector<CgArg> cp, px, mv, fpap, up, frr;
+    for (ibmdem_ctx *c : cs) {
+        const LevelDev &L0 = c->lev64[0].d;
+        const LaunchShape sh = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);
+        const dim3 mvg = half_grid(L0.nx, L0.ny, L0.nz);
+        const int mvb = (int)(mvg.x * mvg.y * mvg.z);
+        cp.push_back(cg_arg(c, dim3(stream_blocks(c->n)), dim3(RED_T), 0));
+        px.push_back(cg_arg(c, dim3(stream_blocks(c->n)), dim3(RED_T), 0));
+        mv.push_back(cg_arg(c, mvg, sh.block, 0));
+        fpap.push_back(cg_arg(c, dim3(1), dim3(FIN_T), mvb));
+        up.push_back(cg_arg(c, dim3(RED_BLOCKS), dim3(RED_T), 0));
+        frr.push_back(cg_arg(c, dim3(1), dim3(FIN_T), RED_BLOCKS));
+    }
+    if ((rc = add_op(ar, begin, launch_copy, RED_T, cp, BOp(), err))) return rc;
+    // cg_iteration
+    BOp o;
+    o.skip_first = true;
+    if ((rc = add_op(ar, iter, launch_px, RED_T, px, o, err))) return rc;
+    if ((rc = add_op(ar, iter, launch_matvec, RB, mv, BOp(), err))) return rc;
+    o = BOp();
+    o.slot = SC_PAP; o.op = 1; o.with_it = true;
+    if ((rc = add_op(ar, iter, launch_final, FIN_T, fpap, o, err))) return rc;
+    if ((rc = add_op(ar, iter, launch_update_r, RED_T, up, BOp(), err))) return rc;
+    o.slot = SC_RR; o.op = 3;
+    if ((rc = add_op(ar, iter, launch_final, FIN_T, frr, o, err))) return rc;
+    return plan_precondition_dot<T>(ar, iter, cs, SC_RZ_NEW, 2, err);
+}
+
+int run_ops(const std::vector<BOp> &ops, hipStream_t s, int it, std::string &err) {
+    for (const BOp &o : ops) {
+        if (o.skip_first && it == 1) continue;
+        o.fn(o, s, it);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            err = std::string("batched solver launch: ") + hipGetErrorString(e);
+            return IBMDEM_ERR_HIP;
+        }
+    }
+    return IBMDEM_OK;
+}
+
+}  // namespace
+
+struct PoissonBatch {
+    std::vector<ibmdem_ctx *> cs;
+    std::vector<BOp> begin, iter;
+    char *arena = nullptr;             // device copy of the tables
+    const double **scal_ptrs = nullptr;   // in the arena
+    double *gather_dev = nullptr;
+    double *gather_host = nullptr;     // pinned
+};
+
+bool poisson_batchable(const ibmdem_ctx *c) {
+    // the V-cycle must leave r.z with its last sweeps (precondition_dot)
+    return c->nlev >= 2 && c->cs.mg_post >= 1;
+}
+
+void poisson_batch_destroy(PoissonBatch *pb) {
+    if (!pb) return;
+    if (pb->arena) (void)hipFree(pb->arena);
+    if (pb->gather_dev) (void)hipFree(pb->gather_dev);
+    if (pb->gather_host) (void)hipHostFree(pb->gather_host);
+    delete pb;
+}
+
+int poisson_batch_create(const std::vector<ibmdem_ctx *> &cs, PoissonBatch **out, std::string &err) {
+    *out = nullptr;
+    if (cs.empty()) {
+        err = "empty batch group";
+        return IBMDEM_ERR_ARG;
+    }
+    const ibmdem_ctx *c0 = cs[0];
+    for (const ibmdem_ctx *c : cs) {
+        if (!poisson_batchable(c) || c->cs.precision != c0->cs.precision || c->cs.mg_pre != c0->cs.mg_pre ||
+            c->cs.mg_post != c0->cs.mg_post || c->cs.poisson_max_iter != c0->cs.poisson_max_iter) {
+            err = "cases of a batched solve must share precision, smoothing counts and iteration cap";
+            return IBMDEM_ERR_ARG;
+        }
+    }
+    PoissonBatch *pb = new (std::nothrow) PoissonBatch();
+    if (!pb) return IBMDEM_ERR_ALLOC;
+    pb->cs = cs;
+    // the FP64 V-cycle's level 0 is the CG vectors (as precondition sets it)
+    for (ibmdem_ctx *c : cs) {
+        c->lev64[0].b = c->cg_r;
+        c->lev64[0].x = c->cg_z;
+    }
+    Arena ar;
+    int rc = c0->cs.precision == IBMDEM_MIXED ? plan_all<float>(ar, pb->begin, pb->iter, cs, err)
+                                              : plan_all<double>(ar, pb->begin, pb->iter, cs, err);
+    std::vector<const double *> sp;
+    for (ibmdem_ctx *c : cs) sp.push_back(c->scal);
+    const size_t sp_off = ar.put(sp.data(), sp.size() * sizeof(double *));
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        e = hipMalloc((void **)&pb->arena, ar.bytes.size());
+        if (e == hipSuccess) e = hipMemcpy(pb->arena, ar.bytes.data(), ar.bytes.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc((void **)&pb->gather_dev, cs.size() * SC_COUNT * sizeof(double));
+        if (e == hipSuccess)
+            e = hipHostMalloc((void **)&pb->gather_host, cs.size() * SC_COUNT * sizeof(double), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            err = std::string("batched solver set-up: ") + hipGetErrorString(e);
+            rc = IBMDEM_ERR_HIP;
+        }
+    }
+    if (rc) {
+        poisson_batch_destroy(pb);
+        return rc;
+    }
+    for (auto *ops : {&pb->begin, &pb->iter})
+        for (BOp &o : *ops) {
+            o.first = (const int *)(pb->arena + o.first_off);
+            o.arg = pb->arena + o.arg_off;
+        }
+    pb->scal_ptrs = (const double **)(pb->arena + sp_off);
+    *out = pb;
+    return IBMDEM_OK;
+}
+
+int poisson_batch_begin(PoissonBatch *pb, hipStream_t s, std::string &err) { return run_ops(pb->begin, s, 0, err); }
+
+int poisson_batch_iteration(PoissonBatch *pb, hipStream_t s, int it, std::string &err) {
+    return run_ops(pb->iter, s, it, err);
+}
+
+int poisson_batch_gather(PoissonBatch *pb, hipStream_t s, std::string &err) {
+    const int n = (int)pb->cs.size();
+    hipLaunchKernelGGL(kb_gather_scalars, dim3(n), dim3(64), 0, s, pb->scal_ptrs, n, pb->gather_dev);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pb->gather_host, pb->gather_dev, (size_t)n * SC_COUNT * sizeof(double),
+                           hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) {
+        err = std::string("batched solver scalars: ") + hipGetErrorString(e);
+        return IBMDEM_ERR_HIP;
+    }
+    return IBMDEM_OK;
+}
+
+const double *poisson_batch_scalars(const PoissonBatch *pb, int q) { return pb->gather_host + (size_t)q * SC_COUNT; }
+
+int poisson_batch_launches(const PoissonBatch *pb, int k) {
+    int n = (int)pb->begin.size();
+    for (int it = 1; it <= k; ++it)
+        for (const BOp &o : pb->iter) n += !(o.skip_first && it == 1);
+    return n;
 }

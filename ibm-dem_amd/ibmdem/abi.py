@@ -153,6 +153,8 @@ _FUNCS = {
     "ibmdem_batch_context": [C.c_void_p, _i32, C.POINTER(C.c_void_p)],
     "ibmdem_batch_run": [C.c_void_p, _i32, C.POINTER(SolveInfo)],
     "ibmdem_batch_autotune": [C.c_void_p, _i32],
+    "ibmdem_batch_set_fused": [C.c_void_p, _i32],
+    "ibmdem_batch_fused_solves": [C.c_void_p, C.POINTER(C.c_int64)],
 }
 EXPORTED = tuple(_FUNCS) + ("ibmdem_last_error", "ibmdem_batch_last_error")
 
@@ -526,6 +528,17 @@ class Batch:
 
     def autotune(self, trial_steps: int = 3) -> None:
         self._check(self.lib.ibmdem_batch_autotune(self.h, int(trial_steps)), "ibmdem_batch_autotune")
+
+    def set_fused(self, on: bool) -> None:
+        """Batched Poisson solves (the default) or each case's solve on its
+        own stream."""
+        self._check(self.lib.ibmdem_batch_set_fused(self.h, 1 if on else 0), "ibmdem_batch_set_fused")
+
+    def fused_solves(self) -> int:
+        """Group solves that have run batched so far."""
+        n = C.c_int64()
+        self._check(self.lib.ibmdem_batch_fused_solves(self.h, C.byref(n)), "ibmdem_batch_fused_solves")
+        return int(n.value)
 
     def close(self) -> None:
         if getattr(self, "h", None):

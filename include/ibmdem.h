@@ -285,8 +285,11 @@ int ibmdem_set_stencil_tile(ibmdem_ctx *ctx, int32_t tile);
 /* ---- batched form -----------------------------------------------------------
  * Many independent cases of uneven size advanced together on one GPU, each a
  * full context with its own HIP stream; a batch step interleaves the cases'
- * captured step graphs so their kernels overlap. No counterpart in the
- * reference, which runs one case per MPI job (main.F90). */
+ * captured step graphs so their kernels overlap, and runs the Poisson solves
+ * of cases that share the precision, the multigrid smoothing counts and the
+ * iteration cap as one captured graph of launches that each cover all of
+ * them. Every case's result is bitwise that of stepping it alone. No
+ * counterpart in the reference, which runs one case per MPI job (main.F90). */
 typedef struct ibmdem_batch ibmdem_batch;
 int ibmdem_batch_create(int32_t n, const ibmdem_case *cases, ibmdem_batch **out);
 int ibmdem_batch_destroy(ibmdem_batch *batch);
@@ -296,11 +299,16 @@ const char *ibmdem_batch_last_error(const ibmdem_batch *batch);
 /* Advance every case n_steps (FirstOrderForwardEuler + CloudEvolutionMS
  * each); info (may be NULL) receives one solve record per case. */
 int ibmdem_batch_run(ibmdem_batch *batch, int32_t n_steps, ibmdem_solve_info *info);
-/* Per case, time trial_steps steps (<= 0: 3) with each candidate multigrid
- * cycle from the case's current state and keep the fastest, then each
- * predictor schedule (stencil_tile 0..3) under that cycle; the state is
- * restored after every trial. */
+/* Time trial_steps batch steps (<= 0: 3) with each candidate multigrid
+ * cycle (the same for every case, so the solves stay batched) from the
+ * current state and keep the fastest, then, per case, each predictor
+ * schedule (stencil_tile 0..3); the state is restored after every trial. */
 int ibmdem_batch_autotune(ibmdem_batch *batch, int32_t trial_steps);
+/* on = 0: every case's solve runs on its own stream, as a single context's
+ * does (for comparison); on = 1 (the default): batched solves. */
+int ibmdem_batch_set_fused(ibmdem_batch *batch, int32_t on);
+/* the number of group solves that ran batched so far */
+int ibmdem_batch_fused_solves(const ibmdem_batch *batch, int64_t *count);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Kernel ids for ibmdem_profile_* (HIP events around every launch of that

@@ -1898,6 +1898,17 @@ int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {
     return b ? IBMDEM_OK : IBMDEM_ERR_ARG;
 }
 
+/* the oracle has no batched solve: one case after another either way */
+int ibmdem_batch_set_fused(ibmdem_batch *b, int32_t on) {
+    (void)on;
+    return b ? IBMDEM_OK : IBMDEM_ERR_ARG;
+}
+int ibmdem_batch_fused_solves(const ibmdem_batch *b, int64_t *count) {
+    if (!b || !count) return IBMDEM_ERR_ARG;
+    *count = 0;
+    return IBMDEM_OK;
+}
+
 /* ------------------------------------------------------------------------
  * Test hooks (oracle only): the scalar building blocks the closed-form tests
  * pin directly.

@@ -32,10 +32,13 @@ def _single(lib, setup, n):
     return out
 
 
-def _batched(lib, setups, n, tune=0):
+def _batched(lib, setups, n, tune=0, fused=True, mg=None, solves=None):
     with abi.Batch(lib, [s.case for s in setups]) as b:
         for q, s in enumerate(setups):
             cases.load(b[q], s)
+            if mg and q in mg:
+                b[q].set_mg_params(*mg[q])
+        b.set_fused(fused)
         if tune:
             before = [b[q].download_fields() + _planes(b[q]) for q in range(len(b))]
             b.autotune(tune)
@@ -43,6 +46,8 @@ def _batched(lib, setups, n, tune=0):
                 for x, y in zip(before[q], b[q].download_fields() + _planes(b[q])):
                     np.testing.assert_array_equal(x, y)
         infos = b.run(n)
+        if solves is not None:
+            solves.append(b.fused_solves())
         return [(b[q].download_fields(), b[q].download_markers().x, b[q].download_bodies().x,
                  infos[q].iterations, b[q].mg_params(), b[q].stencil_tile()) for q in range(len(b))]
 
@@ -69,17 +74,72 @@ def test_batch_equals_single_cases_on_the_oracle(oracle_lib):
         np.testing.assert_array_equal(g[2], want[2])
 
 
+def _assert_same(got, want):
+    for a, b in zip(got[0], want[0]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(got[1], want[1])
+    np.testing.assert_array_equal(got[2], want[2])
+    assert got[3] == want[3]
+
+
 @pytest.mark.gpu
 def test_batch_equals_single_cases_on_the_gpu(gpu_lib):
+    """The solves run batched (one launch per solver operation over all
+    cases) from the second step on; every case still gets exactly what it
+    gets alone."""
     setups = _cases()
-    got = _batched(gpu_lib, setups, 4)
+    solves = []
+    got = _batched(gpu_lib, setups, 4, solves=solves)
+    assert solves[0] >= 3   # steps 2..4 (the first learns the iteration counts)
     for s, g in zip(setups, got):
-        want = _single(gpu_lib, s, 4)
-        for a, b in zip(g[0], want[0]):
-            np.testing.assert_array_equal(a, b)
-        np.testing.assert_array_equal(g[1], want[1])
-        np.testing.assert_array_equal(g[2], want[2])
-        assert g[3] == want[3]
+        _assert_same(g, _single(gpu_lib, s, 4))
+
+
+@pytest.mark.gpu
+def test_batched_solve_equals_per_case_solves(gpu_lib):
+    """ibmdem_batch_set_fused(0) runs each case's solve on its own stream;
+    the batched solve gives the same bits."""
+    setups = _cases()
+    solves = []
+    a = _batched(gpu_lib, setups, 3, fused=False, solves=solves)
+    assert solves[0] == 0
+    for x, y in zip(a, _batched(gpu_lib, setups, 3)):
+        _assert_same(x, y)
+
+
+@pytest.mark.gpu
+def test_batched_mixed_precision_cases(gpu_lib):
+    """Mixed-precision cases (FP32 V-cycle, FP64 CG) batched, with uneven
+    grids whose V-cycles turn at different levels."""
+    setups = [cases.small_periodic(gap=0.75, precision=abi.MIXED),
+              cases.small_walls(precision=abi.MIXED),
+              cases.small_periodic(n=(64, 48, 40), n_spheres=5, D=8.0, seed=cases.SEED_BASE + 22, gap=1.0,
+                                   precision=abi.MIXED),
+              cases.small_open(axis=2, precision=abi.MIXED)]
+    solves = []
+    got = _batched(gpu_lib, setups, 4, solves=solves)
+    assert solves[0] >= 3
+    for s, g in zip(setups, got):
+        _assert_same(g, _single(gpu_lib, s, 4))
+
+
+@pytest.mark.gpu
+def test_batched_groups_with_different_cycles(gpu_lib):
+    """Cases with different smoothing counts solve in separate groups; cases
+    of one group may differ in their coarsest-level sweeps."""
+    setups = _cases()
+    mg = {0: (1, 2, 12), 1: (2, 2, 12), 3: (1, 2, 24)}
+    solves = []
+    got = _batched(gpu_lib, setups, 3, mg=mg, solves=solves)
+    assert solves[0] >= 4   # two groups, steps 2 and 3
+    for q, (s, g) in enumerate(zip(setups, got)):
+        with abi.Simulation(gpu_lib, s.case) as one:
+            cases.load(one, s)
+            if q in mg:
+                one.set_mg_params(*mg[q])
+            info = one.run(3)
+            want = (one.download_fields(), one.download_markers().x, one.download_bodies().x, info.iterations)
+        _assert_same(g, want)
 
 
 @pytest.mark.gpu
