@@ -19,10 +19,12 @@
 // one, as ibmdem_batch_set_fused(b, 0) does for every case.
 //
 // The autotuner picks the multigrid cycle (pre/post smoothing sweeps,
-// coarsest-level sweeps) with the shortest measured batch step, then, per
-// case, the predictor's tile schedule (case.stencil_tile).
+// coarsest-level sweeps) with the shortest measured batch step, then the
+// number of solve groups, then, per case, the predictor's tile schedule
+// (case.stencil_tile).
 #include "common.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <map>
 #include <new>
@@ -38,6 +40,7 @@ struct BatchGroup {
     hipGraphExec_t exec = nullptr;    // begin + exec_k iterations
     int exec_k = -1;
     bool active = false;              // solves batched this step
+    hipEvent_t t0 = nullptr, t1 = nullptr;   // around the solve (GPU time)
 };
 
 }  // namespace
@@ -49,7 +52,16 @@ struct ibmdem_batch {
     std::vector<BatchGroup> groups;
     std::vector<int32_t> plan_key;   // the cases' parameters the groups were planned with
     std::vector<hipEvent_t> ev;      // per case: end of its step's first part
-    int64_t fused_solves = 0;        // group solves run batched
+    int split = 1;                   // solve groups per set of shared parameters
+    // statistics: group solves run batched; GPU ms of those solves; host ms
+    // queueing first parts, waiting for solves, queueing second parts,
+    // waiting for the cases' ends
+    int64_t fused_solves = 0;
+    double ms[5] = {0, 0, 0, 0, 0};
+    // profiling of the batched solves, per IBMDEM_BK_* kind
+    bool prof = false;
+    double prof_ms[9] = {}, prof_bytes[9] = {};
+    int64_t prof_n[9] = {};
 };
 
 namespace {
@@ -131,6 +143,8 @@ void free_groups(ibmdem_batch *b) {
     for (BatchGroup &g : b->groups) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (g.stream) (void)hipStreamDestroy(g.stream);
+        if (g.t0) (void)hipEventDestroy(g.t0);
+        if (g.t1) (void)hipEventDestroy(g.t1);
         poisson_batch_destroy(g.pb);
     }
     b->groups.clear();
@@ -144,10 +158,13 @@ int group_error(ibmdem_batch *b, const BatchGroup &g, int rc, const std::string 
 }
 
 // Groups of the cases whose solve can run batched, rebuilt when a case's
-// cycle, precision or iteration cap has changed since they were planned.
+// cycle, precision or iteration cap, or the split, has changed since they
+// were planned. The cases that share those parameters are split into
+// b->split groups of about equal cell count (largest case first to the
+// lightest group), so one group's solve overlaps other groups' step parts.
 int plan_groups(ibmdem_batch *b) {
     const int n = (int)b->ctx.size();
-    std::vector<int32_t> key;
+    std::vector<int32_t> key{b->split};
     for (ibmdem_ctx *c : b->ctx)
         key.insert(key.end(), {poisson_batchable(c) ? 1 : 0, c->cs.precision, c->cs.mg_pre, c->cs.mg_post,
                                c->cs.mg_coarse_sweeps, c->cs.poisson_max_iter});
@@ -159,17 +176,36 @@ int plan_groups(ibmdem_batch *b) {
         if (poisson_batchable(c))
             by[std::make_tuple(c->cs.precision, c->cs.mg_pre, c->cs.mg_post, c->cs.poisson_max_iter)].push_back(q);
     }
+    std::vector<std::vector<int>> parts;
     for (auto &kv : by) {
+        const int k = std::min<int>(b->split, (int)kv.second.size());
+        std::vector<int> order = kv.second;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->ctx[x]->n > b->ctx[y]->n; });
+        std::vector<std::vector<int>> part(k);
+        std::vector<long> load(k, 0);
+        for (int q : order) {
+            const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            part[j].push_back(q);
+            load[j] += b->ctx[q]->n;
+        }
+        for (auto &pm : part) {
+            std::sort(pm.begin(), pm.end());
+            parts.push_back(pm);
+        }
+    }
+    for (auto &pm : parts) {
         b->groups.emplace_back();
         BatchGroup &g = b->groups.back();
-        g.member = kv.second;
+        g.member = pm;
         std::vector<ibmdem_ctx *> cs;
         for (int q : g.member) cs.push_back(b->ctx[q]);
         std::string err;
         int rc = poisson_batch_create(cs, &g.pb, err);
-        if (!rc && hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess) {
+        if (!rc) poisson_batch_set_profile(g.pb, b->prof);
+        if (!rc && (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess ||
+                    hipEventCreate(&g.t0) != hipSuccess || hipEventCreate(&g.t1) != hipSuccess)) {
             rc = IBMDEM_ERR_HIP;
-            err = "batch stream creation failed";
+            err = "batch stream or event creation failed";
         }
         if (rc) {
             b->err = err;
@@ -229,12 +265,27 @@ int group_solve_queue(ibmdem_batch *b, BatchGroup &g) {
         k = std::max(k, b->ctx[q]->graphs.cg_k);
     }
     int rc;
-    if (!g.exec || g.exec_k != k)
-        if ((rc = capture_group(b, g, k))) return rc;
-    if (hipGraphLaunch(g.exec, g.stream) != hipSuccess)
-        return group_error(b, g, IBMDEM_ERR_HIP, "batched solve graph launch failed");
     std::string err;
+    // t0 completes once every member's first part has
+    if (hipEventRecord(g.t0, g.stream) != hipSuccess)
+        return group_error(b, g, IBMDEM_ERR_HIP, "batched solve event record failed");
+    if (b->prof) {
+        // profiled: the same launches queued one by one, an event after each
+        poisson_batch_profile_arm(g.pb, true);
+        rc = poisson_batch_begin(g.pb, g.stream, err);
+        for (int it = 1; it <= k && !rc; ++it) rc = poisson_batch_iteration(g.pb, g.stream, it, err);
+        poisson_batch_profile_arm(g.pb, false);
+        if (rc) return group_error(b, g, rc, err);
+        g.exec_k = k;
+    } else {
+        if (!g.exec || g.exec_k != k)
+            if ((rc = capture_group(b, g, k))) return rc;
+        if (hipGraphLaunch(g.exec, g.stream) != hipSuccess)
+            return group_error(b, g, IBMDEM_ERR_HIP, "batched solve graph launch failed");
+    }
     if ((rc = poisson_batch_gather(g.pb, g.stream, err))) return group_error(b, g, rc, err);
+    if (hipEventRecord(g.t1, g.stream) != hipSuccess)
+        return group_error(b, g, IBMDEM_ERR_HIP, "batched solve event record failed");
     b->fused_solves++;
     return IBMDEM_OK;
 }
@@ -245,9 +296,17 @@ int group_solve_finish(ibmdem_batch *b, BatchGroup &g) {
     std::string err;
     int it = g.exec_k;
     const int maxit = b->ctx[g.member[0]]->cs.poisson_max_iter;
+    bool first = true;
     while (true) {
         if (hipStreamSynchronize(g.stream) != hipSuccess)
             return group_error(b, g, IBMDEM_ERR_HIP, "batched solve: stream synchronization failed");
+        if (first) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, g.t0, g.t1) == hipSuccess) b->ms[0] += ms;
+            if (poisson_batch_profile_collect(g.pb, b->prof_ms, b->prof_n, b->prof_bytes))
+                return group_error(b, g, IBMDEM_ERR_HIP, "batched solve profiling: event times unavailable");
+            first = false;
+        }
         bool all = true;
         for (size_t i = 0; i < g.member.size(); ++i) all = all && poisson_batch_scalars(g.pb, (int)i)[SC_CONV] != 0.0;
         if (all || it >= maxit) return IBMDEM_OK;
@@ -276,7 +335,14 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
         failed[q] = 1;
         if (first_q < 0 || q < first_q) { first_q = q; first_rc = r; }
     };
+    using clk = std::chrono::steady_clock;
+    auto lap = [&](clk::time_point &t, int slot) {
+        const clk::time_point now = clk::now();
+        b->ms[slot] += std::chrono::duration<double, std::milli>(now - t).count();
+        t = now;
+    };
     for (int s = 0; s < n_steps && first_q < 0; ++s) {
+        clk::time_point t = clk::now();
         std::fill(batched.begin(), batched.end(), 0);
         for (BatchGroup &g : b->groups) {
             g.active = true;
@@ -304,9 +370,12 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
                 g.active = false;
             }
         }
+        lap(t, 1);
         for (BatchGroup &g : b->groups) {
             if (!g.active) continue;
-            if ((rc = group_solve_finish(b, g))) {
+            rc = group_solve_finish(b, g);
+            lap(t, 2);
+            if (rc) {
                 for (int q : g.member) if (!failed[q]) fail(q, rc);
                 continue;
             }
@@ -315,15 +384,18 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
                 if (failed[q]) continue;
                 if ((rc = step_phase2_batched(b->ctx[q], poisson_batch_scalars(g.pb, (int)i), &inf[q]))) fail(q, rc);
             }
+            lap(t, 3);
         }
         for (int q = 0; q < n; ++q) {
             if (batched[q] || done[q] || failed[q]) continue;
             if ((rc = step_phase2(b->ctx[q], &inf[q]))) fail(q, rc);
         }
+        lap(t, 3);
         for (int q = 0; q < n; ++q) {
             if (done[q] || failed[q]) continue;
             if ((rc = step_phase3(b->ctx[q]))) fail(q, rc);
         }
+        lap(t, 4);
     }
     return first_q < 0 ? IBMDEM_OK : batch_error(b, first_q, first_rc);
 }
@@ -436,9 +508,51 @@ int ibmdem_batch_set_fused(ibmdem_batch *b, int32_t on) {
     return IBMDEM_OK;
 }
 
-int ibmdem_batch_fused_solves(const ibmdem_batch *b, int64_t *count) {
-    if (!b || !count) return IBMDEM_ERR_ARG;
-    *count = b->fused_solves;
+int ibmdem_batch_set_solve_groups(ibmdem_batch *b, int32_t groups) {
+    if (!b) return IBMDEM_ERR_ARG;
+    if (groups < 1 || groups > 64) {
+        b->err = "solve groups must be 1..64";
+        return IBMDEM_ERR_ARG;
+    }
+    for (BatchGroup &g : b->groups)
+        if (g.stream) (void)hipStreamSynchronize(g.stream);
+    b->split = groups;   // the groups are re-planned at the next run
+    return IBMDEM_OK;
+}
+
+int ibmdem_batch_get_solve_groups(const ibmdem_batch *b, int32_t *groups) {
+    if (!b || !groups) return IBMDEM_ERR_ARG;
+    *groups = b->split;
+    return IBMDEM_OK;
+}
+
+int ibmdem_batch_profile(ibmdem_batch *b, int32_t on) {
+    if (!b) return IBMDEM_ERR_ARG;
+    b->prof = on != 0;
+    for (BatchGroup &g : b->groups) {
+        if (g.stream) (void)hipStreamSynchronize(g.stream);
+        poisson_batch_set_profile(g.pb, b->prof);
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.exec_k = -1;
+    }
+    return IBMDEM_OK;
+}
+
+int ibmdem_batch_profile_read(const ibmdem_batch *b, int32_t kind, double *total_ms, int64_t *launches,
+                              double *bytes) {
+    if (!b || kind < 1 || kind > 8 || !total_ms || !launches || !bytes) return IBMDEM_ERR_ARG;
+    *total_ms = b->prof_ms[kind];
+    *launches = b->prof_n[kind];
+    *bytes = b->prof_bytes[kind];
+    return IBMDEM_OK;
+}
+
+int ibmdem_batch_stats(const ibmdem_batch *b, int64_t *fused_solves, double *ms) {
+    if (!b) return IBMDEM_ERR_ARG;
+    if (fused_solves) *fused_solves = b->fused_solves;
+    if (ms)
+        for (int i = 0; i < 5; ++i) ms[i] = b->ms[i];
     return IBMDEM_OK;
 }
 
@@ -488,6 +602,25 @@ int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {
         if ((rc = ibmdem_set_mg_params(b->ctx[q], best_i >= 0 ? cand[best_i] : &orig[3 * q])))
             return batch_error(b, q, rc);
     if ((rc = restore_all())) return rc;
+    // then the number of solve groups, timed the same way
+    if (b->fused) {
+        const int split0 = b->split;
+        double best_s = 1e300;
+        int best_split = split0;
+        for (int sp : {1, 2, 4}) {
+            b->split = sp;
+            double sec = 0.0;
+            rc = run_steps(b, 2, info.data());
+            if (rc == IBMDEM_OK) {
+                const auto t0 = std::chrono::steady_clock::now();
+                rc = run_steps(b, steps, info.data());
+                sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / steps;
+            }
+            if (rc == IBMDEM_OK && sec < best_s) { best_s = sec; best_split = sp; }
+            if ((rc = restore_all())) return rc;
+        }
+        b->split = best_split;
+    }
     // then, per case, the predictor schedule (every schedule gives the same bits)
     for (int q = 0; q < n; ++q) {
         ibmdem_ctx *c = b->ctx[q];

@@ -618,6 +618,14 @@ int poisson_batch_gather(PoissonBatch *pb, hipStream_t s, std::string &err);
 const double *poisson_batch_scalars(const PoissonBatch *pb, int q);
 // launches of the begin part and k iterations
 int poisson_batch_launches(const PoissonBatch *pb, int k);
+// Profiling (IBMDEM_BK_* kinds, 0..8): while on, a solve queued between
+// arm(true) and arm(false) records an event after every launch; collect
+// (after the solve has drained) adds the launches' times, counts and
+// algorithmic bytes to ms, launches and bytes, indexed by kind.
+void poisson_batch_set_profile(PoissonBatch *pb, bool on);
+bool poisson_batch_profiling(const PoissonBatch *pb);
+void poisson_batch_profile_arm(PoissonBatch *pb, bool armed);
+int poisson_batch_profile_collect(PoissonBatch *pb, double *ms, int64_t *launches, double *bytes);
 // ibm.hip
 int ibm_begin_step(ibmdem_ctx *c);
 int ibm_interpolate(ibmdem_ctx *c);

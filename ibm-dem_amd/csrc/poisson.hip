@@ -1618,26 +1618,53 @@ struct BGrid {
     int gx, gy, gz;   // the case's grid of blocks
     int dx, dy;       // its block shape: dx a power of two, dx * dy the launch's block size
     int lgdx;         // log2(dx)
+    int pb;           // physical blocks of the case in the launch (each takes every pb-th block)
 };
 
 BGrid bgrid(dim3 grid, dim3 block) {
     int lg = 0;
     while ((1 << lg) < (int)block.x) ++lg;
-    return BGrid{(int)grid.x, (int)grid.y, (int)grid.z, (int)block.x, (int)block.y, lg};
+    return BGrid{(int)grid.x, (int)grid.y, (int)grid.z, (int)block.x, (int)block.y, lg,
+                 (int)(grid.x * grid.y * grid.z)};
+}
+
+// Pointers read from a launch's tables are generic to the compiler, which
+// would then access memory through them with flat instructions (both wait
+// counters, 64-bit vector addresses). Device arrays are global memory, and
+// the tables and level descriptors are read-only while a kernel runs, so
+// they are marked as such: global (AS 1) and constant (AS 4, scalar loads
+// where the address is uniform).
+// (Through an integer: a generic-to-global-to-generic cast pair would be
+// folded away.)
+template <typename T>
+__device__ __forceinline__ T *glb(T *p) {
+    return (T *)((__attribute__((address_space(1))) T *)(uintptr_t)p);
+}
+template <typename T>
+__device__ __forceinline__ const T *cst(const T *p) {
+    return (const T *)((const __attribute__((address_space(4))) T *)(uintptr_t)p);
 }
 
 template <class A>
 struct BTable {
-    const int *first;   // first block of each case, ascending
+    const int *first;   // first physical block of each case, ascending
     const A *arg;       // the case's arguments
     int ncase;
 };
 
-// the block's case (the number of cases whose first block is at most
-// blockIdx.x, less one: a wavefront compares 64 first blocks per round) and
-// its coordinates in that case's own grid; null for a padding block
-template <class A>
-__device__ __forceinline__ const A *batch_arg(const BTable<A> &tab, Blk &kb) {
+// Runs f(args, kb) for each of the case's own blocks this block takes. The
+// block's case is the number of cases whose first block is at most
+// blockIdx.x, less one (a wavefront compares 64 first blocks per round).
+// Blocks go to the XCDs round robin (block i to XCD i % 8), and a case with
+// 8 or more blocks starts on XCD 0 with a multiple of 8 physical blocks, so
+// physical block p runs on XCD p % 8: XCD s takes the s-th of 8 contiguous
+// ranges of the case's blocks (whole planes where the case has 8 or more),
+// its physical blocks striding through the range together. Planes and their
+// z neighbours are then read through one XCD's L2, whatever a plane's block
+// count. A barrier separates a physical block's blocks, as the bodies reuse
+// their LDS.
+template <class A, class F>
+__device__ __forceinline__ void batch_for(const BTable<A> &tab, F &&f) {
     const int bid = blockIdx.x, lane = threadIdx.x & 63;
     int q = -1;
     for (int c0 = 0; c0 < tab.ncase; c0 += 64) {
@@ -1647,18 +1674,41 @@ __device__ __forceinline__ const A *batch_arg(const BTable<A> &tab, Blk &kb) {
         if (cnt < 64) break;
     }
     q = __builtin_amdgcn_readfirstlane(q);
-    const A *a = tab.arg + q;
-    const BGrid g = a->g;
-    const int lb = bid - tab.first[q];
-    if (lb >= g.gx * g.gy * g.gz) return nullptr;
+    const A &a = *cst(tab.arg + q);
+    const int p = bid - *cst(tab.first + q);
+    const BGrid g = a.g;
+    if (p >= g.pb) return;
+    const int plane = g.gx * g.gy, nb = plane * g.gz;
+    int lb0, hi, step;
+    if (nb >= 8) {
+        const int xs = p & 7, per = g.pb >> 3;
+        int lo;
+        if (g.gz >= 8) {
+            lo = (xs * g.gz >> 3) * plane;
+            hi = ((xs + 1) * g.gz >> 3) * plane;
+        } else {
+            lo = (int)((long)xs * nb >> 3);
+            hi = (int)((long)(xs + 1) * nb >> 3);
+        }
+        lb0 = lo + (p >> 3);
+        step = per;
+    } else {
+        lb0 = p;
+        hi = nb;
+        step = g.pb;
+    }
+    Blk kb;
     kb.gx = g.gx; kb.gy = g.gy; kb.gz = g.gz;
-    kb.bx = lb % g.gx;
-    kb.by = (lb / g.gx) % g.gy;
-    kb.bz = lb / (g.gx * g.gy);
     kb.dx = g.dx; kb.dy = g.dy;
     kb.tx = threadIdx.x & (g.dx - 1);
     kb.ty = threadIdx.x >> g.lgdx;
-    return a;
+    for (int lb = lb0; lb < hi; lb += step) {
+        kb.bx = lb % g.gx;
+        kb.by = (lb / g.gx) % g.gy;
+        kb.bz = lb / plane;
+        f(a, kb);
+        if (lb + step < hi) __syncthreads();
+    }
 }
 
 template <typename T>
@@ -1676,11 +1726,10 @@ struct RbArg {
 };
 template <typename T, bool DOT, int RES, bool ZI>
 __global__ __launch_bounds__(RB) void kb_rb_half(BTable<RbArg<T>> tab, int color) {
-    Blk kb;
-    const RbArg<T> *a = batch_arg(tab, kb);
-    if (!a) return;
-    rb_half_body<T, T, DOT, RES, ZI>(kb, *a->L, a->x, a->b, color, a->stop, a->partial, a->pofs, a->r, a->r64,
-                                     a->z64);
+    batch_for(tab, [&](const RbArg<T> &a, const Blk &kb) {
+        rb_half_body<T, T, DOT, RES, ZI>(kb, *cst(a.L), glb(a.x), glb(a.b), color, glb(a.stop), glb(a.partial), a.pofs,
+                                         glb(a.r), glb(a.r64), glb(a.z64));
+    });
 }
 
 template <typename T>
@@ -1693,10 +1742,9 @@ struct RsArg {
 };
 template <typename T>
 __global__ __launch_bounds__(RB) void kb_restrict(BTable<RsArg<T>> tab) {
-    Blk kb;
-    const RsArg<T> *a = batch_arg(tab, kb);
-    if (!a) return;
-    restrict_body<T>(kb, *a->F, a->rf, *a->C, a->bc, a->stop);
+    batch_for(tab, [&](const RsArg<T> &a, const Blk &kb) {
+        restrict_body<T>(kb, *cst(a.F), glb(a.rf), *cst(a.C), glb(a.bc), glb(a.stop));
+    });
 }
 
 template <typename T>
@@ -1709,10 +1757,9 @@ struct PrArg {
 };
 template <typename T>
 __global__ __launch_bounds__(RB) void kb_prolong(BTable<PrArg<T>> tab) {
-    Blk kb;
-    const PrArg<T> *a = batch_arg(tab, kb);
-    if (!a) return;
-    prolong_body<T>(kb, *a->F, a->xf, *a->C, a->xc, a->stop);
+    batch_for(tab, [&](const PrArg<T> &a, const Blk &kb) {
+        prolong_body<T>(kb, *cst(a.F), glb(a.xf), *cst(a.C), glb(a.xc), glb(a.stop));
+    });
 }
 
 // the bottom of a case's V-cycle: the one-work-group tail (first < last),
@@ -1729,24 +1776,22 @@ struct BtArg {
 };
 template <typename T>
 __global__ __launch_bounds__(TAIL_T) void kb_tail(BTable<BtArg<T>> tab) {
-    Blk kb;
-    const BtArg<T> *a = batch_arg(tab, kb);
-    if (!a) return;
-    vcycle_tail_body<T>(kb, a->lev, a->first, a->last, a->b, a->x, a->pre, a->post, a->sweeps, a->stop);
+    batch_for(tab, [&](const BtArg<T> &a, const Blk &kb) {
+        vcycle_tail_body<T>(kb, cst(a.lev), a.first, a.last, glb(a.b), glb(a.x), a.pre, a.post, a.sweeps,
+                            glb(a.stop));
+    });
 }
 template <typename T>
 __global__ __launch_bounds__(64) void kb_coarse_lds(BTable<BtArg<T>> tab) {
-    Blk kb;
-    const BtArg<T> *a = batch_arg(tab, kb);
-    if (!a) return;
-    coarse_lds_body<T>(kb, a->lev[a->last], a->x, a->b, a->sweeps, a->stop);
+    batch_for(tab, [&](const BtArg<T> &a, const Blk &kb) {
+        coarse_lds_body<T>(kb, *cst(a.lev + a.last), glb(a.x), glb(a.b), a.sweeps, glb(a.stop));
+    });
 }
 template <typename T>
 __global__ __launch_bounds__(COARSE_T) void kb_coarse(BTable<BtArg<T>> tab) {
-    Blk kb;
-    const BtArg<T> *a = batch_arg(tab, kb);
-    if (!a) return;
-    coarse_body<T>(kb, a->lev[a->last], a->x, a->b, a->tmp, a->sweeps, a->stop);
+    batch_for(tab, [&](const BtArg<T> &a, const Blk &kb) {
+        coarse_body<T>(kb, *cst(a.lev + a.last), glb(a.x), glb(a.b), glb(a.tmp), a.sweeps, glb(a.stop));
+    });
 }
 
 // the CG vector operations and the scalar step
@@ -1762,41 +1807,32 @@ struct CgArg {
     double rtol;
 };
 __global__ __launch_bounds__(RB) void kb_matvec_dot(BTable<CgArg> tab) {
-    Blk kb;
-    const CgArg *a = batch_arg(tab, kb);
-    if (!a) return;
-    matvec_dot_body(kb, *a->L, a->p, a->Ap, a->partial, a->scal);
+    batch_for(tab, [&](const CgArg &a, const Blk &kb) {
+        matvec_dot_body(kb, *cst(a.L), glb(a.p), glb(a.Ap), glb(a.partial), glb(a.scal));
+    });
 }
 __global__ __launch_bounds__(RED_T) void kb_cg_px(BTable<CgArg> tab) {
-    Blk kb;
-    const CgArg *a = batch_arg(tab, kb);
-    if (!a) return;
-    cg_px_body(kb, a->n, a->p, a->z, a->x, a->scal);
+    batch_for(tab, [&](const CgArg &a, const Blk &kb) { cg_px_body(kb, a.n, glb(a.p), glb(a.z), glb(a.x), glb(a.scal)); });
 }
 __global__ __launch_bounds__(RED_T) void kb_cg_update_r(BTable<CgArg> tab) {
-    Blk kb;
-    const CgArg *a = batch_arg(tab, kb);
-    if (!a) return;
-    cg_update_r_body(kb, a->n, a->r, a->Ap, a->scal, a->partial, a->r32);
+    batch_for(tab, [&](const CgArg &a, const Blk &kb) {
+        cg_update_r_body(kb, a.n, glb(a.r), glb(a.Ap), glb(a.scal), glb(a.partial), glb(a.r32));
+    });
 }
 __global__ __launch_bounds__(RED_T) void kb_copy(BTable<CgArg> tab) {
-    Blk kb;
-    const CgArg *a = batch_arg(tab, kb);
-    if (!a) return;
-    copy_body(kb, a->n, a->p, a->z);
+    batch_for(tab, [&](const CgArg &a, const Blk &kb) { copy_body(kb, a.n, glb(a.p), glb(a.z)); });
 }
 __global__ __launch_bounds__(FIN_T) void kb_final(BTable<CgArg> tab, int slot, int op, int it) {
-    Blk kb;
-    const CgArg *a = batch_arg(tab, kb);
-    if (!a) return;
-    final_body(kb, a->partial, a->np, a->scal, slot, op, it, a->rtol);
+    batch_for(tab, [&](const CgArg &a, const Blk &kb) {
+        final_body(kb, glb(a.partial), a.np, glb(a.scal), slot, op, it, a.rtol);
+    });
 }
 
 // every case's solver scalars into one array (case q at q * SC_COUNT)
 __global__ __launch_bounds__(64) void kb_gather_scalars(const double *const *__restrict__ scal, int ncase,
                                                         double *__restrict__ out) {
     const int q = blockIdx.x;
-    if (q < ncase && threadIdx.x < SC_COUNT) out[(long)q * SC_COUNT + threadIdx.x] = scal[q][threadIdx.x];
+    if (q < ncase && threadIdx.x < SC_COUNT) out[(long)q * SC_COUNT + threadIdx.x] = glb(scal[q])[threadIdx.x];
 }
 
 // One batched launch: its kernel (launched by fn with the uniform arguments
@@ -1812,6 +1848,21 @@ struct BOp {
     int color = 0, slot = 0, op = 0;
     bool with_it = false;      // k_final's iteration argument is the iteration's number
     bool skip_first = false;   // not in iteration 1 (the deferred x update)
+    int kind = 0;              // IBMDEM_BK_* (profiling)
+    double bytes = 0.0;        // algorithmic HBM bytes of the launch (0: not stated)
+};
+
+// Profiling of the batched solve: while armed, run_ops records an event
+// before the first launch and after every launch, so the elapsed time
+// between two events is one launch's run time once the stream runs behind
+// the host (the launches are queued far faster than they run).
+constexpr int BK_COUNT = 9;
+struct BProf {
+    bool on = false, armed = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    std::vector<double> bytes;
+    int n = 0;                          // launches recorded
 };
 template <class A>
 BTable<A> table(const BOp &o) {
@@ -1871,22 +1922,36 @@ struct Arena {
 };
 
 // one launch of `block` threads per block over the cases with arguments
-// args (a case's grid and block shape in args[i].g)
+// args (a case's grid and block shape in args[i].g). A physical block takes
+// up to BLOCKS_PER_PHYS of its case's blocks (fewer where the launch would
+// otherwise have under 4096 physical blocks), so its start (finding its
+// case and loading the case's arguments) is paid once for several.
+constexpr int BLOCKS_PER_PHYS = 8;
 template <class A>
-int add_op(Arena &ar, std::vector<BOp> &ops, BLaunch fn, int block, const std::vector<A> &args, BOp o,
+int add_op(Arena &ar, std::vector<BOp> &ops, BLaunch fn, int block, std::vector<A> args, BOp o,
            std::string &err) {
     if (args.empty()) return IBMDEM_OK;
+    long total = 0;
+    for (const A &a : args) total += (long)a.g.gx * a.g.gy * a.g.gz;
+    const int per = (int)std::max<long>(1, std::min<long>(BLOCKS_PER_PHYS, total / 4096));
     std::vector<int> first(args.size());
     long tot = 0;
     for (size_t i = 0; i < args.size(); ++i) {
-        const BGrid &g = args[i].g;
+        BGrid &g = args[i].g;
         if (g.dx * g.dy != block || (1 << g.lgdx) != g.dx) {
             err = "batched launch: a case's block shape does not fill the launch's blocks";
             return IBMDEM_ERR_ARG;
         }
-        tot = (tot + 7) & ~7L;
+        const long nb = (long)g.gx * g.gy * g.gz;
+        long pb = (nb + per - 1) / per;
+        if (nb >= 8) {
+            // the case starts on XCD 0 and its blocks keep their XCD
+            pb = (pb + 7) & ~7L;
+            tot = (tot + 7) & ~7L;
+        }
+        g.pb = (int)pb;
         first[i] = (int)tot;
-        tot += (long)args[i].g.gx * args[i].g.gy * args[i].g.gz;
+        tot += pb;
         if (tot >= (1L << 31)) {
             err = "batched launch of more than 2^31 blocks";
             return IBMDEM_ERR_ARG;
@@ -1945,6 +2010,19 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
         }
         BOp o;
         o.color = color;
+        o.kind = l == 0 ? IBMDEM_BK_SMOOTH0 : IBMDEM_BK_SMOOTH;
+        if (l == 0) {
+            // per cell of the level: the half's b and the other colour read,
+            // the half's x written (the first sweep reads no x); the residual
+            // variants also read the half's own x or write its r; the mixed
+            // form's r.z variant also reads the FP64 r and writes the FP64 z
+            const double per = dot_slot >= 0 ? 1.5 * sizeof(T) + (mixed ? 8.0 : 0.0)
+                               : res          ? 2.0 * sizeof(T)
+                               : zero         ? 1.0 * sizeof(T)
+                                              : 1.5 * sizeof(T);
+            for (ibmdem_ctx *c : cs)
+                if (l < bottom_level(c)) o.bytes += per * (double)c->n;
+        }
         BLaunch fn = dot_slot >= 0 ? launch_rb<T, true, 0, false>
                      : res == 2    ? launch_rb<T, false, 2, false>
                      : res == 1    ? launch_rb<T, false, 1, false>
@@ -1981,9 +2059,11 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
             }
         }
         int r;
-        if ((r = add_op(ar, ops, launch_tail<T>, TAIL_T, tail, BOp(), err))) return r;
-        if ((r = add_op(ar, ops, launch_coarse_lds<T>, 64, lds, BOp(), err))) return r;
-        return add_op(ar, ops, launch_coarse<T>, COARSE_T, glob, BOp(), err);
+        BOp o;
+        o.kind = IBMDEM_BK_BOTTOM;
+        if ((r = add_op(ar, ops, launch_tail<T>, TAIL_T, tail, o, err))) return r;
+        if ((r = add_op(ar, ops, launch_coarse_lds<T>, 64, lds, o, err))) return r;
+        return add_op(ar, ops, launch_coarse<T>, COARSE_T, glob, o, err);
     };
     for (int l = 0; l < depth; ++l) {
         for (int q = 0; q < pre; ++q) {
@@ -2004,7 +2084,12 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
             a.stop = c->scal + SC_CONV;
             rs.push_back(a);
         }
-        if ((rc = add_op(ar, ops, launch_restrict<T>, RB, rs, BOp(), err))) return rc;
+        BOp ors;
+        ors.kind = IBMDEM_BK_RESTRICT;
+        // the fine residual read, the coarse right-hand side written
+        for (ibmdem_ctx *c : cs)
+            if (l < bottom_level(c)) ors.bytes += sizeof(T) * (double)(levels<T>(c)[l].n + levels<T>(c)[l + 1].n);
+        if ((rc = add_op(ar, ops, launch_restrict<T>, RB, rs, ors, err))) return rc;
         if ((rc = bottoms(l + 1))) return rc;
     }
     for (int l = depth - 1; l >= 0; --l) {
@@ -2026,7 +2111,13 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
             a.stop = c->scal + SC_CONV;
             pr.push_back(a);
         }
-        if ((rc = add_op(ar, ops, launch_prolong<T>, RB, pr, BOp(), err))) return rc;
+        BOp opr;
+        opr.kind = IBMDEM_BK_PROLONG;
+        // the coarse solution read, the fine one read and written
+        for (ibmdem_ctx *c : cs)
+            if (l < bottom_level(c))
+                opr.bytes += sizeof(T) * (double)(2 * levels<T>(c)[l].n + levels<T>(c)[l + 1].n);
+        if ((rc = add_op(ar, ops, launch_prolong<T>, RB, pr, opr, err))) return rc;
         for (int q = 0; q < post; ++q) {
             const bool last = l == 0 && q == post - 1;
             if ((rc = half(l, 1, 0, last ? 0 : -1, 0))) return rc;
@@ -2048,6 +2139,7 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
     BOp o;
     o.slot = slot;
     o.op = op;
+    o.kind = IBMDEM_BK_SCALAR;
     return add_op(ar, ops, launch_final, FIN_T, fin, o, err);
 }
 
@@ -2085,22 +2177,54 @@ int plan_all(Arena &ar, std::vector<BOp> &begin, std::vector<BOp> &iter, const s
         up.push_back(cg_arg(c, dim3(RED_BLOCKS), dim3(RED_T), 0));
         frr.push_back(cg_arg(c, dim3(1), dim3(FIN_T), RED_BLOCKS));
     }
-    if ((rc = add_op(ar, begin, launch_copy, RED_T, cp, BOp(), err))) return rc;
-    // cg_iteration
+    double cells = 0.0;
+    for (ibmdem_ctx *c : cs) cells += (double)c->n;
+    const bool mixed = !std::is_same<T, double>::value;
     BOp o;
+    o.kind = IBMDEM_BK_VECTOR;
+    o.bytes = 16.0 * cells;   // z read, p written
+    if ((rc = add_op(ar, begin, launch_copy, RED_T, cp, o, err))) return rc;
+    // cg_iteration
     o.skip_first = true;
+    o.bytes = 40.0 * cells;   // p, x read and written, z read
     if ((rc = add_op(ar, iter, launch_px, RED_T, px, o, err))) return rc;
-    if ((rc = add_op(ar, iter, launch_matvec, RB, mv, BOp(), err))) return rc;
     o = BOp();
+    o.kind = IBMDEM_BK_MATVEC;
+    o.bytes = 16.0 * cells;   // p read, Ap written
+    if ((rc = add_op(ar, iter, launch_matvec, RB, mv, o, err))) return rc;
+    o = BOp();
+    o.kind = IBMDEM_BK_SCALAR;
     o.slot = SC_PAP; o.op = 1; o.with_it = true;
     if ((rc = add_op(ar, iter, launch_final, FIN_T, fpap, o, err))) return rc;
-    if ((rc = add_op(ar, iter, launch_update_r, RED_T, up, BOp(), err))) return rc;
+    BOp ou;
+    ou.kind = IBMDEM_BK_VECTOR;
+    ou.bytes = (24.0 + (mixed ? 4.0 : 0.0)) * cells;   // r read and written, Ap read (FP32 r written)
+    if ((rc = add_op(ar, iter, launch_update_r, RED_T, up, ou, err))) return rc;
     o.slot = SC_RR; o.op = 3;
     if ((rc = add_op(ar, iter, launch_final, FIN_T, frr, o, err))) return rc;
     return plan_precondition_dot<T>(ar, iter, cs, SC_RZ_NEW, 2, err);
 }
 
-int run_ops(const std::vector<BOp> &ops, hipStream_t s, int it, std::string &err) {
+int prof_event(BProf &pr, int i, hipStream_t s, std::string &err) {
+    while ((int)pr.ev.size() <= i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) {
+            err = "profiling event creation failed";
+            return IBMDEM_ERR_HIP;
+        }
+        pr.ev.push_back(e);
+    }
+    if (hipEventRecord(pr.ev[i], s) != hipSuccess) {
+        err = "profiling event record failed";
+        return IBMDEM_ERR_HIP;
+    }
+    return IBMDEM_OK;
+}
+
+int run_ops(const std::vector<BOp> &ops, hipStream_t s, int it, BProf &pr, std::string &err) {
+    const bool rec = pr.on && pr.armed;
+    int rc;
+    if (rec && pr.n == 0 && (rc = prof_event(pr, 0, s, err))) return rc;
     for (const BOp &o : ops) {
         if (o.skip_first && it == 1) continue;
         o.fn(o, s, it);
@@ -2108,6 +2232,14 @@ int run_ops(const std::vector<BOp> &ops, hipStream_t s, int it, std::string &err
         if (e != hipSuccess) {
             err = std::string("batched solver launch: ") + hipGetErrorString(e);
             return IBMDEM_ERR_HIP;
+        }
+        if (rec) {
+            if ((rc = prof_event(pr, pr.n + 1, s, err))) return rc;
+            pr.kind.resize(pr.n + 1);
+            pr.bytes.resize(pr.n + 1);
+            pr.kind[pr.n] = o.kind;
+            pr.bytes[pr.n] = o.bytes;
+            pr.n++;
         }
     }
     return IBMDEM_OK;
@@ -2118,6 +2250,7 @@ int run_ops(const std::vector<BOp> &ops, hipStream_t s, int it, std::string &err
 struct PoissonBatch {
     std::vector<ibmdem_ctx *> cs;
     std::vector<BOp> begin, iter;
+    BProf prof;
     char *arena = nullptr;             // device copy of the tables
     const double **scal_ptrs = nullptr;   // in the arena
     double *gather_dev = nullptr;
@@ -2131,6 +2264,7 @@ bool poisson_batchable(const ibmdem_ctx *c) {
 
 void poisson_batch_destroy(PoissonBatch *pb) {
     if (!pb) return;
+    for (hipEvent_t e : pb->prof.ev) (void)hipEventDestroy(e);
     if (pb->arena) (void)hipFree(pb->arena);
     if (pb->gather_dev) (void)hipFree(pb->gather_dev);
     if (pb->gather_host) (void)hipHostFree(pb->gather_host);
@@ -2191,10 +2325,35 @@ int poisson_batch_create(const std::vector<ibmdem_ctx *> &cs, PoissonBatch **out
     return IBMDEM_OK;
 }
 
-int poisson_batch_begin(PoissonBatch *pb, hipStream_t s, std::string &err) { return run_ops(pb->begin, s, 0, err); }
+int poisson_batch_begin(PoissonBatch *pb, hipStream_t s, std::string &err) {
+    return run_ops(pb->begin, s, 0, pb->prof, err);
+}
 
 int poisson_batch_iteration(PoissonBatch *pb, hipStream_t s, int it, std::string &err) {
-    return run_ops(pb->iter, s, it, err);
+    return run_ops(pb->iter, s, it, pb->prof, err);
+}
+
+void poisson_batch_set_profile(PoissonBatch *pb, bool on) { pb->prof.on = on; }
+bool poisson_batch_profiling(const PoissonBatch *pb) { return pb->prof.on; }
+
+void poisson_batch_profile_arm(PoissonBatch *pb, bool armed) {
+    pb->prof.armed = armed;
+    if (armed) pb->prof.n = 0;
+}
+
+int poisson_batch_profile_collect(PoissonBatch *pb, double *ms, int64_t *launches, double *bytes) {
+    const BProf &pr = pb->prof;
+    if (!pr.on) return IBMDEM_OK;
+    for (int i = 0; i < pr.n; ++i) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, pr.ev[i], pr.ev[i + 1]) != hipSuccess) return IBMDEM_ERR_HIP;
+        const int k = pr.kind[i];
+        if (k < 0 || k >= BK_COUNT) continue;
+        ms[k] += t;
+        launches[k] += 1;
+        bytes[k] += pr.bytes[i];
+    }
+    return IBMDEM_OK;
 }
 
 int poisson_batch_gather(PoissonBatch *pb, hipStream_t s, std::string &err) {

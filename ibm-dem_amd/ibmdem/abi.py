@@ -154,7 +154,11 @@ _FUNCS = {
     "ibmdem_batch_run": [C.c_void_p, _i32, C.POINTER(SolveInfo)],
     "ibmdem_batch_autotune": [C.c_void_p, _i32],
     "ibmdem_batch_set_fused": [C.c_void_p, _i32],
-    "ibmdem_batch_fused_solves": [C.c_void_p, C.POINTER(C.c_int64)],
+    "ibmdem_batch_set_solve_groups": [C.c_void_p, _i32],
+    "ibmdem_batch_get_solve_groups": [C.c_void_p, _pi],
+    "ibmdem_batch_stats": [C.c_void_p, C.POINTER(C.c_int64), _pd],
+    "ibmdem_batch_profile": [C.c_void_p, _i32],
+    "ibmdem_batch_profile_read": [C.c_void_p, _i32, _pd, C.POINTER(C.c_int64), _pd],
 }
 EXPORTED = tuple(_FUNCS) + ("ibmdem_last_error", "ibmdem_batch_last_error")
 
@@ -536,9 +540,40 @@ class Batch:
 
     def fused_solves(self) -> int:
         """Group solves that have run batched so far."""
+        return self.stats()[0]
+
+    def stats(self):
+        """(batched group solves, {solve GPU ms, host ms per phase})"""
         n = C.c_int64()
-        self._check(self.lib.ibmdem_batch_fused_solves(self.h, C.byref(n)), "ibmdem_batch_fused_solves")
-        return int(n.value)
+        ms = (C.c_double * 5)()
+        self._check(self.lib.ibmdem_batch_stats(self.h, C.byref(n), ms), "ibmdem_batch_stats")
+        keys = ("solve_gpu", "queue_first", "wait_solve", "queue_second", "wait_end")
+        return int(n.value), dict(zip(keys, list(ms)))
+
+    BATCH_KINDS = {"smooth0": 1, "smooth": 2, "restrict": 3, "prolong": 4, "bottom": 5, "matvec": 6,
+                   "vector": 7, "scalar": 8}
+
+    def profile(self, on: bool = True) -> None:
+        """Time every launch of the batched solves (by kind) from now on."""
+        self._check(self.lib.ibmdem_batch_profile(self.h, 1 if on else 0), "ibmdem_batch_profile")
+
+    def profile_read(self):
+        """{kind: (total ms, launches, total algorithmic bytes)}"""
+        out = {}
+        for name, k in self.BATCH_KINDS.items():
+            ms, n, by = C.c_double(), C.c_int64(), C.c_double()
+            self._check(self.lib.ibmdem_batch_profile_read(self.h, k, C.byref(ms), C.byref(n), C.byref(by)),
+                        "ibmdem_batch_profile_read")
+            out[name] = (ms.value, int(n.value), by.value)
+        return out
+
+    def set_solve_groups(self, groups: int) -> None:
+        self._check(self.lib.ibmdem_batch_set_solve_groups(self.h, int(groups)), "ibmdem_batch_set_solve_groups")
+
+    def solve_groups(self) -> int:
+        g = (C.c_int32 * 1)()
+        self._check(self.lib.ibmdem_batch_get_solve_groups(self.h, g), "ibmdem_batch_get_solve_groups")
+        return int(g[0])
 
     def close(self) -> None:
         if getattr(self, "h", None):

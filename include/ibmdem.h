@@ -301,14 +301,40 @@ const char *ibmdem_batch_last_error(const ibmdem_batch *batch);
 int ibmdem_batch_run(ibmdem_batch *batch, int32_t n_steps, ibmdem_solve_info *info);
 /* Time trial_steps batch steps (<= 0: 3) with each candidate multigrid
  * cycle (the same for every case, so the solves stay batched) from the
- * current state and keep the fastest, then, per case, each predictor
- * schedule (stencil_tile 0..3); the state is restored after every trial. */
+ * current state and keep the fastest, then the number of solve groups
+ * (1, 2, 4), then, per case, each predictor schedule (stencil_tile 0..3);
+ * the state is restored after every trial. */
 int ibmdem_batch_autotune(ibmdem_batch *batch, int32_t trial_steps);
 /* on = 0: every case's solve runs on its own stream, as a single context's
  * does (for comparison); on = 1 (the default): batched solves. */
 int ibmdem_batch_set_fused(ibmdem_batch *batch, int32_t on);
-/* the number of group solves that ran batched so far */
-int ibmdem_batch_fused_solves(const ibmdem_batch *batch, int64_t *count);
+/* The cases that share precision, smoothing counts and iteration cap solve
+ * in `groups` (1..64, default 1) batched groups of about equal cell count,
+ * each on its own stream, so one group's solve overlaps the other groups'
+ * step parts. */
+int ibmdem_batch_set_solve_groups(ibmdem_batch *batch, int32_t groups);
+int ibmdem_batch_get_solve_groups(const ibmdem_batch *batch, int32_t *groups);
+/* Statistics since creation: group solves that ran batched; ms[0] = GPU
+ * time of those solves, ms[1..4] = host time queueing the cases' first
+ * parts and the solves, waiting for the solves, queueing the second parts,
+ * waiting for the cases' ends (either pointer may be NULL). */
+int ibmdem_batch_stats(const ibmdem_batch *batch, int64_t *fused_solves, double *ms);
+/* Profiling of the batched solves: while on, each group's solve is queued
+ * launch by launch (not from its graph) with an event after every launch,
+ * and the launches' run times are added up per kind. profile_read gives a
+ * kind's total ms, launches and total algorithmic HBM bytes (0 where not
+ * stated). */
+#define IBMDEM_BK_SMOOTH0   1   /* level-0 red-black half sweeps, all variants */
+#define IBMDEM_BK_SMOOTH    2   /* half sweeps of the coarser levels           */
+#define IBMDEM_BK_RESTRICT  3
+#define IBMDEM_BK_PROLONG   4
+#define IBMDEM_BK_BOTTOM    5   /* one-work-group tails, coarsest-level solves */
+#define IBMDEM_BK_MATVEC    6
+#define IBMDEM_BK_VECTOR    7   /* CG vector updates (p and x, r with r.r, p = z) */
+#define IBMDEM_BK_SCALAR    8   /* CG scalar steps                             */
+int ibmdem_batch_profile(ibmdem_batch *batch, int32_t on);
+int ibmdem_batch_profile_read(const ibmdem_batch *batch, int32_t kind, double *total_ms, int64_t *launches,
+                              double *bytes);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Kernel ids for ibmdem_profile_* (HIP events around every launch of that

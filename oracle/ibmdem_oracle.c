@@ -1845,6 +1845,7 @@ int ibmdem_set_mg_params(ibmdem_ctx *c, const int32_t *mg) {
 struct ibmdem_batch {
     int n;
     ibmdem_ctx **c;
+    int split;   /* kept for ibmdem_batch_get_solve_groups */
 };
 
 int ibmdem_batch_destroy(ibmdem_batch *b) {
@@ -1903,9 +1904,34 @@ int ibmdem_batch_set_fused(ibmdem_batch *b, int32_t on) {
     (void)on;
     return b ? IBMDEM_OK : IBMDEM_ERR_ARG;
 }
-int ibmdem_batch_fused_solves(const ibmdem_batch *b, int64_t *count) {
-    if (!b || !count) return IBMDEM_ERR_ARG;
-    *count = 0;
+int ibmdem_batch_set_solve_groups(ibmdem_batch *b, int32_t groups) {
+    if (!b || groups < 1 || groups > 64) return IBMDEM_ERR_ARG;
+    b->split = groups;
+    return IBMDEM_OK;
+}
+int ibmdem_batch_get_solve_groups(const ibmdem_batch *b, int32_t *groups) {
+    if (!b || !groups) return IBMDEM_ERR_ARG;
+    *groups = b->split > 0 ? b->split : 1;
+    return IBMDEM_OK;
+}
+/* no batched solves to profile */
+int ibmdem_batch_profile(ibmdem_batch *b, int32_t on) {
+    (void)on;
+    return b ? IBMDEM_OK : IBMDEM_ERR_ARG;
+}
+int ibmdem_batch_profile_read(const ibmdem_batch *b, int32_t kind, double *total_ms, int64_t *launches,
+                              double *bytes) {
+    if (!b || kind < 1 || kind > 8 || !total_ms || !launches || !bytes) return IBMDEM_ERR_ARG;
+    *total_ms = 0.0;
+    *launches = 0;
+    *bytes = 0.0;
+    return IBMDEM_OK;
+}
+int ibmdem_batch_stats(const ibmdem_batch *b, int64_t *fused_solves, double *ms) {
+    if (!b) return IBMDEM_ERR_ARG;
+    if (fused_solves) *fused_solves = 0;
+    if (ms)
+        for (int i = 0; i < 5; ++i) ms[i] = 0.0;
     return IBMDEM_OK;
 }
 

@@ -32,13 +32,15 @@ def _single(lib, setup, n):
     return out
 
 
-def _batched(lib, setups, n, tune=0, fused=True, mg=None, solves=None):
+def _batched(lib, setups, n, tune=0, fused=True, mg=None, solves=None, groups=1):
     with abi.Batch(lib, [s.case for s in setups]) as b:
         for q, s in enumerate(setups):
             cases.load(b[q], s)
             if mg and q in mg:
                 b[q].set_mg_params(*mg[q])
         b.set_fused(fused)
+        b.set_solve_groups(groups)
+        assert b.solve_groups() == groups
         if tune:
             before = [b[q].download_fields() + _planes(b[q]) for q in range(len(b))]
             b.autotune(tune)
@@ -83,14 +85,15 @@ def _assert_same(got, want):
 
 
 @pytest.mark.gpu
-def test_batch_equals_single_cases_on_the_gpu(gpu_lib):
+@pytest.mark.parametrize("groups", [1, 3])
+def test_batch_equals_single_cases_on_the_gpu(gpu_lib, groups):
     """The solves run batched (one launch per solver operation over all
-    cases) from the second step on; every case still gets exactly what it
-    gets alone."""
+    cases, or over each of `groups` groups) from the second step on; every
+    case still gets exactly what it gets alone."""
     setups = _cases()
     solves = []
-    got = _batched(gpu_lib, setups, 4, solves=solves)
-    assert solves[0] >= 3   # steps 2..4 (the first learns the iteration counts)
+    got = _batched(gpu_lib, setups, 4, solves=solves, groups=groups)
+    assert solves[0] >= 3 * groups   # steps 2..4 (the first learns the iteration counts)
     for s, g in zip(setups, got):
         _assert_same(g, _single(gpu_lib, s, 4))
 
