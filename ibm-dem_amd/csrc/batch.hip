@@ -23,6 +23,7 @@
 // number of solve groups, then, per case, the predictor's tile schedule
 // (case.stencil_tile).
 #include "common.hpp"
+#include "batched.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -33,6 +34,14 @@
 
 namespace {
 
+// The rest of a step after the solve (gauge, projection, the DEM phase,
+// probes) for every member of a group, for one parity of the members (they
+// share it), as one captured graph.
+struct PostPlan {
+    bk::BatchPlan plan;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct BatchGroup {
     std::vector<int> member;          // case indices, ascending
     PoissonBatch *pb = nullptr;
@@ -41,6 +50,10 @@ struct BatchGroup {
     int exec_k = -1;
     bool active = false;              // solves batched this step
     hipEvent_t t0 = nullptr, t1 = nullptr;   // around the solve (GPU time)
+    std::map<int, PostPlan> post;     // by parity key (vcur, pcur, bulk shift)
+    bool post_ran = false;            // this step's post part ran batched
+    const int **flag_ptrs = nullptr;  // device: each member's error flags
+    int *flags_dev = nullptr, *flags_host = nullptr;   // gathered flags (host: pinned)
 };
 
 }  // namespace
@@ -49,6 +62,7 @@ struct ibmdem_batch {
     std::vector<ibmdem_ctx *> ctx;
     std::string err;
     bool fused = true;
+    bool fused_post = true;          // the post part batched too
     std::vector<BatchGroup> groups;
     std::vector<int32_t> plan_key;   // the cases' parameters the groups were planned with
     std::vector<hipEvent_t> ev;      // per case: end of its step's first part
@@ -141,6 +155,13 @@ int restore_state(ibmdem_ctx *c, HostCopy &h) {
 
 void free_groups(ibmdem_batch *b) {
     for (BatchGroup &g : b->groups) {
+        for (auto &kv : g.post) {
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+            kv.second.plan.release();
+        }
+        if (g.flag_ptrs) (void)hipFree(g.flag_ptrs);
+        if (g.flags_dev) (void)hipFree(g.flags_dev);
+        if (g.flags_host) (void)hipHostFree(g.flags_host);
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (g.stream) (void)hipStreamDestroy(g.stream);
         if (g.t0) (void)hipEventDestroy(g.t0);
@@ -167,14 +188,17 @@ int plan_groups(ibmdem_batch *b) {
     std::vector<int32_t> key{b->split};
     for (ibmdem_ctx *c : b->ctx)
         key.insert(key.end(), {poisson_batchable(c) ? 1 : 0, c->cs.precision, c->cs.mg_pre, c->cs.mg_post,
-                               c->cs.mg_coarse_sweeps, c->cs.poisson_max_iter});
+                               c->cs.mg_coarse_sweeps, c->cs.poisson_max_iter, c->cs.couple_interval,
+                               (int32_t)c->graphs.epoch});
     if (key == b->plan_key) return IBMDEM_OK;
     free_groups(b);
-    std::map<std::tuple<int, int, int, int>, std::vector<int>> by;
+    std::map<std::tuple<int, int, int, int, int>, std::vector<int>> by;
     for (int q = 0; q < n; ++q) {
         const ibmdem_ctx *c = b->ctx[q];
         if (poisson_batchable(c))
-            by[std::make_tuple(c->cs.precision, c->cs.mg_pre, c->cs.mg_post, c->cs.poisson_max_iter)].push_back(q);
+            by[std::make_tuple(c->cs.precision, c->cs.mg_pre, c->cs.mg_post, c->cs.poisson_max_iter,
+                               c->cs.couple_interval)]
+                .push_back(q);
     }
     std::vector<std::vector<int>> parts;
     for (auto &kv : by) {
@@ -206,6 +230,18 @@ int plan_groups(ibmdem_batch *b) {
                     hipEventCreate(&g.t0) != hipSuccess || hipEventCreate(&g.t1) != hipSuccess)) {
             rc = IBMDEM_ERR_HIP;
             err = "batch stream or event creation failed";
+        }
+        if (!rc) {
+            std::vector<const int *> fp;
+            for (ibmdem_ctx *c : cs) fp.push_back(c->flags);
+            const size_t nf = cs.size() * FL_COUNT * sizeof(int);
+            if (hipMalloc((void **)&g.flag_ptrs, fp.size() * sizeof(int *)) != hipSuccess ||
+                hipMemcpy(g.flag_ptrs, fp.data(), fp.size() * sizeof(int *), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMalloc((void **)&g.flags_dev, nf) != hipSuccess ||
+                hipHostMalloc((void **)&g.flags_host, nf, hipHostMallocDefault) != hipSuccess) {
+                rc = IBMDEM_ERR_HIP;
+                err = "batch flag buffers: allocation failed";
+            }
         }
         if (rc) {
             b->err = err;
@@ -317,6 +353,92 @@ int group_solve_finish(ibmdem_batch *b, BatchGroup &g) {
     }
 }
 
+// every member's error flags into one array (member i at i * FL_COUNT)
+__global__ __launch_bounds__(64) void k_gather_flags(const int *const *__restrict__ src, int n, int *__restrict__ out) {
+    const int i = blockIdx.x;
+    if (i < n && threadIdx.x < FL_COUNT) out[i * FL_COUNT + threadIdx.x] = src[i][threadIdx.x];
+}
+
+// the members' parity key, or -1 where they differ
+int parity_key(const ibmdem_batch *b, const BatchGroup &g) {
+    int key = -1;
+    for (int q : g.member) {
+        const ibmdem_ctx *c = b->ctx[q];
+        const int k = c->vcur | (c->pcur << 1) | ((c->bulk_pending ? 1 : 0) << 2);
+        if (key >= 0 && k != key) return -1;
+        key = k;
+    }
+    return key;
+}
+
+// The group's post part for the members' current parity: the batched plan
+// (the launches of step_post for every member, in step_post's order) and its
+// captured graph, built at the first use of that parity.
+int post_plan(ibmdem_batch *b, BatchGroup &g, int key, PostPlan **out) {
+    auto it = g.post.find(key);
+    if (it != g.post.end()) {
+        *out = &it->second;
+        return IBMDEM_OK;
+    }
+    PostPlan &pp = g.post[key];
+    std::vector<ibmdem_ctx *> cs;
+    for (int q : g.member) cs.push_back(b->ctx[q]);
+    std::string err;
+    bk::BatchPlan &P = pp.plan;
+    const size_t cases = dem_plan_cases(P, cs);
+    const int ci = cs[0]->cs.couple_interval;
+    int rc = poisson_plan_gauge(P, cs, err);
+    if (!rc) rc = fluid_plan_project(P, cs, err);
+    if (!rc) rc = dem_plan_body_hydro(P, cases, cs, err);
+    for (int s = 0; s < ci && !rc; ++s) rc = dem_plan_substep(P, cases, cs, s == ci - 1, err);
+    if (!rc) rc = dem_plan_probe(P, cases, cs, err);
+    if (!rc) rc = P.finalize(err);
+    hipGraph_t graph = nullptr;
+    if (!rc && hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        rc = IBMDEM_ERR_HIP;
+        err = "batched post part: capture failed to begin";
+    } else if (!rc) {
+        rc = P.run(g.stream, err);
+        const hipError_t e = hipStreamEndCapture(g.stream, &graph);
+        if (!rc && e != hipSuccess) {
+            rc = IBMDEM_ERR_HIP;
+            err = std::string("batched post part capture: ") + hipGetErrorString(e);
+        }
+        if (!rc && hipGraphInstantiate(&pp.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            pp.exec = nullptr;
+            rc = IBMDEM_ERR_HIP;
+            err = "batched post part: graph instantiation failed";
+        }
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+    if (rc) {
+        if (pp.exec) (void)hipGraphExecDestroy(pp.exec);
+        pp.plan.release();
+        g.post.erase(key);
+        return group_error(b, g, rc, err);
+    }
+    *out = &pp;
+    return IBMDEM_OK;
+}
+
+// Queue the group's post part (all members converged), then its flags to
+// the host; the members' host state moves on as step_post would move it.
+int group_post_queue(ibmdem_batch *b, BatchGroup &g, int key) {
+    PostPlan *pp = nullptr;
+    int rc;
+    if ((rc = post_plan(b, g, key, &pp))) return rc;
+    const int n = (int)g.member.size();
+    if (hipGraphLaunch(pp->exec, g.stream) != hipSuccess)
+        return group_error(b, g, IBMDEM_ERR_HIP, "batched post part: graph launch failed");
+    hipLaunchKernelGGL(k_gather_flags, dim3(n), dim3(64), 0, g.stream, g.flag_ptrs, n, g.flags_dev);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(g.flags_host, g.flags_dev, (size_t)n * FL_COUNT * sizeof(int), hipMemcpyDeviceToHost,
+                       g.stream) != hipSuccess)
+        return group_error(b, g, IBMDEM_ERR_HIP, "batched post part: flag copy failed");
+    for (int q : g.member) step_post_batched_state(b->ctx[q]);
+    return IBMDEM_OK;
+}
+
 // `n_steps` batch steps. A case that fails stops there (its context keeps
 // its own message, as a single context's failed step does); every other case
 // still completes the step in progress, so no context is left between its
@@ -372,6 +494,7 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
         }
         lap(t, 1);
         for (BatchGroup &g : b->groups) {
+            g.post_ran = false;
             if (!g.active) continue;
             rc = group_solve_finish(b, g);
             lap(t, 2);
@@ -379,10 +502,25 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
                 for (int q : g.member) if (!failed[q]) fail(q, rc);
                 continue;
             }
+            bool all = true;
             for (size_t i = 0; i < g.member.size(); ++i) {
                 const int q = g.member[i];
-                if (failed[q]) continue;
-                if ((rc = step_phase2_batched(b->ctx[q], poisson_batch_scalars(g.pb, (int)i), &inf[q]))) fail(q, rc);
+                if (!failed[q] && (rc = step_check_batched(b->ctx[q], poisson_batch_scalars(g.pb, (int)i), &inf[q])))
+                    fail(q, rc);
+                all = all && !failed[q];
+            }
+            // the post part batched when every member goes on and they
+            // share their parity; else on each member's own stream
+            const int key = all && b->fused_post ? parity_key(b, g) : -1;
+            if (key >= 0) {
+                if ((rc = group_post_queue(b, g, key))) {
+                    for (int q : g.member) if (!failed[q]) fail(q, rc);
+                } else {
+                    g.post_ran = true;
+                }
+            } else {
+                for (int q : g.member)
+                    if (!failed[q] && (rc = step_post_own(b->ctx[q]))) fail(q, rc);
             }
             lap(t, 3);
         }
@@ -391,8 +529,22 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
             if ((rc = step_phase2(b->ctx[q], &inf[q]))) fail(q, rc);
         }
         lap(t, 3);
+        std::fill(batched.begin(), batched.end(), 0);
+        for (BatchGroup &g : b->groups) {
+            if (!g.post_ran) continue;
+            for (int q : g.member) batched[q] = 1;
+            if (hipStreamSynchronize(g.stream) != hipSuccess) {
+                group_error(b, g, IBMDEM_ERR_HIP, "batched post part: stream synchronization failed");
+                for (int q : g.member) if (!failed[q]) fail(q, IBMDEM_ERR_HIP);
+                continue;
+            }
+            for (size_t i = 0; i < g.member.size(); ++i) {
+                const int q = g.member[i];
+                if (!failed[q] && (rc = dem_flags_status(b->ctx[q], g.flags_host + i * FL_COUNT))) fail(q, rc);
+            }
+        }
         for (int q = 0; q < n; ++q) {
-            if (done[q] || failed[q]) continue;
+            if (batched[q] || done[q] || failed[q]) continue;
             if ((rc = step_phase3(b->ctx[q]))) fail(q, rc);
         }
         lap(t, 4);
@@ -500,10 +652,11 @@ int ibmdem_batch_run(ibmdem_batch *b, int32_t n_steps, ibmdem_solve_info *info) 
 }
 
 int ibmdem_batch_set_fused(ibmdem_batch *b, int32_t on) {
-    if (!b) return IBMDEM_ERR_ARG;
+    if (!b || on < 0 || on > 2) return IBMDEM_ERR_ARG;
     for (BatchGroup &g : b->groups)
         if (g.stream) (void)hipStreamSynchronize(g.stream);
     b->fused = on != 0;
+    b->fused_post = on == 1;
     if (!b->fused) free_groups(b);
     return IBMDEM_OK;
 }

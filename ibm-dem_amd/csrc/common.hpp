@@ -417,6 +417,7 @@ struct StepGraphs {
     CapturedPart pre[4], post[4];
     CapturedPart pre_rhs[4];     // pre without the solve's preconditioning and iterations (batched solve)
     int64_t launches = 0;        // graph launches (for tests)
+    int64_t epoch = 0;           // drop_graphs calls (a batch re-plans when it changes)
 };
 
 struct ibmdem_ctx {
@@ -579,7 +580,12 @@ void drop_graphs(ibmdem_ctx *c);
 // solve and queues the rest of the step (phase 3 as above)
 bool step_batchable(const ibmdem_ctx *c);
 int step_phase1_batched(ibmdem_ctx *c);
-int step_phase2_batched(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info);
+// the solve's report from the batch's copy of the solver scalars
+int step_check_batched(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info);
+// the rest of the step on the context's own stream (its captured graph)
+int step_post_own(ibmdem_ctx *c);
+// the host state after a post part that the batch ran for the context
+void step_post_batched_state(ibmdem_ctx *c);
 // dem.hip: exclusive scan of n counts into start[0..n] (start[n] = total)
 int scan_counts(ibmdem_ctx *c, const int *cnt, int *start, long n);
 // fluid.hip
@@ -591,6 +597,23 @@ int fluid_upload_bc(ibmdem_ctx *c, int reset_planes);
 int fluid_save_uold(ibmdem_ctx *c);
 int fluid_predictor_general(ibmdem_ctx *c, double a, double b, double cc);
 // poisson.hip
+// Batched plans (batched.hpp): each appends the launches of one part of a
+// step, for every context of a group, to a plan.
+namespace bk {
+struct BatchPlan;
+}
+size_t dem_plan_cases(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs);
+int dem_plan_body_hydro(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, std::string &err);
+int dem_plan_substep(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, bool last,
+                     std::string &err);
+int dem_plan_probe(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, std::string &err);
+// the step's error status from a host copy of the context's flags (clears them)
+int dem_flags_status(ibmdem_ctx *c, const int *flags);
+// poisson_gauge and fluid_project(c, 1) of every context (the current
+// parity of each)
+int poisson_plan_gauge(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
+int fluid_plan_project(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
+
 int poisson_setup(ibmdem_ctx *c);
 long poisson_partials(int nx, int ny, int nz);
 int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info);

@@ -334,16 +334,29 @@ int step_phase1_batched(ibmdem_ctx *c) {
     return run_part(c, part);
 }
 
-int step_phase2_batched(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info) {
+int step_check_batched(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info) {
     StepGraphs &G = c->graphs;
     TRY(poisson_check_scalars(c, scal, info));
     if (c->last_cg_iterations > G.cg_k) G.cg_k = c->last_cg_iterations;
-    CapturedPart &post = G.post[c->vcur * 2 + c->pcur];
+    return IBMDEM_OK;
+}
+
+int step_post_own(ibmdem_ctx *c) {
+    CapturedPart &post = c->graphs.post[c->vcur * 2 + c->pcur];
     if (!post.exec || !(post.before == host_state(c))) TRY(capture_part(c, post, [&] { return step_post(c); }));
     return run_part(c, post);
 }
 
+// The host state step_post leaves (fluid_project's pressure swap and the
+// consumed bulk shift; the DEM part changes none, rmax_dirty being clear in
+// a batchable context), for a post part that a batch ran for the context.
+void step_post_batched_state(ibmdem_ctx *c) {
+    c->pcur = 1 - c->pcur;
+    c->bulk_pending = false;
+}
+
 void drop_graphs(ibmdem_ctx *c) {
+    c->graphs.epoch++;
     for (int q = 0; q < 4; ++q) {
         drop_part(c->graphs.pre[q]);
         drop_part(c->graphs.post[q]);

@@ -24,10 +24,13 @@
 //                    AdjustParcelPosition (couplesolverCylinder.F90:44-74,
 //                    ZJS_EVOLUTION_DEM.F90:121-150).
 #include "common.hpp"
+#include "batched.hpp"
 
 #include <cmath>
 
 namespace {
+
+using namespace bk;
 
 constexpr int T = 256;
 constexpr int CHUNK = 4096;
@@ -36,10 +39,10 @@ constexpr int BT = 256;   // threads per body work-group
 // ---------------------------------------------------------------------------
 // binning
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(T) void k_wrap_cells_count(int M, GridDev g, double *__restrict__ mx,
+__device__ __forceinline__ void wrap_cells_count_body(const Blk &kb, int M, GridDev g, double *__restrict__ mx,
                                                         int *__restrict__ mcell, int *__restrict__ bin_count,
                                                         int *__restrict__ flags) {
-    const int m = blockIdx.x * T + threadIdx.x;
+    const int m = kb.bx * T + kb.tx;
     if (m >= M) return;
     const int n[3] = {g.nx, g.ny, g.nz};
     const double dd[3] = {g.dx, g.dy, g.dz};
@@ -76,6 +79,11 @@ __global__ __launch_bounds__(T) void k_wrap_cells_count(int M, GridDev g, double
     mcell[m] = cl;
     atomicAdd(&bin_count[cl], 1);
 }
+__global__ __launch_bounds__(T) void k_wrap_cells_count(int M, GridDev g, double *__restrict__ mx,
+                                                        int *__restrict__ mcell, int *__restrict__ bin_count,
+                                                        int *__restrict__ flags) {
+    wrap_cells_count_body(hw_blk(), M, g, mx, mcell, bin_count, flags);
+}
 
 // sum of the counts of each CHUNK cells (integers: any order is exact), by
 // wavefront shuffles and the T/64 wavefront totals
@@ -84,27 +92,30 @@ __device__ __forceinline__ int wave_sum_int(int v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
-__global__ __launch_bounds__(T) void k_chunk_sum(long n, const int *__restrict__ a, int *__restrict__ out) {
+__device__ __forceinline__ void chunk_sum_body(const Blk &kb, long n, const int *__restrict__ a, int *__restrict__ out) {
     __shared__ int red[T / 64];
-    const long lo = (long)blockIdx.x * CHUNK;
+    const long lo = (long)kb.bx * CHUNK;
     int s = 0;
-    for (long q = lo + threadIdx.x; q < min(n, lo + CHUNK); q += T) s += a[q];
+    for (long q = lo + kb.tx; q < min(n, lo + CHUNK); q += T) s += a[q];
     s = wave_sum_int(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    if ((kb.tx & 63) == 0) red[kb.tx >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (kb.tx == 0) {
         int t = 0;
         for (int w = 0; w < T / 64; ++w) t += red[w];
-        out[blockIdx.x] = t;
+        out[kb.bx] = t;
     }
+}
+__global__ __launch_bounds__(T) void k_chunk_sum(long n, const int *__restrict__ a, int *__restrict__ out) {
+    chunk_sum_body(hw_blk(), n, a, out);
 }
 
 // exclusive scan of the nb chunk sums in place (a[nb] = total), one
 // work-group: 4096 values per pass, four consecutive per thread, a scan
 // across each wavefront by shuffles and the 16 wavefront totals from LDS
-__global__ __launch_bounds__(1024) void k_scan_blocks(int nb, int *__restrict__ a) {
+__device__ __forceinline__ void scan_blocks_body(const Blk &kb, int nb, int *__restrict__ a) {
     __shared__ int wsum[16];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int t = kb.tx, lane = t & 63, w = t >> 6;
     int carry = 0;
     for (int base = 0; base < nb; base += 4096) {
         int v[4], s = 0;
@@ -141,32 +152,35 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int nb, int *__restrict__ 
     }
     if (t == 0) a[nb] = carry;
 }
+__global__ __launch_bounds__(1024) void k_scan_blocks(int nb, int *__restrict__ a) {
+    scan_blocks_body(hw_blk(), nb, a);
+}
 
 // exclusive scan of counts into start (CHUNK cells per block, 16 per
 // thread). Counts and starts cross HBM in coalesced rows through a padded
 // LDS tile (element e at e + e/32, so a thread's 16 consecutive elements
 // fall in distinct banks across the wavefront).
-__global__ __launch_bounds__(T) void k_chunk_scan(long n, const int *__restrict__ cnt, const int *__restrict__ off,
+__device__ __forceinline__ void chunk_scan_body(const Blk &kb, long n, const int *__restrict__ cnt, const int *__restrict__ off,
                                                   int *__restrict__ start, int nb) {
     __shared__ int tile[CHUNK + CHUNK / 32];
     __shared__ int buf[T / 64];
     constexpr int PER = CHUNK / T;
-    const long base = (long)blockIdx.x * CHUNK;
+    const long base = (long)kb.bx * CHUNK;
     for (int q = 0; q < PER; ++q) {
-        const int e = q * T + threadIdx.x;
+        const int e = q * T + kb.tx;
         tile[e + (e >> 5)] = base + e < n ? cnt[base + e] : 0;
     }
     __syncthreads();
     int loc[PER];
     int s = 0;
     for (int q = 0; q < PER; ++q) {
-        const int e = threadIdx.x * PER + q;
+        const int e = kb.tx * PER + q;
         loc[q] = tile[e + (e >> 5)];
         s += loc[q];
     }
     // exclusive prefix of the thread totals: a scan across each wavefront
     // by shuffles, then the T/64 wavefront totals
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = kb.tx & 63, wv = kb.tx >> 6;
     int x = s;
 #pragma unroll
     for (int st = 1; st < 64; st <<= 1) {
@@ -177,40 +191,52 @@ __global__ __launch_bounds__(T) void k_chunk_scan(long n, const int *__restrict_
     __syncthreads();
     int before = 0;
     for (int w = 0; w < T / 64; ++w) before += w < wv ? buf[w] : 0;
-    int pos = off[blockIdx.x] + before + x - s;
+    int pos = off[kb.bx] + before + x - s;
     for (int q = 0; q < PER; ++q) {
-        const int e = threadIdx.x * PER + q;
+        const int e = kb.tx * PER + q;
         tile[e + (e >> 5)] = pos;
         pos += loc[q];
     }
     __syncthreads();
     for (int q = 0; q < PER; ++q) {
-        const int e = q * T + threadIdx.x;
+        const int e = q * T + kb.tx;
         if (base + e < n) start[base + e] = tile[e + (e >> 5)];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) start[n] = off[nb];
+    if (kb.bx == 0 && kb.tx == 0) start[n] = off[nb];
+}
+__global__ __launch_bounds__(T) void k_chunk_scan(long n, const int *__restrict__ cnt, const int *__restrict__ off,
+                                                  int *__restrict__ start, int nb) {
+    chunk_scan_body(hw_blk(), n, cnt, off, start, nb);
 }
 
 // unordered placement; the counts return to zero
-__global__ __launch_bounds__(T) void k_bin_scatter(int M, const int *__restrict__ mcell, const int *__restrict__ start,
+__device__ __forceinline__ void bin_scatter_body(const Blk &kb, int M, const int *__restrict__ mcell, const int *__restrict__ start,
                                                    int *__restrict__ cnt, int *__restrict__ tmp) {
-    const int m = blockIdx.x * T + threadIdx.x;
+    const int m = kb.bx * T + kb.tx;
     if (m >= M) return;
     const int cl = mcell[m];
     const int slot = start[cl] + atomicSub(&cnt[cl], 1) - 1;
     tmp[slot] = m;
 }
+__global__ __launch_bounds__(T) void k_bin_scatter(int M, const int *__restrict__ mcell, const int *__restrict__ start,
+                                                   int *__restrict__ cnt, int *__restrict__ tmp) {
+    bin_scatter_body(hw_blk(), M, mcell, start, cnt, tmp);
+}
 
 // ascending id inside each cell: position = start + #{smaller ids in cell}
-__global__ __launch_bounds__(T) void k_bin_rank(int M, const int *__restrict__ mcell, const int *__restrict__ start,
+__device__ __forceinline__ void bin_rank_body(const Blk &kb, int M, const int *__restrict__ mcell, const int *__restrict__ start,
                                                 const int *__restrict__ tmp, int *__restrict__ items) {
-    const int m = blockIdx.x * T + threadIdx.x;
+    const int m = kb.bx * T + kb.tx;
     if (m >= M) return;
     const int cl = mcell[m];
     const int q0 = start[cl], q1 = start[cl + 1];
     int r = 0;
     for (int q = q0; q < q1; ++q) r += tmp[q] < m ? 1 : 0;
     items[q0 + r] = m;
+}
+__global__ __launch_bounds__(T) void k_bin_rank(int M, const int *__restrict__ mcell, const int *__restrict__ start,
+                                                const int *__restrict__ tmp, int *__restrict__ items) {
+    bin_rank_body(hw_blk(), M, mcell, start, tmp, items);
 }
 
 // ---------------------------------------------------------------------------
@@ -267,23 +293,26 @@ __global__ __launch_bounds__(T) void k_body_rmax(int M, int B, GridDev g, const 
 // reach is missed.
 constexpr int NEAR_CH = 64;
 constexpr int NEAR_MAX = 32;
-__global__ __launch_bounds__(64) void k_body_near_clear(int B, int *__restrict__ near) {
-    const int b = blockIdx.x * 64 + threadIdx.x;
+__device__ __forceinline__ void body_near_clear_body(const Blk &kb, int B, int *__restrict__ near) {
+    const int b = kb.bx * 64 + kb.tx;
     if (b < B) near[b] = 0;
 }
-__global__ __launch_bounds__(64) void k_body_near(int B, GridDev g, const double *__restrict__ bx,
+__global__ __launch_bounds__(64) void k_body_near_clear(int B, int *__restrict__ near) {
+    body_near_clear_body(hw_blk(), B, near);
+}
+__device__ __forceinline__ void body_near_body(const Blk &kb, int B, GridDev g, const double *__restrict__ bx,
                                                   const double *__restrict__ rmax, double lim, int *__restrict__ near,
                                                   int *__restrict__ list) {
     __shared__ double sx[NEAR_CH], sy[NEAR_CH], sz[NEAR_CH], sr[NEAR_CH];
-    const int c0 = blockIdx.y * NEAR_CH, nc = min(NEAR_CH, B - c0);
-    for (int q = threadIdx.x; q < nc; q += 64) {
+    const int c0 = kb.by * NEAR_CH, nc = min(NEAR_CH, B - c0);
+    for (int q = kb.tx; q < nc; q += 64) {
         sx[q] = bx[c0 + q];
         sy[q] = bx[(long)B + c0 + q];
         sz[q] = bx[2L * B + c0 + q];
         sr[q] = rmax[c0 + q];
     }
     __syncthreads();
-    const int b = blockIdx.x * 64 + threadIdx.x;
+    const int b = kb.bx * 64 + kb.tx;
     if (b >= B) return;
     const double xb0 = bx[b], xb1 = bx[(long)B + b], xb2 = bx[2L * B + b];
     const double rb = rmax[b] + lim + 1e-6 * (g.dx + lim);
@@ -303,10 +332,15 @@ __global__ __launch_bounds__(64) void k_body_near(int B, GridDev g, const double
         }
     }
 }
+__global__ __launch_bounds__(64) void k_body_near(int B, GridDev g, const double *__restrict__ bx,
+                                                  const double *__restrict__ rmax, double lim, int *__restrict__ near,
+                                                  int *__restrict__ list) {
+    body_near_body(hw_blk(), B, g, bx, rmax, lim, near, list);
+}
 
 // fcol of marker i (ZJS_COLLISION_DEM.F90:31-72, 93-146)
 template <bool PAIRS>
-__global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *__restrict__ mx,
+__device__ __forceinline__ void contact_body(const Blk &kb, int M, GridDev g, const double *__restrict__ mx,
                                                const double *__restrict__ mv, const double *__restrict__ mmass,
                                                const double *__restrict__ mrad, const int *__restrict__ mbody,
                                                const int *__restrict__ mcell, const int *__restrict__ bin_start,
@@ -316,7 +350,7 @@ __global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *_
                                                int *__restrict__ pshift, const int *__restrict__ bnear,
                                                const int *__restrict__ blist, const double *__restrict__ bx, int B,
                                                const double *__restrict__ brmax, double lim) {
-    const int i = blockIdx.x * T + threadIdx.x;
+    const int i = kb.bx * T + kb.tx;
     if (i >= M) return;
     const int cl = mcell[i];
     const int ci = cl % g.nx + 1, cj = (cl / g.nx) % g.ny + 1, ck = cl / (g.nx * g.ny) + 1;
@@ -447,6 +481,19 @@ __global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *_
     fcol[(long)M + i] = f1;
     fcol[2L * M + i] = f2;
 }
+template <bool PAIRS>
+__global__ __launch_bounds__(T) void k_contact(int M, GridDev g, const double *__restrict__ mx,
+                                               const double *__restrict__ mv, const double *__restrict__ mmass,
+                                               const double *__restrict__ mrad, const int *__restrict__ mbody,
+                                               const int *__restrict__ mcell, const int *__restrict__ bin_start,
+                                               const int *__restrict__ bin_items, double *__restrict__ fcol,
+                                               ContactParams cp, int max_pairs, int *__restrict__ flags,
+                                               int cap, int *__restrict__ pcount, int *__restrict__ ppartner,
+                                               int *__restrict__ pshift, const int *__restrict__ bnear,
+                                               const int *__restrict__ blist, const double *__restrict__ bx, int B,
+                                               const double *__restrict__ brmax, double lim) {
+    contact_body<PAIRS>(hw_blk(), M, g, mx, mv, mmass, mrad, mbody, mcell, bin_start, bin_items, fcol, cp, max_pairs, flags, cap, pcount, ppartner, pshift, bnear, blist, bx, B, brmax, lim);
+}
 
 // ---------------------------------------------------------------------------
 // rigid bodies
@@ -557,17 +604,17 @@ __device__ void evolve_body(const BodyState &s, int B, int b, double dtD, double
 // mode 0: fcol, fmomcol only; 1: all four sums (AssembleLocalForceMS);
 // 2: all four + CloudEvolutionMS hydro terms (couplesolverCylinder.F90:194-211)
 // integrate: EvolutionMS after the sums.
-__global__ __launch_bounds__(BT) void k_body_reduce(int M, int B, GridDev g, const int *__restrict__ bstart,
+__device__ __forceinline__ void body_reduce_body(const Blk &kb, int M, int B, GridDev g, const int *__restrict__ bstart,
                                                     const double *__restrict__ mx, const double *__restrict__ fdrag,
                                                     const double *__restrict__ fcol, const double *__restrict__ mvol,
                                                     BodyState s, int mode, int integrate, double rhof, double dt,
                                                     double dtD, double g0, double g1, double g2) {
     __shared__ double red[12][BT];
-    const int b = blockIdx.x;
+    const int b = kb.bx;
     const int m0 = bstart[b], m1 = bstart[b + 1];
     double acc[12];
     for (int q = 0; q < 12; ++q) acc[q] = 0.0;
-    for (int m = m0 + threadIdx.x; m < m1; m += BT) {
+    for (int m = m0 + kb.tx; m < m1; m += BT) {
         const double xm[3] = {mx[m], mx[(long)M + m], mx[2L * M + m]};
         double xb[3], rpc[3], fv[3], fc[3], mt[3], mc[3];
         body_image(g, s.x, B, b, xm, xb);
@@ -589,14 +636,14 @@ __global__ __launch_bounds__(BT) void k_body_reduce(int M, int B, GridDev g, con
             }
         }
     }
-    for (int q = 0; q < 12; ++q) red[q][threadIdx.x] = acc[q];
+    for (int q = 0; q < 12; ++q) red[q][kb.tx] = acc[q];
     __syncthreads();
     for (int st = BT / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st)
-            for (int q = 0; q < 12; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+        if (kb.tx < st)
+            for (int q = 0; q < 12; ++q) red[q][kb.tx] += red[q][kb.tx + st];
         __syncthreads();
     }
-    if (threadIdx.x != 0) return;
+    if (kb.tx != 0) return;
     for (int d = 0; d < 3; ++d) {
         s.fcol[(long)d * B + b] = red[3 + d][0];
         s.fmomcol[(long)d * B + b] = red[9 + d][0];
@@ -630,6 +677,13 @@ __global__ __launch_bounds__(BT) void k_body_reduce(int M, int B, GridDev g, con
         evolve_body(s, B, b, dtD, rhof, grav);
     }
 }
+__global__ __launch_bounds__(BT) void k_body_reduce(int M, int B, GridDev g, const int *__restrict__ bstart,
+                                                    const double *__restrict__ mx, const double *__restrict__ fdrag,
+                                                    const double *__restrict__ fcol, const double *__restrict__ mvol,
+                                                    BodyState s, int mode, int integrate, double rhof, double dt,
+                                                    double dtD, double g0, double g1, double g2) {
+    body_reduce_body(hw_blk(), M, B, g, bstart, mx, fdrag, fcol, mvol, s, mode, integrate, rhof, dt, dtD, g0, g1, g2);
+}
 
 __global__ void k_body_integrate(int B, BodyState s, double dtD, double rhof, double g0, double g1, double g2) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -639,10 +693,10 @@ __global__ void k_body_integrate(int B, BodyState s, double dtD, double rhof, do
 }
 
 // ResetVelocityAndPositionLagrangianPoints + AdjustParcelPosition
-__global__ __launch_bounds__(T) void k_follow(int M, int B, GridDev g, double *__restrict__ mx,
+__device__ __forceinline__ void follow_body(const Blk &kb, int M, int B, GridDev g, double *__restrict__ mx,
                                               double *__restrict__ mv, const int *__restrict__ mbody, BodyState s,
                                               double dtD) {
-    const int m = blockIdx.x * T + threadIdx.x;
+    const int m = kb.bx * T + kb.tx;
     if (m >= M) return;
     const int b = mbody[m];
     double xm[3] = {mx[m], mx[(long)M + m], mx[2L * M + m]};
@@ -684,6 +738,11 @@ __global__ __launch_bounds__(T) void k_follow(int M, int B, GridDev g, double *_
         if (fabs(x - g.L[d]) < e[d]) x = g.L[d] + e[d];
         mx[(long)d * M + m] = x;
     }
+}
+__global__ __launch_bounds__(T) void k_follow(int M, int B, GridDev g, double *__restrict__ mx,
+                                              double *__restrict__ mv, const int *__restrict__ mbody, BodyState s,
+                                              double dtD) {
+    follow_body(hw_blk(), M, B, g, mx, mv, mbody, s, dtD);
 }
 
 BodyState body_state(ibmdem_ctx *c) {
@@ -842,14 +901,7 @@ int dem_substep_fused(ibmdem_ctx *c, bool last) {
 int dem_check_flags(ibmdem_ctx *c) {
     HIPX(c, hipMemcpyAsync(c->flags_host, c->flags, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPX(c, hipStreamSynchronize(c->stream));
-    const int *f = c->flags_host;
-    if (!(f[FL_NAN] || f[FL_OUTSIDE] || f[FL_CAPACITY] || f[FL_SPREAD])) return IBMDEM_OK;
-    HIPX(c, hipMemsetAsync(c->flags, 0, FL_COUNT * sizeof(int), c->stream));
-    HIPX(c, hipStreamSynchronize(c->stream));
-    if (f[FL_NAN]) return set_error(c, IBMDEM_ERR_NAN, "non-finite marker position or body state");
-    if (f[FL_OUTSIDE]) return set_error(c, IBMDEM_ERR_OUTSIDE, "a marker left the domain through a wall");
-    if (f[FL_CAPACITY]) return set_error(c, IBMDEM_ERR_CAPACITY, "contact candidates exceed maxCollisionNo");
-    return set_error(c, IBMDEM_ERR_CAPACITY, "IBM spread entry list exceeds its capacity");
+    return dem_flags_status(c, c->flags_host);
 }
 
 int dem_setup(ibmdem_ctx *c) { (void)c; return IBMDEM_OK; }
@@ -865,12 +917,12 @@ int dem_setup(ibmdem_ctx *c) { (void)c; return IBMDEM_OK; }
 constexpr int PROBE_VALUES = 18;
 
 namespace {
-__global__ __launch_bounds__(64) void k_probe_append(int n, int cap, int B, const int *__restrict__ list, BodyState s,
+__device__ __forceinline__ void probe_append_body(const Blk &kb, int n, int cap, int B, const int *__restrict__ list, BodyState s,
                                                      double *__restrict__ ring, long long *__restrict__ count) {
     __shared__ long long slot;
-    if (threadIdx.x == 0) slot = *count % cap;
+    if (kb.tx == 0) slot = *count % cap;
     __syncthreads();
-    for (int q = threadIdx.x; q < n * PROBE_VALUES; q += 64) {
+    for (int q = kb.tx; q < n * PROBE_VALUES; q += 64) {
         const int i = q / PROBE_VALUES, v = q % PROBE_VALUES, b = list[i];
         const int grp = v / 3, d = v % 3;
         const double *src = grp == 0 ? s.x : grp == 1 ? s.v : grp == 2 ? s.om : grp == 3 ? s.fdrag
@@ -878,7 +930,11 @@ __global__ __launch_bounds__(64) void k_probe_append(int n, int cap, int B, cons
         ring[(slot * n + i) * PROBE_VALUES + v] = src[(long)d * B + b];
     }
     __syncthreads();
-    if (threadIdx.x == 0) *count = *count + 1;
+    if (kb.tx == 0) *count = *count + 1;
+}
+__global__ __launch_bounds__(64) void k_probe_append(int n, int cap, int B, const int *__restrict__ list, BodyState s,
+                                                     double *__restrict__ ring, long long *__restrict__ count) {
+    probe_append_body(hw_blk(), n, cap, B, list, s, ring, count);
 }
 }  // namespace
 
@@ -942,4 +998,259 @@ extern "C" int ibmdem_probe_read(ibmdem_ctx *c, int32_t *n_steps, int64_t *first
                                hipMemcpyDeviceToHost, c->stream));
     HIPX(c, hipStreamSynchronize(c->stream));
     return IBMDEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The DEM part of a batched step: every launch of the step's DEM phase
+// (CloudEvolutionMS's hydrodynamic terms, the couple_interval sub-steps,
+// the probe record) as one launch over all cases of a group. A case's
+// arguments are one DemCase in the plan's arena; each launch's table holds
+// the case's grid of blocks and its index there.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DemCase {
+    int M, B, nb, max_pairs;
+    long n;
+    GridDev g;
+    double *mx, *mv;
+    const double *mmass, *mrad, *mvol, *fdrag;
+    const int *mbody, *body_start;
+    int *mcell, *bin_start, *bin_count, *bin_tmp, *bin_items, *flags, *scan_blocks, *bnear, *blist;
+    double *fcol;
+    const double *brmax;
+    BodyState s;
+    ContactParams cp;
+    double lim, rhof, dt, dtD, grav[3];
+    int probe_n, probe_cap;
+    const int *probe_list;
+    double *probe_ring;
+    long long *probe_count;
+};
+
+struct DArg {
+    BGrid g;
+    int q;   // the case's DemCase
+};
+using DTab = BTable<DArg>;
+
+__device__ __forceinline__ BodyState glb_state(const BodyState &s) {
+    return BodyState{glb(s.x), glb(s.v), glb(s.vold), glb(s.om), glb(s.L), glb(s.Lold), glb(s.rot), glb(s.rotg),
+                     glb(s.I), glb(s.m), glb(s.rho), glb(s.fixed), glb(s.fdrag), glb(s.fmom), glb(s.fcol),
+                     glb(s.fmomcol), glb(s.fhydro), glb(s.fmomhydro), glb(s.flags)};
+}
+
+__global__ __launch_bounds__(T) void kb_wrap_cells_count(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        wrap_cells_count_body(kb, c.M, c.g, glb(c.mx), glb(c.mcell), glb(c.bin_count), glb(c.flags));
+    });
+}
+__global__ __launch_bounds__(T) void kb_chunk_sum(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        chunk_sum_body(kb, c.n, glb(c.bin_count), glb(c.scan_blocks));
+    });
+}
+__global__ __launch_bounds__(1024) void kb_scan_blocks(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        scan_blocks_body(kb, c.nb, glb(c.scan_blocks));
+    });
+}
+__global__ __launch_bounds__(T) void kb_chunk_scan(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        chunk_scan_body(kb, c.n, glb(c.bin_count), glb(c.scan_blocks), glb(c.bin_start), c.nb);
+    });
+}
+__global__ __launch_bounds__(T) void kb_bin_scatter(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        bin_scatter_body(kb, c.M, glb(c.mcell), glb(c.bin_start), glb(c.bin_count), glb(c.bin_tmp));
+    });
+}
+__global__ __launch_bounds__(T) void kb_bin_rank(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        bin_rank_body(kb, c.M, glb(c.mcell), glb(c.bin_start), glb(c.bin_tmp), glb(c.bin_items));
+    });
+}
+__global__ __launch_bounds__(64) void kb_body_near_clear(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        body_near_clear_body(kb, c.B, glb(c.bnear));
+    });
+}
+__global__ __launch_bounds__(64) void kb_body_near(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        body_near_body(kb, c.B, c.g, glb((const double *)c.s.x), glb(c.brmax), c.lim, glb(c.bnear), glb(c.blist));
+    });
+}
+__global__ __launch_bounds__(T) void kb_contact(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        contact_body<false>(kb, c.M, c.g, glb((const double *)c.mx), glb((const double *)c.mv), glb(c.mmass),
+                            glb(c.mrad), glb(c.mbody), glb((const int *)c.mcell), glb((const int *)c.bin_start),
+                            glb((const int *)c.bin_items), glb(c.fcol), c.cp, c.max_pairs, glb(c.flags), 0,
+                            (int *)nullptr, (int *)nullptr, (int *)nullptr, glb((const int *)c.bnear),
+                            glb((const int *)c.blist), glb((const double *)c.s.x), c.B, glb(c.brmax), c.lim);
+    });
+}
+// uniform arguments: mode, integrate
+__global__ __launch_bounds__(BT) void kb_body_reduce(DTab tab, const DemCase *cs, int mode, int integrate) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        body_reduce_body(kb, c.M, c.B, c.g, glb(c.body_start), glb((const double *)c.mx), glb(c.fdrag),
+                         glb((const double *)c.fcol), glb(c.mvol), glb_state(c.s), mode, integrate, c.rhof, c.dt,
+                         c.dtD, c.grav[0], c.grav[1], c.grav[2]);
+    });
+}
+__global__ __launch_bounds__(T) void kb_follow(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        follow_body(kb, c.M, c.B, c.g, glb(c.mx), glb(c.mv), glb(c.mbody), glb_state(c.s), c.dtD);
+    });
+}
+__global__ __launch_bounds__(64) void kb_probe_append(DTab tab, const DemCase *cs) {
+    batch_for(tab, [&](const DArg &a, const Blk &kb) {
+        const DemCase &c = *cst(cs + a.q);
+        probe_append_body(kb, c.probe_n, c.probe_cap, c.B, glb(c.probe_list), glb_state(c.s), glb(c.probe_ring),
+                          glb(c.probe_count));
+    });
+}
+
+#define DEM_LAUNCHER(name, threads)                                                                   \
+    void launch_##name(const BOp &o, hipStream_t s, int) {                                           \
+        hipLaunchKernelGGL(kb_##name, dim3(o.blocks), dim3(threads), 0, s, table<DArg>(o),            \
+                           (const DemCase *)o.aux);                                                  \
+    }
+DEM_LAUNCHER(wrap_cells_count, T)
+DEM_LAUNCHER(chunk_sum, T)
+DEM_LAUNCHER(scan_blocks, 1024)
+DEM_LAUNCHER(chunk_scan, T)
+DEM_LAUNCHER(bin_scatter, T)
+DEM_LAUNCHER(bin_rank, T)
+DEM_LAUNCHER(body_near_clear, 64)
+DEM_LAUNCHER(body_near, 64)
+DEM_LAUNCHER(contact, T)
+DEM_LAUNCHER(follow, T)
+DEM_LAUNCHER(probe_append, 64)
+#undef DEM_LAUNCHER
+void launch_body_reduce(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_body_reduce, dim3(o.blocks), dim3(BT), 0, s, table<DArg>(o), (const DemCase *)o.aux,
+                       o.slot, o.op);
+}
+
+// one launch over the cases for which grid(c) is not empty
+template <class G>
+int dem_op(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, BLaunch fn, int threads, G grid,
+           BOp o, std::string &err) {
+    std::vector<DArg> args;
+    for (int q = 0; q < (int)cs.size(); ++q) {
+        const dim3 gr = grid(cs[q]);
+        if (gr.x == 0 || gr.y == 0 || gr.z == 0) continue;
+        args.push_back(DArg{bgrid(gr, dim3(threads)), q});
+    }
+    o.aux_off = cases;
+    return add_op(P.ar, P.ops, fn, threads, args, o, err);
+}
+
+}  // namespace
+
+size_t dem_plan_cases(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs) {
+    std::vector<DemCase> v;
+    for (ibmdem_ctx *c : cs) {
+        DemCase d{};
+        d.M = c->M;
+        d.B = c->B;
+        d.n = c->n;
+        d.nb = div_up(c->n, CHUNK);
+        d.max_pairs = c->cs.max_pairs;
+        d.g = c->g;
+        d.mx = c->mx; d.mv = c->mv;
+        d.mmass = c->mmass; d.mrad = c->mrad; d.mvol = c->mvol; d.fdrag = c->fdrag;
+        d.mbody = c->mbody; d.body_start = c->body_start;
+        d.mcell = c->mcell; d.bin_start = c->bin_start; d.bin_count = c->bin_count; d.bin_tmp = c->bin_tmp;
+        d.bin_items = c->bin_items; d.flags = c->flags; d.scan_blocks = c->scan_blocks;
+        d.bnear = c->bnear; d.blist = c->blist;
+        d.fcol = c->fcol;
+        d.brmax = c->brmax;
+        d.s = body_state(c);
+        d.cp = contact_params(c);
+        d.lim = near_lim(c);
+        d.rhof = c->cs.rhof;
+        d.dt = c->cs.dt;
+        d.dtD = c->cs.dt / c->cs.couple_interval;
+        for (int k = 0; k < 3; ++k) d.grav[k] = c->cs.gravity[k];
+        d.probe_n = c->probe_n; d.probe_cap = c->probe_cap; d.probe_list = c->probe_list;
+        d.probe_ring = c->probe_ring; d.probe_count = c->probe_count;
+        v.push_back(d);
+    }
+    return P.ar.put(v.data(), v.size() * sizeof(DemCase));
+}
+
+// dem_body_hydro of every case
+int dem_plan_body_hydro(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    BOp o;
+    o.slot = 2;   // mode
+    o.op = 0;     // integrate
+    return dem_op(P, cases, cs, launch_body_reduce, BT, [](ibmdem_ctx *c) { return dim3(c->B); }, o, err);
+}
+
+// dem_substep_fused of every case (the contact scan's body culling with the
+// near lists; the caller has made sure no case needs its body radii again)
+int dem_plan_substep(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, bool last,
+                     std::string &err) {
+    int rc;
+    auto mk = [](ibmdem_ctx *c) { return (unsigned)(c->M > 0 ? div_up(c->M, T) : 0); };
+    auto near = [](ibmdem_ctx *c) { return (unsigned)(c->M > 0 && c->B > 0 ? div_up(c->B, 64) : 0); };
+    // dem_contact: dem_near, then the contact scan
+    if ((rc = dem_op(P, cases, cs, launch_body_near_clear, 64, [&](ibmdem_ctx *c) { return dim3(near(c)); },
+                     BOp(), err)))
+        return rc;
+    if ((rc = dem_op(P, cases, cs, launch_body_near, 64,
+                     [&](ibmdem_ctx *c) { return dim3(near(c), near(c) ? div_up(c->B, NEAR_CH) : 0); }, BOp(),
+                     err)))
+        return rc;
+    if ((rc = dem_op(P, cases, cs, launch_contact, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(), err)))
+        return rc;
+    BOp o;
+    o.slot = last ? 1 : 0;   // mode
+    o.op = 1;                // integrate
+    if ((rc = dem_op(P, cases, cs, launch_body_reduce, BT, [](ibmdem_ctx *c) { return dim3(c->B); }, o, err)))
+        return rc;
+    if ((rc = dem_op(P, cases, cs, launch_follow, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(), err)))
+        return rc;
+    // dem_rebuild
+    auto chunks = [](ibmdem_ctx *c) { return dim3(c->M > 0 ? div_up(c->n, CHUNK) : 0); };
+    if ((rc = dem_op(P, cases, cs, launch_wrap_cells_count, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(),
+                     err)))
+        return rc;
+    if ((rc = dem_op(P, cases, cs, launch_chunk_sum, T, chunks, BOp(), err))) return rc;
+    if ((rc = dem_op(P, cases, cs, launch_scan_blocks, 1024, [](ibmdem_ctx *c) { return dim3(c->M > 0 ? 1 : 0); },
+                     BOp(), err)))
+        return rc;
+    if ((rc = dem_op(P, cases, cs, launch_chunk_scan, T, chunks, BOp(), err))) return rc;
+    if ((rc = dem_op(P, cases, cs, launch_bin_scatter, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(), err)))
+        return rc;
+    return dem_op(P, cases, cs, launch_bin_rank, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(), err);
+}
+
+// dem_probe_append of every case that records probes
+int dem_plan_probe(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    return dem_op(P, cases, cs, launch_probe_append, 64, [](ibmdem_ctx *c) { return dim3(c->probe_n > 0 ? 1 : 0); },
+                  BOp(), err);
+}
+
+// the step's error flags as dem_check_flags reports them, from a host copy
+int dem_flags_status(ibmdem_ctx *c, const int *f) {
+    if (!(f[FL_NAN] || f[FL_OUTSIDE] || f[FL_CAPACITY] || f[FL_SPREAD])) return IBMDEM_OK;
+    HIPX(c, hipMemsetAsync(c->flags, 0, FL_COUNT * sizeof(int), c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    if (f[FL_NAN]) return set_error(c, IBMDEM_ERR_NAN, "non-finite marker position or body state");
+    if (f[FL_OUTSIDE]) return set_error(c, IBMDEM_ERR_OUTSIDE, "a marker left the domain through a wall");
+    if (f[FL_CAPACITY]) return set_error(c, IBMDEM_ERR_CAPACITY, "contact candidates exceed maxCollisionNo");
+    return set_error(c, IBMDEM_ERR_CAPACITY, "IBM spread entry list exceeds its capacity");
 }

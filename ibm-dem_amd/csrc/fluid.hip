@@ -6,12 +6,15 @@
 // one z plane; the block index is remapped so that each of the 8 XCDs walks
 // a contiguous slab of z planes (neighbouring planes share its L2).
 #include "common.hpp"
+#include "batched.hpp"
 
 #include <type_traits>
 
 #include <cmath>
 
 namespace {
+
+using namespace bk;
 
 constexpr int BX = 64, BY = 4;
 constexpr int RED_BLOCKS = 1024;   // fixed: reductions never depend on timing
@@ -28,6 +31,12 @@ __device__ __forceinline__ void xcd_block(int &bx, int &by, int &bz) {
     bx = (int)(b % gx);
     by = (int)((b / gx) % gy);
     bz = (int)(b / ((long)gx * gy));
+}
+// the hardware block with xcd_block's coordinates
+__device__ __forceinline__ Blk xcd_blk() {
+    Blk kb = hw_blk();
+    xcd_block(kb.bx, kb.by, kb.bz);
+    return kb;
 }
 
 using Map3 = AxisMap<-1, 3>;
@@ -676,14 +685,12 @@ __global__ __launch_bounds__(BX *BY) void k_divergence(GridDev g, const double *
 // OPEN: the thread of the last cell before an outflow side corrects the
 // outflow plane by the same formula, with the ghost pressures -P and -P_new.
 template <bool OPEN>
-__global__ __launch_bounds__(BX *BY) void k_project(GridDev g, double *u, double *v, double *w,
+__device__ __forceinline__ void project_body(const Blk &kb, GridDev g, double *u, double *v, double *w,
                                                     const double *__restrict__ P, const double *__restrict__ Pc,
                                                     double *__restrict__ Pn, const double *__restrict__ scal,
                                                     int shifted, double dt, double cRK, Divisor ddx, Divisor ddy,
                                                     Divisor ddz, Divisor drho) {
-    int bx, by, bz;
-    xcd_block(bx, by, bz);
-    const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
+    const int i = kb.bx * BX + kb.tx + 1, j = kb.by * BY + kb.ty + 1, k = kb.bz + 1;
     if (i > g.nx || j > g.ny) return;
     const long o = cidx(g, i, j, k);
     double *vel[3] = {u, v, w};
@@ -730,6 +737,14 @@ __global__ __launch_bounds__(BX *BY) void k_project(GridDev g, double *u, double
             g.plane[c][po] = us + cRK * Pold_grad * dt - dv(dv(cRK * (png - pn0), d[c]) * dt, drho);
         }
     }
+}
+template <bool OPEN>
+__global__ __launch_bounds__(BX *BY) void k_project(GridDev g, double *u, double *v, double *w,
+                                                    const double *__restrict__ P, const double *__restrict__ Pc,
+                                                    double *__restrict__ Pn, const double *__restrict__ scal,
+                                                    int shifted, double dt, double cRK, Divisor ddx, Divisor ddy,
+                                                    Divisor ddz, Divisor drho) {
+    project_body<OPEN>(xcd_blk(), g, u, v, w, P, Pc, Pn, scal, shifted, dt, cRK, ddx, ddy, ddz, drho);
 }
 
 // upload-time Boundary: the low boundary faces take their fixed values (zero
@@ -1017,4 +1032,56 @@ int fluid_upload_bc(ibmdem_ctx *c, int reset_planes) {
                        c->v[c->vcur], c->w[c->vcur], reset_planes);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fluid_project(c, 1) of every context of a group as one launch per kind
+// (closed, open boundaries); each case's pointers follow its own parity.
+// ---------------------------------------------------------------------------
+namespace {
+struct ProjArg {
+    BGrid g;
+    GridDev grid;
+    double *u, *v, *w, *Pn;
+    const double *P, *Pc, *scal;
+    int shifted;
+    double dt;
+    Divisor ddx, ddy, ddz, drho;
+};
+template <bool OPEN>
+__global__ __launch_bounds__(BX *BY) void kb_project(BTable<ProjArg> tab) {
+    batch_for(tab, [&](const ProjArg &a, const Blk &kb) {
+        GridDev g = a.grid;
+        for (int d = 0; d < 3; ++d) g.plane[d] = glb(g.plane[d]);
+        project_body<OPEN>(kb, g, glb(a.u), glb(a.v), glb(a.w), glb(a.P), glb(a.Pc), glb(a.Pn), glb(a.scal),
+                           a.shifted, a.dt, 1.0, a.ddx, a.ddy, a.ddz, a.drho);
+    });
+}
+template <bool OPEN>
+void launch_project(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_project<OPEN>, dim3(o.blocks), dim3(BX * BY), 0, s, table<ProjArg>(o));
+}
+}  // namespace
+
+int fluid_plan_project(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    std::vector<ProjArg> open, closed;
+    for (ibmdem_ctx *c : cs) {
+        const GridDev &g = c->g;
+        ProjArg a{};
+        a.g = bgrid(field_grid(g), dim3(BX, BY));
+        a.grid = g;
+        a.u = c->u[c->vcur]; a.v = c->v[c->vcur]; a.w = c->w[c->vcur];
+        a.P = c->P[c->pcur];
+        a.Pc = c->pcorr;
+        a.Pn = c->P[1 - c->pcur];
+        a.scal = c->scal;
+        a.shifted = c->bulk_pending ? 1 : 0;
+        a.dt = c->cs.dt;
+        a.ddx = divisor(g.dx); a.ddy = divisor(g.dy); a.ddz = divisor(g.dz);
+        a.drho = divisor(c->cs.rhof);
+        (grid_open(g) ? open : closed).push_back(a);
+    }
+    int rc;
+    if ((rc = add_op(P.ar, P.ops, launch_project<false>, BX * BY, closed, BOp(), err))) return rc;
+    return add_op(P.ar, P.ops, launch_project<true>, BX * BY, open, BOp(), err);
 }

@@ -28,6 +28,7 @@
 // Reductions have a fixed association (fixed block counts, LDS trees), so
 // results are run-to-run identical.
 #include "common.hpp"
+#include "batched.hpp"
 
 #include <cmath>
 #include <cstdlib>
@@ -37,6 +38,8 @@
 
 namespace {
 
+using namespace bk;
+
 constexpr int RB = 256;           // threads per block for level kernels
 constexpr int RED_BLOCKS = 1024;
 constexpr int RED_T = 256;
@@ -45,21 +48,6 @@ constexpr int COARSE_T = 1024;
 struct LaunchShape {
     dim3 grid, block;
 };
-
-// Block coordinates a kernel body works with: the hardware's in a launch for
-// one case (hw_blk), or the case's own in a launch over a batch of cases,
-// where each case's blocks are a range of one flattened grid and its 2-D
-// block shape is laid over the launch's 1-D blocks (see k_batched below).
-struct Blk {
-    int bx, by, bz;   // block index
-    int gx, gy, gz;   // grid dimensions
-    int tx, ty;       // thread index in the block
-    int dx, dy;       // block dimensions
-};
-__device__ __forceinline__ Blk hw_blk() {
-    return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z,
-               (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.x, (int)blockDim.y};
-}
 
 // Barrier of one wavefront over LDS: its LDS operations complete in program
 // order, so a compiler-level fence suffices.
@@ -1164,15 +1152,15 @@ __global__ __launch_bounds__(RED_T) void k_convert(long n, TD *__restrict__ d, c
 // field order (no shift with an outflow side, where P' = 0 on the face), x
 // taking the last completed CG iteration's deferred update x + alpha p
 // (thread per x-pair of a row, blocks of shape_for over the x-pairs)
-__global__ __launch_bounds__(RB) void k_gauge(LevelDev L, const double *__restrict__ x,
+__device__ __forceinline__ void gauge_body(const Blk &kb, const LevelDev &L, const double *__restrict__ x,
                                               const double *__restrict__ p, const double *__restrict__ scal,
                                               double *__restrict__ pc) {
     const int done = (int)scal[SC_DONE];
     const double alpha = done > 0 ? scal[SC_ALPHA] : 0.0;
     auto xv = [&](long q) { return done > 0 ? x[q] + alpha * p[q] : x[q]; };
     const double x0 = L.dirichlet ? 0.0 : xv(0);
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    const int ip = kb.bx * kb.dx + kb.tx;
+    const int j = kb.by * kb.dy + kb.ty, k = kb.bz;
     if (j >= L.ny || 2 * ip >= L.nx) return;
     // both cells' values are loaded before the stores (the second from the
     // first's place when the row ends at an odd length)
@@ -1181,6 +1169,11 @@ __global__ __launch_bounds__(RB) void k_gauge(LevelDev L, const double *__restri
     const long f0 = ((long)k * L.ny + j) * L.nx + 2 * ip;
     pc[f0] = v0 - x0;
     if (two) pc[f0 + 1] = v1 - x0;
+}
+__global__ __launch_bounds__(RB) void k_gauge(LevelDev L, const double *__restrict__ x,
+                                              const double *__restrict__ p, const double *__restrict__ scal,
+                                              double *__restrict__ pc) {
+    gauge_body(hw_blk(), L, x, p, scal, pc);
 }
 
 int stream_blocks(long n) { return (int)std::min<long>(4096, std::max<long>(1, (n + RED_T - 1) / RED_T)); }
@@ -1614,103 +1607,6 @@ int poisson_solve(ibmdem_ctx *c, ibmdem_solve_info *info) {
 // ---------------------------------------------------------------------------
 namespace {
 
-struct BGrid {
-    int gx, gy, gz;   // the case's grid of blocks
-    int dx, dy;       // its block shape: dx a power of two, dx * dy the launch's block size
-    int lgdx;         // log2(dx)
-    int pb;           // physical blocks of the case in the launch (each takes every pb-th block)
-};
-
-BGrid bgrid(dim3 grid, dim3 block) {
-    int lg = 0;
-    while ((1 << lg) < (int)block.x) ++lg;
-    return BGrid{(int)grid.x, (int)grid.y, (int)grid.z, (int)block.x, (int)block.y, lg,
-                 (int)(grid.x * grid.y * grid.z)};
-}
-
-// Pointers read from a launch's tables are generic to the compiler, which
-// would then access memory through them with flat instructions (both wait
-// counters, 64-bit vector addresses). Device arrays are global memory, and
-// the tables and level descriptors are read-only while a kernel runs, so
-// they are marked as such: global (AS 1) and constant (AS 4, scalar loads
-// where the address is uniform).
-// (Through an integer: a generic-to-global-to-generic cast pair would be
-// folded away.)
-template <typename T>
-__device__ __forceinline__ T *glb(T *p) {
-    return (T *)((__attribute__((address_space(1))) T *)(uintptr_t)p);
-}
-template <typename T>
-__device__ __forceinline__ const T *cst(const T *p) {
-    return (const T *)((const __attribute__((address_space(4))) T *)(uintptr_t)p);
-}
-
-template <class A>
-struct BTable {
-    const int *first;   // first physical block of each case, ascending
-    const A *arg;       // the case's arguments
-    int ncase;
-};
-
-// Runs f(args, kb) for each of the case's own blocks this block takes. The
-// block's case is the number of cases whose first block is at most
-// blockIdx.x, less one (a wavefront compares 64 first blocks per round).
-// Blocks go to the XCDs round robin (block i to XCD i % 8), and a case with
-// 8 or more blocks starts on XCD 0 with a multiple of 8 physical blocks, so
-// physical block p runs on XCD p % 8: XCD s takes the s-th of 8 contiguous
-// ranges of the case's blocks (whole planes where the case has 8 or more),
-// its physical blocks striding through the range together. Planes and their
-// z neighbours are then read through one XCD's L2, whatever a plane's block
-// count. A barrier separates a physical block's blocks, as the bodies reuse
-// their LDS.
-template <class A, class F>
-__device__ __forceinline__ void batch_for(const BTable<A> &tab, F &&f) {
-    const int bid = blockIdx.x, lane = threadIdx.x & 63;
-    int q = -1;
-    for (int c0 = 0; c0 < tab.ncase; c0 += 64) {
-        const int cq = c0 + lane;
-        const int cnt = __popcll(__ballot(cq < tab.ncase && tab.first[cq] <= bid));
-        q += cnt;
-        if (cnt < 64) break;
-    }
-    q = __builtin_amdgcn_readfirstlane(q);
-    const A &a = *cst(tab.arg + q);
-    const int p = bid - *cst(tab.first + q);
-    const BGrid g = a.g;
-    if (p >= g.pb) return;
-    const int plane = g.gx * g.gy, nb = plane * g.gz;
-    int lb0, hi, step;
-    if (nb >= 8) {
-        const int xs = p & 7, per = g.pb >> 3;
-        int lo;
-        if (g.gz >= 8) {
-            lo = (xs * g.gz >> 3) * plane;
-            hi = ((xs + 1) * g.gz >> 3) * plane;
-        } else {
-            lo = (int)((long)xs * nb >> 3);
-            hi = (int)((long)(xs + 1) * nb >> 3);
-        }
-        lb0 = lo + (p >> 3);
-        step = per;
-    } else {
-        lb0 = p;
-        hi = nb;
-        step = g.pb;
-    }
-    Blk kb;
-    kb.gx = g.gx; kb.gy = g.gy; kb.gz = g.gz;
-    kb.dx = g.dx; kb.dy = g.dy;
-    kb.tx = threadIdx.x & (g.dx - 1);
-    kb.ty = threadIdx.x >> g.lgdx;
-    for (int lb = lb0; lb < hi; lb += step) {
-        kb.bx = lb % g.gx;
-        kb.by = (lb / g.gx) % g.gy;
-        kb.bz = lb / plane;
-        f(a, kb);
-        if (lb + step < hi) __syncthreads();
-    }
-}
-
 template <typename T>
 struct RbArg {
     BGrid g;
@@ -1835,23 +1731,6 @@ __global__ __launch_bounds__(64) void kb_gather_scalars(const double *const *__r
     if (q < ncase && threadIdx.x < SC_COUNT) out[(long)q * SC_COUNT + threadIdx.x] = glb(scal[q])[threadIdx.x];
 }
 
-// One batched launch: its kernel (launched by fn with the uniform arguments
-// below) and its table in the group's device arena.
-struct BOp;
-using BLaunch = void (*)(const BOp &, hipStream_t, int it);
-struct BOp {
-    BLaunch fn = nullptr;
-    size_t first_off = 0, arg_off = 0;   // table offsets in the arena
-    const int *first = nullptr;
-    const void *arg = nullptr;
-    int ncase = 0, blocks = 0;
-    int color = 0, slot = 0, op = 0;
-    bool with_it = false;      // k_final's iteration argument is the iteration's number
-    bool skip_first = false;   // not in iteration 1 (the deferred x update)
-    int kind = 0;              // IBMDEM_BK_* (profiling)
-    double bytes = 0.0;        // algorithmic HBM bytes of the launch (0: not stated)
-};
-
 // Profiling of the batched solve: while armed, run_ops records an event
 // before the first launch and after every launch, so the elapsed time
 // between two events is one launch's run time once the stream runs behind
@@ -1864,11 +1743,6 @@ struct BProf {
     std::vector<double> bytes;
     int n = 0;                          // launches recorded
 };
-template <class A>
-BTable<A> table(const BOp &o) {
-    return BTable<A>{o.first, (const A *)o.arg, o.ncase};
-}
-
 template <typename T, bool DOT, int RES, bool ZI>
 void launch_rb(const BOp &o, hipStream_t s, int) {
     hipLaunchKernelGGL((kb_rb_half<T, DOT, RES, ZI>), dim3(o.blocks), dim3(RB), 0, s, table<RbArg<T>>(o), o.color);
@@ -1908,62 +1782,6 @@ void launch_copy(const BOp &o, hipStream_t s, int) {
 void launch_final(const BOp &o, hipStream_t s, int it) {
     hipLaunchKernelGGL(kb_final, dim3(o.blocks), dim3(FIN_T), 0, s, table<CgArg>(o), o.slot, o.op,
                        o.with_it ? it : 0);
-}
-
-// Host bytes of the tables, copied to the device once the plan is built.
-struct Arena {
-    std::vector<char> bytes;
-    size_t put(const void *p, size_t n) {
-        const size_t off = (bytes.size() + 15) & ~size_t(15);
-        bytes.resize(off + n);
-        std::memcpy(bytes.data() + off, p, n);
-        return off;
-    }
-};
-
-// one launch of `block` threads per block over the cases with arguments
-// args (a case's grid and block shape in args[i].g). A physical block takes
-// up to BLOCKS_PER_PHYS of its case's blocks (fewer where the launch would
-// otherwise have under 4096 physical blocks), so its start (finding its
-// case and loading the case's arguments) is paid once for several.
-constexpr int BLOCKS_PER_PHYS = 8;
-template <class A>
-int add_op(Arena &ar, std::vector<BOp> &ops, BLaunch fn, int block, std::vector<A> args, BOp o,
-           std::string &err) {
-    if (args.empty()) return IBMDEM_OK;
-    long total = 0;
-    for (const A &a : args) total += (long)a.g.gx * a.g.gy * a.g.gz;
-    const int per = (int)std::max<long>(1, std::min<long>(BLOCKS_PER_PHYS, total / 4096));
-    std::vector<int> first(args.size());
-    long tot = 0;
-    for (size_t i = 0; i < args.size(); ++i) {
-        BGrid &g = args[i].g;
-        if (g.dx * g.dy != block || (1 << g.lgdx) != g.dx) {
-            err = "batched launch: a case's block shape does not fill the launch's blocks";
-            return IBMDEM_ERR_ARG;
-        }
-        const long nb = (long)g.gx * g.gy * g.gz;
-        long pb = (nb + per - 1) / per;
-        if (nb >= 8) {
-            // the case starts on XCD 0 and its blocks keep their XCD
-            pb = (pb + 7) & ~7L;
-            tot = (tot + 7) & ~7L;
-        }
-        g.pb = (int)pb;
-        first[i] = (int)tot;
-        tot += pb;
-        if (tot >= (1L << 31)) {
-            err = "batched launch of more than 2^31 blocks";
-            return IBMDEM_ERR_ARG;
-        }
-    }
-    o.fn = fn;
-    o.ncase = (int)args.size();
-    o.blocks = (int)tot;
-    o.first_off = ar.put(first.data(), first.size() * sizeof(int));
-    o.arg_off = ar.put(args.data(), args.size() * sizeof(A));
-    ops.push_back(o);
-    return IBMDEM_OK;
 }
 
 template <typename T>
@@ -2377,4 +2195,32 @@ int poisson_batch_launches(const PoissonBatch *pb, int k) {
     for (int it = 1; it <= k; ++it)
         for (const BOp &o : pb->iter) n += !(o.skip_first && it == 1);
     return n;
+}
+
+// poisson_gauge of every context of a group as one launch
+namespace {
+struct GaugeArg {
+    BGrid g;
+    const LevelDev *L;
+    const double *x, *p, *scal;
+    double *pc;
+};
+__global__ __launch_bounds__(RB) void kb_gauge(BTable<GaugeArg> tab) {
+    batch_for(tab, [&](const GaugeArg &a, const Blk &kb) {
+        gauge_body(kb, *cst(a.L), glb(a.x), glb(a.p), glb(a.scal), glb(a.pc));
+    });
+}
+void launch_gauge(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_gauge, dim3(o.blocks), dim3(RB), 0, s, table<GaugeArg>(o));
+}
+}  // namespace
+
+int poisson_plan_gauge(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    std::vector<GaugeArg> args;
+    for (ibmdem_ctx *c : cs) {
+        const LevelDev &L0 = c->lev64[0].d;
+        const LaunchShape fs = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);
+        args.push_back(GaugeArg{bgrid(fs.grid, fs.block), c->dlev, c->cg_x, c->cg_p, c->scal, c->pcorr});
+    }
+    return add_op(P.ar, P.ops, launch_gauge, RB, args, BOp(), err);
 }

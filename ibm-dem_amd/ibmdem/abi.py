@@ -533,10 +533,10 @@ class Batch:
     def autotune(self, trial_steps: int = 3) -> None:
         self._check(self.lib.ibmdem_batch_autotune(self.h, int(trial_steps)), "ibmdem_batch_autotune")
 
-    def set_fused(self, on: bool) -> None:
-        """Batched Poisson solves (the default) or each case's solve on its
-        own stream."""
-        self._check(self.lib.ibmdem_batch_set_fused(self.h, 1 if on else 0), "ibmdem_batch_set_fused")
+    def set_fused(self, mode) -> None:
+        """0/False: each case's step on its own stream; 1/True (the default):
+        the solve and the post-solve part batched; 2: only the solve."""
+        self._check(self.lib.ibmdem_batch_set_fused(self.h, int(mode)), "ibmdem_batch_set_fused")
 
     def fused_solves(self) -> int:
         """Group solves that have run batched so far."""

@@ -1900,9 +1900,9 @@ int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {
 }
 
 /* the oracle has no batched solve: one case after another either way */
-int ibmdem_batch_set_fused(ibmdem_batch *b, int32_t on) {
-    (void)on;
-    return b ? IBMDEM_OK : IBMDEM_ERR_ARG;
+int ibmdem_batch_set_fused(ibmdem_batch *b, int32_t mode) {
+    if (!b || mode < 0 || mode > 2) return IBMDEM_ERR_ARG;
+    return IBMDEM_OK;
 }
 int ibmdem_batch_set_solve_groups(ibmdem_batch *b, int32_t groups) {
     if (!b || groups < 1 || groups > 64) return IBMDEM_ERR_ARG;

@@ -100,14 +100,44 @@ def test_batch_equals_single_cases_on_the_gpu(gpu_lib, groups):
 
 @pytest.mark.gpu
 def test_batched_solve_equals_per_case_solves(gpu_lib):
-    """ibmdem_batch_set_fused(0) runs each case's solve on its own stream;
-    the batched solve gives the same bits."""
+    """ibmdem_batch_set_fused(0) runs each case's step on its own stream,
+    (2) batches only the solve, (1) the solve and the post-solve part; all
+    three give the same bits."""
     setups = _cases()
     solves = []
-    a = _batched(gpu_lib, setups, 3, fused=False, solves=solves)
+    a = _batched(gpu_lib, setups, 3, fused=0, solves=solves)
     assert solves[0] == 0
-    for x, y in zip(a, _batched(gpu_lib, setups, 3)):
-        _assert_same(x, y)
+    for mode in (2, 1):
+        for x, y in zip(a, _batched(gpu_lib, setups, 3, fused=mode)):
+            _assert_same(x, y)
+
+
+@pytest.mark.gpu
+def test_batched_post_part_with_probes(gpu_lib):
+    """The post-solve part batched with probe rings recorded for some cases:
+    the rings and the state equal stepping each case alone."""
+    setups = _cases()
+    probes = {0: [0, 1], 2: [3]}
+    steps = 4
+    with abi.Batch(gpu_lib, [s.case for s in setups]) as b:
+        for q, s in enumerate(setups):
+            cases.load(b[q], s)
+            if q in probes:
+                b[q].probe_bodies(probes[q], 8)
+        b.run(steps)
+        assert b.fused_solves() >= steps - 1
+        got = [(b[q].download_bodies().x, b[q].probe_read() if q in probes else None) for q in range(len(b))]
+    for q, s in enumerate(setups):
+        with abi.Simulation(gpu_lib, s.case) as one:
+            cases.load(one, s)
+            if q in probes:
+                one.probe_bodies(probes[q], 8)
+            one.run(steps)
+            np.testing.assert_array_equal(got[q][0], one.download_bodies().x)
+            if q in probes:
+                ring_b, ring_1 = got[q][1], one.probe_read()
+                assert ring_b[0] == ring_1[0]
+                np.testing.assert_array_equal(ring_b[1], ring_1[1])
 
 
 @pytest.mark.gpu
