@@ -34,9 +34,11 @@
 
 namespace {
 
-// The rest of a step after the solve (gauge, projection, the DEM phase,
-// probes) for every member of a group, for one parity of the members (they
-// share it), as one captured graph.
+// A batched part of a step for every member of a group, for one parity of
+// the members (they share it), as one captured graph: the forcing loop
+// (keyed by the velocity parity), or the rest of the step after the solve
+// (gauge, projection, the DEM phase, probes; keyed by the velocity and
+// pressure parities and the pending bulk shift).
 struct PostPlan {
     bk::BatchPlan plan;
     hipGraphExec_t exec = nullptr;
@@ -52,6 +54,9 @@ struct BatchGroup {
     hipEvent_t t0 = nullptr, t1 = nullptr;   // around the solve (GPU time)
     std::map<int, PostPlan> post;     // by parity key (vcur, pcur, bulk shift)
     bool post_ran = false;            // this step's post part ran batched
+    std::map<int, PostPlan> mdf;      // by velocity parity
+    bool mdf_now = false;             // this step's forcing loop runs batched
+    hipEvent_t mdf_done = nullptr;
     const int **flag_ptrs = nullptr;  // device: each member's error flags
     int *flags_dev = nullptr, *flags_host = nullptr;   // gathered flags (host: pinned)
 };
@@ -63,9 +68,11 @@ struct ibmdem_batch {
     std::string err;
     bool fused = true;
     bool fused_post = true;          // the post part batched too
+    bool fused_mdf = true;           // the forcing loop batched too
     std::vector<BatchGroup> groups;
     std::vector<int32_t> plan_key;   // the cases' parameters the groups were planned with
     std::vector<hipEvent_t> ev;      // per case: end of its step's first part
+    std::vector<hipEvent_t> ev_a;    // per case: end of the first part's IBM set-up (batched forcing)
     int split = 1;                   // solve groups per set of shared parameters
     // statistics: group solves run batched; GPU ms of those solves; host ms
     // queueing first parts, waiting for solves, queueing second parts,
@@ -155,10 +162,12 @@ int restore_state(ibmdem_ctx *c, HostCopy &h) {
 
 void free_groups(ibmdem_batch *b) {
     for (BatchGroup &g : b->groups) {
-        for (auto &kv : g.post) {
-            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
-            kv.second.plan.release();
-        }
+        for (auto *m : {&g.post, &g.mdf})
+            for (auto &kv : *m) {
+                if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+                kv.second.plan.release();
+            }
+        if (g.mdf_done) (void)hipEventDestroy(g.mdf_done);
         if (g.flag_ptrs) (void)hipFree(g.flag_ptrs);
         if (g.flags_dev) (void)hipFree(g.flags_dev);
         if (g.flags_host) (void)hipHostFree(g.flags_host);
@@ -189,15 +198,15 @@ int plan_groups(ibmdem_batch *b) {
     for (ibmdem_ctx *c : b->ctx)
         key.insert(key.end(), {poisson_batchable(c) ? 1 : 0, c->cs.precision, c->cs.mg_pre, c->cs.mg_post,
                                c->cs.mg_coarse_sweeps, c->cs.poisson_max_iter, c->cs.couple_interval,
-                               (int32_t)c->graphs.epoch});
+                               c->cs.n_mdf, (int32_t)c->graphs.epoch});
     if (key == b->plan_key) return IBMDEM_OK;
     free_groups(b);
-    std::map<std::tuple<int, int, int, int, int>, std::vector<int>> by;
+    std::map<std::tuple<int, int, int, int, int, int>, std::vector<int>> by;
     for (int q = 0; q < n; ++q) {
         const ibmdem_ctx *c = b->ctx[q];
         if (poisson_batchable(c))
             by[std::make_tuple(c->cs.precision, c->cs.mg_pre, c->cs.mg_post, c->cs.poisson_max_iter,
-                               c->cs.couple_interval)]
+                               c->cs.couple_interval, c->cs.n_mdf)]
                 .push_back(q);
     }
     std::vector<std::vector<int>> parts;
@@ -227,7 +236,8 @@ int plan_groups(ibmdem_batch *b) {
         int rc = poisson_batch_create(cs, &g.pb, err);
         if (!rc) poisson_batch_set_profile(g.pb, b->prof);
         if (!rc && (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess ||
-                    hipEventCreate(&g.t0) != hipSuccess || hipEventCreate(&g.t1) != hipSuccess)) {
+                    hipEventCreate(&g.t0) != hipSuccess || hipEventCreate(&g.t1) != hipSuccess ||
+                    hipEventCreateWithFlags(&g.mdf_done, hipEventDisableTiming) != hipSuccess)) {
             rc = IBMDEM_ERR_HIP;
             err = "batch stream or event creation failed";
         }
@@ -249,17 +259,16 @@ int plan_groups(ibmdem_batch *b) {
             return rc;
         }
     }
-    if ((int)b->ev.size() < n) {
-        for (int q = (int)b->ev.size(); q < n; ++q) {
+    for (auto *evs : {&b->ev, &b->ev_a})
+        for (int q = (int)evs->size(); q < n; ++q) {
             hipEvent_t e = nullptr;
             if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
                 b->err = "event creation failed";
                 free_groups(b);
                 return IBMDEM_ERR_HIP;
             }
-            b->ev.push_back(e);
+            evs->push_back(e);
         }
-    }
     b->plan_key = key;
     return IBMDEM_OK;
 }
@@ -371,27 +380,22 @@ int parity_key(const ibmdem_batch *b, const BatchGroup &g) {
     return key;
 }
 
-// The group's post part for the members' current parity: the batched plan
-// (the launches of step_post for every member, in step_post's order) and its
-// captured graph, built at the first use of that parity.
-int post_plan(ibmdem_batch *b, BatchGroup &g, int key, PostPlan **out) {
-    auto it = g.post.find(key);
-    if (it != g.post.end()) {
+// A batched part for the members' current parity (key in `plans`): the
+// plan that fill builds and its captured graph, built at the first use of
+// that parity.
+template <class F>
+int graph_plan(ibmdem_batch *b, BatchGroup &g, std::map<int, PostPlan> &plans, int key, F fill, PostPlan **out) {
+    auto it = plans.find(key);
+    if (it != plans.end()) {
         *out = &it->second;
         return IBMDEM_OK;
     }
-    PostPlan &pp = g.post[key];
+    PostPlan &pp = plans[key];
     std::vector<ibmdem_ctx *> cs;
     for (int q : g.member) cs.push_back(b->ctx[q]);
     std::string err;
     bk::BatchPlan &P = pp.plan;
-    const size_t cases = dem_plan_cases(P, cs);
-    const int ci = cs[0]->cs.couple_interval;
-    int rc = poisson_plan_gauge(P, cs, err);
-    if (!rc) rc = fluid_plan_project(P, cs, err);
-    if (!rc) rc = dem_plan_body_hydro(P, cases, cs, err);
-    for (int s = 0; s < ci && !rc; ++s) rc = dem_plan_substep(P, cases, cs, s == ci - 1, err);
-    if (!rc) rc = dem_plan_probe(P, cases, cs, err);
+    int rc = fill(P, cs, err);
     if (!rc) rc = P.finalize(err);
     hipGraph_t graph = nullptr;
     if (!rc && hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -414,10 +418,48 @@ int post_plan(ibmdem_batch *b, BatchGroup &g, int key, PostPlan **out) {
     if (rc) {
         if (pp.exec) (void)hipGraphExecDestroy(pp.exec);
         pp.plan.release();
-        g.post.erase(key);
+        plans.erase(key);
         return group_error(b, g, rc, err);
     }
     *out = &pp;
+    return IBMDEM_OK;
+}
+
+// the launches of step_post for every member, in step_post's order
+int fill_post(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    const size_t cases = dem_plan_cases(P, cs);
+    const int ci = cs[0]->cs.couple_interval;
+    int rc = poisson_plan_gauge(P, cs, err);
+    if (!rc) rc = fluid_plan_project(P, cs, err);
+    if (!rc) rc = dem_plan_body_hydro(P, cases, cs, err);
+    for (int s = 0; s < ci && !rc; ++s) rc = dem_plan_substep(P, cases, cs, s == ci - 1, err);
+    if (!rc) rc = dem_plan_probe(P, cases, cs, err);
+    return rc;
+}
+
+// the velocity parity the members share, or -1
+int velocity_key(const ibmdem_batch *b, const BatchGroup &g) {
+    int key = -1;
+    for (int q : g.member) {
+        const int k = b->ctx[q]->vcur;
+        if (key >= 0 && k != key) return -1;
+        key = k;
+    }
+    return key;
+}
+
+// Queue the group's forcing loop after its members' IBM set-up; the members'
+// host state moves on as ibm_spread would move it.
+int group_mdf_queue(ibmdem_batch *b, BatchGroup &g, int key) {
+    PostPlan *pp = nullptr;
+    int rc;
+    if ((rc = graph_plan(b, g, g.mdf, key, ibm_plan_mdf, &pp))) return rc;
+    for (int q : g.member)
+        if (hipStreamWaitEvent(g.stream, b->ev_a[q], 0) != hipSuccess)
+            return group_error(b, g, IBMDEM_ERR_HIP, "batched forcing: stream wait failed");
+    if (hipGraphLaunch(pp->exec, g.stream) != hipSuccess || hipEventRecord(g.mdf_done, g.stream) != hipSuccess)
+        return group_error(b, g, IBMDEM_ERR_HIP, "batched forcing: graph launch failed");
+    for (int q : g.member) step_mdf_batched_state(b->ctx[q]);
     return IBMDEM_OK;
 }
 
@@ -426,7 +468,7 @@ int post_plan(ibmdem_batch *b, BatchGroup &g, int key, PostPlan **out) {
 int group_post_queue(ibmdem_batch *b, BatchGroup &g, int key) {
     PostPlan *pp = nullptr;
     int rc;
-    if ((rc = post_plan(b, g, key, &pp))) return rc;
+    if ((rc = graph_plan(b, g, g.post, key, fill_post, &pp))) return rc;
     const int n = (int)g.member.size();
     if (hipGraphLaunch(pp->exec, g.stream) != hipSuccess)
         return group_error(b, g, IBMDEM_ERR_HIP, "batched post part: graph launch failed");
@@ -472,10 +514,21 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
             if (g.active)
                 for (int q : g.member) batched[q] = 1;
         }
+        // the forcing loop batched where the members share their velocity parity
+        std::vector<char> mdf(n, 0);
+        for (BatchGroup &g : b->groups) {
+            g.mdf_now = g.active && b->fused_mdf && velocity_key(b, g) >= 0;
+            if (g.mdf_now)
+                for (int q : g.member) mdf[q] = 1;
+        }
         for (int q = 0; q < n; ++q) {
             ibmdem_ctx *c = b->ctx[q];
             bool d = false;
-            if (batched[q]) {
+            if (mdf[q]) {
+                rc = step_phase1a_batched(c);
+                if (!rc && hipEventRecord(b->ev_a[q], c->stream) != hipSuccess)
+                    rc = set_error(c, IBMDEM_ERR_HIP, "event record failed");
+            } else if (batched[q]) {
                 rc = step_phase1_batched(c);
                 if (!rc && hipEventRecord(b->ev[q], c->stream) != hipSuccess)
                     rc = set_error(c, IBMDEM_ERR_HIP, "event record failed");
@@ -484,6 +537,25 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
             }
             done[q] = d;
             if (rc) fail(q, rc);
+        }
+        for (BatchGroup &g : b->groups) {
+            if (!g.mdf_now) continue;
+            if ((rc = group_mdf_queue(b, g, velocity_key(b, g)))) {
+                for (int q : g.member) if (!failed[q]) fail(q, rc);
+                g.mdf_now = false;
+                g.active = false;
+                continue;
+            }
+            for (int q : g.member) {
+                if (failed[q]) continue;
+                ibmdem_ctx *c = b->ctx[q];
+                if (hipStreamWaitEvent(c->stream, g.mdf_done, 0) != hipSuccess)
+                    rc = set_error(c, IBMDEM_ERR_HIP, "stream wait failed");
+                else rc = step_phase1b_batched(c);
+                if (!rc && hipEventRecord(b->ev[q], c->stream) != hipSuccess)
+                    rc = set_error(c, IBMDEM_ERR_HIP, "event record failed");
+                if (rc) fail(q, rc);
+            }
         }
         for (BatchGroup &g : b->groups) {
             if (!g.active) continue;
@@ -632,6 +704,7 @@ int ibmdem_batch_destroy(ibmdem_batch *b) {
         if (g.stream) (void)hipStreamSynchronize(g.stream);
     free_groups(b);
     for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->ev_a) (void)hipEventDestroy(e);
     for (ibmdem_ctx *c : b->ctx) ibmdem_destroy(c);
     delete b;
     return IBMDEM_OK;
@@ -652,11 +725,12 @@ int ibmdem_batch_run(ibmdem_batch *b, int32_t n_steps, ibmdem_solve_info *info) 
 }
 
 int ibmdem_batch_set_fused(ibmdem_batch *b, int32_t on) {
-    if (!b || on < 0 || on > 2) return IBMDEM_ERR_ARG;
+    if (!b || on < 0 || on > 3) return IBMDEM_ERR_ARG;
     for (BatchGroup &g : b->groups)
         if (g.stream) (void)hipStreamSynchronize(g.stream);
     b->fused = on != 0;
-    b->fused_post = on == 1;
+    b->fused_post = on == 1 || on == 3;
+    b->fused_mdf = on == 1;
     if (!b->fused) free_groups(b);
     return IBMDEM_OK;
 }

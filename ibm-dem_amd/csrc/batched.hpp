@@ -38,13 +38,14 @@ struct BGrid {
     int dx, dy;       // its block shape: dx a power of two, dx * dy the launch's block size
     int lgdx;         // log2(dx)
     int pb;           // physical blocks of the case in the launch (each takes every pb-th block)
+    int ident;        // 1: physical block p is the case's block p (the body maps blocks to XCDs itself)
 };
 
 inline BGrid bgrid(dim3 grid, dim3 block) {
     int lg = 0;
     while ((1 << lg) < (int)block.x) ++lg;
     return BGrid{(int)grid.x, (int)grid.y, (int)grid.z, (int)block.x, (int)block.y, lg,
-                 (int)(grid.x * grid.y * grid.z)};
+                 (int)(grid.x * grid.y * grid.z), 0};
 }
 
 // Pointers read from a launch's tables are generic to the compiler, which
@@ -98,7 +99,11 @@ __device__ __forceinline__ void batch_for(const BTable<A> &tab, F &&f) {
     if (p >= g.pb) return;
     const int plane = g.gx * g.gy, nb = plane * g.gz;
     int lb0, hi, step;
-    if (nb >= 8) {
+    if (g.ident) {
+        lb0 = p;
+        hi = nb;
+        step = g.pb;
+    } else if (nb >= 8) {
         const int xs = p & 7, per = g.pb >> 3;
         int lo;
         if (g.gz >= 8) {
@@ -167,15 +172,18 @@ struct Arena {
 // args (a case's grid and block shape in args[i].g). A physical block takes
 // up to BLOCKS_PER_PHYS of its case's blocks (fewer where the launch would
 // otherwise have under 4096 physical blocks), so its start (finding its
-// case and loading the case's arguments) is paid once for several.
+// case and loading the case's arguments) is paid once for several. With
+// ident, physical block p is the case's block p (every case's blocks start
+// on XCD 0, as in its own launch).
 constexpr int BLOCKS_PER_PHYS = 8;
 template <class A>
 int add_op(Arena &ar, std::vector<BOp> &ops, BLaunch fn, int block, std::vector<A> args, BOp o,
-           std::string &err) {
+           std::string &err, bool ident = false) {
     if (args.empty()) return IBMDEM_OK;
     long total = 0;
     for (const A &a : args) total += (long)a.g.gx * a.g.gy * a.g.gz;
-    const int per = (int)std::max<long>(1, std::min<long>(BLOCKS_PER_PHYS, total / 4096));
+    const int per =
+        ident ? 1 : (int)std::max<long>(1, std::min<long>(BLOCKS_PER_PHYS, total / 4096));
     std::vector<int> first(args.size());
     long tot = 0;
     for (size_t i = 0; i < args.size(); ++i) {
@@ -186,7 +194,10 @@ int add_op(Arena &ar, std::vector<BOp> &ops, BLaunch fn, int block, std::vector<
         }
         const long nb = (long)g.gx * g.gy * g.gz;
         long pb = (nb + per - 1) / per;
-        if (nb >= 8) {
+        g.ident = ident ? 1 : 0;
+        if (ident) {
+            tot = (tot + 7) & ~7L;
+        } else if (nb >= 8) {
             // the case starts on XCD 0 and its blocks keep their XCD
             pb = (pb + 7) & ~7L;
             tot = (tot + 7) & ~7L;

@@ -416,6 +416,7 @@ struct StepGraphs {
     int cg_k = -1;               // CG iterations to capture (-1: not yet known)
     CapturedPart pre[4], post[4];
     CapturedPart pre_rhs[4];     // pre without the solve's preconditioning and iterations (batched solve)
+    CapturedPart pre_a[4], pre_b[4];   // pre_rhs before and after the forcing loop (batched forcing)
     int64_t launches = 0;        // graph launches (for tests)
     int64_t epoch = 0;           // drop_graphs calls (a batch re-plans when it changes)
 };
@@ -580,6 +581,10 @@ void drop_graphs(ibmdem_ctx *c);
 // solve and queues the rest of the step (phase 3 as above)
 bool step_batchable(const ibmdem_ctx *c);
 int step_phase1_batched(ibmdem_ctx *c);
+// pre_rhs without the forcing loop, which the batch runs between the two
+int step_phase1a_batched(ibmdem_ctx *c);
+int step_phase1b_batched(ibmdem_ctx *c);
+void step_mdf_batched_state(ibmdem_ctx *c);
 // the solve's report from the batch's copy of the solver scalars
 int step_check_batched(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info);
 // the rest of the step on the context's own stream (its captured graph)
@@ -613,6 +618,8 @@ int dem_flags_status(ibmdem_ctx *c, const int *flags);
 // parity of each)
 int poisson_plan_gauge(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
 int fluid_plan_project(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
+// the n_mdf iterations of ibm_interpolate and ibm_spread of every context
+int ibm_plan_mdf(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
 
 int poisson_setup(ibmdem_ctx *c);
 long poisson_partials(int nx, int ny, int nz);

@@ -183,18 +183,33 @@ void set_host_state(ibmdem_ctx *c, const HostState &h) {
     c->spread_valid = h.spread_valid; c->rmax_dirty = h.rmax_dirty;
 }
 
-// the step up to the solve's right-hand side (r = b, ||b||)
-int step_pre_rhs(ibmdem_ctx *c) {
+// the step up to the solve's right-hand side (r = b, ||b||), in three
+// parts: the IBM set-up of the step, the multi-direct-forcing loop, then the
+// predictor, bulk forcing, divergence and the solve's right-hand side
+int step_pre_a(ibmdem_ctx *c) {
     TRY(fluid_save_uold(c));
-    TRY(ibm_begin_step(c));
+    return ibm_begin_step(c);
+}
+
+int step_mdf(ibmdem_ctx *c) {
     for (int it = 0; it < c->cs.n_mdf; ++it) {
         TRY(ibm_interpolate(c));
         TRY(ibm_spread(c));
     }
+    return IBMDEM_OK;
+}
+
+int step_pre_b(ibmdem_ctx *c) {
     TRY(fluid_predictor(c, 1.0, 0.0, 1.0));
     TRY(fluid_bulk(c, false));
     TRY(fluid_divergence(c, 1.0));
     return poisson_begin_rhs(c);
+}
+
+int step_pre_rhs(ibmdem_ctx *c) {
+    TRY(step_pre_a(c));
+    TRY(step_mdf(c));
+    return step_pre_b(c);
 }
 
 int step_pre(ibmdem_ctx *c) {
@@ -334,6 +349,28 @@ int step_phase1_batched(ibmdem_ctx *c) {
     return run_part(c, part);
 }
 
+// The same first phase with the forcing loop left to the batch as well:
+// part a before it, part b after it.
+int step_phase1a_batched(ibmdem_ctx *c) {
+    CapturedPart &part = c->graphs.pre_a[c->vcur * 2 + c->pcur];
+    if (!part.exec || !(part.before == host_state(c))) TRY(capture_part(c, part, [&] { return step_pre_a(c); }));
+    return run_part(c, part);
+}
+
+// the host state ibm_spread leaves, for a forcing loop the batch ran
+void step_mdf_batched_state(ibmdem_ctx *c) {
+    if (c->M > 0 && c->cs.n_mdf > 0) {
+        c->first_mdf = false;
+        c->spread_valid = true;
+    }
+}
+
+int step_phase1b_batched(ibmdem_ctx *c) {
+    CapturedPart &part = c->graphs.pre_b[c->vcur * 2 + c->pcur];
+    if (!part.exec || !(part.before == host_state(c))) TRY(capture_part(c, part, [&] { return step_pre_b(c); }));
+    return run_part(c, part);
+}
+
 int step_check_batched(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info) {
     StepGraphs &G = c->graphs;
     TRY(poisson_check_scalars(c, scal, info));
@@ -361,6 +398,8 @@ void drop_graphs(ibmdem_ctx *c) {
         drop_part(c->graphs.pre[q]);
         drop_part(c->graphs.post[q]);
         drop_part(c->graphs.pre_rhs[q]);
+        drop_part(c->graphs.pre_a[q]);
+        drop_part(c->graphs.pre_b[q]);
     }
 }
 

@@ -38,11 +38,14 @@
 // others are skipped (their f is structurally zero in every iteration of
 // the step).
 #include "common.hpp"
+#include "batched.hpp"
 
 #include <algorithm>
 #include <mutex>
 
 namespace {
+
+using namespace bk;
 
 constexpr int T = 256;
 constexpr int CHUNK = 4096;   // flags per compaction block
@@ -199,7 +202,7 @@ __device__ __forceinline__ double tap4(const double *__restrict__ f, const GridD
 // markers of a wavefront are neighbours and share their velocity taps in
 // L1/L2; each marker's arithmetic is independent of the order.
 template <bool P2, bool OPEN>
-__global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__restrict__ order,
+__device__ __forceinline__ void interp_body(const Blk &kb, int M, GridDev g, const int *__restrict__ order,
                                               const double *__restrict__ u,
                                               const double *__restrict__ v, const double *__restrict__ w,
                                               double *__restrict__ rec, const double *__restrict__ mv,
@@ -207,9 +210,9 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__res
                                               double idV, double dt) {
     // consecutive bin-ordered markers (one body's neighbourhood) stay on one
     // XCD, so the field taps they share come from one L2
-    const int slot = tile_slot(blockIdx.x, (M + T - 1) / T);
+    const int slot = tile_slot(kb.bx, (M + T - 1) / T);
     if (slot < 0) return;
-    const int t = slot * T + threadIdx.x;
+    const int t = slot * T + kb.tx;
     if (t >= M) return;
     const int m = order[t];
     double *r = rec + 8L * m;
@@ -333,6 +336,15 @@ __global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__res
     const double vol = mvol[m];
     reinterpret_cast<double2 *>(r)[2] = make_double2(f0 * vol, f1 * vol);
     r[6] = f2 * vol;
+}
+template <bool P2, bool OPEN>
+__global__ __launch_bounds__(T) void k_interp(int M, GridDev g, const int *__restrict__ order,
+                                              const double *__restrict__ u,
+                                              const double *__restrict__ v, const double *__restrict__ w,
+                                              double *__restrict__ rec, const double *__restrict__ mv,
+                                              double *__restrict__ fdrag, const double *__restrict__ mvol, double dV,
+                                              double idV, double dt) {
+    interp_body<P2, OPEN>(hw_blk(), M, g, order, u, v, w, rec, mv, fdrag, mvol, dV, idV, dt);
 }
 
 struct TileGeom {
@@ -790,7 +802,7 @@ __device__ __forceinline__ void tile_epilogue(const SpreadArgs &a, int slot, dou
 }
 
 template <bool P2>
-__global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
+__device__ __forceinline__ void spread_tiles_body(const Blk &kb, const SpreadArgs &a) {
     __shared__ double s_acc[3][ACC_N];         // the tile's force
     __shared__ unsigned s_touch[TC / 32];      // cells some entry reached
     // per staged entry: (face, centre) factor pairs for 3 dimensions x 4
@@ -798,8 +810,8 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
     // support corner and fdrag*vol stay in the staging lane's registers
     __shared__ double2 s_w[CAP][WST2];
     const GridDev &g = a.g;
-    const int lane = threadIdx.x;
-    const int slot = tile_slot(blockIdx.x, a.meta[0]);
+    const int lane = kb.tx;
+    const int slot = tile_slot(kb.bx, a.meta[0]);
     if (slot < 0) return;
     const int elo = a.estart[slot], ehi = a.estart[slot + 1];
     if (ehi > a.ent_cap) return;   // the fill flagged a capacity error
@@ -874,6 +886,10 @@ __global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
         wave_sync();   // the rows are rewritten by the next chunk
     }
     tile_epilogue(a, slot, s_acc, s_touch, i0, j0, k0);
+}
+template <bool P2>
+__global__ __launch_bounds__(64) void k_spread_tiles(SpreadArgs a) {
+    spread_tiles_body<P2>(hw_blk(), a);
 }
 
 __global__ __launch_bounds__(64) void k_scatter_force(const int *__restrict__ meta, const int *__restrict__ tiles,
@@ -1179,6 +1195,126 @@ extern "C" int ibmdem_download_force_noalloc(ibmdem_ctx *c, double *fx, double *
                                        hipMemcpyDeviceToHost, s));
         // the next slab reuses the scratch: its copies must have landed
         HIPX(c, hipStreamSynchronize(s));
+    }
+    return IBMDEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The multi-direct-forcing loop of a batched step: the n_mdf iterations of
+// interpolation and spread of every case of a group, each as one launch per
+// kernel variant (power-of-two spacing, open boundaries) over all cases.
+// Both kernels map their blocks to XCDs themselves (tile_slot), so a case's
+// physical blocks are its own launch's blocks, starting on XCD 0.
+// ---------------------------------------------------------------------------
+namespace {
+struct InterpArg {
+    BGrid g;
+    int M;
+    GridDev grid;
+    const int *order;
+    const double *u, *v, *w;
+    double *rec;
+    const double *mv;
+    double *fdrag;
+    const double *mvol;
+    double dV, idV, dt;
+};
+template <bool P2, bool OPEN>
+__global__ __launch_bounds__(T) void kb_interp(BTable<InterpArg> tab) {
+    batch_for(tab, [&](const InterpArg &a, const Blk &kb) {
+        interp_body<P2, OPEN>(kb, a.M, a.grid, glb(a.order), glb(a.u), glb(a.v), glb(a.w), glb(a.rec), glb(a.mv),
+                              glb(a.fdrag), glb(a.mvol), a.dV, a.idV, a.dt);
+    });
+}
+struct SpreadArg {
+    BGrid g;
+    SpreadArgs s;
+};
+// uniform argument: first (the step's first iteration)
+template <bool P2>
+__global__ __launch_bounds__(64) void kb_spread(BTable<SpreadArg> tab, int first) {
+    batch_for(tab, [&](const SpreadArg &a, const Blk &kb) {
+        SpreadArgs s = a.s;
+        s.meta = glb(s.meta); s.tiles = glb(s.tiles); s.estart = glb(s.estart); s.ent = glb(s.ent);
+        s.rec = glb(s.rec); s.fac = glb(s.fac); s.fact = glb(s.fact); s.mask = glb(s.mask);
+        s.u = glb(s.u); s.v = glb(s.v); s.w = glb(s.w);
+        s.first = first;
+        spread_tiles_body<P2>(kb, s);
+    });
+}
+template <bool P2, bool OPEN>
+void launch_interp(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL((kb_interp<P2, OPEN>), dim3(o.blocks), dim3(T), 0, s, table<InterpArg>(o));
+}
+template <bool P2>
+void launch_spread(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_spread<P2>, dim3(o.blocks), dim3(64), 0, s, table<SpreadArg>(o), o.slot);
+}
+
+SpreadArgs spread_args(ibmdem_ctx *c) {
+    const int cur = c->vcur;
+    SpreadArgs a;
+    a.g = c->g;
+    a.tg = tile_geom(c->g);
+    a.meta = c->act_meta;
+    a.tiles = c->act_list;
+    a.estart = c->tile_estart;
+    a.ent = c->tile_ent;
+    a.ent_cap = c->ent_cap;
+    a.rec = c->mrec;
+    a.fac = c->mfac;
+    a.fact = c->fact;
+    a.fstride = (long)TC * c->act_cap;
+    a.mask = c->tile_mask;
+    a.u = c->u[cur]; a.v = c->v[cur]; a.w = c->w[cur];
+    a.dV = c->dV;
+    a.idV = 1.0 / c->dV;
+    a.dt = c->cs.dt;
+    a.first = c->first_mdf ? 1 : 0;
+    return a;
+}
+}  // namespace
+
+// the n_mdf iterations of ibm_interpolate and ibm_spread of every context
+// with markers (all share n_mdf; each at its current velocity parity)
+int ibm_plan_mdf(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    std::vector<InterpArg> ia[4];   // by (pow2, open)
+    std::vector<SpreadArg> sa[2];   // by pow2
+    for (ibmdem_ctx *c : cs) {
+        if (c->M == 0) continue;
+        const int cur = c->vcur;
+        const bool p2 = pow2_grid(c), open = grid_open(c->g);
+        InterpArg a{};
+        a.g = bgrid(dim3(8 * div_up(div_up(c->M, T), 8)), dim3(T));
+        a.M = c->M;
+        a.grid = c->g;
+        a.order = c->bin_items;
+        a.u = c->u[cur]; a.v = c->v[cur]; a.w = c->w[cur];
+        a.rec = c->mrec;
+        a.mv = c->mv;
+        a.fdrag = c->fdrag;
+        a.mvol = c->mvol;
+        a.dV = c->dV;
+        a.idV = 1.0 / c->dV;
+        a.dt = c->cs.dt;
+        ia[(p2 ? 2 : 0) + (open ? 1 : 0)].push_back(a);
+        SpreadArg b{};
+        b.g = bgrid(dim3(c->act_grid), dim3(64));
+        b.s = spread_args(c);
+        sa[p2 ? 1 : 0].push_back(b);
+    }
+    const BLaunch fi[4] = {launch_interp<false, false>, launch_interp<false, true>, launch_interp<true, false>,
+                           launch_interp<true, true>};
+    const BLaunch fs[2] = {launch_spread<false>, launch_spread<true>};
+    const int n_mdf = cs.empty() ? 0 : cs[0]->cs.n_mdf;
+    int rc;
+    for (int it = 0; it < n_mdf; ++it) {
+        for (int v = 0; v < 4; ++v)
+            if ((rc = add_op(P.ar, P.ops, fi[v], T, ia[v], BOp(), err, true))) return rc;
+        BOp o;
+        o.slot = it == 0 ? 1 : 0;   // first
+        for (int v = 0; v < 2; ++v)
+            if ((rc = add_op(P.ar, P.ops, fs[v], 64, sa[v], o, err, true))) return rc;
     }
     return IBMDEM_OK;
 }

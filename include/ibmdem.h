@@ -285,10 +285,11 @@ int ibmdem_set_stencil_tile(ibmdem_ctx *ctx, int32_t tile);
 /* ---- batched form -----------------------------------------------------------
  * Many independent cases of uneven size advanced together on one GPU, each a
  * full context with its own HIP stream; a batch step interleaves the cases'
- * captured step graphs so their kernels overlap, and runs the Poisson solve
- * and the rest of the step after it for cases that share the precision, the
- * multigrid smoothing counts, the iteration cap and the DEM sub-step count
- * as captured graphs of launches that each cover all of them. Every case's
+ * captured step graphs so their kernels overlap, and runs the forcing loop,
+ * the Poisson solve and the rest of the step after it for cases that share
+ * the precision, the multigrid smoothing counts, the iteration cap, the
+ * forcing iteration count and the DEM sub-step count as captured graphs of
+ * launches that each cover all of them. Every case's
  * result is bitwise that of stepping it alone. No
  * counterpart in the reference, which runs one case per MPI job (main.F90). */
 typedef struct ibmdem_batch ibmdem_batch;
@@ -307,9 +308,10 @@ int ibmdem_batch_run(ibmdem_batch *batch, int32_t n_steps, ibmdem_solve_info *in
  * the state is restored after every trial. */
 int ibmdem_batch_autotune(ibmdem_batch *batch, int32_t trial_steps);
 /* mode 0: every case's step runs on its own stream, as a single context's
- * does (for comparison); 1 (the default): the solve and the rest of the
- * step after it (projection, the DEM phase, probes) run batched; 2: only
- * the solve runs batched. */
+ * does (for comparison); 1 (the default): the multi-direct-forcing loop,
+ * the solve and the rest of the step after it (projection, the DEM phase,
+ * probes) run batched; 2: only the solve runs batched; 3: the solve and the
+ * part after it. */
 int ibmdem_batch_set_fused(ibmdem_batch *batch, int32_t mode);
 /* The cases that share precision, smoothing counts and iteration cap solve
  * in `groups` (1..64, default 1) batched groups of about equal cell count,

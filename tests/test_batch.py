@@ -101,13 +101,13 @@ def test_batch_equals_single_cases_on_the_gpu(gpu_lib, groups):
 @pytest.mark.gpu
 def test_batched_solve_equals_per_case_solves(gpu_lib):
     """ibmdem_batch_set_fused(0) runs each case's step on its own stream,
-    (2) batches only the solve, (1) the solve and the post-solve part; all
-    three give the same bits."""
+    (2) batches only the solve, (3) the solve and the post-solve part, (1)
+    the forcing loop as well; all give the same bits."""
     setups = _cases()
     solves = []
     a = _batched(gpu_lib, setups, 3, fused=0, solves=solves)
     assert solves[0] == 0
-    for mode in (2, 1):
+    for mode in (2, 3, 1):
         for x, y in zip(a, _batched(gpu_lib, setups, 3, fused=mode)):
             _assert_same(x, y)
 
