@@ -181,8 +181,10 @@ def measured_traffic():
 def batched_run(steps, warmup):
     """The batched form on 64 independent settling cases of uneven size
     (grids 32..96 x 32..96 x 64..192, 1..40 spheres of D in [8h, 16h]):
-    mixed precision with autotuned multigrid cycles, timed over `steps`
-    batch steps; the same cases in FP64 give the per-case error."""
+    mixed precision with the autotuned multigrid cycle and solve-group split
+    (the forcing loop, the solve and the post-solve part each run as
+    launches over all cases of a group), timed over `steps` batch steps; the
+    same cases in FP64 give the per-case error."""
     import numpy as np
 
     import ibmdem
@@ -199,21 +201,31 @@ def batched_run(steps, warmup):
             t_tune = time.perf_counter()
             if prec == MIXED:
                 b.autotune(2)
+                groups = b.solve_groups()
+            else:
+                b.set_solve_groups(groups)   # the mixed run's tuned split
             t_tune = time.perf_counter() - t_tune
             b.run(warmup)
             for q in range(len(b)):
                 b[q].sync()
+            n0, ms0 = b.stats()
             t0 = time.perf_counter()
             infos = b.run(steps)
             for q in range(len(b)):
                 b[q].sync()
             secs = time.perf_counter() - t0
+            n1, ms1 = b.stats()
             states[name] = [(b[q].download_fields(), b[q].download_bodies().x) for q in range(len(b))]
             cells = sum(s.cells for s in setups)
             out[name] = {"seconds": secs, "batch_steps_per_s": steps / secs,
                          "gcell_per_s": cells * steps / secs / 1e9,
                          "cg_iterations_mean": float(np.mean([i.iterations for i in infos])),
                          "mg_cycles": sorted({b[q].mg_params() for q in range(len(b))}),
+                         "solve_groups": b.solve_groups(),
+                         "batched_group_solves": n1 - n0,
+                         # GPU ms of the batched solves (summed over groups, which overlap), and
+                         # host ms per phase of a batch step
+                         "per_step_ms": {k: (ms1[k] - ms0[k]) / steps for k in ms1},
                          "autotune_s": t_tune}
             out["cases"] = len(setups)
             out["cells"] = cells
