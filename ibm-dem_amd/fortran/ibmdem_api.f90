@@ -316,6 +316,71 @@ module ibmdem_api
             type(c_ptr), value :: batch
             integer(c_int32_t), value :: trial_steps
         end function
+        integer(c_int) function ibmdem_batch_set_fused(batch, mode) bind(C, name="ibmdem_batch_set_fused")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: batch
+            integer(c_int32_t), value :: mode
+        end function
+        integer(c_int) function ibmdem_batch_set_solve_groups(batch, groups) bind(C, name="ibmdem_batch_set_solve_groups")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: batch
+            integer(c_int32_t), value :: groups
+        end function
+        integer(c_int) function ibmdem_batch_get_solve_groups(batch, groups) bind(C, name="ibmdem_batch_get_solve_groups")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: batch
+            integer(c_int32_t), intent(out) :: groups
+        end function
+        integer(c_int) function ibmdem_batch_stats(batch, fused_solves, ms) bind(C, name="ibmdem_batch_stats")
+            import :: c_int, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: batch
+            integer(c_int64_t), intent(out) :: fused_solves
+            real(c_double), intent(out) :: ms(5)
+        end function
+        integer(c_int) function ibmdem_batch_profile(batch, on) bind(C, name="ibmdem_batch_profile")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: batch
+            integer(c_int32_t), value :: on
+        end function
+        integer(c_int) function ibmdem_batch_profile_read(batch, kind, total_ms, launches, bytes) &
+                bind(C, name="ibmdem_batch_profile_read")
+            import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+            type(c_ptr), value :: batch
+            integer(c_int32_t), value :: kind
+            real(c_double), intent(out) :: total_ms, bytes
+            integer(c_int64_t), intent(out) :: launches
+        end function
+
+        ! ---- predictor schedule, probes, force download without allocation ---------
+        integer(c_int) function ibmdem_get_stencil_tile(ctx, tile) bind(C, name="ibmdem_get_stencil_tile")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), intent(out) :: tile
+        end function
+        integer(c_int) function ibmdem_set_stencil_tile(ctx, tile) bind(C, name="ibmdem_set_stencil_tile")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: tile
+        end function
+        integer(c_int) function ibmdem_probe_bodies(ctx, n, bodies, capacity) bind(C, name="ibmdem_probe_bodies")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: n, capacity
+            integer(c_int32_t), intent(in) :: bodies(*)
+        end function
+        integer(c_int) function ibmdem_probe_read(ctx, n_steps, first_step, records) bind(C, name="ibmdem_probe_read")
+            import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), intent(out) :: n_steps
+            integer(c_int64_t), intent(out) :: first_step
+            real(c_double), intent(inout) :: records(*)
+        end function
+        integer(c_int) function ibmdem_download_force_noalloc(ctx, fx, fy, fz) &
+                bind(C, name="ibmdem_download_force_noalloc")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(inout) :: fx(*), fy(*), fz(*)
+        end function
 
         ! ---- measurement ------------------------------------------------------------
         integer(c_int) function ibmdem_profile_kernel(ctx, kernel_id) bind(C, name="ibmdem_profile_kernel")
