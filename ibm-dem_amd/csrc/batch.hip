@@ -352,8 +352,15 @@ int group_solve_finish(ibmdem_batch *b, BatchGroup &g) {
                 return group_error(b, g, IBMDEM_ERR_HIP, "batched solve profiling: event times unavailable");
             first = false;
         }
+        // a member with a non-finite right-hand side or residual fails its
+        // check whatever further iterations do, so it does not hold the
+        // others' loop
         bool all = true;
-        for (size_t i = 0; i < g.member.size(); ++i) all = all && poisson_batch_scalars(g.pb, (int)i)[SC_CONV] != 0.0;
+        for (size_t i = 0; i < g.member.size(); ++i) {
+            const double *sc = poisson_batch_scalars(g.pb, (int)i);
+            const bool finite = sc[SC_BNORM2] - sc[SC_BNORM2] == 0.0 && sc[SC_RR] == sc[SC_RR];
+            all = all && (sc[SC_CONV] != 0.0 || !finite);
+        }
         if (all || it >= maxit) return IBMDEM_OK;
         ++it;
         int rc = poisson_batch_iteration(g.pb, g.stream, it, err);

@@ -140,3 +140,30 @@ def test_batch_failure_leaves_the_other_cases_complete(lib):
         for a, c in zip(got[q][0], want[0]):
             np.testing.assert_array_equal(a, c)
         np.testing.assert_array_equal(got[q][1], want[1])
+
+
+def test_batch_failure_inside_a_batched_group(lib):
+    """A case that fails in a step whose solve runs batched (a NaN uploaded
+    into its velocity after two good steps): the batch reports ERR_NAN and
+    the other cases of its group complete that step exactly as when they
+    are stepped alone."""
+    setups = _batch_setups()
+    with abi.Batch(lib, [s.case for s in setups]) as b:
+        for q, s in enumerate(setups):
+            cases.load(b[q], s)
+        b.run(2)
+        u, v, w, p = b[1].download_fields()
+        u[3, 4, 5] = np.nan
+        b[1].upload_fields(u, v, w, p)
+        with pytest.raises(IbmdemError) as e:
+            b.run(1)
+        assert e.value.code == ERR_NAN
+        got = {q: (b[q].download_fields(), b[q].download_markers().x) for q in (0, 2)}
+    for q in (0, 2):
+        with Simulation(lib, setups[q].case) as s:
+            cases.load(s, setups[q])
+            s.run(3)
+            want = (s.download_fields(), s.download_markers().x)
+        for a, c in zip(got[q][0], want[0]):
+            np.testing.assert_array_equal(a, c)
+        np.testing.assert_array_equal(got[q][1], want[1])
