@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=256, help="grid edge (256 is the workload)")
     ap.add_argument("--spheres", type=int, default=2000)
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="also time the batched solve's launches by kind (ibmdem_batch_profile)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched form (64 uneven settling cases)")
     ap.add_argument("--no-configs", action="store_true",
@@ -178,7 +180,7 @@ def measured_traffic():
     return out
 
 
-def batched_run(steps, warmup):
+def batched_run(steps, warmup, profile=False):
     """The batched form on 64 independent settling cases of uneven size
     (grids 32..96 x 32..96 x 64..192, 1..40 spheres of D in [8h, 16h]):
     mixed precision with the autotuned multigrid cycle and solve-group split
@@ -227,6 +229,29 @@ def batched_run(steps, warmup):
                          # host ms per phase of a batch step
                          "per_step_ms": {k: (ms1[k] - ms0[k]) / steps for k in ms1},
                          "autotune_s": t_tune}
+            solve_profile = None
+            if profile and prec == MIXED:
+                # after the timed steps and the state download: the batched
+                # solve launch by launch with an event after each (not from
+                # its graph), one solve group, per kind over `steps` more
+                # batch steps
+                b.set_solve_groups(1)
+                b.profile(True)
+                b.run(1)
+                p0 = b.profile_read()
+                b.run(steps)
+                p1 = b.profile_read()
+                b.profile(False)
+                solve_profile = {}
+                for k in p1:
+                    ms = (p1[k][0] - p0[k][0]) / steps
+                    n = (p1[k][1] - p0[k][1]) / steps
+                    by = (p1[k][2] - p0[k][2]) / steps
+                    solve_profile[k] = {"ms_per_step": ms, "launches_per_step": n,
+                                        "us_per_launch": 1e3 * ms / n if n else None,
+                                        "algorithmic_GBps": by / (ms * 1e-3) / 1e9 if ms > 0 and by > 0 else None}
+            if solve_profile:
+                out[name]["solve_profile"] = solve_profile
             out["cases"] = len(setups)
             out["cells"] = cells
             out["markers"] = sum(s.case.n_markers for s in setups)
@@ -508,7 +533,7 @@ def main():
     if not args.no_configs:
         line["configs"] = config_runs(csteps, cwarm)
     if not args.no_batched:
-        line["batched"] = batched_run(csteps, cwarm)
+        line["batched"] = batched_run(csteps, cwarm, args.batch_profile)
     if args.cylinder_steps > 0:
         line["cylinder_shedding"] = cylinder_shedding(args.cylinder_steps)
     if not args.no_cpu:
