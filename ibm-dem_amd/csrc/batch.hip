@@ -35,10 +35,12 @@
 namespace {
 
 // A batched part of a step for every member of a group, for one parity of
-// the members (they share it), as one captured graph: the forcing loop
-// (keyed by the velocity parity), or the rest of the step after the solve
-// (gauge, projection, the DEM phase, probes; keyed by the velocity and
-// pressure parities and the pending bulk shift).
+// the members (they share it), as one captured graph: the first part up to
+// the solve's right-hand side (keyed by the velocity and pressure
+// parities), the forcing loop alone (keyed by the velocity parity), or the
+// rest of the step after the solve (gauge, projection, the DEM phase,
+// probes; keyed by the velocity and pressure parities and the pending bulk
+// shift).
 struct PostPlan {
     bk::BatchPlan plan;
     hipGraphExec_t exec = nullptr;
@@ -57,6 +59,9 @@ struct BatchGroup {
     std::map<int, PostPlan> mdf;      // by velocity parity
     bool mdf_now = false;             // this step's forcing loop runs batched
     hipEvent_t mdf_done = nullptr;
+    std::map<int, PostPlan> pre;      // by velocity and pressure parity
+    bool pre_now = false;             // this step's first part runs batched
+    bool synced = false;              // no member has work on its own stream that the group stream does not follow
     const int **flag_ptrs = nullptr;  // device: each member's error flags
     int *flags_dev = nullptr, *flags_host = nullptr;   // gathered flags (host: pinned)
 };
@@ -69,6 +74,8 @@ struct ibmdem_batch {
     bool fused = true;
     bool fused_post = true;          // the post part batched too
     bool fused_mdf = true;           // the forcing loop batched too
+    bool fused_pre = true;           // the whole first part batched (IBM set-up, forcing loop, predictor,
+                                     // divergence, right-hand side)
     std::vector<BatchGroup> groups;
     std::vector<int32_t> plan_key;   // the cases' parameters the groups were planned with
     std::vector<hipEvent_t> ev;      // per case: end of its step's first part
@@ -162,7 +169,7 @@ int restore_state(ibmdem_ctx *c, HostCopy &h) {
 
 void free_groups(ibmdem_batch *b) {
     for (BatchGroup &g : b->groups) {
-        for (auto *m : {&g.post, &g.mdf})
+        for (auto *m : {&g.post, &g.mdf, &g.pre})
             for (auto &kv : *m) {
                 if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
                 kv.second.plan.release();
@@ -305,7 +312,8 @@ int capture_group(ibmdem_batch *b, BatchGroup &g, int k) {
 int group_solve_queue(ibmdem_batch *b, BatchGroup &g) {
     int k = 0;
     for (int q : g.member) {
-        if (hipStreamWaitEvent(g.stream, b->ev[q], 0) != hipSuccess)
+        // a first part the group ran is already on its stream
+        if (!g.pre_now && hipStreamWaitEvent(g.stream, b->ev[q], 0) != hipSuccess)
             return group_error(b, g, IBMDEM_ERR_HIP, "batched solve: stream wait failed");
         k = std::max(k, b->ctx[q]->graphs.cg_k);
     }
@@ -444,6 +452,57 @@ int fill_post(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string
     return rc;
 }
 
+// the launches of step_pre_rhs for every member, in its order; the
+// divergence and the right-hand side are planned at the host state the
+// predictor leaves
+int fill_pre(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    int rc = ibm_plan_begin_step(P, cs, err);
+    if (!rc) rc = ibm_plan_mdf(P, cs, err);
+    if (!rc) rc = fluid_plan_predictor(P, cs, err);
+    if (rc) return rc;
+    std::vector<HostState> before;
+    for (ibmdem_ctx *c : cs) {
+        before.push_back(host_state(c));
+        step_pre_rhs_batched_state(c);
+    }
+    rc = fluid_plan_divergence(P, cs, err);
+    if (!rc) rc = poisson_plan_rhs(P, cs, err);
+    for (size_t i = 0; i < cs.size(); ++i) set_host_state(cs[i], before[i]);
+    return rc;
+}
+
+// the velocity and pressure parities the members share, or -1
+int pre_key(const ibmdem_batch *b, const BatchGroup &g) {
+    int key = -1;
+    for (int q : g.member) {
+        const ibmdem_ctx *c = b->ctx[q];
+        const int k = c->vcur | (c->pcur << 1);
+        if (key >= 0 && k != key) return -1;
+        key = k;
+    }
+    return key;
+}
+
+// Queue the group's first part on its stream (after any work still queued
+// on the members' own streams); the members' host state moves on as
+// step_pre_rhs would move it.
+int group_pre_queue(ibmdem_batch *b, BatchGroup &g, int key) {
+    PostPlan *pp = nullptr;
+    int rc;
+    if ((rc = graph_plan(b, g, g.pre, key, fill_pre, &pp))) return rc;
+    if (!g.synced) {
+        for (int q : g.member)
+            if (hipEventRecord(b->ev_a[q], b->ctx[q]->stream) != hipSuccess ||
+                hipStreamWaitEvent(g.stream, b->ev_a[q], 0) != hipSuccess)
+                return group_error(b, g, IBMDEM_ERR_HIP, "batched first part: stream wait failed");
+        g.synced = true;
+    }
+    if (hipGraphLaunch(pp->exec, g.stream) != hipSuccess)
+        return group_error(b, g, IBMDEM_ERR_HIP, "batched first part: graph launch failed");
+    for (int q : g.member) step_pre_rhs_batched_state(b->ctx[q]);
+    return IBMDEM_OK;
+}
+
 // the velocity parity the members share, or -1
 int velocity_key(const ibmdem_batch *b, const BatchGroup &g) {
     int key = -1;
@@ -502,6 +561,8 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
     }
     std::vector<char> done(n), failed(n, 0), batched(n, 0);
     int first_rc = IBMDEM_OK, first_q = -1;
+    // the members' streams may hold work queued outside this run
+    for (BatchGroup &g : b->groups) g.synced = false;
     auto fail = [&](int q, int r) {
         failed[q] = 1;
         if (first_q < 0 || q < first_q) { first_q = q; first_rc = r; }
@@ -521,17 +582,33 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
             if (g.active)
                 for (int q : g.member) batched[q] = 1;
         }
-        // the forcing loop batched where the members share their velocity parity
-        std::vector<char> mdf(n, 0);
+        // the whole first part batched where the members share their
+        // parities, else the forcing loop alone where they share their
+        // velocity parity
+        std::vector<char> mdf(n, 0), pre(n, 0);
         for (BatchGroup &g : b->groups) {
-            g.mdf_now = g.active && b->fused_mdf && velocity_key(b, g) >= 0;
-            if (g.mdf_now)
-                for (int q : g.member) mdf[q] = 1;
+            g.pre_now = g.active && b->fused_pre && pre_key(b, g) >= 0;
+            g.mdf_now = g.active && !g.pre_now && b->fused_mdf && velocity_key(b, g) >= 0;
+            if (!g.pre_now) g.synced = false;   // members queue on their own streams
+            for (int q : g.member) {
+                pre[q] = g.pre_now;
+                mdf[q] = g.mdf_now;
+            }
+        }
+        for (BatchGroup &g : b->groups) {
+            if (!g.pre_now) continue;
+            if ((rc = group_pre_queue(b, g, pre_key(b, g)))) {
+                for (int q : g.member) if (!failed[q]) fail(q, rc);
+                g.pre_now = false;
+                g.active = false;
+            }
         }
         for (int q = 0; q < n; ++q) {
             ibmdem_ctx *c = b->ctx[q];
             bool d = false;
-            if (mdf[q]) {
+            if (pre[q]) {
+                // queued with its group above
+            } else if (mdf[q]) {
                 rc = step_phase1a_batched(c);
                 if (!rc && hipEventRecord(b->ev_a[q], c->stream) != hipSuccess)
                     rc = set_error(c, IBMDEM_ERR_HIP, "event record failed");
@@ -598,6 +675,7 @@ int run_steps(ibmdem_batch *b, int n_steps, ibmdem_solve_info *inf) {
                     g.post_ran = true;
                 }
             } else {
+                g.synced = false;
                 for (int q : g.member)
                     if (!failed[q] && (rc = step_post_own(b->ctx[q]))) fail(q, rc);
             }
@@ -732,12 +810,13 @@ int ibmdem_batch_run(ibmdem_batch *b, int32_t n_steps, ibmdem_solve_info *info) 
 }
 
 int ibmdem_batch_set_fused(ibmdem_batch *b, int32_t on) {
-    if (!b || on < 0 || on > 3) return IBMDEM_ERR_ARG;
+    if (!b || on < 0 || on > 4) return IBMDEM_ERR_ARG;
     for (BatchGroup &g : b->groups)
         if (g.stream) (void)hipStreamSynchronize(g.stream);
     b->fused = on != 0;
-    b->fused_post = on == 1 || on == 3;
-    b->fused_mdf = on == 1;
+    b->fused_post = on == 1 || on == 3 || on == 4;
+    b->fused_mdf = on == 1 || on == 4;
+    b->fused_pre = on == 1;
     if (!b->fused) free_groups(b);
     return IBMDEM_OK;
 }

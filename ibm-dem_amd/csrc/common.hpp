@@ -585,6 +585,12 @@ int step_phase1_batched(ibmdem_ctx *c);
 int step_phase1a_batched(ibmdem_ctx *c);
 int step_phase1b_batched(ibmdem_ctx *c);
 void step_mdf_batched_state(ibmdem_ctx *c);
+// the host state after a first part (up to the solve's right-hand side) that
+// the batch ran for the context; the context's host state (for planning
+// batched parts at the state a step's later launches see)
+void step_pre_rhs_batched_state(ibmdem_ctx *c);
+HostState host_state(const ibmdem_ctx *c);
+void set_host_state(ibmdem_ctx *c, const HostState &h);
 // the solve's report from the batch's copy of the solver scalars
 int step_check_batched(ibmdem_ctx *c, const double *scal, ibmdem_solve_info *info);
 // the rest of the step on the context's own stream (its captured graph)
@@ -620,6 +626,21 @@ int poisson_plan_gauge(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, st
 int fluid_plan_project(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
 // the n_mdf iterations of ibm_interpolate and ibm_spread of every context
 int ibm_plan_mdf(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
+// ibm_begin_step of every context
+int ibm_plan_begin_step(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
+// fluid_predictor(c, 1, 0, 1) with its bulk sums, and fluid_divergence(c, 1),
+// of every context (at the parity and pending bulk shift it has when planned)
+int fluid_plan_predictor(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
+int fluid_plan_divergence(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
+// reduce_sum's launches for several sums: out = sum_{q >= skip} a[q] (b[q])
+struct ReduceJob {
+    const double *a, *b;
+    long n, skip;
+    double *partial, *out;
+};
+int plan_reduce_sum(bk::BatchPlan &P, const std::vector<ReduceJob> &jobs, std::string &err);
+// poisson_begin_rhs of every context
+int poisson_plan_rhs(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err);
 
 int poisson_setup(ibmdem_ctx *c);
 long poisson_partials(int nx, int ny, int nz);

@@ -154,6 +154,15 @@ int finish(ibmdem_ctx *c) {
 
 }  // namespace
 
+HostState host_state(const ibmdem_ctx *c) {
+    return HostState{c->vcur, c->pcur, c->bulk_pending, c->first_mdf, c->spread_valid, c->rmax_dirty};
+}
+void set_host_state(ibmdem_ctx *c, const HostState &h) {
+    c->vcur = h.vcur; c->pcur = h.pcur;
+    c->bulk_pending = h.bulk_pending; c->first_mdf = h.first_mdf;
+    c->spread_valid = h.spread_valid; c->rmax_dirty = h.rmax_dirty;
+}
+
 #define CTX_OR_ARG(c) if (!(c)) return IBMDEM_ERR_ARG
 #define TRY(expr) do { int rc_ = (expr); if (rc_) return rc_; } while (0)
 
@@ -174,14 +183,6 @@ namespace {
 // projection swap ping-pong buffers, so a step's launches depend on the
 // (vcur, pcur) parity: each parity has its own pair of graphs.
 // ---------------------------------------------------------------------------
-HostState host_state(const ibmdem_ctx *c) {
-    return HostState{c->vcur, c->pcur, c->bulk_pending, c->first_mdf, c->spread_valid, c->rmax_dirty};
-}
-void set_host_state(ibmdem_ctx *c, const HostState &h) {
-    c->vcur = h.vcur; c->pcur = h.pcur;
-    c->bulk_pending = h.bulk_pending; c->first_mdf = h.first_mdf;
-    c->spread_valid = h.spread_valid; c->rmax_dirty = h.rmax_dirty;
-}
 
 // the step up to the solve's right-hand side (r = b, ||b||), in three
 // parts: the IBM set-up of the step, the multi-direct-forcing loop, then the
@@ -363,6 +364,16 @@ void step_mdf_batched_state(ibmdem_ctx *c) {
         c->first_mdf = false;
         c->spread_valid = true;
     }
+}
+
+// the host state step_pre_rhs leaves: ibm_begin_step's, the forcing
+// loop's, the predictor's parity swap and fluid_bulk's pending shift
+void step_pre_rhs_batched_state(ibmdem_ctx *c) {
+    c->first_mdf = true;
+    c->spread_valid = false;
+    step_mdf_batched_state(c);
+    c->vcur = 1 - c->vcur;
+    c->bulk_pending = c->cs.bulk_switch[0] == 1 || c->cs.bulk_switch[1] == 1 || c->cs.bulk_switch[2] == 1;
 }
 
 int step_phase1b_batched(ibmdem_ctx *c) {
@@ -587,7 +598,9 @@ int ibmdem_upload_markers(ibmdem_ctx *c, const ibmdem_markers *m) {
     TRY(h2d(c, c->mmass, m->mass, v1));
     TRY(h2d(c, c->mrad, m->radeff, v1));
     TRY(h2d(c, c->mbody, m->body, (size_t)M * sizeof(int)));
-    c->rmax_dirty = true;
+    // the contact scan's body radii, measured at its next run (a context
+    // without markers or bodies never runs it)
+    c->rmax_dirty = c->M > 0 && c->B > 0;
     TRY(dem_rebuild(c));
     TRY(dem_check_flags(c));
     return finish(c);
@@ -621,7 +634,7 @@ int ibmdem_upload_bodies(ibmdem_ctx *c, const ibmdem_bodies *b) {
     TRY(h2d(c, c->bm, b->mass, v1)); TRY(h2d(c, c->brho, b->rho, v1));
     TRY(h2d(c, c->bfixed, b->fixed, (size_t)B * sizeof(int)));
     TRY(h2d(c, c->bfhydro, b->fhydro, v3)); TRY(h2d(c, c->bfmomhydro, b->fmomhydro, v3));
-    c->rmax_dirty = true;
+    c->rmax_dirty = c->M > 0 && c->B > 0;
     return finish(c);
 }
 

@@ -8,6 +8,7 @@
 #include "common.hpp"
 #include "batched.hpp"
 
+#include <map>
 #include <type_traits>
 
 #include <cmath>
@@ -148,12 +149,12 @@ __device__ __forceinline__ double wave_sum64(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
-// rows of this thread's wave into red[row][c] (row = threadIdx.y)
-__device__ __forceinline__ void row_sums(const double s[3], double (*red)[3]) {
+// rows of this thread's wave into red[row][c] (row = kb.ty)
+__device__ __forceinline__ void row_sums(const Blk &kb, const double s[3], double (*red)[3]) {
     double r[3];
     for (int q = 0; q < 3; ++q) r[q] = wave_sum64(s[q]);
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 3; ++q) red[threadIdx.y][q] = r[q];
+    if (kb.tx == 0)
+        for (int q = 0; q < 3; ++q) red[kb.ty][q] = r[q];
 }
 __device__ __forceinline__ void group_partial(double (*red)[3], int g0, double *part) {
     for (int q = 0; q < 3; ++q) part[q] = (red[g0][q] + red[g0 + 1][q]) + (red[g0 + 2][q] + red[g0 + 3][q]);
@@ -166,15 +167,13 @@ __device__ __forceinline__ long group_index(const GridDev &g, int bx, int jg, in
 // Because u~ = uold + dt*f after the multi-direct-forcing loop, this equals
 // the reference's uold + dt*(a*RHS(u~) + 0*RHSl - c*gradP) for a = 1.
 template <typename CT, bool BULK, bool P2, bool OPEN>
-__global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *__restrict__ u,
-                                                      const double *__restrict__ v,
-                                                      const double *__restrict__ w,
-                                                      const double *__restrict__ P, double *__restrict__ un,
-                                                      double *__restrict__ vn, double *__restrict__ wn,
-                                                      PredParams pp, double *__restrict__ partial) {
-    int bx, by, bz;
-    xcd_block(bx, by, bz);
-    const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
+__device__ __forceinline__ void predictor_body(const Blk &kb, const GridDev &g, const double *__restrict__ u,
+                                               const double *__restrict__ v, const double *__restrict__ w,
+                                               const double *__restrict__ P, double *__restrict__ un,
+                                               double *__restrict__ vn, double *__restrict__ wn,
+                                               const PredParams &pp, double *__restrict__ partial) {
+    const int bx = kb.bx, by = kb.by, bz = kb.bz;
+    const int i = bx * BX + kb.tx + 1, j = by * BY + kb.ty + 1, k = bz + 1;
     double s[3] = {0.0, 0.0, 0.0};
     if (i <= g.nx && j <= g.ny) {
         const long o = cidx(g, i, j, k);
@@ -198,10 +197,19 @@ __global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *_
     }
     if (BULK) {
         __shared__ double red[BY][3];
-        row_sums(s, red);
+        row_sums(kb, s, red);
         __syncthreads();
-        if (threadIdx.x == 0 && threadIdx.y == 0) group_partial(red, 0, partial + 3 * group_index(g, bx, by, bz + 1));
+        if (kb.tx == 0 && kb.ty == 0) group_partial(red, 0, partial + 3 * group_index(g, bx, by, bz + 1));
     }
+}
+template <typename CT, bool BULK, bool P2, bool OPEN>
+__global__ __launch_bounds__(BX *BY) void k_predictor(GridDev g, const double *__restrict__ u,
+                                                      const double *__restrict__ v,
+                                                      const double *__restrict__ w,
+                                                      const double *__restrict__ P, double *__restrict__ un,
+                                                      double *__restrict__ vn, double *__restrict__ wn,
+                                                      PredParams pp, double *__restrict__ partial) {
+    predictor_body<CT, BULK, P2, OPEN>(xcd_blk(), g, u, v, w, P, un, vn, wn, pp, partial);
 }
 
 // VelocityGuess(a,b,c) in the reference's own form (fluid.F90:116-131):
@@ -222,9 +230,9 @@ __global__ __launch_bounds__(BX *BY) void k_predictor_general(GridDev g, const d
                                                               double *__restrict__ vn, double *__restrict__ wn,
                                                               PredParams pp, GeneralArgs ga,
                                                               double *__restrict__ partial) {
-    int bx, by, bz;
-    xcd_block(bx, by, bz);
-    const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
+    const Blk kb = xcd_blk();
+    const int bx = kb.bx, by = kb.by, bz = kb.bz;
+    const int i = bx * BX + kb.tx + 1, j = by * BY + kb.ty + 1, k = bz + 1;
     double s[3] = {0.0, 0.0, 0.0};
     if (i <= g.nx && j <= g.ny) {
         const long o = cidx(g, i, j, k);
@@ -250,9 +258,9 @@ __global__ __launch_bounds__(BX *BY) void k_predictor_general(GridDev g, const d
     }
     if (BULK) {
         __shared__ double red[BY][3];
-        row_sums(s, red);
+        row_sums(kb, s, red);
         __syncthreads();
-        if (threadIdx.x == 0 && threadIdx.y == 0) group_partial(red, 0, partial + 3 * group_index(g, bx, by, bz + 1));
+        if (kb.tx == 0 && kb.ty == 0) group_partial(red, 0, partial + 3 * group_index(g, bx, by, bz + 1));
     }
 }
 
@@ -298,8 +306,7 @@ struct LTapP {
 // beyond the halo of the valid cells are not needed and not loaded
 template <typename LT, int TY, int C, bool OPEN>
 __device__ __forceinline__ void load_plane(LT (*dst)[LX], const double *__restrict__ f, const GridDev &g, int i0,
-                                           int j0, int k) {
-    const int tid = threadIdx.x + BX * threadIdx.y;
+                                           int j0, int k, int tid) {
     for (int e = tid; e < (TY + 2) * LX; e += BX * TY) {
         const int ly = e / LX, lx = e - ly * LX;
         const int i = i0 - 1 + lx, j = j0 - 1 + ly;
@@ -326,8 +333,7 @@ struct PlaneSlots {
     bool has[NS];      // the slot is one of the plane's elements
 };
 template <int TY>
-__device__ __forceinline__ void plane_slots(PlaneSlots<TY> &S, const GridDev &g, int i0, int j0) {
-    const int tid = threadIdx.x + BX * threadIdx.y;
+__device__ __forceinline__ void plane_slots(PlaneSlots<TY> &S, const GridDev &g, int i0, int j0, int tid) {
     S.neg = 0u;
     S.zero = 0u;
 #pragma unroll
@@ -361,7 +367,7 @@ __device__ __forceinline__ void plane_slots(PlaneSlots<TY> &S, const GridDev &g,
 // plane k of field F (0..2 velocity components, 3 P) through the slots
 template <typename LT, int TY, int F>
 __device__ __forceinline__ void load_plane_slots(LT (*dst)[LX], const double *__restrict__ f, const GridDev &g,
-                                                 const PlaneSlots<TY> &S, int k) {
+                                                 const PlaneSlots<TY> &S, int k, int tid) {
     int kk = k;
     double sg = 1.0;
     bool ok = true;
@@ -370,7 +376,6 @@ __device__ __forceinline__ void load_plane_slots(LT (*dst)[LX], const double *__
     else ok = map_index<false>(kk, g.nz, g.bt[4], g.bt[5], sg);
     const long zoff = ok ? (long)(kk - 1) * g.nx * g.ny : 0;
     const unsigned zneg = sg < 0 ? 1u : 0u;
-    const int tid = threadIdx.x + BX * threadIdx.y;
     double v[PlaneSlots<TY>::NS];
 #pragma unroll
     for (int q = 0; q < PlaneSlots<TY>::NS; ++q) {
@@ -415,8 +420,8 @@ __device__ __forceinline__ void plane_fetch(PlaneFetch<TY> &P, const double *__r
     }
 }
 template <typename LT, int TY, int F>
-__device__ __forceinline__ void plane_commit(LT (*dst)[LX], const PlaneFetch<TY> &P, const PlaneSlots<TY> &S) {
-    const int tid = threadIdx.x + BX * threadIdx.y;
+__device__ __forceinline__ void plane_commit(LT (*dst)[LX], const PlaneFetch<TY> &P, const PlaneSlots<TY> &S,
+                                             int tid) {
 #pragma unroll
     for (int q = 0; q < PlaneSlots<TY>::NS; ++q) {
         const bool z = !P.zok || ((S.zero >> (4 * q + F)) & 1u);
@@ -428,29 +433,27 @@ __device__ __forceinline__ void plane_commit(LT (*dst)[LX], const PlaneFetch<TY>
 }
 
 template <typename CT, bool BULK, bool P2, bool OPEN, int TY, bool GEN>
-__global__ __launch_bounds__(BX *TY) void k_predictor_lds(GridDev g, const double *__restrict__ u,
-                                                          const double *__restrict__ v,
-                                                          const double *__restrict__ w,
-                                                          const double *__restrict__ P, double *__restrict__ un,
-                                                          double *__restrict__ vn, double *__restrict__ wn,
-                                                          PredParams pp, GeneralArgs ga, int kz,
-                                                          double *__restrict__ partial) {
+__device__ __forceinline__ void predictor_lds_body(const Blk &kb, const GridDev &g, const double *__restrict__ u,
+                                                   const double *__restrict__ v, const double *__restrict__ w,
+                                                   const double *__restrict__ P, double *__restrict__ un,
+                                                   double *__restrict__ vn, double *__restrict__ wn,
+                                                   const PredParams &pp, const GeneralArgs &ga, int kz,
+                                                   double *__restrict__ partial) {
     __shared__ LdsPlanes<CT, TY> L;
     __shared__ double red[TY][3];
-    int bx, by, bz;
-    xcd_block(bx, by, bz);
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int bx = kb.bx, by = kb.by, bz = kb.bz;
+    const int tx = kb.tx, ty = kb.ty, tid = tx + BX * ty;
     const int i0 = bx * BX + 1, j0 = by * TY + 1;
     const int i = i0 + tx, j = j0 + ty;
     const int k0 = bz * kz + 1, k1 = min(g.nz, k0 + kz - 1);
     const bool mine = i <= g.nx && j <= g.ny;
     PlaneSlots<TY> S;
-    if (!OPEN) plane_slots<TY>(S, g, i0, j0);
+    if (!OPEN) plane_slots<TY>(S, g, i0, j0, tid);
     // plane kk of field F into dst
     auto plane = [&](auto fld, CT (*dst)[LX], const double *__restrict__ f, int kk) {
         constexpr int F = decltype(fld)::value;
-        if (OPEN) load_plane<CT, TY, F, OPEN>(dst, f, g, i0, j0, kk);
-        else load_plane_slots<CT, TY, F>(dst, f, g, S, kk);
+        if (OPEN) load_plane<CT, TY, F, OPEN>(dst, f, g, i0, j0, kk, tid);
+        else load_plane_slots<CT, TY, F>(dst, f, g, S, kk, tid);
     };
     using F0 = std::integral_constant<int, 0>;
     using F1 = std::integral_constant<int, 1>;
@@ -481,10 +484,10 @@ __global__ __launch_bounds__(BX *TY) void k_predictor_lds(GridDev g, const doubl
             plane(F2{}, L.v[2][sp], w, k + 1);
             plane(F3{}, L.p, P, k);
         } else {
-            plane_commit<CT, TY, 0>(L.v[0][sp], fu, S);
-            plane_commit<CT, TY, 1>(L.v[1][sp], fv, S);
-            plane_commit<CT, TY, 2>(L.v[2][sp], fw, S);
-            plane_commit<CT, TY, 3>(L.p, fp, S);
+            plane_commit<CT, TY, 0>(L.v[0][sp], fu, S, tid);
+            plane_commit<CT, TY, 1>(L.v[1][sp], fv, S, tid);
+            plane_commit<CT, TY, 2>(L.v[2][sp], fw, S, tid);
+            plane_commit<CT, TY, 3>(L.p, fp, S, tid);
             if (k < k1) {
                 plane_fetch<TY, 0>(fu, u, g, S, k + 2);
                 plane_fetch<TY, 1>(fv, v, g, S, k + 2);
@@ -521,12 +524,22 @@ __global__ __launch_bounds__(BX *TY) void k_predictor_lds(GridDev g, const doubl
                 s[q] = nv;
             }
         }
-        if (BULK) row_sums(s, red);
+        if (BULK) row_sums(kb, s, red);
         __syncthreads();   // slot sm is reloaded next iteration; red is complete
         // a group of rows beyond ny (64x8 tiles over the last rows) has no partial
         if (BULK && tx == 0 && (ty & 3) == 0 && j0 + ty <= g.ny)
             group_partial(red, ty, partial + 3 * group_index(g, bx, (j0 - 1) / 4 + ty / 4, k));
     }
+}
+template <typename CT, bool BULK, bool P2, bool OPEN, int TY, bool GEN>
+__global__ __launch_bounds__(BX *TY) void k_predictor_lds(GridDev g, const double *__restrict__ u,
+                                                          const double *__restrict__ v,
+                                                          const double *__restrict__ w,
+                                                          const double *__restrict__ P, double *__restrict__ un,
+                                                          double *__restrict__ vn, double *__restrict__ wn,
+                                                          PredParams pp, GeneralArgs ga, int kz,
+                                                          double *__restrict__ partial) {
+    predictor_lds_body<CT, BULK, P2, OPEN, TY, GEN>(xcd_blk(), g, u, v, w, P, un, vn, wn, pp, ga, kz, partial);
 }
 
 __global__ __launch_bounds__(256) void k_copy3(long n, const double *__restrict__ a, const double *__restrict__ b,
@@ -542,12 +555,12 @@ __global__ __launch_bounds__(256) void k_copy3(long n, const double *__restrict_
 // First stage of the bulk sums: block b adds the row-group partials of its
 // contiguous chunk (thread t takes every 256th, then a tree) into out[3b + c].
 constexpr int BULK_BLOCKS = 256;
-__global__ __launch_bounds__(256) void k_bulk_reduce(const double *__restrict__ partial, long nparts,
-                                                     double *__restrict__ out) {
+__device__ __forceinline__ void bulk_reduce_body(const Blk &kb, const double *__restrict__ partial, long nparts,
+                                                 double *__restrict__ out) {
     __shared__ double red[3][256];
-    const int t = threadIdx.x;
-    const long chunk = (nparts + gridDim.x - 1) / gridDim.x;
-    const long lo = (long)blockIdx.x * chunk, hi = min(nparts, lo + chunk);
+    const int t = kb.tx;
+    const long chunk = (nparts + kb.gx - 1) / kb.gx;
+    const long lo = (long)kb.bx * chunk, hi = min(nparts, lo + chunk);
     double s[3] = {0, 0, 0};
     for (long q = lo + t; q < hi; q += 256)
         for (int c = 0; c < 3; ++c) s[c] += partial[3 * q + c];
@@ -558,17 +571,20 @@ __global__ __launch_bounds__(256) void k_bulk_reduce(const double *__restrict__ 
             for (int c = 0; c < 3; ++c) red[c][t] += red[c][t + st];
         __syncthreads();
     }
-    if (t < 3) out[3 * blockIdx.x + t] = red[t][0];
+    if (t < 3) out[3 * kb.bx + t] = red[t][0];
+}
+__global__ __launch_bounds__(256) void k_bulk_reduce(const double *__restrict__ partial, long nparts,
+                                                     double *__restrict__ out) {
+    bulk_reduce_body(hw_blk(), partial, nparts, out);
 }
 
 // ModifyBulkVelocity finalize (fluid.F90:177-183): one block sums the block
 // partials in a fixed order and stores the shifts dt*modifyP.
-__global__ __launch_bounds__(256) void k_bulk_finalize(const double *__restrict__ partial, long nparts,
-                                                       double *__restrict__ scal, int sw0, int sw1, int sw2,
-                                                       double t0, double t1, double t2, double ntot,
-                                                       double dt) {
+__device__ __forceinline__ void bulk_finalize_body(const Blk &kb, const double *__restrict__ partial, long nparts,
+                                                   double *__restrict__ scal, int sw0, int sw1, int sw2, double t0,
+                                                   double t1, double t2, double ntot, double dt) {
     __shared__ double red[3][256];
-    const int t = threadIdx.x;
+    const int t = kb.tx;
     double s[3] = {0, 0, 0};
     for (long q = t; q < nparts; q += 256)
         for (int c = 0; c < 3; ++c) s[c] += partial[3 * q + c];
@@ -590,6 +606,12 @@ __global__ __launch_bounds__(256) void k_bulk_finalize(const double *__restrict_
             scal[SC_MODP + c] = dt * mod;
         }
     }
+}
+__global__ __launch_bounds__(256) void k_bulk_finalize(const double *__restrict__ partial, long nparts,
+                                                       double *__restrict__ scal, int sw0, int sw1, int sw2,
+                                                       double t0, double t1, double t2, double ntot,
+                                                       double dt) {
+    bulk_finalize_body(hw_blk(), partial, nparts, scal, sw0, sw1, sw2, t0, t1, t2, ntot, dt);
 }
 
 __global__ void k_bulk_from_sums(double *scal, int sw0, int sw1, int sw2, double t0, double t1, double t2,
@@ -634,15 +656,13 @@ __device__ __forceinline__ double dv(double x, const Divisor &D) {
 }
 
 template <bool OPEN>
-__global__ __launch_bounds__(BX *BY) void k_divergence(GridDev g, const double *__restrict__ u,
-                                                       const double *__restrict__ v,
-                                                       const double *__restrict__ w, double *__restrict__ rhs,
-                                                       const double *__restrict__ scal, int shifted, Divisor ddx,
-                                                       Divisor ddy, Divisor ddz, Divisor ddt, Divisor dcrk,
-                                                       double rhof) {
-    int bx, by, bz;
-    xcd_block(bx, by, bz);
-    const int i = bx * BX + threadIdx.x + 1, j = by * BY + threadIdx.y + 1, k = bz + 1;
+__device__ __forceinline__ void divergence_body(const Blk &kb, const GridDev &g, const double *__restrict__ u,
+                                                const double *__restrict__ v, const double *__restrict__ w,
+                                                double *__restrict__ rhs, const double *__restrict__ scal,
+                                                int shifted, const Divisor &ddx, const Divisor &ddy,
+                                                const Divisor &ddz, const Divisor &ddt, const Divisor &dcrk,
+                                                double rhof) {
+    const int i = kb.bx * BX + kb.tx + 1, j = kb.by * BY + kb.ty + 1, k = kb.bz + 1;
     if (i > g.nx || j > g.ny) return;
     double su = 0.0, sv = 0.0, sw = 0.0;
     if (shifted) { su = scal[SC_MODP]; sv = scal[SC_MODP + 1]; sw = scal[SC_MODP + 2]; }
@@ -677,6 +697,15 @@ __global__ __launch_bounds__(BX *BY) void k_divergence(GridDev g, const double *
     }
     double d = dv(u1 - u0, ddx) + dv(v1 - v0, ddy) + dv(w1 - w0, ddz);
     rhs[cidx(g, i, j, k)] = dv(dv(-d, ddt), dcrk) * rhof;
+}
+template <bool OPEN>
+__global__ __launch_bounds__(BX *BY) void k_divergence(GridDev g, const double *__restrict__ u,
+                                                       const double *__restrict__ v,
+                                                       const double *__restrict__ w, double *__restrict__ rhs,
+                                                       const double *__restrict__ scal, int shifted, Divisor ddx,
+                                                       Divisor ddy, Divisor ddz, Divisor ddt, Divisor dcrk,
+                                                       double rhof) {
+    divergence_body<OPEN>(xcd_blk(), g, u, v, w, rhs, scal, shifted, ddx, ddy, ddz, ddt, dcrk, rhof);
 }
 
 // PressureCorrection + VelocityCorrection(c) (fluid.F90:206, 221-223):
@@ -769,38 +798,49 @@ __global__ __launch_bounds__(BX *BY) void k_wall_faces(GridDev g, double *u, dou
 // thread-strided, then an LDS tree; the finalize block sums the partials the
 // same way. The association is fixed by (n, RED_BLOCKS, RED_THREADS).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const double *__restrict__ a,
-                                                                const double *__restrict__ b, long n,
-                                                                long skip, double *__restrict__ partial) {
+__device__ __forceinline__ void reduce_partial_body(const Blk &kb, const double *__restrict__ a,
+                                                    const double *__restrict__ b, long n, long skip,
+                                                    double *__restrict__ partial) {
     __shared__ double red[RED_THREADS];
-    const long chunk = (n + gridDim.x - 1) / gridDim.x;
-    const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+    const int t = kb.tx;
+    const long chunk = (n + kb.gx - 1) / kb.gx;
+    const long lo = (long)kb.bx * chunk, hi = min(n, lo + chunk);
     double s = 0.0;
-    for (long q = lo + threadIdx.x; q < hi; q += RED_THREADS) {
+    for (long q = lo + t; q < hi; q += RED_THREADS) {
         if (q < skip) continue;
         s += b ? a[q] * b[q] : a[q];
     }
-    red[threadIdx.x] = s;
+    red[t] = s;
     __syncthreads();
     for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        if (t < st) red[t] += red[t + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    if (t == 0) partial[kb.bx] = red[0];
+}
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_partial(const double *__restrict__ a,
+                                                                const double *__restrict__ b, long n,
+                                                                long skip, double *__restrict__ partial) {
+    reduce_partial_body(hw_blk(), a, b, n, skip, partial);
 }
 
-__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__restrict__ partial, int np,
-                                                              double *__restrict__ out) {
+__device__ __forceinline__ void reduce_final_body(const Blk &kb, const double *__restrict__ partial, int np,
+                                                  double *__restrict__ out) {
     __shared__ double red[RED_THREADS];
+    const int t = kb.tx;
     double s = 0.0;
-    for (int q = threadIdx.x; q < np; q += RED_THREADS) s += partial[q];
-    red[threadIdx.x] = s;
+    for (int q = t; q < np; q += RED_THREADS) s += partial[q];
+    red[t] = s;
     __syncthreads();
     for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        if (t < st) red[t] += red[t + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = red[0];
+    if (t == 0) *out = red[0];
+}
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__restrict__ partial, int np,
+                                                              double *__restrict__ out) {
+    reduce_final_body(hw_blk(), partial, np, out);
 }
 
 dim3 field_grid(const GridDev &g) { return dim3(div_up(g.nx, BX), div_up(g.ny, BY), g.nz); }
@@ -1084,4 +1124,239 @@ int fluid_plan_project(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, st
     int rc;
     if ((rc = add_op(P.ar, P.ops, launch_project<false>, BX * BY, closed, BOp(), err))) return rc;
     return add_op(P.ar, P.ops, launch_project<true>, BX * BY, open, BOp(), err);
+}
+
+// ---------------------------------------------------------------------------
+// The predictor (with its bulk sums) and the divergence of a batched step,
+// and reduce_sum's launches for several sums: every context of a group in
+// one launch per kernel variant (precision, bulk forcing, power-of-two
+// spacing, open sides, tile schedule). Each case's pointers follow its own
+// parity, as fluid_predictor and fluid_divergence read it.
+// ---------------------------------------------------------------------------
+namespace {
+struct PredArg {
+    BGrid g;
+    GridDev grid;
+    const double *u, *v, *w, *P;
+    double *un, *vn, *wn;
+    PredParams pp;
+    int kz;
+    double *partial;
+};
+__device__ __forceinline__ GridDev glb_grid(const GridDev &src) {
+    GridDev g = src;
+    for (int d = 0; d < 3; ++d) g.plane[d] = glb(g.plane[d]);
+    return g;
+}
+template <typename CT, bool BULK, bool P2, bool OPEN>
+__global__ __launch_bounds__(BX *BY) void kb_predictor(BTable<PredArg> tab) {
+    batch_for(tab, [&](const PredArg &a, const Blk &kb) {
+        const GridDev g = glb_grid(a.grid);
+        predictor_body<CT, BULK, P2, OPEN>(kb, g, glb(a.u), glb(a.v), glb(a.w), glb(a.P), glb(a.un), glb(a.vn),
+                                           glb(a.wn), a.pp, glb(a.partial));
+    });
+}
+template <typename CT, bool BULK, bool P2, bool OPEN, int TY>
+__global__ __launch_bounds__(BX *TY) void kb_predictor_lds(BTable<PredArg> tab) {
+    batch_for(tab, [&](const PredArg &a, const Blk &kb) {
+        const GridDev g = glb_grid(a.grid);
+        predictor_lds_body<CT, BULK, P2, OPEN, TY, false>(kb, g, glb(a.u), glb(a.v), glb(a.w), glb(a.P), glb(a.un),
+                                                          glb(a.vn), glb(a.wn), a.pp, GeneralArgs{}, a.kz,
+                                                          glb(a.partial));
+    });
+}
+template <typename CT, bool BULK, bool P2, bool OPEN>
+void launch_predictor(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL((kb_predictor<CT, BULK, P2, OPEN>), dim3(o.blocks), dim3(BX * BY), 0, s, table<PredArg>(o));
+}
+template <typename CT, bool BULK, bool P2, bool OPEN, int TY>
+void launch_predictor_lds(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL((kb_predictor_lds<CT, BULK, P2, OPEN, TY>), dim3(o.blocks), dim3(BX * TY), 0, s,
+                       table<PredArg>(o));
+}
+// the launcher of a tile schedule (PredShape::ty: 0 direct, 4 or 8)
+template <typename CT, bool BULK, bool P2, bool OPEN>
+BLaunch pred_launcher(int ty) {
+    return ty == 0 ? launch_predictor<CT, BULK, P2, OPEN>
+         : ty == 8 ? launch_predictor_lds<CT, BULK, P2, OPEN, 8>
+                   : launch_predictor_lds<CT, BULK, P2, OPEN, 4>;
+}
+// dispatch_pred's variant choice (open sides exclude bulk forcing)
+template <typename CT>
+BLaunch pred_launcher(bool bulk, bool p2, bool open, int ty) {
+    if (open) return p2 ? pred_launcher<CT, false, true, true>(ty) : pred_launcher<CT, false, false, true>(ty);
+    if (bulk) return p2 ? pred_launcher<CT, true, true, false>(ty) : pred_launcher<CT, true, false, false>(ty);
+    return p2 ? pred_launcher<CT, false, true, false>(ty) : pred_launcher<CT, false, false, false>(ty);
+}
+
+struct BulkArg {
+    BGrid g;
+    const double *partial;
+    long nparts;
+    double *second, *scal;
+    int sw[3];
+    double tg[3];
+    double ntot, dt;
+};
+__global__ __launch_bounds__(256) void kb_bulk_reduce(BTable<BulkArg> tab) {
+    batch_for(tab, [&](const BulkArg &a, const Blk &kb) {
+        bulk_reduce_body(kb, glb(a.partial), a.nparts, glb(a.second));
+    });
+}
+__global__ __launch_bounds__(256) void kb_bulk_finalize(BTable<BulkArg> tab) {
+    batch_for(tab, [&](const BulkArg &a, const Blk &kb) {
+        bulk_finalize_body(kb, glb((const double *)a.second), BULK_BLOCKS, glb(a.scal), a.sw[0], a.sw[1], a.sw[2],
+                           a.tg[0], a.tg[1], a.tg[2], a.ntot, a.dt);
+    });
+}
+void launch_bulk_reduce(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_bulk_reduce, dim3(o.blocks), dim3(256), 0, s, table<BulkArg>(o));
+}
+void launch_bulk_finalize(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_bulk_finalize, dim3(o.blocks), dim3(256), 0, s, table<BulkArg>(o));
+}
+
+struct DivArg {
+    BGrid g;
+    GridDev grid;
+    const double *u, *v, *w, *scal;
+    double *rhs;
+    int shifted;
+    Divisor ddx, ddy, ddz, ddt, dcrk;
+    double rhof;
+};
+template <bool OPEN>
+__global__ __launch_bounds__(BX *BY) void kb_divergence(BTable<DivArg> tab) {
+    batch_for(tab, [&](const DivArg &a, const Blk &kb) {
+        const GridDev g = glb_grid(a.grid);
+        divergence_body<OPEN>(kb, g, glb(a.u), glb(a.v), glb(a.w), glb(a.rhs), glb(a.scal), a.shifted, a.ddx, a.ddy,
+                              a.ddz, a.ddt, a.dcrk, a.rhof);
+    });
+}
+template <bool OPEN>
+void launch_divergence(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_divergence<OPEN>, dim3(o.blocks), dim3(BX * BY), 0, s, table<DivArg>(o));
+}
+
+struct RedArg {
+    BGrid g;
+    const double *a, *b;
+    long n, skip;
+    double *partial, *out;
+    int np;
+};
+__global__ __launch_bounds__(RED_THREADS) void kb_reduce_partial(BTable<RedArg> tab) {
+    batch_for(tab, [&](const RedArg &a, const Blk &kb) {
+        reduce_partial_body(kb, glb(a.a), a.b ? glb(a.b) : nullptr, a.n, a.skip, glb(a.partial));
+    });
+}
+__global__ __launch_bounds__(RED_THREADS) void kb_reduce_final(BTable<RedArg> tab) {
+    batch_for(tab, [&](const RedArg &a, const Blk &kb) {
+        reduce_final_body(kb, glb((const double *)a.partial), a.np, glb(a.out));
+    });
+}
+void launch_reduce_partial(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_reduce_partial, dim3(o.blocks), dim3(RED_THREADS), 0, s, table<RedArg>(o));
+}
+void launch_reduce_final(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_reduce_final, dim3(o.blocks), dim3(RED_THREADS), 0, s, table<RedArg>(o));
+}
+
+bool bulk_on(const ibmdem_ctx *c) {
+    return c->cs.bulk_switch[0] == 1 || c->cs.bulk_switch[1] == 1 || c->cs.bulk_switch[2] == 1;
+}
+}  // namespace
+
+int fluid_plan_predictor(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    // the cases of each launcher, with its block size
+    std::map<BLaunch, std::pair<int, std::vector<PredArg>>> by;
+    std::vector<BulkArg> bulk_red, bulk_fin;
+    for (ibmdem_ctx *c : cs) {
+        if (c->uold[0]) {
+            err = "batched predictor: the general VelocityGuess (RK3 contexts) is not batched";
+            return IBMDEM_ERR_UNSUPPORTED;
+        }
+        const GridDev &g = c->g;
+        const PredShape S = pred_shape(c);
+        const bool p2 = is_pow2(g.dx) && is_pow2(g.dy) && is_pow2(g.dz) && is_pow2(c->cs.rhof);
+        const bool bulk = bulk_on(c), open = grid_open(g);
+        const BLaunch fn = c->cs.precision == IBMDEM_MIXED ? pred_launcher<float>(bulk, p2, open, S.ty)
+                                                           : pred_launcher<double>(bulk, p2, open, S.ty);
+        const int cur = c->vcur, nxt = 1 - c->vcur;
+        PredArg a{};
+        a.g = bgrid(S.grid, S.blk);
+        a.grid = g;
+        a.u = c->u[cur]; a.v = c->v[cur]; a.w = c->w[cur];
+        a.P = c->P[c->pcur];
+        a.un = c->u[nxt]; a.vn = c->v[nxt]; a.wn = c->w[nxt];
+        a.pp = pred_params(c, 1.0, 1.0);
+        a.kz = S.kz;
+        a.partial = c->partial;
+        auto &slot = by[fn];
+        slot.first = (int)(S.blk.x * S.blk.y);
+        slot.second.push_back(a);
+        if (bulk) {
+            BulkArg b{};
+            b.partial = c->partial;
+            b.nparts = (long)div_up(g.nx, BX) * div_up(g.ny, 4) * g.nz;
+            b.second = c->partial + 3 * b.nparts;
+            b.scal = c->scal;
+            for (int q = 0; q < 3; ++q) {
+                b.sw[q] = c->cs.bulk_switch[q];
+                b.tg[q] = c->cs.bulk_target[q];
+            }
+            b.ntot = (double)((long)g.nx * g.ny * g.nz);
+            b.dt = c->cs.dt;
+            b.g = bgrid(dim3(BULK_BLOCKS), dim3(256));
+            bulk_red.push_back(b);
+            b.g = bgrid(dim3(1), dim3(256));
+            bulk_fin.push_back(b);
+        }
+    }
+    int rc;
+    for (auto &kv : by)
+        if ((rc = add_op(P.ar, P.ops, kv.first, kv.second.first, kv.second.second, BOp(), err))) return rc;
+    if ((rc = add_op(P.ar, P.ops, launch_bulk_reduce, 256, bulk_red, BOp(), err))) return rc;
+    return add_op(P.ar, P.ops, launch_bulk_finalize, 256, bulk_fin, BOp(), err);
+}
+
+int fluid_plan_divergence(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    std::vector<DivArg> open, closed;
+    for (ibmdem_ctx *c : cs) {
+        const GridDev &g = c->g;
+        DivArg a{};
+        a.g = bgrid(field_grid(g), dim3(BX, BY));
+        a.grid = g;
+        a.u = c->u[c->vcur]; a.v = c->v[c->vcur]; a.w = c->w[c->vcur];
+        a.scal = c->scal;
+        a.rhs = c->rhs;
+        a.shifted = c->bulk_pending ? 1 : 0;
+        a.ddx = divisor(g.dx); a.ddy = divisor(g.dy); a.ddz = divisor(g.dz);
+        a.ddt = divisor(c->cs.dt);
+        a.dcrk = divisor(1.0);
+        a.rhof = c->cs.rhof;
+        (grid_open(g) ? open : closed).push_back(a);
+    }
+    int rc;
+    if ((rc = add_op(P.ar, P.ops, launch_divergence<false>, BX * BY, closed, BOp(), err))) return rc;
+    return add_op(P.ar, P.ops, launch_divergence<true>, BX * BY, open, BOp(), err);
+}
+
+int plan_reduce_sum(bk::BatchPlan &P, const std::vector<ReduceJob> &jobs, std::string &err) {
+    std::vector<RedArg> part, fin;
+    for (const ReduceJob &j : jobs) {
+        const int nb = (int)std::min<long>(RED_BLOCKS, std::max<long>(1, j.n));
+        RedArg a{};
+        a.a = j.a; a.b = j.b;
+        a.n = j.n; a.skip = j.skip;
+        a.partial = j.partial; a.out = j.out;
+        a.np = nb;
+        a.g = bgrid(dim3(nb), dim3(RED_THREADS));
+        part.push_back(a);
+        a.g = bgrid(dim3(1), dim3(RED_THREADS));
+        fin.push_back(a);
+    }
+    int rc;
+    if ((rc = add_op(P.ar, P.ops, launch_reduce_partial, RED_THREADS, part, BOp(), err))) return rc;
+    return add_op(P.ar, P.ops, launch_reduce_final, RED_THREADS, fin, BOp(), err);
 }

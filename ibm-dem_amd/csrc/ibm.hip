@@ -39,6 +39,7 @@
 // the step).
 #include "common.hpp"
 #include "batched.hpp"
+#include "scan.hpp"
 
 #include <algorithm>
 #include <mutex>
@@ -49,6 +50,7 @@ using namespace bk;
 
 constexpr int T = 256;
 constexpr int CHUNK = 4096;   // flags per compaction block
+static_assert(T == scan::SCAN_T && CHUNK == scan::SCAN_CHUNK, "the entry-count scan uses the shared chunk kernels");
 constexpr int TL = IBM_TL;    // tile edge (cells)
 constexpr int TC = IBM_TC;    // cells per tile
 constexpr int SRC = TL + 3;   // source cells per tile edge
@@ -97,10 +99,10 @@ __device__ __forceinline__ int tile_slot(int b, int nact) {
 // Markers do not move during the five interpolate/spread iterations of a
 // step, so these are computed once here and read by both.
 template <bool P2>
-__global__ __launch_bounds__(T) void k_records(int M, GridDev g, const double *__restrict__ mx,
-                                               const int *__restrict__ mcell, double *__restrict__ rec,
-                                               double *__restrict__ fac, double *__restrict__ fdrag) {
-    const int m = blockIdx.x * T + threadIdx.x;
+__device__ __forceinline__ void records_body(const Blk &kb, int M, const GridDev &g, const double *__restrict__ mx,
+                                             const int *__restrict__ mcell, double *__restrict__ rec,
+                                             double *__restrict__ fac, double *__restrict__ fdrag) {
+    const int m = kb.bx * T + kb.tx;
     if (m >= M) return;
     double *r = rec + 8L * m;
     const double xp[3] = {mx[m], mx[(long)M + m], mx[2L * M + m]};
@@ -118,6 +120,12 @@ __global__ __launch_bounds__(T) void k_records(int M, GridDev g, const double *_
             f[d * 4 + o] = make_double2(wt_face<P2, true>(xp[d], cc[d] + o - 1, dd[d], id),
                                         o < 3 ? wt_cent<P2, true>(xp[d], cc[d] + o - 1, dd[d], id) : 0.0);
     }
+}
+template <bool P2>
+__global__ __launch_bounds__(T) void k_records(int M, GridDev g, const double *__restrict__ mx,
+                                               const int *__restrict__ mcell, double *__restrict__ rec,
+                                               double *__restrict__ fac, double *__restrict__ fdrag) {
+    records_body<P2>(hw_blk(), M, g, mx, mcell, rec, fac, fdrag);
 }
 
 // Ghost rules of the four taps at offsets -1..+2 of one dimension around a
@@ -352,9 +360,9 @@ struct TileGeom {
 };
 
 // mark the tiles holding an interior cell that a marker (or an image) can reach
-__global__ __launch_bounds__(T) void k_mark_tiles(int M, GridDev g, TileGeom tg, const int *__restrict__ mcell,
-                                                  unsigned char *__restrict__ flag) {
-    const int m = blockIdx.x * T + threadIdx.x;
+__device__ __forceinline__ void mark_tiles_body(const Blk &kb, int M, const GridDev &g, TileGeom tg,
+                                                const int *__restrict__ mcell, unsigned char *__restrict__ flag) {
+    const int m = kb.bx * T + kb.tx;
     if (m >= M) return;
     const int cl = mcell[m];
     const int cc[3] = {cl % g.nx + 1, (cl / g.nx) % g.ny + 1, cl / (g.nx * g.ny) + 1};
@@ -378,70 +386,87 @@ __global__ __launch_bounds__(T) void k_mark_tiles(int M, GridDev g, TileGeom tg,
         }
     }
 }
+__global__ __launch_bounds__(T) void k_mark_tiles(int M, GridDev g, TileGeom tg, const int *__restrict__ mcell,
+                                                  unsigned char *__restrict__ flag) {
+    mark_tiles_body(hw_blk(), M, g, tg, mcell, flag);
+}
 
 // per-block counts of set flags (block b owns flags [b*CHUNK, (b+1)*CHUNK))
-__global__ __launch_bounds__(T) void k_flag_count(long n, const unsigned char *__restrict__ flag, int *__restrict__ cnt) {
+__device__ __forceinline__ void flag_count_body(const Blk &kb, long n, const unsigned char *__restrict__ flag,
+                                                int *__restrict__ cnt) {
     __shared__ int red[T];
-    const long lo = (long)blockIdx.x * CHUNK;
+    const long lo = (long)kb.bx * CHUNK;
     int s = 0;
-    for (long q = lo + threadIdx.x; q < min(n, lo + CHUNK); q += T) s += flag[q];
-    red[threadIdx.x] = s;
+    for (long q = lo + kb.tx; q < min(n, lo + CHUNK); q += T) s += flag[q];
+    red[kb.tx] = s;
     __syncthreads();
     for (int st = T / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        if (kb.tx < st) red[kb.tx] += red[kb.tx + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) cnt[blockIdx.x] = red[0];
+    if (kb.tx == 0) cnt[kb.bx] = red[0];
+}
+__global__ __launch_bounds__(T) void k_flag_count(long n, const unsigned char *__restrict__ flag, int *__restrict__ cnt) {
+    flag_count_body(hw_blk(), n, flag, cnt);
 }
 
 // exclusive scan of nb ints in one block; total into a[nb] and *total
-__global__ __launch_bounds__(1024) void k_scan_small(int nb, int *__restrict__ a, int *__restrict__ total) {
+__device__ __forceinline__ void scan_small_body(const Blk &kb, int nb, int *__restrict__ a, int *__restrict__ total) {
     __shared__ int buf[1024];
     __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
+    const int t = kb.tx;
+    if (t == 0) carry = 0;
     __syncthreads();
     for (int base = 0; base < nb; base += 1024) {
-        const int q = base + threadIdx.x;
+        const int q = base + t;
         int v = q < nb ? a[q] : 0;
-        buf[threadIdx.x] = v;
+        buf[t] = v;
         __syncthreads();
         for (int st = 1; st < 1024; st <<= 1) {
-            int t = threadIdx.x >= st ? buf[threadIdx.x - st] : 0;
+            int y = t >= st ? buf[t - st] : 0;
             __syncthreads();
-            buf[threadIdx.x] += t;
+            buf[t] += y;
             __syncthreads();
         }
-        if (q < nb) a[q] = carry + buf[threadIdx.x] - v;
+        if (q < nb) a[q] = carry + buf[t] - v;
         __syncthreads();
-        if (threadIdx.x == 1023) carry += buf[1023];
+        if (t == 1023) carry += buf[1023];
         __syncthreads();
     }
-    if (threadIdx.x == 0) { a[nb] = carry; *total = carry; }
+    if (t == 0) { a[nb] = carry; *total = carry; }
+}
+__global__ __launch_bounds__(1024) void k_scan_small(int nb, int *__restrict__ a, int *__restrict__ total) {
+    scan_small_body(hw_blk(), nb, a, total);
 }
 
 // ordered compaction of set flags into list; clears the flags
-__global__ __launch_bounds__(T) void k_flag_compact(long n, unsigned char *__restrict__ flag,
-                                                    const int *__restrict__ off, int *__restrict__ list) {
+__device__ __forceinline__ void flag_compact_body(const Blk &kb, long n, unsigned char *__restrict__ flag,
+                                                  const int *__restrict__ off, int *__restrict__ list) {
     __shared__ int buf[T];
     constexpr int PER = CHUNK / T;   // 16 consecutive flags per thread
-    const long lo = (long)blockIdx.x * CHUNK + (long)threadIdx.x * PER;
+    const int tx = kb.tx;
+    const long lo = (long)kb.bx * CHUNK + (long)tx * PER;
     int s = 0;
     for (int q = 0; q < PER; ++q)
         if (lo + q < n) s += flag[lo + q];
-    buf[threadIdx.x] = s;
+    buf[tx] = s;
     __syncthreads();
     for (int st = 1; st < T; st <<= 1) {
-        int t = threadIdx.x >= st ? buf[threadIdx.x - st] : 0;
+        int t = tx >= st ? buf[tx - st] : 0;
         __syncthreads();
-        buf[threadIdx.x] += t;
+        buf[tx] += t;
         __syncthreads();
     }
-    int pos = off[blockIdx.x] + buf[threadIdx.x] - s;
+    int pos = off[kb.bx] + buf[tx] - s;
     for (int q = 0; q < PER; ++q)
         if (lo + q < n && flag[lo + q]) {
             list[pos++] = (int)(lo + q);
             flag[lo + q] = 0;
         }
+}
+__global__ __launch_bounds__(T) void k_flag_compact(long n, unsigned char *__restrict__ flag,
+                                                    const int *__restrict__ off, int *__restrict__ list) {
+    flag_compact_body(hw_blk(), n, flag, off, list);
 }
 
 __global__ __launch_bounds__(T) void k_zero(long n, double *__restrict__ a) {
@@ -550,37 +575,45 @@ __device__ __forceinline__ void tile_origin(int tile, TileGeom tg, int &tx, int 
 
 // entries per active tile; slots nact..cap-1 get 0 (slot q is written by
 // block q / 64 whatever that block's own tile is, so each slot has one writer)
-__global__ __launch_bounds__(64) void k_tile_count(GridDev g, TileGeom tg, const int *__restrict__ meta,
-                                                   const int *__restrict__ tiles, const int *__restrict__ bin_start,
-                                                   int *__restrict__ ecount, long cap) {
+// (64-thread blocks: tile_bins' lane is the thread index)
+__device__ __forceinline__ void tile_count_body(const Blk &kb, const GridDev &g, TileGeom tg,
+                                                const int *__restrict__ meta, const int *__restrict__ tiles,
+                                                const int *__restrict__ bin_start, int *__restrict__ ecount,
+                                                long cap) {
     __shared__ int s_off[NSRC + 1];
     __shared__ int s_gstart[NSRC];
     const int nact = meta[0];
-    const long q = (long)blockIdx.x * 64 + threadIdx.x;
+    const long q = (long)kb.bx * 64 + kb.tx;
     if (q >= nact && q < cap) ecount[q] = 0;
-    const int b = tile_slot(blockIdx.x, nact);
+    const int b = tile_slot(kb.bx, nact);
     if (b < 0) return;
     int tx, ty, tz;
     tile_origin(tiles[b], tg, tx, ty, tz);
     tile_bins(g, tx * TL - 1, ty * TL - 1, tz * TL - 1, bin_start, s_off, s_gstart);
-    if (threadIdx.x == 0) ecount[b] = s_off[NSRC];
+    if (kb.tx == 0) ecount[b] = s_off[NSRC];
+}
+__global__ __launch_bounds__(64) void k_tile_count(GridDev g, TileGeom tg, const int *__restrict__ meta,
+                                                   const int *__restrict__ tiles, const int *__restrict__ bin_start,
+                                                   int *__restrict__ ecount, long cap) {
+    tile_count_body(hw_blk(), g, tg, meta, tiles, bin_start, ecount, cap);
 }
 
 // write each active tile's entries at its scanned offset
-__global__ __launch_bounds__(64) void k_tile_fill(GridDev g, TileGeom tg, const int *__restrict__ meta,
-                                                  const int *__restrict__ tiles, const int *__restrict__ bin_start,
-                                                  const int *__restrict__ bin_items, const int *__restrict__ estart,
-                                                  uint32_t *__restrict__ ent, long ent_cap, int *__restrict__ flags) {
+__device__ __forceinline__ void tile_fill_body(const Blk &kb0, const GridDev &g, TileGeom tg,
+                                               const int *__restrict__ meta, const int *__restrict__ tiles,
+                                               const int *__restrict__ bin_start, const int *__restrict__ bin_items,
+                                               const int *__restrict__ estart, uint32_t *__restrict__ ent,
+                                               long ent_cap, int *__restrict__ flags) {
     __shared__ int s_off[NSRC + 1];
     __shared__ int s_gstart[NSRC];
-    const int b = tile_slot(blockIdx.x, meta[0]);
+    const int b = tile_slot(kb0.bx, meta[0]);
     if (b < 0) return;
     int tx, ty, tz;
     tile_origin(tiles[b], tg, tx, ty, tz);
     tile_bins(g, tx * TL - 1, ty * TL - 1, tz * TL - 1, bin_start, s_off, s_gstart);
     const long base = estart[b];
     if (base + s_off[NSRC] > ent_cap) {
-        if (threadIdx.x == 0) atomicOr(&flags[FL_SPREAD], 1);
+        if (kb0.tx == 0) atomicOr(&flags[FL_SPREAD], 1);
         return;
     }
     const int ib = tx * TL - 1, jb = ty * TL - 1, kb = tz * TL - 1;
@@ -588,7 +621,7 @@ __global__ __launch_bounds__(64) void k_tile_fill(GridDev g, TileGeom tg, const 
     // last source cell s with s_off[s] <= e (a binary search over the LDS
     // offsets; empty cells share their offset with the next cell)
     const int total = s_off[NSRC];
-    for (int e = threadIdx.x; e < total; e += 64) {
+    for (int e = kb0.tx; e < total; e += 64) {
         int lo = 0, hi = NSRC - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -599,6 +632,12 @@ __global__ __launch_bounds__(64) void k_tile_fill(GridDev g, TileGeom tg, const 
         src_coords(lo, ib, jb, kb, scc);
         ent[base + e] = ((uint32_t)bin_items[s_gstart[lo] + (e - s_off[lo])] << SBITS) | image_code(g, scc);
     }
+}
+__global__ __launch_bounds__(64) void k_tile_fill(GridDev g, TileGeom tg, const int *__restrict__ meta,
+                                                  const int *__restrict__ tiles, const int *__restrict__ bin_start,
+                                                  const int *__restrict__ bin_items, const int *__restrict__ estart,
+                                                  uint32_t *__restrict__ ent, long ent_cap, int *__restrict__ flags) {
+    tile_fill_body(hw_blk(), g, tg, meta, tiles, bin_start, bin_items, estart, ent, ent_cap, flags);
 }
 
 struct SpreadArgs {
@@ -1317,4 +1356,177 @@ int ibm_plan_mdf(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::str
             if ((rc = add_op(P.ar, P.ops, fs[v], 64, sa[v], o, err, true))) return rc;
     }
     return IBMDEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The IBM set-up of a batched step (ibm_begin_step of every case of a group):
+// marker records, active tiles and their entry lists, each kernel one launch
+// over the cases with markers. A case's arguments are one IbmSetup in the
+// plan's arena; each launch's table holds the case's grid of blocks and its
+// index there. The tile kernels map their blocks to XCDs themselves
+// (tile_slot), so their physical blocks are the case's own.
+// ---------------------------------------------------------------------------
+namespace {
+struct IbmSetup {
+    int M, nb, enb;   // markers, flag chunks, entry-count chunks
+    long nt, cap, ent_cap;
+    GridDev g;
+    TileGeom tg;
+    const double *mx;
+    const int *mcell, *bin_start, *bin_items;
+    double *rec, *fac, *fdrag;
+    unsigned char *flag;
+    int *cnt, *meta, *list, *ecount, *estart, *flags;
+    uint32_t *ent;
+};
+struct SArg {
+    BGrid g;
+    int q;   // the case's IbmSetup
+};
+using STab = BTable<SArg>;
+
+template <bool P2>
+__global__ __launch_bounds__(T) void kb_records(STab tab, const IbmSetup *cs) {
+    batch_for(tab, [&](const SArg &a, const Blk &kb) {
+        const IbmSetup &c = *cst(cs + a.q);
+        records_body<P2>(kb, c.M, c.g, glb(c.mx), glb(c.mcell), glb(c.rec), glb(c.fac), glb(c.fdrag));
+    });
+}
+__global__ __launch_bounds__(T) void kb_mark_tiles(STab tab, const IbmSetup *cs) {
+    batch_for(tab, [&](const SArg &a, const Blk &kb) {
+        const IbmSetup &c = *cst(cs + a.q);
+        mark_tiles_body(kb, c.M, c.g, c.tg, glb(c.mcell), glb(c.flag));
+    });
+}
+__global__ __launch_bounds__(T) void kb_flag_count(STab tab, const IbmSetup *cs) {
+    batch_for(tab, [&](const SArg &a, const Blk &kb) {
+        const IbmSetup &c = *cst(cs + a.q);
+        flag_count_body(kb, c.nt, glb((const unsigned char *)c.flag), glb(c.cnt));
+    });
+}
+__global__ __launch_bounds__(1024) void kb_scan_small(STab tab, const IbmSetup *cs) {
+    batch_for(tab, [&](const SArg &a, const Blk &kb) {
+        const IbmSetup &c = *cst(cs + a.q);
+        scan_small_body(kb, c.nb, glb(c.cnt), glb(c.meta));
+    });
+}
+__global__ __launch_bounds__(T) void kb_flag_compact(STab tab, const IbmSetup *cs) {
+    batch_for(tab, [&](const SArg &a, const Blk &kb) {
+        const IbmSetup &c = *cst(cs + a.q);
+        flag_compact_body(kb, c.nt, glb(c.flag), glb((const int *)c.cnt), glb(c.list));
+    });
+}
+__global__ __launch_bounds__(64) void kb_tile_count(STab tab, const IbmSetup *cs) {
+    batch_for(tab, [&](const SArg &a, const Blk &kb) {
+        const IbmSetup &c = *cst(cs + a.q);
+        tile_count_body(kb, c.g, c.tg, glb((const int *)c.meta), glb((const int *)c.list), glb(c.bin_start),
+                        glb(c.ecount), c.cap);
+    });
+}
+__global__ __launch_bounds__(T) void kb_ecount_sum(STab tab, const IbmSetup *cs) {
+    batch_for(tab, [&](const SArg &a, const Blk &kb) {
+        const IbmSetup &c = *cst(cs + a.q);
+        scan::chunk_sum_body(kb, c.cap, glb((const int *)c.ecount), glb(c.cnt));
+    });
+}
+__global__ __launch_bounds__(1024) void kb_ecount_scan_blocks(STab tab, const IbmSetup *cs) {
+    batch_for(tab, [&](const SArg &a, const Blk &kb) {
+        const IbmSetup &c = *cst(cs + a.q);
+        scan::scan_blocks_body(kb, c.enb, glb(c.cnt));
+    });
+}
+__global__ __launch_bounds__(T) void kb_ecount_scan(STab tab, const IbmSetup *cs) {
+    batch_for(tab, [&](const SArg &a, const Blk &kb) {
+        const IbmSetup &c = *cst(cs + a.q);
+        scan::chunk_scan_body(kb, c.cap, glb((const int *)c.ecount), glb((const int *)c.cnt), glb(c.estart), c.enb);
+    });
+}
+__global__ __launch_bounds__(64) void kb_tile_fill(STab tab, const IbmSetup *cs) {
+    batch_for(tab, [&](const SArg &a, const Blk &kb) {
+        const IbmSetup &c = *cst(cs + a.q);
+        tile_fill_body(kb, c.g, c.tg, glb((const int *)c.meta), glb((const int *)c.list), glb(c.bin_start),
+                       glb(c.bin_items), glb((const int *)c.estart), glb(c.ent), c.ent_cap, glb(c.flags));
+    });
+}
+
+#define IBM_SETUP_LAUNCHER(name, kernel, threads)                                                     \
+    void launch_##name(const BOp &o, hipStream_t s, int) {                                           \
+        hipLaunchKernelGGL(kernel, dim3(o.blocks), dim3(threads), 0, s, table<SArg>(o),               \
+                           (const IbmSetup *)o.aux);                                                 \
+    }
+IBM_SETUP_LAUNCHER(records_p2, kb_records<true>, T)
+IBM_SETUP_LAUNCHER(records, kb_records<false>, T)
+IBM_SETUP_LAUNCHER(mark_tiles, kb_mark_tiles, T)
+IBM_SETUP_LAUNCHER(flag_count, kb_flag_count, T)
+IBM_SETUP_LAUNCHER(scan_small, kb_scan_small, 1024)
+IBM_SETUP_LAUNCHER(flag_compact, kb_flag_compact, T)
+IBM_SETUP_LAUNCHER(tile_count, kb_tile_count, 64)
+IBM_SETUP_LAUNCHER(ecount_sum, kb_ecount_sum, T)
+IBM_SETUP_LAUNCHER(ecount_scan_blocks, kb_ecount_scan_blocks, 1024)
+IBM_SETUP_LAUNCHER(ecount_scan, kb_ecount_scan, T)
+IBM_SETUP_LAUNCHER(tile_fill, kb_tile_fill, 64)
+#undef IBM_SETUP_LAUNCHER
+}  // namespace
+
+// ibm_begin_step of every context with markers, in its order
+int ibm_plan_begin_step(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    std::vector<IbmSetup> v;
+    std::vector<int> idx;   // the contexts with markers
+    for (int q = 0; q < (int)cs.size(); ++q) {
+        ibmdem_ctx *c = cs[q];
+        if (c->M == 0) continue;
+        IbmSetup d{};
+        d.M = c->M;
+        d.g = c->g;
+        d.tg = tile_geom(c->g);
+        d.nt = (long)d.tg.ntx * d.tg.nty * d.tg.ntz;
+        d.nb = div_up(d.nt, CHUNK);
+        d.cap = c->act_cap;
+        d.enb = div_up(c->act_cap, CHUNK);
+        if (d.enb + 1 > c->scan_nblocks) {
+            err = "batched IBM set-up: scan longer than its block buffer";
+            return IBMDEM_ERR_CAPACITY;
+        }
+        d.ent_cap = c->ent_cap;
+        d.mx = c->mx;
+        d.mcell = c->mcell; d.bin_start = c->bin_start; d.bin_items = c->bin_items;
+        d.rec = c->mrec; d.fac = c->mfac; d.fdrag = c->fdrag;
+        d.flag = c->act_flag;
+        d.cnt = c->scan_blocks; d.meta = c->act_meta; d.list = c->act_list;
+        d.ecount = c->tile_ecount; d.estart = c->tile_estart; d.flags = c->flags;
+        d.ent = c->tile_ent;
+        idx.push_back(q);
+        v.push_back(d);
+    }
+    if (v.empty()) return IBMDEM_OK;
+    const size_t aux = P.ar.put(v.data(), v.size() * sizeof(IbmSetup));
+    // one launch of `threads` per block over the cases (those with pow2
+    // spacing or not, for the records) with grid(case)
+    auto op = [&](BLaunch fn, int threads, auto grid, int p2, bool ident) {
+        std::vector<SArg> args;
+        for (int i = 0; i < (int)v.size(); ++i) {
+            if (p2 >= 0 && (pow2_grid(cs[idx[i]]) ? 1 : 0) != p2) continue;
+            args.push_back(SArg{bgrid(grid(cs[idx[i]], v[i]), dim3(threads)), i});
+        }
+        BOp o;
+        o.aux_off = aux;
+        return add_op(P.ar, P.ops, fn, threads, args, o, err, ident);
+    };
+    auto markers = [](ibmdem_ctx *c, const IbmSetup &) { return dim3(div_up(c->M, T)); };
+    auto flags = [](ibmdem_ctx *, const IbmSetup &d) { return dim3(d.nb); };
+    auto one = [](ibmdem_ctx *, const IbmSetup &) { return dim3(1); };
+    auto tiles = [](ibmdem_ctx *c, const IbmSetup &) { return dim3(c->act_grid); };
+    auto ent = [](ibmdem_ctx *, const IbmSetup &d) { return dim3(d.enb); };
+    int rc;
+    if ((rc = op(launch_records_p2, T, markers, 1, false))) return rc;
+    if ((rc = op(launch_records, T, markers, 0, false))) return rc;
+    if ((rc = op(launch_mark_tiles, T, markers, -1, false))) return rc;
+    if ((rc = op(launch_flag_count, T, flags, -1, false))) return rc;
+    if ((rc = op(launch_scan_small, 1024, one, -1, false))) return rc;
+    if ((rc = op(launch_flag_compact, T, flags, -1, false))) return rc;
+    if ((rc = op(launch_tile_count, 64, tiles, -1, true))) return rc;
+    if ((rc = op(launch_ecount_sum, T, ent, -1, false))) return rc;
+    if ((rc = op(launch_ecount_scan_blocks, 1024, one, -1, false))) return rc;
+    if ((rc = op(launch_ecount_scan, T, ent, -1, false))) return rc;
+    return op(launch_tile_fill, 64, tiles, -1, true);
 }

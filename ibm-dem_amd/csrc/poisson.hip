@@ -1054,16 +1054,21 @@ __global__ __launch_bounds__(RED_T) void k_cg_update_r(long n, double *__restric
 }
 
 // partial sums of a.b
-__global__ __launch_bounds__(RED_T) void k_dot(long n, const double *__restrict__ a, const double *__restrict__ b,
-                                               double *__restrict__ partial, const double *__restrict__ scal) {
+__device__ __forceinline__ void dot_body(const Blk &kb, long n, const double *__restrict__ a,
+                                         const double *__restrict__ b, double *__restrict__ partial,
+                                         const double *__restrict__ scal) {
     if (scal[SC_CONV] != 0.0) return;
     __shared__ double red[RED_T];
-    const long chunk = (n + gridDim.x - 1) / gridDim.x;
-    const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+    const long chunk = (n + kb.gx - 1) / kb.gx;
+    const long lo = (long)kb.bx * chunk, hi = min(n, lo + chunk);
     double s = 0.0;
-    for (long q = lo + threadIdx.x; q < hi; q += RED_T) s += a[q] * b[q];
+    for (long q = lo + kb.tx; q < hi; q += RED_T) s += a[q] * b[q];
     double t = block_reduce(s, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (kb.tx == 0) partial[kb.bx] = t;
+}
+__global__ __launch_bounds__(RED_T) void k_dot(long n, const double *__restrict__ a, const double *__restrict__ b,
+                                               double *__restrict__ partial, const double *__restrict__ scal) {
+    dot_body(hw_blk(), n, a, b, partial, scal);
 }
 
 // finalize a partial sum into scal[slot]; op: 0 store, 1 alpha = rz/pAp,
@@ -1109,13 +1114,13 @@ __global__ __launch_bounds__(FIN_T) void k_final(const double *__restrict__ part
 // r = rhs with the pinned row, x = 0; r and x in level-0 storage, rhs in
 // field order (cell (0,0,0) is position 0 of both). A thread takes the
 // x-pair (2p, 2p+1) of row (j, k), blocks of shape_for over the level's x-pairs.
-__global__ __launch_bounds__(RB) void k_cg_init(LevelDev L, const double *__restrict__ rhs,
-                                                double *__restrict__ r, double *__restrict__ x,
-                                                double *__restrict__ scal, float *__restrict__ r32) {
-    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+__device__ __forceinline__ void cg_init_body(const Blk &kb, const LevelDev &L, const double *__restrict__ rhs,
+                                             double *__restrict__ r, double *__restrict__ x,
+                                             double *__restrict__ scal, float *__restrict__ r32) {
+    const int ip = kb.bx * kb.dx + kb.tx;
+    const int j = kb.by * kb.dy + kb.ty, k = kb.bz;
     const double pin = scal[SC_PIN];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    if (kb.bx == 0 && kb.by == 0 && kb.bz == 0 && kb.tx == 0 && kb.ty == 0) {
         scal[SC_CONV] = 0.0;
         scal[SC_ITER] = 0.0;
         scal[SC_DONE] = 0.0;
@@ -1133,6 +1138,11 @@ __global__ __launch_bounds__(RB) void k_cg_init(LevelDev L, const double *__rest
         if (r32) r32[q] = float(rq);   // the mixed form's FP32 copy
         x[q] = 0.0;
     }
+}
+__global__ __launch_bounds__(RB) void k_cg_init(LevelDev L, const double *__restrict__ rhs,
+                                                double *__restrict__ r, double *__restrict__ x,
+                                                double *__restrict__ scal, float *__restrict__ r32) {
+    cg_init_body(hw_blk(), L, rhs, r, x, scal, r32);
 }
 
 __device__ __forceinline__ void copy_body(const Blk &kb, long n, double *__restrict__ d, const double *__restrict__ s) {
@@ -1701,6 +1711,7 @@ struct CgArg {
     float *r32;
     int np;           // partial sums k_final adds
     double rtol;
+    const double *rhs;   // the solve's right-hand side (field order)
 };
 __global__ __launch_bounds__(RB) void kb_matvec_dot(BTable<CgArg> tab) {
     batch_for(tab, [&](const CgArg &a, const Blk &kb) {
@@ -2223,4 +2234,48 @@ int poisson_plan_gauge(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, st
         args.push_back(GaugeArg{bgrid(fs.grid, fs.block), c->dlev, c->cg_x, c->cg_p, c->scal, c->pcorr});
     }
     return add_op(P.ar, P.ops, launch_gauge, RB, args, BOp(), err);
+}
+
+// poisson_begin_rhs of every context of a group: the pinned row's sum (the
+// contexts without a Dirichlet side), r = b and x = 0, then ||b||
+namespace {
+__global__ __launch_bounds__(RB) void kb_cg_init(BTable<CgArg> tab) {
+    batch_for(tab, [&](const CgArg &a, const Blk &kb) {
+        cg_init_body(kb, *cst(a.L), glb(a.rhs), glb(a.r), glb(a.x), glb(a.scal), glb(a.r32));
+    });
+}
+__global__ __launch_bounds__(RED_T) void kb_dot_rr(BTable<CgArg> tab) {
+    batch_for(tab, [&](const CgArg &a, const Blk &kb) {
+        dot_body(kb, a.n, glb((const double *)a.r), glb((const double *)a.r), glb(a.partial), glb(a.scal));
+    });
+}
+void launch_cg_init(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_cg_init, dim3(o.blocks), dim3(RB), 0, s, table<CgArg>(o));
+}
+void launch_dot_rr(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_dot_rr, dim3(o.blocks), dim3(RED_T), 0, s, table<CgArg>(o));
+}
+}  // namespace
+
+int poisson_plan_rhs(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    std::vector<ReduceJob> pin;
+    std::vector<CgArg> init, dot, fin;
+    for (ibmdem_ctx *c : cs) {
+        const LevelDev &L0 = c->lev64[0].d;
+        if (!L0.dirichlet) pin.push_back(ReduceJob{c->rhs, nullptr, c->n, 1, c->partial, c->scal + SC_PIN});
+        const LaunchShape fs = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);
+        CgArg a = cg_arg(c, fs.grid, fs.block, 0);
+        a.rhs = c->rhs;
+        init.push_back(a);
+        dot.push_back(cg_arg(c, dim3(RED_BLOCKS), dim3(RED_T), 0));
+        fin.push_back(cg_arg(c, dim3(1), dim3(FIN_T), RED_BLOCKS));
+    }
+    int rc;
+    if ((rc = plan_reduce_sum(P, pin, err))) return rc;
+    if ((rc = add_op(P.ar, P.ops, launch_cg_init, RB, init, BOp(), err))) return rc;
+    if ((rc = add_op(P.ar, P.ops, launch_dot_rr, RED_T, dot, BOp(), err))) return rc;
+    BOp o;
+    o.slot = SC_BNORM2;
+    o.op = 4;
+    return add_op(P.ar, P.ops, launch_final, FIN_T, fin, o, err);
 }

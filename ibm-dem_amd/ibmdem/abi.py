@@ -535,8 +535,9 @@ class Batch:
 
     def set_fused(self, mode) -> None:
         """0/False: each case's step on its own stream; 1/True (the default):
-        the forcing loop, the solve and the post-solve part batched; 2: only
-        the solve; 3: the solve and the post-solve part."""
+        the whole step batched; 2: only the solve; 3: the solve and the
+        post-solve part; 4: the forcing loop, the solve and the post-solve
+        part."""
         self._check(self.lib.ibmdem_batch_set_fused(self.h, int(mode)), "ibmdem_batch_set_fused")
 
     def fused_solves(self) -> int:

@@ -308,10 +308,12 @@ int ibmdem_batch_run(ibmdem_batch *batch, int32_t n_steps, ibmdem_solve_info *in
  * the state is restored after every trial. */
 int ibmdem_batch_autotune(ibmdem_batch *batch, int32_t trial_steps);
 /* mode 0: every case's step runs on its own stream, as a single context's
- * does (for comparison); 1 (the default): the multi-direct-forcing loop,
- * the solve and the rest of the step after it (projection, the DEM phase,
- * probes) run batched; 2: only the solve runs batched; 3: the solve and the
- * part after it. */
+ * does (for comparison); 1 (the default): the whole step runs batched (the
+ * IBM set-up, the multi-direct-forcing loop, predictor, divergence and the
+ * solve's right-hand side; the solve; the rest of the step after it:
+ * projection, the DEM phase, probes); 2: only the solve runs batched; 3: the
+ * solve and the part after it; 4: the forcing loop, the solve and the part
+ * after it (the rest of the first part on each case's own stream). */
 int ibmdem_batch_set_fused(ibmdem_batch *batch, int32_t mode);
 /* The cases that share precision, smoothing counts and iteration cap solve
  * in `groups` (1..64, default 1) batched groups of about equal cell count,

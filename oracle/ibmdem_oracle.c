@@ -1901,7 +1901,7 @@ int ibmdem_batch_autotune(ibmdem_batch *b, int32_t trial_steps) {
 
 /* the oracle has no batched solve: one case after another either way */
 int ibmdem_batch_set_fused(ibmdem_batch *b, int32_t mode) {
-    if (!b || mode < 0 || mode > 3) return IBMDEM_ERR_ARG;
+    if (!b || mode < 0 || mode > 4) return IBMDEM_ERR_ARG;
     return IBMDEM_OK;
 }
 int ibmdem_batch_set_solve_groups(ibmdem_batch *b, int32_t groups) {

@@ -32,12 +32,14 @@ def _single(lib, setup, n):
     return out
 
 
-def _batched(lib, setups, n, tune=0, fused=True, mg=None, solves=None, groups=1):
+def _batched(lib, setups, n, tune=0, fused=True, mg=None, solves=None, groups=1, tiles=None):
     with abi.Batch(lib, [s.case for s in setups]) as b:
         for q, s in enumerate(setups):
             cases.load(b[q], s)
             if mg and q in mg:
                 b[q].set_mg_params(*mg[q])
+            if tiles and q in tiles:
+                b[q].set_stencil_tile(tiles[q])
         b.set_fused(fused)
         b.set_solve_groups(groups)
         assert b.solve_groups() == groups
@@ -101,15 +103,49 @@ def test_batch_equals_single_cases_on_the_gpu(gpu_lib, groups):
 @pytest.mark.gpu
 def test_batched_solve_equals_per_case_solves(gpu_lib):
     """ibmdem_batch_set_fused(0) runs each case's step on its own stream,
-    (2) batches only the solve, (3) the solve and the post-solve part, (1)
-    the forcing loop as well; all give the same bits."""
+    (2) batches only the solve, (3) the solve and the post-solve part, (4)
+    the forcing loop as well, (1) the whole step; all give the same bits."""
     setups = _cases()
     solves = []
     a = _batched(gpu_lib, setups, 3, fused=0, solves=solves)
     assert solves[0] == 0
-    for mode in (2, 3, 1):
+    for mode in (2, 3, 4, 1):
         for x, y in zip(a, _batched(gpu_lib, setups, 3, fused=mode)):
             _assert_same(x, y)
+
+
+def _spaced(setup, h):
+    """The setup on a grid of spacing h (not a power of two): positions
+    scaled with the grid, so every marker keeps its cell."""
+    c = setup.case
+    c.dx = c.dy = c.dz = float(h)
+    setup.markers.x *= h
+    setup.bodies.x *= h
+    return setup
+
+
+@pytest.mark.gpu
+def test_batched_first_part_variants(gpu_lib):
+    """The whole step batched over cases that take different kernel variants
+    of its first part: bulk forcing (periodic), a grid spacing that is not a
+    power of two (closed box), no particles at all (open channel), and the
+    four predictor tile schedules; each case equals stepping it alone with
+    the same schedule."""
+    setups = [cases.small_periodic(gap=0.75),
+              _spaced(cases.small_walls(), 1.25),
+              cases.small_open(axis=0, spheres=False),
+              cases.small_periodic(n=(40, 32, 24), n_spheres=4, D=7.0, seed=cases.SEED_BASE + 21, gap=1.0)]
+    tiles = {0: 1, 1: 2, 2: 3, 3: 0}
+    solves = []
+    got = _batched(gpu_lib, setups, 4, solves=solves, tiles=tiles)
+    assert solves[0] >= 3
+    for q, (s, g) in enumerate(zip(setups, got)):
+        with abi.Simulation(gpu_lib, s.case) as one:
+            cases.load(one, s)
+            one.set_stencil_tile(tiles[q])
+            info = one.run(4)
+            want = (one.download_fields(), one.download_markers().x, one.download_bodies().x, info.iterations)
+        _assert_same(g, want)
 
 
 @pytest.mark.gpu
