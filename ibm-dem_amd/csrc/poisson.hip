@@ -219,13 +219,207 @@ __device__ __forceinline__ T nbsum_pair_vals(const PairGeo &g, int h, T part, T 
 // cells of `color`, x unchanged. Together they give r = b - A x.
 constexpr int RB_ROWS = 2;
 static_assert(RB_ROWS == 2, "the half sweep shares loads between its two adjacent rows");
-template <typename T, typename TB, bool DOT = false, int RES = 0, bool ZI = false>
+// Two-value accesses of a colour half (8 bytes in FP32, 16 in FP64).
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> { using type = float2; };
+template <>
+struct Vec2<double> { using type = double2; };
+template <typename T>
+__device__ __forceinline__ void ld2(const T *p, T &v0, T &v1) {
+    const typename Vec2<T>::type v = *reinterpret_cast<const typename Vec2<T>::type *>(p);
+    v0 = v.x;
+    v1 = v.y;
+}
+template <typename T>
+__device__ __forceinline__ void st2(T *p, T v0, T v1) {
+    typename Vec2<T>::type v;
+    v.x = v0;
+    v.y = v1;
+    *reinterpret_cast<typename Vec2<T>::type *>(p) = v;
+}
+
+// A half sweep with each thread on two adjacent x-pairs (pc, pc + 1, pc
+// even) of its RB_ROWS rows, on split levels whose rows hold an even number
+// of pairs: the two pairs' cells are adjacent in every colour half (and
+// every row starts at an even position), so each load and store of the
+// thread moves both values at once, half the memory instructions per cell
+// of rb_half_body. Each cell's neighbours, their association and the
+// update are rb_half_body's (nbsum_pair_vals), so x, r and z are the same
+// bit for bit; the r.z partial sums (DOT) follow this kernel's blocks.
+template <typename T, bool DOT, int RES, bool ZI>
+__device__ __forceinline__ void rb_half2_body(const Blk &kb, const LevelDev &L, T *__restrict__ x,
+                                              const T *__restrict__ b, int color, double *__restrict__ partial,
+                                              long pofs, T *__restrict__ r, const double *__restrict__ r64,
+                                              double *__restrict__ z64) {
+    const T id_int = T(L.idiag[DIAG_INTERIOR]), dg_int = T(L.diag[DIAG_INTERIOR]);
+    const T ihx2 = T(L.ihx2), ihy2 = T(L.ihy2), ihz2 = T(L.ihz2);
+    const int rl = L.nx >> 1, pl = rl * L.ny;
+    const int p0 = 2 * (kb.bx * kb.dx + kb.tx), k = kb.bz;
+    const int j0 = (kb.by * kb.dy + kb.ty) * RB_ROWS;
+    const bool px = L.bt[0] == IBMDEM_BC_PERIODIC, py = L.bt[2] == IBMDEM_BC_PERIODIC,
+               pz = L.bt[4] == IBMDEM_BC_PERIODIC;
+    // pairs past the row's end load from the last two (valid addresses)
+    // and are dropped at the stores
+    const int pc = min(p0, rl - 2);
+    T *own = x + (long)color * L.half;
+    T *oth = x + (long)(1 - color) * L.half;
+    // x neighbours outside the two pairs: the odd cell of pair pc - 1 (west
+    // of an even cell of pair pc) and the even cell of pair pc + 2 (east of
+    // an odd cell of pair pc + 1), as offsets from pair pc
+    const bool hw = pc > 0 || px, he = pc + 2 < rl || px;
+    const int dw = pc > 0 ? -1 : rl - 1, de = pc + 2 < rl ? 2 : 2 - rl;
+    const bool hb = k > 0 || pz, ht = k < L.nz - 1 || pz;
+    const int db = k > 0 ? -pl : (L.nz - 1) * pl, dt = k < L.nz - 1 ? pl : -(L.nz - 1) * pl;
+    const int cz = diag_count(k, L.nz, L.bt[4], L.bt[5]);
+    bool ok[RB_ROWS], hs[RB_ROWS], hn[RB_ROWS];
+    int pos[RB_ROWS], h[RB_ROWS], cyz[RB_ROWS];
+    T bo[RB_ROWS][2], av[RB_ROWS][2], xo[RB_ROWS], ys[RB_ROWS][2], yn[RB_ROWS][2], zb[RB_ROWS][2],
+        zt[RB_ROWS][2], ov[RB_ROWS][2];
+    const bool adj = j0 + 1 < L.ny;   // both rows exist: they share their y loads
+#pragma unroll
+    for (int rr = 0; rr < RB_ROWS; ++rr) {
+        const int j = min(j0 + rr, L.ny - 1);
+        ok[rr] = j0 + rr < L.ny && p0 < rl;
+        pos[rr] = (k * L.ny + j) * rl + pc;
+        h[rr] = ((j + k) & 1) == color ? 0 : 1;
+        hs[rr] = j > 0 || py;
+        hn[rr] = j < L.ny - 1 || py;
+        cyz[rr] = diag_count(j, L.ny, L.bt[2], L.bt[3]) * 4 + cz;
+        const T *q = oth + pos[rr];
+        ld2(b + (long)color * L.half + pos[rr], bo[rr][0], bo[rr][1]);
+        if (RES == 2) ld2(own + pos[rr], ov[rr][0], ov[rr][1]);
+        if (!ZI || RES == 2) {
+            ld2(q, av[rr][0], av[rr][1]);
+            xo[rr] = h[rr] == 0 ? q[hw ? dw : 0] : q[he ? de : 0];
+            ld2(q + (hb ? db : 0), zb[rr][0], zb[rr][1]);
+            ld2(q + (ht ? dt : 0), zt[rr][0], zt[rr][1]);
+        }
+    }
+    T sn[RB_ROWS][2];
+    if (ZI && RES != 2) {
+#pragma unroll
+        for (int rr = 0; rr < RB_ROWS; ++rr) sn[rr][0] = sn[rr][1] = T(0);
+    } else {
+        // row 0's north neighbours are row 1's pair values and row 1's
+        // south neighbours row 0's (when both rows exist)
+        const T *q0 = oth + pos[0], *q1 = oth + pos[1];
+        const int jr0 = min(j0, L.ny - 1), jr1 = min(j0 + 1, L.ny - 1);   // the rows' (clamped) indices
+        const int ds0 = jr0 > 0 ? -rl : (L.ny - 1) * rl;
+        const int dn1 = jr1 < L.ny - 1 ? rl : -(L.ny - 1) * rl;
+        ld2(q0 + (hs[0] ? ds0 : 0), ys[0][0], ys[0][1]);
+        ld2(q1 + (hn[1] ? dn1 : 0), yn[1][0], yn[1][1]);
+        if (adj) {
+            yn[0][0] = av[1][0]; yn[0][1] = av[1][1];
+            ys[1][0] = av[0][0]; ys[1][1] = av[0][1];
+        } else {
+            const int dn0 = jr0 < L.ny - 1 ? rl : -(L.ny - 1) * rl;
+            ld2(q0 + (hn[0] ? dn0 : 0), yn[0][0], yn[0][1]);
+            ys[1][0] = ys[0][0]; ys[1][1] = ys[0][1];   // row 1 does not exist
+        }
+#pragma unroll
+        for (int rr = 0; rr < RB_ROWS; ++rr)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                // nbsum_pair_vals of the cell of pair pc + c: its pair value
+                // and its other x neighbour (inside the two pairs but for
+                // the outer ones)
+                const bool even = h[rr] == 0;
+                const T part = av[rr][c];
+                const T oxv = even ? (c == 0 ? xo[rr] : av[rr][0]) : (c == 0 ? av[rr][1] : xo[rr]);
+                const bool pw = even ? (c == 0 ? hw : true) : true;
+                const bool pe = even ? true : (c == 0 ? true : he);
+                const T vw = even ? oxv : part, ve = even ? part : oxv;
+                const T xw = pw ? vw : T(0), xe = pe ? ve : T(0);
+                const T xs = hs[rr] ? ys[rr][c] : T(0), xn = hn[rr] ? yn[rr][c] : T(0);
+                const T xb = hb ? zb[rr][c] : T(0), xt = ht ? zt[rr][c] : T(0);
+                const T sx = (xw + xe) * ihx2;
+                const T syy = (xs + xn) * ihy2;
+                const T szz = (xb + xt) * ihz2;
+                sn[rr][c] = sx + syy + szz;
+            }
+    }
+    double dsum = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < RB_ROWS; ++rr) {
+        if (!ok[rr]) continue;
+        const int o = pos[rr];
+        const long go = (long)color * L.half + o;
+        T out[2], res[2];
+        double z[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int di = diag_count(2 * (pc + c) + h[rr], L.nx, L.bt[0], L.bt[1]) * 16 + cyz[rr];
+            if (RES == 2) {
+                T dg;
+                if (di == DIAG_INTERIOR) dg = dg_int;
+                else dg = T(L.diag[di]);
+                res[c] = bo[rr][c] - (dg * ov[rr][c] - sn[rr][c]);
+                continue;
+            }
+            T id;
+            if (di == DIAG_INTERIOR) id = id_int;
+            else id = T(L.idiag[di]);
+            if (ZI) {
+                out[c] = (bo[rr][c] + T(0)) * id;
+            } else {
+                const T xn = (bo[rr][c] + sn[rr][c]) * id;
+                out[c] = xn;
+                if (RES == 1) {
+                    T dg;
+                    if (di == DIAG_INTERIOR) dg = dg_int;
+                    else dg = T(L.diag[di]);
+                    res[c] = bo[rr][c] - (dg * xn - sn[rr][c]);
+                }
+                z[c] = double(xn);
+            }
+        }
+        if (RES == 2) {
+            st2(r + go, res[0], res[1]);
+            continue;
+        }
+        if (ZI) st2(oth + o, T(0), T(0));
+        st2(own + o, out[0], out[1]);
+        if (RES == 1) st2(r + go, res[0], res[1]);
+        if (DOT) {
+            if (z64) {
+                double r0, r1;
+                ld2(r64 + go, r0, r1);
+                st2(z64 + go, z[0], z[1]);
+                dsum += r0 * z[0];
+                dsum += r1 * z[1];
+            } else {
+                dsum += double(bo[rr][0]) * z[0];
+                dsum += double(bo[rr][1]) * z[1];
+            }
+        }
+    }
+    if (DOT) {
+        __shared__ double red[RB];
+        const int nt = kb.dx * kb.dy, t = kb.ty * kb.dx + kb.tx;
+        red[t] = dsum;
+        __syncthreads();
+        for (int st = nt / 2; st > 0; st >>= 1) {
+            if (t < st) red[t] += red[t + st];
+            __syncthreads();
+        }
+        if (t == 0) partial[pofs + ((long)kb.bz * kb.gy + kb.by) * kb.gx + kb.bx] = red[0];
+    }
+}
+
+template <typename T, typename TB, bool DOT = false, int RES = 0, bool ZI = false, bool X2 = false>
 __device__ __forceinline__ void rb_half_body(const Blk &kb, const LevelDev &L, T *__restrict__ x, const TB *__restrict__ b,
                                                 int color, const double *__restrict__ stop,
                                                 double *__restrict__ partial, long pofs, T *__restrict__ r,
                                                 const double *__restrict__ r64,
                                                 double *__restrict__ z64) {
     if (*stop != 0.0) return;
+    if constexpr (X2) {
+        static_assert(std::is_same<T, TB>::value, "the two-pair sweep reads b in the level's type");
+        rb_half2_body<T, DOT, RES, ZI>(kb, L, x, b, color, partial, pofs, r, r64, z64);
+        return;
+    }
     const T id_int = T(L.idiag[DIAG_INTERIOR]), dg_int = T(L.diag[DIAG_INTERIOR]);
     const int ip = kb.bx * kb.dx + kb.tx;
     const int k = kb.bz;
@@ -365,18 +559,31 @@ __device__ __forceinline__ void rb_half_body(const Blk &kb, const LevelDev &L, T
         if (t == 0) partial[pofs + ((long)kb.bz * kb.gy + kb.by) * kb.gx + kb.bx] = red[0];
     }
 }
-template <typename T, typename TB, bool DOT = false, int RES = 0, bool ZI = false>
+template <typename T, typename TB, bool DOT = false, int RES = 0, bool ZI = false, bool X2 = false>
 __global__ __launch_bounds__(RB) void k_rb_half(LevelDev L, T *__restrict__ x, const TB *__restrict__ b,
                                                 int color, const double *__restrict__ stop,
                                                 double *__restrict__ partial, long pofs, T *__restrict__ r,
                                                 const double *__restrict__ r64 = nullptr,
                                                 double *__restrict__ z64 = nullptr) {
-    rb_half_body<T, TB, DOT, RES, ZI>(hw_blk(), L, x, b, color, stop, partial, pofs, r, r64, z64);
+    rb_half_body<T, TB, DOT, RES, ZI, X2>(hw_blk(), L, x, b, color, stop, partial, pofs, r, r64, z64);
 }
 
 dim3 half_grid(int nx, int ny, int nz) {
     const LaunchShape rb = shape_for(div_up(nx, 2), ny, nz);
     return dim3(rb.grid.x, div_up(ny, RB_ROWS * (int)rb.block.y), rb.grid.z);
+}
+
+// The half sweep's launch on a level: two-pair threads (rb_half2_body) where
+// the level is split and its rows hold an even number of pairs, else one
+// pair per thread.
+struct SweepShape {
+    dim3 grid, block;
+    bool x2;
+};
+SweepShape sweep_shape(const LevelDev &d) {
+    const bool x2 = d.split && (d.nx >> 1) % 2 == 0;
+    const LaunchShape sh = shape_for(x2 ? d.nx / 4 : div_up(d.nx, 2), d.ny, d.nz);
+    return {dim3(sh.grid.x, div_up(d.ny, RB_ROWS * (int)sh.block.y), sh.grid.z), sh.block, x2};
 }
 
 // cell index of a virtual coordinate one step beyond the level: a wall side
@@ -1216,6 +1423,19 @@ LevelDev make_level(int nx, int ny, int nz, const int bt[6], const double h[3]) 
     return L;
 }
 
+// one half sweep of a level, two-pair or one-pair threads as sweep_shape says
+template <typename T, bool DOT, int RES, bool ZI>
+void launch_half(const SweepShape &sw, hipStream_t s, const LevelDev &d, T *x, const T *b, int color,
+                 const double *stop, double *partial, long pofs, T *r, const double *r64 = nullptr,
+                 double *z64 = nullptr) {
+    if (sw.x2)
+        hipLaunchKernelGGL((k_rb_half<T, T, DOT, RES, ZI, true>), sw.grid, sw.block, 0, s, d, x, b, color, stop,
+                           partial, pofs, r, r64, z64);
+    else
+        hipLaunchKernelGGL((k_rb_half<T, T, DOT, RES, ZI>), sw.grid, sw.block, 0, s, d, x, b, color, stop, partial,
+                           pofs, r, r64, z64);
+}
+
 template <typename T>
 int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     MGLevel<T> &L = lev[l];
@@ -1245,8 +1465,8 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     const double hb = l == 0 ? 1.5 * sizeof(T) * (double)L.n : 0.0;
     const double hb_res = hb + (l == 0 ? 0.5 * sizeof(T) * (double)L.n : 0.0);
     const double hb_dot = hb + (l == 0 && !std::is_same<T, double>::value ? 8.0 * (double)L.n : 0.0);
-    const dim3 rbh = half_grid(d.nx, d.ny, d.nz);
-    const long nbh = (long)rbh.x * rbh.y * rbh.z;
+    const SweepShape sw = sweep_shape(d);
+    const long nbh = (long)sw.grid.x * sw.grid.y * sw.grid.z;
     // The pre- and the post-smoothing sweeps at level 0 are each timed as one
     // group of back-to-back launches (one event pair per group).
     double grp_bytes = 0.0;
@@ -1265,21 +1485,18 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
         if (dot_slot >= 0) {
             // the mixed form reads the FP64 r and writes the FP64 z as well
             const bool mixed = !std::is_same<T, double>::value;
-            hipLaunchKernelGGL((k_rb_half<T, T, true>), rbh, rb.block, 0, s, d, L.x, L.b, color,
-                               c->scal + SC_CONV, c->partial, dot_slot * nbh, (T *)nullptr,
-                               mixed ? c->cg_r : nullptr, mixed ? c->cg_z : nullptr);
+            launch_half<T, true, 0, false>(sw, s, d, L.x, L.b, color, c->scal + SC_CONV, c->partial,
+                                           dot_slot * nbh, (T *)nullptr, mixed ? c->cg_r : nullptr,
+                                           mixed ? c->cg_z : nullptr);
             grp_bytes += hb_dot;
             return;
         }
         if (res)
-            hipLaunchKernelGGL((k_rb_half<T, T, false, 1>), rbh, rb.block, 0, s, d, L.x, L.b, color,
-                               c->scal + SC_CONV, (double *)nullptr, 0L, L.r);
+            launch_half<T, false, 1, false>(sw, s, d, L.x, L.b, color, c->scal + SC_CONV, nullptr, 0L, L.r);
         else if (zero)
-            hipLaunchKernelGGL((k_rb_half<T, T, false, 0, true>), rbh, rb.block, 0, s, d, L.x, L.b, color,
-                               c->scal + SC_CONV, (double *)nullptr, 0L, (T *)nullptr);
+            launch_half<T, false, 0, true>(sw, s, d, L.x, L.b, color, c->scal + SC_CONV, nullptr, 0L, nullptr);
         else
-            hipLaunchKernelGGL((k_rb_half<T, T>), rbh, rb.block, 0, s, d, L.x, L.b, color, c->scal + SC_CONV,
-                               (double *)nullptr, 0L, (T *)nullptr);
+            launch_half<T, false, 0, false>(sw, s, d, L.x, L.b, color, c->scal + SC_CONV, nullptr, 0L, nullptr);
         grp_bytes += res ? hb_res : hb;
     };
     // pre-smoothing (red, black); the last black sweep also leaves r of the
@@ -1291,8 +1508,7 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
         CHECK_LAUNCH(c);
     }
     group_end();
-    hipLaunchKernelGGL((k_rb_half<T, T, false, 2>), rbh, rb.block, 0, s, d, L.x, L.b, 0, c->scal + SC_CONV,
-                       (double *)nullptr, 0L, L.r);
+    launch_half<T, false, 2, false>(sw, s, d, L.x, L.b, 0, c->scal + SC_CONV, nullptr, 0L, L.r);
     CHECK_LAUNCH(c);
     MGLevel<T> &C = lev[l + 1];
     const dim3 cs(div_up(C.d.nx, RS_CX), div_up(C.d.ny, RS_CY), C.d.nz);
@@ -1360,8 +1576,7 @@ int precondition_dot(ibmdem_ctx *c, int slot, int op) {
         rc = vcycle<double>(c, c->lev64, 0, true);
     }
     if (rc) return rc;
-    const LevelDev &d = c->lev64[0].d;
-    const dim3 g = half_grid(d.nx, d.ny, d.nz);
+    const dim3 g = sweep_shape(c->lev64[0].d).grid;   // the r.z sweeps' blocks
     const int np = 2 * (int)(g.x * g.y * g.z);
     hipLaunchKernelGGL(k_final, dim3(1), dim3(FIN_T), 0, c->stream, c->partial, np, c->scal, slot, op, 0,
                        c->cs.poisson_rtol);
@@ -1630,11 +1845,11 @@ struct RbArg {
     const double *r64;
     double *z64;
 };
-template <typename T, bool DOT, int RES, bool ZI>
+template <typename T, bool DOT, int RES, bool ZI, bool X2>
 __global__ __launch_bounds__(RB) void kb_rb_half(BTable<RbArg<T>> tab, int color) {
     batch_for(tab, [&](const RbArg<T> &a, const Blk &kb) {
-        rb_half_body<T, T, DOT, RES, ZI>(kb, *cst(a.L), glb(a.x), glb(a.b), color, glb(a.stop), glb(a.partial), a.pofs,
-                                         glb(a.r), glb(a.r64), glb(a.z64));
+        rb_half_body<T, T, DOT, RES, ZI, X2>(kb, *cst(a.L), glb(a.x), glb(a.b), color, glb(a.stop), glb(a.partial),
+                                             a.pofs, glb(a.r), glb(a.r64), glb(a.z64));
     });
 }
 
@@ -1754,9 +1969,10 @@ struct BProf {
     std::vector<double> bytes;
     int n = 0;                          // launches recorded
 };
-template <typename T, bool DOT, int RES, bool ZI>
+template <typename T, bool DOT, int RES, bool ZI, bool X2>
 void launch_rb(const BOp &o, hipStream_t s, int) {
-    hipLaunchKernelGGL((kb_rb_half<T, DOT, RES, ZI>), dim3(o.blocks), dim3(RB), 0, s, table<RbArg<T>>(o), o.color);
+    hipLaunchKernelGGL((kb_rb_half<T, DOT, RES, ZI, X2>), dim3(o.blocks), dim3(RB), 0, s, table<RbArg<T>>(o),
+                       o.color);
 }
 template <typename T>
 void launch_restrict(const BOp &o, hipStream_t s, int) {
@@ -1815,49 +2031,55 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
     int depth = 0;
     for (ibmdem_ctx *c : cs) depth = std::max(depth, bottom_level(c));
     int rc;
-    // a half sweep of level l of every case whose V-cycle goes below l
+    // a half sweep of level l of every case whose V-cycle goes below l: one
+    // launch for the cases that sweep it with two-pair threads, one for the
+    // others (sweep_shape)
     auto half = [&](int l, int color, int zero, int dot_slot, int res) {
-        std::vector<RbArg<T>> args;
+        std::vector<RbArg<T>> args[2];
+        BOp o[2];
+        // per cell of level 0: the half's b and the other colour read, the
+        // half's x written (the first sweep reads no x); the residual
+        // variants also read the half's own x or write its r; the mixed
+        // form's r.z variant also reads the FP64 r and writes the FP64 z
+        const double per = dot_slot >= 0 ? 1.5 * sizeof(T) + (mixed ? 8.0 : 0.0)
+                           : res          ? 2.0 * sizeof(T)
+                           : zero         ? 1.0 * sizeof(T)
+                                          : 1.5 * sizeof(T);
         for (ibmdem_ctx *c : cs) {
             if (l >= bottom_level(c)) continue;
             MGLevel<T> &L = levels<T>(c)[l];
-            const LevelDev &d = L.d;
-            const LaunchShape sh = shape_for(div_up(d.nx, 2), d.ny, d.nz);
-            const dim3 hg = half_grid(d.nx, d.ny, d.nz);
+            const SweepShape sw = sweep_shape(L.d);
             RbArg<T> a{};
-            a.g = bgrid(hg, sh.block);
+            a.g = bgrid(sw.grid, sw.block);
             a.L = c->dlev + l;
             a.x = L.x;
             a.b = L.b;
             a.stop = c->scal + SC_CONV;
             a.partial = c->partial;
-            a.pofs = dot_slot >= 0 ? dot_slot * (long)hg.x * hg.y * hg.z : 0;
+            a.pofs = dot_slot >= 0 ? dot_slot * (long)sw.grid.x * sw.grid.y * sw.grid.z : 0;
             a.r = res ? L.r : nullptr;
             a.r64 = dot_slot >= 0 && mixed ? c->cg_r : nullptr;
             a.z64 = dot_slot >= 0 && mixed ? c->cg_z : nullptr;
-            args.push_back(a);
+            args[sw.x2].push_back(a);
+            if (l == 0) o[sw.x2].bytes += per * (double)c->n;
         }
-        BOp o;
-        o.color = color;
-        o.kind = l == 0 ? IBMDEM_BK_SMOOTH0 : IBMDEM_BK_SMOOTH;
-        if (l == 0) {
-            // per cell of the level: the half's b and the other colour read,
-            // the half's x written (the first sweep reads no x); the residual
-            // variants also read the half's own x or write its r; the mixed
-            // form's r.z variant also reads the FP64 r and writes the FP64 z
-            const double per = dot_slot >= 0 ? 1.5 * sizeof(T) + (mixed ? 8.0 : 0.0)
-                               : res          ? 2.0 * sizeof(T)
-                               : zero         ? 1.0 * sizeof(T)
-                                              : 1.5 * sizeof(T);
-            for (ibmdem_ctx *c : cs)
-                if (l < bottom_level(c)) o.bytes += per * (double)c->n;
+        for (int v = 0; v < 2; ++v) {
+            o[v].color = color;
+            o[v].kind = l == 0 ? IBMDEM_BK_SMOOTH0 : IBMDEM_BK_SMOOTH;
         }
-        BLaunch fn = dot_slot >= 0 ? launch_rb<T, true, 0, false>
-                     : res == 2    ? launch_rb<T, false, 2, false>
-                     : res == 1    ? launch_rb<T, false, 1, false>
-                     : zero        ? launch_rb<T, false, 0, true>
-                                   : launch_rb<T, false, 0, false>;
-        return add_op(ar, ops, fn, RB, args, o, err);
+        const BLaunch fn[2] = {
+            dot_slot >= 0 ? launch_rb<T, true, 0, false, false>
+            : res == 2    ? launch_rb<T, false, 2, false, false>
+            : res == 1    ? launch_rb<T, false, 1, false, false>
+            : zero        ? launch_rb<T, false, 0, true, false>
+                          : launch_rb<T, false, 0, false, false>,
+            dot_slot >= 0 ? launch_rb<T, true, 0, false, true>
+            : res == 2    ? launch_rb<T, false, 2, false, true>
+            : res == 1    ? launch_rb<T, false, 1, false, true>
+            : zero        ? launch_rb<T, false, 0, true, true>
+                          : launch_rb<T, false, 0, false, true>};
+        int r2 = add_op(ar, ops, fn[1], RB, args[1], o[1], err);
+        return r2 ? r2 : add_op(ar, ops, fn[0], RB, args[0], o[0], err);
     };
     // the bottoms at level l
     auto bottoms = [&](int l) {
@@ -1955,8 +2177,7 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
     }
     std::vector<CgArg> fin;
     for (ibmdem_ctx *c : cs) {
-        const LevelDev &d = c->lev64[0].d;
-        const dim3 g = half_grid(d.nx, d.ny, d.nz);
+        const dim3 g = sweep_shape(c->lev64[0].d).grid;   // the r.z sweeps' blocks
         CgArg a{};
         a.g = bgrid(dim3(1), dim3(FIN_T));
         a.partial = c->partial;
