@@ -781,6 +781,100 @@ __global__ __launch_bounds__(RB) void k_prolong(LevelDev F, T *__restrict__ xf, 
     prolong_body<T>(hw_blk(), F, xf, Cl, xc, stop);
 }
 
+// The same prolongation with each thread on two adjacent x-pairs (pc, pc +
+// 1) of its row, on split fine levels whose rows hold an even number of
+// pairs: the thread's four fine values are two two-value loads (and
+// stores) of the colour halves. The block's coarse parents (its pairs'
+// coarse columns +-1, rows +-1, the plane's two parent planes) are staged in
+// LDS as prolong_body stages them; each fine value gets the same 8 products
+// in the same order.
+constexpr int PRO2_TILE = 1040;   // 2 (by / 2 + 2) (2 bx + 2) for the shapes of shape_for
+template <typename T>
+__device__ __forceinline__ void prolong2_body(const Blk &kb, const LevelDev &F, T *__restrict__ xf,
+                                              const LevelDev &Cl, const T *__restrict__ xc,
+                                              const double *__restrict__ stop) {
+    if (*stop != 0.0) return;
+    __shared__ T tile[PRO2_TILE];
+    const int bx = kb.dx, by = kb.dy;
+    const int TX = 2 * bx + 2, TY = by / 2 + 2;
+    const int I0 = 2 * kb.bx * bx - 1, J0 = (kb.by * by) / 2 - 1;
+    const int k = kb.bz;
+    int pk[2];
+    parents(k, Cl.nz, Cl.bt[4], Cl.bt[5], pk[0], pk[1]);
+    const int t = kb.ty * bx + kb.tx;
+    const int rl = F.nx >> 1;
+    const int p0 = 2 * (kb.bx * bx + kb.tx), pc = min(p0, rl - 2);
+    const int j = kb.by * by + kb.ty, jc = min(j, F.ny - 1);
+    const int ca = (jc + k) & 1;
+    const long pos = (long)(k * F.ny + jc) * rl + pc;
+    // fine values first (even cells of the two pairs in half ca, odd cells in
+    // half 1 - ca), so their latency overlaps the staging
+    T xo[2][2];
+    ld2(xf + (long)ca * F.half + pos, xo[0][0], xo[0][1]);
+    ld2(xf + (long)(1 - ca) * F.half + pos, xo[1][0], xo[1][1]);
+    const int NT = 2 * TY * TX;
+    for (int q0 = t; q0 < NT; q0 += 4 * bx * by) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * bx * by;
+            if (q < NT) {
+                const int x = q % TX, y = (q / TX) % TY, c = q / (TX * TY);
+                const int I = tile_coord(I0 + x, Cl.nx, Cl.bt[0], Cl.bt[1]);
+                const int J = tile_coord(J0 + y, Cl.ny, Cl.bt[2], Cl.bt[3]);
+                v[u] = xc[lidx(Cl, I, J, pk[c])];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * bx * by;
+            if (q < NT) tile[q] = v[u];
+        }
+    }
+    __syncthreads();
+    if (j >= F.ny || p0 >= rl) return;
+    const T wt[2] = {T(0.75), T(0.25)};
+    const int ty[2] = {j / 2 - J0, j / 2 + ((j & 1) ? 1 : -1) - J0};
+    T out[2][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int i = 2 * (pc + c) + h;
+            const int tx[2] = {i / 2 - I0, i / 2 + ((i & 1) ? 1 : -1) - I0};
+            T s = T(0);
+            for (int c3 = 0; c3 < 2; ++c3)
+                for (int c2 = 0; c2 < 2; ++c2)
+                    for (int c1 = 0; c1 < 2; ++c1) {
+                        T w = wt[c3] * wt[c2] * wt[c1];
+                        s = s + w * tile[(c3 * TY + ty[c2]) * TX + tx[c1]];
+                    }
+            out[h][c] = xo[h][c] + s;
+        }
+    st2(xf + (long)ca * F.half + pos, out[0][0], out[0][1]);
+    st2(xf + (long)(1 - ca) * F.half + pos, out[1][0], out[1][1]);
+}
+template <typename T>
+__global__ __launch_bounds__(RB) void k_prolong2(LevelDev F, T *__restrict__ xf, LevelDev Cl,
+                                                 const T *__restrict__ xc, const double *__restrict__ stop) {
+    prolong2_body<T>(hw_blk(), F, xf, Cl, xc, stop);
+}
+
+// The prolongation's launch onto fine level d (one row per thread): two
+// pairs per thread where sweep_shape takes two, else one; ok = false when
+// the block's coarse tile would not fit the kernel's LDS tile.
+struct ProlongShape {
+    dim3 grid, block;
+    bool x2, ok;
+};
+ProlongShape prolong_shape(const LevelDev &d) {
+    const bool x2 = d.split && (d.nx >> 1) % 2 == 0;
+    const LaunchShape sh = shape_for(x2 ? d.nx / 4 : div_up(d.nx, 2), d.ny, d.nz);
+    const unsigned bx = sh.block.x, by = sh.block.y;
+    const unsigned need = x2 ? (2 * bx + 2) * (by / 2 + 2) * 2 : (bx + 2) * (by / 2 + 2) * 2;
+    return {sh.grid, sh.block, x2, by % 2 == 0 && need <= (unsigned)(x2 ? PRO2_TILE : PRO_TILE)};
+}
+
 // coarsest level, one work-group. Small levels (<= COARSE_LDS cells, every
 // level the grids here coarsen to) live in LDS with a precomputed neighbour
 // table whose missing (wall) neighbours point at a zero slot, so the sweeps
@@ -1216,6 +1310,141 @@ __global__ __launch_bounds__(RB) void k_matvec_dot(LevelDev L, const double *__r
     matvec_dot_body(hw_blk(), L, p, Ap, partial, scal);
 }
 
+// The same product with each thread on two adjacent x-pairs (pc, pc + 1,
+// pc even) of its two rows, on split levels whose rows hold an even number
+// of pairs (sweep_shape's condition and launch shape): both cells of the two
+// pairs come from two-value loads of the two colour halves, and a cell's
+// x neighbours are mostly its pairs' other cells. Each Ap value is
+// matvec_dot_body's (the same diagonal product less nbsum_pair_vals' sum);
+// the partial sums follow this kernel's blocks.
+__device__ __forceinline__ void matvec2_dot_body(const Blk &kb, const LevelDev &L, const double *__restrict__ p,
+                                                 double *__restrict__ Ap, double *__restrict__ partial,
+                                                 const double *__restrict__ scal) {
+    if (scal[SC_CONV] != 0.0) return;
+    __shared__ double red[RB];
+    const double ihx2 = L.ihx2, ihy2 = L.ihy2, ihz2 = L.ihz2;
+    const double dg_int = L.diag[DIAG_INTERIOR];
+    const int rl = L.nx >> 1, pl = rl * L.ny;
+    const int p0 = 2 * (kb.bx * kb.dx + kb.tx), k = kb.bz;
+    const int j0 = (kb.by * kb.dy + kb.ty) * 2;
+    const bool px = L.bt[0] == IBMDEM_BC_PERIODIC, py = L.bt[2] == IBMDEM_BC_PERIODIC,
+               pz = L.bt[4] == IBMDEM_BC_PERIODIC;
+    const int pc = min(p0, rl - 2);
+    const bool hw = pc > 0 || px, he = pc + 2 < rl || px;
+    const int dw = pc > 0 ? -1 : rl - 1, de = pc + 2 < rl ? 2 : 2 - rl;
+    const bool hb = k > 0 || pz, ht = k < L.nz - 1 || pz;
+    const int db = k > 0 ? -pl : (L.nz - 1) * pl, dt = k < L.nz - 1 ? pl : -(L.nz - 1) * pl;
+    const int cz = diag_count(k, L.nz, L.bt[4], L.bt[5]);
+    const bool adj = j0 + 1 < L.ny;
+    // per row: v[r][hh][c] = the value of cell h = hh of pair pc + c (hh 0:
+    // the even cell, in half ca; 1: the odd cell, in half 1 - ca)
+    bool ok[2], hs[2], hn[2];
+    int pos[2], ca[2], cyz[2];
+    double v[2][2][2], zb[2][2][2], zt[2][2][2], xw0[2], xe1[2], ys[2][2][2], yn[2][2][2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int j = min(j0 + r, L.ny - 1);
+        ok[r] = j0 + r < L.ny && p0 < rl;
+        pos[r] = (k * L.ny + j) * rl + pc;
+        ca[r] = (j + k) & 1;
+        hs[r] = j > 0 || py;
+        hn[r] = j < L.ny - 1 || py;
+        cyz[r] = diag_count(j, L.ny, L.bt[2], L.bt[3]) * 4 + cz;
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            const double *q = p + (long)(hh ? 1 - ca[r] : ca[r]) * L.half + pos[r];
+            ld2(q, v[r][hh][0], v[r][hh][1]);
+            ld2(q + (hb ? db : 0), zb[r][hh][0], zb[r][hh][1]);
+            ld2(q + (ht ? dt : 0), zt[r][hh][0], zt[r][hh][1]);
+        }
+        // the west neighbour of pair pc's even cell (the odd cell of pair
+        // pc - 1) and the east neighbour of pair pc + 1's odd cell (the even
+        // cell of pair pc + 2)
+        xw0[r] = p[(long)(1 - ca[r]) * L.half + pos[r] + (hw ? dw : 0)];
+        xe1[r] = p[(long)ca[r] * L.half + pos[r] + (he ? de : 0)];
+    }
+    // y neighbours: cell (i, j +- 1) is in the other half of its row at the
+    // same pair position; row 0's north and row 1's south are the other
+    // row's values when both rows exist
+    {
+        const int jr0 = min(j0, L.ny - 1), jr1 = min(j0 + 1, L.ny - 1);
+        const int ds0 = jr0 > 0 ? -rl : (L.ny - 1) * rl, dn1 = jr1 < L.ny - 1 ? rl : -(L.ny - 1) * rl;
+        const int dn0 = jr0 < L.ny - 1 ? rl : -(L.ny - 1) * rl;
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            // cell h of row 0 has colour ca[0] ^ h; its south neighbour lies in
+            // the half of that colour's opposite, i.e. half (ca[0] ^ h) ^ 1 of row jr0 - 1
+            const double *q0 = p + (long)((ca[0] ^ hh) ^ 1) * L.half + pos[0];
+            const double *q1 = p + (long)((ca[1] ^ hh) ^ 1) * L.half + pos[1];
+            ld2(q0 + (hs[0] ? ds0 : 0), ys[0][hh][0], ys[0][hh][1]);
+            ld2(q1 + (hn[1] ? dn1 : 0), yn[1][hh][0], yn[1][hh][1]);
+            if (adj) {
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    yn[0][hh][c] = v[1][hh][c];
+                    ys[1][hh][c] = v[0][hh][c];
+                }
+            } else {
+                ld2(q0 + (hn[0] ? dn0 : 0), yn[0][hh][0], yn[0][hh][1]);
+                ys[1][hh][0] = ys[0][hh][0];
+                ys[1][hh][1] = ys[0][hh][1];
+            }
+        }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        double a[2][2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                // nbsum_pair_vals of cell hh of pair pc + c: its pair's other
+                // cell and its other x neighbour (the adjacent pair's cell)
+                const double part = v[r][1 - hh][c];
+                double xo;
+                bool pw = true, pe = true;
+                if (hh == 0) {
+                    xo = c == 0 ? xw0[r] : v[r][1][0];
+                    pw = c == 0 ? hw : true;
+                } else {
+                    xo = c == 0 ? v[r][0][1] : xe1[r];
+                    pe = c == 0 ? true : he;
+                }
+                const double vw = hh == 0 ? xo : part, ve = hh == 0 ? part : xo;
+                const double xw = pw ? vw : 0.0, xe = pe ? ve : 0.0;
+                const double xs = hs[r] ? ys[r][hh][c] : 0.0, xn = hn[r] ? yn[r][hh][c] : 0.0;
+                const double xb = hb ? zb[r][1 - hh][c] : 0.0, xt = ht ? zt[r][1 - hh][c] : 0.0;
+                const double sx = (xw + xe) * ihx2;
+                const double syy = (xs + xn) * ihy2;
+                const double szz = (xb + xt) * ihz2;
+                const int di = diag_count(2 * (pc + c) + hh, L.nx, L.bt[0], L.bt[1]) * 16 + cyz[r];
+                const double dg = di == DIAG_INTERIOR ? dg_int : L.diag[di];
+                a[hh][c] = dg * v[r][hh][c] - (sx + syy + szz);
+            }
+        if (!ok[r]) continue;
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            st2(Ap + (long)(hh ? 1 - ca[r] : ca[r]) * L.half + pos[r], a[hh][0], a[hh][1]);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) s += v[r][hh][c] * a[hh][c];
+        }
+    }
+    const int nt = kb.dx * kb.dy, t = kb.ty * kb.dx + kb.tx;
+    red[t] = s;
+    __syncthreads();
+    for (int st = nt / 2; st > 0; st >>= 1) {
+        if (t < st) red[t] += red[t + st];
+        __syncthreads();
+    }
+    if (t == 0) partial[((long)kb.bz * kb.gy + kb.by) * kb.gx + kb.bx] = red[0];
+}
+__global__ __launch_bounds__(RB) void k_matvec2_dot(LevelDev L, const double *__restrict__ p,
+                                                    double *__restrict__ Ap, double *__restrict__ partial,
+                                                    const double *__restrict__ scal) {
+    matvec2_dot_body(hw_blk(), L, p, Ap, partial, scal);
+}
+
 // CG iteration k >= 2, first kernel: p_k = z_k + beta_{k-1} p_{k-1} in place,
 // and the deferred x_k = x_{k-1} + alpha_{k-1} p_{k-1}
 __device__ __forceinline__ void cg_px_body(const Blk &kb, long n, double *__restrict__ p, const double *__restrict__ z,
@@ -1516,11 +1745,15 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     CHECK_LAUNCH(c);
     int rc = vcycle<T>(c, lev, l + 1);
     if (rc) return rc;
-    if (rb.block.y % 2 || (rb.block.x + 2) * (rb.block.y / 2 + 2) * 2 > (unsigned)PRO_TILE) {
-        return set_error(c, IBMDEM_ERR_ARG, "prolongation tile of " + std::to_string(rb.block.x) + "x" +
-                                                std::to_string(rb.block.y) + " x-pairs does not fit");
+    const ProlongShape ps = prolong_shape(d);
+    if (!ps.ok) {
+        return set_error(c, IBMDEM_ERR_ARG, "prolongation tile of " + std::to_string(ps.block.x) + "x" +
+                                                std::to_string(ps.block.y) + " threads does not fit");
     }
-    hipLaunchKernelGGL((k_prolong<T>), rb.grid, rb.block, 0, s, d, L.x, C.d, C.x, c->scal + SC_CONV);
+    if (ps.x2)
+        hipLaunchKernelGGL((k_prolong2<T>), ps.grid, ps.block, 0, s, d, L.x, C.d, C.x, c->scal + SC_CONV);
+    else
+        hipLaunchKernelGGL((k_prolong<T>), ps.grid, ps.block, 0, s, d, L.x, C.d, C.x, c->scal + SC_CONV);
     CHECK_LAUNCH(c);
     group_begin();
     for (int q = 0; q < c->cs.mg_post; ++q) {
@@ -1676,16 +1909,18 @@ int cg_iteration(ibmdem_ctx *c, int it) {
     hipStream_t s = c->stream;
     int rc;
     const LevelDev &L0 = c->lev64[0].d;
-    const LaunchShape mv = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);   // block shape
-    const dim3 mvg = half_grid(L0.nx, L0.ny, L0.nz);                     // two rows per thread
-    const int mv_blocks = (int)(mvg.x * mvg.y * mvg.z);
+    const SweepShape mv = sweep_shape(L0);   // two rows per thread, two pairs where the level allows
+    const int mv_blocks = (int)(mv.grid.x * mv.grid.y * mv.grid.z);
     if (it > 1) {
         hipLaunchKernelGGL(k_cg_px, dim3(stream_blocks(n)), dim3(RED_T), 0, s, n, c->cg_p, c->cg_z, c->cg_x,
                            c->scal);
         CHECK_LAUNCH(c);
     }
     prof_begin(c, IBMDEM_K_MATVEC);
-    hipLaunchKernelGGL(k_matvec_dot, mvg, mv.block, 0, s, L0, c->cg_p, c->cg_Ap, c->partial, c->scal);
+    if (mv.x2)
+        hipLaunchKernelGGL(k_matvec2_dot, mv.grid, mv.block, 0, s, L0, c->cg_p, c->cg_Ap, c->partial, c->scal);
+    else
+        hipLaunchKernelGGL(k_matvec_dot, mv.grid, mv.block, 0, s, L0, c->cg_p, c->cg_Ap, c->partial, c->scal);
     prof_end(c, IBMDEM_K_MATVEC, 2.0 * 8.0 * (double)n);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_final, dim3(1), dim3(FIN_T), 0, s, c->partial, mv_blocks, c->scal, SC_PAP, 1, it,
@@ -1882,6 +2117,12 @@ __global__ __launch_bounds__(RB) void kb_prolong(BTable<PrArg<T>> tab) {
         prolong_body<T>(kb, *cst(a.F), glb(a.xf), *cst(a.C), glb(a.xc), glb(a.stop));
     });
 }
+template <typename T>
+__global__ __launch_bounds__(RB) void kb_prolong2(BTable<PrArg<T>> tab) {
+    batch_for(tab, [&](const PrArg<T> &a, const Blk &kb) {
+        prolong2_body<T>(kb, *cst(a.F), glb(a.xf), *cst(a.C), glb(a.xc), glb(a.stop));
+    });
+}
 
 // the bottom of a case's V-cycle: the one-work-group tail (first < last),
 // or the coarsest level alone from LDS tables or in global memory
@@ -1931,6 +2172,11 @@ struct CgArg {
 __global__ __launch_bounds__(RB) void kb_matvec_dot(BTable<CgArg> tab) {
     batch_for(tab, [&](const CgArg &a, const Blk &kb) {
         matvec_dot_body(kb, *cst(a.L), glb(a.p), glb(a.Ap), glb(a.partial), glb(a.scal));
+    });
+}
+__global__ __launch_bounds__(RB) void kb_matvec2_dot(BTable<CgArg> tab) {
+    batch_for(tab, [&](const CgArg &a, const Blk &kb) {
+        matvec2_dot_body(kb, *cst(a.L), glb(a.p), glb(a.Ap), glb(a.partial), glb(a.scal));
     });
 }
 __global__ __launch_bounds__(RED_T) void kb_cg_px(BTable<CgArg> tab) {
@@ -1983,6 +2229,10 @@ void launch_prolong(const BOp &o, hipStream_t s, int) {
     hipLaunchKernelGGL(kb_prolong<T>, dim3(o.blocks), dim3(RB), 0, s, table<PrArg<T>>(o));
 }
 template <typename T>
+void launch_prolong2(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_prolong2<T>, dim3(o.blocks), dim3(RB), 0, s, table<PrArg<T>>(o));
+}
+template <typename T>
 void launch_tail(const BOp &o, hipStream_t s, int) {
     hipLaunchKernelGGL(kb_tail<T>, dim3(o.blocks), dim3(TAIL_T), 0, s, table<BtArg<T>>(o));
 }
@@ -1996,6 +2246,9 @@ void launch_coarse(const BOp &o, hipStream_t s, int) {
 }
 void launch_matvec(const BOp &o, hipStream_t s, int) {
     hipLaunchKernelGGL(kb_matvec_dot, dim3(o.blocks), dim3(RB), 0, s, table<CgArg>(o));
+}
+void launch_matvec2(const BOp &o, hipStream_t s, int) {
+    hipLaunchKernelGGL(kb_matvec2_dot, dim3(o.blocks), dim3(RB), 0, s, table<CgArg>(o));
 }
 void launch_px(const BOp &o, hipStream_t s, int) {
     hipLaunchKernelGGL(kb_cg_px, dim3(o.blocks), dim3(RED_T), 0, s, table<CgArg>(o));
@@ -2144,12 +2397,13 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
         if ((rc = bottoms(l + 1))) return rc;
     }
     for (int l = depth - 1; l >= 0; --l) {
-        std::vector<PrArg<T>> pr;
+        std::vector<PrArg<T>> pr[2];
+        BOp opr[2];
         for (ibmdem_ctx *c : cs) {
             if (l >= bottom_level(c)) continue;
             MGLevel<T> &F = levels<T>(c)[l], &C = levels<T>(c)[l + 1];
-            const LaunchShape sh = shape_for(div_up(F.d.nx, 2), F.d.ny, F.d.nz);
-            if (sh.block.y % 2 || (sh.block.x + 2) * (sh.block.y / 2 + 2) * 2 > (unsigned)PRO_TILE) {
+            const ProlongShape sh = prolong_shape(F.d);
+            if (!sh.ok) {
                 err = "prolongation tile does not fit";
                 return IBMDEM_ERR_ARG;
             }
@@ -2160,15 +2414,13 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
             a.xf = F.x;
             a.xc = C.x;
             a.stop = c->scal + SC_CONV;
-            pr.push_back(a);
+            pr[sh.x2].push_back(a);
+            // the coarse solution read, the fine one read and written
+            opr[sh.x2].bytes += sizeof(T) * (double)(2 * F.n + C.n);
         }
-        BOp opr;
-        opr.kind = IBMDEM_BK_PROLONG;
-        // the coarse solution read, the fine one read and written
-        for (ibmdem_ctx *c : cs)
-            if (l < bottom_level(c))
-                opr.bytes += sizeof(T) * (double)(2 * levels<T>(c)[l].n + levels<T>(c)[l + 1].n);
-        if ((rc = add_op(ar, ops, launch_prolong<T>, RB, pr, opr, err))) return rc;
+        opr[0].kind = opr[1].kind = IBMDEM_BK_PROLONG;
+        if ((rc = add_op(ar, ops, launch_prolong2<T>, RB, pr[1], opr[1], err))) return rc;
+        if ((rc = add_op(ar, ops, launch_prolong<T>, RB, pr[0], opr[0], err))) return rc;
         for (int q = 0; q < post; ++q) {
             const bool last = l == 0 && q == post - 1;
             if ((rc = half(l, 1, 0, last ? 0 : -1, 0))) return rc;
@@ -2214,15 +2466,15 @@ int plan_all(Arena &ar, std::vector<BOp> &begin, std::vector<BOp> &iter, const s
     int rc;
     // poisson_begin_precondition: z = M^-1 r with r.z, p = z
     if ((rc = plan_precondition_dot<T>(ar, begin, cs, SC_RZ, 0, err))) return rc;
-    std::vector<CgArg> cp, px, mv, fpap, up, frr;
+    std::vector<CgArg> cp, px, mv[2], fpap, up, frr;
+    double mv_cells[2] = {0.0, 0.0};
     for (ibmdem_ctx *c : cs) {
-        const LevelDev &L0 = c->lev64[0].d;
-        const LaunchShape sh = shape_for(div_up(L0.nx, 2), L0.ny, L0.nz);
-        const dim3 mvg = half_grid(L0.nx, L0.ny, L0.nz);
-        const int mvb = (int)(mvg.x * mvg.y * mvg.z);
+        const SweepShape sh = sweep_shape(c->lev64[0].d);
+        const int mvb = (int)(sh.grid.x * sh.grid.y * sh.grid.z);
         cp.push_back(cg_arg(c, dim3(stream_blocks(c->n)), dim3(RED_T), 0));
         px.push_back(cg_arg(c, dim3(stream_blocks(c->n)), dim3(RED_T), 0));
-        mv.push_back(cg_arg(c, mvg, sh.block, 0));
+        mv[sh.x2].push_back(cg_arg(c, sh.grid, sh.block, 0));
+        mv_cells[sh.x2] += (double)c->n;
         fpap.push_back(cg_arg(c, dim3(1), dim3(FIN_T), mvb));
         up.push_back(cg_arg(c, dim3(RED_BLOCKS), dim3(RED_T), 0));
         frr.push_back(cg_arg(c, dim3(1), dim3(FIN_T), RED_BLOCKS));
@@ -2240,8 +2492,10 @@ int plan_all(Arena &ar, std::vector<BOp> &begin, std::vector<BOp> &iter, const s
     if ((rc = add_op(ar, iter, launch_px, RED_T, px, o, err))) return rc;
     o = BOp();
     o.kind = IBMDEM_BK_MATVEC;
-    o.bytes = 16.0 * cells;   // p read, Ap written
-    if ((rc = add_op(ar, iter, launch_matvec, RB, mv, o, err))) return rc;
+    o.bytes = 16.0 * mv_cells[1];   // p read, Ap written
+    if ((rc = add_op(ar, iter, launch_matvec2, RB, mv[1], o, err))) return rc;
+    o.bytes = 16.0 * mv_cells[0];
+    if ((rc = add_op(ar, iter, launch_matvec, RB, mv[0], o, err))) return rc;
     o = BOp();
     o.kind = IBMDEM_BK_SCALAR;
     o.slot = SC_PAP; o.op = 1; o.with_it = true;
