@@ -354,7 +354,37 @@ struct LevelDev {
     int dirichlet;               // an outflow side: the operator is non-singular
     int split;                   // colour-split storage (nx even), see lidx
     int half;                    // nx*ny*nz / 2 when split
+    // udiv_small multipliers for nx / 2 (split) or nx, ny, nx*ny
+    unsigned m_row, m_ny, m_nxy;
 };
+
+// ceil(2^32 / d) (0 for d = 1): with it, udiv_small(p, d, m) = p / d exactly
+// for p * d < 2^32 (the error term p * (m d - 2^32) / 2^32 stays below p d /
+// 2^32 < 1 of the fractional part's room), in one multiply-high instead of
+// the integer division sequence
+inline unsigned udiv_magic(unsigned d) {
+    return d <= 1 ? 0u : (unsigned)(((1ull << 32) + d - 1) / d);
+}
+__device__ __forceinline__ int udiv_small(int p, int d, unsigned m) {
+    return d == 1 ? p : (int)__umulhi((unsigned)p, m);
+}
+// lcell for levels of at most 2^16 cells (and dimensions): the divisions
+// through udiv_small
+__device__ __forceinline__ void lcell_small(const LevelDev &L, int q, int &i, int &j, int &k) {
+    if (!L.split) {
+        const int r = udiv_small(q, L.nx, L.m_row);
+        i = q - r * L.nx;
+        k = udiv_small(q, L.nx * L.ny, L.m_nxy);
+        j = r - k * L.ny;
+        return;
+    }
+    const int c = q >= L.half ? 1 : 0;
+    const int r = q - c * L.half, h = L.nx >> 1;
+    const int row = udiv_small(r, h, L.m_row);
+    k = udiv_small(row, L.ny, L.m_ny);
+    j = row - k * L.ny;
+    i = 2 * (r - row * h) + ((j + k + c) & 1);
+}
 
 // Storage of a level's cell (i,j,k), 0-based. With nx even the level is
 // stored colour-split: the (i+j+k)-even cells first, then the odd ones, each

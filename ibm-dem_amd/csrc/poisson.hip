@@ -994,6 +994,9 @@ __global__ __launch_bounds__(COARSE_T) void k_coarse(LevelDev L, T *__restrict__
 // ---------------------------------------------------------------------------
 constexpr int TAIL_T = 1024;
 constexpr int TAIL_BYTES = 112 * 1024;   // x (one extra zero slot on the coarsest level), b, r of the levels
+// (the tail's levels hold at most TAIL_BYTES / 12 cells, so its index
+// divisions go through udiv_small)
+static_assert(TAIL_BYTES / 12 < (1 << 14), "tail levels small enough for udiv_small");
 
 template <typename T>
 __device__ __forceinline__ void tail_half(const LevelDev &L, T *x, const T *b, int color, bool zero) {
@@ -1002,8 +1005,8 @@ __device__ __forceinline__ void tail_half(const LevelDev &L, T *x, const T *b, i
         const int rl = L.nx >> 1;
         T *own = x + color * L.half, *oth = x + (1 - color) * L.half;
         for (int p = threadIdx.x; p < L.half; p += TAIL_T) {
-            const int row = p / rl;
-            const PairGeo g = pair_geo(L, p - row * rl, row % L.ny, row / L.ny);
+            const int row = udiv_small(p, rl, L.m_row), kr = udiv_small(row, L.ny, L.m_ny);
+            const PairGeo g = pair_geo(L, p - row * rl, row - kr * L.ny, kr);
             const int h = g.ca == color ? 0 : 1;
             const T id = T(L.idiag[(h ? g.cx1 : g.cx0) * 16 + g.cyz]);
             const T bo = T(b[color * L.half + p]);
@@ -1019,7 +1022,7 @@ __device__ __forceinline__ void tail_half(const LevelDev &L, T *x, const T *b, i
     }
     for (int q = threadIdx.x; q < n; q += TAIL_T) {
         int i, j, k;
-        lcell(L, q, i, j, k);
+        lcell_small(L, q, i, j, k);
         if (((i + j + k) & 1) != color) {
             if (zero) x[q] = T(0);
             continue;
@@ -1036,8 +1039,8 @@ __device__ __forceinline__ void tail_residual(const LevelDev &L, const T *x, con
     if (L.split) {   // by x-pairs, as the residual of k_rb_half
         const int rl = L.nx >> 1;
         for (int p = threadIdx.x; p < L.half; p += TAIL_T) {
-            const int row = p / rl;
-            const PairGeo g = pair_geo(L, p - row * rl, row % L.ny, row / L.ny);
+            const int row = udiv_small(p, rl, L.m_row), kr = udiv_small(row, L.ny, L.m_ny);
+            const PairGeo g = pair_geo(L, p - row * rl, row - kr * L.ny, kr);
             for (int h = 0; h < 2; ++h) {
                 const int own = (h ? 1 - g.ca : g.ca) * L.half, oth = (h ? g.ca : 1 - g.ca) * L.half;
                 const T dg = T(L.diag[(h ? g.cx1 : g.cx0) * 16 + g.cyz]);
@@ -1049,7 +1052,7 @@ __device__ __forceinline__ void tail_residual(const LevelDev &L, const T *x, con
     }
     for (int q = threadIdx.x; q < n; q += TAIL_T) {
         int i, j, k;
-        lcell(L, q, i, j, k);
+        lcell_small(L, q, i, j, k);
         const T dg = T(L.diag[diag_index(L, i, j, k)]);
         r[q] = T(b[q]) - (dg * x[q] - nbsum(x, L, i, j, k));
     }
@@ -1062,7 +1065,7 @@ __device__ __forceinline__ void tail_restrict(const LevelDev &F, const T *rf, co
     const int n = Cl.nx * Cl.ny * Cl.nz;
     for (int q = threadIdx.x; q < n; q += TAIL_T) {
         int I, J, K;
-        lcell(Cl, q, I, J, K);
+        lcell_small(Cl, q, I, J, K);
         int fi[4], fj[4], fk[4];
         for (int a = 0; a < 4; ++a) {
             fi[a] = tile_coord(2 * I - 1 + a, F.nx, F.bt[0], F.bt[1]);
@@ -1087,7 +1090,7 @@ __device__ __forceinline__ void tail_prolong(const LevelDev &F, T *xf, const Lev
     const int n = F.nx * F.ny * F.nz;
     for (int q = threadIdx.x; q < n; q += TAIL_T) {
         int i, j, k;
-        lcell(F, q, i, j, k);
+        lcell_small(F, q, i, j, k);
         int pi[2], pj[2], pk[2];
         parents(i, Cl.nx, Cl.bt[0], Cl.bt[1], pi[0], pi[1]);
         parents(j, Cl.ny, Cl.bt[2], Cl.bt[3], pj[0], pj[1]);
@@ -1114,7 +1117,7 @@ __device__ __forceinline__ void tail_coarse(const LevelDev &L, T *x, const T *b,
     const int m[3] = {L.nx, L.ny, L.nz};
     for (int q = threadIdx.x; q < n; q += 64) {
         int a[3];
-        lcell(L, q, a[0], a[1], a[2]);
+        lcell_small(L, q, a[0], a[1], a[2]);
         for (int d = 0; d < 3; ++d)
             for (int dir = 0; dir < 2; ++dir) {
                 int p[3] = {a[0], a[1], a[2]};
@@ -1649,6 +1652,9 @@ LevelDev make_level(int nx, int ny, int nz, const int bt[6], const double h[3]) 
         if (bt[2 * d] == IBMDEM_BC_PERIODIC && (n[d] % 2)) L.rb_valid = 0;
     L.split = (nx % 2) == 0 ? 1 : 0;
     L.half = L.split ? nx * ny * nz / 2 : 0;
+    L.m_row = udiv_magic(L.split ? nx / 2 : nx);
+    L.m_ny = udiv_magic(ny);
+    L.m_nxy = udiv_magic(nx * ny);
     return L;
 }
 
