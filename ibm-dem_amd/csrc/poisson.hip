@@ -812,23 +812,28 @@ __device__ __forceinline__ void prolong2_body(const Blk &kb, const LevelDev &F, 
     T xo[2][2];
     ld2(xf + (long)ca * F.half + pos, xo[0][0], xo[0][1]);
     ld2(xf + (long)(1 - ca) * F.half + pos, xo[1][0], xo[1][1]);
-    const int NT = 2 * TY * TX;
-    for (int q0 = t; q0 < NT; q0 += 4 * bx * by) {
-        T v[4];
+    // The tile's rows r = c TY + y (coarse row J0 + y of parent plane c):
+    // wavefront w stages rows w, w + NW, ...; a row's (J, K) are the same on
+    // every lane (scalars), and lane l takes columns l, l + 64, l + 128 (<
+    // TX), whose coarse indices it maps once. Column x is stored at
+    // (x & 1) * TXH + x / 2 of its row: the summing loop's lanes read
+    // columns two apart, which land in consecutive words.
+    const int TXH = bx + 1, NROW = 2 * TY, NW = bx * by / 64;
+    const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
+    int Ic[3];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int q = q0 + u * bx * by;
-            if (q < NT) {
-                const int x = q % TX, y = (q / TX) % TY, c = q / (TX * TY);
-                const int I = tile_coord(I0 + x, Cl.nx, Cl.bt[0], Cl.bt[1]);
-                const int J = tile_coord(J0 + y, Cl.ny, Cl.bt[2], Cl.bt[3]);
-                v[u] = xc[lidx(Cl, I, J, pk[c])];
-            }
-        }
+    for (int u = 0; u < 3; ++u) Ic[u] = tile_coord(I0 + lane + 64 * u, Cl.nx, Cl.bt[0], Cl.bt[1]);
+    for (int r = w; r < NROW; r += NW) {
+        const int y = r % TY, c = r / TY;
+        const int J = tile_coord(J0 + y, Cl.ny, Cl.bt[2], Cl.bt[3]), K = pk[c];
+        T v[3];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int q = q0 + u * bx * by;
-            if (q < NT) tile[q] = v[u];
+        for (int u = 0; u < 3; ++u)
+            if (lane + 64 * u < TX) v[u] = xc[lidx(Cl, Ic[u], J, K)];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int x = lane + 64 * u;
+            if (x < TX) tile[r * TX + (x & 1) * TXH + (x >> 1)] = v[u];
         }
     }
     __syncthreads();
@@ -842,12 +847,13 @@ __device__ __forceinline__ void prolong2_body(const Blk &kb, const LevelDev &F, 
         for (int h = 0; h < 2; ++h) {
             const int i = 2 * (pc + c) + h;
             const int tx[2] = {i / 2 - I0, i / 2 + ((i & 1) ? 1 : -1) - I0};
+            const int sx[2] = {(tx[0] & 1) * TXH + (tx[0] >> 1), (tx[1] & 1) * TXH + (tx[1] >> 1)};
             T s = T(0);
             for (int c3 = 0; c3 < 2; ++c3)
                 for (int c2 = 0; c2 < 2; ++c2)
                     for (int c1 = 0; c1 < 2; ++c1) {
-                        T w = wt[c3] * wt[c2] * wt[c1];
-                        s = s + w * tile[(c3 * TY + ty[c2]) * TX + tx[c1]];
+                        T wgt = wt[c3] * wt[c2] * wt[c1];
+                        s = s + wgt * tile[(c3 * TY + ty[c2]) * TX + sx[c1]];
                     }
             out[h][c] = xo[h][c] + s;
         }
