@@ -614,13 +614,13 @@ __device__ __forceinline__ void restrict_body(const Blk &kb, const LevelDev &F, 
     const int t = kb.tx;
     const int I0 = kb.bx * RS_CX, J0 = kb.by * RS_CY, K = kb.bz;
     // The tile's rows r = z RS_RY + y: wavefront w loads columns 0..63 of
-    // rows w, w + RB/64, ...; a row's (j, k) and its part of the storage
-    // index are the same on every lane (scalars), the lane's part (its
-    // column's i) is computed once. Columns 64.. of every row are one more
-    // load for the first threads. All loads are issued before the first LDS
-    // store waits.
+    // the rows of fine plane z = w; a row's (j, k) and its part of the
+    // storage index are the same on every lane (scalars, k once per
+    // wavefront), the lane's part (its column's i) is computed once. Columns
+    // 64.. of every row are one more load for the first threads. All loads
+    // are issued before the first LDS store waits.
     constexpr int NW = RB / 64, NROW = RS_RY * 4, RPW = NROW / NW, NEXTRA = (RS_RX - 64) * NROW;
-    static_assert(NROW % NW == 0 && RS_RX >= 64 && NEXTRA <= RB, "row split of the restriction tile");
+    static_assert(NW == 4 && RPW == RS_RY && RS_RX >= 64 && NEXTRA <= RB, "row split of the restriction tile");
     const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int half = F.split ? F.half : 0, rl = F.split ? F.nx >> 1 : F.nx;
     const int il = tile_coord(2 * I0 - 1 + lane, F.nx, F.bt[0], F.bt[1]);
@@ -628,11 +628,10 @@ __device__ __forceinline__ void restrict_body(const Blk &kb, const LevelDev &F, 
     const int lo0 = F.split ? (il & 1) * half + (il >> 1) : il;
     const int lo1 = F.split ? ((il & 1) ^ 1) * half + (il >> 1) : il;
     T v[RPW], ve = T(0);
+    const int k = tile_coord(2 * K - 1 + w, F.nz, F.bt[4], F.bt[5]);
 #pragma unroll
     for (int u = 0; u < RPW; ++u) {
-        const int r = w + NW * u, y = r % RS_RY, z = r / RS_RY;
-        const int j = tile_coord(2 * J0 - 1 + y, F.ny, F.bt[2], F.bt[3]);
-        const int k = tile_coord(2 * K - 1 + z, F.nz, F.bt[4], F.bt[5]);
+        const int j = tile_coord(2 * J0 - 1 + u, F.ny, F.bt[2], F.bt[3]);
         const int rowoff = (k * F.ny + j) * rl;
         v[u] = rf[rowoff + (((j + k) & 1) ? lo1 : lo0)];
     }
@@ -644,7 +643,7 @@ __device__ __forceinline__ void restrict_body(const Blk &kb, const LevelDev &F, 
         ve = rf[lidx(F, i, j, k)];
     }
 #pragma unroll
-    for (int u = 0; u < RPW; ++u) tile[(w + NW * u) * RS_RX + lane] = v[u];
+    for (int u = 0; u < RPW; ++u) tile[(w * RS_RY + u) * RS_RX + lane] = v[u];
     if (t < NEXTRA) tile[(t / (RS_RX - 64)) * RS_RX + 64 + t % (RS_RX - 64)] = ve;
     __syncthreads();
     const int I = I0 + t % RS_CX, J = J0 + t / RS_CX;
