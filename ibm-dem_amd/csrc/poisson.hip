@@ -811,28 +811,54 @@ __device__ __forceinline__ void prolong2_body(const Blk &kb, const LevelDev &F, 
     T xo[2][2];
     ld2(xf + (long)ca * F.half + pos, xo[0][0], xo[0][1]);
     ld2(xf + (long)(1 - ca) * F.half + pos, xo[1][0], xo[1][1]);
-    // The tile's rows r = c TY + y (coarse row J0 + y of parent plane c):
-    // wavefront w stages rows w, w + NW, ...; a row's (J, K) are the same on
-    // every lane (scalars), and lane l takes columns l, l + 64, l + 128 (<
-    // TX), whose coarse indices it maps once. Column x is stored at
-    // (x & 1) * TXH + x / 2 of its row: the summing loop's lanes read
-    // columns two apart, which land in consecutive words.
+    // The tile's rows r = c TY + y (coarse row J0 + y of parent plane c).
+    // Rows of 64 or more columns: wavefront w stages rows w, w + NW, ...; a
+    // row's (J, K) are the same on every lane (scalars), and lane l takes
+    // columns l, l + 64, l + 128 (< TX), whose coarse indices it maps once.
+    // Column x is stored at (x & 1) * TXH + x / 2 of its row: the summing
+    // loop's lanes read columns two apart, which land in consecutive words.
     const int TXH = bx + 1, NROW = 2 * TY, NW = bx * by / 64;
-    const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
-    int Ic[3];
+    if (TX >= 64) {
+        const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
+        int Ic[3];
 #pragma unroll
-    for (int u = 0; u < 3; ++u) Ic[u] = tile_coord(I0 + lane + 64 * u, Cl.nx, Cl.bt[0], Cl.bt[1]);
-    for (int r = w; r < NROW; r += NW) {
-        const int y = r % TY, c = r / TY;
-        const int J = tile_coord(J0 + y, Cl.ny, Cl.bt[2], Cl.bt[3]), K = pk[c];
-        T v[3];
+        for (int u = 0; u < 3; ++u) Ic[u] = tile_coord(I0 + lane + 64 * u, Cl.nx, Cl.bt[0], Cl.bt[1]);
+        for (int r = w; r < NROW; r += NW) {
+            const int y = r % TY, c = r / TY;
+            const int J = tile_coord(J0 + y, Cl.ny, Cl.bt[2], Cl.bt[3]), K = pk[c];
+            T v[3];
 #pragma unroll
-        for (int u = 0; u < 3; ++u)
-            if (lane + 64 * u < TX) v[u] = xc[lidx(Cl, Ic[u], J, K)];
+            for (int u = 0; u < 3; ++u)
+                if (lane + 64 * u < TX) v[u] = xc[lidx(Cl, Ic[u], J, K)];
 #pragma unroll
-        for (int u = 0; u < 3; ++u) {
-            const int x = lane + 64 * u;
-            if (x < TX) tile[r * TX + (x & 1) * TXH + (x >> 1)] = v[u];
+            for (int u = 0; u < 3; ++u) {
+                const int x = lane + 64 * u;
+                if (x < TX) tile[r * TX + (x & 1) * TXH + (x >> 1)] = v[u];
+            }
+        }
+    } else {
+        // short rows (blocks narrower than 32 pair-pairs): the tile's
+        // elements dealt to the threads in order, up to four loads of a
+        // thread in flight before the LDS stores wait
+        const int NT = NROW * TX;
+        for (int q0 = t; q0 < NT; q0 += 4 * bx * by) {
+            T v[4];
+            int dst[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = q0 + u * bx * by;
+                dst[u] = -1;
+                if (q < NT) {
+                    const int r = q / TX, x = q - r * TX, y = r % TY, c = r / TY;
+                    const int I = tile_coord(I0 + x, Cl.nx, Cl.bt[0], Cl.bt[1]);
+                    const int J = tile_coord(J0 + y, Cl.ny, Cl.bt[2], Cl.bt[3]);
+                    v[u] = xc[lidx(Cl, I, J, pk[c])];
+                    dst[u] = r * TX + (x & 1) * TXH + (x >> 1);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (dst[u] >= 0) tile[dst[u]] = v[u];
         }
     }
     __syncthreads();
