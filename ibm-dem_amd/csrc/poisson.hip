@@ -1723,7 +1723,6 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
         CHECK_LAUNCH(c);
         return IBMDEM_OK;
     }
-    LaunchShape rb = shape_for(div_up(d.nx, 2), d.ny, d.nz);
     // level-0 half-sweeps are the profiled (dominant) kernel: each reads the
     // right-hand side and the other colour of half the cells and writes them;
     // the residual variant also writes r of its half, the mixed form's r.z
