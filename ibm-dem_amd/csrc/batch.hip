@@ -447,7 +447,7 @@ int fill_post(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs, std::string
     int rc = poisson_plan_gauge(P, cs, err);
     if (!rc) rc = fluid_plan_project(P, cs, err);
     if (!rc) rc = dem_plan_body_hydro(P, cases, cs, err);
-    for (int s = 0; s < ci && !rc; ++s) rc = dem_plan_substep(P, cases, cs, s == ci - 1, err);
+    for (int s = 0; s < ci && !rc; ++s) rc = dem_plan_substep(P, cases, cs, s == 0, s == ci - 1, err);
     if (!rc) rc = dem_plan_probe(P, cases, cs, err);
     return rc;
 }

@@ -524,9 +524,12 @@ struct ibmdem_ctx {
     double *bfdrag = nullptr, *bfmom = nullptr, *bfcol = nullptr, *bfmomcol = nullptr;
     double *bfhydro = nullptr, *bfmomhydro = nullptr;
     double *brmax = nullptr;     // max |x_m - X_b| over the body's markers
-    int *bnear = nullptr;        // number of other bodies within contact reach
+    int *bnear = nullptr;        // number of other bodies within contact reach; [B]: any
+                                 // body has one
     int *blist = nullptr;        // their ids, up to 32 per body
     bool rmax_dirty = true;
+    bool bin_gate = true;        // rebuilds between sub-steps gated by the culling
+                                 // (IBMDEM_NO_BIN_GATE=1 at creation: always bin)
     // pair list for parity (allocated on demand)
     int *pair_count = nullptr, *pair_partner = nullptr, *pair_shift = nullptr;
     int pair_cap = 0;
@@ -645,7 +648,7 @@ struct BatchPlan;
 }
 size_t dem_plan_cases(bk::BatchPlan &P, const std::vector<ibmdem_ctx *> &cs);
 int dem_plan_body_hydro(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, std::string &err);
-int dem_plan_substep(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, bool last,
+int dem_plan_substep(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, bool first, bool last,
                      std::string &err);
 int dem_plan_probe(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, std::string &err);
 // the step's error status from a host copy of the context's flags (clears them)

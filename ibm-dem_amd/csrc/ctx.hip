@@ -9,7 +9,7 @@
 #include <cstring>
 #include <new>
 
-int dem_substep_fused(ibmdem_ctx *c, bool last);
+int dem_substep_fused(ibmdem_ctx *c, bool first, bool last);
 
 int set_error(ibmdem_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
@@ -132,7 +132,9 @@ int alloc_all(ibmdem_ctx *c) {
     A(c->brho, B); A(c->bfixed, B);
     A(c->bfdrag, 3L * B); A(c->bfmom, 3L * B); A(c->bfcol, 3L * B); A(c->bfmomcol, 3L * B);
     A(c->bfhydro, 3L * B); A(c->bfmomhydro, 3L * B);
-    A(c->brmax, B); A(c->bnear, B); A(c->blist, 32L * B);
+    A(c->brmax, B); A(c->bnear, B + 1); A(c->blist, 32L * B);
+    const char *ng = std::getenv("IBMDEM_NO_BIN_GATE");
+    c->bin_gate = !(ng && *ng && *ng != '0');
 #undef A
     return poisson_setup(c);
 }
@@ -222,7 +224,8 @@ int step_post(ibmdem_ctx *c) {
     TRY(poisson_gauge(c));
     TRY(fluid_project(c, 1.0));
     TRY(dem_body_hydro(c));
-    for (int s = 0; s < c->cs.couple_interval; ++s) TRY(dem_substep_fused(c, s == c->cs.couple_interval - 1));
+    for (int s = 0; s < c->cs.couple_interval; ++s)
+        TRY(dem_substep_fused(c, s == 0, s == c->cs.couple_interval - 1));
     return dem_probe_append(c);
 }
 
@@ -688,13 +691,14 @@ int ibmdem_dem_rebuild_lists(ibmdem_ctx *c) {
 }
 int ibmdem_dem_substep(ibmdem_ctx *c) {
     CTX_OR_ARG(c);
-    TRY(dem_substep_fused(c, true));
+    TRY(dem_substep_fused(c, true, true));
     return dem_check_flags(c);
 }
 int ibmdem_cloud_evolution(ibmdem_ctx *c) {
     CTX_OR_ARG(c);
     TRY(dem_body_hydro(c));
-    for (int s = 0; s < c->cs.couple_interval; ++s) TRY(dem_substep_fused(c, s == c->cs.couple_interval - 1));
+    for (int s = 0; s < c->cs.couple_interval; ++s)
+        TRY(dem_substep_fused(c, s == 0, s == c->cs.couple_interval - 1));
     return dem_check_flags(c);
 }
 

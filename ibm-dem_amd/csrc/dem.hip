@@ -43,9 +43,18 @@ static_assert(T == SCAN_T && CHUNK == SCAN_CHUNK, "the binning scans use the sha
 // ---------------------------------------------------------------------------
 // binning
 // ---------------------------------------------------------------------------
+// The cell bins (bin_start, bin_items) are read only by a contact scan of a
+// marker whose body has another body in reach, and by the IBM tables at the
+// start of a step. A rebuild between two sub-steps of a step therefore
+// takes a gate, near[B] of k_body_near for the next sub-step: where it is 0
+// no marker is scanned, and only the cells and the periodic wraps are
+// updated (the counts stay zero, the bins keep their last contents). A null
+// gate always bins.
+__device__ __forceinline__ bool bins_off(const int *gate) { return gate != nullptr && *gate == 0; }
+
 __device__ __forceinline__ void wrap_cells_count_body(const Blk &kb, int M, GridDev g, double *__restrict__ mx,
                                                         int *__restrict__ mcell, int *__restrict__ bin_count,
-                                                        int *__restrict__ flags) {
+                                                        int *__restrict__ flags, const int *gate) {
     const int m = kb.bx * T + kb.tx;
     if (m >= M) return;
     const int n[3] = {g.nx, g.ny, g.nz};
@@ -81,45 +90,50 @@ __device__ __forceinline__ void wrap_cells_count_body(const Blk &kb, int M, Grid
     }
     const int cl = (int)cidx(g, c[0], c[1], c[2]);
     mcell[m] = cl;
-    atomicAdd(&bin_count[cl], 1);
+    if (!bins_off(gate)) atomicAdd(&bin_count[cl], 1);
 }
 __global__ __launch_bounds__(T) void k_wrap_cells_count(int M, GridDev g, double *__restrict__ mx,
                                                         int *__restrict__ mcell, int *__restrict__ bin_count,
-                                                        int *__restrict__ flags) {
-    wrap_cells_count_body(hw_blk(), M, g, mx, mcell, bin_count, flags);
+                                                        int *__restrict__ flags, const int *gate) {
+    wrap_cells_count_body(hw_blk(), M, g, mx, mcell, bin_count, flags, gate);
 }
 
-// the counting scan's three kernels (scan.hpp)
-__global__ __launch_bounds__(T) void k_chunk_sum(long n, const int *__restrict__ a, int *__restrict__ out) {
+// the counting scan's three kernels (scan.hpp); a closed gate ends every
+// block before its first barrier
+__global__ __launch_bounds__(T) void k_chunk_sum(long n, const int *__restrict__ a, int *__restrict__ out,
+                                                 const int *gate) {
+    if (bins_off(gate)) return;
     chunk_sum_body(hw_blk(), n, a, out);
 }
-__global__ __launch_bounds__(1024) void k_scan_blocks(int nb, int *__restrict__ a) {
+__global__ __launch_bounds__(1024) void k_scan_blocks(int nb, int *__restrict__ a, const int *gate) {
+    if (bins_off(gate)) return;
     scan_blocks_body(hw_blk(), nb, a);
 }
 __global__ __launch_bounds__(T) void k_chunk_scan(long n, const int *__restrict__ cnt, const int *__restrict__ off,
-                                                  int *__restrict__ start, int nb) {
+                                                  int *__restrict__ start, int nb, const int *gate) {
+    if (bins_off(gate)) return;
     chunk_scan_body(hw_blk(), n, cnt, off, start, nb);
 }
 
 // unordered placement; the counts return to zero
 __device__ __forceinline__ void bin_scatter_body(const Blk &kb, int M, const int *__restrict__ mcell, const int *__restrict__ start,
-                                                   int *__restrict__ cnt, int *__restrict__ tmp) {
+                                                   int *__restrict__ cnt, int *__restrict__ tmp, const int *gate) {
     const int m = kb.bx * T + kb.tx;
-    if (m >= M) return;
+    if (m >= M || bins_off(gate)) return;
     const int cl = mcell[m];
     const int slot = start[cl] + atomicSub(&cnt[cl], 1) - 1;
     tmp[slot] = m;
 }
 __global__ __launch_bounds__(T) void k_bin_scatter(int M, const int *__restrict__ mcell, const int *__restrict__ start,
-                                                   int *__restrict__ cnt, int *__restrict__ tmp) {
-    bin_scatter_body(hw_blk(), M, mcell, start, cnt, tmp);
+                                                   int *__restrict__ cnt, int *__restrict__ tmp, const int *gate) {
+    bin_scatter_body(hw_blk(), M, mcell, start, cnt, tmp, gate);
 }
 
 // ascending id inside each cell: position = start + #{smaller ids in cell}
 __device__ __forceinline__ void bin_rank_body(const Blk &kb, int M, const int *__restrict__ mcell, const int *__restrict__ start,
-                                                const int *__restrict__ tmp, int *__restrict__ items) {
+                                                const int *__restrict__ tmp, int *__restrict__ items, const int *gate) {
     const int m = kb.bx * T + kb.tx;
-    if (m >= M) return;
+    if (m >= M || bins_off(gate)) return;
     const int cl = mcell[m];
     const int q0 = start[cl], q1 = start[cl + 1];
     int r = 0;
@@ -127,8 +141,8 @@ __device__ __forceinline__ void bin_rank_body(const Blk &kb, int M, const int *_
     items[q0 + r] = m;
 }
 __global__ __launch_bounds__(T) void k_bin_rank(int M, const int *__restrict__ mcell, const int *__restrict__ start,
-                                                const int *__restrict__ tmp, int *__restrict__ items) {
-    bin_rank_body(hw_blk(), M, mcell, start, tmp, items);
+                                                const int *__restrict__ tmp, int *__restrict__ items, const int *gate) {
+    bin_rank_body(hw_blk(), M, mcell, start, tmp, items, gate);
 }
 
 // ---------------------------------------------------------------------------
@@ -176,10 +190,11 @@ __global__ __launch_bounds__(T) void k_body_rmax(int M, int B, GridDev g, const 
 // The bodies c whose markers can come within the candidate distance of body
 // b's: |X_b - X_c| < rmax_b + rmax_c + lim (+ a safety margin). near[b]
 // counts them and list[b * NEAR_MAX + s] holds the first NEAR_MAX (in no
-// particular order: the list only culls, see k_contact). Thread b of block
+// particular order: the list only culls, see k_contact), and near[B] is 1
+// if any body has one (the gate of the bins, above). Thread b of block
 // (x, y) tests body b = 64x + t against the NEAR_CH bodies c of chunk y,
 // staged in LDS (every block would otherwise read all B positions from L2);
-// k_body_near_clear zeroed the counts. The periodic minimum image takes
+// k_body_near_clear zeroed the counts and near[B]. The periodic minimum image takes
 // rint(d * (1/L)): it can differ from rint(d / L) only where both images are
 // L/2 away, within rounding, which the safety margin covers, so no body in
 // reach is missed.
@@ -187,7 +202,7 @@ constexpr int NEAR_CH = 64;
 constexpr int NEAR_MAX = 32;
 __device__ __forceinline__ void body_near_clear_body(const Blk &kb, int B, int *__restrict__ near) {
     const int b = kb.bx * 64 + kb.tx;
-    if (b < B) near[b] = 0;
+    if (b <= B) near[b] = 0;
 }
 __global__ __launch_bounds__(64) void k_body_near_clear(int B, int *__restrict__ near) {
     body_near_clear_body(hw_blk(), B, near);
@@ -221,6 +236,7 @@ __device__ __forceinline__ void body_near_body(const Blk &kb, int B, GridDev g, 
         if (d2 < lim_bc * lim_bc) {
             const int slot = atomicAdd(&near[b], 1);
             if (slot < NEAR_MAX) list[(long)b * NEAR_MAX + slot] = c0 + q;
+            near[B] = 1;
         }
     }
 }
@@ -653,38 +669,41 @@ int scan_counts(ibmdem_ctx *c, const int *cnt, int *start, long n) {
     hipStream_t s = c->stream;
     const int nb = div_up(n, CHUNK);
     if (nb + 1 > c->scan_nblocks) return set_error(c, IBMDEM_ERR_CAPACITY, "scan longer than its block buffer");
-    hipLaunchKernelGGL(k_chunk_sum, dim3(nb), dim3(T), 0, s, n, cnt, c->scan_blocks);
+    const int *none = nullptr;
+    hipLaunchKernelGGL(k_chunk_sum, dim3(nb), dim3(T), 0, s, n, cnt, c->scan_blocks, none);
     CHECK_LAUNCH(c);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, nb, c->scan_blocks);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, nb, c->scan_blocks, none);
     CHECK_LAUNCH(c);
-    hipLaunchKernelGGL(k_chunk_scan, dim3(nb), dim3(T), 0, s, n, cnt, c->scan_blocks, start, nb);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(nb), dim3(T), 0, s, n, cnt, c->scan_blocks, start, nb, none);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
 
-int dem_rebuild(ibmdem_ctx *c) {
+int dem_rebuild(ibmdem_ctx *c, const int *gate) {
     const int M = c->M;
     if (M == 0) return IBMDEM_OK;
     hipStream_t s = c->stream;
     const long n = c->n;
     hipLaunchKernelGGL(k_wrap_cells_count, dim3(div_up(M, T)), dim3(T), 0, s, M, c->g, c->mx, c->mcell,
-                       c->bin_count, c->flags);
+                       c->bin_count, c->flags, gate);
     CHECK_LAUNCH(c);
     const int nb = div_up(n, CHUNK);
-    hipLaunchKernelGGL(k_chunk_sum, dim3(nb), dim3(T), 0, s, n, c->bin_count, c->scan_blocks);
+    hipLaunchKernelGGL(k_chunk_sum, dim3(nb), dim3(T), 0, s, n, c->bin_count, c->scan_blocks, gate);
     CHECK_LAUNCH(c);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, nb, c->scan_blocks);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, nb, c->scan_blocks, gate);
     CHECK_LAUNCH(c);
-    hipLaunchKernelGGL(k_chunk_scan, dim3(nb), dim3(T), 0, s, n, c->bin_count, c->scan_blocks, c->bin_start, nb);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(nb), dim3(T), 0, s, n, c->bin_count, c->scan_blocks, c->bin_start, nb,
+                       gate);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_bin_scatter, dim3(div_up(M, T)), dim3(T), 0, s, M, c->mcell, c->bin_start, c->bin_count,
-                       c->bin_tmp);
+                       c->bin_tmp, gate);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_bin_rank, dim3(div_up(M, T)), dim3(T), 0, s, M, c->mcell, c->bin_start, c->bin_tmp,
-                       c->bin_items);
+                       c->bin_items, gate);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
+int dem_rebuild(ibmdem_ctx *c) { return dem_rebuild(c, nullptr); }
 
 // the candidate distance of the contact scan, as the body culling uses it
 double near_lim(const ibmdem_ctx *c) { return c->cs.collision_dist * (1.0 + c->cs.collision_buffer); }
@@ -698,7 +717,7 @@ int dem_near(ibmdem_ctx *c) {
         c->rmax_dirty = false;
     }
     const double lim = near_lim(c);
-    hipLaunchKernelGGL(k_body_near_clear, dim3(div_up(c->B, 64)), dim3(64), 0, c->stream, c->B, c->bnear);
+    hipLaunchKernelGGL(k_body_near_clear, dim3(div_up(c->B + 1, 64)), dim3(64), 0, c->stream, c->B, c->bnear);
     CHECK_LAUNCH(c);
     hipLaunchKernelGGL(k_body_near, dim3(div_up(c->B, 64), div_up(c->B, NEAR_CH)), dim3(64), 0, c->stream, c->B,
                        c->g, c->bx, c->brmax, lim, c->bnear, c->blist);
@@ -706,11 +725,10 @@ int dem_near(ibmdem_ctx *c) {
     return IBMDEM_OK;
 }
 
-int dem_contact(ibmdem_ctx *c) {
+// the contact scan with the body culling of the last dem_near
+int dem_contact_scan(ibmdem_ctx *c) {
     const int M = c->M;
     if (M == 0) return IBMDEM_OK;
-    int rc;
-    if ((rc = dem_near(c))) return rc;
     prof_begin(c, IBMDEM_K_CONTACT);
     hipLaunchKernelGGL((k_contact<false>), dim3(div_up(M, T)), dim3(T), 0, c->stream, M, c->g, c->mx, c->mv,
                        c->mmass, c->mrad, c->mbody, c->mcell, c->bin_start, c->bin_items, c->fcol,
@@ -719,6 +737,13 @@ int dem_contact(ibmdem_ctx *c) {
     prof_end(c, IBMDEM_K_CONTACT, 0.0);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
+}
+
+int dem_contact(ibmdem_ctx *c) {
+    if (c->M == 0) return IBMDEM_OK;
+    int rc;
+    if ((rc = dem_near(c))) return rc;
+    return dem_contact_scan(c);
 }
 
 int dem_pairs(ibmdem_ctx *c, int cap) {
@@ -775,10 +800,17 @@ int dem_markers_follow(ibmdem_ctx *c) {
     return IBMDEM_OK;
 }
 
-// one CloudEvolutionStepMS with the body reduction and Newton-Euler fused
-int dem_substep_fused(ibmdem_ctx *c, bool last) {
+// one CloudEvolutionStepMS with the body reduction and Newton-Euler fused.
+// The body culling of the first sub-step of a sequence is computed here;
+// each later sub-step's right after the bodies of the one before have moved
+// (its contact scan sees those positions), so that it gates that sub-step's
+// rebuild. The last sub-step's rebuild always bins: the IBM tables of the
+// next step and the pair lists read the bins.
+int dem_substep_fused(ibmdem_ctx *c, bool first, bool last) {
     int rc;
-    if ((rc = dem_contact(c))) return rc;
+    const bool cull = c->M > 0 && c->B > 0;
+    if (first && cull && (rc = dem_near(c))) return rc;
+    if ((rc = dem_contact_scan(c))) return rc;
     if (c->B > 0) {
         hipLaunchKernelGGL(k_body_reduce, dim3(c->B), dim3(BT), 0, c->stream, c->M, c->B, c->g, c->body_start,
                            c->mx, c->fdrag, c->fcol, c->mvol, body_state(c), last ? 1 : 0, 1, c->cs.rhof,
@@ -786,8 +818,9 @@ int dem_substep_fused(ibmdem_ctx *c, bool last) {
                            c->cs.gravity[2]);
         CHECK_LAUNCH(c);
     }
+    if (!last && cull && (rc = dem_near(c))) return rc;
     if ((rc = dem_markers_follow(c))) return rc;
-    return dem_rebuild(c);
+    return dem_rebuild(c, last || !cull || !c->bin_gate ? nullptr : c->bnear + c->B);
 }
 
 int dem_check_flags(ibmdem_ctx *c) {
@@ -932,40 +965,51 @@ __device__ __forceinline__ BodyState glb_state(const BodyState &s) {
                      glb(s.fmomcol), glb(s.fhydro), glb(s.fmomhydro), glb(s.flags)};
 }
 
-__global__ __launch_bounds__(T) void kb_wrap_cells_count(DTab tab, const DemCase *cs) {
+// the bins' gate of a case (uniform argument gated: a rebuild between two
+// sub-steps); the culling runs only where the case has markers and bodies
+__device__ __forceinline__ const int *bin_gate(const DemCase &c, int gated) {
+    return gated && c.M > 0 && c.B > 0 ? glb((const int *)c.bnear) + c.B : nullptr;
+}
+__global__ __launch_bounds__(T) void kb_wrap_cells_count(DTab tab, const DemCase *cs, int gated) {
     batch_for(tab, [&](const DArg &a, const Blk &kb) {
         const DemCase &c = *cst(cs + a.q);
-        wrap_cells_count_body(kb, c.M, c.g, glb(c.mx), glb(c.mcell), glb(c.bin_count), glb(c.flags));
+        wrap_cells_count_body(kb, c.M, c.g, glb(c.mx), glb(c.mcell), glb(c.bin_count), glb(c.flags),
+                              bin_gate(c, gated));
     });
 }
-__global__ __launch_bounds__(T) void kb_chunk_sum(DTab tab, const DemCase *cs) {
+__global__ __launch_bounds__(T) void kb_chunk_sum(DTab tab, const DemCase *cs, int gated) {
     batch_for(tab, [&](const DArg &a, const Blk &kb) {
         const DemCase &c = *cst(cs + a.q);
+        if (bins_off(bin_gate(c, gated))) return;
         chunk_sum_body(kb, c.n, glb(c.bin_count), glb(c.scan_blocks));
     });
 }
-__global__ __launch_bounds__(1024) void kb_scan_blocks(DTab tab, const DemCase *cs) {
+__global__ __launch_bounds__(1024) void kb_scan_blocks(DTab tab, const DemCase *cs, int gated) {
     batch_for(tab, [&](const DArg &a, const Blk &kb) {
         const DemCase &c = *cst(cs + a.q);
+        if (bins_off(bin_gate(c, gated))) return;
         scan_blocks_body(kb, c.nb, glb(c.scan_blocks));
     });
 }
-__global__ __launch_bounds__(T) void kb_chunk_scan(DTab tab, const DemCase *cs) {
+__global__ __launch_bounds__(T) void kb_chunk_scan(DTab tab, const DemCase *cs, int gated) {
     batch_for(tab, [&](const DArg &a, const Blk &kb) {
         const DemCase &c = *cst(cs + a.q);
+        if (bins_off(bin_gate(c, gated))) return;
         chunk_scan_body(kb, c.n, glb(c.bin_count), glb(c.scan_blocks), glb(c.bin_start), c.nb);
     });
 }
-__global__ __launch_bounds__(T) void kb_bin_scatter(DTab tab, const DemCase *cs) {
+__global__ __launch_bounds__(T) void kb_bin_scatter(DTab tab, const DemCase *cs, int gated) {
     batch_for(tab, [&](const DArg &a, const Blk &kb) {
         const DemCase &c = *cst(cs + a.q);
-        bin_scatter_body(kb, c.M, glb(c.mcell), glb(c.bin_start), glb(c.bin_count), glb(c.bin_tmp));
+        bin_scatter_body(kb, c.M, glb(c.mcell), glb(c.bin_start), glb(c.bin_count), glb(c.bin_tmp),
+                         bin_gate(c, gated));
     });
 }
-__global__ __launch_bounds__(T) void kb_bin_rank(DTab tab, const DemCase *cs) {
+__global__ __launch_bounds__(T) void kb_bin_rank(DTab tab, const DemCase *cs, int gated) {
     batch_for(tab, [&](const DArg &a, const Blk &kb) {
         const DemCase &c = *cst(cs + a.q);
-        bin_rank_body(kb, c.M, glb(c.mcell), glb(c.bin_start), glb(c.bin_tmp), glb(c.bin_items));
+        bin_rank_body(kb, c.M, glb(c.mcell), glb(c.bin_start), glb(c.bin_tmp), glb(c.bin_items),
+                      bin_gate(c, gated));
     });
 }
 __global__ __launch_bounds__(64) void kb_body_near_clear(DTab tab, const DemCase *cs) {
@@ -1018,12 +1062,19 @@ __global__ __launch_bounds__(64) void kb_probe_append(DTab tab, const DemCase *c
         hipLaunchKernelGGL(kb_##name, dim3(o.blocks), dim3(threads), 0, s, table<DArg>(o),            \
                            (const DemCase *)o.aux);                                                  \
     }
-DEM_LAUNCHER(wrap_cells_count, T)
-DEM_LAUNCHER(chunk_sum, T)
-DEM_LAUNCHER(scan_blocks, 1024)
-DEM_LAUNCHER(chunk_scan, T)
-DEM_LAUNCHER(bin_scatter, T)
-DEM_LAUNCHER(bin_rank, T)
+// the rebuild's kernels take the gate switch as o.op
+#define DEM_GATED_LAUNCHER(name, threads)                                                             \
+    void launch_##name(const BOp &o, hipStream_t s, int) {                                           \
+        hipLaunchKernelGGL(kb_##name, dim3(o.blocks), dim3(threads), 0, s, table<DArg>(o),            \
+                           (const DemCase *)o.aux, o.op);                                            \
+    }
+DEM_GATED_LAUNCHER(wrap_cells_count, T)
+DEM_GATED_LAUNCHER(chunk_sum, T)
+DEM_GATED_LAUNCHER(scan_blocks, 1024)
+DEM_GATED_LAUNCHER(chunk_scan, T)
+DEM_GATED_LAUNCHER(bin_scatter, T)
+DEM_GATED_LAUNCHER(bin_rank, T)
+#undef DEM_GATED_LAUNCHER
 DEM_LAUNCHER(body_near_clear, 64)
 DEM_LAUNCHER(body_near, 64)
 DEM_LAUNCHER(contact, T)
@@ -1092,20 +1143,23 @@ int dem_plan_body_hydro(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem
 }
 
 // dem_substep_fused of every case (the contact scan's body culling with the
-// near lists; the caller has made sure no case needs its body radii again)
-int dem_plan_substep(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, bool last,
+// near lists, computed first in the first sub-step and after the body update
+// in the others; the caller has made sure no case needs its body radii again)
+static int dem_plan_near(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+    int rc;
+    auto near = [](ibmdem_ctx *c) { return (unsigned)(c->M > 0 && c->B > 0 ? div_up(c->B, 64) : 0); };
+    if ((rc = dem_op(P, cases, cs, launch_body_near_clear, 64,
+                     [&](ibmdem_ctx *c) { return dim3(near(c) ? div_up(c->B + 1, 64) : 0); }, BOp(), err)))
+        return rc;
+    return dem_op(P, cases, cs, launch_body_near, 64,
+                  [&](ibmdem_ctx *c) { return dim3(near(c), near(c) ? div_up(c->B, NEAR_CH) : 0); }, BOp(), err);
+}
+int dem_plan_substep(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, bool first, bool last,
                      std::string &err) {
     int rc;
     auto mk = [](ibmdem_ctx *c) { return (unsigned)(c->M > 0 ? div_up(c->M, T) : 0); };
-    auto near = [](ibmdem_ctx *c) { return (unsigned)(c->M > 0 && c->B > 0 ? div_up(c->B, 64) : 0); };
-    // dem_contact: dem_near, then the contact scan
-    if ((rc = dem_op(P, cases, cs, launch_body_near_clear, 64, [&](ibmdem_ctx *c) { return dim3(near(c)); },
-                     BOp(), err)))
-        return rc;
-    if ((rc = dem_op(P, cases, cs, launch_body_near, 64,
-                     [&](ibmdem_ctx *c) { return dim3(near(c), near(c) ? div_up(c->B, NEAR_CH) : 0); }, BOp(),
-                     err)))
-        return rc;
+    // dem_contact_scan, after the culling
+    if (first && (rc = dem_plan_near(P, cases, cs, err))) return rc;
     if ((rc = dem_op(P, cases, cs, launch_contact, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(), err)))
         return rc;
     BOp o;
@@ -1113,21 +1167,25 @@ int dem_plan_substep(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ct
     o.op = 1;                // integrate
     if ((rc = dem_op(P, cases, cs, launch_body_reduce, BT, [](ibmdem_ctx *c) { return dim3(c->B); }, o, err)))
         return rc;
+    if (!last && (rc = dem_plan_near(P, cases, cs, err))) return rc;
     if ((rc = dem_op(P, cases, cs, launch_follow, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(), err)))
         return rc;
-    // dem_rebuild
+    // dem_rebuild, gated between sub-steps
+    BOp gb;
+    gb.op = last ? 0 : 1;
+    for (ibmdem_ctx *c : cs) gb.op = c->bin_gate ? gb.op : 0;
     auto chunks = [](ibmdem_ctx *c) { return dim3(c->M > 0 ? div_up(c->n, CHUNK) : 0); };
-    if ((rc = dem_op(P, cases, cs, launch_wrap_cells_count, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(),
+    if ((rc = dem_op(P, cases, cs, launch_wrap_cells_count, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, gb,
                      err)))
         return rc;
-    if ((rc = dem_op(P, cases, cs, launch_chunk_sum, T, chunks, BOp(), err))) return rc;
+    if ((rc = dem_op(P, cases, cs, launch_chunk_sum, T, chunks, gb, err))) return rc;
     if ((rc = dem_op(P, cases, cs, launch_scan_blocks, 1024, [](ibmdem_ctx *c) { return dim3(c->M > 0 ? 1 : 0); },
-                     BOp(), err)))
+                     gb, err)))
         return rc;
-    if ((rc = dem_op(P, cases, cs, launch_chunk_scan, T, chunks, BOp(), err))) return rc;
-    if ((rc = dem_op(P, cases, cs, launch_bin_scatter, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(), err)))
+    if ((rc = dem_op(P, cases, cs, launch_chunk_scan, T, chunks, gb, err))) return rc;
+    if ((rc = dem_op(P, cases, cs, launch_bin_scatter, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, gb, err)))
         return rc;
-    return dem_op(P, cases, cs, launch_bin_rank, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(), err);
+    return dem_op(P, cases, cs, launch_bin_rank, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, gb, err);
 }
 
 // dem_probe_append of every case that records probes
