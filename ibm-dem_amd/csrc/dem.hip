@@ -198,7 +198,7 @@ __global__ __launch_bounds__(T) void k_body_rmax(int M, int B, GridDev g, const 
 // rint(d * (1/L)): it can differ from rint(d / L) only where both images are
 // L/2 away, within rounding, which the safety margin covers, so no body in
 // reach is missed.
-constexpr int NEAR_CH = 64;
+constexpr int NEAR_CH = 16;
 constexpr int NEAR_MAX = 32;
 __device__ __forceinline__ void body_near_clear_body(const Blk &kb, int B, int *__restrict__ near) {
     const int b = kb.bx * 64 + kb.tx;
@@ -262,10 +262,28 @@ __device__ __forceinline__ void contact_body(const Blk &kb, int M, GridDev g, co
     if (i >= M) return;
     const int cl = mcell[i];
     const int ci = cl % g.nx + 1, cj = (cl / g.nx) % g.ny + 1, ck = cl / (g.nx * g.ny) + 1;
+    const int bi = mbody[i];
+    const int nnear = bnear[bi];
+    // wall contact reaches the markers of the first/last interior layer
+    bool walls = false;
+    for (int s = 0; s < 6; ++s) walls = walls || g.bt[s] != IBMDEM_BC_PERIODIC;
+    const bool layer = walls && (ci == 1 || ci == g.nx || cj == 1 || cj == g.ny || ck == 1 || ck == g.nz);
+    if (nnear == 0 && !layer) {
+        // no candidate pair and no wall: the sums below stay zero, so they
+        // are written without loading the marker's state
+        if (0 > max_pairs) flags[FL_CAPACITY] = 1;
+        if (PAIRS) {
+            pcount[i] = 0;
+            return;
+        }
+        fcol[i] = 0.0;
+        fcol[(long)M + i] = 0.0;
+        fcol[2L * M + i] = 0.0;
+        return;
+    }
     const double xi0 = mx[i], xi1 = mx[(long)M + i], xi2 = mx[2L * M + i];
     const double vi0 = mv[i], vi1 = mv[(long)M + i], vi2 = mv[2L * M + i];
     const double mi = mmass[i], ri = mrad[i];
-    const int bi = mbody[i];
     double f0 = 0, f1 = 0, f2 = 0;
     int cnt = 0;
     // a marker with no other body in reach has no candidate pairs: skipping
@@ -274,7 +292,6 @@ __device__ __forceinline__ void contact_body(const Blk &kb, int M, GridDev g, co
     // image), so the marker is scanned only if some body on its body's near
     // list is that close (with the body test's margin), or if the list
     // overflowed.
-    const int nnear = bnear[bi];
     bool scan = nnear > NEAR_MAX;
     if (nnear > 0 && !scan) {
         const double iL[3] = {1.0 / g.L[0], 1.0 / g.L[1], 1.0 / g.L[2]};
@@ -362,8 +379,7 @@ __device__ __forceinline__ void contact_body(const Blk &kb, int M, GridDev g, co
         pcount[i] = cnt;
         return;
     }
-    // wall contact for markers in the first/last interior layer
-    if (ci == 1 || ci == g.nx || cj == 1 || cj == g.ny || ck == 1 || ck == g.nz) {
+    if (layer) {
         for (int s = 0; s < 6; ++s) {
             if (g.bt[s] == IBMDEM_BC_PERIODIC) continue;   // walls, inflow and outflow screens
             const double p1 = (s % 2 == 0) ? 0.0 : g.L[0];

@@ -912,6 +912,61 @@ ProlongShape prolong_shape(const LevelDev &d) {
 // evaluate exactly nbsum's expression without index arithmetic; larger
 // coarsest levels run the same sweeps in global memory.
 constexpr int COARSE_LDS = 512;
+
+// The symmetric red-black sweeps of a coarsest level of n <= 64 CQ cells on
+// one wavefront (lane l owns the cells q = l + 64 t): the lane's neighbour
+// indices, 1/diagonal, right-hand side and colour are read from the tables
+// once into registers, so a half sweep is one round of LDS loads of x (the
+// zero slot x[n] for a missing neighbour). Cells of one colour do not read
+// each other, so the lane's order over them does not matter; each cell's
+// expression is the table sweep's.
+template <int CQ, typename T>
+__device__ __forceinline__ void coarse_reg_sweeps(T *x, const T *b, const short (*nb)[COARSE_LDS], const T *id,
+                                                  const unsigned char *col, int n, int sweeps, T ihx2, T ihy2,
+                                                  T ihz2) {
+    const int lane = threadIdx.x & 63;
+    int nbr[CQ][6], cr[CQ];
+    T br[CQ], idr[CQ];
+#pragma unroll
+    for (int t = 0; t < CQ; ++t) {
+        const int q = lane + 64 * t;
+        cr[t] = -1;
+        br[t] = T(0);
+        idr[t] = T(0);
+        for (int d = 0; d < 6; ++d) nbr[t][d] = n;
+        if (q < n) {
+            for (int d = 0; d < 6; ++d) nbr[t][d] = nb[d][q];
+            br[t] = b[q];
+            idr[t] = id[q];
+            cr[t] = col[q];
+        }
+    }
+    auto half = [&](int color) {
+#pragma unroll
+        for (int t = 0; t < CQ; ++t)
+            if (cr[t] == color) {
+                const T sx = (x[nbr[t][0]] + x[nbr[t][1]]) * ihx2;
+                const T sy = (x[nbr[t][2]] + x[nbr[t][3]]) * ihy2;
+                const T sz = (x[nbr[t][4]] + x[nbr[t][5]]) * ihz2;
+                x[lane + 64 * t] = (br[t] + (sx + sy + sz)) * idr[t];
+            }
+        wave_barrier_lds();
+    };
+    for (int s = 0; s < sweeps; ++s) { half(0); half(1); }
+    for (int s = 0; s < sweeps; ++s) { half(1); half(0); }
+}
+// the register-held sweeps for levels of up to 256 cells; false: larger
+template <typename T>
+__device__ __forceinline__ bool coarse_sweeps_small(T *x, const T *b, const short (*nb)[COARSE_LDS], const T *id,
+                                                    const unsigned char *col, int n, int sweeps, T ihx2, T ihy2,
+                                                    T ihz2) {
+    if (n <= 64) coarse_reg_sweeps<1, T>(x, b, nb, id, col, n, sweeps, ihx2, ihy2, ihz2);
+    else if (n <= 128) coarse_reg_sweeps<2, T>(x, b, nb, id, col, n, sweeps, ihx2, ihy2, ihz2);
+    else if (n <= 256) coarse_reg_sweeps<4, T>(x, b, nb, id, col, n, sweeps, ihx2, ihy2, ihz2);
+    else return false;
+    return true;
+}
+
 template <typename T>
 __device__ __forceinline__ void coarse_lds_body(const Blk &kb, const LevelDev &L, T *__restrict__ xg, const T *__restrict__ bg,
                                                    int sweeps, const double *__restrict__ stop) {
@@ -954,8 +1009,10 @@ __device__ __forceinline__ void coarse_lds_body(const Blk &kb, const LevelDev &L
         __syncthreads();
     };
     if (L.rb_valid) {
-        for (int s = 0; s < sweeps; ++s) { half(0); half(1); }
-        for (int s = 0; s < sweeps; ++s) { half(1); half(0); }
+        if (!coarse_sweeps_small<T>(x, b, nb, id, col, n, sweeps, ihx2, ihy2, ihz2)) {
+            for (int s = 0; s < sweeps; ++s) { half(0); half(1); }
+            for (int s = 0; s < sweeps; ++s) { half(1); half(0); }
+        }
     } else {
         const T om = T(0.8);
         for (int s = 0; s < 2 * sweeps; ++s) {
@@ -1025,6 +1082,8 @@ __global__ __launch_bounds__(COARSE_T) void k_coarse(LevelDev L, T *__restrict__
 // ---------------------------------------------------------------------------
 constexpr int TAIL_T = 1024;
 constexpr int TAIL_BYTES = 112 * 1024;   // x (one extra zero slot on the coarsest level), b, r of the levels
+constexpr int TAIL_MAXLEV = 6;           // levels of one tail (their descriptors are copied to LDS)
+static_assert(sizeof(LevelDev) % 4 == 0, "level descriptors copied by words");
 // (the tail's levels hold at most TAIL_BYTES / 12 cells, so its index
 // divisions go through udiv_small)
 static_assert(TAIL_BYTES / 12 < (1 << 14), "tail levels small enough for udiv_small");
@@ -1094,21 +1153,33 @@ template <typename T>
 __device__ __forceinline__ void tail_restrict(const LevelDev &F, const T *rf, const LevelDev &Cl, T *bc) {
     const T wt[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
     const int n = Cl.nx * Cl.ny * Cl.nz;
+    const int rl = F.split ? F.nx >> 1 : F.nx, hs = F.split ? F.half : 0;
     for (int q = threadIdx.x; q < n; q += TAIL_T) {
         int I, J, K;
         lcell_small(Cl, q, I, J, K);
-        int fi[4], fj[4], fk[4];
+        // lidx(F, i, j, k) = ((i & 1) ^ ((j + k) & 1)) * half + row(j, k) + (i >> 1)
+        // when split, row(j, k) + i otherwise: its parts per tap coordinate
+        int xo[4], xp[4], ro[16], rp[16];
         for (int a = 0; a < 4; ++a) {
-            fi[a] = tile_coord(2 * I - 1 + a, F.nx, F.bt[0], F.bt[1]);
-            fj[a] = tile_coord(2 * J - 1 + a, F.ny, F.bt[2], F.bt[3]);
-            fk[a] = tile_coord(2 * K - 1 + a, F.nz, F.bt[4], F.bt[5]);
+            const int fi = tile_coord(2 * I - 1 + a, F.nx, F.bt[0], F.bt[1]);
+            xo[a] = F.split ? fi >> 1 : fi;
+            xp[a] = fi & 1;
+        }
+        for (int c3 = 0; c3 < 4; ++c3) {
+            const int fk = tile_coord(2 * K - 1 + c3, F.nz, F.bt[4], F.bt[5]);
+            for (int c2 = 0; c2 < 4; ++c2) {
+                const int fj = tile_coord(2 * J - 1 + c2, F.ny, F.bt[2], F.bt[3]);
+                ro[4 * c3 + c2] = (fk * F.ny + fj) * rl;
+                rp[4 * c3 + c2] = (fj + fk) & 1;
+            }
         }
         T s = T(0);
         for (int c3 = 0; c3 < 4; ++c3)
             for (int c2 = 0; c2 < 4; ++c2)
                 for (int c1 = 0; c1 < 4; ++c1) {
                     T w = wt[c3] * wt[c2] * wt[c1];
-                    s = s + w * rf[lidx(F, fi[c1], fj[c2], fk[c3])];
+                    const int r = 4 * c3 + c2;
+                    s = s + w * rf[(xp[c1] ^ rp[r]) * hs + ro[r] + xo[c1]];
                 }
         bc[q] = T(0.125) * s;
     }
@@ -1119,6 +1190,7 @@ template <typename T>
 __device__ __forceinline__ void tail_prolong(const LevelDev &F, T *xf, const LevelDev &Cl, const T *xc) {
     const T wt[2] = {T(0.75), T(0.25)};
     const int n = F.nx * F.ny * F.nz;
+    const int rl = Cl.split ? Cl.nx >> 1 : Cl.nx, hs = Cl.split ? Cl.half : 0;
     for (int q = threadIdx.x; q < n; q += TAIL_T) {
         int i, j, k;
         lcell_small(F, q, i, j, k);
@@ -1126,12 +1198,24 @@ __device__ __forceinline__ void tail_prolong(const LevelDev &F, T *xf, const Lev
         parents(i, Cl.nx, Cl.bt[0], Cl.bt[1], pi[0], pi[1]);
         parents(j, Cl.ny, Cl.bt[2], Cl.bt[3], pj[0], pj[1]);
         parents(k, Cl.nz, Cl.bt[4], Cl.bt[5], pk[0], pk[1]);
+        // lidx(Cl, .) by parts, as in tail_restrict
+        int xo[2], xp[2], ro[4], rp[4];
+        for (int a = 0; a < 2; ++a) {
+            xo[a] = Cl.split ? pi[a] >> 1 : pi[a];
+            xp[a] = pi[a] & 1;
+        }
+        for (int c3 = 0; c3 < 2; ++c3)
+            for (int c2 = 0; c2 < 2; ++c2) {
+                ro[2 * c3 + c2] = (pk[c3] * Cl.ny + pj[c2]) * rl;
+                rp[2 * c3 + c2] = (pj[c2] + pk[c3]) & 1;
+            }
         T s = T(0);
         for (int c3 = 0; c3 < 2; ++c3)
             for (int c2 = 0; c2 < 2; ++c2)
                 for (int c1 = 0; c1 < 2; ++c1) {
                     T w = wt[c3] * wt[c2] * wt[c1];
-                    s = s + w * xc[lidx(Cl, pi[c1], pj[c2], pk[c3])];
+                    const int r = 2 * c3 + c2;
+                    s = s + w * xc[(xp[c1] ^ rp[r]) * hs + ro[r] + xo[c1]];
                 }
         xf[q] = xf[q] + s;
     }
@@ -1177,6 +1261,7 @@ __device__ __forceinline__ void tail_coarse(const LevelDev &L, T *x, const T *b,
             }
         wave_barrier_lds();
     };
+    if (coarse_sweeps_small<T>(x, b, nb, id, col, n, sweeps, ihx2, ihy2, ihz2)) return;
     for (int s = 0; s < sweeps; ++s) { half(0); half(1); }
     for (int s = 0; s < sweeps; ++s) { half(1); half(0); }
 }
@@ -1192,15 +1277,26 @@ __device__ __forceinline__ void vcycle_tail_body(const Blk &kb, const LevelDev *
     __shared__ T s_id[COARSE_LDS];
     __shared__ unsigned char s_col[COARSE_LDS];
     __shared__ int s_off[IBMDEM_MAX_LEVELS + 1];   // level l: x at off, then b, then r
-    auto cells = [&](int l) { return lev[l].nx * lev[l].ny * lev[l].nz; };
+    // the levels' descriptors (sizes, boundary types, diagonal tables) in
+    // LDS: every phase reads them, and a level's first phase would otherwise
+    // wait on its scalar and table loads from global memory
+    __shared__ LevelDev s_lev[TAIL_MAXLEV];
+    {
+        const int nw = (last - first + 1) * (int)(sizeof(LevelDev) / 4);
+        const int *src = (const int *)(lev + first);
+        int *dst = (int *)s_lev;
+        for (int w = kb.tx; w < nw; w += TAIL_T) dst[w] = src[w];
+    }
     if (kb.tx == 0) {
         int off = 0;
         for (int l = first; l <= last; ++l) {
             s_off[l] = off;
-            off += 3 * cells(l) + (l == last ? 1 : 0);
+            off += 3 * lev[l].nx * lev[l].ny * lev[l].nz + (l == last ? 1 : 0);
         }
     }
     __syncthreads();
+    auto LV = [&](int l) -> const LevelDev & { return s_lev[l - first]; };
+    auto cells = [&](int l) { return LV(l).nx * LV(l).ny * LV(l).nz; };
     auto X = [&](int l) { return mem + s_off[l]; };
     auto B = [&](int l) { return mem + s_off[l] + cells(l) + (l == last ? 1 : 0); };
     auto R = [&](int l) { return B(l) + cells(l); };
@@ -1208,19 +1304,19 @@ __device__ __forceinline__ void vcycle_tail_body(const Blk &kb, const LevelDev *
     __syncthreads();
     for (int l = first; l < last; ++l) {
         for (int q = 0; q < pre; ++q) {
-            tail_half(lev[l], X(l), B(l), 0, q == 0);
-            tail_half(lev[l], X(l), B(l), 1, false);
+            tail_half(LV(l), X(l), B(l), 0, q == 0);
+            tail_half(LV(l), X(l), B(l), 1, false);
         }
-        tail_residual(lev[l], X(l), B(l), R(l));
-        tail_restrict(lev[l], R(l), lev[l + 1], B(l + 1));
+        tail_residual(LV(l), X(l), B(l), R(l));
+        tail_restrict(LV(l), R(l), LV(l + 1), B(l + 1));
     }
-    if (kb.tx < 64) tail_coarse(lev[last], X(last), B(last), coarse_sweeps, s_nb, s_id, s_col);
+    if (kb.tx < 64) tail_coarse(LV(last), X(last), B(last), coarse_sweeps, s_nb, s_id, s_col);
     __syncthreads();
     for (int l = last - 1; l >= first; --l) {
-        tail_prolong(lev[l], X(l), lev[l + 1], X(l + 1));
+        tail_prolong(LV(l), X(l), LV(l + 1), X(l + 1));
         for (int q = 0; q < post; ++q) {
-            tail_half(lev[l], X(l), B(l), 1, false);
-            tail_half(lev[l], X(l), B(l), 0, false);
+            tail_half(LV(l), X(l), B(l), 1, false);
+            tail_half(LV(l), X(l), B(l), 0, false);
         }
     }
     for (int q = kb.tx; q < cells(first); q += TAIL_T) xg[q] = X(first)[q];
@@ -1921,7 +2017,7 @@ int poisson_setup(ibmdem_ctx *c) {
         size_t need = elem;   // the zero slot
         for (int l = c->nlev - 1; l >= 1; --l) {
             need += 3 * (size_t)c->lev64[l].n * elem;
-            if (need > (size_t)TAIL_BYTES) break;
+            if (need > (size_t)TAIL_BYTES || c->nlev - l > TAIL_MAXLEV) break;
             c->tail_first = l;
         }
         if (c->tail_first == c->nlev - 1) c->tail_first = c->nlev;   // the coarsest alone: k_coarse_lds
