@@ -602,13 +602,16 @@ __device__ __forceinline__ int tile_coord(int v, int n, int lo, int hi) {
 // plane; the fine residual they weigh (2 RS_CX + 2 by 2 RS_CY + 2 by 4 fine
 // cells at virtual coordinates, mapped by tile_coord) is staged in LDS with
 // coalesced loads, and each thread sums its 64 terms from there in the
-// (z, y, x) order.
+// (z, y, x) order. With xc, the coarse level's first smoothing sweep (x = 0
+// before it: colour 0 takes (b + 0)/diag, colour 1 stays 0) is done here
+// too, each cell's value as that sweep computes it from the b just written,
+// and the coarse level's V-cycle starts with its second half sweep.
 constexpr int RS_CX = 32, RS_CY = 8;
 constexpr int RS_RX = 2 * RS_CX + 2, RS_RY = 2 * RS_CY + 2, RS_TILE = RS_RX * RS_RY * 4;
 static_assert(RS_CX * RS_CY == RB, "one coarse cell per thread");
 template <typename T>
 __device__ __forceinline__ void restrict_body(const Blk &kb, const LevelDev &F, const T *__restrict__ rf, const LevelDev &Cl,
-                                                 T *__restrict__ bc, const double *__restrict__ stop) {
+                                                 T *__restrict__ bc, T *__restrict__ xc, const double *__restrict__ stop) {
     if (*stop != 0.0) return;
     __shared__ T tile[RS_TILE];
     const int t = kb.tx;
@@ -657,12 +660,16 @@ __device__ __forceinline__ void restrict_body(const Blk &kb, const LevelDev &F, 
                 T w = wt[c3] * wt[c2] * wt[c1];
                 s = s + w * base[(c3 * RS_RY + c2) * RS_RX + c1];
             }
-    bc[lidx(Cl, I, J, K)] = T(0.125) * s;
+    const T bv = T(0.125) * s;
+    const int o = lidx(Cl, I, J, K);
+    bc[o] = bv;
+    if (xc) xc[o] = ((I + J + K) & 1) ? T(0) : (bv + T(0)) * T(Cl.idiag[diag_index(Cl, I, J, K)]);
 }
 template <typename T>
 __global__ __launch_bounds__(RB) void k_restrict(LevelDev F, const T *__restrict__ rf, LevelDev Cl,
-                                                 T *__restrict__ bc, const double *__restrict__ stop) {
-    restrict_body<T>(hw_blk(), F, rf, Cl, bc, stop);
+                                                 T *__restrict__ bc, T *__restrict__ xc,
+                                                 const double *__restrict__ stop) {
+    restrict_body<T>(hw_blk(), F, rf, Cl, bc, xc, stop);
 }
 
 __device__ __forceinline__ void parents(int f, int nc, int lo, int hi, int &p0, int &p1) {
@@ -1150,7 +1157,7 @@ __device__ __forceinline__ void tail_residual(const LevelDev &L, const T *x, con
 }
 
 template <typename T>
-__device__ __forceinline__ void tail_restrict(const LevelDev &F, const T *rf, const LevelDev &Cl, T *bc) {
+__device__ __forceinline__ void tail_restrict(const LevelDev &F, const T *rf, const LevelDev &Cl, T *bc, T *xc) {
     const T wt[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
     const int n = Cl.nx * Cl.ny * Cl.nz;
     const int rl = F.split ? F.nx >> 1 : F.nx, hs = F.split ? F.half : 0;
@@ -1181,7 +1188,10 @@ __device__ __forceinline__ void tail_restrict(const LevelDev &F, const T *rf, co
                     const int r = 4 * c3 + c2;
                     s = s + w * rf[(xp[c1] ^ rp[r]) * hs + ro[r] + xo[c1]];
                 }
-        bc[q] = T(0.125) * s;
+        const T v = T(0.125) * s;
+        bc[q] = v;
+        // the coarse level's first sweep, as k_restrict does it with xc
+        if (xc) xc[q] = ((I + J + K) & 1) ? T(0) : (v + T(0)) * T(Cl.idiag[diag_index(Cl, I, J, K)]);
     }
     __syncthreads();
 }
@@ -1302,13 +1312,15 @@ __device__ __forceinline__ void vcycle_tail_body(const Blk &kb, const LevelDev *
     auto R = [&](int l) { return B(l) + cells(l); };
     for (int q = kb.tx; q < cells(first); q += TAIL_T) B(first)[q] = bg[q];
     __syncthreads();
+    // below the first level, the restriction into a level also does its
+    // first sweep (the coarsest level sets its own x)
     for (int l = first; l < last; ++l) {
         for (int q = 0; q < pre; ++q) {
-            tail_half(LV(l), X(l), B(l), 0, q == 0);
+            if (q > 0 || l == first) tail_half(LV(l), X(l), B(l), 0, q == 0);
             tail_half(LV(l), X(l), B(l), 1, false);
         }
         tail_residual(LV(l), X(l), B(l), R(l));
-        tail_restrict(LV(l), R(l), LV(l + 1), B(l + 1));
+        tail_restrict(LV(l), R(l), LV(l + 1), B(l + 1), l + 1 < last ? X(l + 1) : (T *)nullptr);
     }
     if (kb.tx < 64) tail_coarse(LV(last), X(last), B(last), coarse_sweeps, s_nb, s_id, s_col);
     __syncthreads();
@@ -1862,9 +1874,10 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     };
     // pre-smoothing (red, black); the last black sweep also leaves r of the
     // black cells, then r of the red cells
+    // below level 0 the restriction that wrote b also did the first sweep
     group_begin();
     for (int q = 0; q < c->cs.mg_pre; ++q) {
-        half(0, q == 0 ? 1 : 0);
+        if (q > 0 || l == 0) half(0, q == 0 ? 1 : 0);
         half(1, 0, -1, q == c->cs.mg_pre - 1 ? 1 : 0);
         CHECK_LAUNCH(c);
     }
@@ -1873,7 +1886,11 @@ int vcycle(ibmdem_ctx *c, MGLevel<T> *lev, int l, bool dot = false) {
     CHECK_LAUNCH(c);
     MGLevel<T> &C = lev[l + 1];
     const dim3 cs(div_up(C.d.nx, RS_CX), div_up(C.d.ny, RS_CY), C.d.nz);
-    hipLaunchKernelGGL((k_restrict<T>), cs, dim3(RB), 0, s, d, L.r, C.d, C.b, c->scal + SC_CONV);
+    // the coarse level's first sweep in the restriction when that level is
+    // swept by launches (not the work-group tail, not the coarsest)
+    const bool czero = l + 1 < c->tail_first && l + 1 != c->nlev - 1;
+    hipLaunchKernelGGL((k_restrict<T>), cs, dim3(RB), 0, s, d, L.r, C.d, C.b, czero ? C.x : (T *)nullptr,
+                       c->scal + SC_CONV);
     CHECK_LAUNCH(c);
     int rc = vcycle<T>(c, lev, l + 1);
     if (rc) return rc;
@@ -2225,13 +2242,14 @@ struct RsArg {
     BGrid g;
     const LevelDev *F, *C;
     const T *rf;
-    T *bc;
+    T *bc, *xc;   // xc: the coarse level's first sweep too (null: not)
     const double *stop;
 };
 template <typename T>
 __global__ __launch_bounds__(RB) void kb_restrict(BTable<RsArg<T>> tab) {
     batch_for(tab, [&](const RsArg<T> &a, const Blk &kb) {
-        restrict_body<T>(kb, *cst(a.F), glb(a.rf), *cst(a.C), glb(a.bc), glb(a.stop));
+        restrict_body<T>(kb, *cst(a.F), glb(a.rf), *cst(a.C), glb(a.bc), a.xc ? glb(a.xc) : (T *)nullptr,
+                         glb(a.stop));
     });
 }
 
@@ -2502,8 +2520,10 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
         return add_op(ar, ops, launch_coarse<T>, COARSE_T, glob, o, err);
     };
     for (int l = 0; l < depth; ++l) {
+        // below level 0 every case that sweeps level l had its first sweep
+        // done by the restriction into l
         for (int q = 0; q < pre; ++q) {
-            if ((rc = half(l, 0, q == 0, -1, 0))) return rc;
+            if ((q > 0 || l == 0) && (rc = half(l, 0, q == 0, -1, 0))) return rc;
             if ((rc = half(l, 1, 0, -1, q == pre - 1 ? 1 : 0))) return rc;
         }
         if ((rc = half(l, 0, 0, -1, 2))) return rc;
@@ -2517,14 +2537,18 @@ int plan_precondition_dot(Arena &ar, std::vector<BOp> &ops, const std::vector<ib
             a.C = c->dlev + l + 1;
             a.rf = F.r;
             a.bc = C.b;
+            a.xc = l + 1 < bottom_level(c) ? C.x : nullptr;
             a.stop = c->scal + SC_CONV;
             rs.push_back(a);
         }
         BOp ors;
         ors.kind = IBMDEM_BK_RESTRICT;
-        // the fine residual read, the coarse right-hand side written
+        // the fine residual read, the coarse right-hand side (and where the
+        // first coarse sweep is fused, the coarse x) written
         for (ibmdem_ctx *c : cs)
-            if (l < bottom_level(c)) ors.bytes += sizeof(T) * (double)(levels<T>(c)[l].n + levels<T>(c)[l + 1].n);
+            if (l < bottom_level(c))
+                ors.bytes += sizeof(T) * (double)(levels<T>(c)[l].n + levels<T>(c)[l + 1].n *
+                                                  (l + 1 < bottom_level(c) ? 2 : 1));
         if ((rc = add_op(ar, ops, launch_restrict<T>, RB, rs, ors, err))) return rc;
         if ((rc = bottoms(l + 1))) return rc;
     }
