@@ -106,3 +106,34 @@ def test_steps_gated_equal_always_binned(gpu_lib, monkeypatch):
     finally:
         g.close()
         ref.close()
+
+
+def test_batch_gated_equal_always_binned(gpu_lib, monkeypatch):
+    """The batched DEM sub-steps gate each case's rebuild by its own culling
+    flag: a batch of the approaching spheres (their gate opens in the third
+    step) and a cluster in contact from the start (its gate always open)
+    equals each case stepped alone by a library that bins after every
+    sub-step."""
+    from ibmdem.abi import Batch
+    setups = [_approaching(distance=9.3, speed=0.12, fluid=True), cases.small_periodic(gap=0.75)]
+    with Batch(gpu_lib, [s.case for s in setups]) as b:
+        for q, s in enumerate(setups):
+            cases.load(b[q], s)
+        b.run(8)
+        got = [(b[q].download_fields(), b[q].download_markers(), b[q].download_bodies()) for q in range(len(b))]
+    for s, (fields, mk, bd) in zip(setups, got):
+        monkeypatch.setenv("IBMDEM_NO_BIN_GATE", "1")
+        ref = Simulation(gpu_lib, s.case)
+        monkeypatch.delenv("IBMDEM_NO_BIN_GATE")
+        try:
+            cases.load(ref, s)
+            ref.run(8)
+            for a, r in zip(fields, ref.download_fields()):
+                np.testing.assert_array_equal(a, r)
+            mr, br = ref.download_markers(), ref.download_bodies()
+            for k in ("x", "v", "fcol"):
+                np.testing.assert_array_equal(getattr(mk, k), getattr(mr, k), err_msg=k)
+            for k in ("x", "v", "omega"):
+                np.testing.assert_array_equal(getattr(bd, k), getattr(br, k), err_msg=k)
+        finally:
+            ref.close()
