@@ -528,11 +528,56 @@ __device__ void evolve_body(const BodyState &s, int B, int b, double dtD, double
 // mode 0: fcol, fmomcol only; 1: all four sums (AssembleLocalForceMS);
 // 2: all four + CloudEvolutionMS hydro terms (couplesolverCylinder.F90:194-211)
 // integrate: EvolutionMS after the sums.
+// ResetVelocityAndPositionLagrangianPoints with RotateLagrangianPoints
+// (couplesolverCylinder.F90:44-74) and AdjustParcelPosition's clamp
+// (ZJS_EVOLUTION_DEM.F90:121-150) for marker m, given its body's state after
+// the sub-step
+__device__ __forceinline__ void follow_marker(int M, const GridDev &g, double *__restrict__ mx,
+                                                double *__restrict__ mv, int m, bool fixed, const double bxv[3],
+                                                const double bv[3], const double om[3], const double mat[9],
+                                                double dtD) {
+    double xm[3] = {mx[m], mx[(long)M + m], mx[2L * M + m]};
+    if (!fixed) {
+        // body_image's rule on the loaded centre
+        double xb[3];
+        for (int d = 0; d < 3; ++d) {
+            xb[d] = bxv[d];
+            if (g.bt[2 * d] == IBMDEM_BC_PERIODIC) {
+                const double dd = xm[d] - xb[d];
+                if (dd > 0.5 * g.L[d]) xb[d] = xb[d] + g.L[d];
+                else if (dd < -0.5 * g.L[d]) xb[d] = xb[d] + (-g.L[d]);
+            }
+        }
+        for (int d = 0; d < 3; ++d) xm[d] = xm[d] + bv[d] * dtD;
+        const double t0 = xm[0] - xb[0], t1 = xm[1] - xb[1], t2 = xm[2] - xb[2];
+        xm[0] = mat[0] * t0 + mat[1] * t1 + mat[2] * t2;
+        xm[1] = mat[3] * t0 + mat[4] * t1 + mat[5] * t2;
+        xm[2] = mat[6] * t0 + mat[7] * t1 + mat[8] * t2;
+        for (int d = 0; d < 3; ++d) xm[d] = xm[d] + xb[d];
+        const double rr[3] = {xm[0] - xb[0], xm[1] - xb[1], xm[2] - xb[2]};
+        double vv[3];
+        cross3(om, rr, vv);
+        for (int d = 0; d < 3; ++d) mv[(long)d * M + m] = vv[d] + bv[d];
+    }
+    const double e[3] = {g.dx * 1e-20, g.dy * 1e-20, g.dz * 1e-20};
+    for (int d = 0; d < 3; ++d) {
+        double x = xm[d];
+        if (fabs(x - 0.0) < e[d]) x = 0.0 + e[d];
+        if (fabs(x - g.L[d]) < e[d]) x = g.L[d] + e[d];
+        mx[(long)d * M + m] = x;
+    }
+}
+
+__device__ __forceinline__ void body_finish(const Blk &kb, int B, int b, BodyState s, int mode, int integrate,
+                                              double rhof, double dt, double dtD, double g0, double g1, double g2,
+                                              double (*red)[BT], int follow, double *sbody, int &sfixed);
+// mx is written too when follow is set (so it is neither const nor restrict)
 __device__ __forceinline__ void body_reduce_body(const Blk &kb, int M, int B, GridDev g, const int *__restrict__ bstart,
-                                                    const double *__restrict__ mx, const double *__restrict__ fdrag,
+                                                    double *mx, const double *__restrict__ fdrag,
                                                     const double *__restrict__ fcol, const double *__restrict__ mvol,
                                                     BodyState s, int mode, int integrate, double rhof, double dt,
-                                                    double dtD, double g0, double g1, double g2) {
+                                                    double dtD, double g0, double g1, double g2, int follow,
+                                                    double *__restrict__ mv) {
     __shared__ double red[12][BT];
     const int b = kb.bx;
     const int m0 = bstart[b], m1 = bstart[b + 1];
@@ -567,7 +612,29 @@ __device__ __forceinline__ void body_reduce_body(const Blk &kb, int M, int B, Gr
             for (int q = 0; q < 12; ++q) red[q][kb.tx] += red[q][kb.tx + st];
         __syncthreads();
     }
-    if (kb.tx != 0) return;
+    // follow: the work-group then moves its body's markers (follow_body's
+    // update, from the body's new state as thread 0 leaves it in LDS)
+    __shared__ double sbody[18];
+    __shared__ int sfixed;
+    if (kb.tx == 0) body_finish(kb, B, b, s, mode, integrate, rhof, dt, dtD, g0, g1, g2, red, follow, sbody, sfixed);
+    if (!follow) return;
+    __syncthreads();
+    const bool fixed = sfixed != 0;
+    double bxv[3], bv[3], om[3], mat[9];
+    for (int d = 0; d < 3; ++d) {
+        bxv[d] = sbody[d];
+        bv[d] = sbody[3 + d];
+        om[d] = sbody[6 + d];
+    }
+    for (int q = 0; q < 9; ++q) mat[q] = sbody[9 + q];
+    for (int m = m0 + kb.tx; m < m1; m += BT)
+        follow_marker(M, g, mx, mv, m, fixed, bxv, bv, om, mat, dtD);
+}
+// thread 0 of body_reduce_body: the body's sums stored, the hydro terms, the
+// Newton-Euler update, and for follow the new state into LDS
+__device__ __forceinline__ void body_finish(const Blk &kb, int B, int b, BodyState s, int mode, int integrate,
+                                              double rhof, double dt, double dtD, double g0, double g1, double g2,
+                                              double (*red)[BT], int follow, double *sbody, int &sfixed) {
     for (int d = 0; d < 3; ++d) {
         s.fcol[(long)d * B + b] = red[3 + d][0];
         s.fmomcol[(long)d * B + b] = red[9 + d][0];
@@ -600,13 +667,25 @@ __device__ __forceinline__ void body_reduce_body(const Blk &kb, int M, int B, Gr
         const double grav[3] = {g0, g1, g2};
         evolve_body(s, B, b, dtD, rhof, grav);
     }
+    if (follow) {
+        // the values follow_body loads, as evolve_body left them
+        for (int d = 0; d < 3; ++d) {
+            sbody[d] = s.x[(long)d * B + b];
+            sbody[3 + d] = s.v[(long)d * B + b];
+            sbody[6 + d] = s.om[(long)d * B + b];
+        }
+        for (int q = 0; q < 9; ++q) sbody[9 + q] = s.rot[9L * b + q];
+        sfixed = s.fixed[b];
+    }
 }
 __global__ __launch_bounds__(BT) void k_body_reduce(int M, int B, GridDev g, const int *__restrict__ bstart,
-                                                    const double *__restrict__ mx, const double *__restrict__ fdrag,
+                                                    double *mx, const double *__restrict__ fdrag,
                                                     const double *__restrict__ fcol, const double *__restrict__ mvol,
                                                     BodyState s, int mode, int integrate, double rhof, double dt,
-                                                    double dtD, double g0, double g1, double g2) {
-    body_reduce_body(hw_blk(), M, B, g, bstart, mx, fdrag, fcol, mvol, s, mode, integrate, rhof, dt, dtD, g0, g1, g2);
+                                                    double dtD, double g0, double g1, double g2, int follow,
+                                                    double *__restrict__ mv) {
+    body_reduce_body(hw_blk(), M, B, g, bstart, mx, fdrag, fcol, mvol, s, mode, integrate, rhof, dt, dtD, g0, g1, g2,
+                     follow, mv);
 }
 
 __global__ void k_body_integrate(int B, BodyState s, double dtD, double rhof, double g0, double g1, double g2) {
@@ -623,7 +702,6 @@ __device__ __forceinline__ void follow_body(const Blk &kb, int M, int B, GridDev
     const int m = kb.bx * T + kb.tx;
     if (m >= M) return;
     const int b = mbody[m];
-    double xm[3] = {mx[m], mx[(long)M + m], mx[2L * M + m]};
     // the body's values are loaded together, before the fixed test
     const bool fixed = s.fixed[b] != 0;
     double bxv[3], bv[3], om[3], mat[9];
@@ -633,35 +711,7 @@ __device__ __forceinline__ void follow_body(const Blk &kb, int M, int B, GridDev
         om[d] = s.om[(long)d * B + b];
     }
     for (int q = 0; q < 9; ++q) mat[q] = s.rot[9L * b + q];
-    if (!fixed) {
-        // body_image's rule on the loaded centre
-        double xb[3];
-        for (int d = 0; d < 3; ++d) {
-            xb[d] = bxv[d];
-            if (g.bt[2 * d] == IBMDEM_BC_PERIODIC) {
-                const double dd = xm[d] - xb[d];
-                if (dd > 0.5 * g.L[d]) xb[d] = xb[d] + g.L[d];
-                else if (dd < -0.5 * g.L[d]) xb[d] = xb[d] + (-g.L[d]);
-            }
-        }
-        for (int d = 0; d < 3; ++d) xm[d] = xm[d] + bv[d] * dtD;
-        const double t0 = xm[0] - xb[0], t1 = xm[1] - xb[1], t2 = xm[2] - xb[2];
-        xm[0] = mat[0] * t0 + mat[1] * t1 + mat[2] * t2;
-        xm[1] = mat[3] * t0 + mat[4] * t1 + mat[5] * t2;
-        xm[2] = mat[6] * t0 + mat[7] * t1 + mat[8] * t2;
-        for (int d = 0; d < 3; ++d) xm[d] = xm[d] + xb[d];
-        const double rr[3] = {xm[0] - xb[0], xm[1] - xb[1], xm[2] - xb[2]};
-        double vv[3];
-        cross3(om, rr, vv);
-        for (int d = 0; d < 3; ++d) mv[(long)d * M + m] = vv[d] + bv[d];
-    }
-    const double e[3] = {g.dx * 1e-20, g.dy * 1e-20, g.dz * 1e-20};
-    for (int d = 0; d < 3; ++d) {
-        double x = xm[d];
-        if (fabs(x - 0.0) < e[d]) x = 0.0 + e[d];
-        if (fabs(x - g.L[d]) < e[d]) x = g.L[d] + e[d];
-        mx[(long)d * M + m] = x;
-    }
+    follow_marker(M, g, mx, mv, m, fixed, bxv, bv, om, mat, dtD);
 }
 __global__ __launch_bounds__(T) void k_follow(int M, int B, GridDev g, double *__restrict__ mx,
                                               double *__restrict__ mv, const int *__restrict__ mbody, BodyState s,
@@ -785,7 +835,8 @@ int dem_body_assemble(ibmdem_ctx *c, bool all_terms) {
     if (c->B == 0) return IBMDEM_OK;
     hipLaunchKernelGGL(k_body_reduce, dim3(c->B), dim3(BT), 0, c->stream, c->M, c->B, c->g, c->body_start, c->mx,
                        c->fdrag, c->fcol, c->mvol, body_state(c), all_terms ? 1 : 0, 0, c->cs.rhof, c->cs.dt,
-                       c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1], c->cs.gravity[2]);
+                       c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1], c->cs.gravity[2], 0,
+                       (double *)nullptr);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
@@ -794,7 +845,8 @@ int dem_body_hydro(ibmdem_ctx *c) {
     if (c->B == 0) return IBMDEM_OK;
     hipLaunchKernelGGL(k_body_reduce, dim3(c->B), dim3(BT), 0, c->stream, c->M, c->B, c->g, c->body_start, c->mx,
                        c->fdrag, c->fcol, c->mvol, body_state(c), 2, 0, c->cs.rhof, c->cs.dt,
-                       c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1], c->cs.gravity[2]);
+                       c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1], c->cs.gravity[2], 0,
+                       (double *)nullptr);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
@@ -821,7 +873,9 @@ int dem_markers_follow(ibmdem_ctx *c) {
 // each later sub-step's right after the bodies of the one before have moved
 // (its contact scan sees those positions), so that it gates that sub-step's
 // rebuild. The last sub-step's rebuild always bins: the IBM tables of the
-// next step and the pair lists read the bins.
+// next step and the pair lists read the bins. The body reduction's
+// work-groups also move their bodies' markers (the marker update needs only
+// its own body's new state), in place of a launch of k_follow.
 int dem_substep_fused(ibmdem_ctx *c, bool first, bool last) {
     int rc;
     const bool cull = c->M > 0 && c->B > 0;
@@ -831,11 +885,12 @@ int dem_substep_fused(ibmdem_ctx *c, bool first, bool last) {
         hipLaunchKernelGGL(k_body_reduce, dim3(c->B), dim3(BT), 0, c->stream, c->M, c->B, c->g, c->body_start,
                            c->mx, c->fdrag, c->fcol, c->mvol, body_state(c), last ? 1 : 0, 1, c->cs.rhof,
                            c->cs.dt, c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1],
-                           c->cs.gravity[2]);
+                           c->cs.gravity[2], c->M > 0 ? 1 : 0, c->mv);
         CHECK_LAUNCH(c);
+    } else if ((rc = dem_markers_follow(c))) {
+        return rc;
     }
     if (!last && cull && (rc = dem_near(c))) return rc;
-    if ((rc = dem_markers_follow(c))) return rc;
     return dem_rebuild(c, last || !cull || !c->bin_gate ? nullptr : c->bnear + c->B);
 }
 
@@ -1050,13 +1105,14 @@ __global__ __launch_bounds__(T) void kb_contact(DTab tab, const DemCase *cs) {
                             glb((const int *)c.blist), glb((const double *)c.s.x), c.B, glb(c.brmax), c.lim);
     });
 }
-// uniform arguments: mode, integrate
-__global__ __launch_bounds__(BT) void kb_body_reduce(DTab tab, const DemCase *cs, int mode, int integrate) {
+// uniform arguments: mode, integrate, follow
+__global__ __launch_bounds__(BT) void kb_body_reduce(DTab tab, const DemCase *cs, int mode, int integrate,
+                                                     int follow) {
     batch_for(tab, [&](const DArg &a, const Blk &kb) {
         const DemCase &c = *cst(cs + a.q);
-        body_reduce_body(kb, c.M, c.B, c.g, glb(c.body_start), glb((const double *)c.mx), glb(c.fdrag),
+        body_reduce_body(kb, c.M, c.B, c.g, glb(c.body_start), glb(c.mx), glb(c.fdrag),
                          glb((const double *)c.fcol), glb(c.mvol), glb_state(c.s), mode, integrate, c.rhof, c.dt,
-                         c.dtD, c.grav[0], c.grav[1], c.grav[2]);
+                         c.dtD, c.grav[0], c.grav[1], c.grav[2], follow && c.M > 0 ? 1 : 0, glb(c.mv));
     });
 }
 __global__ __launch_bounds__(T) void kb_follow(DTab tab, const DemCase *cs) {
@@ -1099,7 +1155,7 @@ DEM_LAUNCHER(probe_append, 64)
 #undef DEM_LAUNCHER
 void launch_body_reduce(const BOp &o, hipStream_t s, int) {
     hipLaunchKernelGGL(kb_body_reduce, dim3(o.blocks), dim3(BT), 0, s, table<DArg>(o), (const DemCase *)o.aux,
-                       o.slot, o.op);
+                       o.slot, o.op, o.color);
 }
 
 // one launch over the cases for which grid(c) is not empty
@@ -1181,11 +1237,14 @@ int dem_plan_substep(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ct
     BOp o;
     o.slot = last ? 1 : 0;   // mode
     o.op = 1;                // integrate
+    o.color = 1;             // follow: the markers moved by their body's work-group
     if ((rc = dem_op(P, cases, cs, launch_body_reduce, BT, [](ibmdem_ctx *c) { return dim3(c->B); }, o, err)))
         return rc;
-    if (!last && (rc = dem_plan_near(P, cases, cs, err))) return rc;
-    if ((rc = dem_op(P, cases, cs, launch_follow, T, [&](ibmdem_ctx *c) { return dim3(mk(c)); }, BOp(), err)))
+    // cases without bodies move their markers (if any) by k_follow's rule
+    if ((rc = dem_op(P, cases, cs, launch_follow, T,
+                     [&](ibmdem_ctx *c) { return dim3(c->B > 0 ? 0u : mk(c)); }, BOp(), err)))
         return rc;
+    if (!last && (rc = dem_plan_near(P, cases, cs, err))) return rc;
     // dem_rebuild, gated between sub-steps
     BOp gb;
     gb.op = last ? 0 : 1;
