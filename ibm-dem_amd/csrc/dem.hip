@@ -577,7 +577,7 @@ __device__ __forceinline__ void body_reduce_body(const Blk &kb, int M, int B, Gr
                                                     const double *__restrict__ fcol, const double *__restrict__ mvol,
                                                     BodyState s, int mode, int integrate, double rhof, double dt,
                                                     double dtD, double g0, double g1, double g2, int follow,
-                                                    double *__restrict__ mv) {
+                                                    double *__restrict__ mv, int *__restrict__ near_clear) {
     __shared__ double red[12][BT];
     const int b = kb.bx;
     const int m0 = bstart[b], m1 = bstart[b + 1];
@@ -616,7 +616,14 @@ __device__ __forceinline__ void body_reduce_body(const Blk &kb, int M, int B, Gr
     // update, from the body's new state as thread 0 leaves it in LDS)
     __shared__ double sbody[18];
     __shared__ int sfixed;
-    if (kb.tx == 0) body_finish(kb, B, b, s, mode, integrate, rhof, dt, dtD, g0, g1, g2, red, follow, sbody, sfixed);
+    if (kb.tx == 0) {
+        body_finish(kb, B, b, s, mode, integrate, rhof, dt, dtD, g0, g1, g2, red, follow, sbody, sfixed);
+        // the next culling's counts (k_body_near_clear's work), when asked
+        if (near_clear) {
+            near_clear[b] = 0;
+            if (b == 0) near_clear[B] = 0;
+        }
+    }
     if (!follow) return;
     __syncthreads();
     const bool fixed = sfixed != 0;
@@ -683,9 +690,9 @@ __global__ __launch_bounds__(BT) void k_body_reduce(int M, int B, GridDev g, con
                                                     const double *__restrict__ fcol, const double *__restrict__ mvol,
                                                     BodyState s, int mode, int integrate, double rhof, double dt,
                                                     double dtD, double g0, double g1, double g2, int follow,
-                                                    double *__restrict__ mv) {
+                                                    double *__restrict__ mv, int *__restrict__ near_clear) {
     body_reduce_body(hw_blk(), M, B, g, bstart, mx, fdrag, fcol, mvol, s, mode, integrate, rhof, dt, dtD, g0, g1, g2,
-                     follow, mv);
+                     follow, mv, near_clear);
 }
 
 __global__ void k_body_integrate(int B, BodyState s, double dtD, double rhof, double g0, double g1, double g2) {
@@ -774,7 +781,9 @@ int dem_rebuild(ibmdem_ctx *c) { return dem_rebuild(c, nullptr); }
 // the candidate distance of the contact scan, as the body culling uses it
 double near_lim(const ibmdem_ctx *c) { return c->cs.collision_dist * (1.0 + c->cs.collision_buffer); }
 
-int dem_near(ibmdem_ctx *c) {
+// cleared: the counts and near[B] are already zero (the body reduction that
+// ran just before cleared them)
+int dem_near(ibmdem_ctx *c, bool cleared) {
     if (c->B == 0) return IBMDEM_OK;
     if (c->rmax_dirty) {
         hipLaunchKernelGGL(k_body_rmax, dim3(c->B), dim3(T), 0, c->stream, c->M, c->B, c->g, c->body_start, c->mx,
@@ -783,13 +792,17 @@ int dem_near(ibmdem_ctx *c) {
         c->rmax_dirty = false;
     }
     const double lim = near_lim(c);
-    hipLaunchKernelGGL(k_body_near_clear, dim3(div_up(c->B + 1, 64)), dim3(64), 0, c->stream, c->B, c->bnear);
-    CHECK_LAUNCH(c);
+    if (!cleared) {
+        hipLaunchKernelGGL(k_body_near_clear, dim3(div_up(c->B + 1, 64)), dim3(64), 0, c->stream, c->B, c->bnear);
+        CHECK_LAUNCH(c);
+    }
     hipLaunchKernelGGL(k_body_near, dim3(div_up(c->B, 64), div_up(c->B, NEAR_CH)), dim3(64), 0, c->stream, c->B,
                        c->g, c->bx, c->brmax, lim, c->bnear, c->blist);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
+
+int dem_near(ibmdem_ctx *c) { return dem_near(c, false); }
 
 // the contact scan with the body culling of the last dem_near
 int dem_contact_scan(ibmdem_ctx *c) {
@@ -836,7 +849,7 @@ int dem_body_assemble(ibmdem_ctx *c, bool all_terms) {
     hipLaunchKernelGGL(k_body_reduce, dim3(c->B), dim3(BT), 0, c->stream, c->M, c->B, c->g, c->body_start, c->mx,
                        c->fdrag, c->fcol, c->mvol, body_state(c), all_terms ? 1 : 0, 0, c->cs.rhof, c->cs.dt,
                        c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1], c->cs.gravity[2], 0,
-                       (double *)nullptr);
+                       (double *)nullptr, (int *)nullptr);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
@@ -846,7 +859,7 @@ int dem_body_hydro(ibmdem_ctx *c) {
     hipLaunchKernelGGL(k_body_reduce, dim3(c->B), dim3(BT), 0, c->stream, c->M, c->B, c->g, c->body_start, c->mx,
                        c->fdrag, c->fcol, c->mvol, body_state(c), 2, 0, c->cs.rhof, c->cs.dt,
                        c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1], c->cs.gravity[2], 0,
-                       (double *)nullptr);
+                       (double *)nullptr, (int *)nullptr);
     CHECK_LAUNCH(c);
     return IBMDEM_OK;
 }
@@ -885,12 +898,12 @@ int dem_substep_fused(ibmdem_ctx *c, bool first, bool last) {
         hipLaunchKernelGGL(k_body_reduce, dim3(c->B), dim3(BT), 0, c->stream, c->M, c->B, c->g, c->body_start,
                            c->mx, c->fdrag, c->fcol, c->mvol, body_state(c), last ? 1 : 0, 1, c->cs.rhof,
                            c->cs.dt, c->cs.dt / c->cs.couple_interval, c->cs.gravity[0], c->cs.gravity[1],
-                           c->cs.gravity[2], c->M > 0 ? 1 : 0, c->mv);
+                           c->cs.gravity[2], c->M > 0 ? 1 : 0, c->mv, !last && cull ? c->bnear : (int *)nullptr);
         CHECK_LAUNCH(c);
     } else if ((rc = dem_markers_follow(c))) {
         return rc;
     }
-    if (!last && cull && (rc = dem_near(c))) return rc;
+    if (!last && cull && (rc = dem_near(c, true))) return rc;
     return dem_rebuild(c, last || !cull || !c->bin_gate ? nullptr : c->bnear + c->B);
 }
 
@@ -1105,14 +1118,16 @@ __global__ __launch_bounds__(T) void kb_contact(DTab tab, const DemCase *cs) {
                             glb((const int *)c.blist), glb((const double *)c.s.x), c.B, glb(c.brmax), c.lim);
     });
 }
-// uniform arguments: mode, integrate, follow
+// uniform arguments: mode, integrate, follow (1: move the markers; 2: also
+// clear the next culling's counts)
 __global__ __launch_bounds__(BT) void kb_body_reduce(DTab tab, const DemCase *cs, int mode, int integrate,
                                                      int follow) {
     batch_for(tab, [&](const DArg &a, const Blk &kb) {
         const DemCase &c = *cst(cs + a.q);
         body_reduce_body(kb, c.M, c.B, c.g, glb(c.body_start), glb(c.mx), glb(c.fdrag),
                          glb((const double *)c.fcol), glb(c.mvol), glb_state(c.s), mode, integrate, c.rhof, c.dt,
-                         c.dtD, c.grav[0], c.grav[1], c.grav[2], follow && c.M > 0 ? 1 : 0, glb(c.mv));
+                         c.dtD, c.grav[0], c.grav[1], c.grav[2], follow && c.M > 0 ? 1 : 0, glb(c.mv),
+                         follow == 2 && c.M > 0 ? glb(c.bnear) : (int *)nullptr);
     });
 }
 __global__ __launch_bounds__(T) void kb_follow(DTab tab, const DemCase *cs) {
@@ -1217,11 +1232,13 @@ int dem_plan_body_hydro(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem
 // dem_substep_fused of every case (the contact scan's body culling with the
 // near lists, computed first in the first sub-step and after the body update
 // in the others; the caller has made sure no case needs its body radii again)
-static int dem_plan_near(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, std::string &err) {
+static int dem_plan_near(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ctx *> &cs, std::string &err,
+                         bool cleared = false) {
     int rc;
     auto near = [](ibmdem_ctx *c) { return (unsigned)(c->M > 0 && c->B > 0 ? div_up(c->B, 64) : 0); };
-    if ((rc = dem_op(P, cases, cs, launch_body_near_clear, 64,
-                     [&](ibmdem_ctx *c) { return dim3(near(c) ? div_up(c->B + 1, 64) : 0); }, BOp(), err)))
+    if (!cleared && (rc = dem_op(P, cases, cs, launch_body_near_clear, 64,
+                                 [&](ibmdem_ctx *c) { return dim3(near(c) ? div_up(c->B + 1, 64) : 0); }, BOp(),
+                                 err)))
         return rc;
     return dem_op(P, cases, cs, launch_body_near, 64,
                   [&](ibmdem_ctx *c) { return dim3(near(c), near(c) ? div_up(c->B, NEAR_CH) : 0); }, BOp(), err);
@@ -1237,14 +1254,15 @@ int dem_plan_substep(bk::BatchPlan &P, size_t cases, const std::vector<ibmdem_ct
     BOp o;
     o.slot = last ? 1 : 0;   // mode
     o.op = 1;                // integrate
-    o.color = 1;             // follow: the markers moved by their body's work-group
+    o.color = last ? 1 : 2;  // follow: the markers moved by their body's work-group (2: and the next
+                             // culling's counts cleared)
     if ((rc = dem_op(P, cases, cs, launch_body_reduce, BT, [](ibmdem_ctx *c) { return dim3(c->B); }, o, err)))
         return rc;
     // cases without bodies move their markers (if any) by k_follow's rule
     if ((rc = dem_op(P, cases, cs, launch_follow, T,
                      [&](ibmdem_ctx *c) { return dim3(c->B > 0 ? 0u : mk(c)); }, BOp(), err)))
         return rc;
-    if (!last && (rc = dem_plan_near(P, cases, cs, err))) return rc;
+    if (!last && (rc = dem_plan_near(P, cases, cs, err, true))) return rc;
     // dem_rebuild, gated between sub-steps
     BOp gb;
     gb.op = last ? 0 : 1;
