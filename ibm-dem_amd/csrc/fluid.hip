@@ -805,12 +805,22 @@ __device__ __forceinline__ void reduce_partial_body(const Blk &kb, const double 
     const int t = kb.tx;
     const long chunk = (n + kb.gx - 1) / kb.gx;
     const long lo = (long)kb.bx * chunk, hi = min(n, lo + chunk);
-    double s = 0.0;
-    for (long q = lo + t; q < hi; q += RED_THREADS) {
-        if (q < skip) continue;
-        s += b ? a[q] * b[q] : a[q];
+    // four independent sums over the thread's strided elements (four rounds
+    // of loads in flight), then (s0 + s1) + (s2 + s3)
+    auto term = [&](long q) { return b ? a[q] * b[q] : a[q]; };
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    long q = lo + t;
+    for (; q + 3 * RED_THREADS < hi; q += 4 * RED_THREADS) {
+        const double v0 = term(q), v1 = term(q + RED_THREADS), v2 = term(q + 2 * RED_THREADS),
+                     v3 = term(q + 3 * RED_THREADS);
+        if (q >= skip) s0 += v0;
+        if (q + RED_THREADS >= skip) s1 += v1;
+        if (q + 2 * RED_THREADS >= skip) s2 += v2;
+        if (q + 3 * RED_THREADS >= skip) s3 += v3;
     }
-    red[t] = s;
+    for (; q < hi; q += RED_THREADS)
+        if (q >= skip) s0 += term(q);
+    red[t] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
         if (t < st) red[t] += red[t + st];

@@ -1639,9 +1639,19 @@ __device__ __forceinline__ void dot_body(const Blk &kb, long n, const double *__
     __shared__ double red[RED_T];
     const long chunk = (n + kb.gx - 1) / kb.gx;
     const long lo = (long)kb.bx * chunk, hi = min(n, lo + chunk);
-    double s = 0.0;
-    for (long q = lo + kb.tx; q < hi; q += RED_T) s += a[q] * b[q];
-    double t = block_reduce(s, red);
+    // four independent sums over the thread's strided elements (four
+    // rounds of loads in flight), then (s0 + s1) + (s2 + s3): a fixed
+    // association set by (n, grid, RED_T)
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    long q = lo + kb.tx;
+    for (; q + 3 * RED_T < hi; q += 4 * RED_T) {
+        s0 += a[q] * b[q];
+        s1 += a[q + RED_T] * b[q + RED_T];
+        s2 += a[q + 2 * RED_T] * b[q + 2 * RED_T];
+        s3 += a[q + 3 * RED_T] * b[q + 3 * RED_T];
+    }
+    for (; q < hi; q += RED_T) s0 += a[q] * b[q];
+    double t = block_reduce((s0 + s1) + (s2 + s3), red);
     if (kb.tx == 0) partial[kb.bx] = t;
 }
 __global__ __launch_bounds__(RED_T) void k_dot(long n, const double *__restrict__ a, const double *__restrict__ b,
