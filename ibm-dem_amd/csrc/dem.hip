@@ -194,7 +194,8 @@ __global__ __launch_bounds__(T) void k_body_rmax(int M, int B, GridDev g, const 
 // if any body has one (the gate of the bins, above). Thread b of block
 // (x, y) tests body b = 64x + t against the NEAR_CH bodies c of chunk y,
 // staged in LDS (every block would otherwise read all B positions from L2);
-// k_body_near_clear zeroed the counts and near[B]. The periodic minimum image takes
+// k_body_near_clear (between sub-steps, the body reduction before it) zeroed
+// the counts and near[B]. The periodic minimum image takes
 // rint(d * (1/L)): it can differ from rint(d / L) only where both images are
 // L/2 away, within rounding, which the safety margin covers, so no body in
 // reach is missed.
