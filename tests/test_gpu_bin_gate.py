@@ -36,6 +36,28 @@ def _approaching(distance=9.4, speed=0.15, fluid=False):
     return setup
 
 
+def _approaching_walls(distance=9.4, speed=0.15):
+    """The closed box's two D = 8 spheres, the lower one at rest on the
+    bottom wall (its lowest markers in wall contact throughout), the upper
+    one moved up to `distance` above it and falling at `speed`."""
+    setup = cases.small_walls()
+    mk, bd = setup.markers, setup.bodies
+    d = bd.x[:, 1] - bd.x[:, 0]
+    lift = np.sqrt(distance ** 2 - d[0] ** 2 - d[1] ** 2) - d[2]
+    bd.x[2, 1] += lift
+    mk.x[2, mk.body == 1] += lift
+    bd.v[:] = 0.0
+    bd.v[2, 1] = -speed
+    bd.vold[:] = bd.v
+    bd.omega[:] = 0.0
+    bd.angmom[:] = 0.0
+    bd.angmom_old[:] = 0.0
+    mk.v[:] = bd.v[:, mk.body]
+    for f in setup.fields:
+        f[:] = 0.0
+    return setup
+
+
 def _pair(gpu_lib, monkeypatch, setup):
     monkeypatch.setenv("IBMDEM_NO_BIN_GATE", "1")
     ref = Simulation(gpu_lib, setup.case)
@@ -62,8 +84,9 @@ def rel(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / scale
 
 
-def test_sub_steps_gated_equal_always_binned(gpu_lib, oracle_lib, monkeypatch):
-    setup = _approaching()
+@pytest.mark.parametrize("walls", [False, True])
+def test_sub_steps_gated_equal_always_binned(gpu_lib, oracle_lib, monkeypatch, walls):
+    setup = _approaching_walls() if walls else _approaching()
     g, ref = _pair(gpu_lib, monkeypatch, setup)
     orc = Simulation(oracle_lib, setup.case)
     cases.load(orc, setup)
@@ -79,10 +102,13 @@ def test_sub_steps_gated_equal_always_binned(gpu_lib, oracle_lib, monkeypatch):
             assert rel(m.v, mo.v) < 1e-10
             for k in ("x", "v"):
                 assert rel(getattr(bg, k), getattr(bo, k)) < 1e-10, k
-        # no contact in the first call (the bodies start out of reach), and
-        # contact forces once they meet (the oracle: calls 3 and 4, after
-        # which they part again and leave culling reach in call 6)
-        assert not touched[0] and any(touched), touched
+        # periodic: no contact in the first call (the bodies start out of
+        # reach), and contact forces once they meet (the oracle: calls 3 and
+        # 4, after which they part again and leave culling reach in call 6);
+        # walls: wall contact from the start, pair contact once they meet
+        assert any(touched), touched
+        if not walls:
+            assert not touched[0], touched
         for a, b in zip(g.pairs(), ref.pairs()):
             np.testing.assert_array_equal(a, b)
     finally:
